@@ -1,0 +1,189 @@
+"""Benchmark: fwd+bwd+Adam frames/s of the nvdiffrast-style fit step on one MI355X.
+
+Default workload: single large plain request -- displaced sphere (49,922 V / 99,840 T),
+1024x1024, 2048^2 x 4 fp32 texture with 12 mip levels, 128 SG lobes, antialias, L2 loss,
+backward, Adam over vertex positions, texture and lobes; one request per launch sequence.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain] [--no-cpu]
+Prints ONE JSON line.  Inputs are resident in HBM before the timed region starts.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+
+
+def algorithmic_bytes(sc, views=1):
+    """Bytes each op must move at its boundary (each op-surface buffer read or written once),
+    per step; texel gathers counted as the level-0/1 footprint is not known a priori, so the
+    texture ops count only per-pixel surfaces (a lower bound)."""
+    P = sc.W * sc.H * views
+    V, T, C, K = sc.V, sc.T, sc.C, sc.K
+    n0 = sc.tw * sc.th * C
+    ncoarse = sum((sc.tw >> l) * (sc.th >> l) * C for l in range(1, sc.L))
+    return {
+        "project": V * 12 + views * V * 16,
+        "rasterize": P * 32 + views * V * 16 + T * 12,
+        "interpolate": P * 56 + V * 8 + T * 12,
+        "normal_axis": P * 28 + V * 12 + T * 12,
+        "view_axis": P * 12,
+        "texture": P * (4 + 8 + 16 + 4 * C),
+        "sg_specular": P * 52,
+        "spherical_gaussian": P * 100 + K * 32,
+        "antialias": P * (8 + 8 * C) + views * V * 16 + T * 12,
+        "mse_loss": P * 12 * C,
+        "antialias_bwd": P * (8 + 12 * C),
+        "spherical_gaussian_bwd": P * (52 + 8 * C) + K * 64,
+        "texture_bwd": P * (4 + 8 + 16 + 4 * C + 8 + 16),
+        "interpolate_bwd": P * 32,
+        "rasterize_bwd": P * 24 + views * V * 40 + T * 12,
+        "project_bwd": V * 40,
+        "texture_adam": n0 * 32 + ncoarse * 4,
+        "texture_mip": (n0 + ncoarse) * 4 + ncoarse * 8,
+        "adam": (V * 3 + K * (4 + C)) * 28,
+    }
+
+
+def cpu_baseline(sc, target, steps, threads):
+    """Time the CPU oracle's float fit step (a C++ restatement of the reference semantics)."""
+    import ctypes as C
+
+    import numpy as np
+
+    import oracle
+    oracle.build()
+    oracle.set_threads(threads)
+    L = oracle.lib()
+    s = oracle.OrcScene()
+    keep = []
+
+    def p(a):
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+    cam = sc.cams[0]
+    s.W, s.H, s.V, s.T, s.C, s.K, s.tw, s.th, s.L = sc.W, sc.H, sc.V, sc.T, sc.C, sc.K, sc.tw, sc.th, sc.L
+    s.pos, s.normals, s.uv, s.idx = p(sc.pos.copy()), p(sc.normals), p(sc.uv), p(sc.idx)
+    s.mat, s.rot, s.pvinv = p(cam.mat), p(np.eye(4, dtype=np.float32).reshape(16)), p(cam.pvinv)
+    s.tex, s.axis, s.sharp, s.amp = p(sc.tex.copy()), p(sc.axis.copy()), p(sc.sharp.copy()), p(sc.amp.copy())
+    s.rough = p(np.full(sc.W * sc.H, sc.rough, np.float32))
+    s.ior, s.target, s.fold_quarter = sc.ior, p(target), 1
+    for nm, n in (("pos", sc.V * 3), ("tex", sc.tw * sc.th * sc.C), ("axis", sc.K * 3), ("sharp", sc.K), ("amp", sc.K * sc.C)):
+        setattr(s, "m_" + nm, p(np.zeros(n, np.float32)))
+        setattr(s, "v_" + nm, p(np.zeros(n, np.float32)))
+    s.lr, s.b1, s.b2, s.eps, s.b1t, s.b2t = 1e-2, 0.9, 0.999, 1e-8, 0.9, 0.999
+    s.out_image = None
+    w = L.orc_work_create()
+    L.orc_fit_step_f32(C.byref(s), w)  # warm-up (allocations)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.orc_fit_step_f32(C.byref(s), w)
+    dt = time.perf_counter() - t0
+    L.orc_work_destroy(w)
+    return steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--mode", default="plain", choices=["plain"])
+    ap.add_argument("--res", type=int, default=1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--breakdown", action="store_true", help="print a per-op table to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only (DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from vcl_amd import ops
+    from vcl_amd import scene as S
+
+    torch.cuda.set_device(0)
+    sc = S.config2_scene(1, W=args.res)
+    tsc = S.config2_scene(2, W=args.res)
+    tf = ops.Fit(tsc)
+    tf.set_target_from_render()
+    target = tf.read("image", (sc.H, sc.W, sc.C))
+    tf.close()
+    fit = ops.Fit(sc, target=target)
+    views = fit.views
+    for _ in range(args.warmup):
+        fit.step()
+    torch.cuda.synchronize()
+    # timed region: the dominant HBM kernel is timed with HIP events on the launch stream
+    ROOF_OP = "texture_adam"
+    fit.profile_begin(max_launches=args.steps + 1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        fit.step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    prof = fit.profile_end()
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = gpu_ms / args.steps
+    value = views * args.steps / (gpu_ms / 1e3)
+    loss = float(fit.read("loss", (1,))[0])
+    ab = algorithmic_bytes(sc, views)
+    kernels = {}
+    for name, (tot, cnt) in prof.items():
+        if cnt == 0:
+            continue
+        avg = tot / cnt
+        b = ab.get(name)
+        kernels[name] = {"avg_ms": round(avg, 5), "launches": cnt, "alg_bytes": b,
+                         "GBps": round(b / (avg * 1e-3) / 1e9, 1) if b else None}
+    rk = kernels.get(ROOF_OP)
+    achieved = rk["GBps"] if rk else None
+    roofline = {"bound": "hbm", "kernel": "k_tex_update (" + ROOF_OP + ")", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None, "traffic": None,
+                "alg_bytes_per_launch": ab[ROOF_OP], "avg_ms": rk["avg_ms"] if rk else None}
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        fps, dt = cpu_baseline(sc, target, args.cpu_steps, threads)
+        cpu = {"value": round(fps, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} fit steps of the same workload ({sc.W}x{sc.H}, {sc.T} tris, K={sc.K}) on the oracle, {dt:.1f} s"}
+    out = {
+        "metric": "fwd+bwd+Adam frames/s at 1024^2, 100k tris (plain per-request path)",
+        "value": round(value, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded displaced sphere, random texture and SG lobes)",
+        "config": {"workload": f"plain fit request: displaced sphere {sc.V} V / {sc.T} T, {sc.W}x{sc.H}, texture {sc.tw}^2x{sc.C} "
+                               f"L={sc.L}, {sc.K} SG lobes, antialias, L2, Adam (pos, texture, lobes)",
+                   "mode": args.mode, "views": views},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "wall_s": round(wall, 3),
+        "loss": loss,
+        "kernels": kernels,
+    }
+    if args.breakdown:
+        tot = sum(v["avg_ms"] for v in kernels.values())
+        for k, v in sorted(kernels.items(), key=lambda kv: -kv[1]["avg_ms"]):
+            print(f"{k:24s} {v['avg_ms']*1e3:9.1f} us  {100*v['avg_ms']/tot:5.1f}%  {v['GBps']} GB/s", file=sys.stderr)
+    fit.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,432 @@
+/* vcl.h -- C ABI of the MI355X differentiable-rendering library (libvcl_amd.so).
+ *
+ * Each op keeps the reference's POD parameter layout ("XKernelParams" / "XKernelGradParams",
+ * passed by value to the kernels in the reference) with the same field names and meanings,
+ * plus a stream and an int status.  Citations name the reference header each struct and
+ * entry point replaces (236088/vcl, nvdiffrast/nvdiffrast/<file>:<line>).
+ *
+ * Conventions kept from the reference:
+ *  - images are row-major with row 0 at the bottom (NDC y = -1); pixel index
+ *    pidx = px + width * (py + height * pz), pz = layer ("depth");
+ *  - a NULL grad pointer means "skip this gradient";
+ *  - parameter grads accumulate (+=); grad.rast (interpolate), grad.uv (texture) and
+ *    grad.in (antialias) are stored, not accumulated;
+ *  - output grads are zeroed by the caller before the forward of a step (the reference's
+ *    forward(XGradParams&) overloads); include/vcl_layers.hpp does this.
+ * Additions: "depth" > 1 gives one clip-coordinate slice (projNum x 4 floats) per layer, so a
+ * batch of views of one mesh renders in one call.  Scratch memory is passed in explicitly
+ * (workspace pointers sized by the *_workspace_size queries), so no entry point allocates.
+ * All entry points are asynchronous on the given stream and capturable in a HIP graph.
+ */
+#ifndef VCL_H_
+#define VCL_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* vcl_stream_t; /* hipStream_t */
+
+enum {
+    VCL_OK = 0,
+    VCL_ERR_INVALID = 1,   /* bad dimensions or a required pointer is NULL */
+    VCL_ERR_OOM = 2,       /* workspace too small / admission refused */
+    VCL_ERR_HIP = 3,       /* a HIP runtime call failed */
+    VCL_ERR_OVERFLOW = 4   /* device-side capacity exceeded (reported by *_status) */
+};
+
+#define VCL_TEX_MAX_MIP_LEVEL 16 /* TEX_MAX_MIP_LEVEL, buffer.h:4 */
+
+const char* vcl_status_string(int status);
+/* Library version and the device architecture the kernels were built for ("gfx950"). */
+const char* vcl_build_info(void);
+
+/* ---------------------------------------------------------------- Project
+ * ProjectKernelParams / ProjectKernelGradParams (project.h:5-19); Project::forward/backward
+ * (project.cu:69-84, 114-138).  out[v][i] = sum_j mat[4j+i] * [vec, 1]_j, mat column-major.
+ * views > 1: mat holds views x 16 values and out views x vecNum x dimention. */
+typedef struct vcl_project_params {
+    int vecNum;
+    int dimention;        /* 3 or 4 (homogeneous) */
+    const float* vec;     /* vecNum x 3 */
+    const float* mat;     /* views x 16 */
+    float* out;           /* views x vecNum x dimention */
+    int views;            /* >= 1 */
+} vcl_project_params;
+
+typedef struct vcl_project_grad_params {
+    float* vec;           /* vecNum x 3, accumulated (summed over views), or NULL */
+    float* mat;           /* views x 16, accumulated, or NULL (race-free here; project.cu:125-129) */
+    const float* out;     /* views x vecNum x dimention */
+} vcl_project_grad_params;
+
+int vcl_project_forward(const vcl_project_params* p, vcl_stream_t stream);
+int vcl_project_backward(const vcl_project_params* p, const vcl_project_grad_params* g, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Rasterize
+ * RasterizeKernelParams (rasterize.h:6-18) plus projNum/idxNum of RasterizeParams
+ * (rasterize.h:30-31); Rasterize::forward (rasterize.cpp:257-282, an OpenGL draw in the
+ * reference, a tiled software rasterizer here) and Rasterize::backward (rasterize.cu:143-148).
+ * out = (u, v, z/w, id+1) per pixel, outDB = (du/dX, du/dY, dv/dX, dv/dY) in pixel units.
+ * Coverage: 8-bit sub-pixel fixed point, top-left fill rule, no culling; visibility: the
+ * minimum of (z/w, triangle id).  Triangle ids must be < 2^24 (exact as float). */
+typedef struct vcl_rasterize_params {
+    int width;
+    int height;
+    int depth;            /* layers; layer z reads proj + z * projNum * 4 */
+    int enableDB;
+    float xs;             /* 2 / width  (rasterize.cpp:73) */
+    float ys;             /* 2 / height (rasterize.cpp:74) */
+    const float* proj;    /* depth x projNum x 4 clip coordinates */
+    const uint32_t* idx;  /* idxNum x 3 */
+    float* out;           /* depth x height x width x 4 */
+    float* outDB;         /* depth x height x width x 4, or NULL when enableDB == 0 */
+    int projNum;
+    int idxNum;
+    void* workspace;      /* device scratch of workspace_size bytes */
+    size_t workspace_size;
+} vcl_rasterize_params;
+
+typedef struct vcl_rasterize_grad_params {
+    float* proj;          /* depth x projNum x 4, accumulated (x, y, w) */
+    const float* out;     /* dL/d(u,v): float2 per pixel, packed (rasterize.cu:94) */
+    const float* outDB;   /* ignored, as in the reference (rasterize.cu:96-98) */
+} vcl_rasterize_grad_params;
+
+size_t vcl_rasterize_workspace_size(int width, int height, int depth, int projNum, int idxNum);
+int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t stream);
+int vcl_rasterize_backward(const vcl_rasterize_params* p, const vcl_rasterize_grad_params* g, vcl_stream_t stream);
+/* Device-side status of the last forward on this workspace (VCL_OK or VCL_ERR_OVERFLOW);
+ * synchronises the stream. */
+int vcl_rasterize_status(const vcl_rasterize_params* p, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Interpolate
+ * InterpolateKernelParams / InterpolateKernelGradParams (interpolate.h:7-29);
+ * Interpolate::forward/backward (interpolate.cu:51-60, 145-150). */
+typedef struct vcl_interpolate_params {
+    int width;
+    int height;
+    int depth;
+    int dimention;
+    int enableDA;
+    const float* rast;
+    const float* rastDB;
+    const float* attr;    /* attrNum x dimention */
+    const uint32_t* idx;  /* triangles x 3 */
+    float* out;           /* P x dimention (background written 0) */
+    float* outDA;         /* P x dimention x 2 */
+    int attrNum;          /* vertex count of attr */
+} vcl_interpolate_params;
+
+typedef struct vcl_interpolate_grad_params {
+    const float* out;
+    const float* outDA;   /* ignored, as in the reference (interpolate.cu:114-143) */
+    float* rast;          /* float2 per pixel, stored for covered pixels */
+    float* rastDB;        /* ignored */
+    float* attr;          /* accumulated, or NULL */
+} vcl_interpolate_grad_params;
+
+int vcl_interpolate_forward(const vcl_interpolate_params* p, vcl_stream_t stream);
+int vcl_interpolate_backward(const vcl_interpolate_params* p, const vcl_interpolate_grad_params* g, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Texturemap
+ * TexturemapKernelParams / TexturemapKernelGradParams (texturemap.h:8-31);
+ * Texturemap::forward/backward (texturemap.cu:116-123, 321-327).  Trilinear mip sampling when
+ * miplevel > 1, bilinear level 0 otherwise.  Levels are HWC, level l is
+ * (texwidth >> l) x (texheight >> l) x channel. */
+typedef struct vcl_texturemap_params {
+    int width;
+    int height;
+    int depth;
+    int texwidth;
+    int texheight;
+    int channel;
+    int miplevel;
+    const float* rast;
+    const float* uv;      /* P x 2 */
+    const float* uvDA;    /* P x 4 */
+    const float* texture[VCL_TEX_MAX_MIP_LEVEL];
+    float* out;           /* P x channel (background written 0) */
+} vcl_texturemap_params;
+
+enum { VCL_TEX_GRAD_DA_REF = 0, VCL_TEX_GRAD_DA_EXACT = 1 };
+
+typedef struct vcl_texturemap_grad_params {
+    const float* out;
+    float* uv;            /* P x 2, stored, or NULL */
+    float* uvDA;          /* P x 4, stored, or NULL */
+    float* texture[VCL_TEX_MAX_MIP_LEVEL]; /* accumulated, or NULL */
+    int daMode;           /* VCL_TEX_GRAD_DA_REF: formula of texturemap.cu:160-197,282;
+                             VCL_TEX_GRAD_DA_EXACT: exact d level / d uvDA */
+} vcl_texturemap_grad_params;
+
+int vcl_texturemap_forward(const vcl_texturemap_params* p, vcl_stream_t stream);
+int vcl_texturemap_backward(const vcl_texturemap_params* p, const vcl_texturemap_grad_params* g, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Texture (mip utilities)
+ * Texture / TextureGrad (buffer.h:54-78): buildMIP (buffer.cu:474-484), clamp (:588-594),
+ * TextureGrad::clear (:607-613), gradSumup (:638-648). */
+typedef struct vcl_texture {
+    int width;
+    int height;
+    int channel;
+    int miplevel;
+    float* texture[VCL_TEX_MAX_MIP_LEVEL];
+} vcl_texture;
+
+/* Mip level count of Texture::init (buffer.cu:380-385). */
+int vcl_texture_mip_levels(int width, int height, int requested);
+int vcl_texture_build_mip(const vcl_texture* t, vcl_stream_t stream);
+int vcl_texture_clamp(const vcl_texture* t, float min, float max, vcl_stream_t stream);
+/* grad holds the per-level gradients.  quarter = 1 scales each folded level by 1/4 (exact
+ * adjoint of buildMIP); quarter = 0 adds it unscaled as the reference does. */
+int vcl_texture_grad_sumup(const vcl_texture* grad, int quarter, vcl_stream_t stream);
+int vcl_texture_grad_clear(const vcl_texture* grad, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Optimizer / Adam
+ * OptimizeKernelParams (optimize.h:5-13), AdamHyperParams (optimize.h:50-59), adamStep
+ * (optimize.cu:95-107).  The caller advances rhomt/rhovt before each step
+ * (Adam::step, optimize.cu:109-115); the update is computed in double. */
+typedef struct vcl_optimize_params {
+    float* param;
+    const float* grad;
+    int size;
+    int width;
+    int height;
+    int depth;
+} vcl_optimize_params;
+
+typedef struct vcl_adam_hyper_params {
+    double rhom;
+    double rhov;
+    double rhomt;
+    double rhovt;
+    double eta;
+    double eps;
+    float* m;
+    float* v;
+} vcl_adam_hyper_params;
+
+int vcl_adam_step(const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream);
+/* Multi-tensor Adam: all n tensors in one launch. */
+int vcl_adam_step_multi(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream);
+/* Fused texture update: gradSumup -> Adam(level 0) -> TextureGrad::clear -> clamp -> buildMIP
+ * (the order of preset_primitives.cpp:200-202 with buffer.cu:588-648). */
+int vcl_texture_adam_update(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h,
+                            float clamp_min, float clamp_max, int quarter, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Antialias
+ * AntialiasKernelParams / AntialiasKernelGradParams (antialias.h:7-27);
+ * Antialias::forward/backward (antialias.cu:97-103, 302-313).  Race-free: out = in plus all
+ * edge-leak terms (the reference's copy races its atomics); the backward uses the forward's
+ * pixel selection and the exact derivative including w. */
+typedef struct vcl_antialias_params {
+    int width;
+    int height;
+    int depth;
+    int channel;
+    float s[2];           /* (width/2, height/2) */
+    const float* proj;    /* depth x projNum x 4 */
+    const uint32_t* idx;
+    const float* rast;
+    const float* in;
+    float* out;
+    int projNum;
+} vcl_antialias_params;
+
+typedef struct vcl_antialias_grad_params {
+    float* proj;          /* accumulated, or NULL */
+    float* in;            /* stored */
+    const float* out;
+} vcl_antialias_grad_params;
+
+int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t stream);
+int vcl_antialias_backward(const vcl_antialias_params* p, const vcl_antialias_grad_params* g, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Material
+ * NormalAxisKernelParams (material.h:9-25), NormalAxis::forward (material.cu:71-77).
+ * out = rot3x3 * normalize(barycentric normal) -- the effective reference semantics
+ * (its normal-map branch is inverted, material.cu:40-48); normalmap is ignored. */
+typedef struct vcl_normal_axis_params {
+    int width;
+    int height;
+    int depth;
+    const float* rast;
+    const float* rot;         /* 16 values, column-major (device) */
+    const float* normal;      /* N x 3 */
+    const uint32_t* normalidx;
+    const float* pos;
+    const uint32_t* posidx;
+    const float* texel;
+    const uint32_t* texelidx;
+    const float* normalmap;
+    float* out;               /* P x 3 (background written 0) */
+} vcl_normal_axis_params;
+
+int vcl_normal_axis_forward(const vcl_normal_axis_params* p, vcl_stream_t stream);
+
+/* ViewAxisKernelParams (material.h:52-65), ViewAxis::forward (material.cu:110-118).
+ * pvinv = inverse(mat3(projection) * mat3(view)) is formed on the host from the host
+ * matrices, as in the reference; one pair of matrices per layer. */
+typedef struct vcl_view_axis_params {
+    int width;
+    int height;
+    int depth;
+    const float* rast;        /* unused, kept for layout */
+    const float* rot;         /* unused, kept for layout */
+    const float* view;        /* host, depth x 16 column-major */
+    const float* projection;  /* host, depth x 16 column-major */
+    float* pvinv;             /* unused (the inverse travels as a kernel argument) */
+    float* out;               /* P x 3 */
+} vcl_view_axis_params;
+
+int vcl_view_axis_forward(const vcl_view_axis_params* p, vcl_stream_t stream);
+
+/* SGSpecularKernelParams / SGSpecularGradKernelParams (material.h:94-122);
+ * SGSpecular::forward/backward (material.cu:185-194, 251-256). */
+typedef struct vcl_sg_specular_params {
+    int width;
+    int height;
+    int depth;
+    const float* rast;
+    const float* normal;
+    const float* view;
+    const float* roughness;   /* P */
+    float ior;
+    float* outAxis;           /* P x 3 */
+    float* outSharpness;      /* P */
+    float* outAmplitude;      /* P */
+} vcl_sg_specular_params;
+
+typedef struct vcl_sg_specular_grad_params {
+    float* normal;            /* unused, as in the reference */
+    float* view;              /* unused */
+    float* roughness;         /* accumulated */
+    float ior;
+    const float* outAxis;     /* unused, as in the reference */
+    const float* outSharpness;
+    const float* outAmplitude;
+} vcl_sg_specular_grad_params;
+
+int vcl_sg_specular_forward(const vcl_sg_specular_params* p, vcl_stream_t stream);
+int vcl_sg_specular_backward(const vcl_sg_specular_params* p, const vcl_sg_specular_grad_params* g, vcl_stream_t stream);
+
+/* SphericalGaussianKernelParams / ...GradKernelParams (material.h:144-183);
+ * SphericalGaussian::forward/backward (material.cu:325-334, 440-445).  Lobe gradients are
+ * reduced in a fixed order (the reference accumulates them non-atomically), so results are
+ * deterministic; the workspace holds the per-block partials. */
+typedef struct vcl_spherical_gaussian_params {
+    int width;
+    int height;
+    int depth;
+    int channel;
+    const float* rast;
+    int sgnum;
+    const float* axis;        /* sgnum x 3 */
+    const float* sharpness;   /* sgnum */
+    const float* amplitude;   /* sgnum x channel */
+    const float* normal;
+    const float* view;        /* unused, kept for layout */
+    const float* diffuse;     /* P x channel */
+    const float* specAxis;
+    const float* specSharpness;
+    const float* specAmplitude;
+    float* out;
+    float* outDiffenv;
+    float* outSpecenv;
+} vcl_spherical_gaussian_params;
+
+typedef struct vcl_spherical_gaussian_grad_params {
+    float* normal;            /* P x 3, accumulated, or NULL */
+    float* diffuse;           /* P x channel, accumulated, or NULL */
+    float* axis;              /* sgnum x 3, accumulated, or NULL */
+    float* sharpness;         /* sgnum, accumulated, or NULL */
+    float* amplitude;         /* sgnum x channel, accumulated, or NULL */
+    float* specAxis;          /* P x 3, accumulated, or NULL */
+    float* specSharpness;     /* P, accumulated, or NULL */
+    float* specAmplitude;     /* P, accumulated, or NULL */
+    const float* out;
+    void* workspace;          /* >= vcl_spherical_gaussian_workspace_size() bytes */
+    size_t workspace_size;
+} vcl_spherical_gaussian_grad_params;
+
+size_t vcl_spherical_gaussian_workspace_size(int width, int height, int depth, int channel, int sgnum);
+int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_params* p, vcl_stream_t stream);
+int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
+                                    vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Loss
+ * LossKernelParams (loss.h:5-15), MSELoss::backward (loss.cu:41-57): grad = predict - target,
+ * loss = 0.5 * sum(grad^2) written to a device float (no host sync); fixed-order reduction. */
+typedef struct vcl_loss_params {
+    const float* predict;
+    const float* target;
+    float* grad;
+    float* buffer;            /* device scratch of vcl_loss_workspace_size() bytes */
+    int width;
+    int height;
+    int depth;
+    int size;                 /* element count */
+    float* loss;              /* device, 1 float */
+} vcl_loss_params;
+
+size_t vcl_loss_workspace_size(int size);
+int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Fit engine
+ * One inverse-rendering request (the chain of preset_primitives.cpp:160-207 generalised to
+ * vertex positions, mip texture and SG lobes): project -> rasterize(+DB) -> interpolate(uv)
+ * -> NormalAxis -> ViewAxis -> texture -> SGSpecular -> SphericalGaussian -> antialias ->
+ * MSE, backward, Adam.  All state stays resident on the device between steps. */
+typedef struct vcl_fit_desc {
+    int width, height, views;         /* views share mesh, texture and lobes */
+    int V, T;                         /* vertices, triangles */
+    int texwidth, texheight, channel, miplevel;
+    int sgnum;
+    const float* pos;                 /* host V x 3 (initial) */
+    const float* normals;             /* host V x 3 */
+    const float* uv;                  /* host V x 2 */
+    const uint32_t* idx;              /* host T x 3 */
+    const float* texture;             /* host, all levels back to back */
+    const float* axis;                /* host K x 3 */
+    const float* sharpness;           /* host K */
+    const float* amplitude;           /* host K x channel */
+    const float* mats;                /* host views x 16 (P*V*R) */
+    const float* pvinv;               /* host views x 9 */
+    const float* rot;                 /* host 16 */
+    const float* target;              /* host views x H x W x channel, or NULL */
+    float roughness, ior;
+    double lr, beta1, beta2, eps;
+    int fold_quarter;                 /* 1: exact mip-grad fold (default) */
+    int fused;                        /* 0: reference op-by-op sequence; 1: fused kernels */
+} vcl_fit_desc;
+
+typedef struct vcl_fit vcl_fit;       /* opaque */
+
+int vcl_fit_create(const vcl_fit_desc* d, vcl_fit** out);
+int vcl_fit_destroy(vcl_fit* f);
+/* Render the current parameters and store the image as the fit target. */
+int vcl_fit_set_target_from_render(vcl_fit* f, vcl_stream_t stream);
+/* One fwd+bwd+Adam iteration on the stream. */
+int vcl_fit_step(vcl_fit* f, vcl_stream_t stream);
+/* Device pointers of named buffers ("pos", "texture", "axis", "sharpness", "amplitude",
+ * "image", "loss", "clip", "rast", "db", "grad_pos", ...); NULL if unknown. */
+void* vcl_fit_buffer(vcl_fit* f, const char* name);
+size_t vcl_fit_device_bytes(const vcl_fit* f);
+/* Copy bytes between a named device buffer and host memory; synchronises the stream. */
+int vcl_fit_read(vcl_fit* f, const char* name, void* host, size_t bytes, vcl_stream_t stream);
+int vcl_fit_write(vcl_fit* f, const char* name, const void* host, size_t bytes, vcl_stream_t stream);
+/* Per-op HIP-event timing of the following steps (up to max_launches launches per op), the
+ * per-op timing the reference prints only as whole-step fps (main.cpp:39-49).  _end syncs and
+ * returns, per op, its name (names[i*name_len ...]), summed milliseconds and launch count. */
+int vcl_fit_profile_begin(vcl_fit* f, int max_launches);
+int vcl_fit_profile_end(vcl_fit* f, int max_ops, int* nops, char* names, int name_len, double* total_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VCL_H_ */

@@ -1,0 +1,218 @@
+"""Shared test helpers: seeded inputs, oracle calls on numpy arrays, comparisons."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+from vcl_amd import scene as S
+
+F32 = np.float32
+
+
+def rel_err(a, b):
+    """Normwise relative error max|a-b| / max(max|b|, tiny)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if a.size == 0:
+        return 0.0
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def assert_close(a, b, rtol, what, atol_frac=None):
+    """Elementwise |a-b| <= rtol*|b| + atol with atol = (atol_frac or rtol) * max|b|."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    assert a.shape == b.shape, f"{what}: shape {a.shape} vs {b.shape}"
+    scale = max(np.max(np.abs(b)) if b.size else 0.0, 1e-30)
+    atol = (rtol if atol_frac is None else atol_frac) * scale
+    bad = np.abs(a - b) > rtol * np.abs(b) + atol
+    n = int(bad.sum())
+    if n:
+        i = np.unravel_index(np.argmax(np.abs(a - b) - rtol * np.abs(b)), a.shape)
+        raise AssertionError(f"{what}: {n} of {a.size} elements out of tolerance (rtol {rtol}); worst at {i}: "
+                             f"{a[i]!r} vs {b[i]!r}; normwise rel err {rel_err(a, b):.3e}")
+
+
+# ------------------------------------------------------------------ oracle wrappers (numpy)
+def o_project(pos, mats, views=1, dt=F32):
+    V = pos.shape[0]
+    clip = np.zeros((views, V, 4), dt)
+    getattr(O.lib(), f"orc_project_fwd_{O.sfx(dt)}")(V, views, O.P(pos.astype(dt)), O.P(np.ascontiguousarray(mats, dt)), O.P(clip))
+    return clip
+
+
+def o_raster(clip, idx, W, H, views=1, dt=F32):
+    clip = np.ascontiguousarray(clip, dt)
+    rast = np.zeros((views, H, W, 4), dt)
+    db = np.zeros((views, H, W, 4), dt)
+    getattr(O.lib(), f"orc_raster_fwd_{O.sfx(dt)}")(W, H, views, clip.shape[-2], O.P(clip), idx.shape[0], O.P(idx), O.P(rast), O.P(db))
+    return rast, db
+
+
+def o_raster_bwd(clip, idx, rast, grast, dt=F32):
+    views, H, W = rast.shape[:3]
+    V = clip.shape[-2]
+    g = np.zeros((views, V, 4), dt)
+    getattr(O.lib(), f"orc_raster_bwd_{O.sfx(dt)}")(W, H, views, V, O.P(np.ascontiguousarray(clip, dt)), O.P(idx),
+                                                   O.P(np.ascontiguousarray(rast, dt)), O.P(np.ascontiguousarray(grast, dt)), O.P(g))
+    return g
+
+
+def o_interp(rast, db, attr, idx, dt=F32):
+    views, H, W = rast.shape[:3]
+    D = attr.shape[1]
+    out = np.zeros((views, H, W, D), dt)
+    da = np.zeros((views, H, W, D, 2), dt)
+    getattr(O.lib(), f"orc_interp_fwd_{O.sfx(dt)}")(W, H, views, D, O.P(np.ascontiguousarray(rast, dt)), O.P(np.ascontiguousarray(db, dt)),
+                                                   O.P(np.ascontiguousarray(attr, dt)), O.P(idx), O.P(out), O.P(da))
+    return out, da
+
+
+def o_interp_bwd(rast, attr, idx, gout, want_attr=True, dt=F32):
+    views, H, W = rast.shape[:3]
+    D = attr.shape[1]
+    ga = np.zeros(attr.shape, dt) if want_attr else None
+    gr = np.zeros((views, H, W, 2), dt)
+    getattr(O.lib(), f"orc_interp_bwd_{O.sfx(dt)}")(W, H, views, D, attr.shape[0], O.P(np.ascontiguousarray(rast, dt)),
+                                                   O.P(np.ascontiguousarray(attr, dt)), O.P(idx), O.P(np.ascontiguousarray(gout, dt)),
+                                                   O.P(ga), O.P(gr))
+    return ga, gr
+
+
+def o_tex(rast, uv, uvDA, tex, tw, th, C, L, dt=F32):
+    views, H, W = rast.shape[:3]
+    out = np.zeros((views, H, W, C), dt)
+    getattr(O.lib(), f"orc_tex_fwd_{O.sfx(dt)}")(W, H, views, tw, th, C, L, O.P(np.ascontiguousarray(tex, dt)),
+                                                O.P(np.ascontiguousarray(rast, dt)), O.P(np.ascontiguousarray(uv, dt)),
+                                                O.P(np.ascontiguousarray(uvDA, dt)), O.P(out))
+    return out
+
+
+def o_tex_bwd(rast, uv, uvDA, tex, tw, th, C, L, gout, mode=0, dt=F32):
+    views, H, W = rast.shape[:3]
+    gt = np.zeros(tex.shape, dt)
+    guv = np.zeros((views, H, W, 2), dt)
+    gda = np.zeros((views, H, W, 4), dt)
+    getattr(O.lib(), f"orc_tex_bwd_{O.sfx(dt)}")(W, H, views, tw, th, C, L, O.P(np.ascontiguousarray(tex, dt)),
+                                                O.P(np.ascontiguousarray(rast, dt)), O.P(np.ascontiguousarray(uv, dt)),
+                                                O.P(np.ascontiguousarray(uvDA, dt)), O.P(np.ascontiguousarray(gout, dt)), O.P(gt), O.P(guv),
+                                                O.P(gda), mode)
+    return gt, guv, gda
+
+
+def o_normal_axis(rast, normals, idx, rot, dt=F32):
+    views, H, W = rast.shape[:3]
+    out = np.zeros((views, H, W, 3), dt)
+    getattr(O.lib(), f"orc_normal_axis_{O.sfx(dt)}")(W, H, views, O.P(np.ascontiguousarray(rast, dt)), O.P(np.ascontiguousarray(normals, dt)),
+                                                    O.P(idx), O.P(np.ascontiguousarray(rot, dt)), O.P(out))
+    return out
+
+
+def o_view_axis(W, H, pvinv, dt=F32):
+    pv = np.ascontiguousarray(pvinv, dt).reshape(-1, 9)
+    views = pv.shape[0]
+    out = np.zeros((views, H, W, 3), dt)
+    getattr(O.lib(), f"orc_view_axis_{O.sfx(dt)}")(W, H, views, O.P(pv), O.P(out))
+    return out
+
+
+def o_spec(rast, normal, view, rough, ior, dt=F32):
+    views, H, W = rast.shape[:3]
+    ax = np.zeros((views, H, W, 3), dt)
+    sh = np.zeros((views, H, W), dt)
+    am = np.zeros((views, H, W), dt)
+    getattr(O.lib(), f"orc_sg_spec_fwd_{O.sfx(dt)}")(W, H, views, O.P(np.ascontiguousarray(rast, dt)), O.P(np.ascontiguousarray(normal, dt)),
+                                                    O.P(np.ascontiguousarray(view, dt)), O.P(np.ascontiguousarray(rough, dt)), ior,
+                                                    O.P(ax), O.P(sh), O.P(am))
+    return ax, sh, am
+
+
+def o_sg(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, dt=F32):
+    views, H, W = rast.shape[:3]
+    C = diffuse.shape[-1]
+    out = np.zeros((views, H, W, C), dt)
+    de = np.zeros_like(out)
+    se = np.zeros_like(out)
+    c = lambda a: O.P(np.ascontiguousarray(a, dt))  # noqa: E731
+    keep = [np.ascontiguousarray(a, dt) for a in (rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp)]
+    getattr(O.lib(), f"orc_sg_fwd_{O.sfx(dt)}")(W, H, views, C, sharp.size, *[O.P(k) for k in keep], O.P(out), O.P(de), O.P(se))
+    del c
+    return out, de, se
+
+
+def o_sg_bwd(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, gout, dt=F32, spec_grads=False):
+    views, H, W = rast.shape[:3]
+    C = diffuse.shape[-1]
+    K = sharp.size
+    keep = [np.ascontiguousarray(a, dt) for a in (rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, gout)]
+    gd = np.zeros((views, H, W, C), dt)
+    ga = np.zeros((K, 3), dt)
+    gs = np.zeros((K,), dt)
+    gm = np.zeros((K, C), dt)
+    gn = np.zeros((views, H, W, 3), dt) if spec_grads else None
+    gsa = np.zeros((views, H, W, 3), dt) if spec_grads else None
+    gss = np.zeros((views, H, W), dt) if spec_grads else None
+    gsm = np.zeros((views, H, W), dt) if spec_grads else None
+    getattr(O.lib(), f"orc_sg_bwd_{O.sfx(dt)}")(W, H, views, C, K, *[O.P(k) for k in keep], O.P(gd), O.P(ga), O.P(gs), O.P(gm),
+                                               O.P(gn), O.P(gsa), O.P(gss), O.P(gsm))
+    return gd, ga, gs, gm, gn, gsa, gss, gsm
+
+
+def o_aa(clip, idx, rast, inp, dt=F32):
+    views, H, W = rast.shape[:3]
+    C = inp.shape[-1]
+    out = np.zeros((views, H, W, C), dt)
+    keep = [np.ascontiguousarray(a, dt) for a in (clip, rast, inp)]
+    getattr(O.lib(), f"orc_aa_fwd_{O.sfx(dt)}")(W, H, views, C, clip.shape[-2], O.P(keep[0]), O.P(idx), O.P(keep[1]), O.P(keep[2]), O.P(out))
+    return out
+
+
+def o_aa_bwd(clip, idx, rast, inp, gout, dt=F32):
+    views, H, W = rast.shape[:3]
+    C = inp.shape[-1]
+    gin = np.zeros((views, H, W, C), dt)
+    gclip = np.zeros(np.asarray(clip).shape, dt)
+    keep = [np.ascontiguousarray(a, dt) for a in (clip, rast, inp, gout)]
+    getattr(O.lib(), f"orc_aa_bwd_{O.sfx(dt)}")(W, H, views, C, clip.shape[-2], O.P(keep[0]), O.P(idx), O.P(keep[1]), O.P(keep[2]),
+                                               O.P(keep[3]), O.P(gin), O.P(gclip))
+    return gin, gclip
+
+
+# ------------------------------------------------------------------ canned inputs
+def forward_inputs(sc, dt=F32):
+    """Run the oracle forward chain (float) of scene sc's first camera; returns a dict."""
+    cam = sc.cams[0]
+    d = {}
+    d["clip"] = o_project(sc.pos, cam.mat, dt=dt)
+    d["rast"], d["db"] = o_raster(d["clip"], sc.idx, sc.W, sc.H, dt=dt)
+    d["uvp"], d["uvDA"] = o_interp(d["rast"], d["db"], sc.uv, sc.idx, dt=dt)
+    d["nrm"] = o_normal_axis(d["rast"], sc.normals, sc.idx, np.eye(4, dtype=F32).reshape(16), dt=dt)
+    d["view"] = o_view_axis(sc.W, sc.H, cam.pvinv, dt=dt)
+    d["diff"] = o_tex(d["rast"], d["uvp"], d["uvDA"].reshape(1, sc.H, sc.W, 4), sc.tex, sc.tw, sc.th, sc.C, sc.L, dt=dt)
+    d["rough"] = np.full((1, sc.H, sc.W), sc.rough, dt)
+    d["sAxis"], d["sSharp"], d["sAmp"] = o_spec(d["rast"], d["nrm"], d["view"], d["rough"], sc.ior, dt=dt)
+    d["color"], d["denv"], d["senv"] = o_sg(d["rast"], sc.axis, sc.sharp, sc.amp, d["nrm"], d["diff"], d["sAxis"], d["sSharp"], d["sAmp"], dt=dt)
+    d["image"] = o_aa(d["clip"], sc.idx, d["rast"], d["color"], dt=dt)
+    return d
+
+
+def grid_mesh(n, z=0.5, w=1.0, jitter=0.0, seed=0):
+    """n x n quad grid in NDC [-1, 1]^2 (2 n^2 triangles), shared edges, optional jitter."""
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(-1, 1, n + 1)
+    X, Y = np.meshgrid(xs, xs)
+    P = np.stack([X.ravel(), Y.ravel()], 1)
+    if jitter:
+        P[:, :] += rng.uniform(-jitter, jitter, P.shape)
+    clip = np.zeros((len(P), 4), F32)
+    clip[:, 0] = P[:, 0] * w
+    clip[:, 1] = P[:, 1] * w
+    clip[:, 2] = z * w
+    clip[:, 3] = w
+    tris = []
+    for j in range(n):
+        for i in range(n):
+            a = j * (n + 1) + i
+            b, c, d = a + 1, a + n + 1, a + n + 2
+            tris += [(a, b, d), (a, d, c)]
+    return clip, np.array(tris, np.uint32)

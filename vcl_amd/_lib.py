@@ -1,0 +1,196 @@
+"""ctypes binding of libvcl_amd.so (the C ABI declared in include/vcl.h).
+
+Loading fails loudly when the library is missing: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvcl_amd.so")
+
+MAXL = 16
+vp = C.c_void_p
+ci = C.c_int
+cf = C.c_float
+cd = C.c_double
+csz = C.c_size_t
+
+VCL_OK, VCL_ERR_INVALID, VCL_ERR_OOM, VCL_ERR_HIP, VCL_ERR_OVERFLOW = range(5)
+
+
+class ProjectParams(C.Structure):
+    _fields_ = [("vecNum", ci), ("dimention", ci), ("vec", vp), ("mat", vp), ("out", vp), ("views", ci)]
+
+
+class ProjectGradParams(C.Structure):
+    _fields_ = [("vec", vp), ("mat", vp), ("out", vp)]
+
+
+class RasterizeParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("enableDB", ci), ("xs", cf), ("ys", cf),
+                ("proj", vp), ("idx", vp), ("out", vp), ("outDB", vp), ("projNum", ci), ("idxNum", ci),
+                ("workspace", vp), ("workspace_size", csz)]
+
+
+class RasterizeGradParams(C.Structure):
+    _fields_ = [("proj", vp), ("out", vp), ("outDB", vp)]
+
+
+class InterpolateParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("dimention", ci), ("enableDA", ci),
+                ("rast", vp), ("rastDB", vp), ("attr", vp), ("idx", vp), ("out", vp), ("outDA", vp), ("attrNum", ci)]
+
+
+class InterpolateGradParams(C.Structure):
+    _fields_ = [("out", vp), ("outDA", vp), ("rast", vp), ("rastDB", vp), ("attr", vp)]
+
+
+class TexturemapParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("texwidth", ci), ("texheight", ci), ("channel", ci),
+                ("miplevel", ci), ("rast", vp), ("uv", vp), ("uvDA", vp), ("texture", vp * MAXL), ("out", vp)]
+
+
+class TexturemapGradParams(C.Structure):
+    _fields_ = [("out", vp), ("uv", vp), ("uvDA", vp), ("texture", vp * MAXL), ("daMode", ci)]
+
+
+class Texture(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("channel", ci), ("miplevel", ci), ("texture", vp * MAXL)]
+
+
+class OptimizeParams(C.Structure):
+    _fields_ = [("param", vp), ("grad", vp), ("size", ci), ("width", ci), ("height", ci), ("depth", ci)]
+
+
+class AdamHyperParams(C.Structure):
+    _fields_ = [("rhom", cd), ("rhov", cd), ("rhomt", cd), ("rhovt", cd), ("eta", cd), ("eps", cd), ("m", vp), ("v", vp)]
+
+
+class AntialiasParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("channel", ci), ("s", cf * 2), ("proj", vp), ("idx", vp),
+                ("rast", vp), ("in_", vp), ("out", vp), ("projNum", ci)]
+
+
+class AntialiasGradParams(C.Structure):
+    _fields_ = [("proj", vp), ("in_", vp), ("out", vp)]
+
+
+class NormalAxisParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("rast", vp), ("rot", vp), ("normal", vp), ("normalidx", vp),
+                ("pos", vp), ("posidx", vp), ("texel", vp), ("texelidx", vp), ("normalmap", vp), ("out", vp)]
+
+
+class ViewAxisParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("rast", vp), ("rot", vp), ("view", vp), ("projection", vp),
+                ("pvinv", vp), ("out", vp)]
+
+
+class SGSpecularParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("rast", vp), ("normal", vp), ("view", vp), ("roughness", vp),
+                ("ior", cf), ("outAxis", vp), ("outSharpness", vp), ("outAmplitude", vp)]
+
+
+class SGSpecularGradParams(C.Structure):
+    _fields_ = [("normal", vp), ("view", vp), ("roughness", vp), ("ior", cf), ("outAxis", vp), ("outSharpness", vp),
+                ("outAmplitude", vp)]
+
+
+class SphericalGaussianParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("channel", ci), ("rast", vp), ("sgnum", ci), ("axis", vp),
+                ("sharpness", vp), ("amplitude", vp), ("normal", vp), ("view", vp), ("diffuse", vp), ("specAxis", vp),
+                ("specSharpness", vp), ("specAmplitude", vp), ("out", vp), ("outDiffenv", vp), ("outSpecenv", vp)]
+
+
+class SphericalGaussianGradParams(C.Structure):
+    _fields_ = [("normal", vp), ("diffuse", vp), ("axis", vp), ("sharpness", vp), ("amplitude", vp), ("specAxis", vp),
+                ("specSharpness", vp), ("specAmplitude", vp), ("out", vp), ("workspace", vp), ("workspace_size", csz)]
+
+
+class LossParams(C.Structure):
+    _fields_ = [("predict", vp), ("target", vp), ("grad", vp), ("buffer", vp), ("width", ci), ("height", ci), ("depth", ci),
+                ("size", ci), ("loss", vp)]
+
+
+class FitDesc(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("views", ci), ("V", ci), ("T", ci), ("texwidth", ci), ("texheight", ci),
+                ("channel", ci), ("miplevel", ci), ("sgnum", ci),
+                ("pos", vp), ("normals", vp), ("uv", vp), ("idx", vp), ("texture", vp), ("axis", vp), ("sharpness", vp),
+                ("amplitude", vp), ("mats", vp), ("pvinv", vp), ("rot", vp), ("target", vp),
+                ("roughness", cf), ("ior", cf), ("lr", cd), ("beta1", cd), ("beta2", cd), ("eps", cd),
+                ("fold_quarter", ci), ("fused", ci)]
+
+
+# Every entry point declared in include/vcl.h: name -> (argtypes, restype)
+P_ = C.POINTER
+SIGNATURES = {
+    "vcl_status_string": ([ci], C.c_char_p),
+    "vcl_build_info": ([], C.c_char_p),
+    "vcl_project_forward": ([P_(ProjectParams), vp], ci),
+    "vcl_project_backward": ([P_(ProjectParams), P_(ProjectGradParams), vp], ci),
+    "vcl_rasterize_workspace_size": ([ci, ci, ci, ci, ci], csz),
+    "vcl_rasterize_forward": ([P_(RasterizeParams), vp], ci),
+    "vcl_rasterize_backward": ([P_(RasterizeParams), P_(RasterizeGradParams), vp], ci),
+    "vcl_rasterize_status": ([P_(RasterizeParams), vp], ci),
+    "vcl_interpolate_forward": ([P_(InterpolateParams), vp], ci),
+    "vcl_interpolate_backward": ([P_(InterpolateParams), P_(InterpolateGradParams), vp], ci),
+    "vcl_texturemap_forward": ([P_(TexturemapParams), vp], ci),
+    "vcl_texturemap_backward": ([P_(TexturemapParams), P_(TexturemapGradParams), vp], ci),
+    "vcl_texture_mip_levels": ([ci, ci, ci], ci),
+    "vcl_texture_build_mip": ([P_(Texture), vp], ci),
+    "vcl_texture_clamp": ([P_(Texture), cf, cf, vp], ci),
+    "vcl_texture_grad_sumup": ([P_(Texture), ci, vp], ci),
+    "vcl_texture_grad_clear": ([P_(Texture), vp], ci),
+    "vcl_adam_step": ([P_(OptimizeParams), P_(AdamHyperParams), vp], ci),
+    "vcl_adam_step_multi": ([ci, P_(OptimizeParams), P_(AdamHyperParams), vp], ci),
+    "vcl_texture_adam_update": ([P_(Texture), P_(Texture), P_(AdamHyperParams), cf, cf, ci, vp], ci),
+    "vcl_antialias_forward": ([P_(AntialiasParams), vp], ci),
+    "vcl_antialias_backward": ([P_(AntialiasParams), P_(AntialiasGradParams), vp], ci),
+    "vcl_normal_axis_forward": ([P_(NormalAxisParams), vp], ci),
+    "vcl_view_axis_forward": ([P_(ViewAxisParams), vp], ci),
+    "vcl_sg_specular_forward": ([P_(SGSpecularParams), vp], ci),
+    "vcl_sg_specular_backward": ([P_(SGSpecularParams), P_(SGSpecularGradParams), vp], ci),
+    "vcl_spherical_gaussian_workspace_size": ([ci, ci, ci, ci, ci], csz),
+    "vcl_spherical_gaussian_forward": ([P_(SphericalGaussianParams), vp], ci),
+    "vcl_spherical_gaussian_backward": ([P_(SphericalGaussianParams), P_(SphericalGaussianGradParams), vp], ci),
+    "vcl_loss_workspace_size": ([ci], csz),
+    "vcl_mse_loss_backward": ([P_(LossParams), vp], ci),
+    "vcl_fit_create": ([P_(FitDesc), P_(vp)], ci),
+    "vcl_fit_destroy": ([vp], ci),
+    "vcl_fit_set_target_from_render": ([vp, vp], ci),
+    "vcl_fit_step": ([vp, vp], ci),
+    "vcl_fit_buffer": ([vp, C.c_char_p], vp),
+    "vcl_fit_device_bytes": ([vp], csz),
+    "vcl_fit_read": ([vp, C.c_char_p, vp, csz, vp], ci),
+    "vcl_fit_write": ([vp, C.c_char_p, vp, csz, vp], ci),
+    "vcl_fit_profile_begin": ([vp, ci], ci),
+    "vcl_fit_profile_end": ([vp, ci, P_(ci), vp, ci, P_(cd), P_(ci)], ci),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libvcl_amd.so; raises if it has not been built (python -m vcl_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m vcl_amd.build` (no CPU fallback exists)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = lib
+    return _lib
+
+
+class VclError(RuntimeError):
+    pass
+
+
+def check(status: int, what: str = ""):
+    if status != VCL_OK:
+        msg = load().vcl_status_string(status).decode()
+        raise VclError(f"{what}: {msg} ({status})")

@@ -1,0 +1,397 @@
+// engine.hip -- device-resident inverse-rendering fit request (the call chain of
+// preset_primitives.cpp:160-207, generalised to vertex positions + mip texture + SG lobes).
+// vcl_fit_create allocates every buffer once; vcl_fit_step only launches kernels (no host
+// sync, no allocation), so a step can be captured in a HIP graph and replayed.
+#include "common.hpp"
+#include <cstring>
+#include <string>
+#include <vector>
+#include <map>
+
+extern "C" int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream);
+extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo,
+                                             float hi, int quarter, int part, vcl_stream_t stream);
+
+// Per-op HIP-event timing (enabled between vcl_fit_profile_begin/end).
+struct OpTimer {
+    std::vector<std::string> names;
+    std::vector<std::vector<std::pair<hipEvent_t, hipEvent_t>>> ev;  // per op, per launch
+    std::vector<size_t> used;
+    bool on = false;
+    size_t cap = 0;
+};
+
+struct vcl_fit {
+    vcl_fit_desc d;
+    OpTimer prof;
+    size_t P1;          // pixels per view
+    size_t P;           // pixels over all views
+    size_t ntex;        // floats over all texture levels
+    std::vector<size_t> lvl_off;
+    std::map<std::string, void*> buf;
+    std::map<std::string, size_t> bytes;
+    size_t total = 0;
+    double b1t, b2t;    // Adam bias-correction powers (rhomt, rhovt of AdamHyperParams)
+    int it = 0;
+    size_t ws_rast = 0, ws_sg = 0, ws_loss = 0;
+
+    float* f(const char* n) { return (float*)buf.at(n); }
+    template <class T> T* t(const char* n) { return (T*)buf.at(n); }
+};
+
+static int alloc(vcl_fit* F, const char* name, size_t nbytes) {
+    void* p = nullptr;
+    size_t nb = nbytes ? nbytes : 4;
+    if (hipMalloc(&p, nb) != hipSuccess) return VCL_ERR_OOM;
+    if (hipMemset(p, 0, nb) != hipSuccess) return VCL_ERR_HIP;
+    F->buf[name] = p;
+    F->bytes[name] = nb;
+    F->total += nb;
+    return VCL_OK;
+}
+
+static int upload(vcl_fit* F, const char* name, const void* host, size_t nbytes) {
+    int st = alloc(F, name, nbytes);
+    if (st) return st;
+    if (host && nbytes) VCL_HIP_CHECK(hipMemcpy(F->buf[name], host, nbytes, hipMemcpyHostToDevice));
+    return VCL_OK;
+}
+
+static void prof_release(OpTimer& T) {
+    for (auto& v : T.ev)
+        for (auto& e : v) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    T.ev.clear(); T.names.clear(); T.used.clear(); T.on = false;
+}
+
+// Record the start/stop events of op `name` around a launch sequence.
+struct ProfScope {
+    OpTimer* T; int slot = -1; hipStream_t s;
+    ProfScope(vcl_fit* F, const char* name, hipStream_t st) : T(&F->prof), s(st) {
+        if (!T->on) return;
+        for (size_t i = 0; i < T->names.size(); i++) if (T->names[i] == name) { slot = (int)i; break; }
+        if (slot < 0) {
+            if (T->names.size() >= 64) return;
+            T->names.push_back(name);
+            T->ev.emplace_back();
+            T->used.push_back(0);
+            slot = (int)T->names.size() - 1;
+            for (size_t k = 0; k < T->cap; k++) {
+                hipEvent_t a, b;
+                if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) break;
+                T->ev[slot].push_back({a, b});
+            }
+        }
+        if (T->used[slot] >= T->ev[slot].size()) { slot = -1; return; }
+        (void)hipEventRecord(T->ev[slot][T->used[slot]].first, s);
+    }
+    ~ProfScope() {
+        if (slot < 0) return;
+        (void)hipEventRecord(T->ev[slot][T->used[slot]].second, s);
+        T->used[slot]++;
+    }
+};
+
+extern "C" int vcl_fit_profile_begin(vcl_fit* F, int max_launches) {
+    VCL_REQUIRE(F && max_launches > 0);
+    prof_release(F->prof);
+    F->prof.cap = (size_t)max_launches;
+    F->prof.on = true;
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_profile_end(vcl_fit* F, int max_ops, int* nops, char* names, int name_len, double* total_ms, int* count) {
+    VCL_REQUIRE(F && nops);
+    OpTimer& T = F->prof;
+    T.on = false;
+    int n = (int)T.names.size();
+    if (n > max_ops) n = max_ops;
+    *nops = n;
+    for (int i = 0; i < n; i++) {
+        double tot = 0;
+        for (size_t k = 0; k < T.used[i]; k++) {
+            VCL_HIP_CHECK(hipEventSynchronize(T.ev[i][k].second));
+            float ms = 0;
+            VCL_HIP_CHECK(hipEventElapsedTime(&ms, T.ev[i][k].first, T.ev[i][k].second));
+            tot += ms;
+        }
+        if (total_ms) total_ms[i] = tot;
+        if (count) count[i] = (int)T.used[i];
+        if (names && name_len > 0) {
+            std::strncpy(names + (size_t)i * name_len, T.names[i].c_str(), name_len - 1);
+            names[(size_t)i * name_len + name_len - 1] = 0;
+        }
+    }
+    prof_release(T);
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_destroy(vcl_fit* F) {
+    if (!F) return VCL_OK;
+    prof_release(F->prof);
+    for (auto& kv : F->buf) (void)hipFree(kv.second);
+    delete F;
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_create(const vcl_fit_desc* d, vcl_fit** out) {
+    VCL_REQUIRE(d && out && d->width > 0 && d->height > 0 && d->views >= 1 && d->V > 0 && d->T > 0);
+    VCL_REQUIRE(d->pos && d->normals && d->uv && d->idx && d->texture && d->mats && d->pvinv && d->rot);
+    VCL_REQUIRE(d->channel >= 1 && d->channel <= 4 && d->sgnum >= 0 && d->miplevel >= 1 && d->miplevel <= VCL_TEX_MAX_MIP_LEVEL);
+    VCL_REQUIRE(d->miplevel == vcl_texture_mip_levels(d->texwidth, d->texheight, d->miplevel));
+    vcl_fit* F = new vcl_fit();
+    F->d = *d;
+    F->d.target = nullptr;
+    F->P1 = (size_t)d->width * d->height;
+    F->P = F->P1 * d->views;
+    int C = d->channel, K = d->sgnum, V = d->V, Tn = d->T, nv = d->views;
+    size_t off = 0;
+    for (int l = 0; l < d->miplevel; l++) {
+        F->lvl_off.push_back(off);
+        off += (size_t)(d->texwidth >> l) * (d->texheight >> l) * C;
+    }
+    F->ntex = off;
+    size_t P = F->P, f4 = sizeof(float);
+    int st = VCL_OK;
+#define A(name, n) if (!st) st = alloc(F, name, (n))
+#define U(name, host, n) if (!st) st = upload(F, name, host, (n))
+    U("pos", d->pos, (size_t)V * 3 * f4);
+    U("normals", d->normals, (size_t)V * 3 * f4);
+    U("uv", d->uv, (size_t)V * 2 * f4);
+    U("idx", d->idx, (size_t)Tn * 3 * 4);
+    U("texture", d->texture, F->ntex * f4);
+    U("axis", d->axis, (size_t)K * 3 * f4);
+    U("sharpness", d->sharpness, (size_t)K * f4);
+    U("amplitude", d->amplitude, (size_t)K * C * f4);
+    U("mats", d->mats, (size_t)nv * 16 * f4);
+    U("pvinv", d->pvinv, (size_t)nv * 9 * f4);
+    U("rot", d->rot, 16 * f4);
+    A("target", P * C * f4);
+    if (!st && d->target) st = hipMemcpy(F->buf["target"], d->target, P * C * f4, hipMemcpyHostToDevice) == hipSuccess ? VCL_OK : VCL_ERR_HIP;
+    A("rough", P * f4);
+    if (!st) {
+        std::vector<float> r(P, d->roughness);
+        st = hipMemcpy(F->buf["rough"], r.data(), P * f4, hipMemcpyHostToDevice) == hipSuccess ? VCL_OK : VCL_ERR_HIP;
+    }
+    // Adam state
+    A("m_pos", (size_t)V * 3 * f4); A("v_pos", (size_t)V * 3 * f4);
+    A("m_tex", (size_t)d->texwidth * d->texheight * C * f4); A("v_tex", (size_t)d->texwidth * d->texheight * C * f4);
+    A("m_axis", (size_t)K * 3 * f4); A("v_axis", (size_t)K * 3 * f4);
+    A("m_sharp", (size_t)K * f4); A("v_sharp", (size_t)K * f4);
+    A("m_amp", (size_t)K * C * f4); A("v_amp", (size_t)K * C * f4);
+    // intermediates
+    A("clip", (size_t)nv * V * 4 * f4);
+    A("rast", P * 4 * f4); A("db", P * 4 * f4);
+    A("uvp", P * 2 * f4); A("uvDA", P * 4 * f4);
+    A("nrm", P * 3 * f4); A("view", P * 3 * f4);
+    A("diff", P * C * f4);
+    A("sAxis", P * 3 * f4); A("sSharp", P * f4); A("sAmp", P * f4);
+    A("color", P * C * f4); A("denv", P * C * f4); A("senv", P * C * f4);
+    A("image", P * C * f4); A("gimage", P * C * f4); A("gcolor", P * C * f4);
+    A("gdiff", P * C * f4);
+    A("gtex", F->ntex * f4);
+    A("guv", P * 2 * f4); A("guvDA", P * 4 * f4);
+    A("grast", P * 4 * f4);
+    A("gclip", (size_t)nv * V * 4 * f4);
+    A("grad_pos", (size_t)V * 3 * f4);
+    A("grad_axis", (size_t)K * 3 * f4); A("grad_sharpness", (size_t)K * f4); A("grad_amplitude", (size_t)K * C * f4);
+    A("loss", 4 * f4);
+    F->ws_rast = vcl_rasterize_workspace_size(d->width, d->height, nv, V, Tn);
+    F->ws_sg = vcl_spherical_gaussian_workspace_size(d->width, d->height, nv, C, K);
+    F->ws_loss = vcl_loss_workspace_size((int)(P * C));
+    A("ws_rast", F->ws_rast);
+    A("ws_sg", F->ws_sg);
+    A("ws_loss", F->ws_loss);
+#undef A
+#undef U
+    if (st) { vcl_fit_destroy(F); return st; }
+    F->b1t = d->beta1;
+    F->b2t = d->beta2;
+    *out = F;
+    return VCL_OK;
+}
+
+extern "C" size_t vcl_fit_device_bytes(const vcl_fit* F) { return F ? F->total : 0; }
+
+extern "C" void* vcl_fit_buffer(vcl_fit* F, const char* name) {
+    if (!F || !name) return nullptr;
+    if (!std::strcmp(name, "grad_texture")) name = "gtex";
+    auto it = F->buf.find(name);
+    return it == F->buf.end() ? nullptr : it->second;
+}
+
+extern "C" int vcl_fit_read(vcl_fit* F, const char* name, void* host, size_t nbytes, vcl_stream_t s) {
+    VCL_REQUIRE(F && name && host);
+    void* d = vcl_fit_buffer(F, name);
+    VCL_REQUIRE(d);
+    if (!std::strcmp(name, "grad_texture")) name = "gtex";
+    VCL_REQUIRE(nbytes <= F->bytes[name]);
+    VCL_HIP_CHECK(hipMemcpyAsync(host, d, nbytes, hipMemcpyDeviceToHost, as_hip(s)));
+    VCL_HIP_CHECK(hipStreamSynchronize(as_hip(s)));
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_write(vcl_fit* F, const char* name, const void* host, size_t nbytes, vcl_stream_t s) {
+    VCL_REQUIRE(F && name && host);
+    void* d = vcl_fit_buffer(F, name);
+    VCL_REQUIRE(d);
+    if (!std::strcmp(name, "grad_texture")) name = "gtex";
+    VCL_REQUIRE(nbytes <= F->bytes[name]);
+    VCL_HIP_CHECK(hipMemcpyAsync(d, host, nbytes, hipMemcpyHostToDevice, as_hip(s)));
+    VCL_HIP_CHECK(hipStreamSynchronize(as_hip(s)));
+    return VCL_OK;
+}
+
+static void fill_texture(vcl_fit* F, const char* name, vcl_texture& t) {
+    t.width = F->d.texwidth; t.height = F->d.texheight; t.channel = F->d.channel; t.miplevel = F->d.miplevel;
+    for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) t.texture[l] = l < t.miplevel ? F->f(name) + F->lvl_off[l] : nullptr;
+}
+
+// Forward chain up to the antialiased image.
+static int forward(vcl_fit* F, vcl_stream_t s) {
+    const vcl_fit_desc& d = F->d;
+    int W = d.width, H = d.height, nv = d.views, C = d.channel;
+    int st;
+    vcl_project_params pj = {d.V, 4, F->f("pos"), F->f("mats"), F->f("clip"), nv};
+    { ProfScope ps_(F, "project", as_hip(s)); st = vcl_project_forward(&pj, s); }
+    if (st) return st;
+    vcl_rasterize_params rp;
+    rp.width = W; rp.height = H; rp.depth = nv; rp.enableDB = 1;
+    rp.xs = 2.f / W; rp.ys = 2.f / H;
+    rp.proj = F->f("clip"); rp.idx = F->t<uint32_t>("idx");
+    rp.out = F->f("rast"); rp.outDB = F->f("db");
+    rp.projNum = d.V; rp.idxNum = d.T;
+    rp.workspace = F->buf["ws_rast"]; rp.workspace_size = F->ws_rast;
+    { ProfScope ps_(F, "rasterize", as_hip(s)); st = vcl_rasterize_forward(&rp, s); }
+    if (st) return st;
+    vcl_interpolate_params ip = {W, H, nv, 2, 1, F->f("rast"), F->f("db"), F->f("uv"), F->t<uint32_t>("idx"), F->f("uvp"), F->f("uvDA"), d.V};
+    { ProfScope ps_(F, "interpolate", as_hip(s)); st = vcl_interpolate_forward(&ip, s); }
+    if (st) return st;
+    vcl_normal_axis_params np;
+    std::memset(&np, 0, sizeof(np));
+    np.width = W; np.height = H; np.depth = nv; np.rast = F->f("rast"); np.rot = F->f("rot");
+    np.normal = F->f("normals"); np.normalidx = F->t<uint32_t>("idx"); np.out = F->f("nrm");
+    { ProfScope ps_(F, "normal_axis", as_hip(s)); st = vcl_normal_axis_forward(&np, s); }
+    if (st) return st;
+    { ProfScope ps_(F, "view_axis", as_hip(s)); st = vcl_view_axis_forward_dev(W, H, nv, F->f("pvinv"), F->f("view"), s); }
+    if (st) return st;
+    vcl_texturemap_params tp;
+    tp.width = W; tp.height = H; tp.depth = nv; tp.texwidth = d.texwidth; tp.texheight = d.texheight; tp.channel = C; tp.miplevel = d.miplevel;
+    tp.rast = F->f("rast"); tp.uv = F->f("uvp"); tp.uvDA = F->f("uvDA");
+    for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tp.texture[l] = l < d.miplevel ? F->f("texture") + F->lvl_off[l] : nullptr;
+    tp.out = F->f("diff");
+    { ProfScope ps_(F, "texture", as_hip(s)); st = vcl_texturemap_forward(&tp, s); }
+    if (st) return st;
+    vcl_sg_specular_params sp = {W, H, nv, F->f("rast"), F->f("nrm"), F->f("view"), F->f("rough"), d.ior, F->f("sAxis"), F->f("sSharp"), F->f("sAmp")};
+    { ProfScope ps_(F, "sg_specular", as_hip(s)); st = vcl_sg_specular_forward(&sp, s); }
+    if (st) return st;
+    vcl_spherical_gaussian_params gp;
+    gp.width = W; gp.height = H; gp.depth = nv; gp.channel = C; gp.rast = F->f("rast");
+    gp.sgnum = d.sgnum; gp.axis = F->f("axis"); gp.sharpness = F->f("sharpness"); gp.amplitude = F->f("amplitude");
+    gp.normal = F->f("nrm"); gp.view = F->f("view"); gp.diffuse = F->f("diff");
+    gp.specAxis = F->f("sAxis"); gp.specSharpness = F->f("sSharp"); gp.specAmplitude = F->f("sAmp");
+    gp.out = F->f("color"); gp.outDiffenv = F->f("denv"); gp.outSpecenv = F->f("senv");
+    { ProfScope ps_(F, "spherical_gaussian", as_hip(s)); st = vcl_spherical_gaussian_forward(&gp, s); }
+    if (st) return st;
+    vcl_antialias_params ap = {W, H, nv, C, {W / 2.f, H / 2.f}, F->f("clip"), F->t<uint32_t>("idx"), F->f("rast"), F->f("color"), F->f("image"), d.V};
+    { ProfScope ps_(F, "antialias", as_hip(s)); st = vcl_antialias_forward(&ap, s); }
+    if (st) return st;
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_set_target_from_render(vcl_fit* F, vcl_stream_t s) {
+    VCL_REQUIRE(F);
+    int st = forward(F, s);
+    if (st) return st;
+    VCL_HIP_CHECK(hipMemcpyAsync(F->buf["target"], F->buf["image"], F->P * F->d.channel * sizeof(float), hipMemcpyDeviceToDevice, as_hip(s)));
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_step(vcl_fit* F, vcl_stream_t s) {
+    VCL_REQUIRE(F);
+    const vcl_fit_desc& d = F->d;
+    int W = d.width, H = d.height, nv = d.views, C = d.channel, K = d.sgnum;
+    hipStream_t hs = as_hip(s);
+    int st;
+    // output-grad zeroing of the forward(XGradParams&) overloads
+    if ((st = vcl_fill_async(F->buf["gclip"], F->bytes["gclip"], hs))) return st;
+    if ((st = vcl_fill_async(F->buf["gdiff"], F->bytes["gdiff"], hs))) return st;
+    if ((st = forward(F, s))) return st;
+    vcl_loss_params lp = {F->f("image"), F->f("target"), F->f("gimage"), F->f("ws_loss"), W, H, nv, (int)(F->P * C), F->f("loss")};
+    { ProfScope ps_(F, "mse_loss", as_hip(s)); st = vcl_mse_loss_backward(&lp, s); }
+    if (st) return st;
+    // backward
+    vcl_antialias_params ap = {W, H, nv, C, {W / 2.f, H / 2.f}, F->f("clip"), F->t<uint32_t>("idx"), F->f("rast"), F->f("color"), F->f("image"), d.V};
+    vcl_antialias_grad_params ag = {F->f("gclip"), F->f("gcolor"), F->f("gimage")};
+    { ProfScope ps_(F, "antialias_bwd", as_hip(s)); st = vcl_antialias_backward(&ap, &ag, s); }
+    if (st) return st;
+    vcl_spherical_gaussian_params gp;
+    gp.width = W; gp.height = H; gp.depth = nv; gp.channel = C; gp.rast = F->f("rast");
+    gp.sgnum = K; gp.axis = F->f("axis"); gp.sharpness = F->f("sharpness"); gp.amplitude = F->f("amplitude");
+    gp.normal = F->f("nrm"); gp.view = F->f("view"); gp.diffuse = F->f("diff");
+    gp.specAxis = F->f("sAxis"); gp.specSharpness = F->f("sSharp"); gp.specAmplitude = F->f("sAmp");
+    gp.out = F->f("color"); gp.outDiffenv = F->f("denv"); gp.outSpecenv = F->f("senv");
+    vcl_spherical_gaussian_grad_params gg;
+    std::memset(&gg, 0, sizeof(gg));
+    gg.diffuse = F->f("gdiff"); gg.axis = F->f("grad_axis"); gg.sharpness = F->f("grad_sharpness"); gg.amplitude = F->f("grad_amplitude");
+    gg.out = F->f("gcolor"); gg.workspace = F->buf["ws_sg"]; gg.workspace_size = F->ws_sg;
+    { ProfScope ps_(F, "spherical_gaussian_bwd", as_hip(s)); st = vcl_spherical_gaussian_backward(&gp, &gg, s); }
+    if (st) return st;
+    vcl_texturemap_params tp;
+    tp.width = W; tp.height = H; tp.depth = nv; tp.texwidth = d.texwidth; tp.texheight = d.texheight; tp.channel = C; tp.miplevel = d.miplevel;
+    tp.rast = F->f("rast"); tp.uv = F->f("uvp"); tp.uvDA = F->f("uvDA");
+    for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tp.texture[l] = l < d.miplevel ? F->f("texture") + F->lvl_off[l] : nullptr;
+    tp.out = F->f("diff");
+    vcl_texturemap_grad_params tg;
+    tg.out = F->f("gdiff"); tg.uv = F->f("guv"); tg.uvDA = F->f("guvDA"); tg.daMode = VCL_TEX_GRAD_DA_REF;
+    for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tg.texture[l] = l < d.miplevel ? F->f("gtex") + F->lvl_off[l] : nullptr;
+    { ProfScope ps_(F, "texture_bwd", as_hip(s)); st = vcl_texturemap_backward(&tp, &tg, s); }
+    if (st) return st;
+    vcl_interpolate_params ip = {W, H, nv, 2, 1, F->f("rast"), F->f("db"), F->f("uv"), F->t<uint32_t>("idx"), F->f("uvp"), F->f("uvDA"), d.V};
+    vcl_interpolate_grad_params ig = {F->f("guv"), nullptr, F->f("grast"), nullptr, nullptr};
+    { ProfScope ps_(F, "interpolate_bwd", as_hip(s)); st = vcl_interpolate_backward(&ip, &ig, s); }
+    if (st) return st;
+    vcl_rasterize_params rp;
+    std::memset(&rp, 0, sizeof(rp));
+    rp.width = W; rp.height = H; rp.depth = nv; rp.enableDB = 1; rp.xs = 2.f / W; rp.ys = 2.f / H;
+    rp.proj = F->f("clip"); rp.idx = F->t<uint32_t>("idx"); rp.out = F->f("rast"); rp.outDB = F->f("db");
+    rp.projNum = d.V; rp.idxNum = d.T;
+    vcl_rasterize_grad_params rg = {F->f("gclip"), F->f("grast"), nullptr};
+    { ProfScope ps_(F, "rasterize_bwd", as_hip(s)); st = vcl_rasterize_backward(&rp, &rg, s); }
+    if (st) return st;
+    vcl_project_params pj = {d.V, 4, F->f("pos"), F->f("mats"), F->f("clip"), nv};
+    vcl_project_grad_params pg = {F->f("grad_pos"), nullptr, F->f("gclip")};
+    { ProfScope ps_(F, "project_bwd", as_hip(s)); st = vcl_project_backward(&pj, &pg, s); }
+    if (st) return st;
+    // update (Adam::step bumps rhomt/rhovt before the launch, optimize.cu:109-115)
+    F->it++;
+    F->b1t *= d.beta1;
+    F->b2t *= d.beta2;
+    vcl_adam_hyper_params ht = {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_tex"), F->f("v_tex")};
+    vcl_texture tex, gtex;
+    fill_texture(F, "texture", tex);
+    fill_texture(F, "gtex", gtex);
+    { ProfScope ps_(F, "texture_adam", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 1, s); }
+    if (st) return st;
+    { ProfScope ps_(F, "texture_mip", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 2, s); }
+    if (st) return st;
+    vcl_optimize_params op[4] = {
+        {F->f("pos"), F->f("grad_pos"), d.V * 3, d.V, 3, 1},
+        {F->f("axis"), F->f("grad_axis"), K * 3, K, 3, 1},
+        {F->f("sharpness"), F->f("grad_sharpness"), K, K, 1, 1},
+        {F->f("amplitude"), F->f("grad_amplitude"), K * C, K, C, 1}};
+    vcl_adam_hyper_params hp[4] = {
+        {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_pos"), F->f("v_pos")},
+        {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_axis"), F->f("v_axis")},
+        {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_sharp"), F->f("v_sharp")},
+        {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_amp"), F->f("v_amp")}};
+    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi(K > 0 ? 4 : 1, op, hp, s); }
+    if (st) return st;
+    // parameter-grad clears (AttributeGrad::clear, SGBufferGrad::clear)
+    if ((st = vcl_fill_async(F->buf["grad_pos"], F->bytes["grad_pos"], hs))) return st;
+    if (K > 0) {
+        if ((st = vcl_fill_async(F->buf["grad_axis"], F->bytes["grad_axis"], hs))) return st;
+        if ((st = vcl_fill_async(F->buf["grad_sharpness"], F->bytes["grad_sharpness"], hs))) return st;
+        if ((st = vcl_fill_async(F->buf["grad_amplitude"], F->bytes["grad_amplitude"], hs))) return st;
+    }
+    return VCL_OK;
+}

@@ -1,0 +1,1289 @@
+// ops.hip -- per-pixel ops of the vcl hot path on gfx950: project, interpolate, mip texture,
+// NormalAxis, ViewAxis, SGSpecular, SphericalGaussian, antialias, MSE, Adam and the mip
+// utilities.  One thread per pixel over the row-major image (coalesced, background pixels exit
+// early); reductions that the reference does with racy or atomic float adds are done with
+// fixed-order block partials so results are deterministic.
+#include "common.hpp"
+#include <cmath>
+
+__global__ void vcl_fill_u32(u32* p, size_t n, u32 v) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+int vcl_fill_async(void* p, size_t bytes, hipStream_t s) {
+    if (!p || bytes == 0) return VCL_OK;
+    size_t n = bytes / 4;
+    unsigned g = cdiv(n, 256);
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(vcl_fill_u32, dim3(g), dim3(256), 0, s, (u32*)p, n, 0u);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+namespace {
+
+constexpr int BS = 256;
+
+// ---------------------------------------------------------------- deterministic column sums
+// out[c] += sum_r part[r * cols + c] in a fixed order (two stages of fixed segments).
+__global__ void __launch_bounds__(256) k_colsum_stage(const float* part, int rows, int cols, int seg_rows, float* seg) {
+    // block: 64 columns x 4 row groups; grid.y: segment
+    int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    int rg = threadIdx.x >> 6;
+    int r0 = blockIdx.y * seg_rows, r1 = min(rows, r0 + seg_rows);
+    float s = 0.f;
+    if (c < cols)
+        for (int r = r0 + rg; r < r1; r += 4) s += part[(size_t)r * cols + c];
+    __shared__ float sh[4][64];
+    sh[rg][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (rg == 0 && c < cols) seg[(size_t)blockIdx.y * cols + c] = ((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) k_colsum_final(const float* seg, int nseg, int cols, float scale, float* out, int nancheck) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.f;
+    for (int r = 0; r < nseg; r++) s += seg[(size_t)r * cols + c];
+    s *= scale;
+    if (nancheck) add_nan_check(out[c], s);
+    else out[c] += s;
+}
+
+constexpr int COLSUM_SEGS = 32;
+
+size_t colsum_scratch_floats(int cols) { return (size_t)COLSUM_SEGS * cols; }
+
+int colsum(const float* part, int rows, int cols, float* seg_scratch, float* out, float scale, int nancheck, hipStream_t s) {
+    int seg_rows = (rows + COLSUM_SEGS - 1) / COLSUM_SEGS;
+    if (seg_rows < 1) seg_rows = 1;
+    int nseg = (rows + seg_rows - 1) / seg_rows;
+    if (nseg < 1) nseg = 1;
+    hipLaunchKernelGGL(k_colsum_stage, dim3(cdiv(cols, 64), nseg), dim3(256), 0, s, part, rows, cols, seg_rows, seg_scratch);
+    VCL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_colsum_final, dim3(cdiv(cols, 256)), dim3(256), 0, s, seg_scratch, nseg, cols, scale, out, nancheck);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+// ---------------------------------------------------------------- project
+__global__ void __launch_bounds__(BS) k_project_fwd(int V, int dim, int views, const float* vec, const float* mat, float* out) {
+#pragma clang fp contract(off)
+    size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= (size_t)V * views) return;
+    int s = (int)(gid / V), v = (int)(gid % V);
+    const float* m = mat + 16 * s;
+    float x = vec[3 * v], y = vec[3 * v + 1], z = vec[3 * v + 2];
+    float r[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) r[i] = m[i] * x + m[4 + i] * y + m[8 + i] * z + m[12 + i];
+    if (dim == 4) ((float4*)out)[gid] = make_float4(r[0], r[1], r[2], r[3]);
+    else for (int i = 0; i < dim; i++) out[gid * dim + i] = r[i];
+}
+
+// gvec[v] += sum over views of M^T g (views in order).  gmat (camera-matrix grad, off the
+// fit path) is reduced per wave and added with one atomic per wave and entry.
+__global__ void __launch_bounds__(BS) k_project_bwd(int V, int dim, int views, const float* vec, const float* mat, const float* gout,
+                                                    float* gvec, float* gmat) {
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = v < V;
+    float gx = 0, gy = 0, gz = 0;
+    float pv[4] = {ok ? vec[3 * v] : 0.f, ok ? vec[3 * v + 1] : 0.f, ok ? vec[3 * v + 2] : 0.f, ok ? 1.f : 0.f};
+    for (int s = 0; s < views; s++) {
+        const float* m = mat + 16 * s;
+        float g4[4];
+        for (int i = 0; i < 4; i++) {
+            float g = (ok && i < dim) ? gout[((size_t)s * V + v) * dim + i] : 0.f;
+            g4[i] = g;
+            gx += m[i] * g; gy += m[4 + i] * g; gz += m[8 + i] * g;
+        }
+        if (gmat) {
+            for (int j = 0; j < 4; j++)
+                for (int i = 0; i < 4; i++) {
+                    float val = wave_sum(pv[j] * g4[i]);
+                    if ((threadIdx.x & 63) == 0 && val != 0.f) atomicAdd(gmat + 16 * s + 4 * j + i, val);
+                }
+        }
+    }
+    if (ok && gvec) { gvec[3 * v] += gx; gvec[3 * v + 1] += gy; gvec[3 * v + 2] += gz; }
+}
+
+// ---------------------------------------------------------------- interpolate
+__global__ void __launch_bounds__(BS) k_interp_fwd(size_t P, int D, int enableDA, const float* rast, const float* db, const float* attr,
+                                                   const u32* idx, float* out, float* outDA) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float4 r = ((const float4*)rast)[p];
+    int id = (int)r.w - 1;
+    if (id < 0) {
+        for (int i = 0; i < D; i++) {
+            out[p * D + i] = 0.f;
+            if (enableDA) ((float2*)outDA)[p * D + i] = make_float2(0.f, 0.f);
+        }
+        return;
+    }
+    u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
+    float4 b = enableDA ? ((const float4*)db)[p] : make_float4(0, 0, 0, 0);
+    float w2 = 1.0f - r.x - r.y;
+    for (int i = 0; i < D; i++) {
+        float a0 = attr[(size_t)i0 * D + i], a1 = attr[(size_t)i1 * D + i], a2 = attr[(size_t)i2 * D + i];
+        out[p * D + i] = a0 * r.x + a1 * r.y + a2 * w2;
+        if (enableDA) {
+            float dadu = a0 - a2, dadv = a1 - a2;
+            ((float2*)outDA)[p * D + i] = make_float2(dadu * b.x + dadv * b.z, dadu * b.y + dadv * b.w);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_interp_bwd(size_t P, int D, const float* rast, const float* attr, const u32* idx, const float* gout,
+                                                   float* gattr, float* grast) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float4 r = ((const float4*)rast)[p];
+    int id = (int)r.w - 1;
+    if (id < 0) return;
+    u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
+    float gu = 0.f, gv = 0.f;
+    float w2 = 1.0f - r.x - r.y;
+    for (int i = 0; i < D; i++) {
+        float g = gout[p * D + i];
+        if (gattr) {
+            atomicAdd(gattr + (size_t)i0 * D + i, g * r.x);
+            atomicAdd(gattr + (size_t)i1 * D + i, g * r.y);
+            atomicAdd(gattr + (size_t)i2 * D + i, g * w2);
+        }
+        float a2 = attr[(size_t)i2 * D + i];
+        gu += g * (attr[(size_t)i0 * D + i] - a2);
+        gv += g * (attr[(size_t)i1 * D + i] - a2);
+    }
+    if (grast) ((float2*)grast)[p] = make_float2(gu, gv);
+}
+
+// ---------------------------------------------------------------- mip texture
+struct TexArgs {
+    int tw, th, C, L;
+    const float* tex[VCL_TEX_MAX_MIP_LEVEL];
+};
+struct TexGradArgs {
+    float* tex[VCL_TEX_MAX_MIP_LEVEL];
+};
+
+__device__ __forceinline__ void calc_level(float4 da, int tw, int th, int L, int& l0, int& l1, float& f, float& level, float4* dl, int mode) {
+    float dsdx = da.x * tw, dsdy = da.y * tw, dtdx = da.z * th, dtdy = da.w * th;
+    float s2 = dsdx * dsdx + dsdy * dsdy, t2 = dtdx * dtdx + dtdy * dtdy;
+    float b = .5f * (s2 + t2);
+    // c^2 = b^2 - a^2 = (b - a)(b + a), each factor a sum of squares (no cancellation)
+    float pm = dsdx - dtdy, qm = dsdy + dtdx, pp = dsdx + dtdy, qp = dsdy - dtdx;
+    float c2 = .25f * ((pm * pm + qm * qm) * (pp * pp + qp * qp));
+    float c = sqrtf(c2);
+    level = .5f * log2f(b + c);
+    l0 = level <= 0 ? 0 : (float)(L - 2) <= level ? L - 2 : (int)floorf(level);
+    l1 = level <= 1 ? 1 : (float)(L - 1) <= level ? L - 1 : (int)floorf(level) + 1;
+    f = level <= 0 ? 0.f : (float)(L - 1) <= level ? 1.f : level - floorf(level);
+    if (!dl) return;
+    float a = dsdx * dtdy - dtdx * dsdy;
+    if (mode == 0) {
+        float d = b * c + c2;
+        if (fabsf(d) > 1e-6) {
+            d = 0.72134752f / d;
+            float bc = b + c;
+            *dl = make_float4(d * (bc * dsdx - a * dtdy), d * (bc * dsdy + a * dtdx), (bc * dtdx + a * dsdy), (bc * dtdy - a * dsdx));
+        } else if (fabsf(b) > 1e-6) {
+            d = 1.f / b;
+            *dl = make_float4(d * dsdx, d * dsdy, d * dtdx, d * dtdy);
+        } else {
+            *dl = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    } else {
+        float k = 1.f / (2.f * 0.69314718055994530942f * (b + c));
+        bool live = level > 0 && level < (float)(L - 1) && c > 0.f && isfinite(k);
+        if (live) {
+            float ic = 1.f / c;
+            float p0 = (dsdx + (b * dsdx - a * dtdy) * ic) * k;
+            float p1 = (dsdy + (b * dsdy + a * dtdx) * ic) * k;
+            float p2 = (dtdx + (b * dtdx + a * dsdy) * ic) * k;
+            float p3 = (dtdy + (b * dtdy - a * dsdx) * ic) * k;
+            *dl = make_float4(p0 * tw, p1 * tw, p2 * th, p3 * th);
+        } else {
+            *dl = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+}
+
+__device__ __forceinline__ void index_fetch(int tw, int th, int level, float2 uv, int4& id, float& fx, float& fy) {
+    int sx = tw >> level, sy = th >> level;
+    float tx = uv.x * (float)sx, ty = uv.y * (float)sy;
+    int u0 = tx < 0 ? 0 : tx > sx - 1 ? sx - 1 : (int)tx;
+    int u1 = tx < 1 ? 0 : tx > sx - 2 ? sx - 1 : (int)tx + 1;
+    int v0 = ty < 0 ? 0 : ty > sy - 1 ? sy - 1 : (int)ty;
+    int v1 = ty < 1 ? 0 : ty > sy - 2 ? sy - 1 : (int)ty + 1;
+    id = make_int4(v0 * sx + u0, v0 * sx + u1, v1 * sx + u0, v1 * sx + u1);
+    fx = tx < 0 ? 0.f : (float)sx < tx ? 1.f : tx - floorf(tx);
+    fy = ty < 0 ? 0.f : (float)sy < ty ? 1.f : ty - floorf(ty);
+}
+
+__device__ __forceinline__ float lerpf_(float a, float b, float t) { return a + t * (b - a); }
+__device__ __forceinline__ float4 lerp4(float4 a, float4 b, float t) {
+    return make_float4(lerpf_(a.x, b.x, t), lerpf_(a.y, b.y, t), lerpf_(a.z, b.z, t), lerpf_(a.w, b.w, t));
+}
+__device__ __forceinline__ float4 bilerp4(float4 a, float4 b, float4 c, float4 d, float tx, float ty) {
+    return lerp4(lerp4(a, b, tx), lerp4(c, d, tx), ty);
+}
+
+// C == 4 fast path: texels as float4.
+__global__ void __launch_bounds__(BS) k_tex_fwd4(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (rast[4 * p + 3] >= 1.f) {
+        float2 u = ((const float2*)uv)[p];
+        if (t.L > 1) {
+            int l0, l1; float f, lev;
+            calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, nullptr, 0);
+            int4 i0; float fx, fy;
+            index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
+            const float4* T0 = (const float4*)t.tex[l0];
+            o = bilerp4(T0[i0.x], T0[i0.y], T0[i0.z], T0[i0.w], fx, fy);
+            if (f > 0) {
+                int4 i1; float gx, gy;
+                index_fetch(t.tw, t.th, l1, u, i1, gx, gy);
+                const float4* T1 = (const float4*)t.tex[l1];
+                o = lerp4(o, bilerp4(T1[i1.x], T1[i1.y], T1[i1.z], T1[i1.w], gx, gy), f);
+            }
+        } else {
+            int4 i0; float fx, fy;
+            index_fetch(t.tw, t.th, 0, u, i0, fx, fy);
+            const float4* T0 = (const float4*)t.tex[0];
+            o = bilerp4(T0[i0.x], T0[i0.y], T0[i0.z], T0[i0.w], fx, fy);
+        }
+    }
+    ((float4*)out)[p] = o;
+}
+
+__global__ void __launch_bounds__(BS) k_tex_fwd(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    int C = t.C;
+    if (rast[4 * p + 3] < 1.f) { for (int c = 0; c < C; c++) out[p * C + c] = 0.f; return; }
+    float2 u = ((const float2*)uv)[p];
+    int l0 = 0, l1 = 0; float f = 0.f, lev;
+    if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, nullptr, 0);
+    int4 i0, i1; float fx, fy, gx = 0, gy = 0;
+    index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
+    if (t.L > 1) index_fetch(t.tw, t.th, l1, u, i1, gx, gy);
+    const float* T0 = t.tex[l0];
+    const float* T1 = t.tex[l1];
+    for (int c = 0; c < C; c++) {
+        float v = lerpf_(lerpf_(T0[i0.x * C + c], T0[i0.y * C + c], fx), lerpf_(T0[i0.z * C + c], T0[i0.w * C + c], fx), fy);
+        if (t.L > 1 && f > 0) {
+            float v1 = lerpf_(lerpf_(T1[i1.x * C + c], T1[i1.y * C + c], gx), lerpf_(T1[i1.z * C + c], T1[i1.w * C + c], gx), gy);
+            v = lerpf_(v, v1, f);
+        }
+        out[p * C + c] = v;
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_tex_bwd(size_t P, TexArgs t, TexGradArgs g, const float* rast, const float* uv, const float* uvDA,
+                                                const float* gout, float* guv, float* guvDA, int mode) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    if (rast[4 * p + 3] < 1.f) return;
+    int C = t.C;
+    float2 u = ((const float2*)uv)[p];
+    if (t.L <= 1) {
+        int4 i0; float fx, fy;
+        index_fetch(t.tw, t.th, 0, u, i0, fx, fy);
+        float gu = 0.f, gv = 0.f;
+        const float* T0 = t.tex[0];
+        float* G0 = g.tex[0];
+        for (int c = 0; c < C; c++) {
+            float go = gout[p * C + c];
+            if (G0) {
+                atomicAdd(G0 + i0.x * C + c, (1.f - fx) * (1.f - fy) * go);
+                atomicAdd(G0 + i0.y * C + c, fx * (1.f - fy) * go);
+                atomicAdd(G0 + i0.z * C + c, (1.f - fx) * fy * go);
+                atomicAdd(G0 + i0.w * C + c, fx * fy * go);
+            }
+            float t00 = T0[i0.x * C + c], t01 = T0[i0.y * C + c], t10 = T0[i0.z * C + c], t11 = T0[i0.w * C + c];
+            gu += lerpf_(t01 - t00, t11 - t10, fy) * (float)t.tw * go;
+            gv += lerpf_(t10 - t00, t11 - t01, fx) * (float)t.th * go;
+        }
+        if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
+        return;
+    }
+    int l0, l1; float f, lev; float4 dl;
+    calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, &dl, mode);
+    int4 i0, i1; float fx, fy, hx, hy;
+    index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
+    index_fetch(t.tw, t.th, l1, u, i1, hx, hy);
+    const float* T0 = t.tex[l0];
+    const float* T1 = t.tex[l1];
+    float* G0 = g.tex[l0];
+    float* G1 = g.tex[l1];
+    float gu = 0.f, gv = 0.f, gl = 0.f;
+    float w0 = 1.f - f;
+    for (int c = 0; c < C; c++) {
+        float go = gout[p * C + c];
+        if (G0) {
+            atomicAdd(G0 + i0.x * C + c, w0 * (1.f - fx) * (1.f - fy) * go);
+            atomicAdd(G0 + i0.y * C + c, w0 * fx * (1.f - fy) * go);
+            atomicAdd(G0 + i0.z * C + c, w0 * (1.f - fx) * fy * go);
+            atomicAdd(G0 + i0.w * C + c, w0 * fx * fy * go);
+        }
+        float t00 = T0[i0.x * C + c], t01 = T0[i0.y * C + c], t10 = T0[i0.z * C + c], t11 = T0[i0.w * C + c];
+        float du = lerpf_(t01 - t00, t11 - t10, fy) * (float)(t.tw >> l0);
+        float dv = lerpf_(t10 - t00, t11 - t01, fx) * (float)(t.th >> l0);
+        if (f > 0) {
+            float lv = lerpf_(lerpf_(t00, t01, fx), lerpf_(t10, t11, fx), fy);
+            if (G1) {
+                atomicAdd(G1 + i1.x * C + c, f * (1.f - hx) * (1.f - hy) * go);
+                atomicAdd(G1 + i1.y * C + c, f * hx * (1.f - hy) * go);
+                atomicAdd(G1 + i1.z * C + c, f * (1.f - hx) * hy * go);
+                atomicAdd(G1 + i1.w * C + c, f * hx * hy * go);
+            }
+            float s00 = T1[i1.x * C + c], s01 = T1[i1.y * C + c], s10 = T1[i1.z * C + c], s11 = T1[i1.w * C + c];
+            du = lerpf_(du, lerpf_(s01 - s00, s11 - s10, hy) * (float)(t.tw >> l1), f);
+            dv = lerpf_(dv, lerpf_(s10 - s00, s11 - s01, hx) * (float)(t.th >> l1), f);
+            gl += (lerpf_(lerpf_(s00, s01, hx), lerpf_(s10, s11, hx), hy) - lv) * go;
+        }
+        gu += du * go;
+        gv += dv * go;
+    }
+    if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
+    if (guvDA) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
+}
+
+// ---------------------------------------------------------------- mip utilities
+__global__ void __launch_bounds__(BS) k_downsample(const float* src, float* dst, int w, int h, int C, float* zero_grad) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)w * h * C;
+    if (i >= n) return;
+    int c = (int)(i % C);
+    size_t q = i / C;
+    int x = (int)(q % w), y = (int)(q / w);
+    int sw = 2 * w;
+    float p00 = src[((size_t)(2 * y) * sw + 2 * x) * C + c], p01 = src[((size_t)(2 * y) * sw + 2 * x + 1) * C + c];
+    float p10 = src[((size_t)(2 * y + 1) * sw + 2 * x) * C + c], p11 = src[((size_t)(2 * y + 1) * sw + 2 * x + 1) * C + c];
+    dst[i] = (p00 + p01 + p10 + p11) * 0.25f;
+    if (zero_grad) zero_grad[i] = 0.f;
+}
+
+__global__ void __launch_bounds__(BS) k_fold(const float* src, float* dst, int w, int h, int C, float scale) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)w * h * C;
+    if (i >= n) return;
+    int c = (int)(i % C);
+    size_t q = i / C;
+    int x = (int)(q % w), y = (int)(q / w);
+    int dw = 2 * w;
+    float g = src[i] * scale;
+    add_nan_check(dst[((size_t)(2 * y) * dw + 2 * x) * C + c], g);
+    add_nan_check(dst[((size_t)(2 * y) * dw + 2 * x + 1) * C + c], g);
+    add_nan_check(dst[((size_t)(2 * y + 1) * dw + 2 * x) * C + c], g);
+    add_nan_check(dst[((size_t)(2 * y + 1) * dw + 2 * x + 1) * C + c], g);
+}
+
+__global__ void __launch_bounds__(BS) k_clamp(float* x, size_t n, float lo, float hi) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float v = x[i];
+    x[i] = v < lo ? lo : v > hi ? hi : v;
+}
+
+// ---------------------------------------------------------------- Adam
+constexpr int ADAM_MAX = 8;
+struct AdamArgs {
+    int n;
+    float* param[ADAM_MAX];
+    const float* grad[ADAM_MAX];
+    float* m[ADAM_MAX];
+    float* v[ADAM_MAX];
+    long long size[ADAM_MAX];
+    long long block_start[ADAM_MAX + 1];
+    double rhom[ADAM_MAX], rhov[ADAM_MAX], rhomt[ADAM_MAX], rhovt[ADAM_MAX], eta[ADAM_MAX], eps[ADAM_MAX];
+};
+
+__device__ __forceinline__ void adam_elem(float& param, float g, float& m, float& v, double rhom, double rhov, double rhomt, double rhovt,
+                                          double eta, double eps) {
+#pragma clang fp contract(off)
+    m = (float)(rhom * (double)m + (1 - rhom) * (double)g);
+    v = (float)(rhov * (double)v + (1 - rhov) * (double)g * (double)g);
+    double mh = (double)m / (1 - rhomt);
+    double vh = (double)v / (1 - rhovt);
+    add_nan_check(param, (float)(-mh * eta / (sqrt(vh) + eps)));
+}
+
+constexpr int ADAM_PER_BLOCK = BS * 4;
+
+__global__ void __launch_bounds__(BS) k_adam(AdamArgs a) {
+    long long b = blockIdx.x;
+    int t = 0;
+    while (t + 1 < a.n && b >= a.block_start[t + 1]) t++;
+    long long base = (b - a.block_start[t]) * ADAM_PER_BLOCK;
+    float* P = a.param[t]; const float* G = a.grad[t]; float* M = a.m[t]; float* Vv = a.v[t];
+    long long n = a.size[t];
+    for (int k = 0; k < 4; k++) {
+        long long i = base + k * BS + threadIdx.x;
+        if (i >= n) break;
+        float p = P[i], m = M[i], v = Vv[i];
+        adam_elem(p, G[i], m, v, a.rhom[t], a.rhov[t], a.rhomt[t], a.rhovt[t], a.eta[t], a.eps[t]);
+        P[i] = p; M[i] = m; Vv[i] = v;
+    }
+}
+
+// Fused texture update on level 0: folded grad (coarse to fine, AddNaNcheck chain of
+// gardAddThrough, buffer.cu:615-636) -> Adam -> clamp; level-0 grad zeroed.
+struct TexUpdArgs {
+    int tw, th, C, L;
+    float* tex0;
+    float* grad[VCL_TEX_MAX_MIP_LEVEL];
+    float* m;
+    float* v;
+    double rhom, rhov, rhomt, rhovt, eta, eps;
+    float lo, hi, scale;
+};
+
+__global__ void __launch_bounds__(BS) k_tex_update(TexUpdArgs a) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)a.tw * a.th * a.C;
+    if (i >= n) return;
+    int c = (int)(i % a.C);
+    size_t q = i / a.C;
+    int x = (int)(q % a.tw), y = (int)(q / a.tw);
+    float acc = 0.f;
+    if (a.L > 1) {
+        int l = a.L - 1;
+        acc = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * a.C + c];
+        for (l = a.L - 2; l >= 1; l--) {
+            float tmp = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * a.C + c];
+            add_nan_check(tmp, acc * a.scale);
+            acc = tmp;
+        }
+    }
+    float g = a.grad[0][i];
+    if (a.L > 1) add_nan_check(g, acc * a.scale);
+    float p = a.tex0[i], m = a.m[i], v = a.v[i];
+    adam_elem(p, g, m, v, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
+    p = p < a.lo ? a.lo : p > a.hi ? a.hi : p;
+    a.tex0[i] = p; a.m[i] = m; a.v[i] = v;
+    a.grad[0][i] = 0.f;
+}
+
+// ---------------------------------------------------------------- NormalAxis / ViewAxis
+__global__ void __launch_bounds__(BS) k_normal_axis(size_t P, const float* rast, const float* rot, const float* normal, const u32* nidx, float* out) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float4 r = ((const float4*)rast)[p];
+    int id = (int)r.w - 1;
+    float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+    if (id >= 0) {
+        const float* n2 = normal + 3 * (size_t)nidx[3 * id + 2];
+        const float* n0 = normal + 3 * (size_t)nidx[3 * id];
+        const float* n1 = normal + 3 * (size_t)nidx[3 * id + 1];
+        float N0 = (n0[0] - n2[0]) * r.x + (n1[0] - n2[0]) * r.y + n2[0];
+        float N1 = (n0[1] - n2[1]) * r.x + (n1[1] - n2[1]) * r.y + n2[1];
+        float N2 = (n0[2] - n2[2]) * r.x + (n1[2] - n2[2]) * r.y + n2[2];
+        float d = 1.f / sqrtf(N0 * N0 + N1 * N1 + N2 * N2);
+        N0 *= d; N1 *= d; N2 *= d;
+        o0 = rot[0] * N0 + rot[4] * N1 + rot[8] * N2;
+        o1 = rot[1] * N0 + rot[5] * N1 + rot[9] * N2;
+        o2 = rot[2] * N0 + rot[6] * N1 + rot[10] * N2;
+    }
+    out[3 * p] = o0; out[3 * p + 1] = o1; out[3 * p + 2] = o2;
+}
+
+struct Mat3x8 { float m[8][9]; };
+
+__global__ void __launch_bounds__(BS) k_view_axis(int W, int H, int depth, const float* pvinv_dev, Mat3x8 pv_small, float* out) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t P = (size_t)W * H * depth;
+    if (p >= P) return;
+    int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
+    const float* m = pvinv_dev ? pvinv_dev + 9 * s : pv_small.m[s];
+    float sx = (float)px / (float)W * 2.f - 1.f, sy = (float)py / (float)H * 2.f - 1.f;
+    float v0 = m[0] * sx + m[3] * sy + m[6];
+    float v1 = m[1] * sx + m[4] * sy + m[7];
+    float v2 = m[2] * sx + m[5] * sy + m[8];
+    float d = 1.f / sqrtf(v0 * v0 + v1 * v1 + v2 * v2);
+    out[3 * p] = v0 * d; out[3 * p + 1] = v1 * d; out[3 * p + 2] = v2 * d;
+}
+
+// ---------------------------------------------------------------- SGSpecular
+__device__ __forceinline__ void spec_terms(float nx, float ny, float nz, float vx_, float vy_, float vz_, float m, float ior, float* p, float& amp,
+                                           float& sharp, float* dA, float* dS) {
+    float vx = -vx_, vy = -vy_, vz = -vz_;
+    float vdn = vx * nx + vy * ny + vz * nz;
+    p[0] = 2.f * vdn * nx - vx; p[1] = 2.f * vdn * ny - vy; p[2] = 2.f * vdn * nz - vz;
+    float m2 = m * m;
+    float vn = fmaxf(vdn, 0.f);
+    float vn2 = vn * vn;
+    float iGv = sqrtf(vn2 + m2 * (1.f - vn2));
+    float Gv = (float)(1.0 / fmax((double)(vn + iGv), 1e-6));
+    float pdn = p[0] * nx + p[1] * ny + p[2] * nz;
+    float pn = fmaxf(pdn, 0.f);
+    float pn2 = pn * pn;
+    float iGp = sqrtf(pn2 + m2 * (1.f - pn2));
+    float Gp = (float)(1.0 / fmax((double)(pn + iGp), 1e-6));
+    float g = sqrtf(ior * ior - 1.f + pn2);
+    float gp = g + pn, gm = g - pn;
+    float f0 = gm / gp;
+    float f1 = (pn * gp - 1.f) / (pn * gm + 1.f);
+    float F = f0 * f0 * (1.f + f1 * f1);
+    amp = (float)(F * Gp * Gv * pn / fmax((double)m2, 1e-6));
+    sharp = (float)(0.5f / fmax((double)(m2 * vn), 1e-6));
+    if (dA) {
+        float dGvdm = (float)(-m * (1.f - vn2) / fmax((double)iGv, 1e-6) * Gv * Gv);
+        float dGpdm = (float)(-m * (1.f - pn2) / fmax((double)iGp, 1e-6) * Gp * Gp);
+        *dA = (float)(F * pn * (-2.0 * Gp * Gv / fmax((double)(m2 * m), 1e-6) + (Gp * dGvdm + Gv * dGpdm) / fmax((double)m2, 1e-6)));
+        *dS = (float)(-1.0 / fmax((double)(m2 * m * vn), 1e-6));
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_sg_spec_fwd(size_t P, const float* rast, const float* normal, const float* view, const float* rough,
+                                                    float ior, float* oAxis, float* oSharp, float* oAmp) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float pa[3] = {0.f, 0.f, 0.f}, amp = 0.f, sh = 0.f;
+    if ((int)rast[4 * p + 3] - 1 >= 0)
+        spec_terms(normal[3 * p], normal[3 * p + 1], normal[3 * p + 2], view[3 * p], view[3 * p + 1], view[3 * p + 2], rough[p], ior, pa, amp, sh,
+                   nullptr, nullptr);
+    oAxis[3 * p] = pa[0]; oAxis[3 * p + 1] = pa[1]; oAxis[3 * p + 2] = pa[2];
+    oSharp[p] = sh; oAmp[p] = amp;
+}
+
+__global__ void __launch_bounds__(BS) k_sg_spec_bwd(size_t P, const float* rast, const float* normal, const float* view, const float* rough,
+                                                    float ior, const float* gAmp, const float* gSharp, float* grough) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    if ((int)rast[4 * p + 3] - 1 < 0) return;
+    float pa[3], amp, sh, dA, dS;
+    spec_terms(normal[3 * p], normal[3 * p + 1], normal[3 * p + 2], view[3 * p], view[3 * p + 1], view[3 * p + 2], rough[p], ior, pa, amp, sh, &dA,
+               &dS);
+    float g = grough[p];
+    add_nan_check(g, gAmp[p] * dA);
+    add_nan_check(g, gSharp[p] * dS);
+    grough[p] = g;
+}
+
+// ---------------------------------------------------------------- SphericalGaussian
+constexpr float SG_DS = 2.133f, SG_DA = 2.34f;
+constexpr int SG_MAXC = 4;
+
+__device__ __forceinline__ float sg_g(float l, float c, float amp) {
+    return (float)((expf(l - c) - expf(-l - c)) / fmax((double)l, 1e-6) * amp);
+}
+
+__global__ void __launch_bounds__(BS) k_sg_fwd(size_t P, int C, int K, const float* rast, const float* __restrict__ axis, const float* __restrict__ sharp,
+                                               const float* __restrict__ amp, const float* normal, const float* diffuse, const float* sAxis,
+                                               const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float de[SG_MAXC] = {0.f, 0.f, 0.f, 0.f}, se[SG_MAXC] = {0.f, 0.f, 0.f, 0.f};
+    bool cov = (int)rast[4 * p + 3] - 1 >= 0;
+    if (cov) {
+        float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
+        float ss = sSharp[p], sa = sAmp[p];
+        float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
+        for (int i = 0; i < K; i++) {
+            float lam = sharp[i];
+            float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
+            float wx = dnx + gx, wy = dny + gy, wz = dnz + gz;
+            float dl = sqrtf(wx * wx + wy * wy + wz * wz);
+            float dg = sg_g(dl, SG_DS + lam, SG_DA);
+            float qx = snx + gx, qy = sny + gy, qz = snz + gz;
+            float sl = sqrtf(qx * qx + qy * qy + qz * qz);
+            float sg = sg_g(sl, ss + lam, sa);
+            for (int k = 0; k < C; k++) {
+                float a = amp[i * C + k];
+                add_nan_check(de[k], dg * a);
+                add_nan_check(se[k], sg * a);
+            }
+        }
+    }
+    for (int k = 0; k < C; k++) {
+        out[p * C + k] = cov ? diffuse[p * C + k] * de[k] + se[k] : 0.f;
+        denv[p * C + k] = de[k];
+        senv[p * C + k] = se[k];
+    }
+}
+
+// Backward: each block owns SG_PIX_PER_THREAD * 256 pixels per sweep and a fixed set of
+// sweeps; lobe grads are summed over its pixels (wave reductions, then LDS) and written as
+// one row of partials; rows are summed in a fixed order afterwards.
+constexpr int SG_NPIX = 2;
+constexpr int SG_COLS = 3 + 1 + SG_MAXC;  // axis, sharpness, amplitude
+
+__global__ void __launch_bounds__(BS) k_sg_bwd(size_t P, int C, int K, const float* rast, const float* __restrict__ axis,
+                                               const float* __restrict__ sharp, const float* __restrict__ amp, const float* normal,
+                                               const float* diffuse, const float* sAxis, const float* sSharp, const float* sAmp, const float* gout,
+                                               float* gdiffuse, float* gnormal, float* gsAxis, float* gsSharp, float* gsAmp, float* part,
+                                               int want_lobes) {
+    extern __shared__ float lds[];  // [4 waves][K][SG_COLS], only when want_lobes
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (want_lobes) {
+        for (int i = threadIdx.x; i < 4 * K * SG_COLS; i += BS) lds[i] = 0.f;
+        __syncthreads();
+    }
+    size_t sweep = (size_t)BS * SG_NPIX;
+    for (size_t base = (size_t)blockIdx.x * sweep; base < P; base += (size_t)gridDim.x * sweep) {
+        // per-pixel state in registers
+        bool cov[SG_NPIX];
+        size_t pp[SG_NPIX];
+        float dn[SG_NPIX][3], sn[SG_NPIX][3], ss_[SG_NPIX], sa_[SG_NPIX], go[SG_NPIX][SG_MAXC], df[SG_NPIX][SG_MAXC];
+        float gn[SG_NPIX][3], gp[SG_NPIX][3], gss[SG_NPIX], gsa[SG_NPIX], de[SG_NPIX][SG_MAXC];
+#pragma unroll
+        for (int j = 0; j < SG_NPIX; j++) {
+            size_t p = base + (size_t)j * BS + threadIdx.x;
+            pp[j] = p;
+            cov[j] = p < P && (int)rast[4 * p + 3] - 1 >= 0;
+            for (int k = 0; k < 3; k++) { gn[j][k] = 0.f; gp[j][k] = 0.f; dn[j][k] = 0.f; sn[j][k] = 0.f; }
+            gss[j] = 0.f; gsa[j] = 0.f; ss_[j] = 0.f; sa_[j] = 0.f;
+            for (int k = 0; k < SG_MAXC; k++) { go[j][k] = 0.f; df[j][k] = 0.f; de[j][k] = 0.f; }
+            if (cov[j]) {
+                for (int k = 0; k < 3; k++) dn[j][k] = normal[3 * p + k] * SG_DS;
+                ss_[j] = sSharp[p]; sa_[j] = sAmp[p];
+                for (int k = 0; k < 3; k++) sn[j][k] = sAxis[3 * p + k] * ss_[j];
+                for (int k = 0; k < C; k++) { go[j][k] = gout[p * C + k]; df[j][k] = diffuse[p * C + k]; }
+            }
+        }
+        for (int i = 0; i < K; i++) {
+            float lam = sharp[i];
+            float mx = axis[3 * i], my = axis[3 * i + 1], mz = axis[3 * i + 2];
+            float gx = mx * lam, gy = my * lam, gz = mz * lam;
+            float aa[SG_MAXC];
+            for (int k = 0; k < C; k++) aa[k] = amp[i * C + k];
+            float acc[SG_COLS];
+            for (int k = 0; k < SG_COLS; k++) acc[k] = 0.f;
+#pragma unroll
+            for (int j = 0; j < SG_NPIX; j++) {
+                if (!cov[j]) continue;
+                float wx = dn[j][0] + gx, wy = dn[j][1] + gy, wz = dn[j][2] + gz;
+                float l = sqrtf(wx * wx + wy * wy + wz * wz);
+                float il = (float)(1.0 / fmax((double)l, 1e-6));
+                float e1 = expf(l - SG_DS - lam), e2 = expf(-l - SG_DS - lam);
+                float dgv = (e1 - e2) * il;
+                float dgdl = ((1.f - il) * e1 + (1.f + il) * e2) * il;
+                float qx = sn[j][0] + gx, qy = sn[j][1] + gy, qz = sn[j][2] + gz;
+                float sl = sqrtf(qx * qx + qy * qy + qz * qz);
+                float isl = (float)(1.0 / fmax((double)sl, 1e-6));
+                float f1 = expf(sl - ss_[j] - lam), f2 = expf(-sl - ss_[j] - lam);
+                float sgv = (f1 - f2) * isl;
+                float sgdl = ((1.f - isl) * f1 + (1.f + isl) * f2) * isl;
+                float sdiff = 0.f, ssum = 0.f;
+                float dgf = (float)((e1 - e2) / fmax((double)l, 1e-6) * SG_DA);
+                for (int k = 0; k < C; k++) {
+                    float g = go[j][k];
+                    sdiff += g * aa[k] * df[j][k];
+                    ssum += g * aa[k];
+                    acc[4 + k] += g * (dgv * SG_DA * df[j][k] + sgv * sa_[j]);
+                    add_nan_check(de[j][k], dgf * aa[k]);
+                }
+                float cd = sdiff * SG_DA * dgdl * il, cs = ssum * sa_[j] * sgdl * isl;
+                float vx = cd * wx + cs * qx, vy = cd * wy + cs * qy, vz = cd * wz + cs * qz;
+                acc[0] += vx * lam; acc[1] += vy * lam; acc[2] += vz * lam;
+                float mdw = mx * wx + my * wy + mz * wz, mdq = mx * qx + my * qy + mz * qz;
+                acc[3] += sdiff * SG_DA * (dgdl * mdw * il - dgv) + ssum * sa_[j] * (sgdl * mdq * isl - sgv);
+                gn[j][0] += cd * wx * SG_DS; gn[j][1] += cd * wy * SG_DS; gn[j][2] += cd * wz * SG_DS;
+                gp[j][0] += cs * qx * ss_[j]; gp[j][1] += cs * qy * ss_[j]; gp[j][2] += cs * qz * ss_[j];
+                float pdq = (sn[j][0] * qx + sn[j][1] * qy + sn[j][2] * qz) / (ss_[j] != 0.f ? ss_[j] : 1.f);
+                gss[j] += ssum * sa_[j] * (sgdl * pdq * isl - sgv);
+                gsa[j] += ssum * sgv;
+            }
+            if (want_lobes) {
+                for (int k = 0; k < 4 + C; k++) {
+                    float v = wave_sum(acc[k]);
+                    if (lane == 0) lds[(wid * K + i) * SG_COLS + k] += v;
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SG_NPIX; j++) {
+            if (!cov[j]) continue;
+            size_t p = pp[j];
+            if (gdiffuse) for (int k = 0; k < C; k++) add_nan_check(gdiffuse[p * C + k], go[j][k] * de[j][k]);
+            if (gnormal) for (int k = 0; k < 3; k++) gnormal[3 * p + k] += gn[j][k];
+            if (gsAxis) for (int k = 0; k < 3; k++) gsAxis[3 * p + k] += gp[j][k];
+            if (gsSharp) gsSharp[p] += gss[j];
+            if (gsAmp) gsAmp[p] += gsa[j];
+        }
+    }
+    if (!want_lobes) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < K * SG_COLS; i += BS) {
+        float s = ((lds[i] + lds[K * SG_COLS + i]) + lds[2 * K * SG_COLS + i]) + lds[3 * K * SG_COLS + i];
+        part[(size_t)blockIdx.x * K * SG_COLS + i] = s;
+    }
+}
+
+// scatter the reduced [K][SG_COLS] sums into the three lobe-grad arrays
+__global__ void k_sg_scatter(const float* red, int K, int C, float* gaxis, float* gsharp, float* gamp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    const float* r = red + (size_t)i * SG_COLS;
+    if (gaxis) for (int k = 0; k < 3; k++) add_nan_check(gaxis[3 * i + k], r[k]);
+    if (gsharp) add_nan_check(gsharp[i], r[3]);
+    if (gamp) for (int k = 0; k < C; k++) add_nan_check(gamp[i * C + k], r[4 + k]);
+}
+
+// ---------------------------------------------------------------- antialias
+struct AAEdge { float alpha, ia, pax, pay, pbx, pby, sg; int va, vb, horiz, target_o; };
+
+struct AAArgs {
+    int W, H, depth, C, V;
+    float sx, sy;
+    const float* proj;
+    const u32* idx;
+    const float* rast;
+};
+
+// Evaluate pair (current pixel (cx,cy), neighbour left/below); returns edge count written.
+__device__ int aa_pair(const AAArgs& a, const float4* clipv, const float4* rastv, int cx, int cy, int vertical, int& pp, int& op, AAEdge* E) {
+#pragma clang fp contract(off)
+    int pid = cy * a.W + cx;
+    int nid;
+    if (!vertical) { if (cx <= 0) return 0; nid = pid - 1; } else { if (cy <= 0) return 0; nid = pid - a.W; }
+    float4 t = rastv[pid], nb = rastv[nid];
+    if (nb.w == t.w) return 0;
+    pp = (int)t.w ? pid : nid;
+    op = (int)nb.w ? pid : nid;
+    if (pp == op) { if (t.z > nb.z) op = nid; else pp = nid; }
+    float fx, fy, ox, oy;
+    if (!vertical) { fx = (float)cx + (pp > op ? .5f : -.5f); fy = (float)cy + .5f; ox = pp > op ? -1.f : 1.f; oy = 0.f; }
+    else { fx = (float)cx + .5f; fy = (float)cy + (pp > op ? .5f : -.5f); ox = 0.f; oy = pp > op ? -1.f : 1.f; }
+    int id = (int)rastv[pp].w - 1;
+    u32 vi[3] = {a.idx[3 * id], a.idx[3 * id + 1], a.idx[3 * id + 2]};
+    float vx[3], vy[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float4 c = clipv[vi[k]];
+        float iw = 1.f / c.w;
+        vx[k] = (c.x * iw + 1.f) * a.sx - fx;
+        vy[k] = (c.y * iw + 1.f) * a.sy - fy;
+    }
+    int ne = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        int ia_ = k, ib_ = (k + 1) % 3;
+        float pax = vx[ia_], pay = vy[ia_], pbx = vx[ib_], pby = vy[ib_];
+        float av = pax * pby - pay * pbx;
+        float oa = (pax - ox) * (pby - oy) - (pay - oy) * (pbx - ox);
+        if (av * oa > 0.f) continue;
+        float ex = pax - pbx, ey = pay - pby;
+        float n = (ox + oy) * (ox - oy);
+        if ((ex + ey) * (ex - ey) * n > 0.f) continue;
+        float sg = ox + oy;
+        ex *= sg; ey *= sg;
+        float D = n > 0.f ? -ey : ex;
+        float ia = (float)(1.0 / ((double)D + (D > 0 ? 1e-3 : -1e-3)));
+        float alpha = av * ia - .5f;
+        AAEdge& e = E[ne++];
+        e.alpha = alpha; e.ia = ia; e.pax = pax; e.pay = pay; e.pbx = pbx; e.pby = pby; e.sg = sg;
+        e.va = (int)vi[ia_]; e.vb = (int)vi[ib_]; e.horiz = n > 0.f; e.target_o = alpha > 0;
+    }
+    return ne;
+}
+
+__global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, const float* in, float* out) {
+#pragma clang fp contract(off)
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t P = (size_t)a.W * a.H * a.depth;
+    if (g >= P) return;
+    int px = (int)(g % a.W), py = (int)((g / a.W) % a.H), s = (int)(g / ((size_t)a.W * a.H));
+    const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
+    const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
+    const float* inv = in + (size_t)s * a.W * a.H * a.C;
+    int q = py * a.W + px;
+    int C = a.C;
+    float acc[16];
+    for (int c = 0; c < C; c++) acc[c] = inv[(size_t)q * C + c];
+    const int cand[4][3] = {{0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 1}};
+    for (int k = 0; k < 4; k++) {
+        int cx = px + cand[k][0], cy = py + cand[k][1];
+        if (cx >= a.W || cy >= a.H) continue;
+        int pp, op;
+        AAEdge E[3];
+        int ne = aa_pair(a, clipv, rastv, cx, cy, cand[k][2], pp, op, E);
+        for (int e = 0; e < ne; e++) {
+            int tgt = E[e].target_o ? op : pp;
+            if (tgt != q) continue;
+            for (int c = 0; c < C; c++) acc[c] += E[e].alpha * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+        }
+    }
+    float* o = out + g * C;
+    for (int c = 0; c < C; c++) o[c] = acc[c];
+}
+
+__global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, const float* in, const float* gout, float* gin, float* gproj) {
+#pragma clang fp contract(off)
+    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t P = (size_t)a.W * a.H * a.depth;
+    if (g >= P) return;
+    int px = (int)(g % a.W), py = (int)((g / a.W) % a.H), s = (int)(g / ((size_t)a.W * a.H));
+    const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
+    const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
+    size_t off = (size_t)s * a.W * a.H * a.C;
+    const float* inv = in + off;
+    const float* gov = gout + off;
+    float* gcs = gproj ? gproj + (size_t)s * a.V * 4 : nullptr;
+    int q = py * a.W + px;
+    int C = a.C;
+    float acc[16];
+    for (int c = 0; c < C; c++) acc[c] = gov[(size_t)q * C + c];
+    const int cand[4][3] = {{0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 1}};
+    for (int k = 0; k < 4; k++) {
+        int cx = px + cand[k][0], cy = py + cand[k][1];
+        if (cx >= a.W || cy >= a.H) continue;
+        int pp, op;
+        AAEdge E[3];
+        int ne = aa_pair(a, clipv, rastv, cx, cy, cand[k][2], pp, op, E);
+        for (int e = 0; e < ne; e++) {
+            int tgt = E[e].target_o ? op : pp;
+            float al = E[e].alpha;
+            if (pp == q) for (int c = 0; c < C; c++) acc[c] += al * gov[(size_t)tgt * C + c];
+            if (op == q) for (int c = 0; c < C; c++) acc[c] -= al * gov[(size_t)tgt * C + c];
+            if (gcs && k < 2) {
+                float d = 0.f;
+                for (int c = 0; c < C; c++) d += gov[(size_t)tgt * C + c] * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+                const AAEdge& ed = E[e];
+                float ia = ed.ia, av = ed.pax * ed.pby - ed.pay * ed.pbx;
+                float dDax, dDay, dDbx, dDby;
+                if (ed.horiz) { dDax = 0.f; dDay = -ed.sg; dDbx = 0.f; dDby = ed.sg; }
+                else { dDax = ed.sg; dDay = 0.f; dDbx = -ed.sg; dDby = 0.f; }
+                float k2 = av * ia * ia;
+                float gax = d * (ia * ed.pby - k2 * dDax), gay = d * (-ia * ed.pbx - k2 * dDay);
+                float gbx = d * (-ia * ed.pay - k2 * dDbx), gby = d * (ia * ed.pax - k2 * dDby);
+                int vv[2] = {ed.va, ed.vb};
+                float gxs[2] = {gax, gbx}, gys[2] = {gay, gby};
+                for (int j = 0; j < 2; j++) {
+                    float4 c = clipv[vv[j]];
+                    float iw = 1.f / c.w;
+                    float gvx = gxs[j] * a.sx * iw, gvy = gys[j] * a.sy * iw;
+                    float* gp = gcs + 4 * (size_t)vv[j];
+                    atomicAdd(gp, gvx);
+                    atomicAdd(gp + 1, gvy);
+                    atomicAdd(gp + 3, -(gvx * c.x + gvy * c.y) * iw);
+                }
+            }
+        }
+    }
+    float* o = gin + g * C;
+    for (int c = 0; c < C; c++) o[c] = acc[c];
+}
+
+// ---------------------------------------------------------------- MSE
+__global__ void __launch_bounds__(BS) k_mse(size_t n, const float* pred, const float* target, float* grad, float* part) {
+    size_t i0 = ((size_t)blockIdx.x * BS + threadIdx.x) * 4;
+    float s = 0.f;
+    for (int k = 0; k < 4; k++) {
+        size_t i = i0 + k;
+        if (i < n) { float g = pred[i] - target[i]; grad[i] = g; s += g * g; }
+    }
+    s = wave_sum(s);
+    __shared__ float sh[BS / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
+}
+
+__global__ void k_zero1(float* x) { *x = 0.f; }
+
+}  // namespace
+
+// =============================================================================== C ABI
+extern "C" const char* vcl_status_string(int s) {
+    switch (s) {
+        case VCL_OK: return "ok";
+        case VCL_ERR_INVALID: return "invalid argument";
+        case VCL_ERR_OOM: return "out of memory / workspace too small";
+        case VCL_ERR_HIP: return "HIP runtime error";
+        case VCL_ERR_OVERFLOW: return "device capacity overflow";
+        default: return "unknown status";
+    }
+}
+
+extern "C" const char* vcl_build_info(void) { return "vcl_amd 0.1 gfx950"; }
+
+extern "C" int vcl_project_forward(const vcl_project_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->vec && p->mat && p->out && p->vecNum >= 0 && (p->dimention == 3 || p->dimention == 4));
+    int views = p->views < 1 ? 1 : p->views;
+    size_t n = (size_t)p->vecNum * views;
+    if (!n) return VCL_OK;
+    hipLaunchKernelGGL(k_project_fwd, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat, p->out);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_project_backward(const vcl_project_params* p, const vcl_project_grad_params* g, vcl_stream_t stream) {
+    VCL_REQUIRE(p && g && p->vec && p->mat && g->out && (p->dimention == 3 || p->dimention == 4));
+    int views = p->views < 1 ? 1 : p->views;
+    if (p->vecNum <= 0 || (!g->vec && !g->mat)) return VCL_OK;
+    hipLaunchKernelGGL(k_project_bwd, dim3(cdiv(p->vecNum, BS)), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat, g->out,
+                       g->vec, g->mat);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_interpolate_forward(const vcl_interpolate_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->rast && p->attr && p->idx && p->out && p->dimention > 0);
+    VCL_REQUIRE(!p->enableDA || (p->rastDB && p->outDA));
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_interp_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->dimention, p->enableDA, p->rast, p->rastDB, p->attr,
+                       p->idx, p->out, p->outDA);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_interpolate_backward(const vcl_interpolate_params* p, const vcl_interpolate_grad_params* g, vcl_stream_t stream) {
+    VCL_REQUIRE(p && g && p->rast && p->attr && p->idx && g->out);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P || (!g->attr && !g->rast)) return VCL_OK;
+    hipLaunchKernelGGL(k_interp_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->dimention, p->rast, p->attr, p->idx, g->out, g->attr,
+                       g->rast);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+static int tex_args(const vcl_texturemap_params* p, TexArgs& t) {
+    VCL_REQUIRE(p->texwidth > 0 && p->texheight > 0 && p->channel > 0 && p->miplevel >= 1 && p->miplevel <= VCL_TEX_MAX_MIP_LEVEL);
+    t.tw = p->texwidth; t.th = p->texheight; t.C = p->channel; t.L = p->miplevel;
+    for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) t.tex[l] = l < t.L ? p->texture[l] : nullptr;
+    for (int l = 0; l < t.L; l++) VCL_REQUIRE(t.tex[l]);
+    if (t.L > 1) VCL_REQUIRE(p->uvDA);
+    return VCL_OK;
+}
+
+extern "C" int vcl_texturemap_forward(const vcl_texturemap_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->rast && p->uv && p->out);
+    TexArgs t;
+    int st = tex_args(p, t);
+    if (st) return st;
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    if (t.C == 4)
+        hipLaunchKernelGGL(k_tex_fwd4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, p->rast, p->uv, p->uvDA, p->out);
+    else
+        hipLaunchKernelGGL(k_tex_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, p->rast, p->uv, p->uvDA, p->out);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_texturemap_backward(const vcl_texturemap_params* p, const vcl_texturemap_grad_params* g, vcl_stream_t stream) {
+    VCL_REQUIRE(p && g && p->rast && p->uv && g->out);
+    TexArgs t;
+    int st = tex_args(p, t);
+    if (st) return st;
+    TexGradArgs ga;
+    for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) ga.tex[l] = l < t.L ? g->texture[l] : nullptr;
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_tex_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, ga, p->rast, p->uv, p->uvDA, g->out, g->uv,
+                       t.L > 1 ? g->uvDA : nullptr, g->daMode);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_texture_mip_levels(int w, int h, int requested) {
+    unsigned v = (unsigned)(w | h);
+    int lsb = 0;
+    while (lsb < 31 && !((v >> lsb) & 1u)) lsb++;
+    int maxl = lsb + 1 < 16 ? lsb + 1 : 16;
+    return requested < 1 ? 1 : (maxl < requested ? maxl : requested);
+}
+
+extern "C" int vcl_texture_build_mip(const vcl_texture* t, vcl_stream_t stream) {
+    VCL_REQUIRE(t && t->miplevel >= 1 && t->miplevel <= VCL_TEX_MAX_MIP_LEVEL && t->channel > 0);
+    for (int l = 1; l < t->miplevel; l++) {
+        VCL_REQUIRE(t->texture[l] && t->texture[l - 1]);
+        int w = t->width >> l, h = t->height >> l;
+        size_t n = (size_t)w * h * t->channel;
+        if (!n) continue;
+        hipLaunchKernelGGL(k_downsample, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), t->texture[l - 1], t->texture[l], w, h, t->channel,
+                           (float*)nullptr);
+        VCL_LAUNCH_CHECK();
+    }
+    return VCL_OK;
+}
+
+extern "C" int vcl_texture_clamp(const vcl_texture* t, float lo, float hi, vcl_stream_t stream) {
+    VCL_REQUIRE(t && t->texture[0]);
+    size_t n = (size_t)t->width * t->height * t->channel;
+    if (n) {
+        hipLaunchKernelGGL(k_clamp, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), t->texture[0], n, lo, hi);
+        VCL_LAUNCH_CHECK();
+    }
+    return vcl_texture_build_mip(t, stream);
+}
+
+extern "C" int vcl_texture_grad_sumup(const vcl_texture* g, int quarter, vcl_stream_t stream) {
+    VCL_REQUIRE(g && g->miplevel >= 1 && g->miplevel <= VCL_TEX_MAX_MIP_LEVEL);
+    for (int l = g->miplevel - 1; l > 0; l--) {
+        VCL_REQUIRE(g->texture[l] && g->texture[l - 1]);
+        int w = g->width >> l, h = g->height >> l;
+        size_t n = (size_t)w * h * g->channel;
+        if (!n) continue;
+        hipLaunchKernelGGL(k_fold, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), g->texture[l], g->texture[l - 1], w, h, g->channel,
+                           quarter ? 0.25f : 1.f);
+        VCL_LAUNCH_CHECK();
+    }
+    return VCL_OK;
+}
+
+extern "C" int vcl_texture_grad_clear(const vcl_texture* g, vcl_stream_t stream) {
+    VCL_REQUIRE(g && g->miplevel >= 1 && g->miplevel <= VCL_TEX_MAX_MIP_LEVEL);
+    for (int l = 0; l < g->miplevel; l++) {
+        size_t n = (size_t)(g->width >> l) * (g->height >> l) * g->channel;
+        int st = vcl_fill_async(g->texture[l], n * sizeof(float), as_hip(stream));
+        if (st) return st;
+    }
+    return VCL_OK;
+}
+
+extern "C" int vcl_adam_step_multi(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream) {
+    VCL_REQUIRE(n >= 0 && (n == 0 || (p && h)));
+    for (int base = 0; base < n; base += ADAM_MAX) {
+        AdamArgs a;
+        a.n = n - base < ADAM_MAX ? n - base : ADAM_MAX;
+        long long blocks = 0;
+        for (int i = 0; i < a.n; i++) {
+            const vcl_optimize_params& q = p[base + i];
+            const vcl_adam_hyper_params& hh = h[base + i];
+            VCL_REQUIRE(q.size >= 0 && (q.size == 0 || (q.param && q.grad && hh.m && hh.v)));
+            a.param[i] = q.param; a.grad[i] = q.grad; a.m[i] = hh.m; a.v[i] = hh.v; a.size[i] = q.size;
+            a.rhom[i] = hh.rhom; a.rhov[i] = hh.rhov; a.rhomt[i] = hh.rhomt; a.rhovt[i] = hh.rhovt; a.eta[i] = hh.eta; a.eps[i] = hh.eps;
+            a.block_start[i] = blocks;
+            blocks += (q.size + ADAM_PER_BLOCK - 1) / ADAM_PER_BLOCK;
+        }
+        a.block_start[a.n] = blocks;
+        if (blocks > 0) {
+            hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(BS), 0, as_hip(stream), a);
+            VCL_LAUNCH_CHECK();
+        }
+    }
+    return VCL_OK;
+}
+
+extern "C" int vcl_adam_step(const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream) {
+    return vcl_adam_step_multi(1, p, h, stream);
+}
+
+// part bit 1: fold + Adam + clamp on level 0; part bit 2: rebuild mips (and clear coarse grads).
+extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo,
+                                             float hi, int quarter, int part, vcl_stream_t stream) {
+    VCL_REQUIRE(tex && grad && h && h->m && h->v && tex->miplevel == grad->miplevel && tex->width == grad->width && tex->height == grad->height &&
+                tex->channel == grad->channel && tex->miplevel >= 1 && tex->miplevel <= VCL_TEX_MAX_MIP_LEVEL);
+    TexUpdArgs a;
+    a.tw = tex->width; a.th = tex->height; a.C = tex->channel; a.L = tex->miplevel;
+    a.tex0 = tex->texture[0];
+    for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) a.grad[l] = l < a.L ? grad->texture[l] : nullptr;
+    for (int l = 0; l < a.L; l++) VCL_REQUIRE(a.grad[l] && tex->texture[l]);
+    a.m = h->m; a.v = h->v;
+    a.rhom = h->rhom; a.rhov = h->rhov; a.rhomt = h->rhomt; a.rhovt = h->rhovt; a.eta = h->eta; a.eps = h->eps;
+    a.lo = lo; a.hi = hi; a.scale = quarter ? 0.25f : 1.f;
+    size_t n = (size_t)a.tw * a.th * a.C;
+    hipStream_t s = as_hip(stream);
+    if (part & 1) {
+        hipLaunchKernelGGL(k_tex_update, dim3(cdiv(n, BS)), dim3(BS), 0, s, a);
+        VCL_LAUNCH_CHECK();
+    }
+    // rebuild mips from the new level 0; the same pass zeroes the coarse grads
+    for (int l = 1; (part & 2) && l < a.L; l++) {
+        int w = a.tw >> l, hh = a.th >> l;
+        size_t m = (size_t)w * hh * a.C;
+        if (!m) continue;
+        hipLaunchKernelGGL(k_downsample, dim3(cdiv(m, BS)), dim3(BS), 0, s, tex->texture[l - 1], tex->texture[l], w, hh, a.C, a.grad[l]);
+        VCL_LAUNCH_CHECK();
+    }
+    return VCL_OK;
+}
+
+extern "C" int vcl_texture_adam_update(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
+                                       int quarter, vcl_stream_t stream) {
+    return vcl_texture_adam_update_parts(tex, grad, h, lo, hi, quarter, 3, stream);
+}
+
+extern "C" int vcl_normal_axis_forward(const vcl_normal_axis_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->rast && p->rot && p->normal && p->normalidx && p->out);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_normal_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->rot, p->normal, p->normalidx, p->out);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+// inverse(mat3(P) * mat3(V)), column-major, computed in double and rounded (glm's float
+// inverse is not reproduced bit for bit).
+static void pv_inverse(const float* Pm, const float* Vm, float* out9) {
+    double A[3][3];  // math matrix A[row][col] = sum_k P[row][k] V[k][col]
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += (double)Pm[k * 4 + r] * (double)Vm[c * 4 + k];
+            A[r][c] = s;
+        }
+    double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                 A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    double id = 1.0 / det;
+    double I[3][3];
+    I[0][0] = (A[1][1] * A[2][2] - A[1][2] * A[2][1]) * id;
+    I[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+    I[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+    I[1][0] = (A[1][2] * A[2][0] - A[1][0] * A[2][2]) * id;
+    I[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+    I[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+    I[2][0] = (A[1][0] * A[2][1] - A[1][1] * A[2][0]) * id;
+    I[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+    I[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+    for (int c = 0; c < 3; c++)
+        for (int r = 0; r < 3; r++) out9[c * 3 + r] = (float)I[r][c];
+}
+
+extern "C" int vcl_view_axis_forward(const vcl_view_axis_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->out && p->view && p->projection && p->depth >= 1);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    VCL_REQUIRE(p->depth <= 8);
+    Mat3x8 m;
+    for (int s = 0; s < p->depth; s++) pv_inverse(p->projection + 16 * s, p->view + 16 * s, m.m[s]);
+    hipLaunchKernelGGL(k_view_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), p->width, p->height, p->depth, (const float*)nullptr, m,
+                       p->out);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+// Device-side pvinv variant used by the fit engine (one 3x3 per layer already on the device).
+extern "C" int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream) {
+    VCL_REQUIRE(pvinv_dev && out);
+    size_t P = (size_t)W * H * depth;
+    if (!P) return VCL_OK;
+    Mat3x8 m;
+    hipLaunchKernelGGL(k_view_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), W, H, depth, pvinv_dev, m, out);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_sg_specular_forward(const vcl_sg_specular_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->rast && p->normal && p->view && p->roughness && p->outAxis && p->outSharpness && p->outAmplitude);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_sg_spec_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->normal, p->view, p->roughness, p->ior,
+                       p->outAxis, p->outSharpness, p->outAmplitude);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_sg_specular_backward(const vcl_sg_specular_params* p, const vcl_sg_specular_grad_params* g, vcl_stream_t stream) {
+    VCL_REQUIRE(p && g && p->rast && p->normal && p->view && p->roughness);
+    if (!g->roughness) return VCL_OK;
+    VCL_REQUIRE(g->outAmplitude && g->outSharpness);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_sg_spec_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->normal, p->view, p->roughness, p->ior,
+                       g->outAmplitude, g->outSharpness, g->roughness);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+static unsigned sg_bwd_blocks(size_t P) {
+    size_t sweep = (size_t)BS * SG_NPIX;
+    size_t nb = (P + sweep - 1) / sweep;
+    if (nb > 1024) nb = 1024;
+    if (nb < 1) nb = 1;
+    return (unsigned)nb;
+}
+
+extern "C" size_t vcl_spherical_gaussian_workspace_size(int W, int H, int depth, int channel, int sgnum) {
+    (void)channel;
+    size_t P = (size_t)W * H * depth;
+    size_t cols = (size_t)sgnum * SG_COLS;
+    return sizeof(float) * ((size_t)sg_bwd_blocks(P) * cols + colsum_scratch_floats((int)cols) + cols) + 256;
+}
+
+extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->rast && p->normal && p->diffuse && p->specAxis && p->specSharpness && p->specAmplitude && p->out && p->outDiffenv &&
+                p->outSpecenv && p->channel >= 1 && p->channel <= SG_MAXC && p->sgnum >= 0);
+    VCL_REQUIRE(p->sgnum == 0 || (p->axis && p->sharpness && p->amplitude));
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_sg_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
+                       p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv, p->outSpecenv);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
+                                               vcl_stream_t stream) {
+    VCL_REQUIRE(p && g && p->rast && p->normal && p->diffuse && p->specAxis && p->specSharpness && p->specAmplitude && g->out &&
+                p->channel >= 1 && p->channel <= SG_MAXC && p->sgnum >= 0);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    int K = p->sgnum, C = p->channel;
+    int want = (g->axis || g->sharpness || g->amplitude) && K > 0;
+    size_t need = vcl_spherical_gaussian_workspace_size(p->width, p->height, p->depth, C, K);
+    if (want && (!g->workspace || g->workspace_size < need)) return VCL_ERR_OOM;
+    unsigned nb = sg_bwd_blocks(P);
+    size_t cols = (size_t)K * SG_COLS;
+    float* part = (float*)g->workspace;
+    size_t lds = sizeof(float) * 4 * (size_t)K * SG_COLS;
+    VCL_REQUIRE(lds <= 160 * 1024);
+    hipStream_t s = as_hip(stream);
+    hipLaunchKernelGGL(k_sg_bwd, dim3(nb), dim3(BS), want ? lds : 0, s, P, C, K, p->rast, p->axis, p->sharpness, p->amplitude, p->normal,
+                       p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, g->out, g->diffuse, g->normal, g->specAxis, g->specSharpness,
+                       g->specAmplitude, part, want);
+    VCL_LAUNCH_CHECK();
+    if (!want) return VCL_OK;
+    float* seg = part + (size_t)nb * cols;
+    float* red = seg + colsum_scratch_floats((int)cols);
+    int st = vcl_fill_async(red, cols * sizeof(float), s);
+    if (st) return st;
+    st = colsum(part, (int)nb, (int)cols, seg, red, 1.f, 0, s);
+    if (st) return st;
+    hipLaunchKernelGGL(k_sg_scatter, dim3(cdiv(K, 64)), dim3(64), 0, s, red, K, C, g->axis, g->sharpness, g->amplitude);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+static AAArgs aa_args(const vcl_antialias_params* p) {
+    AAArgs a;
+    a.W = p->width; a.H = p->height; a.depth = p->depth; a.C = p->channel; a.V = p->projNum;
+    a.sx = p->s[0]; a.sy = p->s[1];
+    a.proj = p->proj; a.idx = p->idx; a.rast = p->rast;
+    return a;
+}
+
+extern "C" int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->proj && p->idx && p->rast && p->in && p->out && p->channel >= 1 && p->channel <= 16);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_aa_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), aa_args(p), p->in, p->out);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_antialias_backward(const vcl_antialias_params* p, const vcl_antialias_grad_params* g, vcl_stream_t stream) {
+    VCL_REQUIRE(p && g && p->proj && p->idx && p->rast && p->in && g->out && g->in && p->channel >= 1 && p->channel <= 16);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_aa_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), aa_args(p), p->in, g->out, g->in, g->proj);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" size_t vcl_loss_workspace_size(int size) {
+    size_t nb = cdiv((size_t)(size > 0 ? size : 1), (size_t)BS * 4);
+    return sizeof(float) * (nb + colsum_scratch_floats(1)) + 256;
+}
+
+extern "C" int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->predict && p->target && p->grad && p->buffer && p->loss && p->size >= 0);
+    hipStream_t s = as_hip(stream);
+    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, p->loss);
+    VCL_LAUNCH_CHECK();
+    if (p->size == 0) return VCL_OK;
+    size_t nb = cdiv((size_t)p->size, (size_t)BS * 4);
+    hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(BS), 0, s, (size_t)p->size, p->predict, p->target, p->grad, p->buffer);
+    VCL_LAUNCH_CHECK();
+    return colsum(p->buffer, (int)nb, 1, p->buffer + nb, p->loss, 0.5f, 0, s);
+}

@@ -1,0 +1,482 @@
+// raster.hip -- tiled software rasterizer for gfx950 (replaces the OpenGL draw of
+// Rasterize::forward, rasterize.cpp:257-282) and its backward (rasterize.cu:84-141).
+//
+// Pipeline per call (all on one stream, no host sync, no allocation):
+//   1. clear      : zero tile counters and the overflow / big-list counters
+//   2. setup      : one thread per (layer, triangle).  Guard-band test, near/guard-band
+//                   clipping when needed, 8-bit sub-pixel snapping, int64 edge equations
+//                   with the top-left rule; one 64-byte record per piece.  Pieces spanning at
+//                   most BIN_MAX_TILES tiles bump those tiles' counters; larger ones go to a
+//                   "big" list that every tile of their layer scans.
+//   3. scan       : exclusive prefix sum of the tile counters (one workgroup)
+//   4. fill       : pieces written into their tiles' bins (counter decrement gives the slot)
+//   5. fine       : one 256-thread workgroup per 16x16 tile.  Records of the tile's bin are
+//                   staged through LDS 256 at a time; every lane keeps the minimum
+//                   (ordered z_ndc, triangle id) key of its pixel in registers, so the result
+//                   does not depend on bin order.  The winner's (u, v, z/w, id+1) and DB are
+//                   written with 16-byte stores.
+// Coverage and depth use the exact float/int arithmetic of the CPU oracle's spec; this file
+// is compiled without FP contraction so both produce identical triangle IDs.
+#pragma clang fp contract(off)
+#include "common.hpp"
+
+namespace {
+
+constexpr int TILE = 16;
+constexpr int BIN_MAX_TILES = 16;
+constexpr float RAST_GB = 16.0f;
+constexpr float RAST_WEPS = 1e-6f;
+constexpr u32 INVALID = 0xffffffffu;
+
+struct __align__(16) RRec {
+    int A[3];
+    int B[3];
+    int pxr;     // px0 | px1 << 16
+    int pyr;     // py0 | py1 << 16
+    long long C[3];
+    u32 id;
+    u32 view_tl; // view << 8 | top-left bits
+};
+static_assert(sizeof(RRec) == 64, "record size");
+
+struct Layout {
+    int tilesX, tilesY, ntiles, NT;
+    size_t nrec_direct, ovf_cap, nrec_cap, bin_cap;
+    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_big, total;
+};
+
+Layout make_layout(int W, int H, int depth, int T) {
+    Layout L;
+    L.tilesX = (W + TILE - 1) / TILE;
+    L.tilesY = (H + TILE - 1) / TILE;
+    L.ntiles = L.tilesX * L.tilesY;
+    L.NT = L.ntiles * depth;
+    L.nrec_direct = (size_t)depth * T;
+    L.ovf_cap = L.nrec_direct / 2 + 1024;
+    L.nrec_cap = L.nrec_direct + L.ovf_cap;
+    L.bin_cap = L.nrec_cap * BIN_MAX_TILES;
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t o = 0;
+    L.off_counts = o; o = al(o + sizeof(u32) * (size_t)L.NT);
+    L.off_offsets = o; o = al(o + sizeof(u32) * ((size_t)L.NT + 1));
+    L.off_misc = o; o = al(o + sizeof(u32) * 64);   // [0]=ovf count [1]=big count [2]=overflow flag
+    L.off_rec = o; o = al(o + sizeof(RRec) * L.nrec_cap);
+    L.off_bins = o; o = al(o + sizeof(u32) * L.bin_cap);
+    L.off_big = o; o = al(o + sizeof(u32) * L.nrec_cap);
+    L.total = o;
+    return L;
+}
+
+struct Ws {
+    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins; u32* big;
+};
+
+Ws ws_ptrs(void* base, const Layout& L) {
+    char* b = (char*)base;
+    Ws w;
+    w.counts = (u32*)(b + L.off_counts);
+    w.offsets = (u32*)(b + L.off_offsets);
+    w.misc = (u32*)(b + L.off_misc);
+    w.rec = (RRec*)(b + L.off_rec);
+    w.bins = (u32*)(b + L.off_bins);
+    w.big = (u32*)(b + L.off_big);
+    return w;
+}
+
+struct RastArgs {
+    int W, H, depth, V, T, enableDB;
+    float xs, ys;
+    const float* proj;
+    const u32* idx;
+    float* out;
+    float* outDB;
+    int tilesX, ntiles;
+    size_t nrec_direct, ovf_cap;
+    Ws ws;
+};
+
+__device__ __forceinline__ u32 ord_f32(float z) {
+    u32 b = __float_as_uint(z);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ bool vtx_fast(float4 v) {
+    return v.w > 0.0f && fabsf(v.x) <= RAST_GB * v.w && fabsf(v.y) <= RAST_GB * v.w;
+}
+
+// Snap + edge setup of one piece; false if it has zero area or no pixel centre in its box.
+__device__ bool setup_piece(float4 a, float4 b, float4 c, u32 id, u32 view, int W, int H, RRec& r) {
+    float4 v[3] = {a, b, c};
+    int X[3], Y[3];
+    float halfW = 0.5f * (float)W, halfH = 0.5f * (float)H;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float nx = v[i].x / v[i].w, ny = v[i].y / v[i].w;
+        float fx = (nx + 1.0f) * halfW, fy = (ny + 1.0f) * halfH;
+        X[i] = (int)rintf(fx * 256.0f);
+        Y[i] = (int)rintf(fy * 256.0f);
+    }
+    i64 area = (i64)(X[1] - X[0]) * (Y[2] - Y[0]) - (i64)(Y[1] - Y[0]) * (X[2] - X[0]);
+    if (area == 0) return false;
+    i64 s = area > 0 ? 1 : -1;
+    u32 tl = 0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        int j = (i + 1) % 3;
+        i64 dx = X[j] - X[i], dy = Y[j] - Y[i];
+        i64 A = -s * dy, B = s * dx;
+        r.A[i] = (int)A;
+        r.B[i] = (int)B;
+        r.C[i] = s * (dy * X[i] - dx * Y[i]);
+        if (A > 0 || (A == 0 && B < 0)) tl |= 1u << i;
+    }
+    int xmin = min(X[0], min(X[1], X[2])), xmax = max(X[0], max(X[1], X[2]));
+    int ymin = min(Y[0], min(Y[1], Y[2])), ymax = max(Y[0], max(Y[1], Y[2]));
+    int px0 = max(0, (xmin - 128 + 255) >> 8), px1 = min(W - 1, (xmax - 128) >> 8);
+    int py0 = max(0, (ymin - 128 + 255) >> 8), py1 = min(H - 1, (ymax - 128) >> 8);
+    if (px0 > px1 || py0 > py1) return false;
+    r.pxr = px0 | (px1 << 16);
+    r.pyr = py0 | (py1 << 16);
+    r.id = id;
+    r.view_tl = (view << 8) | tl;
+    return true;
+}
+
+__device__ int clip_poly(const float4* in, int n, float4* out, float p0, float p1, float p2, float p3, float p4) {
+    int m = 0;
+    for (int i = 0; i < n; i++) {
+        float4 a = in[i], b = in[(i + 1) % n];
+        float da = p0 * a.x + p1 * a.y + p2 * a.z + p3 * a.w + p4;
+        float db = p0 * b.x + p1 * b.y + p2 * b.z + p3 * b.w + p4;
+        if (da >= 0.0f) out[m++] = a;
+        if ((da >= 0.0f) != (db >= 0.0f)) {
+            float t = da / (da - db);
+            out[m++] = make_float4(a.x + t * (b.x - a.x), a.y + t * (b.y - a.y), a.z + t * (b.z - a.z), a.w + t * (b.w - a.w));
+        }
+    }
+    return m;
+}
+
+__device__ void bin_count(const RastArgs& a, const RRec& r, u32 slot) {
+    int view = (int)(r.view_tl >> 8);
+    int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
+    int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
+    int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    if (n <= BIN_MAX_TILES) {
+        u32* cnt = a.ws.counts + (size_t)view * a.ntiles;
+        for (int ty = ty0; ty <= ty1; ty++)
+            for (int tx = tx0; tx <= tx1; tx++) atomicAdd(cnt + ty * a.tilesX + tx, 1u);
+    } else {
+        u32 b = atomicAdd(a.ws.misc + 1, 1u);
+        a.ws.big[b] = slot;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_clear(u32* counts, int NT, u32* misc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < NT) counts[i] = 0;
+    if (i < 64) misc[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_setup(RastArgs a) {
+    size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= a.nrec_direct) return;
+    u32 view = (u32)(gid / a.T);
+    u32 t = (u32)(gid % a.T);
+    const float4* clip = (const float4*)a.proj + (size_t)view * a.V;
+    u32 i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
+    float4 v0 = clip[i0], v1 = clip[i1], v2 = clip[i2];
+    RRec r;
+    if (vtx_fast(v0) && vtx_fast(v1) && vtx_fast(v2)) {
+        if (setup_piece(v0, v1, v2, t, view, a.W, a.H, r)) {
+            a.ws.rec[gid] = r;
+            bin_count(a, r, (u32)gid);
+        } else {
+            a.ws.rec[gid].id = INVALID;
+        }
+        return;
+    }
+    a.ws.rec[gid].id = INVALID;
+    // clip path: near (z + w >= 0), w >= eps, guard band |x|,|y| <= GB*w
+    float4 A[12], B[12];
+    A[0] = v0; A[1] = v1; A[2] = v2;
+    int n = 3;
+    const float planes[6][5] = {{0, 0, 1, 1, 0}, {0, 0, 0, 1, -RAST_WEPS}, {-1, 0, 0, RAST_GB, 0},
+                                {1, 0, 0, RAST_GB, 0}, {0, -1, 0, RAST_GB, 0}, {0, 1, 0, RAST_GB, 0}};
+    for (int p = 0; p < 6 && n > 0; p++) {
+        n = clip_poly(A, n, B, planes[p][0], planes[p][1], planes[p][2], planes[p][3], planes[p][4]);
+        for (int i = 0; i < n; i++) A[i] = B[i];
+    }
+    for (int i = 1; i + 1 < n; i++) {
+        if (!setup_piece(A[0], A[i], A[i + 1], t, view, a.W, a.H, r)) continue;
+        u32 s = atomicAdd(a.ws.misc + 0, 1u);
+        if (s >= a.ovf_cap) { atomicOr(a.ws.misc + 2, 1u); continue; }
+        u32 slot = (u32)(a.nrec_direct + s);
+        a.ws.rec[slot] = r;
+        bin_count(a, r, slot);
+    }
+}
+
+// Exclusive scan of NT counters (single workgroup of 1024 lanes, chunked).
+__global__ void __launch_bounds__(1024) k_scan(const u32* counts, u32* offsets, int NT) {
+    __shared__ u32 wsum[16];
+    __shared__ u32 carry;
+    int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (tid == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < NT; base += 1024) {
+        int i = base + tid;
+        u32 v = i < NT ? counts[i] : 0u;
+        // inclusive wave scan
+        u32 x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            u32 y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) wsum[wid] = x;
+        __syncthreads();
+        if (wid == 0) {
+            u32 s = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                u32 y = __shfl_up(s, o, 64);
+                if (lane >= o) s += y;
+            }
+            if (lane < 16) wsum[lane] = s;
+        }
+        __syncthreads();
+        u32 pre = carry + (wid > 0 ? wsum[wid - 1] : 0u) + x - v;
+        if (i < NT) offsets[i] = pre;
+        __syncthreads();
+        if (tid == 1023) carry = pre + v;
+        __syncthreads();
+    }
+    if (tid == 0) offsets[NT] = carry;
+}
+
+__global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
+    size_t slot = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= nslots) return;
+    if (slot >= a.nrec_direct) {
+        u32 novf = min(a.ws.misc[0], (u32)a.ovf_cap);
+        if (slot - a.nrec_direct >= novf) return;
+    }
+    RRec r = a.ws.rec[slot];
+    if (r.id == INVALID) return;
+    int view = (int)(r.view_tl >> 8);
+    int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
+    int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
+    int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+    if (n > BIN_MAX_TILES) return;
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1; tx++) {
+            size_t tile = (size_t)view * a.ntiles + ty * a.tilesX + tx;
+            u32 pos = atomicSub(a.ws.counts + tile, 1u) - 1u;
+            a.ws.bins[a.ws.offsets[tile] + pos] = (u32)slot;
+        }
+}
+
+struct BaryF { float u, v, z, wc; };
+
+__device__ __forceinline__ void bary_f32(float4 c0, float4 c1, float4 c2, int px, int py, float xs, float ys, BaryF& r) {
+    float pixx = (float)((px + 0.5) * (double)xs - 1.0);
+    float pixy = (float)((py + 0.5) * (double)ys - 1.0);
+    float P0x = c0.w * pixx - c0.x, P0y = c0.w * pixy - c0.y;
+    float P1x = c1.w * pixx - c1.x, P1y = c1.w * pixy - c1.y;
+    float P2x = c2.w * pixx - c2.x, P2y = c2.w * pixy - c2.y;
+    float E0x = P0x - P2x, E0y = P0y - P2y, E1x = P1x - P2x, E1y = P1y - P2y;
+    float D = E0x * E1y - E0y * E1x;
+    float u = (P1x * P2y - P1y * P2x) / D;
+    float v = (P2x * P0y - P2y * P0x) / D;
+    float b2 = 1.0f - u - v;
+    float zc = c0.z * u + c1.z * v + c2.z * b2;
+    float wc = c0.w * u + c1.w * v + c2.w * b2;
+    r.u = u; r.v = v; r.wc = wc; r.z = zc / wc;
+}
+
+__device__ __forceinline__ void test_record(const RRec& r, int px, int py, long long X, long long Y, u32 view,
+                                            const float4* clip, const u32* idx, float xs, float ys, u64& best) {
+    if ((r.view_tl >> 8) != view) return;
+    int px0 = r.pxr & 0xffff, px1 = r.pxr >> 16, py0 = r.pyr & 0xffff, py1 = r.pyr >> 16;
+    if (px < px0 || px > px1 || py < py0 || py > py1) return;
+    u32 tl = r.view_tl;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        long long e = (long long)r.A[i] * X + (long long)r.B[i] * Y + r.C[i];
+        if (!(e > 0 || (e == 0 && ((tl >> i) & 1u)))) return;
+    }
+    u32 t = r.id;
+    float4 c0 = clip[idx[3 * t]], c1 = clip[idx[3 * t + 1]], c2 = clip[idx[3 * t + 2]];
+    BaryF b;
+    bary_f32(c0, c1, c2, px, py, xs, ys, b);
+    if (!(b.wc > 0.0f && b.z >= -1.0f && b.z <= 1.0f)) return;
+    u64 k = ((u64)ord_f32(b.z) << 32) | t;
+    if (k < best) best = k;
+}
+
+__global__ void __launch_bounds__(256) k_fine(RastArgs a) {
+    __shared__ RRec lds[256];
+    int tile = blockIdx.x;
+    u32 view = (u32)(tile / a.ntiles);
+    int tl = tile % a.ntiles;
+    int tx = tl % a.tilesX, ty = tl / a.tilesX;
+    int tid = threadIdx.x;
+    int px = tx * TILE + (tid & (TILE - 1));
+    int py = ty * TILE + (tid >> 4);
+    long long X = (long long)px * 256 + 128, Y = (long long)py * 256 + 128;
+    const float4* clip = (const float4*)a.proj + (size_t)view * a.V;
+    u64 best = ~0ull;
+    u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
+    for (u32 b = begin; b < end; b += 256) {
+        __syncthreads();
+        if (b + tid < end) lds[tid] = a.ws.rec[a.ws.bins[b + tid]];
+        __syncthreads();
+        int n = (int)min(256u, end - b);
+        for (int j = 0; j < n; j++) test_record(lds[j], px, py, X, Y, view, clip, a.idx, a.xs, a.ys, best);
+    }
+    u32 nbig = a.ws.misc[1];
+    for (u32 b = 0; b < nbig; b += 256) {
+        __syncthreads();
+        if (b + tid < nbig) lds[tid] = a.ws.rec[a.ws.big[b + tid]];
+        __syncthreads();
+        int n = (int)min(256u, nbig - b);
+        for (int j = 0; j < n; j++) test_record(lds[j], px, py, X, Y, view, clip, a.idx, a.xs, a.ys, best);
+    }
+    if (px >= a.W || py >= a.H) return;
+    size_t p = ((size_t)view * a.H + py) * a.W + px;
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
+    if (best != ~0ull) {
+        u32 t = (u32)(best & 0xffffffffu);
+        float4 c0 = clip[a.idx[3 * t]], c1 = clip[a.idx[3 * t + 1]], c2 = clip[a.idx[3 * t + 2]];
+        float xs = a.xs, ys = a.ys;
+        float pixx = (float)((px + 0.5) * (double)xs - 1.0);
+        float pixy = (float)((py + 0.5) * (double)ys - 1.0);
+        float w0 = c0.w, w1 = c1.w, w2 = c2.w;
+        float P0x = w0 * pixx - c0.x, P0y = w0 * pixy - c0.y;
+        float P1x = w1 * pixx - c1.x, P1y = w1 * pixy - c1.y;
+        float P2x = w2 * pixx - c2.x, P2y = w2 * pixy - c2.y;
+        float E0x = P0x - P2x, E0y = P0y - P2y, E1x = P1x - P2x, E1y = P1y - P2y;
+        float D = E0x * E1y - E0y * E1x;
+        float u = (P1x * P2y - P1y * P2x) / D;
+        float v = (P2x * P0y - P2y * P0x) / D;
+        float b2 = 1.0f - u - v;
+        float zc = c0.z * u + c1.z * v + c2.z * b2;
+        float wc = w0 * u + w1 * v + w2 * b2;
+        o = make_float4(u, v, zc / wc, (float)(t + 1));
+        if (a.outDB) {
+            float dNuX = xs * (w1 * P2y - w2 * P1y), dNuY = ys * (w2 * P1x - w1 * P2x);
+            float dNvX = xs * (w2 * P0y - w0 * P2y), dNvY = ys * (w0 * P2x - w2 * P0x);
+            float dDX = xs * ((w0 - w2) * E1y - (w1 - w2) * E0y);
+            float dDY = ys * ((w1 - w2) * E0x - (w0 - w2) * E1x);
+            d = make_float4((dNuX - u * dDX) / D, (dNuY - u * dDY) / D, (dNvX - v * dDX) / D, (dNvY - v * dDY) / D);
+        }
+    }
+    ((float4*)a.out)[p] = o;
+    if (a.outDB) ((float4*)a.outDB)[p] = d;
+}
+
+// RasterizeBackwardKernel (rasterize.cu:84-141), one thread per pixel.
+__global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int depth, int V, float xs, float ys, const float* proj,
+                                                    const u32* idx, const float* rast, const float* grast, float* gproj) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t P = (size_t)W * H * depth;
+    if (p >= P) return;
+    float4 r = ((const float4*)rast)[p];
+    int id = (int)r.w - 1;
+    if (id < 0) return;
+    float2 g = ((const float2*)grast)[p];
+    int bitcheck = __float_as_int(g.x) | __float_as_int(g.y);
+    if ((bitcheck << 1) == 0) return;
+    int px = (int)(p % W);
+    int py = (int)((p / W) % H);
+    int view = (int)(p / ((size_t)W * H));
+    float pixx = (float)((px + 0.5) * (double)xs - 1.0);
+    float pixy = (float)((py + 0.5) * (double)ys - 1.0);
+    const float4* clip = (const float4*)proj + (size_t)view * V;
+    float* gp = gproj + (size_t)view * V * 4;
+    u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
+    float4 c0 = clip[i0], c1 = clip[i1], c2 = clip[i2];
+    float p0x = c0.w * pixx - c0.x, p0y = c0.w * pixy - c0.y;
+    float p1x = c1.w * pixx - c1.x, p1y = c1.w * pixy - c1.y;
+    float p2x = c2.w * pixx - c2.x, p2y = c2.w * pixy - c2.y;
+    float e0x = p0x - p2x, e0y = p0y - p2y, e1x = p1x - p2x, e1y = p1y - p2y;
+    float aa = e0x * e1y - e0y * e1x;
+    float ia = 1.f / (aa + (aa > 0 ? 1e-6f : -1e-6f));
+    float kuv = r.x * g.x + r.y * g.y;
+    float gx0 = (g.y * p2y + kuv * e1y) * ia;
+    float gy0 = (-g.y * p2x - kuv * e1x) * ia;
+    float gw0 = -gx0 * pixx - gy0 * pixy;
+    float gx1 = (-g.x * p2y - kuv * e0y) * ia;
+    float gy1 = (g.x * p2x + kuv * e0x) * ia;
+    float gw1 = -gx1 * pixx - gy1 * pixy;
+    float gx2 = ((-g.y * p0y + g.x * p1y) - kuv * (-p0y + p1y)) * ia;
+    float gy2 = ((g.y * p0x - g.x * p1x) - kuv * (p0x - p1x)) * ia;
+    float gw2 = -gx2 * pixx - gy2 * pixy;
+    atomicAdd(gp + 4 * i0, gx0); atomicAdd(gp + 4 * i0 + 1, gy0); atomicAdd(gp + 4 * i0 + 3, gw0);
+    atomicAdd(gp + 4 * i1, gx1); atomicAdd(gp + 4 * i1 + 1, gy1); atomicAdd(gp + 4 * i1 + 3, gw1);
+    atomicAdd(gp + 4 * i2, gx2); atomicAdd(gp + 4 * i2 + 1, gy2); atomicAdd(gp + 4 * i2 + 3, gw2);
+}
+
+}  // namespace
+
+extern "C" size_t vcl_rasterize_workspace_size(int width, int height, int depth, int projNum, int idxNum) {
+    (void)projNum;
+    if (width <= 0 || height <= 0 || depth <= 0 || idxNum < 0) return 0;
+    return make_layout(width, height, depth, idxNum).total;
+}
+
+extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->out && p->workspace && (p->idxNum == 0 || (p->proj && p->idx)));
+    VCL_REQUIRE(p->width > 0 && p->height > 0 && p->depth > 0 && p->width <= 16384 && p->height <= 16384);
+    VCL_REQUIRE(p->idxNum >= 0 && p->idxNum < (1 << 24) && (p->projNum > 0 || p->idxNum == 0) && p->depth < (1 << 23));
+    VCL_REQUIRE(!p->enableDB || p->outDB);
+    Layout L = make_layout(p->width, p->height, p->depth, p->idxNum);
+    if (p->workspace_size < L.total) return VCL_ERR_OOM;
+    hipStream_t s = as_hip(stream);
+    RastArgs a;
+    a.W = p->width; a.H = p->height; a.depth = p->depth; a.V = p->projNum; a.T = p->idxNum;
+    a.enableDB = p->enableDB;
+    a.xs = 2.f / (float)p->width; a.ys = 2.f / (float)p->height;
+    a.proj = p->proj; a.idx = p->idx; a.out = p->out; a.outDB = p->enableDB ? p->outDB : nullptr;
+    a.tilesX = L.tilesX; a.ntiles = L.ntiles;
+    a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap;
+    a.ws = ws_ptrs(p->workspace, L);
+    int nclear = L.NT > 64 ? L.NT : 64;
+    hipLaunchKernelGGL(k_clear, dim3(cdiv(nclear, 256)), dim3(256), 0, s, a.ws.counts, L.NT, a.ws.misc);
+    VCL_LAUNCH_CHECK();
+    if (L.nrec_direct > 0) {
+        hipLaunchKernelGGL(k_setup, dim3(cdiv(L.nrec_direct, 256)), dim3(256), 0, s, a);
+        VCL_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.ws.counts, a.ws.offsets, L.NT);
+    VCL_LAUNCH_CHECK();
+    if (L.nrec_cap > 0) {
+        hipLaunchKernelGGL(k_fill, dim3(cdiv(L.nrec_cap, 256)), dim3(256), 0, s, a, L.nrec_cap);
+        VCL_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_fine, dim3(L.NT), dim3(256), 0, s, a);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_rasterize_status(const vcl_rasterize_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->workspace);
+    Layout L = make_layout(p->width, p->height, p->depth, p->idxNum);
+    u32 misc[3];
+    VCL_HIP_CHECK(hipMemcpyAsync(misc, (char*)p->workspace + L.off_misc, sizeof(misc), hipMemcpyDeviceToHost, as_hip(stream)));
+    VCL_HIP_CHECK(hipStreamSynchronize(as_hip(stream)));
+    return misc[2] ? VCL_ERR_OVERFLOW : VCL_OK;
+}
+
+extern "C" int vcl_rasterize_backward(const vcl_rasterize_params* p, const vcl_rasterize_grad_params* g, vcl_stream_t stream) {
+    VCL_REQUIRE(p && g && p->proj && p->idx && p->out);
+    if (!g->proj) return VCL_OK;
+    VCL_REQUIRE(g->out);
+    size_t P = (size_t)p->width * p->height * p->depth;
+    if (P == 0) return VCL_OK;
+    hipLaunchKernelGGL(k_raster_bwd, dim3(cdiv(P, 256)), dim3(256), 0, as_hip(stream), p->width, p->height, p->depth,
+                       p->projNum, 2.f / (float)p->width, 2.f / (float)p->height, p->proj, p->idx, p->out, g->out, g->proj);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}

@@ -1,0 +1,374 @@
+"""Python mirror of the reference op surface over the C ABI (torch tensors as device memory).
+
+Each function launches the HIP kernels of libvcl_amd.so on torch's current stream; torch is
+only the allocator and stream provider.  Names follow the reference layers
+(Project/Rasterize/Interpolate/Texturemap/Antialias/NormalAxis/ViewAxis/SGSpecular/
+SphericalGaussian/MSELoss/Adam, nvdiffrast/nvdiffrast/*.h) and keep their argument meaning;
+errors raise VclError (the reference prints and exits).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+F32 = torch.float32
+
+
+def _p(t):
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous(), "device tensors must be contiguous CUDA tensors"
+    return C.c_void_p(t.data_ptr())
+
+
+def _s():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _lib():
+    return L.load()
+
+
+def empty(*shape, dtype=F32):
+    return torch.empty(*shape, dtype=dtype, device="cuda")
+
+
+def zeros(*shape, dtype=F32):
+    return torch.zeros(*shape, dtype=dtype, device="cuda")
+
+
+# ---------------------------------------------------------------- Project (project.h)
+def project(pos, mats, views=1):
+    V = pos.shape[0]
+    out = empty(views, V, 4)
+    p = L.ProjectParams(V, 4, _p(pos).value, _p(mats).value, _p(out).value, views)
+    L.check(_lib().vcl_project_forward(C.byref(p), _s()), "project")
+    return out
+
+
+def project_backward(pos, mats, gclip, gpos=None, gmat=None, views=1):
+    p = L.ProjectParams(pos.shape[0], 4, _p(pos).value, _p(mats).value, None, views)
+    g = L.ProjectGradParams(_p(gpos), _p(gmat), _p(gclip))
+    L.check(_lib().vcl_project_backward(C.byref(p), C.byref(g), _s()), "project_backward")
+
+
+# ---------------------------------------------------------------- Rasterize (rasterize.h)
+class Rasterizer:
+    """Holds the rasterizer workspace for one (W, H, depth, V, T) shape."""
+
+    def __init__(self, W, H, depth, V, T):
+        self.W, self.H, self.depth, self.V, self.T = W, H, depth, V, T
+        n = _lib().vcl_rasterize_workspace_size(W, H, depth, V, T)
+        self.ws = torch.empty(max(int(n), 4), dtype=torch.uint8, device="cuda")
+
+    def params(self, clip, idx, out, db):
+        return L.RasterizeParams(self.W, self.H, self.depth, 1 if db is not None else 0, 2.0 / self.W, 2.0 / self.H,
+                                 _p(clip).value, _p(idx).value, _p(out).value, _p(db).value if db is not None else None,
+                                 self.V, self.T, _p(self.ws).value, self.ws.numel())
+
+    def forward(self, clip, idx, enable_db=True):
+        out = empty(self.depth, self.H, self.W, 4)
+        db = empty(self.depth, self.H, self.W, 4) if enable_db else None
+        p = self.params(clip, idx, out, db)
+        L.check(_lib().vcl_rasterize_forward(C.byref(p), _s()), "rasterize")
+        return out, db
+
+    def status(self, clip, idx, out, db):
+        p = self.params(clip, idx, out, db)
+        return _lib().vcl_rasterize_status(C.byref(p), _s())
+
+    def backward(self, clip, idx, rast, grast, gclip):
+        p = self.params(clip, idx, rast, None)
+        g = L.RasterizeGradParams(_p(gclip), _p(grast), None)
+        L.check(_lib().vcl_rasterize_backward(C.byref(p), C.byref(g), _s()), "rasterize_backward")
+
+
+def rasterize(clip, idx, W, H, depth=1, enable_db=True):
+    r = Rasterizer(W, H, depth, clip.shape[-2], idx.shape[0])
+    return r.forward(clip, idx, enable_db)
+
+
+# ---------------------------------------------------------------- Interpolate (interpolate.h)
+def _interp_params(rast, db, attr, idx, out, outDA):
+    depth, H, W = rast.shape[0], rast.shape[1], rast.shape[2]
+    D = attr.shape[1]
+    return L.InterpolateParams(W, H, depth, D, 1 if outDA is not None else 0, _p(rast).value, _p(db).value if db is not None else None,
+                               _p(attr).value, _p(idx).value, _p(out).value if out is not None else None,
+                               _p(outDA).value if outDA is not None else None, attr.shape[0])
+
+
+def interpolate(rast, db, attr, idx, enable_da=True):
+    depth, H, W = rast.shape[:3]
+    D = attr.shape[1]
+    out = empty(depth, H, W, D)
+    outDA = empty(depth, H, W, D, 2) if enable_da else None
+    p = _interp_params(rast, db if enable_da else None, attr, idx, out, outDA)
+    L.check(_lib().vcl_interpolate_forward(C.byref(p), _s()), "interpolate")
+    return out, outDA
+
+
+def interpolate_backward(rast, attr, idx, gout, gattr=None, grast=None):
+    p = _interp_params(rast, None, attr, idx, None, None)
+    g = L.InterpolateGradParams(_p(gout), None, _p(grast), None, _p(gattr))
+    L.check(_lib().vcl_interpolate_backward(C.byref(p), C.byref(g), _s()), "interpolate_backward")
+
+
+# ---------------------------------------------------------------- Texturemap (texturemap.h)
+def level_views(flat, tw, th, C_, L_):
+    """Split a flat all-levels tensor into per-level (h, w, C) views."""
+    out, off = [], 0
+    for l in range(L_):
+        n = (tw >> l) * (th >> l) * C_
+        out.append(flat[off:off + n].view(th >> l, tw >> l, C_))
+        off += n
+    return out
+
+
+def _tex_params(rast, uv, uvDA, levels, out):
+    depth, H, W = rast.shape[:3]
+    th, tw, C_ = levels[0].shape
+    tex = (C.c_void_p * L.MAXL)(*([_p(x).value for x in levels] + [None] * (L.MAXL - len(levels))))
+    return L.TexturemapParams(W, H, depth, tw, th, C_, len(levels), _p(rast).value, _p(uv).value,
+                              _p(uvDA).value if uvDA is not None else None, tex, _p(out).value if out is not None else None)
+
+
+def texture(rast, uv, uvDA, levels):
+    depth, H, W = rast.shape[:3]
+    out = empty(depth, H, W, levels[0].shape[2])
+    p = _tex_params(rast, uv, uvDA, levels, out)
+    L.check(_lib().vcl_texturemap_forward(C.byref(p), _s()), "texture")
+    return out
+
+
+def texture_backward(rast, uv, uvDA, levels, gout, glevels=None, guv=None, guvDA=None, da_mode=0):
+    p = _tex_params(rast, uv, uvDA, levels, None)
+    gt = (C.c_void_p * L.MAXL)(*([(_p(x).value if x is not None else None) for x in (glevels or [])] + [None] * (L.MAXL - len(glevels or []))))
+    g = L.TexturemapGradParams(_p(gout), _p(guv), _p(guvDA), gt, da_mode)
+    L.check(_lib().vcl_texturemap_backward(C.byref(p), C.byref(g), _s()), "texture_backward")
+
+
+def _texture_struct(levels):
+    th, tw, C_ = levels[0].shape
+    arr = (C.c_void_p * L.MAXL)(*([_p(x).value for x in levels] + [None] * (L.MAXL - len(levels))))
+    return L.Texture(tw, th, C_, len(levels), arr)
+
+
+def build_mip(levels):
+    t = _texture_struct(levels)
+    L.check(_lib().vcl_texture_build_mip(C.byref(t), _s()), "build_mip")
+
+
+def grad_sumup(glevels, quarter=True):
+    t = _texture_struct(glevels)
+    L.check(_lib().vcl_texture_grad_sumup(C.byref(t), 1 if quarter else 0, _s()), "grad_sumup")
+
+
+def texture_clamp(levels, lo=0.0, hi=1.0):
+    t = _texture_struct(levels)
+    L.check(_lib().vcl_texture_clamp(C.byref(t), lo, hi, _s()), "texture_clamp")
+
+
+def texture_adam_update(levels, glevels, m, v, hyper, lo=0.0, hi=1.0, quarter=True):
+    t, g = _texture_struct(levels), _texture_struct(glevels)
+    h = L.AdamHyperParams(*hyper, _p(m).value, _p(v).value)
+    L.check(_lib().vcl_texture_adam_update(C.byref(t), C.byref(g), C.byref(h), lo, hi, 1 if quarter else 0, _s()), "texture_adam_update")
+
+
+# ---------------------------------------------------------------- Adam (optimize.h)
+def adam_step(params, grads, ms, vs, hyper):
+    """hyper = (rhom, rhov, rhomt, rhovt, eta, eps): one multi-tensor launch."""
+    n = len(params)
+    op = (L.OptimizeParams * n)(*[L.OptimizeParams(_p(p).value, _p(g).value, p.numel(), p.numel(), 1, 1) for p, g in zip(params, grads)])
+    hp = (L.AdamHyperParams * n)(*[L.AdamHyperParams(*hyper, _p(m).value, _p(v).value) for m, v in zip(ms, vs)])
+    L.check(_lib().vcl_adam_step_multi(n, op, hp, _s()), "adam")
+
+
+# ---------------------------------------------------------------- Material (material.h)
+def normal_axis(rast, normals, nidx, rot):
+    depth, H, W = rast.shape[:3]
+    out = empty(depth, H, W, 3)
+    p = L.NormalAxisParams(W, H, depth, _p(rast).value, _p(rot).value, _p(normals).value, _p(nidx).value, None, None, None, None, None,
+                           _p(out).value)
+    L.check(_lib().vcl_normal_axis_forward(C.byref(p), _s()), "normal_axis")
+    return out
+
+
+def view_axis(W, H, views_np, projs_np):
+    """views_np / projs_np: host float32 arrays of depth x 16 (column-major)."""
+    depth = views_np.reshape(-1, 16).shape[0]
+    out = empty(depth, H, W, 3)
+    vh = np.ascontiguousarray(views_np, np.float32).reshape(-1)
+    ph = np.ascontiguousarray(projs_np, np.float32).reshape(-1)
+    p = L.ViewAxisParams(W, H, depth, None, None, vh.ctypes.data, ph.ctypes.data, None, _p(out).value)
+    L.check(_lib().vcl_view_axis_forward(C.byref(p), _s()), "view_axis")
+    return out
+
+
+def sg_specular(rast, normal, view, rough, ior):
+    depth, H, W = rast.shape[:3]
+    ax, sh, am = empty(depth, H, W, 3), empty(depth, H, W), empty(depth, H, W)
+    p = L.SGSpecularParams(W, H, depth, _p(rast).value, _p(normal).value, _p(view).value, _p(rough).value, ior, _p(ax).value,
+                           _p(sh).value, _p(am).value)
+    L.check(_lib().vcl_sg_specular_forward(C.byref(p), _s()), "sg_specular")
+    return ax, sh, am
+
+
+def sg_specular_backward(rast, normal, view, rough, ior, gAmp, gSharp, grough):
+    depth, H, W = rast.shape[:3]
+    p = L.SGSpecularParams(W, H, depth, _p(rast).value, _p(normal).value, _p(view).value, _p(rough).value, ior, None, None, None)
+    g = L.SGSpecularGradParams(None, None, _p(grough).value, ior, None, _p(gSharp).value, _p(gAmp).value)
+    L.check(_lib().vcl_sg_specular_backward(C.byref(p), C.byref(g), _s()), "sg_specular_backward")
+
+
+def _sg_params(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out=None, denv=None, senv=None):
+    depth, H, W = rast.shape[:3]
+    C_ = diffuse.shape[-1]
+    return L.SphericalGaussianParams(W, H, depth, C_, _p(rast).value, sharp.numel(), _p(axis).value, _p(sharp).value, _p(amp).value,
+                                     _p(normal).value, None, _p(diffuse).value, _p(sAxis).value, _p(sSharp).value, _p(sAmp).value,
+                                     _p(out), _p(denv), _p(senv))
+
+
+def spherical_gaussian(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp):
+    out, denv, senv = torch.empty_like(diffuse), torch.empty_like(diffuse), torch.empty_like(diffuse)
+    p = _sg_params(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out, denv, senv)
+    L.check(_lib().vcl_spherical_gaussian_forward(C.byref(p), _s()), "spherical_gaussian")
+    return out, denv, senv
+
+
+def spherical_gaussian_backward(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, gout, gdiffuse=None, gaxis=None,
+                                gsharp=None, gamp=None, gnormal=None, gsAxis=None, gsSharp=None, gsAmp=None):
+    depth, H, W = rast.shape[:3]
+    p = _sg_params(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp)
+    n = _lib().vcl_spherical_gaussian_workspace_size(W, H, depth, diffuse.shape[-1], sharp.numel())
+    ws = torch.empty(int(n), dtype=torch.uint8, device="cuda")
+    g = L.SphericalGaussianGradParams(_p(gnormal), _p(gdiffuse), _p(gaxis), _p(gsharp), _p(gamp), _p(gsAxis), _p(gsSharp), _p(gsAmp),
+                                      _p(gout), _p(ws), ws.numel())
+    L.check(_lib().vcl_spherical_gaussian_backward(C.byref(p), C.byref(g), _s()), "spherical_gaussian_backward")
+
+
+# ---------------------------------------------------------------- Antialias (antialias.h)
+def _aa_params(clip, idx, rast, inp, out):
+    depth, H, W = rast.shape[:3]
+    s = (C.c_float * 2)(W / 2.0, H / 2.0)
+    return L.AntialiasParams(W, H, depth, inp.shape[-1], s, _p(clip).value, _p(idx).value, _p(rast).value, _p(inp).value, _p(out),
+                             clip.shape[-2])
+
+
+def antialias(clip, idx, rast, inp):
+    out = torch.empty_like(inp)
+    p = _aa_params(clip, idx, rast, inp, out)
+    L.check(_lib().vcl_antialias_forward(C.byref(p), _s()), "antialias")
+    return out
+
+
+def antialias_backward(clip, idx, rast, inp, gout, gclip=None):
+    gin = torch.empty_like(inp)
+    p = _aa_params(clip, idx, rast, inp, None)
+    g = L.AntialiasGradParams(_p(gclip), _p(gin).value, _p(gout).value)
+    L.check(_lib().vcl_antialias_backward(C.byref(p), C.byref(g), _s()), "antialias_backward")
+    return gin
+
+
+# ---------------------------------------------------------------- MSELoss (loss.h)
+def mse_loss(pred, target):
+    n = pred.numel()
+    grad = torch.empty_like(pred)
+    ws = torch.empty(int(_lib().vcl_loss_workspace_size(n)), dtype=torch.uint8, device="cuda")
+    loss = empty(1)
+    p = L.LossParams(_p(pred).value, _p(target).value, _p(grad).value, _p(ws).value, n, 1, 1, n, _p(loss).value)
+    L.check(_lib().vcl_mse_loss_backward(C.byref(p), _s()), "mse")
+    return grad, loss
+
+
+# ---------------------------------------------------------------- Fit engine
+class Fit:
+    """One device-resident fit request (vcl_fit_* of include/vcl.h)."""
+
+    def __init__(self, scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_quarter=True, fused=False, views=None):
+        cams = scene.cams if views is None else views
+        self._keep = []
+
+        def h(a, dt=np.float32):
+            a = np.ascontiguousarray(a, dt)
+            self._keep.append(a)
+            return a.ctypes.data
+
+        mats = np.stack([c.mat for c in cams]).astype(np.float32)
+        pvinv = np.stack([c.pvinv for c in cams]).astype(np.float32)
+        d = L.FitDesc()
+        d.width, d.height, d.views = scene.W, scene.H, len(cams)
+        d.V, d.T = scene.V, scene.T
+        d.texwidth, d.texheight, d.channel, d.miplevel, d.sgnum = scene.tw, scene.th, scene.C, scene.L, scene.K
+        d.pos, d.normals, d.uv = h(scene.pos), h(scene.normals), h(scene.uv)
+        d.idx = h(scene.idx, np.uint32)
+        d.texture = h(scene.tex)
+        d.axis, d.sharpness, d.amplitude = h(scene.axis), h(scene.sharp), h(scene.amp)
+        d.mats, d.pvinv, d.rot = h(mats), h(pvinv), h(np.eye(4, dtype=np.float32).reshape(16))
+        d.target = h(target) if target is not None else None
+        d.roughness, d.ior = scene.rough, scene.ior
+        d.lr, d.beta1, d.beta2, d.eps = lr, betas[0], betas[1], eps
+        d.fold_quarter = 1 if fold_quarter else 0
+        d.fused = 1 if fused else 0
+        self.desc = d
+        self.scene = scene
+        self.views = len(cams)
+        h_ = C.c_void_p()
+        L.check(_lib().vcl_fit_create(C.byref(d), C.byref(h_)), "fit_create")
+        self.h = h_
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().vcl_fit_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_target_from_render(self):
+        L.check(_lib().vcl_fit_set_target_from_render(self.h, _s()), "fit_set_target")
+
+    def step(self):
+        L.check(_lib().vcl_fit_step(self.h, _s()), "fit_step")
+
+    def ptr(self, name):
+        p = _lib().vcl_fit_buffer(self.h, name.encode())
+        if not p:
+            raise KeyError(name)
+        return p
+
+    def read(self, name, shape, dtype=np.float32):
+        """Synchronous copy of a device buffer into numpy (tests and reporting only)."""
+        out = np.empty(shape, dtype)
+        L.check(_lib().vcl_fit_read(self.h, name.encode(), out.ctypes.data, out.nbytes, _s()), f"fit_read {name}")
+        return out
+
+    def write(self, name, arr):
+        arr = np.ascontiguousarray(arr)
+        L.check(_lib().vcl_fit_write(self.h, name.encode(), arr.ctypes.data, arr.nbytes, _s()), f"fit_write {name}")
+
+    def profile_begin(self, max_launches=4096):
+        L.check(_lib().vcl_fit_profile_begin(self.h, max_launches), "profile_begin")
+
+    def profile_end(self):
+        """{op name: (total ms, launches)} for the steps since profile_begin (synchronises)."""
+        n = C.c_int()
+        names = C.create_string_buffer(64 * 64)
+        tot = (C.c_double * 64)()
+        cnt = (C.c_int * 64)()
+        L.check(_lib().vcl_fit_profile_end(self.h, 64, C.byref(n), names, 64, tot, cnt), "profile_end")
+        out = {}
+        for i in range(n.value):
+            nm = names.raw[i * 64:(i + 1) * 64].split(b"\0")[0].decode()
+            out[nm] = (tot[i], cnt[i])
+        return out
+
+    def device_bytes(self):
+        return int(_lib().vcl_fit_device_bytes(self.h))
