@@ -1,0 +1,583 @@
+"""CPU tests that pin the oracle: closed forms, brute force, invariants and central finite
+differences in FP64.  None of these retype an oracle formula; each uses an independent route
+(numpy linear algebra, brute force, geometry with a known answer, or FD of the forward)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import (F32, grid_mesh, o_aa, o_aa_bwd, o_interp, o_interp_bwd, o_normal_axis, o_project, o_raster, o_raster_bwd, o_sg,
+                     o_sg_bwd, o_spec, o_tex, o_tex_bwd, o_view_axis)
+from vcl_amd import scene as S
+
+D64 = np.float64
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _lib(orc):
+    O.set_threads(2)
+    yield
+
+
+# ------------------------------------------------------------------ project
+def test_project_closed_form_and_adjoint():
+    rng = np.random.default_rng(0)
+    pos = rng.normal(size=(50, 3))
+    mats = rng.normal(size=(2, 16))
+    clip = o_project(pos, mats, views=2, dt=D64)
+    for s in range(2):
+        M = mats[s].reshape(4, 4).T  # column-major storage -> math matrix
+        ref = (M @ np.c_[pos, np.ones(50)].T).T
+        np.testing.assert_allclose(clip[s], ref, rtol=1e-12, atol=1e-12)
+    g = rng.normal(size=clip.shape)
+    gp = np.zeros((50, 3))
+    O.lib().orc_project_bwd_f64(50, 2, O.P(np.ascontiguousarray(mats)), O.P(np.ascontiguousarray(g)), O.P(gp))
+    x = rng.normal(size=(50, 3))
+    jx = o_project(x, mats, views=2, dt=D64) - o_project(np.zeros((50, 3)), mats, views=2, dt=D64)
+    assert abs(np.sum(jx * g) - np.sum(x * gp)) < 1e-9 * max(1, abs(np.sum(x * gp)))
+
+
+# ------------------------------------------------------------------ rasterizer
+def _ids(clip, idx, W, H):
+    r, _ = o_raster(clip, idx, W, H)
+    return r[0, ..., 3].astype(np.uint32)
+
+
+def _brute(clip, idx, W, H):
+    out = np.zeros((H, W), np.uint32)
+    O.lib().orc_raster_brute(W, H, O.P(np.ascontiguousarray(clip, F32)), idx.shape[0], O.P(idx), O.P(out))
+    return out
+
+
+def test_raster_matches_bruteforce_config1():
+    sc = S.config1_scene(1, W=96)
+    clip = o_project(sc.pos, sc.cams[0].mat)[0]
+    assert np.array_equal(_ids(clip, sc.idx, 96, 96), _brute(clip, sc.idx, 96, 96))
+
+
+def test_raster_matches_bruteforce_soup_with_clipping():
+    rng = np.random.default_rng(1)
+    n = 400
+    clip = rng.uniform(-1.5, 1.5, (3 * n, 4)).astype(F32)
+    clip[:, 3] = rng.uniform(0.3, 2, 3 * n)
+    clip[:, 2] = rng.uniform(-1.4, 1.4, 3 * n) * clip[:, 3]
+    clip[:60, 3] *= -1          # behind the camera plane -> clipped
+    clip[60:90, :2] *= 80.0     # outside the guard band -> clipped
+    clip[100:130] = clip[130:160]  # exact duplicates: ties broken by the smaller id
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    ids = _ids(clip, idx, 48, 40)
+    assert np.array_equal(ids, _brute(clip, idx, 48, 40))
+    # triangles 44..52 duplicate 34..42 exactly: the later copy never wins a tie
+    # (ids holds triangle id + 1)
+    assert np.isin(ids, np.arange(35, 44)).any()
+    assert not np.isin(ids, np.arange(45, 54)).any()
+
+
+@pytest.mark.parametrize("W,H,n", [(64, 64, 8), (50, 30, 5), (33, 97, 11)])
+def test_raster_planar_grid_is_watertight(W, H, n):
+    clip, idx = grid_mesh(n)  # covers [-1, 1]^2 exactly with shared edges on pixel centres
+    cnt = np.zeros((H, W), np.uint32)
+    O.lib().orc_raster_coverage_count(W, H, O.P(clip), idx.shape[0], O.P(idx), O.P(cnt))
+    assert (cnt == 1).all(), np.bincount(cnt.ravel())
+    # jittered interior vertices keep the partition watertight
+    clip, idx = grid_mesh(n, jitter=0.3 / n, seed=2)
+    clip[:, :2] = np.clip(clip[:, :2], -1, 1)
+    edge = (np.abs(grid_mesh(n)[0][:, :2]) == 1).any(1)
+    clip[edge] = grid_mesh(n)[0][edge]
+    O.lib().orc_raster_coverage_count(W, H, O.P(clip), idx.shape[0], O.P(idx), O.P(cnt))
+    assert (cnt == 1).all(), np.bincount(cnt.ravel())
+
+
+def test_raster_closed_mesh_even_coverage():
+    sc = S.config1_scene(3, W=128)
+    clip = o_project(sc.pos, sc.cams[0].mat)[0]
+    cnt = np.zeros((128, 128), np.uint32)
+    O.lib().orc_raster_coverage_count(128, 128, O.P(clip), sc.T, O.P(sc.idx), O.P(cnt))
+    assert (cnt % 2 == 0).all()
+    assert (cnt > 0).sum() > 4000
+
+
+def test_raster_affine_barycentrics_closed_form():
+    # w = 1: perspective-correct barycentrics reduce to screen-space ones
+    v = np.array([[-0.8, -0.7, 0.1, 1], [0.9, -0.5, 0.3, 1], [-0.2, 0.85, -0.4, 1]], D64)
+    idx = np.array([[0, 1, 2]], np.uint32)
+    W, H = 40, 32
+    rast, db = o_raster(v, idx, W, H, dt=D64)
+    A = np.array([[v[0, 0] - v[2, 0], v[1, 0] - v[2, 0]], [v[0, 1] - v[2, 1], v[1, 1] - v[2, 1]]])
+    Ai = np.linalg.inv(A)
+    cov = rast[0, ..., 3] > 0
+    assert cov.sum() > 300
+    xs, ys = float(np.float32(2 / W)), float(np.float32(2 / H))  # RasterizeKernelParams::xs,ys are float
+    for py, px in zip(*np.nonzero(cov)):
+        p = np.array([(px + 0.5) * xs - 1, (py + 0.5) * ys - 1])
+        uv = Ai @ (p - v[2, :2])
+        np.testing.assert_allclose(rast[0, py, px, :2], uv, atol=1e-12)
+        z = uv[0] * v[0, 2] + uv[1] * v[1, 2] + (1 - uv.sum()) * v[2, 2]
+        assert abs(rast[0, py, px, 2] - z) < 1e-12
+        # DB: constant screen gradient of the barycentrics, per pixel
+        np.testing.assert_allclose(db[0, py, px], [Ai[0, 0] * xs, Ai[0, 1] * ys, Ai[1, 0] * xs, Ai[1, 1] * ys], rtol=1e-10, atol=1e-12)
+
+
+def _perspective_tri():
+    v = np.array([[-0.8, -0.6, 0.3, 1.3], [1.1, -0.4, 0.9, 2.1], [-0.1, 0.9, 0.2, 0.9]], D64)
+    return v, np.array([[0, 1, 2]], np.uint32)
+
+
+def test_raster_db_matches_finite_differences():
+    v, idx = _perspective_tri()
+    W, H = 32, 32
+    r0, db = o_raster(v, idx, W, H, dt=D64)
+    h = 1e-4
+    for axis, (c_u, c_v) in ((0, (0, 2)), (1, (1, 3))):
+        sc = 2.0 / (W if axis == 0 else H)
+        vp, vm = v.copy(), v.copy()
+        # moving the sample point by +h pixels == moving the triangle by -h pixels
+        vp[:, axis] -= h * sc * v[:, 3]
+        vm[:, axis] += h * sc * v[:, 3]
+        rp, _ = o_raster(vp, idx, W, H, dt=D64)
+        rm, _ = o_raster(vm, idx, W, H, dt=D64)
+        ok = (r0[0, ..., 3] > 0) & (rp[0, ..., 3] > 0) & (rm[0, ..., 3] > 0)
+        fd_u = (rp[0, ..., 0] - rm[0, ..., 0]) / (2 * h)
+        fd_v = (rp[0, ..., 1] - rm[0, ..., 1]) / (2 * h)
+        np.testing.assert_allclose(db[0, ..., c_u][ok], fd_u[ok], rtol=1e-5, atol=1e-7)
+        np.testing.assert_allclose(db[0, ..., c_v][ok], fd_v[ok], rtol=1e-5, atol=1e-7)
+
+
+def test_raster_bwd_matches_finite_differences():
+    v, idx = _perspective_tri()
+    W, H = 24, 24
+    rast, _ = o_raster(v, idx, W, H, dt=D64)
+    rng = np.random.default_rng(3)
+    g = rng.normal(size=(1, H, W, 2))
+    cov = rast[0, ..., 3] > 0
+    grad = o_raster_bwd(v[None], idx, rast, g, dt=D64)[0]
+
+    def loss(vv):
+        r, _ = o_raster(vv, idx, W, H, dt=D64)
+        return np.sum((r[0, ..., :2] * g[0])[cov])
+    h = 1e-6
+    for i in range(3):
+        for c in (0, 1, 3):
+            vp, vm = v.copy(), v.copy()
+            vp[i, c] += h
+            vm[i, c] -= h
+            fd = (loss(vp) - loss(vm)) / (2 * h)
+            assert abs(fd - grad[i, c]) <= 1e-3 * abs(fd) + 1e-6, (i, c, fd, grad[i, c])
+
+
+def test_raster_float_path_close_to_fp64():
+    sc = S.config1_scene(1, W=128)
+    clip = o_project(sc.pos, sc.cams[0].mat)[0]
+    r32, d32 = o_raster(clip, sc.idx, 128, 128)
+    r64, d64 = o_raster(clip.astype(D64), sc.idx, 128, 128, dt=D64)
+    assert np.array_equal(r32[..., 3], r64[..., 3].astype(F32))
+    err = np.max(np.abs(r32[..., :3] - r64[..., :3]))
+    print("float raster (u, v, z) max abs error vs fp64:", err)
+    assert err < 5e-5
+
+
+# ------------------------------------------------------------------ interpolate
+def test_interpolate_linear_attributes_and_adjoint():
+    v, idx = _perspective_tri()
+    W, H = 24, 24
+    rast, db = o_raster(v, idx, W, H, dt=D64)
+    rng = np.random.default_rng(4)
+    attr = rng.normal(size=(3, 3))
+    out, da = o_interp(rast, db, attr, idx, dt=D64)
+    cov = rast[0, ..., 3] > 0
+    u, w_ = rast[0, ..., 0], rast[0, ..., 1]
+    ref = u[..., None] * attr[0] + w_[..., None] * attr[1] + (1 - u - w_)[..., None] * attr[2]
+    np.testing.assert_allclose(out[0][cov], ref[cov], atol=1e-12)
+    assert not out[0][~cov].any()
+    # adjoint of the attribute map: <interp(a), g> = <a, interp_bwd(g)>
+    g = rng.normal(size=out.shape)
+    ga, _ = o_interp_bwd(rast, attr, idx, g, dt=D64)
+    assert abs(np.sum(out * g) - np.sum(attr * ga)) < 1e-9 * np.abs(np.sum(out * g)).max()
+    # DA: d attr / d pixel by central differences of the interpolated attribute
+    h = 1e-4
+    vp, vm = v.copy(), v.copy()
+    vp[:, 0] -= h * (2 / W) * v[:, 3]
+    vm[:, 0] += h * (2 / W) * v[:, 3]
+    op, _ = o_interp(*o_raster(vp, idx, W, H, dt=D64), attr, idx, dt=D64)
+    om, _ = o_interp(*o_raster(vm, idx, W, H, dt=D64), attr, idx, dt=D64)
+    np.testing.assert_allclose(da[0, ..., 0][cov], ((op - om) / (2 * h))[0][cov], rtol=1e-5, atol=1e-7)
+
+
+# ------------------------------------------------------------------ texture
+def _level_total(tw, th, C, L):
+    return sum((tw >> l) * (th >> l) * C for l in range(L))
+
+
+def test_texture_constant_and_exact_level():
+    tw = th = 64
+    C = 4
+    L = S.mip_levels(tw, th, 16)
+    lvl0 = np.broadcast_to(np.array([0.1, 0.2, 0.3, 0.4]), (th, tw, C)).copy()
+    tex = S.build_mips_np(lvl0.astype(F32), L).astype(D64)
+    rng = np.random.default_rng(5)
+    P = 50
+    rast = np.zeros((1, 1, P, 4))
+    rast[..., 3] = 1
+    uv = rng.uniform(0, 1, (1, 1, P, 2))
+    da = rng.uniform(-0.1, 0.1, (1, 1, P, 4))
+    out = o_tex(rast, uv, da, tex, tw, th, C, L, dt=D64)
+    np.testing.assert_allclose(out[0, 0], np.broadcast_to([0.1, 0.2, 0.3, 0.4], (P, 4)), rtol=1e-6)
+    _, guv, _ = o_tex_bwd(rast, uv, da, tex, tw, th, C, L, np.ones((1, 1, P, C)), dt=D64)
+    assert np.max(np.abs(guv)) < 1e-6
+    # footprint of exactly 2^k texels per pixel -> level k, f = 0: the output is the plain
+    # bilinear sample of level k at a texel corner of that level (= the texel itself)
+    tex = S.build_mips_np(rng.uniform(0, 1, (th, tw, C)).astype(F32), L).astype(D64)
+    for k in (1, 2, 3):
+        s = tw >> k
+        i, j = 5, 7
+        uvk = np.array([[[[(i + 0.0) / s + 1e-12, (j + 0.0) / s + 1e-12]]]])
+        dak = np.array([[[[2.0 ** k / tw, 0, 0, 2.0 ** k / th]]]])
+        o = o_tex(np.array([[[[0, 0, 0, 1.0]]]]), uvk, dak, tex, tw, th, C, L, dt=D64)
+        off = sum((tw >> l) * (th >> l) * C for l in range(k))
+        texel = tex[off + (j * s + i) * C: off + (j * s + i) * C + C]
+        np.testing.assert_allclose(o[0, 0, 0], texel, rtol=1e-9)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_texture_bwd_adjoint_and_fd(mode):
+    tw, th, C = 32, 16, 4
+    L = S.mip_levels(tw, th, 16)
+    rng = np.random.default_rng(6)
+    tex = S.build_mips_np(rng.uniform(0, 1, (th, tw, C)).astype(F32), L).astype(D64)
+    P = 40
+    rast = np.zeros((1, 1, P, 4))
+    rast[..., 3] = 1
+    uv = rng.uniform(0.1, 0.9, (1, 1, P, 2))
+    da = np.zeros((1, 1, P, 4))
+    da[..., 0] = rng.uniform(1.5, 5, P) / tw
+    da[..., 3] = rng.uniform(1.5, 5, P) / th
+    da[..., 1] = rng.uniform(-0.5, 0.5, P) / tw
+    da[..., 2] = rng.uniform(-0.5, 0.5, P) / th
+    g = rng.normal(size=(1, 1, P, C))
+    gt, guv, gda = o_tex_bwd(rast, uv, da, tex, tw, th, C, L, g, mode=mode, dt=D64)
+    # texel grads: the op is linear in the texels -> adjoint identity with random texels
+    x = rng.normal(size=tex.shape)
+    assert abs(np.sum(o_tex(rast, uv, da, x, tw, th, C, L, dt=D64) * g) - np.sum(x * gt)) < 1e-9 * max(1, np.sum(np.abs(x * gt)))
+    # uv grads by central differences (away from texel boundaries)
+
+    def f(uv_):
+        return np.sum(o_tex(rast, uv_, da, tex, tw, th, C, L, dt=D64) * g, axis=-1)
+    h = 1e-7
+    for c in (0, 1):
+        up, um = uv.copy(), uv.copy()
+        up[..., c] += h
+        um[..., c] -= h
+        fd = (f(up) - f(um)) / (2 * h)
+        np.testing.assert_allclose(guv[0, 0, :, c], fd[0, 0], rtol=1e-4, atol=1e-4)
+    if mode == 1:
+        # exact d level / d uvDA chain against central differences
+        for c in range(4):
+            dp, dm = da.copy(), da.copy()
+            hh = 1e-7 * np.abs(da[..., c]).max()
+            dp[..., c] += hh
+            dm[..., c] -= hh
+            fd = (np.sum(o_tex(rast, uv, dp, tex, tw, th, C, L, dt=D64) * g, -1) -
+                  np.sum(o_tex(rast, uv, dm, tex, tw, th, C, L, dt=D64) * g, -1)) / (2 * hh)
+            np.testing.assert_allclose(gda[0, 0, :, c], fd[0, 0], rtol=1e-4, atol=1e-4 * np.abs(fd).max())
+
+
+def test_mip_build_and_fold_adjoint():
+    tw, th, C = 32, 16, 3
+    L = S.mip_levels(tw, th, 16)
+    rng = np.random.default_rng(7)
+    n = _level_total(tw, th, C, L)
+    x = np.zeros(n)
+    x[: tw * th * C] = rng.normal(size=tw * th * C)
+    b = x.copy()
+    O.lib().orc_mip_build_f64(tw, th, C, L, O.P(b))
+    # level 1 is the mean of 2x2 blocks of level 0
+    l0 = x[: tw * th * C].reshape(th, tw, C)
+    l1 = b[tw * th * C: tw * th * C + (tw // 2) * (th // 2) * C].reshape(th // 2, tw // 2, C)
+    np.testing.assert_allclose(l1, l0.reshape(th // 2, 2, tw // 2, 2, C).mean((1, 3)), rtol=1e-12)
+    # the exact fold (1/4 per level) is the adjoint of build: <build(x), g> = <x, fold(g)|_0>
+    g = rng.normal(size=n)
+    f = g.copy()
+    O.lib().orc_mip_fold_f64(tw, th, C, L, O.P(f), 1)
+    assert abs(np.sum(b * g) - np.sum(x[: tw * th * C] * f[: tw * th * C])) < 1e-9 * np.sum(np.abs(b * g))
+    # the reference's fold (x1) is 4^l times larger per level
+    f1 = g.copy()
+    O.lib().orc_mip_fold_f64(tw, th, C, L, O.P(f1), 0)
+    assert not np.allclose(f1[: tw * th * C], f[: tw * th * C])
+
+
+# ------------------------------------------------------------------ material
+def test_normal_and_view_axis_closed_form():
+    v, idx = _perspective_tri()
+    W, H = 16, 16
+    rast, _ = o_raster(v, idx, W, H, dt=D64)
+    rng = np.random.default_rng(8)
+    normals = rng.normal(size=(3, 3))
+    ang = 0.7
+    R = np.eye(4)
+    R[:3, :3] = [[math.cos(ang), -math.sin(ang), 0], [math.sin(ang), math.cos(ang), 0], [0, 0, 1]]
+    rot = R.T.reshape(16)  # column-major storage
+    out = o_normal_axis(rast, normals, idx, rot, dt=D64)
+    cov = rast[0, ..., 3] > 0
+    u, w_ = rast[0, ..., 0], rast[0, ..., 1]
+    n = u[..., None] * normals[0] + w_[..., None] * normals[1] + (1 - u - w_)[..., None] * normals[2]
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    np.testing.assert_allclose(out[0][cov], (n @ R[:3, :3].T)[cov], atol=1e-12)
+    cam = S.make_camera((0.3, 0.2, 3.0))
+    va = o_view_axis(W, H, cam.pvinv.astype(D64), dt=D64)[0]
+    # view axis = normalised inverse(P3 V3) (sx, sy, 1) with sx = px/W*2-1 (no half pixel)
+    Pm = cam.proj.astype(D64)[:3, :3].T
+    Vm = cam.view.astype(D64)[:3, :3].T
+    Minv = np.linalg.inv(Pm @ Vm)
+    for py, px in ((0, 0), (5, 11), (15, 15)):
+        d = Minv @ np.array([px / W * 2 - 1, py / H * 2 - 1, 1.0])
+        np.testing.assert_allclose(va[py, px], d / np.linalg.norm(d), rtol=2e-6)
+
+
+def _sg_inputs(rng, P=12, K=4, C=4):
+    rast = np.zeros((1, 1, P, 4))
+    rast[..., 3] = 1
+    n = rng.normal(size=(1, 1, P, 3))
+    n /= np.linalg.norm(n, axis=-1, keepdims=True)
+    axis = rng.normal(size=(K, 3))
+    axis /= np.linalg.norm(axis, axis=-1, keepdims=True)
+    sharp = rng.uniform(1, 10, K)
+    amp = rng.uniform(0, 1, (K, C))
+    diff = rng.uniform(0, 1, (1, 1, P, C))
+    sAx = rng.normal(size=(1, 1, P, 3))
+    sAx /= np.linalg.norm(sAx, axis=-1, keepdims=True)
+    sSh = rng.uniform(1, 8, (1, 1, P))
+    sAm = rng.uniform(0, 1, (1, 1, P))
+    return [rast, axis, sharp, amp, n, diff, sAx, sSh, sAm]
+
+
+def test_sg_aligned_closed_form():
+    # with the lobe axis equal to the normal and zero specular amplitude, each lobe gives
+    # A_d a_i (1 - e^{-2 l}) / l, l = 2.133 + lambda_i
+    rng = np.random.default_rng(9)
+    args = _sg_inputs(rng, P=1, K=3)
+    args[1] = np.repeat(args[4][0, 0], 3, axis=0)
+    args[8][:] = 0
+    out, de, se = o_sg(*args, dt=D64)
+    lam = args[2]
+    l = 2.133 + lam
+    ref = (2.34 * (1 - np.exp(-2 * l)) / l) @ args[3]
+    np.testing.assert_allclose(de[0, 0, 0], ref, rtol=1e-6)
+    assert not se.any()
+    np.testing.assert_allclose(out[0, 0, 0], args[5][0, 0, 0] * ref, rtol=1e-6)
+
+
+def test_sg_bwd_finite_differences():
+    rng = np.random.default_rng(10)
+    args = _sg_inputs(rng)
+    g = rng.normal(size=args[5].shape)
+    gd, ga, gs, gm, gn, gsa, gss, gsm = o_sg_bwd(*args, g, dt=D64, spec_grads=True)
+    grads = {1: ga, 2: gs, 3: gm, 4: gn, 5: gd, 6: gsa, 7: gss, 8: gsm}
+
+    def loss(a):
+        return np.sum(o_sg(*a, dt=D64)[0] * g)
+    h = 1e-6
+    for k, gr in grads.items():
+        base = args[k]
+        flat = base.reshape(-1)
+        for i in range(0, flat.size, max(1, flat.size // 9)):
+            ap = [a.copy() for a in args]
+            am = [a.copy() for a in args]
+            ap[k].reshape(-1)[i] += h
+            am[k].reshape(-1)[i] -= h
+            fd = (loss(ap) - loss(am)) / (2 * h)
+            got = gr.reshape(-1)[i]
+            assert abs(fd - got) <= 1e-3 * abs(fd) + 1e-7, (k, i, fd, got)
+
+
+def test_sg_specular_roughness_fd():
+    rng = np.random.default_rng(11)
+    P = 20
+    rast = np.zeros((1, 1, P, 4))
+    rast[..., 3] = 1
+    n = np.tile([0.0, 0.0, 1.0], (1, 1, P, 1))
+    view = rng.normal(size=(1, 1, P, 3))
+    view[..., 2] = -np.abs(view[..., 2]) - 0.5
+    view /= np.linalg.norm(view, axis=-1, keepdims=True)
+    rough = rng.uniform(0.2, 0.9, (1, 1, P))
+    ga = rng.normal(size=(1, 1, P))
+    gs = rng.normal(size=(1, 1, P)) * 0.01
+    gr = np.zeros((1, 1, P))
+    O.lib().orc_sg_spec_bwd_f64(P, 1, 1, O.P(rast), O.P(n), O.P(view), O.P(rough), 1.45, O.P(ga), O.P(gs), O.P(gr))
+
+    def f(r):
+        _, sh, am = o_spec(rast, n, view, r, 1.45, dt=D64)
+        return am * ga + sh * gs
+    h = 1e-7
+    fd = (f(rough + h) - f(rough - h)) / (2 * h)
+    # the reference's d amplitude / d m (material.cu:247) is exact: it must agree with FD
+    np.testing.assert_allclose(gr, fd, rtol=1e-4, atol=1e-6)
+
+
+# ------------------------------------------------------------------ antialias
+def _edge_scene(W=16, H=8, x_edge_ndc=0.05):
+    # a large triangle whose right edge is a vertical line at x = x_edge_ndc
+    v = np.array([[-3.0, -3.0, 0.0, 1.0], [x_edge_ndc, -3.0, 0.0, 1.0], [x_edge_ndc, 3.0, 0.0, 1.0]], D64)
+    return v, np.array([[0, 1, 2]], np.uint32)
+
+
+def test_aa_straight_edge_closed_form():
+    W, H = 16, 8
+    xe = 0.05  # edge at pixel coordinate X = (xe + 1) * W / 2 = 8.4
+    v, idx = _edge_scene(W, H, xe)
+    rast, _ = o_raster(v, idx, W, H, dt=D64)
+    inp = np.zeros((1, H, W, 1))
+    inp[rast[..., 3] > 0] = 1.0
+    out = o_aa(v[None], idx, rast, inp, dt=D64)
+    X = (xe + 1) * W / 2
+    # pixel 7 (centre 7.5) is covered, pixel 8 (centre 8.5) is not; the edge crosses the
+    # segment 7.5 -> 8.5 at distance d0 = 0.9 from the covered centre: alpha = 0.4 > 0 so the
+    # empty pixel receives 0.4 * (1 - 0) (antialias.cu:24-36)
+    d0 = X - 7.5
+    Dpx = 6.0 * H / 2  # edge length along the pair's normal direction, pixels
+    np.testing.assert_allclose(out[0, :, 8, 0], d0 * Dpx / (Dpx + 1e-3) - 0.5, rtol=1e-9)
+    np.testing.assert_allclose(out[0, :, 7, 0], 1.0)
+    np.testing.assert_allclose(out[0, :, :7, 0], 1.0)
+    np.testing.assert_allclose(out[0, :, 9:, 0], 0.0)
+
+
+def test_aa_mirror_symmetry():
+    W, H = 16, 8
+    v, idx = _edge_scene(W, H, 0.05)
+    vm = v.copy()
+    vm[:, 0] = -vm[:, 0]
+    vm = vm[[0, 2, 1]]
+    rast, _ = o_raster(v, idx, W, H, dt=D64)
+    rastm, _ = o_raster(vm, idx, W, H, dt=D64)
+    inp = (rast[..., 3:] > 0).astype(D64)
+    inpm = (rastm[..., 3:] > 0).astype(D64)
+    out = o_aa(v[None], idx, rast, inp, dt=D64)
+    outm = o_aa(vm[None], idx, rastm, inpm, dt=D64)
+    np.testing.assert_allclose(out[0, :, ::-1], outm[0], atol=1e-12)
+
+
+def test_aa_bwd_finite_differences():
+    sc = S.config1_scene(4, W=40)
+    clip = o_project(sc.pos, sc.cams[0].mat)[0].astype(D64)
+    rast, _ = o_raster(clip, sc.idx, 40, 40, dt=D64)
+    rng = np.random.default_rng(12)
+    inp = rng.uniform(0, 1, (1, 40, 40, 3))
+    g = rng.normal(size=inp.shape)
+    gin, gclip = o_aa_bwd(clip[None], sc.idx, rast, inp, g, dt=D64)
+    # input colours: AA is linear in them -> adjoint identity
+    x = rng.normal(size=inp.shape)
+    assert abs(np.sum(o_aa(clip[None], sc.idx, rast, x, dt=D64) * g) - np.sum(x * gin)) < 1e-9 * np.sum(np.abs(x * gin))
+    # clip coordinates: FD at fixed rasterization (rast held fixed, as the op sees it)
+
+    def loss(c):
+        return np.sum(o_aa(c[None], sc.idx, rast, inp, dt=D64) * g)
+    active = np.nonzero(np.abs(gclip[0]).sum(1) > 1e-3)[0]
+    assert len(active) > 20
+    h = 1e-6
+    checked = 0
+    for vi in active[:: max(1, len(active) // 15)]:
+        for c in (0, 1, 3):
+            cp, cm = clip.copy(), clip.copy()
+            cp[vi, c] += h
+            cm[vi, c] -= h
+            fd = (loss(cp) - loss(cm)) / (2 * h)
+            assert abs(fd - gclip[0, vi, c]) <= 1e-3 * abs(fd) + 1e-6, (vi, c, fd, gclip[0, vi, c])
+            checked += 1
+    assert checked >= 30
+
+
+# ------------------------------------------------------------------ loss / Adam
+def test_mse_closed_form():
+    rng = np.random.default_rng(13)
+    a, b = rng.normal(size=1001), rng.normal(size=1001)
+    g = np.zeros(1001)
+    loss = O.lib().orc_mse_f64(1001, O.P(a), O.P(b), O.P(g))
+    np.testing.assert_allclose(g, a - b)
+    assert abs(loss - 0.5 * np.sum((a - b) ** 2)) < 1e-10
+
+
+def test_adam_recurrence_fp64():
+    rng = np.random.default_rng(14)
+    n = 64
+    p = rng.normal(size=n)
+    m = np.zeros(n)
+    v = np.zeros(n)
+    pr, mr, vr = p.copy(), m.copy(), v.copy()
+    b1, b2, lr, eps = 0.9, 0.999, 1e-2, 1e-8
+    b1t, b2t = b1, b2
+    for t in range(1, 6):
+        g = rng.normal(size=n)
+        b1t *= b1
+        b2t *= b2
+        O.lib().orc_adam_f64(n, O.P(p), O.P(g), O.P(m), O.P(v), b1, b2, b1t, b2t, lr, eps)
+        # textbook Adam with the reference's bias-correction offset beta^(t+1) (A-10)
+        mr = b1 * mr + (1 - b1) * g
+        vr = b2 * vr + (1 - b2) * g * g
+        pr = pr - lr * (mr / (1 - b1 ** (t + 1))) / (np.sqrt(vr / (1 - b2 ** (t + 1))) + eps)
+    np.testing.assert_allclose(p, pr, rtol=1e-12, atol=1e-14)
+
+
+def test_nan_check_semantics():
+    # AddNaNcheck (common.h:181): a non-finite update is dropped, a non-finite value is reset
+    p = np.array([1.0, np.inf, 2.0])
+    g = np.array([np.nan, 1.0, 1.0])
+    m, v = np.zeros(3), np.zeros(3)
+    O.lib().orc_adam_f64(3, O.P(p), O.P(g), O.P(m), O.P(v), 0.9, 0.999, 0.81, 0.998001, 1e-2, 1e-8)
+    assert p[0] == 1.0 and p[1] == 0.0 and p[2] < 2.0
+
+
+# ------------------------------------------------------------------ scenes and cameras
+def test_scene_generators():
+    v, f = S.icosphere(3)
+    assert v.shape == (642, 3) and f.shape == (1280, 3)
+    v, f, uv = S.uv_sphere(320, 157)
+    assert v.shape == (49922, 3) and f.shape == (99840, 3)
+    assert S.mip_levels(2048, 2048, 12) == 12 and S.mip_levels(512, 512, 10) == 10 and S.mip_levels(96, 64, 16) == 6
+    assert O.lib().orc_mip_levels(2048, 2048, 16) == 12
+    cam = S.make_camera((0.0, 0.0, 3.0))
+    # textbook look-at + perspective for eye (0,0,3), fov 45, near 0.1, far 10
+    f_ = 1 / math.tan(math.radians(22.5))
+    n_, fa = 0.1, 10.0
+    Pm = np.array([[f_, 0, 0, 0], [0, f_, 0, 0], [0, 0, -(fa + n_) / (fa - n_), -2 * fa * n_ / (fa - n_)], [0, 0, -1, 0]])
+    Vm = np.eye(4)
+    Vm[2, 3] = -3
+    np.testing.assert_allclose(cam.mat.reshape(4, 4).T, Pm @ Vm, rtol=1e-6, atol=1e-6)
+    pts = S.fibonacci_sphere(64, 3.0)
+    np.testing.assert_allclose(np.linalg.norm(pts, axis=1), 3.0)
+    # look-at from an arbitrary eye maps the centre to the view axis -z
+    cam = S.make_camera((1.0, 2.0, 2.0))
+    c = cam.view.T @ np.array([0, 0, 0, 1.0])
+    assert abs(c[0]) < 1e-6 and abs(c[1]) < 1e-6 and c[2] < 0
+
+
+def test_oracle_fit_step_decreases_loss(orc):
+    import ctypes as C
+    sc = S.config1_scene(1, W=64)
+    tgt = S.config1_scene(2, W=64)
+    from helpers import forward_inputs
+    target = forward_inputs(tgt)["image"][0].copy()
+    s = O.OrcScene()
+    keep = []
+
+    def p(a):
+        a = np.ascontiguousarray(a)
+        keep.append(a)
+        return a.ctypes.data_as(C.c_void_p)
+    cam = sc.cams[0]
+    s.W, s.H, s.V, s.T, s.C, s.K, s.tw, s.th, s.L = sc.W, sc.H, sc.V, sc.T, sc.C, sc.K, sc.tw, sc.th, sc.L
+    s.pos, s.normals, s.uv, s.idx = p(sc.pos.copy()), p(sc.normals), p(sc.uv), p(sc.idx)
+    s.mat, s.rot, s.pvinv = p(cam.mat), p(np.eye(4, dtype=F32).reshape(16)), p(cam.pvinv)
+    s.tex, s.axis, s.sharp, s.amp = p(sc.tex.copy()), p(sc.axis.copy()), p(sc.sharp.copy()), p(sc.amp.copy())
+    s.rough = p(np.full(sc.W * sc.H, 0.5, F32))
+    s.ior, s.target, s.fold_quarter = 1.45, p(target), 1
+    for nm, n in (("pos", sc.V * 3), ("tex", sc.tw * sc.th * sc.C), ("axis", sc.K * 3), ("sharp", sc.K), ("amp", sc.K * sc.C)):
+        setattr(s, "m_" + nm, p(np.zeros(n, F32)))
+        setattr(s, "v_" + nm, p(np.zeros(n, F32)))
+    s.lr, s.b1, s.b2, s.eps, s.b1t, s.b2t = 1e-2, 0.9, 0.999, 1e-8, 0.9, 0.999
+    w = O.lib().orc_work_create()
+    losses = []
+    for _ in range(4):
+        O.lib().orc_fit_step_f32(C.byref(s), w)
+        losses.append(s.loss)
+    O.lib().orc_work_destroy(w)
+    assert losses[-1] < losses[0]
