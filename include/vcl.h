@@ -419,10 +419,11 @@ size_t vcl_fit_device_bytes(const vcl_fit* f);
 /* Copy bytes between a named device buffer and host memory; synchronises the stream. */
 int vcl_fit_read(vcl_fit* f, const char* name, void* host, size_t bytes, vcl_stream_t stream);
 int vcl_fit_write(vcl_fit* f, const char* name, const void* host, size_t bytes, vcl_stream_t stream);
-/* Per-op HIP-event timing of the following steps (up to max_launches launches per op), the
- * per-op timing the reference prints only as whole-step fps (main.cpp:39-49).  _end syncs and
- * returns, per op, its name (names[i*name_len ...]), summed milliseconds and launch count. */
-int vcl_fit_profile_begin(vcl_fit* f, int max_launches);
+/* Per-op HIP-event timing of the following steps (up to max_launches launches per op; only the
+ * op named `only` when it is not NULL), the per-op timing the reference prints only as
+ * whole-step fps (main.cpp:39-49).  _end syncs and returns, per op, its name
+ * (names[i*name_len ...]), summed milliseconds and launch count. */
+int vcl_fit_profile_begin(vcl_fit* f, int max_launches, const char* only);
 int vcl_fit_profile_end(vcl_fit* f, int max_ops, int* nops, char* names, int name_len, double* total_ms, int* count);
 
 #ifdef __cplusplus

@@ -164,7 +164,7 @@ SIGNATURES = {
     "vcl_fit_device_bytes": ([vp], csz),
     "vcl_fit_read": ([vp, C.c_char_p, vp, csz, vp], ci),
     "vcl_fit_write": ([vp, C.c_char_p, vp, csz, vp], ci),
-    "vcl_fit_profile_begin": ([vp, ci], ci),
+    "vcl_fit_profile_begin": ([vp, ci, C.c_char_p], ci),
     "vcl_fit_profile_end": ([vp, ci, P_(ci), vp, ci, P_(cd), P_(ci)], ci),
 }
 

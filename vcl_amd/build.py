@@ -60,5 +60,28 @@ def build(force: bool = False) -> str:
     return LIB
 
 
+TOOLS = {
+    # tests/cpp/preset_fit: host program on the drop-in layer header (include/vcl_layers.hpp)
+    os.path.join(ROOT, "tests", "cpp", "preset_fit"): os.path.join(ROOT, "tests", "cpp", "preset_fit.cpp"),
+}
+
+
+def build_tools(force: bool = False) -> list:
+    """Host programs linked against libvcl_amd.so (rpath to vcl_amd/)."""
+    out = []
+    for exe, src in TOOLS.items():
+        deps = [src, LIB, os.path.join(ROOT, "include", "vcl_layers.hpp"), os.path.join(ROOT, "include", "vcl.h")]
+        if force or not os.path.exists(exe) or any(os.path.getmtime(d) > os.path.getmtime(exe) for d in deps):
+            rel = os.path.relpath(HERE, os.path.dirname(exe))
+            cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
+                   "-L", HERE, "-lvcl_amd", f"-Wl,-rpath,$ORIGIN/{rel}"]
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode != 0:
+                raise RuntimeError(f"tool build failed for {src}:\n{r.stderr[-6000:]}")
+        out.append(exe)
+    return out
+
+
 if __name__ == "__main__":
     print(build(force="--force" in sys.argv))
+    print(build_tools(force="--force" in sys.argv))

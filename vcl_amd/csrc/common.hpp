@@ -42,6 +42,37 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// Open-addressing hash table in LDS: key -> NV float sums (ds_add_f32).  Used to pre-aggregate
+// gradient scatter (vertex / texel grads) within a workgroup before the global atomics.
+constexpr u32 LDS_EMPTY = 0xffffffffu;
+
+template <int SLOTS, int NV>
+struct LdsTable {
+    u32 key[SLOTS];
+    float val[NV][SLOTS];
+};
+
+template <int SLOTS, int NV>
+__device__ __forceinline__ void lds_table_clear(LdsTable<SLOTS, NV>& t) {
+    for (int i = threadIdx.x; i < SLOTS; i += blockDim.x) {
+        t.key[i] = LDS_EMPTY;
+#pragma unroll
+        for (int k = 0; k < NV; k++) t.val[k][i] = 0.f;
+    }
+}
+
+// Slot of key k (inserted if absent), or -1 when MAXPROBE probes find no room.
+template <int SLOTS, int NV, int MAXPROBE = 24>
+__device__ __forceinline__ int lds_table_slot(LdsTable<SLOTS, NV>& t, u32 k) {
+    u32 h = (k * 2654435761u) & (SLOTS - 1);
+    for (int probe = 0; probe < MAXPROBE; probe++) {
+        u32 old = atomicCAS(&t.key[h], LDS_EMPTY, k);
+        if (old == LDS_EMPTY || old == k) return (int)h;
+        h = (h + 1) & (SLOTS - 1);
+    }
+    return -1;
+}
+
 // Zero-fill kernel (used instead of hipMemsetAsync so sequences stay graph-safe).
 __global__ void vcl_fill_u32(u32* p, size_t n, u32 v);
 int vcl_fill_async(void* p, size_t bytes, hipStream_t s);

@@ -19,6 +19,7 @@ struct OpTimer {
     std::vector<size_t> used;
     bool on = false;
     size_t cap = 0;
+    std::string only;  // empty: every op
 };
 
 struct vcl_fit {
@@ -68,6 +69,7 @@ struct ProfScope {
     OpTimer* T; int slot = -1; hipStream_t s;
     ProfScope(vcl_fit* F, const char* name, hipStream_t st) : T(&F->prof), s(st) {
         if (!T->on) return;
+        if (!T->only.empty() && T->only != name) return;
         for (size_t i = 0; i < T->names.size(); i++) if (T->names[i] == name) { slot = (int)i; break; }
         if (slot < 0) {
             if (T->names.size() >= 64) return;
@@ -91,9 +93,10 @@ struct ProfScope {
     }
 };
 
-extern "C" int vcl_fit_profile_begin(vcl_fit* F, int max_launches) {
+extern "C" int vcl_fit_profile_begin(vcl_fit* F, int max_launches, const char* only) {
     VCL_REQUIRE(F && max_launches > 0);
     prof_release(F->prof);
+    F->prof.only = only ? only : "";
     F->prof.cap = (size_t)max_launches;
     F->prof.on = true;
     return VCL_OK;

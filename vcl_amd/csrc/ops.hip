@@ -113,6 +113,7 @@ __global__ void __launch_bounds__(BS) k_project_bwd(int V, int dim, int views, c
 // ---------------------------------------------------------------- interpolate
 __global__ void __launch_bounds__(BS) k_interp_fwd(size_t P, int D, int enableDA, const float* rast, const float* db, const float* attr,
                                                    const u32* idx, float* out, float* outDA) {
+#pragma clang fp contract(off)
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     float4 r = ((const float4*)rast)[p];
@@ -126,10 +127,11 @@ __global__ void __launch_bounds__(BS) k_interp_fwd(size_t P, int D, int enableDA
     }
     u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
     float4 b = enableDA ? ((const float4*)db)[p] : make_float4(0, 0, 0, 0);
-    float w2 = 1.0f - r.x - r.y;
+    // interpolate.cu:38: a0*u + a1*v in float, the (1.0 - u - v) term in double
+    double w2 = 1.0 - (double)r.x - (double)r.y;
     for (int i = 0; i < D; i++) {
         float a0 = attr[(size_t)i0 * D + i], a1 = attr[(size_t)i1 * D + i], a2 = attr[(size_t)i2 * D + i];
-        out[p * D + i] = a0 * r.x + a1 * r.y + a2 * w2;
+        out[p * D + i] = (float)((double)(a0 * r.x + a1 * r.y) + (double)a2 * w2);
         if (enableDA) {
             float dadu = a0 - a2, dadv = a1 - a2;
             ((float2*)outDA)[p * D + i] = make_float2(dadu * b.x + dadv * b.z, dadu * b.y + dadv * b.w);
@@ -355,8 +357,109 @@ __global__ void __launch_bounds__(BS) k_tex_bwd(size_t P, TexArgs t, TexGradArgs
     if (guvDA) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
 }
 
+// C == 4: one 16x16 pixel tile per workgroup; texel grads of the tile are summed in an LDS
+// hash table keyed by (level, texel) and flushed with one set of atomics per touched texel.
+constexpr int TB_SLOTS = 2048;
+
+__device__ __forceinline__ void tex_splat(LdsTable<TB_SLOTS, 4>& tab, float* G, int level, int texel, float w, float4 go) {
+    u32 key = ((u32)level << 24) | (u32)texel;
+    int sl = lds_table_slot(tab, key);
+    if (sl >= 0) {
+        atomicAdd(&tab.val[0][sl], w * go.x);
+        atomicAdd(&tab.val[1][sl], w * go.y);
+        atomicAdd(&tab.val[2][sl], w * go.z);
+        atomicAdd(&tab.val[3][sl], w * go.w);
+    } else {
+        float* q = G + (size_t)texel * 4;
+        atomicAdd(q, w * go.x); atomicAdd(q + 1, w * go.y); atomicAdd(q + 2, w * go.z); atomicAdd(q + 3, w * go.w);
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
+                                                       const float* uvDA, const float* gout, float* guv, float* guvDA, int mode) {
+    __shared__ LdsTable<TB_SLOTS, 4> tab;
+    bool want_tex = g.tex[0] != nullptr;
+    if (want_tex) lds_table_clear(tab);
+    __syncthreads();
+    int view = blockIdx.y;
+    int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
+    int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
+    if (px < W && py < H) {
+        size_t p = ((size_t)view * H + py) * W + px;
+        if (rast[4 * p + 3] >= 1.f) {
+            float2 u = ((const float2*)uv)[p];
+            float4 go = ((const float4*)gout)[p];
+            int l0 = 0, l1 = 0; float f = 0.f, lev; float4 dl = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, &dl, mode);
+            int4 i0; float fx, fy;
+            index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
+            const float4* T0 = (const float4*)t.tex[l0];
+            float4 a00 = T0[i0.x], a01 = T0[i0.y], a10 = T0[i0.z], a11 = T0[i0.w];
+            float w0 = 1.f - f;
+            if (want_tex) {
+                float* G0 = g.tex[l0];
+                tex_splat(tab, G0, l0, i0.x, w0 * (1.f - fx) * (1.f - fy), go);
+                tex_splat(tab, G0, l0, i0.y, w0 * fx * (1.f - fy), go);
+                tex_splat(tab, G0, l0, i0.z, w0 * (1.f - fx) * fy, go);
+                tex_splat(tab, G0, l0, i0.w, w0 * fx * fy, go);
+            }
+            float s0 = (float)(t.tw >> l0), s1 = (float)(t.th >> l0);
+            float gu = (lerpf_(a01.x - a00.x, a11.x - a10.x, fy) * s0) * go.x + (lerpf_(a01.y - a00.y, a11.y - a10.y, fy) * s0) * go.y +
+                       (lerpf_(a01.z - a00.z, a11.z - a10.z, fy) * s0) * go.z + (lerpf_(a01.w - a00.w, a11.w - a10.w, fy) * s0) * go.w;
+            float gv = (lerpf_(a10.x - a00.x, a11.x - a01.x, fx) * s1) * go.x + (lerpf_(a10.y - a00.y, a11.y - a01.y, fx) * s1) * go.y +
+                       (lerpf_(a10.z - a00.z, a11.z - a01.z, fx) * s1) * go.z + (lerpf_(a10.w - a00.w, a11.w - a01.w, fx) * s1) * go.w;
+            float gl = 0.f;
+            if (t.L > 1 && f > 0) {
+                int4 i1; float hx, hy;
+                index_fetch(t.tw, t.th, l1, u, i1, hx, hy);
+                const float4* T1 = (const float4*)t.tex[l1];
+                float4 b00 = T1[i1.x], b01 = T1[i1.y], b10 = T1[i1.z], b11 = T1[i1.w];
+                if (want_tex) {
+                    float* G1 = g.tex[l1];
+                    tex_splat(tab, G1, l1, i1.x, f * (1.f - hx) * (1.f - hy), go);
+                    tex_splat(tab, G1, l1, i1.y, f * hx * (1.f - hy), go);
+                    tex_splat(tab, G1, l1, i1.z, f * (1.f - hx) * hy, go);
+                    tex_splat(tab, G1, l1, i1.w, f * hx * hy, go);
+                }
+                float r0 = (float)(t.tw >> l1), r1 = (float)(t.th >> l1);
+                // per channel: lerp(level-0 derivative, level-1 derivative, f), as texturemap.cu:273-274
+                float du[4], dv[4];
+                float ax[4] = {a00.x, a00.y, a00.z, a00.w}, bx[4] = {a01.x, a01.y, a01.z, a01.w};
+                float cx[4] = {a10.x, a10.y, a10.z, a10.w}, dx[4] = {a11.x, a11.y, a11.z, a11.w};
+                float ex[4] = {b00.x, b00.y, b00.z, b00.w}, fx1[4] = {b01.x, b01.y, b01.z, b01.w};
+                float gx1[4] = {b10.x, b10.y, b10.z, b10.w}, hx1[4] = {b11.x, b11.y, b11.z, b11.w};
+                float gg[4] = {go.x, go.y, go.z, go.w};
+                gu = 0.f; gv = 0.f;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    du[c] = lerpf_(lerpf_(bx[c] - ax[c], dx[c] - cx[c], fy) * s0, lerpf_(fx1[c] - ex[c], hx1[c] - gx1[c], hy) * r0, f);
+                    dv[c] = lerpf_(lerpf_(cx[c] - ax[c], dx[c] - bx[c], fx) * s1, lerpf_(gx1[c] - ex[c], hx1[c] - fx1[c], hx) * r1, f);
+                    float lv = lerpf_(lerpf_(ax[c], bx[c], fx), lerpf_(cx[c], dx[c], fx), fy);
+                    gl += (lerpf_(lerpf_(ex[c], fx1[c], hx), lerpf_(gx1[c], hx1[c], hx), hy) - lv) * gg[c];
+                    gu += du[c] * gg[c];
+                    gv += dv[c] * gg[c];
+                }
+            }
+            if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
+            if (guvDA && t.L > 1) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
+        }
+    }
+    if (!want_tex) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < TB_SLOTS; i += BS) {
+        u32 k = tab.key[i];
+        if (k == LDS_EMPTY) continue;
+        float* q = g.tex[k >> 24] + (size_t)(k & 0xffffffu) * 4;
+        atomicAdd(q, tab.val[0][i]);
+        atomicAdd(q + 1, tab.val[1][i]);
+        atomicAdd(q + 2, tab.val[2][i]);
+        atomicAdd(q + 3, tab.val[3][i]);
+    }
+}
+
 // ---------------------------------------------------------------- mip utilities
 __global__ void __launch_bounds__(BS) k_downsample(const float* src, float* dst, int w, int h, int C, float* zero_grad) {
+#pragma clang fp contract(off)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)w * h * C;
     if (i >= n) return;
@@ -371,6 +474,7 @@ __global__ void __launch_bounds__(BS) k_downsample(const float* src, float* dst,
 }
 
 __global__ void __launch_bounds__(BS) k_fold(const float* src, float* dst, int w, int h, int C, float scale) {
+#pragma clang fp contract(off)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)w * h * C;
     if (i >= n) return;
@@ -446,6 +550,7 @@ struct TexUpdArgs {
 };
 
 __global__ void __launch_bounds__(BS) k_tex_update(TexUpdArgs a) {
+#pragma clang fp contract(off)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t n = (size_t)a.tw * a.th * a.C;
     if (i >= n) return;
@@ -471,8 +576,66 @@ __global__ void __launch_bounds__(BS) k_tex_update(TexUpdArgs a) {
     a.grad[0][i] = 0.f;
 }
 
+// C == 4: one thread per level-0 texel, float4 traffic; every ancestor grad is loaded up front
+// (compile-time-unrolled levels) so the fold costs one memory latency, not one per level.
+__device__ __forceinline__ void add_nan_check4(float4& a, float4 b) {
+    add_nan_check(a.x, b.x); add_nan_check(a.y, b.y); add_nan_check(a.z, b.z); add_nan_check(a.w, b.w);
+}
+
+__global__ void __launch_bounds__(BS) k_tex_update4(TexUpdArgs a) {
+#pragma clang fp contract(off)
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t n = (size_t)a.tw * a.th;
+    if (i >= n) return;
+    int x = (int)(i % a.tw), y = (int)(i / a.tw);
+    float4 anc[VCL_TEX_MAX_MIP_LEVEL];
+#pragma unroll
+    for (int l = 1; l < VCL_TEX_MAX_MIP_LEVEL; l++)
+        if (l < a.L) anc[l] = ((const float4*)a.grad[l])[(size_t)(y >> l) * (a.tw >> l) + (x >> l)];
+    float4 g = ((const float4*)a.grad[0])[i];
+    float4 p = ((const float4*)a.tex0)[i], m = ((const float4*)a.m)[i], v = ((const float4*)a.v)[i];
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int l = VCL_TEX_MAX_MIP_LEVEL - 1; l >= 1; l--) {
+        if (l == a.L - 1) acc = anc[l];
+        else if (l < a.L - 1) {
+            float4 t = anc[l];
+            add_nan_check4(t, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
+            acc = t;
+        }
+    }
+    if (a.L > 1) add_nan_check4(g, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
+    adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
+    adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
+    adam_elem(p.z, g.z, m.z, v.z, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
+    adam_elem(p.w, g.w, m.w, v.w, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
+    p.x = p.x < a.lo ? a.lo : p.x > a.hi ? a.hi : p.x;
+    p.y = p.y < a.lo ? a.lo : p.y > a.hi ? a.hi : p.y;
+    p.z = p.z < a.lo ? a.lo : p.z > a.hi ? a.hi : p.z;
+    p.w = p.w < a.lo ? a.lo : p.w > a.hi ? a.hi : p.w;
+    ((float4*)a.tex0)[i] = p;
+    ((float4*)a.m)[i] = m;
+    ((float4*)a.v)[i] = v;
+    ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// C == 4 mip level from the level above (float4 per texel), clearing that level's grads.
+__global__ void __launch_bounds__(BS) k_downsample4(const float4* src, float4* dst, int w, int h, float4* zero_grad) {
+#pragma clang fp contract(off)
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)w * h) return;
+    int x = (int)(i % w), y = (int)(i / w);
+    size_t sw = 2 * (size_t)w;
+    float4 p00 = src[(2 * y) * sw + 2 * x], p01 = src[(2 * y) * sw + 2 * x + 1];
+    float4 p10 = src[(2 * y + 1) * sw + 2 * x], p11 = src[(2 * y + 1) * sw + 2 * x + 1];
+    dst[i] = make_float4((((p00.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p00.y + p01.y) + p10.y) + p11.y) * 0.25f,
+                         (((p00.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p00.w + p01.w) + p10.w) + p11.w) * 0.25f);
+    if (zero_grad) zero_grad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 // ---------------------------------------------------------------- NormalAxis / ViewAxis
 __global__ void __launch_bounds__(BS) k_normal_axis(size_t P, const float* rast, const float* rot, const float* normal, const u32* nidx, float* out) {
+#pragma clang fp contract(off)
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     float4 r = ((const float4*)rast)[p];
@@ -727,8 +890,165 @@ __global__ void k_sg_scatter(const float* red, int K, int C, float* gaxis, float
     if (gamp) for (int k = 0; k < C; k++) add_nan_check(gamp[i * C + k], r[4 + k]);
 }
 
+// ---- fast SG kernels: native exp2 / sqrt / rcp (1-ulp hardware ops), lobes staged in LDS.
+constexpr float L2E = 1.4426950408889634f;
+constexpr int SG_MAXK = 256;
+
+__device__ __forceinline__ float nexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float nsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float nrcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// AddNaNcheck for an accumulator that is finite by construction.
+__device__ __forceinline__ void acc_checked(float& a, float b) { float v = a + b; a = isfinite(v) ? v : a; }
+
+struct SgLobe { float gx, gy, gz, lam, mx, my, mz, pad, a[4]; };
+
+__device__ __forceinline__ void sg_stage_lobes(SgLobe* L, int K, int C, const float* axis, const float* sharp, const float* amp) {
+    for (int i = threadIdx.x; i < K; i += blockDim.x) {
+        float lam = sharp[i];
+        SgLobe l;
+        l.mx = axis[3 * i]; l.my = axis[3 * i + 1]; l.mz = axis[3 * i + 2];
+        l.gx = l.mx * lam; l.gy = l.my * lam; l.gz = l.mz * lam; l.lam = lam; l.pad = 0.f;
+        for (int k = 0; k < 4; k++) l.a[k] = k < C ? amp[i * C + k] : 0.f;
+        L[i] = l;
+    }
+}
+
+// One lobe term: g = (e^{l-c} - e^{-l-c}) / max(l, 1e-6), plus (optionally) dg/dl.
+__device__ __forceinline__ float sg_term(float wx, float wy, float wz, float base, float lam, float& l, float& il, float& e1, float& e2) {
+    l = nsqrt(wx * wx + wy * wy + wz * wz);
+    il = nrcp(fmaxf(l, 1e-6f));
+    e1 = nexp2(((l - base) - lam) * L2E);
+    e2 = nexp2(((-l - base) - lam) * L2E);
+    return (e1 - e2) * il;
+}
+
+__global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, const float* rast, const float* axis, const float* sharp,
+                                                    const float* amp, const float* normal, const float* diffuse, const float* sAxis,
+                                                    const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) {
+    __shared__ SgLobe lobes[SG_MAXK];
+    sg_stage_lobes(lobes, K, C, axis, sharp, amp);
+    __syncthreads();
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float de[4] = {0.f, 0.f, 0.f, 0.f}, se[4] = {0.f, 0.f, 0.f, 0.f};
+    bool cov = (int)rast[4 * p + 3] - 1 >= 0;
+    if (cov) {
+        float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
+        float ss = sSharp[p], sa = sAmp[p];
+        float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
+        for (int i = 0; i < K; i++) {
+            const SgLobe& L = lobes[i];
+            float l, il, e1, e2;
+            float dg = sg_term(dnx + L.gx, dny + L.gy, dnz + L.gz, SG_DS, L.lam, l, il, e1, e2) * SG_DA;
+            float sg = sg_term(snx + L.gx, sny + L.gy, snz + L.gz, ss, L.lam, l, il, e1, e2) * sa;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (k < C) { acc_checked(de[k], dg * L.a[k]); acc_checked(se[k], sg * L.a[k]); }
+            }
+        }
+    }
+    if (C == 4) {
+        ((float4*)out)[p] = cov ? make_float4(diffuse[4 * p] * de[0] + se[0], diffuse[4 * p + 1] * de[1] + se[1], diffuse[4 * p + 2] * de[2] + se[2],
+                                              diffuse[4 * p + 3] * de[3] + se[3])
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+        ((float4*)denv)[p] = make_float4(de[0], de[1], de[2], de[3]);
+        ((float4*)senv)[p] = make_float4(se[0], se[1], se[2], se[3]);
+    } else {
+        for (int k = 0; k < C; k++) {
+            out[p * C + k] = cov ? diffuse[p * C + k] * de[k] + se[k] : 0.f;
+            denv[p * C + k] = de[k];
+            senv[p * C + k] = se[k];
+        }
+    }
+}
+
+// Lobe-parallel backward for the lobe gradients (axis, sharpness, amplitude).  Thread t owns
+// lobe t % KP for pixel sub-group t / KP; covered pixels of a 256-pixel chunk are compacted into
+// LDS (deterministic order), every wave walks them in the same order (LDS broadcast), and the
+// eight lobe sums stay in registers across all chunks of the block.  grad.diffuse is
+// gout * diffenv (the forward's lobe-ordered sum) written in the staging pass.
+struct SgPix { float dnx, dny, dnz, ss, snx, sny, snz, sa, X[4], G[4]; };
+
+__global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int KP, const float* rast, const float* axis, const float* sharp,
+                                                     const float* amp, const float* normal, const float* diffuse, const float* sAxis,
+                                                     const float* sSharp, const float* sAmp, const float* gout, const float* denv, float* gdiffuse,
+                                                     float* part, int want_lobes) {
+    __shared__ SgLobe lobes[SG_MAXK];
+    __shared__ SgPix pix[BS];
+    __shared__ int wcount[BS / 64];
+    __shared__ float red[BS][SG_COLS];
+    sg_stage_lobes(lobes, K, C, axis, sharp, amp);
+    int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    int li = tid % KP, grp = tid / KP, G = BS / KP;
+    float acc[SG_COLS];
+#pragma unroll
+    for (int k = 0; k < SG_COLS; k++) acc[k] = 0.f;
+    for (size_t base = (size_t)blockIdx.x * BS; base < P; base += (size_t)gridDim.x * BS) {
+        size_t p = base + tid;
+        bool cov = p < P && (int)rast[4 * p + 3] - 1 >= 0;
+        SgPix q;
+        if (cov) {
+            float go[4] = {0.f, 0.f, 0.f, 0.f}, df[4] = {0.f, 0.f, 0.f, 0.f};
+            for (int k = 0; k < C; k++) { go[k] = gout[p * C + k]; df[k] = diffuse[p * C + k]; }
+            if (gdiffuse)
+                for (int k = 0; k < C; k++) add_nan_check(gdiffuse[p * C + k], go[k] * denv[p * C + k]);
+            q.dnx = normal[3 * p] * SG_DS; q.dny = normal[3 * p + 1] * SG_DS; q.dnz = normal[3 * p + 2] * SG_DS;
+            q.ss = sSharp[p]; q.sa = sAmp[p];
+            q.snx = sAxis[3 * p] * q.ss; q.sny = sAxis[3 * p + 1] * q.ss; q.snz = sAxis[3 * p + 2] * q.ss;
+            for (int k = 0; k < 4; k++) { q.X[k] = go[k] * df[k]; q.G[k] = go[k]; }
+        }
+        if (!want_lobes) continue;
+        // deterministic compaction: wave ballot + prefix over waves
+        unsigned long long m = __ballot(cov);
+        int wpos = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wcount[wid] = __popcll(m);
+        __syncthreads();
+        int off = 0, ncov = 0;
+        for (int w = 0; w < BS / 64; w++) { if (w < wid) off += wcount[w]; ncov += wcount[w]; }
+        if (cov) pix[off + wpos] = q;
+        __syncthreads();
+        if (li < K) {
+            const SgLobe& L = lobes[li];
+            for (int j = grp; j < ncov; j += G) {
+                const SgPix& x = pix[j];
+                float sdiff = 0.f, ssum = 0.f;
+#pragma unroll
+                for (int k = 0; k < 4; k++) { sdiff += x.X[k] * L.a[k]; ssum += x.G[k] * L.a[k]; }
+                float wx = x.dnx + L.gx, wy = x.dny + L.gy, wz = x.dnz + L.gz;
+                float l, il, e1, e2;
+                float dgv = sg_term(wx, wy, wz, SG_DS, L.lam, l, il, e1, e2);
+                float dgdl = ((1.f - il) * e1 + (1.f + il) * e2) * il;
+                float qx = x.snx + L.gx, qy = x.sny + L.gy, qz = x.snz + L.gz;
+                float sl, isl, f1, f2;
+                float sgv = sg_term(qx, qy, qz, x.ss, L.lam, sl, isl, f1, f2);
+                float sgdl = ((1.f - isl) * f1 + (1.f + isl) * f2) * isl;
+#pragma unroll
+                for (int k = 0; k < 4; k++) acc[4 + k] += x.X[k] * (dgv * SG_DA) + x.G[k] * (sgv * x.sa);
+                float cd = sdiff * SG_DA * dgdl * il, cs = ssum * x.sa * sgdl * isl;
+                acc[0] += (cd * wx + cs * qx) * L.lam;
+                acc[1] += (cd * wy + cs * qy) * L.lam;
+                acc[2] += (cd * wz + cs * qz) * L.lam;
+                float mdw = L.mx * wx + L.my * wy + L.mz * wz, mdq = L.mx * qx + L.my * qy + L.mz * qz;
+                acc[3] += sdiff * SG_DA * (dgdl * mdw * il - dgv) + ssum * x.sa * (sgdl * mdq * isl - sgv);
+            }
+        }
+        __syncthreads();
+    }
+    if (!want_lobes) return;
+#pragma unroll
+    for (int k = 0; k < SG_COLS; k++) red[tid][k] = acc[k];
+    __syncthreads();
+    for (int i = tid; i < K * SG_COLS; i += BS) {
+        int lobe = i / SG_COLS, col = i % SG_COLS;
+        float sum = 0.f;
+        for (int g2 = 0; g2 < G; g2++) sum += red[g2 * KP + lobe][col];
+        part[(size_t)blockIdx.x * K * SG_COLS + i] = col < 4 + C ? sum : 0.f;
+    }
+}
+
 // ---------------------------------------------------------------- antialias
-struct AAEdge { float alpha, ia, pax, pay, pbx, pby, sg; int va, vb, horiz, target_o; };
+struct AAEdge { float alpha, ia, pax, pay, pbx, pby, sg; int va, vb, horiz, target_o, valid; };
+constexpr int AA_MAXC = 4;
 
 struct AAArgs {
     int W, H, depth, C, V;
@@ -743,6 +1063,7 @@ __device__ int aa_pair(const AAArgs& a, const float4* clipv, const float4* rastv
 #pragma clang fp contract(off)
     int pid = cy * a.W + cx;
     int nid;
+    E[0].valid = E[1].valid = E[2].valid = 0;
     if (!vertical) { if (cx <= 0) return 0; nid = pid - 1; } else { if (cy <= 0) return 0; nid = pid - a.W; }
     float4 t = rastv[pid], nb = rastv[nid];
     if (nb.w == t.w) return 0;
@@ -766,6 +1087,8 @@ __device__ int aa_pair(const AAArgs& a, const float4* clipv, const float4* rastv
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         int ia_ = k, ib_ = (k + 1) % 3;
+        AAEdge& e = E[k];
+        e.valid = 0;
         float pax = vx[ia_], pay = vy[ia_], pbx = vx[ib_], pby = vy[ib_];
         float av = pax * pby - pay * pbx;
         float oa = (pax - ox) * (pby - oy) - (pay - oy) * (pbx - ox);
@@ -778,9 +1101,10 @@ __device__ int aa_pair(const AAArgs& a, const float4* clipv, const float4* rastv
         float D = n > 0.f ? -ey : ex;
         float ia = (float)(1.0 / ((double)D + (D > 0 ? 1e-3 : -1e-3)));
         float alpha = av * ia - .5f;
-        AAEdge& e = E[ne++];
+        e.valid = 1;
         e.alpha = alpha; e.ia = ia; e.pax = pax; e.pay = pay; e.pbx = pbx; e.pby = pby; e.sg = sg;
         e.va = (int)vi[ia_]; e.vb = (int)vi[ib_]; e.horiz = n > 0.f; e.target_o = alpha > 0;
+        ne++;
     }
     return ne;
 }
@@ -796,23 +1120,30 @@ __global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, const float* in, float*
     const float* inv = in + (size_t)s * a.W * a.H * a.C;
     int q = py * a.W + px;
     int C = a.C;
-    float acc[16];
-    for (int c = 0; c < C; c++) acc[c] = inv[(size_t)q * C + c];
-    const int cand[4][3] = {{0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 1}};
+    float acc[AA_MAXC];
+#pragma unroll
+    for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? inv[(size_t)q * C + c] : 0.f;
+    // pairs touching q, in order: (q, left), (q, below), (right, q), (above, q)
     for (int k = 0; k < 4; k++) {
-        int cx = px + cand[k][0], cy = py + cand[k][1];
+        int cx = px + (k == 2), cy = py + (k == 3);
         if (cx >= a.W || cy >= a.H) continue;
-        int pp, op;
+        int pp = 0, op = 0;
         AAEdge E[3];
-        int ne = aa_pair(a, clipv, rastv, cx, cy, cand[k][2], pp, op, E);
-        for (int e = 0; e < ne; e++) {
+        if (!aa_pair(a, clipv, rastv, cx, cy, k & 1, pp, op, E)) continue;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            if (!E[e].valid) continue;
             int tgt = E[e].target_o ? op : pp;
             if (tgt != q) continue;
-            for (int c = 0; c < C; c++) acc[c] += E[e].alpha * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+#pragma unroll
+            for (int c = 0; c < AA_MAXC; c++)
+                if (c < C) acc[c] += E[e].alpha * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
         }
     }
     float* o = out + g * C;
-    for (int c = 0; c < C; c++) o[c] = acc[c];
+#pragma unroll
+    for (int c = 0; c < AA_MAXC; c++)
+        if (c < C) o[c] = acc[c];
 }
 
 __global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, const float* in, const float* gout, float* gin, float* gproj) {
@@ -829,23 +1160,31 @@ __global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, const float* in, const 
     float* gcs = gproj ? gproj + (size_t)s * a.V * 4 : nullptr;
     int q = py * a.W + px;
     int C = a.C;
-    float acc[16];
-    for (int c = 0; c < C; c++) acc[c] = gov[(size_t)q * C + c];
-    const int cand[4][3] = {{0, 0, 0}, {0, 0, 1}, {1, 0, 0}, {0, 1, 1}};
+    float acc[AA_MAXC];
+#pragma unroll
+    for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? gov[(size_t)q * C + c] : 0.f;
     for (int k = 0; k < 4; k++) {
-        int cx = px + cand[k][0], cy = py + cand[k][1];
+        int cx = px + (k == 2), cy = py + (k == 3);
         if (cx >= a.W || cy >= a.H) continue;
-        int pp, op;
+        int pp = 0, op = 0;
         AAEdge E[3];
-        int ne = aa_pair(a, clipv, rastv, cx, cy, cand[k][2], pp, op, E);
-        for (int e = 0; e < ne; e++) {
+        if (!aa_pair(a, clipv, rastv, cx, cy, k & 1, pp, op, E)) continue;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            if (!E[e].valid) continue;
             int tgt = E[e].target_o ? op : pp;
             float al = E[e].alpha;
-            if (pp == q) for (int c = 0; c < C; c++) acc[c] += al * gov[(size_t)tgt * C + c];
-            if (op == q) for (int c = 0; c < C; c++) acc[c] -= al * gov[(size_t)tgt * C + c];
+#pragma unroll
+            for (int c = 0; c < AA_MAXC; c++) {
+                if (c >= C) continue;
+                if (pp == q) acc[c] += al * gov[(size_t)tgt * C + c];
+                if (op == q) acc[c] -= al * gov[(size_t)tgt * C + c];
+            }
             if (gcs && k < 2) {
                 float d = 0.f;
-                for (int c = 0; c < C; c++) d += gov[(size_t)tgt * C + c] * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+#pragma unroll
+                for (int c = 0; c < AA_MAXC; c++)
+                    if (c < C) d += gov[(size_t)tgt * C + c] * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
                 const AAEdge& ed = E[e];
                 float ia = ed.ia, av = ed.pax * ed.pby - ed.pay * ed.pbx;
                 float dDax, dDay, dDbx, dDby;
@@ -869,7 +1208,9 @@ __global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, const float* in, const 
         }
     }
     float* o = gin + g * C;
-    for (int c = 0; c < C; c++) o[c] = acc[c];
+#pragma unroll
+    for (int c = 0; c < AA_MAXC; c++)
+        if (c < C) o[c] = acc[c];
 }
 
 // ---------------------------------------------------------------- MSE
@@ -979,8 +1320,16 @@ extern "C" int vcl_texturemap_backward(const vcl_texturemap_params* p, const vcl
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) ga.tex[l] = l < t.L ? g->texture[l] : nullptr;
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_tex_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, ga, p->rast, p->uv, p->uvDA, g->out, g->uv,
-                       t.L > 1 ? g->uvDA : nullptr, g->daMode);
+    bool all_or_none = true;
+    for (int l = 1; l < t.L; l++) all_or_none &= ((ga.tex[l] != nullptr) == (ga.tex[0] != nullptr));
+    if (t.C == 4 && (size_t)t.tw * t.th <= (1u << 24) && all_or_none) {
+        int tilesX = (p->width + 15) / 16, tilesY = (p->height + 15) / 16;
+        hipLaunchKernelGGL(k_tex_bwd4_tiled, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga,
+                           p->rast, p->uv, p->uvDA, g->out, g->uv, t.L > 1 ? g->uvDA : nullptr, g->daMode);
+    } else {
+        hipLaunchKernelGGL(k_tex_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, ga, p->rast, p->uv, p->uvDA, g->out, g->uv,
+                           t.L > 1 ? g->uvDA : nullptr, g->daMode);
+    }
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -1084,8 +1433,14 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
     a.lo = lo; a.hi = hi; a.scale = quarter ? 0.25f : 1.f;
     size_t n = (size_t)a.tw * a.th * a.C;
     hipStream_t s = as_hip(stream);
+    bool vec4 = a.C == 4;
+    for (int l = 0; l < a.L; l++) vec4 &= ((uintptr_t)a.grad[l] % 16 == 0) && ((uintptr_t)tex->texture[l] % 16 == 0);
+    vec4 &= ((uintptr_t)a.m % 16 == 0) && ((uintptr_t)a.v % 16 == 0);
     if (part & 1) {
-        hipLaunchKernelGGL(k_tex_update, dim3(cdiv(n, BS)), dim3(BS), 0, s, a);
+        if (vec4)
+            hipLaunchKernelGGL(k_tex_update4, dim3(cdiv((size_t)a.tw * a.th, BS)), dim3(BS), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_tex_update, dim3(cdiv(n, BS)), dim3(BS), 0, s, a);
         VCL_LAUNCH_CHECK();
     }
     // rebuild mips from the new level 0; the same pass zeroes the coarse grads
@@ -1093,7 +1448,11 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
         int w = a.tw >> l, hh = a.th >> l;
         size_t m = (size_t)w * hh * a.C;
         if (!m) continue;
-        hipLaunchKernelGGL(k_downsample, dim3(cdiv(m, BS)), dim3(BS), 0, s, tex->texture[l - 1], tex->texture[l], w, hh, a.C, a.grad[l]);
+        if (vec4)
+            hipLaunchKernelGGL(k_downsample4, dim3(cdiv((size_t)w * hh, BS)), dim3(BS), 0, s, (const float4*)tex->texture[l - 1],
+                               (float4*)tex->texture[l], w, hh, (float4*)a.grad[l]);
+        else
+            hipLaunchKernelGGL(k_downsample, dim3(cdiv(m, BS)), dim3(BS), 0, s, tex->texture[l - 1], tex->texture[l], w, hh, a.C, a.grad[l]);
         VCL_LAUNCH_CHECK();
     }
     return VCL_OK;
@@ -1207,8 +1566,14 @@ extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_param
     VCL_REQUIRE(p->sgnum == 0 || (p->axis && p->sharpness && p->amplitude));
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_sg_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
-                       p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv, p->outSpecenv);
+    if (p->sgnum <= SG_MAXK)
+        hipLaunchKernelGGL(k_sg_fwd_fast, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
+                           p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
+                           p->outSpecenv);
+    else
+        hipLaunchKernelGGL(k_sg_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
+                           p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
+                           p->outSpecenv);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -1226,12 +1591,20 @@ extern "C" int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_para
     unsigned nb = sg_bwd_blocks(P);
     size_t cols = (size_t)K * SG_COLS;
     float* part = (float*)g->workspace;
-    size_t lds = sizeof(float) * 4 * (size_t)K * SG_COLS;
-    VCL_REQUIRE(lds <= 160 * 1024);
     hipStream_t s = as_hip(stream);
-    hipLaunchKernelGGL(k_sg_bwd, dim3(nb), dim3(BS), want ? lds : 0, s, P, C, K, p->rast, p->axis, p->sharpness, p->amplitude, p->normal,
-                       p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, g->out, g->diffuse, g->normal, g->specAxis, g->specSharpness,
-                       g->specAmplitude, part, want);
+    bool per_pixel = g->normal || g->specAxis || g->specSharpness || g->specAmplitude;
+    if (!per_pixel && K <= SG_MAXK && p->outDiffenv) {
+        int KP = 1;
+        while (KP < K) KP <<= 1;
+        hipLaunchKernelGGL(k_sg_bwd_lobes, dim3(nb), dim3(BS), 0, s, P, C, K, KP, p->rast, p->axis, p->sharpness, p->amplitude, p->normal,
+                           p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, g->out, p->outDiffenv, g->diffuse, part, want);
+    } else {
+        size_t lds = sizeof(float) * 4 * (size_t)K * SG_COLS;
+        VCL_REQUIRE(lds <= 160 * 1024);
+        hipLaunchKernelGGL(k_sg_bwd, dim3(nb), dim3(BS), want ? lds : 0, s, P, C, K, p->rast, p->axis, p->sharpness, p->amplitude, p->normal,
+                           p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, g->out, g->diffuse, g->normal, g->specAxis,
+                           g->specSharpness, g->specAmplitude, part, want);
+    }
     VCL_LAUNCH_CHECK();
     if (!want) return VCL_OK;
     float* seg = part + (size_t)nb * cols;
@@ -1254,7 +1627,7 @@ static AAArgs aa_args(const vcl_antialias_params* p) {
 }
 
 extern "C" int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t stream) {
-    VCL_REQUIRE(p && p->proj && p->idx && p->rast && p->in && p->out && p->channel >= 1 && p->channel <= 16);
+    VCL_REQUIRE(p && p->proj && p->idx && p->rast && p->in && p->out && p->channel >= 1 && p->channel <= AA_MAXC);
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
     hipLaunchKernelGGL(k_aa_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), aa_args(p), p->in, p->out);
@@ -1263,7 +1636,7 @@ extern "C" int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t
 }
 
 extern "C" int vcl_antialias_backward(const vcl_antialias_params* p, const vcl_antialias_grad_params* g, vcl_stream_t stream) {
-    VCL_REQUIRE(p && g && p->proj && p->idx && p->rast && p->in && g->out && g->in && p->channel >= 1 && p->channel <= 16);
+    VCL_REQUIRE(p && g && p->proj && p->idx && p->rast && p->in && g->out && g->in && p->channel >= 1 && p->channel <= AA_MAXC);
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
     hipLaunchKernelGGL(k_aa_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), aa_args(p), p->in, g->out, g->in, g->proj);

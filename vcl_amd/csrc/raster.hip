@@ -2,14 +2,15 @@
 // Rasterize::forward, rasterize.cpp:257-282) and its backward (rasterize.cu:84-141).
 //
 // Pipeline per call (all on one stream, no host sync, no allocation):
-//   1. clear      : zero tile counters and the overflow / big-list counters
+//   1. clear      : zero tile counters and the overflow counters
 //   2. setup      : one thread per (layer, triangle).  Guard-band test, near/guard-band
 //                   clipping when needed, 8-bit sub-pixel snapping, int64 edge equations
-//                   with the top-left rule; one 64-byte record per piece.  Pieces spanning at
-//                   most BIN_MAX_TILES tiles bump those tiles' counters; larger ones go to a
-//                   "big" list that every tile of their layer scans.
+//                   with the top-left rule; one 64-byte record per piece.  The piece bumps
+//                   the counter of every tile whose pixel-centre lattice it can reach
+//                   (conservative edge test at the tile's extreme corner).
 //   3. scan       : exclusive prefix sum of the tile counters (one workgroup)
-//   4. fill       : pieces written into their tiles' bins (counter decrement gives the slot)
+//   4. fill       : pieces written into their tiles' bins (counter decrement gives the slot);
+//                   a tile whose bin would pass the bin capacity scans every piece instead
 //   5. fine       : one 256-thread workgroup per 16x16 tile.  Records of the tile's bin are
 //                   staged through LDS 256 at a time; every lane keeps the minimum
 //                   (ordered z_ndc, triangle id) key of its pixel in registers, so the result
@@ -23,7 +24,8 @@
 namespace {
 
 constexpr int TILE = 16;
-constexpr int BIN_MAX_TILES = 16;
+constexpr size_t BIN_ENTRIES_PER_PIECE = 32;   // bin capacity per piece (average over pieces)
+constexpr size_t BIN_MIN_CAP = (size_t)1 << 22;
 constexpr float RAST_GB = 16.0f;
 constexpr float RAST_WEPS = 1e-6f;
 constexpr u32 INVALID = 0xffffffffu;
@@ -42,7 +44,7 @@ static_assert(sizeof(RRec) == 64, "record size");
 struct Layout {
     int tilesX, tilesY, ntiles, NT;
     size_t nrec_direct, ovf_cap, nrec_cap, bin_cap;
-    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_big, total;
+    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, total;
 };
 
 Layout make_layout(int W, int H, int depth, int T) {
@@ -54,21 +56,21 @@ Layout make_layout(int W, int H, int depth, int T) {
     L.nrec_direct = (size_t)depth * T;
     L.ovf_cap = L.nrec_direct / 2 + 1024;
     L.nrec_cap = L.nrec_direct + L.ovf_cap;
-    L.bin_cap = L.nrec_cap * BIN_MAX_TILES;
+    L.bin_cap = L.nrec_cap * BIN_ENTRIES_PER_PIECE;
+    if (L.bin_cap < BIN_MIN_CAP) L.bin_cap = BIN_MIN_CAP;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t o = 0;
     L.off_counts = o; o = al(o + sizeof(u32) * (size_t)L.NT);
     L.off_offsets = o; o = al(o + sizeof(u32) * ((size_t)L.NT + 1));
-    L.off_misc = o; o = al(o + sizeof(u32) * 64);   // [0]=ovf count [1]=big count [2]=overflow flag
+    L.off_misc = o; o = al(o + sizeof(u32) * 64);   // [0]=overflow-piece count [2]=overflow flag
     L.off_rec = o; o = al(o + sizeof(RRec) * L.nrec_cap);
     L.off_bins = o; o = al(o + sizeof(u32) * L.bin_cap);
-    L.off_big = o; o = al(o + sizeof(u32) * L.nrec_cap);
     L.total = o;
     return L;
 }
 
 struct Ws {
-    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins; u32* big;
+    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins;
 };
 
 Ws ws_ptrs(void* base, const Layout& L) {
@@ -79,7 +81,6 @@ Ws ws_ptrs(void* base, const Layout& L) {
     w.misc = (u32*)(b + L.off_misc);
     w.rec = (RRec*)(b + L.off_rec);
     w.bins = (u32*)(b + L.off_bins);
-    w.big = (u32*)(b + L.off_big);
     return w;
 }
 
@@ -91,7 +92,7 @@ struct RastArgs {
     float* out;
     float* outDB;
     int tilesX, ntiles;
-    size_t nrec_direct, ovf_cap;
+    size_t nrec_direct, ovf_cap, bin_cap;
     Ws ws;
 };
 
@@ -157,19 +158,41 @@ __device__ int clip_poly(const float4* in, int n, float4* out, float p0, float p
     return m;
 }
 
-__device__ void bin_count(const RastArgs& a, const RRec& r, u32 slot) {
+// Can any pixel centre of tile (tx, ty) be inside the piece?  Each edge function is maximised
+// over the tile's centre lattice at one corner; the piece misses the tile when that maximum
+// fails the inside test (E > 0, or E == 0 on a top-left edge).
+__device__ __forceinline__ bool tile_overlaps(const RRec& r, int tx, int ty) {
+    long long x0 = (long long)(tx * TILE) * 256 + 128, x1 = x0 + (TILE - 1) * 256;
+    long long y0 = (long long)(ty * TILE) * 256 + 128, y1 = y0 + (TILE - 1) * 256;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        long long X = r.A[i] > 0 ? x1 : x0, Y = r.B[i] > 0 ? y1 : y0;
+        long long e = (long long)r.A[i] * X + (long long)r.B[i] * Y + r.C[i];
+        if (e < 0 || (e == 0 && !((r.view_tl >> i) & 1u))) return false;
+    }
+    return true;
+}
+
+template <bool FILL>
+__device__ void bin_piece(const RastArgs& a, const RRec& r, u32 slot) {
     int view = (int)(r.view_tl >> 8);
     int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
     int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
-    int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-    if (n <= BIN_MAX_TILES) {
-        u32* cnt = a.ws.counts + (size_t)view * a.ntiles;
-        for (int ty = ty0; ty <= ty1; ty++)
-            for (int tx = tx0; tx <= tx1; tx++) atomicAdd(cnt + ty * a.tilesX + tx, 1u);
-    } else {
-        u32 b = atomicAdd(a.ws.misc + 1, 1u);
-        a.ws.big[b] = slot;
-    }
+    bool single = tx0 == tx1 && ty0 == ty1;
+    u32* cnt = a.ws.counts + (size_t)view * a.ntiles;
+    const u32* off = a.ws.offsets + (size_t)view * a.ntiles;
+    for (int ty = ty0; ty <= ty1; ty++)
+        for (int tx = tx0; tx <= tx1; tx++) {
+            if (!single && !tile_overlaps(r, tx, ty)) continue;
+            int t = ty * a.tilesX + tx;
+            if (!FILL) {
+                atomicAdd(cnt + t, 1u);
+            } else {
+                u32 pos = atomicSub(cnt + t, 1u) - 1u;
+                size_t at = (size_t)off[t] + pos;
+                if (at < a.bin_cap) a.ws.bins[at] = slot;
+            }
+        }
 }
 
 __global__ void __launch_bounds__(256) k_clear(u32* counts, int NT, u32* misc) {
@@ -190,7 +213,7 @@ __global__ void __launch_bounds__(256) k_setup(RastArgs a) {
     if (vtx_fast(v0) && vtx_fast(v1) && vtx_fast(v2)) {
         if (setup_piece(v0, v1, v2, t, view, a.W, a.H, r)) {
             a.ws.rec[gid] = r;
-            bin_count(a, r, (u32)gid);
+            bin_piece<false>(a, r, (u32)gid);
         } else {
             a.ws.rec[gid].id = INVALID;
         }
@@ -213,7 +236,7 @@ __global__ void __launch_bounds__(256) k_setup(RastArgs a) {
         if (s >= a.ovf_cap) { atomicOr(a.ws.misc + 2, 1u); continue; }
         u32 slot = (u32)(a.nrec_direct + s);
         a.ws.rec[slot] = r;
-        bin_count(a, r, slot);
+        bin_piece<false>(a, r, slot);
     }
 }
 
@@ -264,24 +287,12 @@ __global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
     }
     RRec r = a.ws.rec[slot];
     if (r.id == INVALID) return;
-    int view = (int)(r.view_tl >> 8);
-    int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
-    int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
-    int n = (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
-    if (n > BIN_MAX_TILES) return;
-    for (int ty = ty0; ty <= ty1; ty++)
-        for (int tx = tx0; tx <= tx1; tx++) {
-            size_t tile = (size_t)view * a.ntiles + ty * a.tilesX + tx;
-            u32 pos = atomicSub(a.ws.counts + tile, 1u) - 1u;
-            a.ws.bins[a.ws.offsets[tile] + pos] = (u32)slot;
-        }
+    bin_piece<true>(a, r, (u32)slot);
 }
 
 struct BaryF { float u, v, z, wc; };
 
-__device__ __forceinline__ void bary_f32(float4 c0, float4 c1, float4 c2, int px, int py, float xs, float ys, BaryF& r) {
-    float pixx = (float)((px + 0.5) * (double)xs - 1.0);
-    float pixy = (float)((py + 0.5) * (double)ys - 1.0);
+__device__ __forceinline__ void bary_f32(float4 c0, float4 c1, float4 c2, float pixx, float pixy, BaryF& r) {
     float P0x = c0.w * pixx - c0.x, P0y = c0.w * pixy - c0.y;
     float P1x = c1.w * pixx - c1.x, P1y = c1.w * pixy - c1.y;
     float P2x = c2.w * pixx - c2.x, P2y = c2.w * pixy - c2.y;
@@ -296,7 +307,7 @@ __device__ __forceinline__ void bary_f32(float4 c0, float4 c1, float4 c2, int px
 }
 
 __device__ __forceinline__ void test_record(const RRec& r, int px, int py, long long X, long long Y, u32 view,
-                                            const float4* clip, const u32* idx, float xs, float ys, u64& best) {
+                                            const float4* clip, const u32* idx, float pixx, float pixy, u64& best) {
     if ((r.view_tl >> 8) != view) return;
     int px0 = r.pxr & 0xffff, px1 = r.pxr >> 16, py0 = r.pyr & 0xffff, py1 = r.pyr >> 16;
     if (px < px0 || px > px1 || py < py0 || py > py1) return;
@@ -309,7 +320,7 @@ __device__ __forceinline__ void test_record(const RRec& r, int px, int py, long 
     u32 t = r.id;
     float4 c0 = clip[idx[3 * t]], c1 = clip[idx[3 * t + 1]], c2 = clip[idx[3 * t + 2]];
     BaryF b;
-    bary_f32(c0, c1, c2, px, py, xs, ys, b);
+    bary_f32(c0, c1, c2, pixx, pixy, b);
     if (!(b.wc > 0.0f && b.z >= -1.0f && b.z <= 1.0f)) return;
     u64 k = ((u64)ord_f32(b.z) << 32) | t;
     if (k < best) best = k;
@@ -326,22 +337,31 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
     int py = ty * TILE + (tid >> 4);
     long long X = (long long)px * 256 + 128, Y = (long long)py * 256 + 128;
     const float4* clip = (const float4*)a.proj + (size_t)view * a.V;
+    const float pixx = (float)((px + 0.5) * (double)a.xs - 1.0);
+    const float pixy = (float)((py + 0.5) * (double)a.ys - 1.0);
     u64 best = ~0ull;
     u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
-    for (u32 b = begin; b < end; b += 256) {
-        __syncthreads();
-        if (b + tid < end) lds[tid] = a.ws.rec[a.ws.bins[b + tid]];
-        __syncthreads();
-        int n = (int)min(256u, end - b);
-        for (int j = 0; j < n; j++) test_record(lds[j], px, py, X, Y, view, clip, a.idx, a.xs, a.ys, best);
-    }
-    u32 nbig = a.ws.misc[1];
-    for (u32 b = 0; b < nbig; b += 256) {
-        __syncthreads();
-        if (b + tid < nbig) lds[tid] = a.ws.rec[a.ws.big[b + tid]];
-        __syncthreads();
-        int n = (int)min(256u, nbig - b);
-        for (int j = 0; j < n; j++) test_record(lds[j], px, py, X, Y, view, clip, a.idx, a.xs, a.ys, best);
+    if ((size_t)end <= a.bin_cap) {
+        for (u32 b = begin; b < end; b += 256) {
+            __syncthreads();
+            if (b + tid < end) lds[tid] = a.ws.rec[a.ws.bins[b + tid]];
+            __syncthreads();
+            int n = (int)min(256u, end - b);
+            for (int j = 0; j < n; j++) test_record(lds[j], px, py, X, Y, view, clip, a.idx, pixx, pixy, best);
+        }
+    } else {
+        // bin storage exhausted (pathological geometry): scan every piece of every layer
+        size_t nrec = a.nrec_direct + min(a.ws.misc[0], (u32)a.ovf_cap);
+        for (size_t b = 0; b < nrec; b += 256) {
+            __syncthreads();
+            if (b + tid < nrec) lds[tid] = a.ws.rec[b + tid];
+            __syncthreads();
+            int n = (int)min((size_t)256, nrec - b);
+            for (int j = 0; j < n; j++) {
+                if (lds[j].id == INVALID) continue;
+                test_record(lds[j], px, py, X, Y, view, clip, a.idx, pixx, pixy, best);
+            }
+        }
     }
     if (px >= a.W || py >= a.H) return;
     size_t p = ((size_t)view * a.H + py) * a.W + px;
@@ -350,8 +370,6 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
         u32 t = (u32)(best & 0xffffffffu);
         float4 c0 = clip[a.idx[3 * t]], c1 = clip[a.idx[3 * t + 1]], c2 = clip[a.idx[3 * t + 2]];
         float xs = a.xs, ys = a.ys;
-        float pixx = (float)((px + 0.5) * (double)xs - 1.0);
-        float pixy = (float)((py + 0.5) * (double)ys - 1.0);
         float w0 = c0.w, w1 = c1.w, w2 = c2.w;
         float P0x = w0 * pixx - c0.x, P0y = w0 * pixy - c0.y;
         float P1x = w1 * pixx - c1.x, P1y = w1 * pixy - c1.y;
@@ -376,46 +394,72 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
     if (a.outDB) ((float4*)a.outDB)[p] = d;
 }
 
-// RasterizeBackwardKernel (rasterize.cu:84-141), one thread per pixel.
-__global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int depth, int V, float xs, float ys, const float* proj,
-                                                    const u32* idx, const float* rast, const float* grast, float* gproj) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t P = (size_t)W * H * depth;
-    if (p >= P) return;
-    float4 r = ((const float4*)rast)[p];
-    int id = (int)r.w - 1;
-    if (id < 0) return;
-    float2 g = ((const float2*)grast)[p];
-    int bitcheck = __float_as_int(g.x) | __float_as_int(g.y);
-    if ((bitcheck << 1) == 0) return;
-    int px = (int)(p % W);
-    int py = (int)((p / W) % H);
-    int view = (int)(p / ((size_t)W * H));
-    float pixx = (float)((px + 0.5) * (double)xs - 1.0);
-    float pixy = (float)((py + 0.5) * (double)ys - 1.0);
+// RasterizeBackwardKernel (rasterize.cu:84-141).  One 16x16 pixel tile per workgroup; the
+// nine (x, y, w) vertex terms of every covered pixel are summed per vertex in an LDS hash
+// table, then each touched vertex gets one set of global atomics per tile.
+constexpr int RB_SLOTS = 1024;
+
+__global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float xs, float ys, const float* proj, const u32* idx,
+                                                    const float* rast, const float* grast, float* gproj, int tilesX) {
+    __shared__ LdsTable<RB_SLOTS, 3> tab;
+    lds_table_clear(tab);
+    __syncthreads();
+    int view = blockIdx.y;
+    int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
+    int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
     const float4* clip = (const float4*)proj + (size_t)view * V;
     float* gp = gproj + (size_t)view * V * 4;
-    u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
-    float4 c0 = clip[i0], c1 = clip[i1], c2 = clip[i2];
-    float p0x = c0.w * pixx - c0.x, p0y = c0.w * pixy - c0.y;
-    float p1x = c1.w * pixx - c1.x, p1y = c1.w * pixy - c1.y;
-    float p2x = c2.w * pixx - c2.x, p2y = c2.w * pixy - c2.y;
-    float e0x = p0x - p2x, e0y = p0y - p2y, e1x = p1x - p2x, e1y = p1y - p2y;
-    float aa = e0x * e1y - e0y * e1x;
-    float ia = 1.f / (aa + (aa > 0 ? 1e-6f : -1e-6f));
-    float kuv = r.x * g.x + r.y * g.y;
-    float gx0 = (g.y * p2y + kuv * e1y) * ia;
-    float gy0 = (-g.y * p2x - kuv * e1x) * ia;
-    float gw0 = -gx0 * pixx - gy0 * pixy;
-    float gx1 = (-g.x * p2y - kuv * e0y) * ia;
-    float gy1 = (g.x * p2x + kuv * e0x) * ia;
-    float gw1 = -gx1 * pixx - gy1 * pixy;
-    float gx2 = ((-g.y * p0y + g.x * p1y) - kuv * (-p0y + p1y)) * ia;
-    float gy2 = ((g.y * p0x - g.x * p1x) - kuv * (p0x - p1x)) * ia;
-    float gw2 = -gx2 * pixx - gy2 * pixy;
-    atomicAdd(gp + 4 * i0, gx0); atomicAdd(gp + 4 * i0 + 1, gy0); atomicAdd(gp + 4 * i0 + 3, gw0);
-    atomicAdd(gp + 4 * i1, gx1); atomicAdd(gp + 4 * i1 + 1, gy1); atomicAdd(gp + 4 * i1 + 3, gw1);
-    atomicAdd(gp + 4 * i2, gx2); atomicAdd(gp + 4 * i2 + 1, gy2); atomicAdd(gp + 4 * i2 + 3, gw2);
+    if (px < W && py < H) {
+        size_t p = ((size_t)view * H + py) * W + px;
+        float4 r = ((const float4*)rast)[p];
+        int id = (int)r.w - 1;
+        float2 g = id >= 0 ? ((const float2*)grast)[p] : make_float2(0.f, 0.f);
+        int bitcheck = __float_as_int(g.x) | __float_as_int(g.y);
+        if (id >= 0 && (bitcheck << 1) != 0) {
+            float pixx = (float)((px + 0.5) * (double)xs - 1.0);
+            float pixy = (float)((py + 0.5) * (double)ys - 1.0);
+            u32 vi[3] = {idx[3 * id], idx[3 * id + 1], idx[3 * id + 2]};
+            float4 c0 = clip[vi[0]], c1 = clip[vi[1]], c2 = clip[vi[2]];
+            float p0x = c0.w * pixx - c0.x, p0y = c0.w * pixy - c0.y;
+            float p1x = c1.w * pixx - c1.x, p1y = c1.w * pixy - c1.y;
+            float p2x = c2.w * pixx - c2.x, p2y = c2.w * pixy - c2.y;
+            float e0x = p0x - p2x, e0y = p0y - p2y, e1x = p1x - p2x, e1y = p1y - p2y;
+            float aa = e0x * e1y - e0y * e1x;
+            float ia = 1.f / (aa + (aa > 0 ? 1e-6f : -1e-6f));
+            float kuv = r.x * g.x + r.y * g.y;
+            float gv[3][3];
+            gv[0][0] = (g.y * p2y + kuv * e1y) * ia;
+            gv[0][1] = (-g.y * p2x - kuv * e1x) * ia;
+            gv[0][2] = -gv[0][0] * pixx - gv[0][1] * pixy;
+            gv[1][0] = (-g.x * p2y - kuv * e0y) * ia;
+            gv[1][1] = (g.x * p2x + kuv * e0x) * ia;
+            gv[1][2] = -gv[1][0] * pixx - gv[1][1] * pixy;
+            gv[2][0] = ((-g.y * p0y + g.x * p1y) - kuv * (-p0y + p1y)) * ia;
+            gv[2][1] = ((g.y * p0x - g.x * p1x) - kuv * (p0x - p1x)) * ia;
+            gv[2][2] = -gv[2][0] * pixx - gv[2][1] * pixy;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                int sl = lds_table_slot(tab, vi[k]);
+                if (sl >= 0) {
+                    atomicAdd(&tab.val[0][sl], gv[k][0]);
+                    atomicAdd(&tab.val[1][sl], gv[k][1]);
+                    atomicAdd(&tab.val[2][sl], gv[k][2]);
+                } else {
+                    float* q = gp + 4 * (size_t)vi[k];
+                    atomicAdd(q, gv[k][0]); atomicAdd(q + 1, gv[k][1]); atomicAdd(q + 3, gv[k][2]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < RB_SLOTS; i += 256) {
+        u32 k = tab.key[i];
+        if (k == LDS_EMPTY) continue;
+        float* q = gp + 4 * (size_t)k;
+        atomicAdd(q, tab.val[0][i]);
+        atomicAdd(q + 1, tab.val[1][i]);
+        atomicAdd(q + 3, tab.val[2][i]);
+    }
 }
 
 }  // namespace
@@ -440,7 +484,7 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     a.xs = 2.f / (float)p->width; a.ys = 2.f / (float)p->height;
     a.proj = p->proj; a.idx = p->idx; a.out = p->out; a.outDB = p->enableDB ? p->outDB : nullptr;
     a.tilesX = L.tilesX; a.ntiles = L.ntiles;
-    a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap;
+    a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap; a.bin_cap = L.bin_cap;
     a.ws = ws_ptrs(p->workspace, L);
     int nclear = L.NT > 64 ? L.NT : 64;
     hipLaunchKernelGGL(k_clear, dim3(cdiv(nclear, 256)), dim3(256), 0, s, a.ws.counts, L.NT, a.ws.misc);
@@ -473,10 +517,10 @@ extern "C" int vcl_rasterize_backward(const vcl_rasterize_params* p, const vcl_r
     VCL_REQUIRE(p && g && p->proj && p->idx && p->out);
     if (!g->proj) return VCL_OK;
     VCL_REQUIRE(g->out);
-    size_t P = (size_t)p->width * p->height * p->depth;
-    if (P == 0) return VCL_OK;
-    hipLaunchKernelGGL(k_raster_bwd, dim3(cdiv(P, 256)), dim3(256), 0, as_hip(stream), p->width, p->height, p->depth,
-                       p->projNum, 2.f / (float)p->width, 2.f / (float)p->height, p->proj, p->idx, p->out, g->out, g->proj);
+    if ((size_t)p->width * p->height * p->depth == 0) return VCL_OK;
+    int tilesX = (p->width + 15) / 16, tilesY = (p->height + 15) / 16;
+    hipLaunchKernelGGL(k_raster_bwd, dim3(tilesX * tilesY, p->depth), dim3(256), 0, as_hip(stream), p->width, p->height, p->projNum,
+                       2.f / (float)p->width, 2.f / (float)p->height, p->proj, p->idx, p->out, g->out, g->proj, tilesX);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }

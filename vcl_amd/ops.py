@@ -225,6 +225,7 @@ def sg_specular_backward(rast, normal, view, rough, ior, gAmp, gSharp, grough):
 
 
 def _sg_params(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out=None, denv=None, senv=None):
+    """SphericalGaussianKernelParams; denv (the forward's outDiffenv) enables the lobe-parallel backward."""
     depth, H, W = rast.shape[:3]
     C_ = diffuse.shape[-1]
     return L.SphericalGaussianParams(W, H, depth, C_, _p(rast).value, sharp.numel(), _p(axis).value, _p(sharp).value, _p(amp).value,
@@ -240,9 +241,9 @@ def spherical_gaussian(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, s
 
 
 def spherical_gaussian_backward(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, gout, gdiffuse=None, gaxis=None,
-                                gsharp=None, gamp=None, gnormal=None, gsAxis=None, gsSharp=None, gsAmp=None):
+                                gsharp=None, gamp=None, gnormal=None, gsAxis=None, gsSharp=None, gsAmp=None, denv=None):
     depth, H, W = rast.shape[:3]
-    p = _sg_params(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp)
+    p = _sg_params(rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, denv=denv)
     n = _lib().vcl_spherical_gaussian_workspace_size(W, H, depth, diffuse.shape[-1], sharp.numel())
     ws = torch.empty(int(n), dtype=torch.uint8, device="cuda")
     g = L.SphericalGaussianGradParams(_p(gnormal), _p(gdiffuse), _p(gaxis), _p(gsharp), _p(gamp), _p(gsAxis), _p(gsSharp), _p(gsAmp),
@@ -354,8 +355,8 @@ class Fit:
         arr = np.ascontiguousarray(arr)
         L.check(_lib().vcl_fit_write(self.h, name.encode(), arr.ctypes.data, arr.nbytes, _s()), f"fit_write {name}")
 
-    def profile_begin(self, max_launches=4096):
-        L.check(_lib().vcl_fit_profile_begin(self.h, max_launches), "profile_begin")
+    def profile_begin(self, max_launches=4096, only=None):
+        L.check(_lib().vcl_fit_profile_begin(self.h, max_launches, only.encode() if only else None), "profile_begin")
 
     def profile_end(self):
         """{op name: (total ms, launches)} for the steps since profile_begin (synchronises)."""
