@@ -20,6 +20,20 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_tex_update4 launch at the default workload, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by what the counters read for a 256 MiB device copy
+TRAFFIC_FILE = "profiles/fit_step_1024_100k_hbm_traffic.md"
+
+
+def measured_traffic():
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            for line in f:
+                if line.startswith("k_tex_update4 traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def algorithmic_bytes(sc, views=1):
@@ -102,6 +116,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--breakdown", action="store_true", help="print a per-op table to stderr")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for rocprofv3 --pmc runs: one 256 MiB device copy (counter calibration) then 3 fit steps")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only (DESIGN.md)",
@@ -115,6 +131,14 @@ def main():
     from vcl_amd import scene as S
 
     torch.cuda.set_device(0)
+    if args.pmc_probe:
+        # a plain copy of known size: FETCH_SIZE / WRITE_SIZE of this dispatch calibrate the
+        # counters (256 MiB read + 256 MiB written)
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     sc = S.config2_scene(1, W=args.res)
     tsc = S.config2_scene(2, W=args.res)
     tf = ops.Fit(tsc)
@@ -123,6 +147,14 @@ def main():
     tf.close()
     # warm-up on a throwaway request, then the timed fit starts from the initial state: the
     # timed steps are iterations 1..K of the request (the mesh evolves as it is fitted)
+    if args.pmc_probe:
+        pf = ops.Fit(sc, target=target)
+        for _ in range(3):
+            pf.step()
+        torch.cuda.synchronize()
+        pf.close()
+        print(json.dumps({"pmc_probe": "done"}))
+        return
     wf = ops.Fit(sc, target=target)
     for _ in range(args.warmup):
         wf.step()
@@ -170,8 +202,10 @@ def main():
                          "GBps": round(b / (avg * 1e-3) / 1e9, 1) if b else None}
     rk = kernels.get(ROOF_OP)
     achieved = rk["GBps"] if rk else None
-    roofline = {"bound": "hbm", "kernel": "k_tex_update (" + ROOF_OP + ")", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None, "traffic": None,
+    traffic = measured_traffic() if args.res == 1024 else None
+    roofline = {"bound": "hbm", "kernel": "k_tex_update4 (" + ROOF_OP + ")", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None, "traffic": traffic,
+                "traffic_source": TRAFFIC_FILE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)" if traffic else None,
                 "alg_bytes_per_launch": ab[ROOF_OP], "avg_ms": rk["avg_ms"] if rk else None}
     cpu = None
     if not args.no_cpu:

@@ -120,11 +120,13 @@ static void project_bwd(int V, int views, const T* mats, const T* gclip, T* gpos
 //    interior made positive; a centre exactly on an edge is covered only for a "top-left"
 //    edge: A > 0 || (A == 0 && B < 0) where E = A*X + B*Y + C.  Zero-area pieces are skipped.
 //    No culling (reference never enables GL_CULL_FACE).
-//  * Depth.  Perspective-correct barycentrics of the ORIGINAL triangle from the closed form
-//    of rasterize.cu:101-121 (pix from double, as there), z_ndc = (z.b) / (w.b).  Fragments
-//    with !(w.b > 0) or z_ndc outside [-1, 1] are dropped (GL clip volume).  The visible
-//    fragment is the lexicographic minimum of (z_ndc, triangle id): GL_LESS with draw-order
-//    ties (rasterize.cpp:252-254).
+//  * Depth.  z_ndc of the ORIGINAL triangle at the pixel centre pix (from double, as in
+//    rasterize.cu:101-104) as Z(pix) / W(pix), the ratio of two affine functions whose
+//    coefficients are fixed per triangle (depth_planes).  Fragments with w <= 0 (W and the
+//    barycentric denominator D of opposite signs) or z_ndc outside [-1, 1] are dropped (GL clip
+//    volume).  The visible fragment is the lexicographic minimum of (z_ndc, triangle id):
+//    GL_LESS with draw-order ties (rasterize.cpp:252-254).  Barycentrics of the winner come from
+//    the closed form of rasterize.cu:114-121; rast.z is the key's z_ndc.
 //  * Output rast = (u, v, z_ndc, id + 1) (rasterize.cpp:175), background 0 (glClear).
 //    DB = (du/dX, du/dY, dv/dX, dv/dY) per pixel, analytic (rasterize.cpp:100-164).
 // pinned: brute force over all triangles per pixel (test_raster_bruteforce), watertight
@@ -134,7 +136,44 @@ static void project_bwd(int V, int views, const T* mats, const T* gclip, T* gpos
 static const float RAST_GB = 16.0f;
 static const float RAST_WEPS = 1e-6f;
 
-struct RPiece { int X[3], Y[3]; int px0, px1, py0, py1; u32 id; i64 A[3], B[3], C[3]; int tl[3]; };
+struct RPiece { int X[3], Y[3]; int px0, px1, py0, py1; u32 id; i64 A[3], B[3], C[3]; int tl[3]; float zp[9]; };
+
+// Depth planes of the ORIGINAL triangle.  With P_i = w_i*pix - p_i the closed-form numerators
+// N_i = cross(P_j, P_k), (j, k) = (i+1, i+2) mod 3, are affine in pix:
+//   N_i = (y_j w_k - w_j y_k) X + (w_j x_k - x_j w_k) Y + (x_j y_k - y_j x_k),
+// so Z = sum z_i N_i, W = sum w_i N_i and D = sum N_i are affine too, and z/w = Z / W
+// (the barycentric denominator cancels).  zp = (Az, Bz, Cz, Aw, Bw, Cw, Ad, Bd, Cd).
+static void depth_planes(const float* c0, const float* c1, const float* c2, float* zp) {
+    const float* c[3] = {c0, c1, c2};
+    float a[3], b[3], d[3];
+    for (int i = 0; i < 3; i++) {
+        const float* j = c[(i + 1) % 3];
+        const float* k = c[(i + 2) % 3];
+        a[i] = j[1] * k[3] - j[3] * k[1];
+        b[i] = j[3] * k[0] - j[0] * k[3];
+        d[i] = j[0] * k[1] - j[1] * k[0];
+    }
+    zp[0] = (c0[2] * a[0] + c1[2] * a[1]) + c2[2] * a[2];
+    zp[1] = (c0[2] * b[0] + c1[2] * b[1]) + c2[2] * b[2];
+    zp[2] = (c0[2] * d[0] + c1[2] * d[1]) + c2[2] * d[2];
+    zp[3] = (c0[3] * a[0] + c1[3] * a[1]) + c2[3] * a[2];
+    zp[4] = (c0[3] * b[0] + c1[3] * b[1]) + c2[3] * b[2];
+    zp[5] = (c0[3] * d[0] + c1[3] * d[1]) + c2[3] * d[2];
+    zp[6] = (a[0] + a[1]) + a[2];
+    zp[7] = (b[0] + b[1]) + b[2];
+    zp[8] = (d[0] + d[1]) + d[2];
+}
+
+// Depth key of a fragment: false if the fragment is outside the clip volume (w <= 0 or
+// z/w outside [-1, 1]), else z/w.
+static inline bool frag_depth(const float* zp, float X, float Y, float& z) {
+    float Zp = zp[0] * X + zp[1] * Y + zp[2];
+    float Wp = zp[3] * X + zp[4] * Y + zp[5];
+    float Dp = zp[6] * X + zp[7] * Y + zp[8];
+    if (!((Wp > 0.0f && Dp > 0.0f) || (Wp < 0.0f && Dp < 0.0f))) return false;
+    z = Zp / Wp;
+    return z >= -1.0f && z <= 1.0f;
+}
 
 static inline u32 ord_f32(float z) {
     u32 b; std::memcpy(&b, &z, 4);
@@ -201,8 +240,13 @@ static int clip_poly(const float in[][4], int n, float out[][4], const float pl[
 
 // Produce the raster pieces of triangle `id`; returns count (<= 8).
 static int make_pieces(const float* v0, const float* v1, const float* v2, u32 id, int W, int H, RPiece* out) {
-    if (vtx_fast(v0) && vtx_fast(v1) && vtx_fast(v2))
-        return setup_piece(v0, v1, v2, id, W, H, out[0]) ? 1 : 0;
+    float zp[9];
+    depth_planes(v0, v1, v2, zp);
+    if (vtx_fast(v0) && vtx_fast(v1) && vtx_fast(v2)) {
+        if (!setup_piece(v0, v1, v2, id, W, H, out[0])) return 0;
+        std::memcpy(out[0].zp, zp, sizeof(zp));
+        return 1;
+    }
     static const float planes[6][5] = {
         {0, 0, 1, 1, 0}, {0, 0, 0, 1, -RAST_WEPS},
         {-1, 0, 0, RAST_GB, 0}, {1, 0, 0, RAST_GB, 0}, {0, -1, 0, RAST_GB, 0}, {0, 1, 0, RAST_GB, 0}};
@@ -215,7 +259,7 @@ static int make_pieces(const float* v0, const float* v1, const float* v2, u32 id
     }
     int cnt = 0;
     for (int i = 1; i + 1 < n; i++)
-        if (setup_piece(A[0], A[i], A[i + 1], id, W, H, out[cnt])) cnt++;
+        if (setup_piece(A[0], A[i], A[i + 1], id, W, H, out[cnt])) { std::memcpy(out[cnt].zp, zp, sizeof(zp)); cnt++; }
     return cnt;
 }
 
@@ -271,6 +315,26 @@ static void bary_db(const T* c0, const T* c1, const T* c2, int px, int py, T xs,
     }
 }
 
+// z/w of the winner at the pixel centre by the depth planes (same ops as the key for float).
+template <class T>
+static T depth_at(const T* c0, const T* c1, const T* c2, int px, int py, T xs, T ys) {
+    const T* c[3] = {c0, c1, c2};
+    T a[3], b[3], d[3];
+    for (int i = 0; i < 3; i++) {
+        const T* j = c[(i + 1) % 3];
+        const T* k = c[(i + 2) % 3];
+        a[i] = j[1] * k[3] - j[3] * k[1];
+        b[i] = j[3] * k[0] - j[0] * k[3];
+        d[i] = j[0] * k[1] - j[1] * k[0];
+    }
+    T X = (T)((px + 0.5) * (double)xs - 1.0), Y = (T)((py + 0.5) * (double)ys - 1.0);
+    T Az = (c0[2] * a[0] + c1[2] * a[1]) + c2[2] * a[2], Bz = (c0[2] * b[0] + c1[2] * b[1]) + c2[2] * b[2];
+    T Cz = (c0[2] * d[0] + c1[2] * d[1]) + c2[2] * d[2];
+    T Aw = (c0[3] * a[0] + c1[3] * a[1]) + c2[3] * a[2], Bw = (c0[3] * b[0] + c1[3] * b[1]) + c2[3] * b[2];
+    T Cw = (c0[3] * d[0] + c1[3] * d[1]) + c2[3] * d[2];
+    return (Az * X + Bz * Y + Cz) / (Aw * X + Bw * Y + Cw);
+}
+
 // Winner keys per pixel of one view: (ord(z) << 32) | id, ~0 = empty.
 static void raster_keys(int W, int H, const float* clip, int T_, const u32* idx, std::vector<u64>& key) {
     key.assign((size_t)W * H, ~0ull);
@@ -290,18 +354,17 @@ static void raster_keys(int W, int H, const float* clip, int T_, const u32* idx,
         for (const RPiece& P : pieces) {
             int ya = std::max(P.py0, y0), yb = std::min(P.py1, y1 - 1);
             if (ya > yb) continue;
-            const float* c0 = clip + 4 * (size_t)idx[3 * P.id];
-            const float* c1 = clip + 4 * (size_t)idx[3 * P.id + 1];
-            const float* c2 = clip + 4 * (size_t)idx[3 * P.id + 2];
-            for (int py = ya; py <= yb; py++)
+            for (int py = ya; py <= yb; py++) {
+                float Y = (float)((py + 0.5) * (double)ys - 1.0);
                 for (int px = P.px0; px <= P.px1; px++) {
                     if (!piece_covers(P, px, py)) continue;
-                    BaryF b; bary_f32(c0, c1, c2, px, py, xs, ys, b);
-                    if (!frag_ok(b)) continue;
-                    u64 k = ((u64)ord_f32(b.z) << 32) | P.id;
+                    float X = (float)((px + 0.5) * (double)xs - 1.0), z;
+                    if (!frag_depth(P.zp, X, Y, z)) continue;
+                    u64 k = ((u64)ord_f32(z) << 32) | P.id;
                     u64& dst = key[(size_t)py * W + px];
                     if (k < dst) dst = k;
                 }
+            }
         }
     });
 }
@@ -321,6 +384,7 @@ static void raster_resolve(int W, int H, const T* clip, const u32* idx, const st
                 const T* c1 = clip + 4 * (size_t)idx[3 * id + 1];
                 const T* c2 = clip + 4 * (size_t)idx[3 * id + 2];
                 bary_db<T>(c0, c1, c2, px, py, xs, ys, r, d);
+                r[2] = depth_at<T>(c0, c1, c2, px, py, xs, ys);
                 r[3] = (T)(float)(id + 1);
             }
     });
@@ -337,12 +401,12 @@ extern "C" void orc_raster_brute(int W, int H, const float* clip, int T_, const 
     for (int py = 0; py < H; py++)
         for (int px = 0; px < W; px++) {
             u64 best = ~0ull;
+            float X = (float)((px + 0.5) * (double)xs - 1.0), Y = (float)((py + 0.5) * (double)ys - 1.0);
             for (const RPiece& P : pieces) {
                 if (!piece_covers(P, px, py)) continue;
-                BaryF b;
-                bary_f32(clip + 4 * (size_t)idx[3 * P.id], clip + 4 * (size_t)idx[3 * P.id + 1], clip + 4 * (size_t)idx[3 * P.id + 2], px, py, xs, ys, b);
-                if (!frag_ok(b)) continue;
-                u64 k = ((u64)ord_f32(b.z) << 32) | P.id;
+                float z;
+                if (!frag_depth(P.zp, X, Y, z)) continue;
+                u64 k = ((u64)ord_f32(z) << 32) | P.id;
                 if (k < best) best = k;
             }
             ids[(size_t)py * W + px] = best == ~0ull ? 0u : (u32)(best & 0xffffffffu) + 1u;

@@ -38,13 +38,18 @@ struct __align__(16) RRec {
     long long C[3];
     u32 id;
     u32 view_tl; // view << 8 | top-left bits
+    float zp[9]; // depth planes of the original triangle: Az Bz Cz Aw Bw Cw Ad Bd Cd
+    float pad[3];
 };
-static_assert(sizeof(RRec) == 64, "record size");
+static_assert(sizeof(RRec) == 112, "record size");
+
+constexpr int SMALL_TILES = 4;  // pieces whose box spans more tiles are binned by a whole wave
+constexpr int LARGE_BLOCKS = 512;
 
 struct Layout {
     int tilesX, tilesY, ntiles, NT;
     size_t nrec_direct, ovf_cap, nrec_cap, bin_cap;
-    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, total;
+    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_large, total;
 };
 
 Layout make_layout(int W, int H, int depth, int T) {
@@ -62,15 +67,16 @@ Layout make_layout(int W, int H, int depth, int T) {
     size_t o = 0;
     L.off_counts = o; o = al(o + sizeof(u32) * (size_t)L.NT);
     L.off_offsets = o; o = al(o + sizeof(u32) * ((size_t)L.NT + 1));
-    L.off_misc = o; o = al(o + sizeof(u32) * 64);   // [0]=overflow-piece count [2]=overflow flag
+    L.off_misc = o; o = al(o + sizeof(u32) * 64);   // [0]=overflow-piece count [2]=overflow flag [3]=large count
     L.off_rec = o; o = al(o + sizeof(RRec) * L.nrec_cap);
     L.off_bins = o; o = al(o + sizeof(u32) * L.bin_cap);
+    L.off_large = o; o = al(o + sizeof(u32) * L.nrec_cap);
     L.total = o;
     return L;
 }
 
 struct Ws {
-    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins;
+    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins; u32* large;
 };
 
 Ws ws_ptrs(void* base, const Layout& L) {
@@ -81,6 +87,7 @@ Ws ws_ptrs(void* base, const Layout& L) {
     w.misc = (u32*)(b + L.off_misc);
     w.rec = (RRec*)(b + L.off_rec);
     w.bins = (u32*)(b + L.off_bins);
+    w.large = (u32*)(b + L.off_large);
     return w;
 }
 
@@ -143,6 +150,28 @@ __device__ bool setup_piece(float4 a, float4 b, float4 c, u32 id, u32 view, int 
     return true;
 }
 
+// Depth planes of the original triangle (see the oracle's depth_planes: same float ops).
+__device__ __forceinline__ void depth_planes(float4 c0, float4 c1, float4 c2, float* zp) {
+    float4 c[3] = {c0, c1, c2};
+    float a[3], b[3], d[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        float4 j = c[(i + 1) % 3], k = c[(i + 2) % 3];
+        a[i] = j.y * k.w - j.w * k.y;
+        b[i] = j.w * k.x - j.x * k.w;
+        d[i] = j.x * k.y - j.y * k.x;
+    }
+    zp[0] = (c0.z * a[0] + c1.z * a[1]) + c2.z * a[2];
+    zp[1] = (c0.z * b[0] + c1.z * b[1]) + c2.z * b[2];
+    zp[2] = (c0.z * d[0] + c1.z * d[1]) + c2.z * d[2];
+    zp[3] = (c0.w * a[0] + c1.w * a[1]) + c2.w * a[2];
+    zp[4] = (c0.w * b[0] + c1.w * b[1]) + c2.w * b[2];
+    zp[5] = (c0.w * d[0] + c1.w * d[1]) + c2.w * d[2];
+    zp[6] = (a[0] + a[1]) + a[2];
+    zp[7] = (b[0] + b[1]) + b[2];
+    zp[8] = (d[0] + d[1]) + d[2];
+}
+
 __device__ int clip_poly(const float4* in, int n, float4* out, float p0, float p1, float p2, float p3, float p4) {
     int m = 0;
     for (int i = 0; i < n; i++) {
@@ -173,6 +202,12 @@ __device__ __forceinline__ bool tile_overlaps(const RRec& r, int tx, int ty) {
     return true;
 }
 
+__device__ __forceinline__ int box_tiles(const RRec& r) {
+    int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
+    int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
+    return (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
+}
+
 template <bool FILL>
 __device__ void bin_piece(const RastArgs& a, const RRec& r, u32 slot) {
     int view = (int)(r.view_tl >> 8);
@@ -195,6 +230,47 @@ __device__ void bin_piece(const RastArgs& a, const RRec& r, u32 slot) {
         }
 }
 
+// One wave per large piece: lanes walk the piece's tiles, so the (returning) counter atomics of
+// a big sliver overlap instead of running one after another in a single lane.
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_bin_large(RastArgs a) {
+    int lane = threadIdx.x & 63;
+    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    int nwaves = (gridDim.x * blockDim.x) >> 6;
+    u32 nlarge = a.ws.misc[3];
+    for (u32 i = wave; i < nlarge; i += nwaves) {
+        u32 slot = a.ws.large[i];
+        const RRec& r = a.ws.rec[slot];
+        int view = (int)(r.view_tl >> 8);
+        int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
+        int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
+        int ntx = tx1 - tx0 + 1, n = ntx * (ty1 - ty0 + 1);
+        u32* cnt = a.ws.counts + (size_t)view * a.ntiles;
+        const u32* off = a.ws.offsets + (size_t)view * a.ntiles;
+        for (int k = lane; k < n; k += 64) {
+            int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
+            if (!tile_overlaps(r, tx, ty)) continue;
+            int t = ty * a.tilesX + tx;
+            if (!FILL) {
+                atomicAdd(cnt + t, 1u);
+            } else {
+                u32 pos = atomicSub(cnt + t, 1u) - 1u;
+                size_t at = (size_t)off[t] + pos;
+                if (at < a.bin_cap) a.ws.bins[at] = slot;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void bin_or_defer(const RastArgs& a, const RRec& r, u32 slot) {
+    if (box_tiles(r) <= SMALL_TILES) {
+        bin_piece<false>(a, r, slot);
+    } else {
+        u32 i = atomicAdd(a.ws.misc + 3, 1u);
+        a.ws.large[i] = slot;
+    }
+}
+
 __global__ void __launch_bounds__(256) k_clear(u32* counts, int NT, u32* misc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < NT) counts[i] = 0;
@@ -212,8 +288,9 @@ __global__ void __launch_bounds__(256) k_setup(RastArgs a) {
     RRec r;
     if (vtx_fast(v0) && vtx_fast(v1) && vtx_fast(v2)) {
         if (setup_piece(v0, v1, v2, t, view, a.W, a.H, r)) {
+            depth_planes(v0, v1, v2, r.zp);
             a.ws.rec[gid] = r;
-            bin_piece<false>(a, r, (u32)gid);
+            bin_or_defer(a, r, (u32)gid);
         } else {
             a.ws.rec[gid].id = INVALID;
         }
@@ -230,13 +307,16 @@ __global__ void __launch_bounds__(256) k_setup(RastArgs a) {
         n = clip_poly(A, n, B, planes[p][0], planes[p][1], planes[p][2], planes[p][3], planes[p][4]);
         for (int i = 0; i < n; i++) A[i] = B[i];
     }
+    float zp[9];
+    depth_planes(v0, v1, v2, zp);
     for (int i = 1; i + 1 < n; i++) {
         if (!setup_piece(A[0], A[i], A[i + 1], t, view, a.W, a.H, r)) continue;
+        for (int k = 0; k < 9; k++) r.zp[k] = zp[k];
         u32 s = atomicAdd(a.ws.misc + 0, 1u);
         if (s >= a.ovf_cap) { atomicOr(a.ws.misc + 2, 1u); continue; }
         u32 slot = (u32)(a.nrec_direct + s);
         a.ws.rec[slot] = r;
-        bin_piece<false>(a, r, slot);
+        bin_or_defer(a, r, slot);
     }
 }
 
@@ -286,28 +366,12 @@ __global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
         if (slot - a.nrec_direct >= novf) return;
     }
     RRec r = a.ws.rec[slot];
-    if (r.id == INVALID) return;
+    if (r.id == INVALID || box_tiles(r) > SMALL_TILES) return;
     bin_piece<true>(a, r, (u32)slot);
 }
 
-struct BaryF { float u, v, z, wc; };
-
-__device__ __forceinline__ void bary_f32(float4 c0, float4 c1, float4 c2, float pixx, float pixy, BaryF& r) {
-    float P0x = c0.w * pixx - c0.x, P0y = c0.w * pixy - c0.y;
-    float P1x = c1.w * pixx - c1.x, P1y = c1.w * pixy - c1.y;
-    float P2x = c2.w * pixx - c2.x, P2y = c2.w * pixy - c2.y;
-    float E0x = P0x - P2x, E0y = P0y - P2y, E1x = P1x - P2x, E1y = P1y - P2y;
-    float D = E0x * E1y - E0y * E1x;
-    float u = (P1x * P2y - P1y * P2x) / D;
-    float v = (P2x * P0y - P2y * P0x) / D;
-    float b2 = 1.0f - u - v;
-    float zc = c0.z * u + c1.z * v + c2.z * b2;
-    float wc = c0.w * u + c1.w * v + c2.w * b2;
-    r.u = u; r.v = v; r.wc = wc; r.z = zc / wc;
-}
-
-__device__ __forceinline__ void test_record(const RRec& r, int px, int py, long long X, long long Y, u32 view,
-                                            const float4* clip, const u32* idx, float pixx, float pixy, u64& best) {
+__device__ __forceinline__ void test_record(const RRec& r, int px, int py, long long X, long long Y, u32 view, float pixx, float pixy,
+                                            u64& best) {
     if ((r.view_tl >> 8) != view) return;
     int px0 = r.pxr & 0xffff, px1 = r.pxr >> 16, py0 = r.pyr & 0xffff, py1 = r.pyr >> 16;
     if (px < px0 || px > px1 || py < py0 || py > py1) return;
@@ -317,12 +381,13 @@ __device__ __forceinline__ void test_record(const RRec& r, int px, int py, long 
         long long e = (long long)r.A[i] * X + (long long)r.B[i] * Y + r.C[i];
         if (!(e > 0 || (e == 0 && ((tl >> i) & 1u)))) return;
     }
-    u32 t = r.id;
-    float4 c0 = clip[idx[3 * t]], c1 = clip[idx[3 * t + 1]], c2 = clip[idx[3 * t + 2]];
-    BaryF b;
-    bary_f32(c0, c1, c2, pixx, pixy, b);
-    if (!(b.wc > 0.0f && b.z >= -1.0f && b.z <= 1.0f)) return;
-    u64 k = ((u64)ord_f32(b.z) << 32) | t;
+    float Zp = r.zp[0] * pixx + r.zp[1] * pixy + r.zp[2];
+    float Wp = r.zp[3] * pixx + r.zp[4] * pixy + r.zp[5];
+    float Dp = r.zp[6] * pixx + r.zp[7] * pixy + r.zp[8];
+    if (!((Wp > 0.0f && Dp > 0.0f) || (Wp < 0.0f && Dp < 0.0f))) return;
+    float z = Zp / Wp;
+    if (!(z >= -1.0f && z <= 1.0f)) return;
+    u64 k = ((u64)ord_f32(z) << 32) | r.id;
     if (k < best) best = k;
 }
 
@@ -347,7 +412,7 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
             if (b + tid < end) lds[tid] = a.ws.rec[a.ws.bins[b + tid]];
             __syncthreads();
             int n = (int)min(256u, end - b);
-            for (int j = 0; j < n; j++) test_record(lds[j], px, py, X, Y, view, clip, a.idx, pixx, pixy, best);
+            for (int j = 0; j < n; j++) test_record(lds[j], px, py, X, Y, view, pixx, pixy, best);
         }
     } else {
         // bin storage exhausted (pathological geometry): scan every piece of every layer
@@ -359,7 +424,7 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
             int n = (int)min((size_t)256, nrec - b);
             for (int j = 0; j < n; j++) {
                 if (lds[j].id == INVALID) continue;
-                test_record(lds[j], px, py, X, Y, view, clip, a.idx, pixx, pixy, best);
+                test_record(lds[j], px, py, X, Y, view, pixx, pixy, best);
             }
         }
     }
@@ -378,10 +443,10 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
         float D = E0x * E1y - E0y * E1x;
         float u = (P1x * P2y - P1y * P2x) / D;
         float v = (P2x * P0y - P2y * P0x) / D;
-        float b2 = 1.0f - u - v;
-        float zc = c0.z * u + c1.z * v + c2.z * b2;
-        float wc = w0 * u + w1 * v + w2 * b2;
-        o = make_float4(u, v, zc / wc, (float)(t + 1));
+        float zp[9];
+        depth_planes(c0, c1, c2, zp);
+        float zz = (zp[0] * pixx + zp[1] * pixy + zp[2]) / (zp[3] * pixx + zp[4] * pixy + zp[5]);
+        o = make_float4(u, v, zz, (float)(t + 1));
         if (a.outDB) {
             float dNuX = xs * (w1 * P2y - w2 * P1y), dNuY = ys * (w2 * P1x - w1 * P2x);
             float dNvX = xs * (w2 * P0y - w0 * P2y), dNvY = ys * (w0 * P2x - w2 * P0x);
@@ -493,12 +558,16 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
         hipLaunchKernelGGL(k_setup, dim3(cdiv(L.nrec_direct, 256)), dim3(256), 0, s, a);
         VCL_LAUNCH_CHECK();
     }
+    hipLaunchKernelGGL(k_bin_large<false>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
+    VCL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.ws.counts, a.ws.offsets, L.NT);
     VCL_LAUNCH_CHECK();
     if (L.nrec_cap > 0) {
         hipLaunchKernelGGL(k_fill, dim3(cdiv(L.nrec_cap, 256)), dim3(256), 0, s, a, L.nrec_cap);
         VCL_LAUNCH_CHECK();
     }
+    hipLaunchKernelGGL(k_bin_large<true>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
+    VCL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_fine, dim3(L.NT), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
