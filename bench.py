@@ -117,7 +117,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--breakdown", action="store_true", help="print a per-op table to stderr")
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="for rocprofv3 --pmc runs: one 256 MiB device copy (counter calibration) then 3 fit steps")
+                    help="for rocprofv3 --pmc runs: one 256 MiB device copy (counter calibration) then --probe-steps fit steps")
+    ap.add_argument("--probe-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only (DESIGN.md)",
@@ -149,7 +150,7 @@ def main():
     # timed steps are iterations 1..K of the request (the mesh evolves as it is fitted)
     if args.pmc_probe:
         pf = ops.Fit(sc, target=target)
-        for _ in range(3):
+        for _ in range(args.probe_steps):
             pf.step()
         torch.cuda.synchronize()
         pf.close()

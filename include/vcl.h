@@ -102,6 +102,11 @@ int vcl_rasterize_backward(const vcl_rasterize_params* p, const vcl_rasterize_gr
 /* Device-side status of the last forward on this workspace (VCL_OK or VCL_ERR_OVERFLOW);
  * synchronises the stream. */
 int vcl_rasterize_status(const vcl_rasterize_params* p, vcl_stream_t stream);
+/* Testing hook (no counterpart in the reference): fix the tile-bin capacity, in entries, of
+ * workspaces sized after this call (0 restores the default sizing).  A forward whose bins
+ * would pass the capacity takes the slower scan-every-piece path for the affected tiles and
+ * stays exact. */
+void vcl_rasterize_set_bin_capacity(size_t entries);
 
 /* ---------------------------------------------------------------- Interpolate
  * InterpolateKernelParams / InterpolateKernelGradParams (interpolate.h:7-29);

@@ -93,6 +93,38 @@ def test_raster_soup_ties_and_clipping(gpu):
     _check_raster(gpu, clip, idx, 96, 80, what="soup")
 
 
+def test_raster_deep_bins(gpu):
+    # thousands of small overlapping triangles stacked over a few tiles: bins far deeper than
+    # one fine-raster chunk (256 pieces), so the partial depth keys of several workgroups merge
+    rng = np.random.default_rng(11)
+    n = 6000
+    ctr = rng.uniform(-0.5, 0.5, (n, 1, 2))
+    xy = ctr + rng.uniform(-0.12, 0.12, (n, 3, 2))
+    w = rng.uniform(0.8, 1.5, (n, 3, 1))
+    z = rng.uniform(-0.9, 0.9, (n, 3, 1))
+    clip = np.concatenate([xy * w, z * w, w], 2).reshape(-1, 4).astype(F32)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    out, _ = _check_raster(gpu, clip, idx, 80, 72, what="deep bins")
+    assert (out[..., 3] > 0).sum() > 1500
+
+
+def test_raster_bin_overflow_fallback(gpu):
+    # bins forced far below what the soup needs: overflowing tiles scan every piece, same result
+    from vcl_amd import _lib
+    rng = np.random.default_rng(5)
+    n = 1500
+    clip = rng.uniform(-1.2, 1.2, (3 * n, 4)).astype(F32)
+    clip[:, 3] = rng.uniform(0.5, 2.0, 3 * n)
+    clip[:, 2] = rng.uniform(-1.0, 1.0, 3 * n) * clip[:, 3]
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    lib = _lib.load()
+    lib.vcl_rasterize_set_bin_capacity(5000)
+    try:
+        _check_raster(gpu, clip, idx, 72, 56, what="bin overflow")
+    finally:
+        lib.vcl_rasterize_set_bin_capacity(0)
+
+
 def test_raster_multiview_and_empty(gpu, cfg1):
     sc, _ = cfg1
     cams = S.multiview_cameras(3)

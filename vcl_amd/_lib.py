@@ -133,6 +133,7 @@ SIGNATURES = {
     "vcl_rasterize_forward": ([P_(RasterizeParams), vp], ci),
     "vcl_rasterize_backward": ([P_(RasterizeParams), P_(RasterizeGradParams), vp], ci),
     "vcl_rasterize_status": ([P_(RasterizeParams), vp], ci),
+    "vcl_rasterize_set_bin_capacity": ([csz], None),
     "vcl_interpolate_forward": ([P_(InterpolateParams), vp], ci),
     "vcl_interpolate_backward": ([P_(InterpolateParams), P_(InterpolateGradParams), vp], ci),
     "vcl_texturemap_forward": ([P_(TexturemapParams), vp], ci),

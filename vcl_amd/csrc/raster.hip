@@ -46,11 +46,24 @@ static_assert(sizeof(RRec) == 112, "record size");
 constexpr int SMALL_TILES = 4;  // pieces whose box spans more tiles are binned by a whole wave
 constexpr int LARGE_BLOCKS = 512;
 
+// Fine-raster work unit: up to FINE_CHUNK pieces of one tile's bin.  A tile whose bin holds more
+// pieces is split over several chunks (and workgroups); the last of them to finish merges
+// the partial depth keys.
+constexpr u32 FINE_CHUNK = 256;
+constexpr u32 CHUNK_SCAN_ALL = 0x80000000u;   // count flag: scan every piece of every layer
+constexpr int FINE_MAX_GRID = 65536;
+
+struct Chunk {
+    u32 tile, first, count, k;   // pieces [first, first + count) of the bin; k-th chunk of the tile
+};
+
 struct Layout {
     int tilesX, tilesY, ntiles, NT;
-    size_t nrec_direct, ovf_cap, nrec_cap, bin_cap;
-    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_large, total;
+    size_t nrec_direct, ovf_cap, nrec_cap, bin_cap, chunk_cap;
+    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_large, off_chunks, off_part, off_done, total;
 };
+
+size_t g_bin_cap_override = 0;   // vcl_rasterize_set_bin_capacity (0: sized from the piece count)
 
 Layout make_layout(int W, int H, int depth, int T) {
     Layout L;
@@ -63,6 +76,8 @@ Layout make_layout(int W, int H, int depth, int T) {
     L.nrec_cap = L.nrec_direct + L.ovf_cap;
     L.bin_cap = L.nrec_cap * BIN_ENTRIES_PER_PIECE;
     if (L.bin_cap < BIN_MIN_CAP) L.bin_cap = BIN_MIN_CAP;
+    if (g_bin_cap_override) L.bin_cap = g_bin_cap_override;
+    L.chunk_cap = 2 * (size_t)L.NT + L.bin_cap / FINE_CHUNK + 1;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t o = 0;
     L.off_counts = o; o = al(o + sizeof(u32) * (size_t)L.NT);
@@ -71,12 +86,15 @@ Layout make_layout(int W, int H, int depth, int T) {
     L.off_rec = o; o = al(o + sizeof(RRec) * L.nrec_cap);
     L.off_bins = o; o = al(o + sizeof(u32) * L.bin_cap);
     L.off_large = o; o = al(o + sizeof(u32) * L.nrec_cap);
+    L.off_chunks = o; o = al(o + sizeof(Chunk) * L.chunk_cap);
+    L.off_part = o; o = al(o + sizeof(u64) * TILE * TILE * L.chunk_cap);
+    L.off_done = o; o = al(o + sizeof(u32) * (size_t)L.NT);
     L.total = o;
     return L;
 }
 
 struct Ws {
-    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins; u32* large;
+    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins; u32* large; Chunk* chunks; u64* part; u32* done;
 };
 
 Ws ws_ptrs(void* base, const Layout& L) {
@@ -88,6 +106,9 @@ Ws ws_ptrs(void* base, const Layout& L) {
     w.rec = (RRec*)(b + L.off_rec);
     w.bins = (u32*)(b + L.off_bins);
     w.large = (u32*)(b + L.off_large);
+    w.chunks = (Chunk*)(b + L.off_chunks);
+    w.part = (u64*)(b + L.off_part);
+    w.done = (u32*)(b + L.off_done);
     return w;
 }
 
@@ -99,7 +120,7 @@ struct RastArgs {
     float* out;
     float* outDB;
     int tilesX, ntiles;
-    size_t nrec_direct, ovf_cap, bin_cap;
+    size_t nrec_direct, ovf_cap, bin_cap, chunk_cap;
     Ws ws;
 };
 
@@ -271,9 +292,9 @@ __device__ __forceinline__ void bin_or_defer(const RastArgs& a, const RRec& r, u
     }
 }
 
-__global__ void __launch_bounds__(256) k_clear(u32* counts, int NT, u32* misc) {
+__global__ void __launch_bounds__(256) k_clear(u32* counts, u32* done, int NT, u32* misc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < NT) counts[i] = 0;
+    if (i < NT) { counts[i] = 0; done[i] = 0; }
     if (i < 64) misc[i] = 0;
 }
 
@@ -320,42 +341,68 @@ __global__ void __launch_bounds__(256) k_setup(RastArgs a) {
     }
 }
 
-// Exclusive scan of NT counters (single workgroup of 1024 lanes, chunked).
-__global__ void __launch_bounds__(1024) k_scan(const u32* counts, u32* offsets, int NT) {
-    __shared__ u32 wsum[16];
-    __shared__ u32 carry;
+// Exclusive prefix of one value per thread over a 1024-lane workgroup, continuing from *carry
+// (which is advanced by the workgroup's total).
+__device__ u32 scan1024(u32 v, u32* wsum, u32* carry) {
     int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) carry = 0;
+    u32 x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        u32 y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wid] = x;
     __syncthreads();
+    if (wid == 0) {
+        u32 s = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            u32 y = __shfl_up(s, o, 64);
+            if (lane >= o) s += y;
+        }
+        if (lane < 16) wsum[lane] = s;
+    }
+    __syncthreads();
+    u32 pre = *carry + (wid > 0 ? wsum[wid - 1] : 0u) + x - v;
+    __syncthreads();
+    if (tid == 1023) *carry = pre + v;
+    __syncthreads();
+    return pre;
+}
+
+// Single workgroup: exclusive scan of the NT tile counters into bin offsets, then the fine-raster
+// work list: each tile's bin cut into chunks of FINE_CHUNK pieces (one chunk for an empty tile,
+// and one scan-everything chunk for a tile whose bin would pass the bin capacity).
+__global__ void __launch_bounds__(1024) k_scan(RastArgs a, int NT) {
+    __shared__ u32 wsum[16];
+    __shared__ u32 carry[2];
+    int tid = threadIdx.x;
+    if (tid == 0) { carry[0] = 0; carry[1] = 0; }
+    __syncthreads();
+    const u32 nrec = (u32)(a.nrec_direct + min(a.ws.misc[0], (u32)a.ovf_cap));
     for (int base = 0; base < NT; base += 1024) {
         int i = base + tid;
-        u32 v = i < NT ? counts[i] : 0u;
-        // inclusive wave scan
-        u32 x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            u32 y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
+        u32 v = i < NT ? a.ws.counts[i] : 0u;
+        u32 pre = scan1024(v, wsum, &carry[0]);
+        if (i < NT) a.ws.offsets[i] = pre;
+        bool all = (size_t)pre + v > a.bin_cap;
+        u32 nch = i >= NT ? 0u : all ? 1u : max(1u, (v + FINE_CHUNK - 1) / FINE_CHUNK);
+        u32 cpre = scan1024(nch, wsum, &carry[1]);
+        for (u32 k = 0; k < nch; k++) {
+            size_t c = (size_t)cpre + k;
+            if (c >= a.chunk_cap) break;
+            Chunk ch;
+            ch.tile = (u32)i;
+            ch.first = all ? 0u : pre + k * FINE_CHUNK;
+            ch.count = all ? (nrec | CHUNK_SCAN_ALL) : min(FINE_CHUNK, v - k * FINE_CHUNK);
+            ch.k = k;
+            a.ws.chunks[c] = ch;
         }
-        if (lane == 63) wsum[wid] = x;
-        __syncthreads();
-        if (wid == 0) {
-            u32 s = lane < 16 ? wsum[lane] : 0u;
-#pragma unroll
-            for (int o = 1; o < 16; o <<= 1) {
-                u32 y = __shfl_up(s, o, 64);
-                if (lane >= o) s += y;
-            }
-            if (lane < 16) wsum[lane] = s;
-        }
-        __syncthreads();
-        u32 pre = carry + (wid > 0 ? wsum[wid - 1] : 0u) + x - v;
-        if (i < NT) offsets[i] = pre;
-        __syncthreads();
-        if (tid == 1023) carry = pre + v;
-        __syncthreads();
     }
-    if (tid == 0) offsets[NT] = carry;
+    if (tid == 0) {
+        a.ws.offsets[NT] = carry[0];
+        a.ws.misc[5] = (u32)min((size_t)carry[1], a.chunk_cap);
+    }
 }
 
 __global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
@@ -370,65 +417,38 @@ __global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
     bin_piece<true>(a, r, (u32)slot);
 }
 
-__device__ __forceinline__ void test_record(const RRec& r, int px, int py, long long X, long long Y, u32 view, float pixx, float pixy,
-                                            u64& best) {
-    if ((r.view_tl >> 8) != view) return;
-    int px0 = r.pxr & 0xffff, px1 = r.pxr >> 16, py0 = r.pyr & 0xffff, py1 = r.pyr >> 16;
-    if (px < px0 || px > px1 || py < py0 || py > py1) return;
-    u32 tl = r.view_tl;
+// Covered pixel columns of one piece in one row of a tile.  Edge i passes at column x (pixel
+// centre X = X0 + 256 x) iff K_i + dA_i x > 0, with K_i = A_i X0 + B_i Y + C_i + tl_i (the
+// integer form of "E > 0, or E == 0 on a top-left edge") and dA_i = 256 A_i; each edge is
+// monotone along the row, so its boundary is found by bisection with exact int64 tests.
+__device__ __forceinline__ bool edge_in(long long K, long long dA, int x) { return K + dA * x > 0; }
+
+__device__ __forceinline__ void row_span(const long long* K, const long long* dA, int& lo, int& hi) {
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-        long long e = (long long)r.A[i] * X + (long long)r.B[i] * Y + r.C[i];
-        if (!(e > 0 || (e == 0 && ((tl >> i) & 1u)))) return;
+        if (lo > hi) return;
+        if (dA[i] == 0) {
+            if (K[i] <= 0) { lo = 1; hi = 0; }
+        } else if (dA[i] > 0) {   // false ... true: first passing column
+            if (edge_in(K[i], dA[i], lo)) continue;
+            if (!edge_in(K[i], dA[i], hi)) { lo = 1; hi = 0; continue; }
+            int l = lo, h = hi;   // pred(l) false, pred(h) true
+            while (h - l > 1) { int m = (l + h) >> 1; if (edge_in(K[i], dA[i], m)) h = m; else l = m; }
+            lo = h;
+        } else {                  // true ... false: last passing column
+            if (edge_in(K[i], dA[i], hi)) continue;
+            if (!edge_in(K[i], dA[i], lo)) { lo = 1; hi = 0; continue; }
+            int l = lo, h = hi;   // pred(l) true, pred(h) false
+            while (h - l > 1) { int m = (l + h) >> 1; if (edge_in(K[i], dA[i], m)) l = m; else h = m; }
+            hi = l;
+        }
     }
-    float Zp = r.zp[0] * pixx + r.zp[1] * pixy + r.zp[2];
-    float Wp = r.zp[3] * pixx + r.zp[4] * pixy + r.zp[5];
-    float Dp = r.zp[6] * pixx + r.zp[7] * pixy + r.zp[8];
-    if (!((Wp > 0.0f && Dp > 0.0f) || (Wp < 0.0f && Dp < 0.0f))) return;
-    float z = Zp / Wp;
-    if (!(z >= -1.0f && z <= 1.0f)) return;
-    u64 k = ((u64)ord_f32(z) << 32) | r.id;
-    if (k < best) best = k;
 }
 
-__global__ void __launch_bounds__(256) k_fine(RastArgs a) {
-    __shared__ RRec lds[256];
-    int tile = blockIdx.x;
-    u32 view = (u32)(tile / a.ntiles);
-    int tl = tile % a.ntiles;
-    int tx = tl % a.tilesX, ty = tl / a.tilesX;
-    int tid = threadIdx.x;
-    int px = tx * TILE + (tid & (TILE - 1));
-    int py = ty * TILE + (tid >> 4);
-    long long X = (long long)px * 256 + 128, Y = (long long)py * 256 + 128;
-    const float4* clip = (const float4*)a.proj + (size_t)view * a.V;
-    const float pixx = (float)((px + 0.5) * (double)a.xs - 1.0);
-    const float pixy = (float)((py + 0.5) * (double)a.ys - 1.0);
-    u64 best = ~0ull;
-    u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
-    if ((size_t)end <= a.bin_cap) {
-        for (u32 b = begin; b < end; b += 256) {
-            __syncthreads();
-            if (b + tid < end) lds[tid] = a.ws.rec[a.ws.bins[b + tid]];
-            __syncthreads();
-            int n = (int)min(256u, end - b);
-            for (int j = 0; j < n; j++) test_record(lds[j], px, py, X, Y, view, pixx, pixy, best);
-        }
-    } else {
-        // bin storage exhausted (pathological geometry): scan every piece of every layer
-        size_t nrec = a.nrec_direct + min(a.ws.misc[0], (u32)a.ovf_cap);
-        for (size_t b = 0; b < nrec; b += 256) {
-            __syncthreads();
-            if (b + tid < nrec) lds[tid] = a.ws.rec[b + tid];
-            __syncthreads();
-            int n = (int)min((size_t)256, nrec - b);
-            for (int j = 0; j < n; j++) {
-                if (lds[j].id == INVALID) continue;
-                test_record(lds[j], px, py, X, Y, view, pixx, pixy, best);
-            }
-        }
-    }
+// Winner of one pixel -> (u, v, z/w, id+1) and DB, written with 16-byte stores.
+__device__ void fine_resolve(const RastArgs& a, u32 view, int px, int py, float pixx, float pixy, u64 best) {
     if (px >= a.W || py >= a.H) return;
+    const float4* clip = (const float4*)a.proj + (size_t)view * a.V;
     size_t p = ((size_t)view * a.H + py) * a.W + px;
     float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
     if (best != ~0ull) {
@@ -457,6 +477,105 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
     }
     ((float4*)a.out)[p] = o;
     if (a.outDB) ((float4*)a.outDB)[p] = d;
+}
+
+// One 256-thread workgroup per work-list chunk (up to FINE_CHUNK pieces of one 16x16 tile's
+// bin, so a deep bin is spread over several workgroups).  Thread i takes piece i of the chunk
+// and walks its exact covered column span in each of its rows (bisection on the integer edge
+// functions, so no per-pixel edge test), evaluating the piece's depth planes at each covered
+// pixel centre and taking the LDS 64-bit minimum of the (ordered z_ndc, triangle id) key; the
+// result therefore does not depend on bin order or on how the bin is split.  A tile with one
+// chunk resolves its pixels at once; otherwise each chunk stores its 256 partial keys and the
+// last chunk of the tile to finish (device-scope counter) takes the minimum over all of them.
+__global__ void __launch_bounds__(256) k_fine(RastArgs a) {
+    __shared__ u64 key[256];
+    __shared__ float spix[2][TILE];
+    __shared__ u32 s_last;
+    const int tid = threadIdx.x;
+    const u32 nchunks = a.ws.misc[5];
+    for (u32 c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const Chunk ch = a.ws.chunks[c];
+        const int tile = (int)ch.tile;
+        const u32 view = (u32)(tile / a.ntiles);
+        const int tl = tile % a.ntiles;
+        const int tx = tl % a.tilesX, ty = tl / a.tilesX;
+        key[tid] = ~0ull;
+        if (tid < 2 * TILE) {
+            int cc = tid & (TILE - 1);
+            spix[tid >> 4][cc] = tid < TILE ? (float)((tx * TILE + cc + 0.5) * (double)a.xs - 1.0)
+                                            : (float)((ty * TILE + cc + 0.5) * (double)a.ys - 1.0);
+        }
+        const int x0 = tx * TILE, y0 = ty * TILE;
+        const long long X0 = (long long)x0 * 256 + 128;
+        const bool scan_all = (ch.count & CHUNK_SCAN_ALL) != 0u;
+        const u32 n = ch.count & ~CHUNK_SCAN_ALL;
+        __syncthreads();
+        for (u32 q = tid; q < n; q += 256) {
+            u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
+            const RRec& r = a.ws.rec[slot];
+            u32 id = r.id, vtl = r.view_tl;
+            int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TILE - 1);
+            int yl = max((r.pyr & 0xffff) - y0, 0), yh = min((r.pyr >> 16) - y0, TILE - 1);
+            if (id == INVALID || (vtl >> 8) != view || xl > xh || yl > yh) continue;
+            float zp[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) zp[k] = r.zp[k];
+            long long K[3], dA[3], dB[3];
+            long long Y = (long long)(y0 + yl) * 256 + 128;
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                K[i] = (long long)r.A[i] * X0 + (long long)r.B[i] * Y + r.C[i] + ((vtl >> i) & 1u);
+                dA[i] = (long long)r.A[i] * 256;
+                dB[i] = (long long)r.B[i] * 256;
+            }
+            for (int y = yl; y <= yh; y++) {
+                int lo = xl, hi = xh;
+                row_span(K, dA, lo, hi);
+#pragma unroll
+                for (int i = 0; i < 3; i++) K[i] += dB[i];
+                if (lo > hi) continue;
+                const float pixy = spix[1][y];
+                const float zy = zp[1] * pixy, wy = zp[4] * pixy, dy = zp[7] * pixy;
+                for (int x = lo; x <= hi; x++) {
+                    const float pixx = spix[0][x];
+                    float Zp = zp[0] * pixx + zy + zp[2];
+                    float Wp = zp[3] * pixx + wy + zp[5];
+                    float Dp = zp[6] * pixx + dy + zp[8];
+                    if (!((Wp > 0.0f && Dp > 0.0f) || (Wp < 0.0f && Dp < 0.0f))) continue;
+                    float z = Zp / Wp;
+                    if (!(z >= -1.0f && z <= 1.0f)) continue;
+                    atomicMin(&key[y * TILE + x], ((u64)ord_f32(z) << 32) | id);
+                }
+            }
+        }
+        __syncthreads();
+        u64 best = key[tid];
+        bool resolve = true;
+        if (!scan_all) {
+            const u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
+            const u32 nch = max(1u, (end - begin + FINE_CHUNK - 1) / FINE_CHUNK);
+            if (nch > 1) {
+                a.ws.part[(size_t)c * 256 + tid] = best;
+                __threadfence();
+                __syncthreads();
+                if (tid == 0) s_last = atomicAdd(a.ws.done + tile, 1u) == nch - 1 ? 1u : 0u;
+                __syncthreads();
+                resolve = s_last != 0u;
+                if (resolve) {
+                    __threadfence();
+                    const u32 c0 = c - ch.k;
+                    for (u32 i = 0; i < nch; i++) {
+                        if (c0 + i == c) continue;
+                        u64 o = a.ws.part[(size_t)(c0 + i) * 256 + tid];
+                        best = o < best ? o : best;
+                    }
+                }
+            }
+        }
+        if (resolve)
+            fine_resolve(a, view, x0 + (tid & (TILE - 1)), y0 + (tid >> 4), spix[0][tid & (TILE - 1)], spix[1][tid >> 4], best);
+        __syncthreads();
+    }
 }
 
 // RasterizeBackwardKernel (rasterize.cu:84-141).  One 16x16 pixel tile per workgroup; the
@@ -549,10 +668,10 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     a.xs = 2.f / (float)p->width; a.ys = 2.f / (float)p->height;
     a.proj = p->proj; a.idx = p->idx; a.out = p->out; a.outDB = p->enableDB ? p->outDB : nullptr;
     a.tilesX = L.tilesX; a.ntiles = L.ntiles;
-    a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap; a.bin_cap = L.bin_cap;
+    a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap; a.bin_cap = L.bin_cap; a.chunk_cap = L.chunk_cap;
     a.ws = ws_ptrs(p->workspace, L);
     int nclear = L.NT > 64 ? L.NT : 64;
-    hipLaunchKernelGGL(k_clear, dim3(cdiv(nclear, 256)), dim3(256), 0, s, a.ws.counts, L.NT, a.ws.misc);
+    hipLaunchKernelGGL(k_clear, dim3(cdiv(nclear, 256)), dim3(256), 0, s, a.ws.counts, a.ws.done, L.NT, a.ws.misc);
     VCL_LAUNCH_CHECK();
     if (L.nrec_direct > 0) {
         hipLaunchKernelGGL(k_setup, dim3(cdiv(L.nrec_direct, 256)), dim3(256), 0, s, a);
@@ -560,7 +679,7 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     }
     hipLaunchKernelGGL(k_bin_large<false>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a.ws.counts, a.ws.offsets, L.NT);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a, L.NT);
     VCL_LAUNCH_CHECK();
     if (L.nrec_cap > 0) {
         hipLaunchKernelGGL(k_fill, dim3(cdiv(L.nrec_cap, 256)), dim3(256), 0, s, a, L.nrec_cap);
@@ -568,10 +687,12 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     }
     hipLaunchKernelGGL(k_bin_large<true>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fine, dim3(L.NT), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_fine, dim3((unsigned)std::min(L.chunk_cap, (size_t)FINE_MAX_GRID)), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
+
+extern "C" void vcl_rasterize_set_bin_capacity(size_t entries) { g_bin_cap_override = entries; }
 
 extern "C" int vcl_rasterize_status(const vcl_rasterize_params* p, vcl_stream_t stream) {
     VCL_REQUIRE(p && p->workspace);
