@@ -2,19 +2,24 @@
 // Rasterize::forward, rasterize.cpp:257-282) and its backward (rasterize.cu:84-141).
 //
 // Pipeline per call (all on one stream, no host sync, no allocation):
-//   1. clear      : zero tile counters and the overflow counters
+//   1. clear      : zero tile counters, merge counters and the overflow counters
 //   2. setup      : one thread per (layer, triangle).  Guard-band test, near/guard-band
 //                   clipping when needed, 8-bit sub-pixel snapping, int64 edge equations
-//                   with the top-left rule; one 64-byte record per piece.  The piece bumps
-//                   the counter of every tile whose pixel-centre lattice it can reach
-//                   (conservative edge test at the tile's extreme corner).
-//   3. scan       : exclusive prefix sum of the tile counters (one workgroup)
-//   4. fill       : pieces written into their tiles' bins (counter decrement gives the slot);
-//                   a tile whose bin would pass the bin capacity scans every piece instead
-//   5. fine       : one 256-thread workgroup per 16x16 tile.  Records of the tile's bin are
-//                   staged through LDS 256 at a time; every lane keeps the minimum
-//                   (ordered z_ndc, triangle id) key of its pixel in registers, so the result
-//                   does not depend on bin order.  The winner's (u, v, z/w, id+1) and DB are
+//                   with the top-left rule, depth planes of the original triangle; one
+//                   112-byte record per piece.  A small piece (box of at most 4 tiles) counts
+//                   itself into every tile whose pixel-centre lattice it can reach (exact
+//                   edge test at the tile's extreme corner), aggregated per workgroup in an
+//                   LDS hash table; larger pieces go to a list.
+//   3. bin_large  : one wave per listed piece counts its tiles.
+//   4. scan       : one workgroup: exclusive prefix of the tile counters (bin offsets) and the
+//                   fine-raster work list (each bin cut into chunks of up to 256 pieces).
+//   5. fill       : small pieces written into their tiles' bins (LDS-aggregated reservation),
+//                   then large pieces by one wave each.  A tile whose bin would pass the bin
+//                   capacity scans every piece instead.
+//   6. fine       : one 256-thread workgroup per chunk; each thread walks the exact covered
+//                   row spans of one piece and takes the LDS minimum of the (ordered z_ndc,
+//                   triangle id) key per pixel; a tile split over several chunks merges the
+//                   partial keys in its last chunk.  The winner's (u, v, z/w, id+1) and DB are
 //                   written with 16-byte stores.
 // Coverage and depth use the exact float/int arithmetic of the CPU oracle's spec; this file
 // is compiled without FP contraction so both produce identical triangle IDs.
@@ -229,36 +234,91 @@ __device__ __forceinline__ int box_tiles(const RRec& r) {
     return (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
 }
 
-template <bool FILL>
-__device__ void bin_piece(const RastArgs& a, const RRec& r, u32 slot) {
+// ---------------------------------------------------------------- binning
+// The per-tile counters are hot (hundreds of pieces land on one tile), so every binning pass
+// first aggregates in an LDS hash table keyed by the global tile index (view * ntiles + tile):
+// one global atomic per distinct tile per workgroup.  A key that finds no room in the table
+// falls back to a direct global atomic.
+constexpr int BIN_SLOTS = 2048;
+
+// key -> val.  Count pass: val = entries.  Fill pass: val = entries, then (bin_fill_reserve)
+// the first position of the workgroup's run in the tile's bin, advanced by each write.
+struct BinTable {
+    u32 key[BIN_SLOTS];
+    u32 val[BIN_SLOTS];
+};
+
+__device__ __forceinline__ void bin_table_clear(BinTable& t) {
+    for (int i = threadIdx.x; i < BIN_SLOTS; i += blockDim.x) { t.key[i] = LDS_EMPTY; t.val[i] = 0; }
+}
+
+__device__ __forceinline__ int bin_slot(BinTable& t, u32 k) {
+    u32 h = (k * 2654435761u) & (BIN_SLOTS - 1);
+    for (int probe = 0; probe < 32; probe++) {
+        u32 old = atomicCAS(&t.key[h], LDS_EMPTY, k);
+        if (old == LDS_EMPTY || old == k) return (int)h;
+        h = (h + 1) & (BIN_SLOTS - 1);
+    }
+    return -1;
+}
+
+// count pass: one piece-tile entry
+__device__ __forceinline__ void bin_count(const RastArgs& a, BinTable& t, u32 g) {
+    int sl = bin_slot(t, g);
+    if (sl >= 0) atomicAdd(&t.val[sl], 1u);
+    else atomicAdd(a.ws.counts + g, 1u);
+}
+
+// count pass: flush the workgroup's per-tile totals (n entries of (key, val))
+__device__ __forceinline__ void bin_count_flush(const RastArgs& a, const u32* key, const u32* val, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (val[i] && (!key || key[i] != LDS_EMPTY)) atomicAdd(a.ws.counts + (key ? key[i] : (u32)i), val[i]);
+}
+
+// fill pass: reserve the workgroup's run of each tile's bin (the counters count down from the
+// totals of the count pass); val becomes the first position of the run within the tile's bin
+__device__ __forceinline__ void bin_fill_reserve(const RastArgs& a, const u32* key, u32* val, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        if (val[i] && (!key || key[i] != LDS_EMPTY)) val[i] = atomicSub(a.ws.counts + (key ? key[i] : (u32)i), val[i]) - val[i];
+}
+
+__device__ __forceinline__ void bin_write(const RastArgs& a, u32 g, u32 pos, u32 slot) {
+    size_t at = (size_t)a.ws.offsets[g] + pos;
+    if (at < a.bin_cap) a.ws.bins[at] = slot;
+}
+
+// fill pass, direct path for an entry whose key found no table slot
+__device__ __forceinline__ void bin_fill_direct(const RastArgs& a, u32 g, u32 slot) {
+    bin_write(a, g, atomicSub(a.ws.counts + g, 1u) - 1u, slot);
+}
+
+__device__ __forceinline__ bool piece_small(const RRec& r) { return box_tiles(r) <= SMALL_TILES; }
+
+// Tiles (global index) of a small piece that its pixel-centre lattice can reach; returns the count.
+__device__ __forceinline__ int small_tiles(const RastArgs& a, const RRec& r, u32* g) {
     int view = (int)(r.view_tl >> 8);
     int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
     int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
     bool single = tx0 == tx1 && ty0 == ty1;
-    u32* cnt = a.ws.counts + (size_t)view * a.ntiles;
-    const u32* off = a.ws.offsets + (size_t)view * a.ntiles;
+    int n = 0;
     for (int ty = ty0; ty <= ty1; ty++)
         for (int tx = tx0; tx <= tx1; tx++) {
             if (!single && !tile_overlaps(r, tx, ty)) continue;
-            int t = ty * a.tilesX + tx;
-            if (!FILL) {
-                atomicAdd(cnt + t, 1u);
-            } else {
-                u32 pos = atomicSub(cnt + t, 1u) - 1u;
-                size_t at = (size_t)off[t] + pos;
-                if (at < a.bin_cap) a.ws.bins[at] = slot;
-            }
+            if (n < SMALL_TILES) g[n++] = (u32)(view * a.ntiles + ty * a.tilesX + tx);
         }
+    return n;
 }
 
-// One wave per large piece: lanes walk the piece's tiles, so the (returning) counter atomics of
-// a big sliver overlap instead of running one after another in a single lane.
+// Large pieces (box over SMALL_TILES tiles): one wave per piece, lanes walk the piece's tiles,
+// so the counter atomics of a long sliver overlap instead of running one after another in a
+// single lane.  These entries spread over many tiles per workgroup, so they go straight to
+// the global counters (LDS aggregation measured slower here).
 template <bool FILL>
 __global__ void __launch_bounds__(256) k_bin_large(RastArgs a) {
-    int lane = threadIdx.x & 63;
-    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    int nwaves = (gridDim.x * blockDim.x) >> 6;
-    u32 nlarge = a.ws.misc[3];
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (gridDim.x * blockDim.x) >> 6;
+    const u32 nlarge = a.ws.misc[3];
     for (u32 i = wave; i < nlarge; i += nwaves) {
         u32 slot = a.ws.large[i];
         const RRec& r = a.ws.rec[slot];
@@ -266,29 +326,35 @@ __global__ void __launch_bounds__(256) k_bin_large(RastArgs a) {
         int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
         int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
         int ntx = tx1 - tx0 + 1, n = ntx * (ty1 - ty0 + 1);
-        u32* cnt = a.ws.counts + (size_t)view * a.ntiles;
-        const u32* off = a.ws.offsets + (size_t)view * a.ntiles;
         for (int k = lane; k < n; k += 64) {
             int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
             if (!tile_overlaps(r, tx, ty)) continue;
-            int t = ty * a.tilesX + tx;
-            if (!FILL) {
-                atomicAdd(cnt + t, 1u);
-            } else {
-                u32 pos = atomicSub(cnt + t, 1u) - 1u;
-                size_t at = (size_t)off[t] + pos;
-                if (at < a.bin_cap) a.ws.bins[at] = slot;
-            }
+            u32 g = (u32)(view * a.ntiles + ty * a.tilesX + tx);
+            if (!FILL) atomicAdd(a.ws.counts + g, 1u);
+            else bin_fill_direct(a, g, slot);
         }
     }
 }
 
-__device__ __forceinline__ void bin_or_defer(const RastArgs& a, const RRec& r, u32 slot) {
-    if (box_tiles(r) <= SMALL_TILES) {
-        bin_piece<false>(a, r, slot);
+// Workgroup-local list of the large pieces found by setup, appended to the global list with
+// one atomic per workgroup.
+constexpr int SETUP_LQ = 512;
+
+struct SetupShared {
+    BinTable tab;
+    u32 lq[SETUP_LQ];
+    u32 nlq, lbase;
+};
+
+__device__ __forceinline__ void bin_or_defer(const RastArgs& a, SetupShared& sh, const RRec& r, u32 slot) {
+    if (piece_small(r)) {
+        u32 g[SMALL_TILES];
+        int n = small_tiles(a, r, g);
+        for (int i = 0; i < n; i++) bin_count(a, sh.tab, g[i]);
     } else {
-        u32 i = atomicAdd(a.ws.misc + 3, 1u);
-        a.ws.large[i] = slot;
+        u32 i = atomicAdd(&sh.nlq, 1u);
+        if (i < SETUP_LQ) sh.lq[i] = slot;
+        else a.ws.large[atomicAdd(a.ws.misc + 3, 1u)] = slot;
     }
 }
 
@@ -298,8 +364,7 @@ __global__ void __launch_bounds__(256) k_clear(u32* counts, u32* done, int NT, u
     if (i < 64) misc[i] = 0;
 }
 
-__global__ void __launch_bounds__(256) k_setup(RastArgs a) {
-    size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void setup_triangle(const RastArgs& a, SetupShared& sh, size_t gid) {
     if (gid >= a.nrec_direct) return;
     u32 view = (u32)(gid / a.T);
     u32 t = (u32)(gid % a.T);
@@ -311,7 +376,7 @@ __global__ void __launch_bounds__(256) k_setup(RastArgs a) {
         if (setup_piece(v0, v1, v2, t, view, a.W, a.H, r)) {
             depth_planes(v0, v1, v2, r.zp);
             a.ws.rec[gid] = r;
-            bin_or_defer(a, r, (u32)gid);
+            bin_or_defer(a, sh, r, (u32)gid);
         } else {
             a.ws.rec[gid].id = INVALID;
         }
@@ -337,8 +402,25 @@ __global__ void __launch_bounds__(256) k_setup(RastArgs a) {
         if (s >= a.ovf_cap) { atomicOr(a.ws.misc + 2, 1u); continue; }
         u32 slot = (u32)(a.nrec_direct + s);
         a.ws.rec[slot] = r;
-        bin_or_defer(a, r, slot);
+        bin_or_defer(a, sh, r, slot);
     }
+}
+
+// One thread per (layer, triangle): guard-band test, clipping when needed, snapping and edge
+// setup, one record per piece; small pieces are counted into their tiles' bins, large pieces
+// are listed for the wave-per-piece pass.
+__global__ void __launch_bounds__(256) k_setup(RastArgs a) {
+    __shared__ SetupShared sh;
+    bin_table_clear(sh.tab);
+    if (threadIdx.x == 0) sh.nlq = 0;
+    __syncthreads();
+    setup_triangle(a, sh, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    __syncthreads();
+    bin_count_flush(a, sh.tab.key, sh.tab.val, BIN_SLOTS);
+    u32 nl = min(sh.nlq, (u32)SETUP_LQ);
+    if (threadIdx.x == 0 && nl) sh.lbase = atomicAdd(a.ws.misc + 3, nl);
+    __syncthreads();
+    for (u32 i = threadIdx.x; i < nl; i += blockDim.x) a.ws.large[sh.lbase + i] = sh.lq[i];
 }
 
 // Exclusive prefix of one value per thread over a 1024-lane workgroup, continuing from *carry
@@ -405,16 +487,38 @@ __global__ void __launch_bounds__(1024) k_scan(RastArgs a, int NT) {
     }
 }
 
+// Small pieces into their tiles' bins: entries counted into LDS (local position from the LDS
+// atomic), each tile's run reserved with one global atomic, then written.
 __global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
+    __shared__ BinTable tab;
+    bin_table_clear(tab);
+    __syncthreads();
     size_t slot = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (slot >= nslots) return;
-    if (slot >= a.nrec_direct) {
+    u32 g[SMALL_TILES], loc[SMALL_TILES];
+    int sl[SMALL_TILES];
+    int n = 0;
+    bool ok = slot < nslots;
+    if (ok && slot >= a.nrec_direct) {
         u32 novf = min(a.ws.misc[0], (u32)a.ovf_cap);
-        if (slot - a.nrec_direct >= novf) return;
+        ok = slot - a.nrec_direct < novf;
     }
-    RRec r = a.ws.rec[slot];
-    if (r.id == INVALID || box_tiles(r) > SMALL_TILES) return;
-    bin_piece<true>(a, r, (u32)slot);
+    if (ok) {
+        const RRec& r = a.ws.rec[slot];
+        if (r.id != INVALID) {
+            RRec rr = r;
+            if (piece_small(rr)) n = small_tiles(a, rr, g);
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        sl[i] = bin_slot(tab, g[i]);
+        if (sl[i] >= 0) loc[i] = atomicAdd(&tab.val[sl[i]], 1u);
+        else bin_fill_direct(a, g[i], (u32)slot);
+    }
+    __syncthreads();
+    bin_fill_reserve(a, tab.key, tab.val, BIN_SLOTS);
+    __syncthreads();
+    for (int i = 0; i < n; i++)
+        if (sl[i] >= 0) bin_write(a, g[i], tab.val[sl[i]] + loc[i], (u32)slot);
 }
 
 // Covered pixel columns of one piece in one row of a tile.  Edge i passes at column x (pixel
