@@ -417,6 +417,11 @@ int vcl_fit_destroy(vcl_fit* f);
 int vcl_fit_set_target_from_render(vcl_fit* f, vcl_stream_t stream);
 /* One fwd+bwd+Adam iteration on the stream. */
 int vcl_fit_step(vcl_fit* f, vcl_stream_t stream);
+/* The same iteration replayed from a HIP graph (captured on the first call, on a stream of
+ * the fit's own; ordered after prior work on `stream`, and later work on `stream` waits for
+ * it).  Equivalent to vcl_fit_step; replaces main.cpp's per-op launch loop (main.cpp:39-49)
+ * with one launch per iteration.  No counterpart in the reference API. */
+int vcl_fit_step_graph(vcl_fit* f, vcl_stream_t stream);
 /* Device pointers of named buffers ("pos", "texture", "axis", "sharpness", "amplitude",
  * "image", "loss", "clip", "rast", "db", "grad_pos", ...); NULL if unknown. */
 void* vcl_fit_buffer(vcl_fit* f, const char* name);

@@ -161,6 +161,7 @@ SIGNATURES = {
     "vcl_fit_destroy": ([vp], ci),
     "vcl_fit_set_target_from_render": ([vp, vp], ci),
     "vcl_fit_step": ([vp, vp], ci),
+    "vcl_fit_step_graph": ([vp, vp], ci),
     "vcl_fit_buffer": ([vp, C.c_char_p], vp),
     "vcl_fit_device_bytes": ([vp], csz),
     "vcl_fit_read": ([vp, C.c_char_p, vp, csz, vp], ci),

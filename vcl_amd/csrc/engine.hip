@@ -2,15 +2,11 @@
 // preset_primitives.cpp:160-207, generalised to vertex positions + mip texture + SG lobes).
 // vcl_fit_create allocates every buffer once; vcl_fit_step only launches kernels (no host
 // sync, no allocation), so a step can be captured in a HIP graph and replayed.
-#include "common.hpp"
+#include "internal.hpp"
 #include <cstring>
 #include <string>
 #include <vector>
 #include <map>
-
-extern "C" int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream);
-extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo,
-                                             float hi, int quarter, int part, vcl_stream_t stream);
 
 // Per-op HIP-event timing (enabled between vcl_fit_profile_begin/end).
 struct OpTimer {
@@ -32,9 +28,13 @@ struct vcl_fit {
     std::map<std::string, void*> buf;
     std::map<std::string, size_t> bytes;
     size_t total = 0;
-    double b1t, b2t;    // Adam bias-correction powers (rhomt, rhovt of AdamHyperParams)
-    int it = 0;
+    int it = 0;         // steps taken (eager or graph)
     size_t ws_rast = 0, ws_sg = 0, ws_loss = 0;
+    // HIP graph of one step (vcl_fit_step_graph): captured once on a private stream
+    hipStream_t gstream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
 
     float* f(const char* n) { return (float*)buf.at(n); }
     template <class T> T* t(const char* n) { return (T*)buf.at(n); }
@@ -131,6 +131,12 @@ extern "C" int vcl_fit_profile_end(vcl_fit* F, int max_ops, int* nops, char* nam
 extern "C" int vcl_fit_destroy(vcl_fit* F) {
     if (!F) return VCL_OK;
     prof_release(F->prof);
+    if (F->gstream) (void)hipStreamSynchronize(F->gstream);
+    if (F->gexec) (void)hipGraphExecDestroy(F->gexec);
+    if (F->graph) (void)hipGraphDestroy(F->graph);
+    if (F->ev_in) (void)hipEventDestroy(F->ev_in);
+    if (F->ev_out) (void)hipEventDestroy(F->ev_out);
+    if (F->gstream) (void)hipStreamDestroy(F->gstream);
     for (auto& kv : F->buf) (void)hipFree(kv.second);
     delete F;
     return VCL_OK;
@@ -207,8 +213,14 @@ extern "C" int vcl_fit_create(const vcl_fit_desc* d, vcl_fit** out) {
 #undef A
 #undef U
     if (st) { vcl_fit_destroy(F); return st; }
-    F->b1t = d->beta1;
-    F->b2t = d->beta2;
+    // Adam bias-correction powers (rhomt, rhovt of AdamHyperParams) live on the device so that a
+    // captured step advances them on every replay
+    if (!st) st = alloc(F, "adam_bt", 2 * sizeof(double));
+    if (!st) {
+        double bt[2] = {d->beta1, d->beta2};
+        st = hipMemcpy(F->buf["adam_bt"], bt, sizeof(bt), hipMemcpyHostToDevice) == hipSuccess ? VCL_OK : VCL_ERR_HIP;
+    }
+    if (st) { vcl_fit_destroy(F); return st; }
     *out = F;
     return VCL_OK;
 }
@@ -309,8 +321,7 @@ extern "C" int vcl_fit_set_target_from_render(vcl_fit* F, vcl_stream_t s) {
     return VCL_OK;
 }
 
-extern "C" int vcl_fit_step(vcl_fit* F, vcl_stream_t s) {
-    VCL_REQUIRE(F);
+static int step_body(vcl_fit* F, vcl_stream_t s) {
     const vcl_fit_desc& d = F->d;
     int W = d.width, H = d.height, nv = d.views, C = d.channel, K = d.sgnum;
     hipStream_t hs = as_hip(s);
@@ -366,16 +377,15 @@ extern "C" int vcl_fit_step(vcl_fit* F, vcl_stream_t s) {
     { ProfScope ps_(F, "project_bwd", as_hip(s)); st = vcl_project_backward(&pj, &pg, s); }
     if (st) return st;
     // update (Adam::step bumps rhomt/rhovt before the launch, optimize.cu:109-115)
-    F->it++;
-    F->b1t *= d.beta1;
-    F->b2t *= d.beta2;
-    vcl_adam_hyper_params ht = {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_tex"), F->f("v_tex")};
+    double* bt = F->t<double>("adam_bt");
+    if ((st = vcl_adam_bump_dev(bt, d.beta1, d.beta2, s))) return st;
+    vcl_adam_hyper_params ht = {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_tex"), F->f("v_tex")};
     vcl_texture tex, gtex;
     fill_texture(F, "texture", tex);
     fill_texture(F, "gtex", gtex);
-    { ProfScope ps_(F, "texture_adam", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 1, s); }
+    { ProfScope ps_(F, "texture_adam", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 1, bt, s); }
     if (st) return st;
-    { ProfScope ps_(F, "texture_mip", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 2, s); }
+    { ProfScope ps_(F, "texture_mip", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 2, bt, s); }
     if (st) return st;
     vcl_optimize_params op[4] = {
         {F->f("pos"), F->f("grad_pos"), d.V * 3, d.V, 3, 1},
@@ -383,11 +393,11 @@ extern "C" int vcl_fit_step(vcl_fit* F, vcl_stream_t s) {
         {F->f("sharpness"), F->f("grad_sharpness"), K, K, 1, 1},
         {F->f("amplitude"), F->f("grad_amplitude"), K * C, K, C, 1}};
     vcl_adam_hyper_params hp[4] = {
-        {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_pos"), F->f("v_pos")},
-        {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_axis"), F->f("v_axis")},
-        {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_sharp"), F->f("v_sharp")},
-        {d.beta1, d.beta2, F->b1t, F->b2t, d.lr, d.eps, F->f("m_amp"), F->f("v_amp")}};
-    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi(K > 0 ? 4 : 1, op, hp, s); }
+        {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_pos"), F->f("v_pos")},
+        {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_axis"), F->f("v_axis")},
+        {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_sharp"), F->f("v_sharp")},
+        {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_amp"), F->f("v_amp")}};
+    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi_dev(K > 0 ? 4 : 1, op, hp, bt, s); }
     if (st) return st;
     // parameter-grad clears (AttributeGrad::clear, SGBufferGrad::clear)
     if ((st = vcl_fill_async(F->buf["grad_pos"], F->bytes["grad_pos"], hs))) return st;
@@ -396,5 +406,42 @@ extern "C" int vcl_fit_step(vcl_fit* F, vcl_stream_t s) {
         if ((st = vcl_fill_async(F->buf["grad_sharpness"], F->bytes["grad_sharpness"], hs))) return st;
         if ((st = vcl_fill_async(F->buf["grad_amplitude"], F->bytes["grad_amplitude"], hs))) return st;
     }
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_step(vcl_fit* F, vcl_stream_t s) {
+    VCL_REQUIRE(F);
+    int st = step_body(F, s);
+    if (!st) F->it++;
+    return st;
+}
+
+// Capture one step (same launches as vcl_fit_step) into a HIP graph on a private stream the
+// first time, then replay it.  The caller's stream and the replay are ordered with events.
+extern "C" int vcl_fit_step_graph(vcl_fit* F, vcl_stream_t s) {
+    VCL_REQUIRE(F);
+    if (!F->gexec) {
+        VCL_HIP_CHECK(hipStreamCreateWithFlags(&F->gstream, hipStreamNonBlocking));
+        VCL_HIP_CHECK(hipEventCreateWithFlags(&F->ev_in, hipEventDisableTiming));
+        VCL_HIP_CHECK(hipEventCreateWithFlags(&F->ev_out, hipEventDisableTiming));
+        bool prof = F->prof.on;
+        F->prof.on = false;
+        VCL_HIP_CHECK(hipStreamBeginCapture(F->gstream, hipStreamCaptureModeThreadLocal));
+        int st = step_body(F, (vcl_stream_t)F->gstream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(F->gstream, &g);
+        F->prof.on = prof;
+        if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+        VCL_HIP_CHECK(e);
+        F->graph = g;
+        VCL_HIP_CHECK(hipGraphInstantiate(&F->gexec, F->graph, nullptr, nullptr, 0));
+    }
+    hipStream_t hs = as_hip(s);
+    VCL_HIP_CHECK(hipEventRecord(F->ev_in, hs));
+    VCL_HIP_CHECK(hipStreamWaitEvent(F->gstream, F->ev_in, 0));
+    VCL_HIP_CHECK(hipGraphLaunch(F->gexec, F->gstream));
+    VCL_HIP_CHECK(hipEventRecord(F->ev_out, F->gstream));
+    VCL_HIP_CHECK(hipStreamWaitEvent(hs, F->ev_out, 0));
+    F->it++;
     return VCL_OK;
 }

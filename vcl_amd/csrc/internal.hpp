@@ -1,0 +1,18 @@
+// internal.hpp -- entry points shared between the library's translation units (not part of
+// the public C ABI in include/vcl.h).
+#pragma once
+#include "common.hpp"
+
+extern "C" {
+// ViewAxis forward with the per-view inverse matrices already on the device (depth x 9 floats).
+int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream);
+// Fused texture update.  part bit 1: fold + Adam + clamp on level 0; bit 2: rebuild the mips and
+// clear the coarse grads.  bt_dev: device {rhomt, rhovt} used instead of h->rhomt/rhovt (null:
+// the host values).
+int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
+                                  int quarter, int part, const double* bt_dev, vcl_stream_t stream);
+// vcl_adam_step_multi with the bias-correction powers read from the device (bt_dev, may be null).
+int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, vcl_stream_t stream);
+// bt_dev[0] *= rhom; bt_dev[1] *= rhov (one thread; the step-start bump of Adam::step).
+int vcl_adam_bump_dev(double* bt_dev, double rhom, double rhov, vcl_stream_t stream);
+}

@@ -507,6 +507,7 @@ struct AdamArgs {
     long long size[ADAM_MAX];
     long long block_start[ADAM_MAX + 1];
     double rhom[ADAM_MAX], rhov[ADAM_MAX], rhomt[ADAM_MAX], rhovt[ADAM_MAX], eta[ADAM_MAX], eps[ADAM_MAX];
+    const double* bt;   // device {rhomt, rhovt} shared by every tensor (graph replays), or null
 };
 
 __device__ __forceinline__ void adam_elem(float& param, float g, float& m, float& v, double rhom, double rhov, double rhomt, double rhovt,
@@ -528,11 +529,12 @@ __global__ void __launch_bounds__(BS) k_adam(AdamArgs a) {
     long long base = (b - a.block_start[t]) * ADAM_PER_BLOCK;
     float* P = a.param[t]; const float* G = a.grad[t]; float* M = a.m[t]; float* Vv = a.v[t];
     long long n = a.size[t];
+    const double rmt = a.bt ? a.bt[0] : a.rhomt[t], rvt = a.bt ? a.bt[1] : a.rhovt[t];
     for (int k = 0; k < 4; k++) {
         long long i = base + k * BS + threadIdx.x;
         if (i >= n) break;
         float p = P[i], m = M[i], v = Vv[i];
-        adam_elem(p, G[i], m, v, a.rhom[t], a.rhov[t], a.rhomt[t], a.rhovt[t], a.eta[t], a.eps[t]);
+        adam_elem(p, G[i], m, v, a.rhom[t], a.rhov[t], rmt, rvt, a.eta[t], a.eps[t]);
         P[i] = p; M[i] = m; Vv[i] = v;
     }
 }
@@ -547,7 +549,14 @@ struct TexUpdArgs {
     float* v;
     double rhom, rhov, rhomt, rhovt, eta, eps;
     float lo, hi, scale;
+    const double* bt;   // device {rhomt, rhovt} (graph replays), or null
 };
+
+// Adam::step bumps rhomt/rhovt before the launch (optimize.cu:109-115); kept on the device so
+// a captured step advances them on every replay.
+__global__ void k_adam_bump(double* bt, double rhom, double rhov) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) { bt[0] *= rhom; bt[1] *= rhov; }
+}
 
 __global__ void __launch_bounds__(BS) k_tex_update(TexUpdArgs a) {
 #pragma clang fp contract(off)
@@ -570,7 +579,7 @@ __global__ void __launch_bounds__(BS) k_tex_update(TexUpdArgs a) {
     float g = a.grad[0][i];
     if (a.L > 1) add_nan_check(g, acc * a.scale);
     float p = a.tex0[i], m = a.m[i], v = a.v[i];
-    adam_elem(p, g, m, v, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
+    adam_elem(p, g, m, v, a.rhom, a.rhov, a.bt ? a.bt[0] : a.rhomt, a.bt ? a.bt[1] : a.rhovt, a.eta, a.eps);
     p = p < a.lo ? a.lo : p > a.hi ? a.hi : p;
     a.tex0[i] = p; a.m[i] = m; a.v[i] = v;
     a.grad[0][i] = 0.f;
@@ -605,10 +614,11 @@ __global__ void __launch_bounds__(BS) k_tex_update4(TexUpdArgs a) {
         }
     }
     if (a.L > 1) add_nan_check4(g, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
-    adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
-    adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
-    adam_elem(p.z, g.z, m.z, v.z, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
-    adam_elem(p.w, g.w, m.w, v.w, a.rhom, a.rhov, a.rhomt, a.rhovt, a.eta, a.eps);
+    const double rmt = a.bt ? a.bt[0] : a.rhomt, rvt = a.bt ? a.bt[1] : a.rhovt;
+    adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+    adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+    adam_elem(p.z, g.z, m.z, v.z, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+    adam_elem(p.w, g.w, m.w, v.w, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
     p.x = p.x < a.lo ? a.lo : p.x > a.hi ? a.hi : p.x;
     p.y = p.y < a.lo ? a.lo : p.y > a.hi ? a.hi : p.y;
     p.z = p.z < a.lo ? a.lo : p.z > a.hi ? a.hi : p.z;
@@ -1390,10 +1400,13 @@ extern "C" int vcl_texture_grad_clear(const vcl_texture* g, vcl_stream_t stream)
     return VCL_OK;
 }
 
-extern "C" int vcl_adam_step_multi(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream) {
+// bt_dev: device {rhomt, rhovt} used instead of the per-tensor host values (null: host values).
+extern "C" int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev,
+                                       vcl_stream_t stream) {
     VCL_REQUIRE(n >= 0 && (n == 0 || (p && h)));
     for (int base = 0; base < n; base += ADAM_MAX) {
         AdamArgs a;
+        a.bt = bt_dev;
         a.n = n - base < ADAM_MAX ? n - base : ADAM_MAX;
         long long blocks = 0;
         for (int i = 0; i < a.n; i++) {
@@ -1414,13 +1427,24 @@ extern "C" int vcl_adam_step_multi(int n, const vcl_optimize_params* p, const vc
     return VCL_OK;
 }
 
+extern "C" int vcl_adam_step_multi(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream) {
+    return vcl_adam_step_multi_dev(n, p, h, nullptr, stream);
+}
+
+extern "C" int vcl_adam_bump_dev(double* bt_dev, double rhom, double rhov, vcl_stream_t stream) {
+    VCL_REQUIRE(bt_dev);
+    hipLaunchKernelGGL(k_adam_bump, dim3(1), dim3(64), 0, as_hip(stream), bt_dev, rhom, rhov);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
 extern "C" int vcl_adam_step(const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream) {
     return vcl_adam_step_multi(1, p, h, stream);
 }
 
 // part bit 1: fold + Adam + clamp on level 0; part bit 2: rebuild mips (and clear coarse grads).
 extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo,
-                                             float hi, int quarter, int part, vcl_stream_t stream) {
+                                             float hi, int quarter, int part, const double* bt_dev, vcl_stream_t stream) {
     VCL_REQUIRE(tex && grad && h && h->m && h->v && tex->miplevel == grad->miplevel && tex->width == grad->width && tex->height == grad->height &&
                 tex->channel == grad->channel && tex->miplevel >= 1 && tex->miplevel <= VCL_TEX_MAX_MIP_LEVEL);
     TexUpdArgs a;
@@ -1431,6 +1455,7 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
     a.m = h->m; a.v = h->v;
     a.rhom = h->rhom; a.rhov = h->rhov; a.rhomt = h->rhomt; a.rhovt = h->rhovt; a.eta = h->eta; a.eps = h->eps;
     a.lo = lo; a.hi = hi; a.scale = quarter ? 0.25f : 1.f;
+    a.bt = bt_dev;
     size_t n = (size_t)a.tw * a.th * a.C;
     hipStream_t s = as_hip(stream);
     bool vec4 = a.C == 4;
@@ -1460,7 +1485,7 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
 
 extern "C" int vcl_texture_adam_update(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
                                        int quarter, vcl_stream_t stream) {
-    return vcl_texture_adam_update_parts(tex, grad, h, lo, hi, quarter, 3, stream);
+    return vcl_texture_adam_update_parts(tex, grad, h, lo, hi, quarter, 3, nullptr, stream);
 }
 
 extern "C" int vcl_normal_axis_forward(const vcl_normal_axis_params* p, vcl_stream_t stream) {

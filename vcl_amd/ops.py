@@ -336,8 +336,11 @@ class Fit:
     def set_target_from_render(self):
         L.check(_lib().vcl_fit_set_target_from_render(self.h, _s()), "fit_set_target")
 
-    def step(self):
-        L.check(_lib().vcl_fit_step(self.h, _s()), "fit_step")
+    def step(self, graph=False):
+        if graph:
+            L.check(_lib().vcl_fit_step_graph(self.h, _s()), "fit_step_graph")
+        else:
+            L.check(_lib().vcl_fit_step(self.h, _s()), "fit_step")
 
     def ptr(self, name):
         p = _lib().vcl_fit_buffer(self.h, name.encode())

@@ -337,3 +337,13 @@ def config2_scene(seed: int, W: int = 1024, S: int = 2048, K: int = 128) -> Scen
 
 def multiview_cameras(n: int, radius: float = 3.0):
     return [make_camera(tuple(p)) for p in fibonacci_sphere(n, radius)]
+
+
+def config3_scene(seed: int, W: int = 512, S: int = 2048, K: int = 64, views: int = 64) -> Scene:
+    """Batched multi-view request: the displaced-sphere mesh (49,922 V / 99,840 T) seen from
+    `views` cameras on a Fibonacci sphere of radius 3 (fov 45) at W x W each, one shared
+    S^2 x 4 texture (12 levels), K lobes."""
+    sc = config2_scene(seed, W=W, S=S, K=K)
+    sc.cams = multiview_cameras(views, 3.0)
+    return sc
+
