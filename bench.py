@@ -103,6 +103,78 @@ def run_batched(ops, S, torch, steps, warmup, views=64, W=512):
                         f"{W}x{W}, shared texture {sc.tw}^2x{sc.C} L={sc.L}, {sc.K} SG lobes; iterations 2..{steps + 1} of the request"}
 
 
+def _served_specs(SV, n, rate, seed):
+    specs = SV.poisson_requests(n, rate, seed=seed)
+    built = []
+    for sp in specs:
+        sc, tsc = SV.build(sp)
+        built.append((sp, sc, SV.render_target(tsc)))
+    return built
+
+
+def _serve_once(SV, built, rate_scale, max_active, use_graph, steps_per_round, budget):
+    """Run one arrival stream through a server; arrival times scaled by rate_scale (0: all at
+    t=0).  Returns latency / throughput / occupancy figures."""
+    srv = SV.Server(hbm_budget=budget, max_active=max_active, use_graph=use_graph, steps_per_round=steps_per_round,
+                    auto_release=True)
+    for sp, sc, tgt in built:
+        srv.submit(sc, tgt, sp["iterations"], sp["arrival_s"] * rate_scale)
+    srv.reset_clock()
+    srv.drain()
+    st = srv.finished()
+    mem = srv.memory()
+    srv.close()
+    assert len(st) == len(built) and all(x["status"] == 0 for x in st), st
+    lat = [x["finish_s"] - x["arrival_s"] for x in st]
+    t_end = max(x["finish_s"] for x in st)
+    px = sum(sp["iterations"] * sc.W * sc.H for sp, sc, _ in built)
+    return {"p50_ms": round(1e3 * SV.percentile(lat, 50), 2), "p99_ms": round(1e3 * SV.percentile(lat, 99), 2),
+            "makespan_s": round(t_end, 3), "req_per_s": round(len(st) / t_end, 3),
+            "eq_frames_per_s": round(px / (1 << 20) / t_end, 2),
+            "hbm_peak_gb": round(mem["peak"] / 1e9, 3), "hbm_mean_gb": round(mem["mean_in_use"] / 1e9, 3)}
+
+
+def run_served(torch, rates, n, seed=3, budget_gb=200, max_active=16):
+    """Served inverse rendering: Poisson arrivals at each rate, uneven request sizes, 20-200
+    iterations each, state resident in HBM, admission against the budget; continuous batching
+    (max_active requests overlapping on the device, graph-replayed steps) against sequential
+    plain execution (one request at a time, eager steps) of the same arrival stream."""
+    from vcl_amd import serve as SV
+    budget = int(budget_gb * 1e9)
+    base = _served_specs(SV, n, 1.0, seed)     # arrival times at 1 req/s, scaled per rate
+    # warm-up: one small request through both paths (kernel loading, graph machinery)
+    _serve_once(SV, base[:1], 0.0, 1, True, 2, budget)
+    out = {"requests": n, "budget_gb": budget_gb, "max_active": max_active,
+           "workload": "fit requests: displaced icospheres 1,280-327,680 T (log-uniform 1k-500k), 256^2-2048^2, "
+                       "textures 256^2-4096^2 x4 with full mips, 16/64/128 lobes, 20-200 Adam iterations, Poisson arrivals",
+           "rates": {}}
+    for rate in rates:
+        cont = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget)
+        seq = _serve_once(SV, base, 1.0 / rate, 1, False, 1, budget)
+        out["rates"][str(rate)] = {"continuous": cont, "sequential": seq,
+                                   "throughput_vs_sequential": round(cont["eq_frames_per_s"] / seq["eq_frames_per_s"], 3)}
+    return out
+
+
+def run_uneven(torch, n=32, iters=20, seed=5, max_active=32):
+    """Batched uneven scenes: n independent requests of uneven size, all admitted at once and
+    stepped concurrently (graph replays on per-request streams), against the same requests run
+    one after another with eager steps."""
+    from vcl_amd import serve as SV
+    specs = SV.uneven_scenes(n, seed)
+    built = []
+    for sp in specs:
+        sc, tsc = SV.build(sp)
+        sp = dict(sp, arrival_s=0.0, iterations=iters)
+        built.append((sp, sc, SV.render_target(tsc)))
+    _serve_once(SV, built[:1], 0.0, 1, True, 2, int(200e9))
+    cont = _serve_once(SV, built, 0.0, max_active, True, 2, int(200e9))
+    seq = _serve_once(SV, built, 0.0, 1, False, 1, int(200e9))
+    return {"scenes": n, "iterations": iters, "batched": cont, "sequential": seq,
+            "value": cont["eq_frames_per_s"], "unit": "1024^2-equivalent frames/s",
+            "vs_sequential": round(cont["eq_frames_per_s"] / seq["eq_frames_per_s"], 3)}
+
+
 def cpu_baseline(sc, target, steps, threads):
     """Time the CPU oracle's float fit step (a C++ restatement of the reference semantics)."""
     import ctypes as C
@@ -156,6 +228,9 @@ def main():
                     help="for rocprofv3 --pmc runs: one 256 MiB device copy (counter calibration) then --probe-steps fit steps")
     ap.add_argument("--probe-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view measurement")
+    ap.add_argument("--served", default="50", help="comma-separated arrival rates (req/s) for the served measurement; '' skips it")
+    ap.add_argument("--served-requests", type=int, default=24)
+    ap.add_argument("--uneven", type=int, default=0, help="run the uneven-scene batch with this many scenes (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-views", type=int, default=64)
     args = ap.parse_args()
@@ -253,6 +328,10 @@ def main():
         fps, dt = cpu_baseline(sc, target, args.cpu_steps, threads)
         cpu = {"value": round(fps, 4), "unit": "frames/s", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} fit steps of the same workload ({sc.W}x{sc.H}, {sc.T} tris, K={sc.K}) on the oracle, {dt:.1f} s"}
+    served = None
+    if args.served:
+        served = run_served(torch, [float(r) for r in args.served.split(",")], args.served_requests)
+    uneven = run_uneven(torch, args.uneven) if args.uneven else None
     batched = None
     if not args.no_batched:
         batched = run_batched(ops, S, torch, args.batched_steps, 3, views=args.batched_views)
@@ -268,6 +347,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "served": served,
+        "uneven": uneven,
         "wall_s": round(wall, 3),
         "loss": loss,
         "kernels": kernels,

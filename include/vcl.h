@@ -412,6 +412,13 @@ typedef struct vcl_fit_desc {
 typedef struct vcl_fit vcl_fit;       /* opaque */
 
 int vcl_fit_create(const vcl_fit_desc* d, vcl_fit** out);
+/* Device bytes of one request's arena (every buffer of vcl_fit_create_in); 0 if d is invalid. */
+size_t vcl_fit_arena_bytes(const vcl_fit_desc* d);
+/* vcl_fit_create inside caller-provided device memory (at least vcl_fit_arena_bytes(d) bytes;
+ * it stays the caller's: vcl_fit_destroy does not free it).  Zeroes the arena and uploads the
+ * host arrays on `stream`, then synchronises it.  No counterpart in the reference (whose init
+ * calls cudaMalloc per buffer and never frees); it is what a serving pool needs. */
+int vcl_fit_create_in(const vcl_fit_desc* d, void* arena, size_t arena_bytes, vcl_stream_t stream, vcl_fit** out);
 int vcl_fit_destroy(vcl_fit* f);
 /* Render the current parameters and store the image as the fit target. */
 int vcl_fit_set_target_from_render(vcl_fit* f, vcl_stream_t stream);
@@ -435,6 +442,55 @@ int vcl_fit_write(vcl_fit* f, const char* name, const void* host, size_t bytes, 
  * (names[i*name_len ...]), summed milliseconds and launch count. */
 int vcl_fit_profile_begin(vcl_fit* f, int max_launches, const char* only);
 int vcl_fit_profile_end(vcl_fit* f, int max_ops, int* nops, char* names, int name_len, double* total_ms, int* count);
+
+/* ---------------------------------------------------------------- Serving
+ * Continuous batching of fit requests on one GPU (no counterpart in the reference, which runs
+ * one preset in a GLUT loop, main.cpp:39-49).  Requests queue with an arrival time on the
+ * server clock; each scheduling round retires completed steps, finishes requests that ran all
+ * their iterations, admits arrived requests in FIFO order while a slot is free and the
+ * request's arena fits the HBM budget, and queues the next steps of every active request on
+ * its own stream, so the steps of different requests overlap on the device.  State stays
+ * resident in the request's arena until it is released. */
+typedef struct vcl_server vcl_server;  /* opaque */
+
+typedef struct vcl_server_config {
+    size_t hbm_budget;    /* bytes of request arenas the server may hold */
+    int max_active;       /* requests admitted and stepping at once */
+    int use_graph;        /* 1: replay each request's step from its HIP graph */
+    int steps_per_round;  /* steps queued per request per scheduling round (>= 1) */
+    int auto_release;     /* 1: free a request's arena as soon as it finishes */
+} vcl_server_config;
+
+typedef struct vcl_request_stats {
+    uint64_t ticket;
+    double arrival_s, admit_s, finish_s;   /* server clock, seconds */
+    int iterations;                        /* steps completed */
+    float final_loss;
+    size_t device_bytes;                   /* arena size */
+    int status;                            /* VCL_OK or the error that ended it */
+} vcl_request_stats;
+
+int vcl_server_create(const vcl_server_config* c, vcl_server** out);
+int vcl_server_destroy(vcl_server* s);
+/* Restart the server clock at 0 (arrival times are on this clock). */
+int vcl_server_reset_clock(vcl_server* s);
+double vcl_server_clock(const vcl_server* s);
+/* Queue a request of `iterations` fit steps arriving at `arrival_s`.  The host arrays of `d`
+ * must stay valid until the request is admitted.  VCL_ERR_OOM if its arena exceeds the budget. */
+int vcl_server_submit(vcl_server* s, const vcl_fit_desc* d, int iterations, double arrival_s, uint64_t* ticket);
+/* One non-blocking scheduling round; *pending = requests queued or active. */
+int vcl_server_progress(vcl_server* s, int* pending);
+/* Scheduling rounds until every submitted request finished. */
+int vcl_server_drain(vcl_server* s);
+/* Stats of up to `max` finished requests not reported before. */
+int vcl_server_finished(vcl_server* s, vcl_request_stats* out, int max, int* n);
+/* Copy a named buffer (vcl_fit_buffer names) of a finished, unreleased request to the host. */
+int vcl_server_read(vcl_server* s, uint64_t ticket, const char* name, void* host, size_t bytes);
+/* Return a finished request's arena to the pool. */
+int vcl_server_release(vcl_server* s, uint64_t ticket);
+/* Arena bytes held by admitted requests now, at peak, cached for reuse, and the time-average
+ * since the clock was reset. */
+int vcl_server_memory(vcl_server* s, size_t* in_use, size_t* peak, size_t* cached, double* mean_in_use);
 
 #ifdef __cplusplus
 }

@@ -122,6 +122,15 @@ class FitDesc(C.Structure):
                 ("fold_quarter", ci), ("fused", ci)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", csz), ("max_active", ci), ("use_graph", ci), ("steps_per_round", ci), ("auto_release", ci)]
+
+
+class RequestStats(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("arrival_s", cd), ("admit_s", cd), ("finish_s", cd), ("iterations", ci),
+                ("final_loss", cf), ("device_bytes", csz), ("status", ci)]
+
+
 # Every entry point declared in include/vcl.h: name -> (argtypes, restype)
 P_ = C.POINTER
 SIGNATURES = {
@@ -158,6 +167,19 @@ SIGNATURES = {
     "vcl_loss_workspace_size": ([ci], csz),
     "vcl_mse_loss_backward": ([P_(LossParams), vp], ci),
     "vcl_fit_create": ([P_(FitDesc), P_(vp)], ci),
+    "vcl_fit_arena_bytes": ([P_(FitDesc)], csz),
+    "vcl_fit_create_in": ([P_(FitDesc), vp, csz, vp, P_(vp)], ci),
+    "vcl_server_create": ([P_(ServerConfig), P_(vp)], ci),
+    "vcl_server_destroy": ([vp], ci),
+    "vcl_server_reset_clock": ([vp], ci),
+    "vcl_server_clock": ([vp], cd),
+    "vcl_server_submit": ([vp, P_(FitDesc), ci, cd, P_(C.c_uint64)], ci),
+    "vcl_server_progress": ([vp, P_(ci)], ci),
+    "vcl_server_drain": ([vp], ci),
+    "vcl_server_finished": ([vp, P_(RequestStats), ci, P_(ci)], ci),
+    "vcl_server_read": ([vp, C.c_uint64, C.c_char_p, vp, csz], ci),
+    "vcl_server_release": ([vp, C.c_uint64], ci),
+    "vcl_server_memory": ([vp, P_(csz), P_(csz), P_(csz), P_(cd)], ci),
     "vcl_fit_destroy": ([vp], ci),
     "vcl_fit_set_target_from_render": ([vp, vp], ci),
     "vcl_fit_step": ([vp, vp], ci),

@@ -28,6 +28,8 @@ struct vcl_fit {
     std::map<std::string, void*> buf;
     std::map<std::string, size_t> bytes;
     size_t total = 0;
+    void* arena = nullptr;   // every buffer of the request
+    bool own_arena = false;  // allocated by vcl_fit_create (else the caller's memory)
     int it = 0;         // steps taken (eager or graph)
     size_t ws_rast = 0, ws_sg = 0, ws_loss = 0;
     // HIP graph of one step (vcl_fit_step_graph): captured once on a private stream
@@ -40,22 +42,57 @@ struct vcl_fit {
     template <class T> T* t(const char* n) { return (T*)buf.at(n); }
 };
 
-static int alloc(vcl_fit* F, const char* name, size_t nbytes) {
-    void* p = nullptr;
-    size_t nb = nbytes ? nbytes : 4;
-    if (hipMalloc(&p, nb) != hipSuccess) return VCL_ERR_OOM;
-    if (hipMemset(p, 0, nb) != hipSuccess) return VCL_ERR_HIP;
-    F->buf[name] = p;
-    F->bytes[name] = nb;
-    F->total += nb;
-    return VCL_OK;
-}
+// Buffers live in one device arena (one allocation per request, or caller-provided memory
+// from a pool): plan() lays them out, 256-byte aligned.
+struct Plan {
+    std::vector<std::pair<std::string, size_t>> items;   // name, bytes
+    size_t total = 0;
+    void add(const char* n, size_t bytes) {
+        size_t nb = bytes ? bytes : 4;
+        items.push_back({n, nb});
+        total += (nb + 255) & ~(size_t)255;
+    }
+};
 
-static int upload(vcl_fit* F, const char* name, const void* host, size_t nbytes) {
-    int st = alloc(F, name, nbytes);
-    if (st) return st;
-    if (host && nbytes) VCL_HIP_CHECK(hipMemcpy(F->buf[name], host, nbytes, hipMemcpyHostToDevice));
-    return VCL_OK;
+static Plan plan(const vcl_fit_desc* d) {
+    Plan L;
+    size_t P = (size_t)d->width * d->height * d->views, f4 = sizeof(float);
+    int C = d->channel, K = d->sgnum, V = d->V, Tn = d->T, nv = d->views;
+    size_t ntex = 0;
+    for (int l = 0; l < d->miplevel; l++) ntex += (size_t)(d->texwidth >> l) * (d->texheight >> l) * C;
+    L.add("pos", (size_t)V * 3 * f4); L.add("normals", (size_t)V * 3 * f4); L.add("uv", (size_t)V * 2 * f4);
+    L.add("idx", (size_t)Tn * 3 * 4); L.add("texture", ntex * f4);
+    L.add("axis", (size_t)K * 3 * f4); L.add("sharpness", (size_t)K * f4); L.add("amplitude", (size_t)K * C * f4);
+    L.add("mats", (size_t)nv * 16 * f4); L.add("pvinv", (size_t)nv * 9 * f4); L.add("rot", 16 * f4);
+    L.add("target", P * C * f4); L.add("rough", P * f4);
+    // Adam state
+    L.add("m_pos", (size_t)V * 3 * f4); L.add("v_pos", (size_t)V * 3 * f4);
+    L.add("m_tex", (size_t)d->texwidth * d->texheight * C * f4); L.add("v_tex", (size_t)d->texwidth * d->texheight * C * f4);
+    L.add("m_axis", (size_t)K * 3 * f4); L.add("v_axis", (size_t)K * 3 * f4);
+    L.add("m_sharp", (size_t)K * f4); L.add("v_sharp", (size_t)K * f4);
+    L.add("m_amp", (size_t)K * C * f4); L.add("v_amp", (size_t)K * C * f4);
+    L.add("adam_bt", 2 * sizeof(double));
+    // intermediates
+    L.add("clip", (size_t)nv * V * 4 * f4);
+    L.add("rast", P * 4 * f4); L.add("db", P * 4 * f4);
+    L.add("uvp", P * 2 * f4); L.add("uvDA", P * 4 * f4);
+    L.add("nrm", P * 3 * f4); L.add("view", P * 3 * f4);
+    L.add("diff", P * C * f4);
+    L.add("sAxis", P * 3 * f4); L.add("sSharp", P * f4); L.add("sAmp", P * f4);
+    L.add("color", P * C * f4); L.add("denv", P * C * f4); L.add("senv", P * C * f4);
+    L.add("image", P * C * f4); L.add("gimage", P * C * f4); L.add("gcolor", P * C * f4);
+    L.add("gdiff", P * C * f4);
+    L.add("gtex", ntex * f4);
+    L.add("guv", P * 2 * f4); L.add("guvDA", P * 4 * f4);
+    L.add("grast", P * 4 * f4);
+    L.add("gclip", (size_t)nv * V * 4 * f4);
+    L.add("grad_pos", (size_t)V * 3 * f4);
+    L.add("grad_axis", (size_t)K * 3 * f4); L.add("grad_sharpness", (size_t)K * f4); L.add("grad_amplitude", (size_t)K * C * f4);
+    L.add("loss", 4 * f4);
+    L.add("ws_rast", vcl_rasterize_workspace_size(d->width, d->height, nv, V, Tn));
+    L.add("ws_sg", vcl_spherical_gaussian_workspace_size(d->width, d->height, nv, C, K));
+    L.add("ws_loss", vcl_loss_workspace_size((int)(P * C)));
+    return L;
 }
 
 static void prof_release(OpTimer& T) {
@@ -137,90 +174,86 @@ extern "C" int vcl_fit_destroy(vcl_fit* F) {
     if (F->ev_in) (void)hipEventDestroy(F->ev_in);
     if (F->ev_out) (void)hipEventDestroy(F->ev_out);
     if (F->gstream) (void)hipStreamDestroy(F->gstream);
-    for (auto& kv : F->buf) (void)hipFree(kv.second);
+    if (F->own_arena && F->arena) (void)hipFree(F->arena);
     delete F;
     return VCL_OK;
 }
 
-extern "C" int vcl_fit_create(const vcl_fit_desc* d, vcl_fit** out) {
-    VCL_REQUIRE(d && out && d->width > 0 && d->height > 0 && d->views >= 1 && d->V > 0 && d->T > 0);
+static bool desc_ok(const vcl_fit_desc* d) {
+    return d && d->width > 0 && d->height > 0 && d->views >= 1 && d->V > 0 && d->T > 0 && d->channel >= 1 && d->channel <= 4 &&
+           d->sgnum >= 0 && d->miplevel >= 1 && d->miplevel <= VCL_TEX_MAX_MIP_LEVEL &&
+           d->miplevel == vcl_texture_mip_levels(d->texwidth, d->texheight, d->miplevel);
+}
+
+extern "C" size_t vcl_fit_arena_bytes(const vcl_fit_desc* d) { return desc_ok(d) ? plan(d).total : 0; }
+
+extern "C" int vcl_fit_create_in(const vcl_fit_desc* d, void* arena, size_t arena_bytes, vcl_stream_t stream, vcl_fit** out) {
+    VCL_REQUIRE(out && desc_ok(d));
     VCL_REQUIRE(d->pos && d->normals && d->uv && d->idx && d->texture && d->mats && d->pvinv && d->rot);
-    VCL_REQUIRE(d->channel >= 1 && d->channel <= 4 && d->sgnum >= 0 && d->miplevel >= 1 && d->miplevel <= VCL_TEX_MAX_MIP_LEVEL);
-    VCL_REQUIRE(d->miplevel == vcl_texture_mip_levels(d->texwidth, d->texheight, d->miplevel));
+    Plan L = plan(d);
+    VCL_REQUIRE(arena && arena_bytes >= L.total);
+    hipStream_t hs = as_hip(stream);
     vcl_fit* F = new vcl_fit();
     F->d = *d;
     F->d.target = nullptr;
+    F->arena = arena;
     F->P1 = (size_t)d->width * d->height;
     F->P = F->P1 * d->views;
-    int C = d->channel, K = d->sgnum, V = d->V, Tn = d->T, nv = d->views;
+    const int C = d->channel;
     size_t off = 0;
     for (int l = 0; l < d->miplevel; l++) {
         F->lvl_off.push_back(off);
         off += (size_t)(d->texwidth >> l) * (d->texheight >> l) * C;
     }
     F->ntex = off;
-    size_t P = F->P, f4 = sizeof(float);
-    int st = VCL_OK;
-#define A(name, n) if (!st) st = alloc(F, name, (n))
-#define U(name, host, n) if (!st) st = upload(F, name, host, (n))
-    U("pos", d->pos, (size_t)V * 3 * f4);
-    U("normals", d->normals, (size_t)V * 3 * f4);
-    U("uv", d->uv, (size_t)V * 2 * f4);
-    U("idx", d->idx, (size_t)Tn * 3 * 4);
-    U("texture", d->texture, F->ntex * f4);
-    U("axis", d->axis, (size_t)K * 3 * f4);
-    U("sharpness", d->sharpness, (size_t)K * f4);
-    U("amplitude", d->amplitude, (size_t)K * C * f4);
-    U("mats", d->mats, (size_t)nv * 16 * f4);
-    U("pvinv", d->pvinv, (size_t)nv * 9 * f4);
-    U("rot", d->rot, 16 * f4);
-    A("target", P * C * f4);
-    if (!st && d->target) st = hipMemcpy(F->buf["target"], d->target, P * C * f4, hipMemcpyHostToDevice) == hipSuccess ? VCL_OK : VCL_ERR_HIP;
-    A("rough", P * f4);
-    if (!st) {
-        std::vector<float> r(P, d->roughness);
-        st = hipMemcpy(F->buf["rough"], r.data(), P * f4, hipMemcpyHostToDevice) == hipSuccess ? VCL_OK : VCL_ERR_HIP;
+    char* base = (char*)arena;
+    size_t o = 0;
+    for (auto& it : L.items) {
+        F->buf[it.first] = base + o;
+        F->bytes[it.first] = it.second;
+        o += (it.second + 255) & ~(size_t)255;
     }
-    // Adam state
-    A("m_pos", (size_t)V * 3 * f4); A("v_pos", (size_t)V * 3 * f4);
-    A("m_tex", (size_t)d->texwidth * d->texheight * C * f4); A("v_tex", (size_t)d->texwidth * d->texheight * C * f4);
-    A("m_axis", (size_t)K * 3 * f4); A("v_axis", (size_t)K * 3 * f4);
-    A("m_sharp", (size_t)K * f4); A("v_sharp", (size_t)K * f4);
-    A("m_amp", (size_t)K * C * f4); A("v_amp", (size_t)K * C * f4);
-    // intermediates
-    A("clip", (size_t)nv * V * 4 * f4);
-    A("rast", P * 4 * f4); A("db", P * 4 * f4);
-    A("uvp", P * 2 * f4); A("uvDA", P * 4 * f4);
-    A("nrm", P * 3 * f4); A("view", P * 3 * f4);
-    A("diff", P * C * f4);
-    A("sAxis", P * 3 * f4); A("sSharp", P * f4); A("sAmp", P * f4);
-    A("color", P * C * f4); A("denv", P * C * f4); A("senv", P * C * f4);
-    A("image", P * C * f4); A("gimage", P * C * f4); A("gcolor", P * C * f4);
-    A("gdiff", P * C * f4);
-    A("gtex", F->ntex * f4);
-    A("guv", P * 2 * f4); A("guvDA", P * 4 * f4);
-    A("grast", P * 4 * f4);
-    A("gclip", (size_t)nv * V * 4 * f4);
-    A("grad_pos", (size_t)V * 3 * f4);
-    A("grad_axis", (size_t)K * 3 * f4); A("grad_sharpness", (size_t)K * f4); A("grad_amplitude", (size_t)K * C * f4);
-    A("loss", 4 * f4);
-    F->ws_rast = vcl_rasterize_workspace_size(d->width, d->height, nv, V, Tn);
-    F->ws_sg = vcl_spherical_gaussian_workspace_size(d->width, d->height, nv, C, K);
-    F->ws_loss = vcl_loss_workspace_size((int)(P * C));
-    A("ws_rast", F->ws_rast);
-    A("ws_sg", F->ws_sg);
-    A("ws_loss", F->ws_loss);
-#undef A
-#undef U
-    if (st) { vcl_fit_destroy(F); return st; }
+    F->total = L.total;
+    F->ws_rast = F->bytes["ws_rast"]; F->ws_sg = F->bytes["ws_sg"]; F->ws_loss = F->bytes["ws_loss"];
+    size_t P = F->P, f4 = sizeof(float);
+    int st = vcl_fill_async(arena, L.total, hs);
+    auto up = [&](const char* name, const void* host, size_t nbytes) {
+        if (!st && host && nbytes && hipMemcpyAsync(F->buf[name], host, nbytes, hipMemcpyHostToDevice, hs) != hipSuccess) st = VCL_ERR_HIP;
+    };
+    up("pos", d->pos, (size_t)d->V * 3 * f4);
+    up("normals", d->normals, (size_t)d->V * 3 * f4);
+    up("uv", d->uv, (size_t)d->V * 2 * f4);
+    up("idx", d->idx, (size_t)d->T * 3 * 4);
+    up("texture", d->texture, F->ntex * f4);
+    up("axis", d->axis, (size_t)d->sgnum * 3 * f4);
+    up("sharpness", d->sharpness, (size_t)d->sgnum * f4);
+    up("amplitude", d->amplitude, (size_t)d->sgnum * C * f4);
+    up("mats", d->mats, (size_t)d->views * 16 * f4);
+    up("pvinv", d->pvinv, (size_t)d->views * 9 * f4);
+    up("rot", d->rot, 16 * f4);
+    up("target", d->target, P * C * f4);
+    std::vector<float> r(P, d->roughness);
+    up("rough", r.data(), P * f4);
     // Adam bias-correction powers (rhomt, rhovt of AdamHyperParams) live on the device so that a
     // captured step advances them on every replay
-    if (!st) st = alloc(F, "adam_bt", 2 * sizeof(double));
-    if (!st) {
-        double bt[2] = {d->beta1, d->beta2};
-        st = hipMemcpy(F->buf["adam_bt"], bt, sizeof(bt), hipMemcpyHostToDevice) == hipSuccess ? VCL_OK : VCL_ERR_HIP;
-    }
+    double bt[2] = {d->beta1, d->beta2};
+    up("adam_bt", bt, sizeof(bt));
+    // the host staging (r, bt and the caller's arrays) must outlive the copies
+    if (!st && hipStreamSynchronize(hs) != hipSuccess) st = VCL_ERR_HIP;
     if (st) { vcl_fit_destroy(F); return st; }
+    *out = F;
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_create(const vcl_fit_desc* d, vcl_fit** out) {
+    VCL_REQUIRE(out && desc_ok(d));
+    size_t nb = plan(d).total;
+    void* arena = nullptr;
+    if (hipMalloc(&arena, nb) != hipSuccess) return VCL_ERR_OOM;
+    vcl_fit* F = nullptr;
+    int st = vcl_fit_create_in(d, arena, nb, nullptr, &F);
+    if (st) { (void)hipFree(arena); return st; }
+    F->own_arena = true;
     *out = F;
     return VCL_OK;
 }

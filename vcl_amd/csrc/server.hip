@@ -1,0 +1,298 @@
+// server.hip -- serving loop for fit requests (continuous batching on one GPU).
+//
+// Requests are queued with an arrival time on the server clock.  A scheduling round
+// (vcl_server_progress) retires completed step batches, finishes requests that ran all their
+// iterations, admits arrived requests in FIFO order while a slot is free and the request's
+// arena fits the HBM budget, and queues the next steps of every active request on that
+// request's own stream, so the steps of different requests overlap on the device.  Every
+// request's state (mesh, mip texture, lobes, Adam moments, intermediates) stays resident in one
+// arena until the request is released; arenas are recycled through a pool so that admission
+// and release do not call hipMalloc/hipFree (hipFree synchronises the whole device) in the
+// steady state.
+#include "internal.hpp"
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+// Device arenas, best-fit reuse of released blocks (no splitting); blocks are only freed to
+// make room when a new block would pass the budget.
+struct ArenaPool {
+    size_t budget = 0, in_use = 0, cached = 0, peak = 0;
+    std::multimap<size_t, void*> free_blocks;   // size -> ptr
+    std::map<void*, size_t> size_of;
+
+    // 0: ok; 1: does not fit the budget now; <0: HIP error
+    int get(size_t bytes, void** out) {
+        auto it = free_blocks.lower_bound(bytes);
+        if (it != free_blocks.end() && it->first <= bytes + bytes / 4) {
+            *out = it->second;
+            cached -= it->first;
+            in_use += it->first;
+            free_blocks.erase(it);
+            peak = std::max(peak, in_use);
+            return 0;
+        }
+        if (in_use + bytes > budget) return 1;
+        while (in_use + cached + bytes > budget && !free_blocks.empty()) {
+            auto big = std::prev(free_blocks.end());
+            if (hipFree(big->second) != hipSuccess) return -1;
+            cached -= big->first;
+            size_of.erase(big->second);
+            free_blocks.erase(big);
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return 1;
+        size_of[p] = bytes;
+        in_use += bytes;
+        peak = std::max(peak, in_use);
+        *out = p;
+        return 0;
+    }
+    void put(void* p) {
+        size_t b = size_of.at(p);
+        in_use -= b;
+        cached += b;
+        free_blocks.insert({b, p});
+    }
+    void release_all() {
+        for (auto& kv : size_of) (void)hipFree(kv.first);
+        size_of.clear();
+        free_blocks.clear();
+        in_use = cached = 0;
+    }
+};
+
+struct Req {
+    uint64_t ticket;
+    vcl_fit_desc d;
+    int iterations;
+    double arrival, admit = -1, finish = -1;
+    size_t bytes = 0;
+    void* arena = nullptr;
+    vcl_fit* fit = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    int launched = 0, done = 0;
+    float loss = 0.f;
+    int status = VCL_OK;
+    bool reported = false;
+};
+
+}  // namespace
+
+struct vcl_server {
+    vcl_server_config cfg;
+    clk::time_point t0;
+    ArenaPool pool;
+    uint64_t next_ticket = 1;
+    std::deque<Req*> queue;                 // submitted, not admitted (FIFO by submission)
+    std::vector<Req*> active;               // admitted, stepping
+    std::vector<Req*> finished;             // all iterations done (arena held until release)
+    std::vector<hipStream_t> streams;       // idle streams for reuse
+    double occ_integral = 0, occ_last_t = 0;
+
+    double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
+};
+
+static void occupancy_sample(vcl_server* S) {
+    double t = S->now();
+    S->occ_integral += (double)S->pool.in_use * (t - S->occ_last_t);
+    S->occ_last_t = t;
+}
+
+static void release_req(vcl_server* S, Req* r) {
+    if (r->fit) { vcl_fit_destroy(r->fit); r->fit = nullptr; }
+    if (r->ev) { (void)hipEventDestroy(r->ev); r->ev = nullptr; }
+    if (r->stream) { S->streams.push_back(r->stream); r->stream = nullptr; }
+    if (r->arena) { occupancy_sample(S); S->pool.put(r->arena); r->arena = nullptr; }
+}
+
+extern "C" int vcl_server_create(const vcl_server_config* c, vcl_server** out) {
+    VCL_REQUIRE(c && out && c->hbm_budget > 0 && c->max_active >= 1 && c->steps_per_round >= 1);
+    vcl_server* S = new vcl_server();
+    S->cfg = *c;
+    S->pool.budget = c->hbm_budget;
+    S->t0 = clk::now();
+    *out = S;
+    return VCL_OK;
+}
+
+extern "C" int vcl_server_destroy(vcl_server* S) {
+    if (!S) return VCL_OK;
+    for (Req* r : S->active) if (r->stream) (void)hipStreamSynchronize(r->stream);
+    for (auto* v : {&S->active, &S->finished}) {
+        for (Req* r : *v) { release_req(S, r); delete r; }
+        v->clear();
+    }
+    for (Req* r : S->queue) delete r;
+    S->queue.clear();
+    for (hipStream_t s : S->streams) (void)hipStreamDestroy(s);
+    S->pool.release_all();
+    delete S;
+    return VCL_OK;
+}
+
+extern "C" int vcl_server_reset_clock(vcl_server* S) {
+    VCL_REQUIRE(S);
+    S->t0 = clk::now();
+    S->occ_integral = 0;
+    S->occ_last_t = 0;
+    return VCL_OK;
+}
+
+extern "C" double vcl_server_clock(const vcl_server* S) { return S ? S->now() : 0.0; }
+
+extern "C" int vcl_server_submit(vcl_server* S, const vcl_fit_desc* d, int iterations, double arrival_s, uint64_t* ticket) {
+    VCL_REQUIRE(S && d && iterations >= 1);
+    size_t bytes = vcl_fit_arena_bytes(d);
+    VCL_REQUIRE(bytes > 0);
+    if (bytes > S->cfg.hbm_budget) return VCL_ERR_OOM;
+    Req* r = new Req();
+    r->ticket = S->next_ticket++;
+    r->d = *d;
+    r->iterations = iterations;
+    r->arrival = arrival_s;
+    r->bytes = bytes;
+    S->queue.push_back(r);
+    if (ticket) *ticket = r->ticket;
+    return VCL_OK;
+}
+
+static int launch_steps(vcl_server* S, Req* r) {
+    int k = std::min(S->cfg.steps_per_round, r->iterations - r->launched);
+    for (int i = 0; i < k; i++) {
+        int st = S->cfg.use_graph ? vcl_fit_step_graph(r->fit, (vcl_stream_t)r->stream) : vcl_fit_step(r->fit, (vcl_stream_t)r->stream);
+        if (st) return st;
+    }
+    r->launched += k;
+    VCL_HIP_CHECK(hipEventRecord(r->ev, r->stream));
+    return VCL_OK;
+}
+
+extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
+    VCL_REQUIRE(S);
+    // 1. retire completed step batches; finish requests
+    for (size_t i = 0; i < S->active.size();) {
+        Req* r = S->active[i];
+        hipError_t q = hipEventQuery(r->ev);
+        if (q == hipErrorNotReady) { i++; continue; }
+        if (q != hipSuccess) return VCL_ERR_HIP;
+        r->done = r->launched;
+        if (r->done < r->iterations) { i++; continue; }
+        r->finish = S->now();
+        float* lp = (float*)vcl_fit_buffer(r->fit, "loss");
+        VCL_HIP_CHECK(hipMemcpyAsync(&r->loss, lp, sizeof(float), hipMemcpyDeviceToHost, r->stream));
+        VCL_HIP_CHECK(hipStreamSynchronize(r->stream));
+        S->active.erase(S->active.begin() + i);
+        S->finished.push_back(r);
+        if (S->cfg.auto_release) release_req(S, r);
+    }
+    // 2. admit arrived requests (FIFO) while a slot is free and the arena fits the budget
+    double t = S->now();
+    while (!S->queue.empty() && (int)S->active.size() < S->cfg.max_active && S->queue.front()->arrival <= t) {
+        Req* r = S->queue.front();
+        occupancy_sample(S);
+        void* arena = nullptr;
+        int g = S->pool.get(r->bytes, &arena);
+        if (g < 0) return VCL_ERR_HIP;
+        if (g > 0) break;   // wait for memory to be released
+        S->queue.pop_front();
+        r->arena = arena;
+        if (!S->streams.empty()) { r->stream = S->streams.back(); S->streams.pop_back(); }
+        else VCL_HIP_CHECK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+        VCL_HIP_CHECK(hipEventCreateWithFlags(&r->ev, hipEventDisableTiming));
+        int st = vcl_fit_create_in(&r->d, arena, r->bytes, (vcl_stream_t)r->stream, &r->fit);
+        if (st) {
+            r->status = st;
+            r->finish = S->now();
+            S->finished.push_back(r);
+            release_req(S, r);
+            continue;
+        }
+        r->admit = S->now();
+        S->active.push_back(r);
+    }
+    // 3. queue the next steps of every idle active request
+    for (Req* r : S->active) {
+        if (r->launched == r->done && r->launched < r->iterations) {
+            int st = launch_steps(S, r);
+            if (st) return st;
+        }
+    }
+    if (pending) *pending = (int)(S->queue.size() + S->active.size());
+    return VCL_OK;
+}
+
+extern "C" int vcl_server_drain(vcl_server* S) {
+    VCL_REQUIRE(S);
+    int pending = 1;
+    while (pending) {
+        int st = vcl_server_progress(S, &pending);
+        if (st) return st;
+        if (pending) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    return VCL_OK;
+}
+
+extern "C" int vcl_server_finished(vcl_server* S, vcl_request_stats* out, int max, int* n) {
+    VCL_REQUIRE(S && n && (max == 0 || out));
+    int k = 0;
+    for (Req* r : S->finished) {
+        if (r->reported || k >= max) continue;
+        vcl_request_stats& o = out[k++];
+        o.ticket = r->ticket;
+        o.arrival_s = r->arrival;
+        o.admit_s = r->admit;
+        o.finish_s = r->finish;
+        o.iterations = r->done;
+        o.final_loss = r->loss;
+        o.device_bytes = r->bytes;
+        o.status = r->status;
+        r->reported = true;
+    }
+    *n = k;
+    return VCL_OK;
+}
+
+static Req* find_finished(vcl_server* S, uint64_t ticket) {
+    for (Req* r : S->finished) if (r->ticket == ticket) return r;
+    return nullptr;
+}
+
+extern "C" int vcl_server_read(vcl_server* S, uint64_t ticket, const char* name, void* host, size_t bytes) {
+    VCL_REQUIRE(S && name && host);
+    Req* r = find_finished(S, ticket);
+    VCL_REQUIRE(r && r->fit);
+    return vcl_fit_read(r->fit, name, host, bytes, (vcl_stream_t)r->stream);
+}
+
+extern "C" int vcl_server_release(vcl_server* S, uint64_t ticket) {
+    VCL_REQUIRE(S);
+    for (size_t i = 0; i < S->finished.size(); i++) {
+        Req* r = S->finished[i];
+        if (r->ticket != ticket) continue;
+        release_req(S, r);
+        delete r;
+        S->finished.erase(S->finished.begin() + i);
+        return VCL_OK;
+    }
+    return VCL_ERR_INVALID;
+}
+
+extern "C" int vcl_server_memory(vcl_server* S, size_t* in_use, size_t* peak, size_t* cached, double* mean_in_use) {
+    VCL_REQUIRE(S);
+    occupancy_sample(S);
+    if (in_use) *in_use = S->pool.in_use;
+    if (peak) *peak = S->pool.peak;
+    if (cached) *cached = S->pool.cached;
+    if (mean_in_use) *mean_in_use = S->occ_last_t > 0 ? S->occ_integral / S->occ_last_t : 0.0;
+    return VCL_OK;
+}

@@ -286,34 +286,42 @@ def mse_loss(pred, target):
 
 
 # ---------------------------------------------------------------- Fit engine
+def fit_desc(scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_quarter=True, fused=False, views=None):
+    """vcl_fit_desc of a Scene; returns (desc, keep) where `keep` holds the host arrays the
+    descriptor points into (they must outlive the upload)."""
+    cams = scene.cams if views is None else views
+    keep = []
+
+    def h(a, dt=np.float32):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    mats = np.stack([c.mat for c in cams]).astype(np.float32)
+    pvinv = np.stack([c.pvinv for c in cams]).astype(np.float32)
+    d = L.FitDesc()
+    d.width, d.height, d.views = scene.W, scene.H, len(cams)
+    d.V, d.T = scene.V, scene.T
+    d.texwidth, d.texheight, d.channel, d.miplevel, d.sgnum = scene.tw, scene.th, scene.C, scene.L, scene.K
+    d.pos, d.normals, d.uv = h(scene.pos), h(scene.normals), h(scene.uv)
+    d.idx = h(scene.idx, np.uint32)
+    d.texture = h(scene.tex)
+    d.axis, d.sharpness, d.amplitude = h(scene.axis), h(scene.sharp), h(scene.amp)
+    d.mats, d.pvinv, d.rot = h(mats), h(pvinv), h(np.eye(4, dtype=np.float32).reshape(16))
+    d.target = h(target) if target is not None else None
+    d.roughness, d.ior = scene.rough, scene.ior
+    d.lr, d.beta1, d.beta2, d.eps = lr, betas[0], betas[1], eps
+    d.fold_quarter = 1 if fold_quarter else 0
+    d.fused = 1 if fused else 0
+    return d, keep
+
+
 class Fit:
     """One device-resident fit request (vcl_fit_* of include/vcl.h)."""
 
     def __init__(self, scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_quarter=True, fused=False, views=None):
         cams = scene.cams if views is None else views
-        self._keep = []
-
-        def h(a, dt=np.float32):
-            a = np.ascontiguousarray(a, dt)
-            self._keep.append(a)
-            return a.ctypes.data
-
-        mats = np.stack([c.mat for c in cams]).astype(np.float32)
-        pvinv = np.stack([c.pvinv for c in cams]).astype(np.float32)
-        d = L.FitDesc()
-        d.width, d.height, d.views = scene.W, scene.H, len(cams)
-        d.V, d.T = scene.V, scene.T
-        d.texwidth, d.texheight, d.channel, d.miplevel, d.sgnum = scene.tw, scene.th, scene.C, scene.L, scene.K
-        d.pos, d.normals, d.uv = h(scene.pos), h(scene.normals), h(scene.uv)
-        d.idx = h(scene.idx, np.uint32)
-        d.texture = h(scene.tex)
-        d.axis, d.sharpness, d.amplitude = h(scene.axis), h(scene.sharp), h(scene.amp)
-        d.mats, d.pvinv, d.rot = h(mats), h(pvinv), h(np.eye(4, dtype=np.float32).reshape(16))
-        d.target = h(target) if target is not None else None
-        d.roughness, d.ior = scene.rough, scene.ior
-        d.lr, d.beta1, d.beta2, d.eps = lr, betas[0], betas[1], eps
-        d.fold_quarter = 1 if fold_quarter else 0
-        d.fused = 1 if fused else 0
+        d, self._keep = fit_desc(scene, target, lr, betas, eps, fold_quarter, fused, cams)
         self.desc = d
         self.scene = scene
         self.views = len(cams)
