@@ -112,11 +112,11 @@ def _served_specs(SV, n, rate, seed):
     return built
 
 
-def _serve_once(SV, built, rate_scale, max_active, use_graph, steps_per_round, budget):
+def _serve_once(SV, built, rate_scale, max_active, use_graph, steps_per_round, budget, policy=0, max_stepping=0):
     """Run one arrival stream through a server; arrival times scaled by rate_scale (0: all at
     t=0).  Returns latency / throughput / occupancy figures."""
     srv = SV.Server(hbm_budget=budget, max_active=max_active, use_graph=use_graph, steps_per_round=steps_per_round,
-                    auto_release=True)
+                    auto_release=True, policy=policy, max_stepping=max_stepping)
     for sp, sc, tgt in built:
         srv.submit(sc, tgt, sp["iterations"], sp["arrival_s"] * rate_scale)
     srv.reset_clock()
@@ -150,9 +150,12 @@ def run_served(torch, rates, n, seed=3, budget_gb=200, max_active=16):
            "rates": {}}
     for rate in rates:
         cont = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget)
+        srpt = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget, policy=SV.Server.SRPT, max_stepping=4)
         seq = _serve_once(SV, base, 1.0 / rate, 1, False, 1, budget)
-        out["rates"][str(rate)] = {"continuous": cont, "sequential": seq,
-                                   "throughput_vs_sequential": round(cont["eq_frames_per_s"] / seq["eq_frames_per_s"], 3)}
+        out["rates"][str(rate)] = {"continuous_fifo": cont, "continuous_srpt": srpt, "sequential": seq,
+                                   "throughput_vs_sequential": round(srpt["eq_frames_per_s"] / seq["eq_frames_per_s"], 3),
+                                   "p50_vs_sequential": round(srpt["p50_ms"] / seq["p50_ms"], 3),
+                                   "p99_vs_sequential": round(srpt["p99_ms"] / seq["p99_ms"], 3)}
     return out
 
 

@@ -453,12 +453,20 @@ int vcl_fit_profile_end(vcl_fit* f, int max_ops, int* nops, char* names, int nam
  * resident in the request's arena until it is released. */
 typedef struct vcl_server vcl_server;  /* opaque */
 
+enum {
+    VCL_SCHED_FIFO = 0,   /* admit in arrival order; every active request steps */
+    VCL_SCHED_SRPT = 1    /* admit the arrived request with the least work first; only the
+                             max_stepping active requests with the least remaining work step */
+};
+
 typedef struct vcl_server_config {
     size_t hbm_budget;    /* bytes of request arenas the server may hold */
-    int max_active;       /* requests admitted and stepping at once */
+    int max_active;       /* requests admitted (resident) at once */
     int use_graph;        /* 1: replay each request's step from its HIP graph */
     int steps_per_round;  /* steps queued per request per scheduling round (>= 1) */
     int auto_release;     /* 1: free a request's arena as soon as it finishes */
+    int policy;           /* VCL_SCHED_FIFO or VCL_SCHED_SRPT */
+    int max_stepping;     /* SRPT: requests stepping at once (0: all active) */
 } vcl_server_config;
 
 typedef struct vcl_request_stats {

@@ -123,7 +123,8 @@ class FitDesc(C.Structure):
 
 
 class ServerConfig(C.Structure):
-    _fields_ = [("hbm_budget", csz), ("max_active", ci), ("use_graph", ci), ("steps_per_round", ci), ("auto_release", ci)]
+    _fields_ = [("hbm_budget", csz), ("max_active", ci), ("use_graph", ci), ("steps_per_round", ci), ("auto_release", ci),
+                ("policy", ci), ("max_stepping", ci)]
 
 
 class RequestStats(C.Structure):

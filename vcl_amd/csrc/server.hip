@@ -2,9 +2,11 @@
 //
 // Requests are queued with an arrival time on the server clock.  A scheduling round
 // (vcl_server_progress) retires completed step batches, finishes requests that ran all their
-// iterations, admits arrived requests in FIFO order while a slot is free and the request's
-// arena fits the HBM budget, and queues the next steps of every active request on that
-// request's own stream, so the steps of different requests overlap on the device.  Every
+// iterations, admits arrived requests while a slot is free and the request's arena fits the
+// HBM budget, and queues the next steps of active requests, each on its own stream, so the
+// steps of different requests overlap on the device.  Policy FIFO admits in arrival order and
+// steps every active request; SRPT admits the arrived request with the least work first and
+// steps only the few active requests with the least remaining work.  Every
 // request's state (mesh, mip texture, lobes, Adam moments, intermediates) stays resident in one
 // arena until the request is released; arenas are recycled through a pool so that admission
 // and release do not call hipMalloc/hipFree (hipFree synchronises the whole device) in the
@@ -81,6 +83,7 @@ struct Req {
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     int launched = 0, done = 0;
+    double cost = 0;     // work per iteration (pixels + 8 per triangle, over all views)
     float loss = 0.f;
     int status = VCL_OK;
     bool reported = false;
@@ -161,6 +164,7 @@ extern "C" int vcl_server_submit(vcl_server* S, const vcl_fit_desc* d, int itera
     r->iterations = iterations;
     r->arrival = arrival_s;
     r->bytes = bytes;
+    r->cost = (double)d->width * d->height * d->views + 8.0 * d->T * d->views;
     S->queue.push_back(r);
     if (ticket) *ticket = r->ticket;
     return VCL_OK;
@@ -195,16 +199,25 @@ extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
         S->finished.push_back(r);
         if (S->cfg.auto_release) release_req(S, r);
     }
-    // 2. admit arrived requests (FIFO) while a slot is free and the arena fits the budget
+    // 2. admit arrived requests while a slot is free and the arena fits the budget: FIFO, or
+    //    under VCL_SCHED_SRPT the arrived request with the least total work first
     double t = S->now();
-    while (!S->queue.empty() && (int)S->active.size() < S->cfg.max_active && S->queue.front()->arrival <= t) {
-        Req* r = S->queue.front();
+    const bool srpt = S->cfg.policy == VCL_SCHED_SRPT;
+    while ((int)S->active.size() < S->cfg.max_active) {
+        auto pick = S->queue.end();
+        for (auto it = S->queue.begin(); it != S->queue.end(); ++it) {
+            if ((*it)->arrival > t) { if (!srpt) break; continue; }
+            if (!srpt) { pick = it; break; }
+            if (pick == S->queue.end() || (*it)->cost * (*it)->iterations < (*pick)->cost * (*pick)->iterations) pick = it;
+        }
+        if (pick == S->queue.end()) break;
+        Req* r = *pick;
         occupancy_sample(S);
         void* arena = nullptr;
         int g = S->pool.get(r->bytes, &arena);
         if (g < 0) return VCL_ERR_HIP;
         if (g > 0) break;   // wait for memory to be released
-        S->queue.pop_front();
+        S->queue.erase(pick);
         r->arena = arena;
         if (!S->streams.empty()) { r->stream = S->streams.back(); S->streams.pop_back(); }
         else VCL_HIP_CHECK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
@@ -220,11 +233,27 @@ extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
         r->admit = S->now();
         S->active.push_back(r);
     }
-    // 3. queue the next steps of every idle active request
-    for (Req* r : S->active) {
-        if (r->launched == r->done && r->launched < r->iterations) {
-            int st = launch_steps(S, r);
-            if (st) return st;
+    // 3. queue the next steps: every idle active request (FIFO), or under VCL_SCHED_SRPT only the
+    //    max_stepping requests with the least remaining work (the others stay resident, paused)
+    if (srpt) {
+        std::vector<Req*> order(S->active);
+        std::stable_sort(order.begin(), order.end(), [](const Req* a, const Req* b) {
+            return a->cost * (a->iterations - a->done) < b->cost * (b->iterations - b->done);
+        });
+        int lim = S->cfg.max_stepping > 0 ? S->cfg.max_stepping : (int)order.size();
+        for (int i = 0; i < (int)order.size() && i < lim; i++) {
+            Req* r = order[i];
+            if (r->launched == r->done && r->launched < r->iterations) {
+                int st = launch_steps(S, r);
+                if (st) return st;
+            }
+        }
+    } else {
+        for (Req* r : S->active) {
+            if (r->launched == r->done && r->launched < r->iterations) {
+                int st = launch_steps(S, r);
+                if (st) return st;
+            }
         }
     }
     if (pending) *pending = (int)(S->queue.size() + S->active.size());
