@@ -67,7 +67,7 @@ def algorithmic_bytes(sc, views=1):
     }
 
 
-def run_batched(ops, S, torch, steps, warmup, views=64, W=512):
+def run_batched(ops, S, torch, steps, warmup, views=64, W=512, tune=True):
     """Batched multi-view request: the 100k-triangle mesh from `views` Fibonacci-sphere cameras
     at W x W, shared 2048^2 x 4 texture, 64 lobes; one fwd+bwd+Adam step over all views per
     iteration, replayed from a HIP graph.  Returns the result object (1024^2-equivalent frames/s
@@ -82,7 +82,17 @@ def run_batched(ops, S, torch, steps, warmup, views=64, W=512):
         wf.step(graph=True)
     torch.cuda.synchronize()
     wf.close()
+    tuning = None
+    if tune:
+        # autotuner on a separate request advanced to mid-run (see the plain path)
+        tf_ = ops.Fit(sc, target=target)
+        for _ in range(steps // 2):
+            tf_.step(graph=True)
+        tuning = tf_.autotune(steps=3)
+        tf_.close()
     fit = ops.Fit(sc, target=target)
+    if tuning:
+        fit.set_tuning(tuning["raster_tile"], tuning["bin_chunk"], tuning["splat_rows"])
     fit.step(graph=True)   # iteration 1 captures the graph (not timed)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -98,7 +108,7 @@ def run_batched(ops, S, torch, steps, warmup, views=64, W=512):
     px = views * W * W
     return {"value": round(px / (1 << 20) / (ms / 1e3), 2), "unit": "1024^2-equivalent frames/s",
             "views_per_s": round(views / (ms / 1e3), 2), "ms_per_step": round(ms, 4), "steps": steps, "warmup": warmup,
-            "graph": True, "loss": loss, "device_bytes": int(hbm),
+            "graph": True, "loss": loss, "device_bytes": int(hbm), "tuning": tuning,
             "workload": f"batched multi-view: displaced sphere {sc.V} V / {sc.T} T from {views} Fibonacci-sphere cameras at "
                         f"{W}x{W}, shared texture {sc.tw}^2x{sc.C} L={sc.L}, {sc.K} SG lobes; iterations 2..{steps + 1} of the request"}
 
@@ -112,17 +122,23 @@ def _served_specs(SV, n, rate, seed):
     return built
 
 
-def _serve_once(SV, built, rate_scale, max_active, use_graph, steps_per_round, budget, policy=0, max_stepping=0):
+def _serve_once(SV, built, rate_scale, max_active, use_graph, steps_per_round, budget, policy=0, max_stepping=0, tunings=None):
     """Run one arrival stream through a server; arrival times scaled by rate_scale (0: all at
-    t=0).  Returns latency / throughput / occupancy figures."""
+    t=0).  Returns latency / throughput / occupancy figures.  tunings: None (no autotuning) or a
+    list of (size-class key, tuning) that pre-fills the server's table and receives it back
+    (classes missing from it are autotuned on admission)."""
     srv = SV.Server(hbm_budget=budget, max_active=max_active, use_graph=use_graph, steps_per_round=steps_per_round,
-                    auto_release=True, policy=policy, max_stepping=max_stepping)
+                    auto_release=True, policy=policy, max_stepping=max_stepping, autotune_steps=3 if tunings is not None else 0)
+    if tunings:
+        srv.load_tuning_table(tunings)
     for sp, sc, tgt in built:
         srv.submit(sc, tgt, sp["iterations"], sp["arrival_s"] * rate_scale)
     srv.reset_clock()
     srv.drain()
     st = srv.finished()
     mem = srv.memory()
+    if tunings is not None:
+        tunings[:] = srv.tuning_table()
     srv.close()
     assert len(st) == len(built) and all(x["status"] == 0 for x in st), st
     lat = [x["finish_s"] - x["arrival_s"] for x in st]
@@ -151,8 +167,15 @@ def run_served(torch, rates, n, seed=3, budget_gb=200, max_active=16):
     for rate in rates:
         cont = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget)
         srpt = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget, policy=SV.Server.SRPT, max_stepping=4)
+        # autotuner: an untimed pass fills the per-size-class table, the timed pass reuses it
+        table = []
+        _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget, policy=SV.Server.SRPT, max_stepping=4, tunings=table)
+        srpt_tuned = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget, policy=SV.Server.SRPT, max_stepping=4,
+                                 tunings=table)
+        srpt_tuned["size_classes"] = {f"px~4^{k & 255} tri~8^{(k >> 8) & 255}": t for k, t in table}
         seq = _serve_once(SV, base, 1.0 / rate, 1, False, 1, budget)
-        out["rates"][str(rate)] = {"continuous_fifo": cont, "continuous_srpt": srpt, "sequential": seq,
+        out["rates"][str(rate)] = {"continuous_fifo": cont, "continuous_srpt": srpt, "continuous_srpt_autotuned": srpt_tuned,
+                                   "sequential": seq,
                                    "throughput_vs_sequential": round(srpt["eq_frames_per_s"] / seq["eq_frames_per_s"], 3),
                                    "p50_vs_sequential": round(srpt["p50_ms"] / seq["p50_ms"], 3),
                                    "p99_vs_sequential": round(srpt["p99_ms"] / seq["p99_ms"], 3)}
@@ -231,6 +254,7 @@ def main():
                     help="for rocprofv3 --pmc runs: one 256 MiB device copy (counter calibration) then --probe-steps fit steps")
     ap.add_argument("--probe-steps", type=int, default=3)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view measurement")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default kernel schedule (no autotuner)")
     ap.add_argument("--served", default="50", help="comma-separated arrival rates (req/s) for the served measurement; '' skips it")
     ap.add_argument("--served-requests", type=int, default=24)
     ap.add_argument("--uneven", type=int, default=0, help="run the uneven-scene batch with this many scenes (0: skip)")
@@ -277,8 +301,31 @@ def main():
     for _ in range(args.warmup):
         wf.step()
     torch.cuda.synchronize()
+    tuning = None
+    default_ms = None
+    if not args.no_tune:
+        # autotuner on a separate request advanced to mid-run (the mesh changes as it is
+        # fitted, so the schedule is chosen where the run spends its time); the timed request
+        # starts fresh with its choice
+        tf_ = ops.Fit(sc, target=target)
+        for _ in range(args.steps // 2):
+            tf_.step(graph=True)
+        tuning = tf_.autotune(steps=5)
+        tf_.close()
+        df = ops.Fit(sc, target=target)
+        d0, d1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        d0.record()
+        for _ in range(args.steps):
+            df.step()
+        d1.record()
+        torch.cuda.synchronize()
+        default_ms = d0.elapsed_time(d1) / args.steps
+        df.close()
     wf.close()
     fit = ops.Fit(sc, target=target)
+    if tuning:
+        fit.set_tuning(tuning["raster_tile"], tuning["bin_chunk"], tuning["splat_rows"])
     views = fit.views
     # timed region: the dominant HBM kernel is timed with HIP events on the launch stream
     ROOF_OP = "texture_adam"
@@ -337,7 +384,7 @@ def main():
     uneven = run_uneven(torch, args.uneven) if args.uneven else None
     batched = None
     if not args.no_batched:
-        batched = run_batched(ops, S, torch, args.batched_steps, 3, views=args.batched_views)
+        batched = run_batched(ops, S, torch, args.batched_steps, 3, views=args.batched_views, tune=not args.no_tune)
         batched["vs_plain"] = round(batched["value"] / value, 3)
     out = {
         "metric": "fwd+bwd+Adam frames/s at 1024^2, 100k tris (plain per-request path)",
@@ -349,6 +396,8 @@ def main():
                    "mode": args.mode, "views": views},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "tuning": tuning,
+        "default_tuning_ms_per_step": round(default_ms, 4) if default_ms else None,
         "batched": batched,
         "served": served,
         "uneven": uneven,

@@ -443,6 +443,30 @@ int vcl_fit_write(vcl_fit* f, const char* name, const void* host, size_t bytes, 
 int vcl_fit_profile_begin(vcl_fit* f, int max_launches, const char* only);
 int vcl_fit_profile_end(vcl_fit* f, int max_ops, int* nops, char* names, int name_len, double* total_ms, int* count);
 
+/* ---------------------------------------------------------------- Tuning
+ * Kernel schedule parameters (no counterpart in the reference): every value gives the same
+ * results (the raster is exact for any tiling; texel gradients differ only in float summation
+ * order), only the speed changes.  vcl_set_tuning applies to launches that follow it in this
+ * process; vcl_fit_autotune picks them for one request. */
+typedef struct vcl_tuning {
+    int raster_tile;   /* fine-raster tile side in pixels: 8, 16 (default) or 32 */
+    int bin_chunk;     /* pieces per fine-raster work item ("bin depth"): 64, 128 or 256 (default) */
+    int splat_rows;    /* texture backward: rows of the 16-pixel-wide region whose texel-gradient
+                          splats share one LDS table ("texel-splat width"): 16 (default), 32 or 64 */
+} vcl_tuning;
+
+int vcl_set_tuning(const vcl_tuning* t);
+int vcl_get_tuning(vcl_tuning* t);
+/* Tuning of one request's steps (NULL: follow the process-wide tuning); drops its graph. */
+int vcl_fit_set_tuning(vcl_fit* f, const vcl_tuning* t);
+int vcl_fit_get_tuning(const vcl_fit* f, vcl_tuning* t);
+/* Autotuner: times `steps` steps of candidate tunings on this request (the current tuning,
+ * then raster tile x bin chunk, then texel-splat rows; every candidate from the same snapshot
+ * of the request's parameters and Adam state, which is restored afterwards).  A candidate
+ * replaces the incumbent only when at least 3 % faster.  Keeps the winner on the request and
+ * returns it with its ms per step.  Allocates a temporary snapshot of the persistent state. */
+int vcl_fit_autotune(vcl_fit* f, int steps, vcl_tuning* best, double* best_ms);
+
 /* ---------------------------------------------------------------- Serving
  * Continuous batching of fit requests on one GPU (no counterpart in the reference, which runs
  * one preset in a GLUT loop, main.cpp:39-49).  Requests queue with an arrival time on the
@@ -467,6 +491,10 @@ typedef struct vcl_server_config {
     int auto_release;     /* 1: free a request's arena as soon as it finishes */
     int policy;           /* VCL_SCHED_FIFO or VCL_SCHED_SRPT */
     int max_stepping;     /* SRPT: requests stepping at once (0: all active) */
+    int autotune_steps;   /* > 0: autotune the first request of each size class (pixels over
+                             all views in buckets of 4x, triangles in buckets of 8x) with this
+                             many timed steps per candidate, and reuse its tuning for the class;
+                             0: no tuning */
 } vcl_server_config;
 
 typedef struct vcl_request_stats {
@@ -499,6 +527,11 @@ int vcl_server_release(vcl_server* s, uint64_t ticket);
 /* Arena bytes held by admitted requests now, at peak, cached for reuse, and the time-average
  * since the clock was reset. */
 int vcl_server_memory(vcl_server* s, size_t* in_use, size_t* peak, size_t* cached, double* mean_in_use);
+/* The tuning table: size-class keys and their tunings (up to `max`). */
+int vcl_server_tunings(vcl_server* s, uint64_t* keys, vcl_tuning* out, int max, int* n);
+/* Pre-fill the tuning table (e.g. from a previous server's vcl_server_tunings), so requests of
+ * those size classes are not tuned on admission. */
+int vcl_server_load_tunings(vcl_server* s, const uint64_t* keys, const vcl_tuning* t, int n);
 
 #ifdef __cplusplus
 }

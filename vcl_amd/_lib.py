@@ -122,9 +122,13 @@ class FitDesc(C.Structure):
                 ("fold_quarter", ci), ("fused", ci)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("raster_tile", ci), ("bin_chunk", ci), ("splat_rows", ci)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", csz), ("max_active", ci), ("use_graph", ci), ("steps_per_round", ci), ("auto_release", ci),
-                ("policy", ci), ("max_stepping", ci)]
+                ("policy", ci), ("max_stepping", ci), ("autotune_steps", ci)]
 
 
 class RequestStats(C.Structure):
@@ -181,6 +185,13 @@ SIGNATURES = {
     "vcl_server_read": ([vp, C.c_uint64, C.c_char_p, vp, csz], ci),
     "vcl_server_release": ([vp, C.c_uint64], ci),
     "vcl_server_memory": ([vp, P_(csz), P_(csz), P_(csz), P_(cd)], ci),
+    "vcl_server_tunings": ([vp, P_(C.c_uint64), P_(Tuning), ci, P_(ci)], ci),
+    "vcl_server_load_tunings": ([vp, P_(C.c_uint64), P_(Tuning), ci], ci),
+    "vcl_set_tuning": ([P_(Tuning)], ci),
+    "vcl_get_tuning": ([P_(Tuning)], ci),
+    "vcl_fit_set_tuning": ([vp, P_(Tuning)], ci),
+    "vcl_fit_get_tuning": ([vp, P_(Tuning)], ci),
+    "vcl_fit_autotune": ([vp, ci, P_(Tuning), P_(cd)], ci),
     "vcl_fit_destroy": ([vp], ci),
     "vcl_fit_set_target_from_render": ([vp, vp], ci),
     "vcl_fit_step": ([vp, vp], ci),

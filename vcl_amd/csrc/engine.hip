@@ -30,6 +30,8 @@ struct vcl_fit {
     size_t total = 0;
     void* arena = nullptr;   // every buffer of the request
     bool own_arena = false;  // allocated by vcl_fit_create (else the caller's memory)
+    vcl_tuning tune{};       // kernel schedule of this request's steps
+    bool has_tune = false;   // false: the process-wide tuning
     int it = 0;         // steps taken (eager or graph)
     size_t ws_rast = 0, ws_sg = 0, ws_loss = 0;
     // HIP graph of one step (vcl_fit_step_graph): captured once on a private stream
@@ -354,7 +356,22 @@ extern "C" int vcl_fit_set_target_from_render(vcl_fit* F, vcl_stream_t s) {
     return VCL_OK;
 }
 
+// Applies the request's tuning to the launches of one step (restored afterwards).
+struct TuneScope {
+    vcl_tuning saved;
+    bool on;
+    explicit TuneScope(const vcl_fit* F) : saved(g_vcl_tuning), on(F->has_tune) { if (on) g_vcl_tuning = F->tune; }
+    ~TuneScope() { if (on) g_vcl_tuning = saved; }
+};
+
+static int step_body_tuned(vcl_fit* F, vcl_stream_t s);
+
 static int step_body(vcl_fit* F, vcl_stream_t s) {
+    TuneScope ts(F);
+    return step_body_tuned(F, s);
+}
+
+static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     const vcl_fit_desc& d = F->d;
     int W = d.width, H = d.height, nv = d.views, C = d.channel, K = d.sgnum;
     hipStream_t hs = as_hip(s);
@@ -478,3 +495,104 @@ extern "C" int vcl_fit_step_graph(vcl_fit* F, vcl_stream_t s) {
     F->it++;
     return VCL_OK;
 }
+
+static void drop_graph(vcl_fit* F) {
+    if (F->gstream) (void)hipStreamSynchronize(F->gstream);
+    if (F->gexec) { (void)hipGraphExecDestroy(F->gexec); F->gexec = nullptr; }
+    if (F->graph) { (void)hipGraphDestroy(F->graph); F->graph = nullptr; }
+}
+
+extern "C" int vcl_fit_set_tuning(vcl_fit* F, const vcl_tuning* t) {
+    VCL_REQUIRE(F);
+    if (t) {
+        vcl_tuning saved = g_vcl_tuning;
+        int st = vcl_set_tuning(t);   // validates
+        g_vcl_tuning = saved;
+        if (st) return st;
+        F->tune = *t;
+    }
+    F->has_tune = t != nullptr;
+    drop_graph(F);                 // a captured step holds the old launch shapes
+    return VCL_OK;
+}
+
+extern "C" int vcl_fit_get_tuning(const vcl_fit* F, vcl_tuning* t) {
+    VCL_REQUIRE(F && t);
+    *t = F->has_tune ? F->tune : g_vcl_tuning;
+    return VCL_OK;
+}
+
+// Times `steps` steps of each candidate tuning, every candidate starting from the same
+// snapshot of the request's persistent state (parameters, Adam moments and bias powers), then
+// restores the snapshot and keeps the fastest candidate.  Coordinate descent: raster tile x bin
+// chunk first, then the texel-splat rows.
+extern "C" int vcl_fit_autotune(vcl_fit* F, int steps, vcl_tuning* best, double* best_ms) {
+    VCL_REQUIRE(F && steps >= 1);
+    static const char* keep[] = {"pos", "texture", "axis", "sharpness", "amplitude", "m_pos", "v_pos", "m_tex", "v_tex",
+                                 "m_axis", "v_axis", "m_sharp", "v_sharp", "m_amp", "v_amp", "adam_bt"};
+    size_t total = 0;
+    for (const char* n : keep) total += (F->bytes.at(n) + 255) & ~(size_t)255;
+    char* snap = nullptr;
+    if (hipMalloc((void**)&snap, total) != hipSuccess) return VCL_ERR_OOM;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = VCL_OK;
+    auto copy = [&](bool save) {
+        size_t o = 0;
+        for (const char* n : keep) {
+            size_t b = F->bytes.at(n);
+            void* dev = F->buf.at(n);
+            if (hipMemcpyAsync(save ? (void*)(snap + o) : dev, save ? dev : (void*)(snap + o), b, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                st = VCL_ERR_HIP;
+            o += (b + 255) & ~(size_t)255;
+        }
+    };
+    const vcl_tuning start = F->has_tune ? F->tune : g_vcl_tuning;
+    const bool had = F->has_tune;
+    vcl_tuning best_t = start;
+    double best_v = 1e30;
+    // a candidate replaces the incumbent only when it is at least 3 % faster, so timing noise
+    // does not move the schedule away from the starting one
+    constexpr double MARGIN = 0.97;
+    auto same = [](const vcl_tuning& a, const vcl_tuning& b) {
+        return a.raster_tile == b.raster_tile && a.bin_chunk == b.bin_chunk && a.splat_rows == b.splat_rows;
+    };
+    auto trial = [&](vcl_tuning cand) {
+        if (st) return;
+        F->tune = cand;
+        F->has_tune = true;
+        copy(false);
+        if (!st) st = step_body(F, (vcl_stream_t)s);   // warm-up (first launch of a variant)
+        copy(false);
+        if (!st && hipEventRecord(e0, s) != hipSuccess) st = VCL_ERR_HIP;
+        for (int i = 0; i < steps && !st; i++) st = step_body(F, (vcl_stream_t)s);
+        if (!st && hipEventRecord(e1, s) != hipSuccess) st = VCL_ERR_HIP;
+        float ms = 0.f;
+        if (!st && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)) st = VCL_ERR_HIP;
+        if (!st && (best_v >= 1e30 || ms / steps < MARGIN * best_v)) { best_v = ms / steps; best_t = cand; }
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = VCL_ERR_HIP;
+    if (!st && hipDeviceSynchronize() != hipSuccess) st = VCL_ERR_HIP;
+    copy(true);
+    static const int tiles[] = {8, 16, 32}, chunks[] = {64, 128, 256}, rows[] = {16, 32, 64};
+    trial(start);   // the incumbent is timed first
+    for (int ti : tiles)
+        for (int ch : chunks)
+            if (!same(vcl_tuning{ti, ch, start.splat_rows}, start)) trial(vcl_tuning{ti, ch, start.splat_rows});
+    const vcl_tuning r1 = best_t;
+    for (int r : rows)
+        if (r != r1.splat_rows) trial(vcl_tuning{r1.raster_tile, r1.bin_chunk, r});
+    copy(false);
+    if (!st && hipStreamSynchronize(s) != hipSuccess) st = VCL_ERR_HIP;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(snap);
+    if (st) { F->tune = start; F->has_tune = had; return st; }
+    F->tune = best_t;
+    F->has_tune = true;
+    drop_graph(F);
+    if (best) *best = best_t;
+    if (best_ms) *best_ms = best_v;
+    return VCL_OK;
+}
+

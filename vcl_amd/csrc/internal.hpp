@@ -3,6 +3,10 @@
 #pragma once
 #include "common.hpp"
 
+// Process-wide kernel tuning (vcl_set_tuning); read by the launch code of the raster and
+// texture-backward entry points.
+extern vcl_tuning g_vcl_tuning;
+
 extern "C" {
 // ViewAxis forward with the per-view inverse matrices already on the device (depth x 9 floats).
 int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream);

@@ -3,7 +3,7 @@
 // utilities.  One thread per pixel over the row-major image (coalesced, background pixels exit
 // early); reductions that the reference does with racy or atomic float adds are done with
 // fixed-order block partials so results are deterministic.
-#include "common.hpp"
+#include "internal.hpp"
 #include <cmath>
 
 __global__ void vcl_fill_u32(u32* p, size_t n, u32 v) {
@@ -375,15 +375,11 @@ __device__ __forceinline__ void tex_splat(LdsTable<TB_SLOTS, 4>& tab, float* G, 
     }
 }
 
-__global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
-                                                       const float* uvDA, const float* gout, float* guv, float* guvDA, int mode) {
-    __shared__ LdsTable<TB_SLOTS, 4> tab;
-    bool want_tex = g.tex[0] != nullptr;
-    if (want_tex) lds_table_clear(tab);
-    __syncthreads();
-    int view = blockIdx.y;
-    int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
-    int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
+// One pixel of the C == 4 texture backward: texel splats into the LDS table, uv / uvDA grads
+// stored.
+__device__ __forceinline__ void tex_bwd4_pixel(LdsTable<TB_SLOTS, 4>& tab, bool want_tex, int W, int H, int view, int px, int py, const TexArgs& t,
+                                               const TexGradArgs& g, const float* rast, const float* uv, const float* uvDA, const float* gout,
+                                               float* guv, float* guvDA, int mode) {
     if (px < W && py < H) {
         size_t p = ((size_t)view * H + py) * W + px;
         if (rast[4 * p + 3] >= 1.f) {
@@ -443,6 +439,26 @@ __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX,
             if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
             if (guvDA && t.L > 1) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
         }
+    }
+}
+
+// Texel-splat region of 16 x ROWS pixels per workgroup (ROWS / 16 pixels per thread): the
+// texel grads of the region are summed in one LDS hash table keyed by (level, texel) and flushed
+// with one set of atomics per touched texel.
+template <int ROWS>
+__global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
+                                                       const float* uvDA, const float* gout, float* guv, float* guvDA, int mode) {
+    __shared__ LdsTable<TB_SLOTS, 4> tab;
+    bool want_tex = g.tex[0] != nullptr;
+    if (want_tex) lds_table_clear(tab);
+    __syncthreads();
+    int view = blockIdx.y;
+    int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
+    int px = tx * 16 + (threadIdx.x & 15);
+#pragma unroll
+    for (int k = 0; k < ROWS / 16; k++) {
+        int py = ty * ROWS + k * 16 + (threadIdx.x >> 4);
+        tex_bwd4_pixel(tab, want_tex, W, H, view, px, py, t, g, rast, uv, uvDA, gout, guv, guvDA, mode);
     }
     if (!want_tex) return;
     __syncthreads();
@@ -1333,9 +1349,19 @@ extern "C" int vcl_texturemap_backward(const vcl_texturemap_params* p, const vcl
     bool all_or_none = true;
     for (int l = 1; l < t.L; l++) all_or_none &= ((ga.tex[l] != nullptr) == (ga.tex[0] != nullptr));
     if (t.C == 4 && (size_t)t.tw * t.th <= (1u << 24) && all_or_none) {
-        int tilesX = (p->width + 15) / 16, tilesY = (p->height + 15) / 16;
-        hipLaunchKernelGGL(k_tex_bwd4_tiled, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga,
-                           p->rast, p->uv, p->uvDA, g->out, g->uv, t.L > 1 ? g->uvDA : nullptr, g->daMode);
+        const int rows = g_vcl_tuning.splat_rows;
+        int tilesX = (p->width + 15) / 16, tilesY = (p->height + rows - 1) / rows;
+        dim3 grid(tilesX * tilesY, p->depth);
+        float* gda = t.L > 1 ? g->uvDA : nullptr;
+        if (rows == 64)
+            hipLaunchKernelGGL(k_tex_bwd4_tiled<64>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
+                               g->out, g->uv, gda, g->daMode);
+        else if (rows == 32)
+            hipLaunchKernelGGL(k_tex_bwd4_tiled<32>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
+                               g->out, g->uv, gda, g->daMode);
+        else
+            hipLaunchKernelGGL(k_tex_bwd4_tiled<16>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
+                               g->out, g->uv, gda, g->daMode);
     } else {
         hipLaunchKernelGGL(k_tex_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, ga, p->rast, p->uv, p->uvDA, g->out, g->uv,
                            t.L > 1 ? g->uvDA : nullptr, g->daMode);

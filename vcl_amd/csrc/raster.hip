@@ -24,11 +24,14 @@
 // Coverage and depth use the exact float/int arithmetic of the CPU oracle's spec; this file
 // is compiled without FP contraction so both produce identical triangle IDs.
 #pragma clang fp contract(off)
-#include "common.hpp"
+#include "internal.hpp"
+#include <algorithm>
 
 namespace {
 
-constexpr int TILE = 16;
+// Tunables (vcl_set_tuning): fine-raster tile side (8, 16 or 32 pixels) and pieces per
+// fine-raster chunk (64, 128 or 256).  Defaults: 16 and 256.
+constexpr int TILE_MAX = 32;
 constexpr size_t BIN_ENTRIES_PER_PIECE = 32;   // bin capacity per piece (average over pieces)
 constexpr size_t BIN_MIN_CAP = (size_t)1 << 22;
 constexpr float RAST_GB = 16.0f;
@@ -51,10 +54,9 @@ static_assert(sizeof(RRec) == 112, "record size");
 constexpr int SMALL_TILES = 4;  // pieces whose box spans more tiles are binned by a whole wave
 constexpr int LARGE_BLOCKS = 512;
 
-// Fine-raster work unit: up to FINE_CHUNK pieces of one tile's bin.  A tile whose bin holds more
+// Fine-raster work unit: up to `chunk` pieces of one tile's bin.  A tile whose bin holds more
 // pieces is split over several chunks (and workgroups); the last of them to finish merges
 // the partial depth keys.
-constexpr u32 FINE_CHUNK = 256;
 constexpr u32 CHUNK_SCAN_ALL = 0x80000000u;   // count flag: scan every piece of every layer
 constexpr int FINE_MAX_GRID = 65536;
 
@@ -70,10 +72,11 @@ struct Layout {
 
 size_t g_bin_cap_override = 0;   // vcl_rasterize_set_bin_capacity (0: sized from the piece count)
 
-Layout make_layout(int W, int H, int depth, int T) {
+Layout make_layout(int W, int H, int depth, int T, int tshift) {
     Layout L;
-    L.tilesX = (W + TILE - 1) / TILE;
-    L.tilesY = (H + TILE - 1) / TILE;
+    const int ts = 1 << tshift;
+    L.tilesX = (W + ts - 1) / ts;
+    L.tilesY = (H + ts - 1) / ts;
     L.ntiles = L.tilesX * L.tilesY;
     L.NT = L.ntiles * depth;
     L.nrec_direct = (size_t)depth * T;
@@ -82,7 +85,7 @@ Layout make_layout(int W, int H, int depth, int T) {
     L.bin_cap = L.nrec_cap * BIN_ENTRIES_PER_PIECE;
     if (L.bin_cap < BIN_MIN_CAP) L.bin_cap = BIN_MIN_CAP;
     if (g_bin_cap_override) L.bin_cap = g_bin_cap_override;
-    L.chunk_cap = 2 * (size_t)L.NT + L.bin_cap / FINE_CHUNK + 1;
+    L.chunk_cap = 2 * (size_t)L.NT + L.bin_cap / 64 + 1;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     size_t o = 0;
     L.off_counts = o; o = al(o + sizeof(u32) * (size_t)L.NT);
@@ -92,7 +95,7 @@ Layout make_layout(int W, int H, int depth, int T) {
     L.off_bins = o; o = al(o + sizeof(u32) * L.bin_cap);
     L.off_large = o; o = al(o + sizeof(u32) * L.nrec_cap);
     L.off_chunks = o; o = al(o + sizeof(Chunk) * L.chunk_cap);
-    L.off_part = o; o = al(o + sizeof(u64) * TILE * TILE * L.chunk_cap);
+    L.off_part = o; o = al(o + sizeof(u64) * ts * ts * L.chunk_cap);
     L.off_done = o; o = al(o + sizeof(u32) * (size_t)L.NT);
     L.total = o;
     return L;
@@ -124,7 +127,8 @@ struct RastArgs {
     const u32* idx;
     float* out;
     float* outDB;
-    int tilesX, ntiles;
+    int tilesX, ntiles, tshift;
+    u32 chunk;
     size_t nrec_direct, ovf_cap, bin_cap, chunk_cap;
     Ws ws;
 };
@@ -216,9 +220,9 @@ __device__ int clip_poly(const float4* in, int n, float4* out, float p0, float p
 // Can any pixel centre of tile (tx, ty) be inside the piece?  Each edge function is maximised
 // over the tile's centre lattice at one corner; the piece misses the tile when that maximum
 // fails the inside test (E > 0, or E == 0 on a top-left edge).
-__device__ __forceinline__ bool tile_overlaps(const RRec& r, int tx, int ty) {
-    long long x0 = (long long)(tx * TILE) * 256 + 128, x1 = x0 + (TILE - 1) * 256;
-    long long y0 = (long long)(ty * TILE) * 256 + 128, y1 = y0 + (TILE - 1) * 256;
+__device__ __forceinline__ bool tile_overlaps(const RRec& r, int tx, int ty, int tshift) {
+    long long x0 = (long long)(tx << tshift) * 256 + 128, x1 = x0 + (long long)((1 << tshift) - 1) * 256;
+    long long y0 = (long long)(ty << tshift) * 256 + 128, y1 = y0 + (long long)((1 << tshift) - 1) * 256;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
         long long X = r.A[i] > 0 ? x1 : x0, Y = r.B[i] > 0 ? y1 : y0;
@@ -228,9 +232,9 @@ __device__ __forceinline__ bool tile_overlaps(const RRec& r, int tx, int ty) {
     return true;
 }
 
-__device__ __forceinline__ int box_tiles(const RRec& r) {
-    int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
-    int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
+__device__ __forceinline__ int box_tiles(const RRec& r, int tshift) {
+    int tx0 = (r.pxr & 0xffff) >> tshift, tx1 = (r.pxr >> 16) >> tshift;
+    int ty0 = (r.pyr & 0xffff) >> tshift, ty1 = (r.pyr >> 16) >> tshift;
     return (tx1 - tx0 + 1) * (ty1 - ty0 + 1);
 }
 
@@ -292,18 +296,18 @@ __device__ __forceinline__ void bin_fill_direct(const RastArgs& a, u32 g, u32 sl
     bin_write(a, g, atomicSub(a.ws.counts + g, 1u) - 1u, slot);
 }
 
-__device__ __forceinline__ bool piece_small(const RRec& r) { return box_tiles(r) <= SMALL_TILES; }
+__device__ __forceinline__ bool piece_small(const RRec& r, int tshift) { return box_tiles(r, tshift) <= SMALL_TILES; }
 
 // Tiles (global index) of a small piece that its pixel-centre lattice can reach; returns the count.
 __device__ __forceinline__ int small_tiles(const RastArgs& a, const RRec& r, u32* g) {
     int view = (int)(r.view_tl >> 8);
-    int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
-    int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
+    int tx0 = (r.pxr & 0xffff) >> a.tshift, tx1 = (r.pxr >> 16) >> a.tshift;
+    int ty0 = (r.pyr & 0xffff) >> a.tshift, ty1 = (r.pyr >> 16) >> a.tshift;
     bool single = tx0 == tx1 && ty0 == ty1;
     int n = 0;
     for (int ty = ty0; ty <= ty1; ty++)
         for (int tx = tx0; tx <= tx1; tx++) {
-            if (!single && !tile_overlaps(r, tx, ty)) continue;
+            if (!single && !tile_overlaps(r, tx, ty, a.tshift)) continue;
             if (n < SMALL_TILES) g[n++] = (u32)(view * a.ntiles + ty * a.tilesX + tx);
         }
     return n;
@@ -323,12 +327,12 @@ __global__ void __launch_bounds__(256) k_bin_large(RastArgs a) {
         u32 slot = a.ws.large[i];
         const RRec& r = a.ws.rec[slot];
         int view = (int)(r.view_tl >> 8);
-        int tx0 = (r.pxr & 0xffff) / TILE, tx1 = (r.pxr >> 16) / TILE;
-        int ty0 = (r.pyr & 0xffff) / TILE, ty1 = (r.pyr >> 16) / TILE;
+        int tx0 = (r.pxr & 0xffff) >> a.tshift, tx1 = (r.pxr >> 16) >> a.tshift;
+        int ty0 = (r.pyr & 0xffff) >> a.tshift, ty1 = (r.pyr >> 16) >> a.tshift;
         int ntx = tx1 - tx0 + 1, n = ntx * (ty1 - ty0 + 1);
         for (int k = lane; k < n; k += 64) {
             int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
-            if (!tile_overlaps(r, tx, ty)) continue;
+            if (!tile_overlaps(r, tx, ty, a.tshift)) continue;
             u32 g = (u32)(view * a.ntiles + ty * a.tilesX + tx);
             if (!FILL) atomicAdd(a.ws.counts + g, 1u);
             else bin_fill_direct(a, g, slot);
@@ -347,7 +351,7 @@ struct SetupShared {
 };
 
 __device__ __forceinline__ void bin_or_defer(const RastArgs& a, SetupShared& sh, const RRec& r, u32 slot) {
-    if (piece_small(r)) {
+    if (piece_small(r, a.tshift)) {
         u32 g[SMALL_TILES];
         int n = small_tiles(a, r, g);
         for (int i = 0; i < n; i++) bin_count(a, sh.tab, g[i]);
@@ -453,7 +457,7 @@ __device__ u32 scan1024(u32 v, u32* wsum, u32* carry) {
 }
 
 // Single workgroup: exclusive scan of the NT tile counters into bin offsets, then the fine-raster
-// work list: each tile's bin cut into chunks of FINE_CHUNK pieces (one chunk for an empty tile,
+// work list: each tile's bin cut into chunks of `chunk` pieces (one chunk for an empty tile,
 // and one scan-everything chunk for a tile whose bin would pass the bin capacity).
 __global__ void __launch_bounds__(1024) k_scan(RastArgs a, int NT) {
     __shared__ u32 wsum[16];
@@ -468,15 +472,15 @@ __global__ void __launch_bounds__(1024) k_scan(RastArgs a, int NT) {
         u32 pre = scan1024(v, wsum, &carry[0]);
         if (i < NT) a.ws.offsets[i] = pre;
         bool all = (size_t)pre + v > a.bin_cap;
-        u32 nch = i >= NT ? 0u : all ? 1u : max(1u, (v + FINE_CHUNK - 1) / FINE_CHUNK);
+        u32 nch = i >= NT ? 0u : all ? 1u : max(1u, (v + a.chunk - 1) / a.chunk);
         u32 cpre = scan1024(nch, wsum, &carry[1]);
         for (u32 k = 0; k < nch; k++) {
             size_t c = (size_t)cpre + k;
             if (c >= a.chunk_cap) break;
             Chunk ch;
             ch.tile = (u32)i;
-            ch.first = all ? 0u : pre + k * FINE_CHUNK;
-            ch.count = all ? (nrec | CHUNK_SCAN_ALL) : min(FINE_CHUNK, v - k * FINE_CHUNK);
+            ch.first = all ? 0u : pre + k * a.chunk;
+            ch.count = all ? (nrec | CHUNK_SCAN_ALL) : min(a.chunk, v - k * a.chunk);
             ch.k = k;
             a.ws.chunks[c] = ch;
         }
@@ -506,7 +510,7 @@ __global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
         const RRec& r = a.ws.rec[slot];
         if (r.id != INVALID) {
             RRec rr = r;
-            if (piece_small(rr)) n = small_tiles(a, rr, g);
+            if (piece_small(rr, a.tshift)) n = small_tiles(a, rr, g);
         }
     }
     for (int i = 0; i < n; i++) {
@@ -583,17 +587,20 @@ __device__ void fine_resolve(const RastArgs& a, u32 view, int px, int py, float 
     if (a.outDB) ((float4*)a.outDB)[p] = d;
 }
 
-// One 256-thread workgroup per work-list chunk (up to FINE_CHUNK pieces of one 16x16 tile's
-// bin, so a deep bin is spread over several workgroups).  Thread i takes piece i of the chunk
-// and walks its exact covered column span in each of its rows (bisection on the integer edge
+// One 256-thread workgroup per work-list chunk (up to `chunk` pieces of one TS x TS tile's bin,
+// so a deep bin is spread over several workgroups).  Thread i takes piece i of the chunk and
+// walks its exact covered column span in each of its rows (bisection on the integer edge
 // functions, so no per-pixel edge test), evaluating the piece's depth planes at each covered
 // pixel centre and taking the LDS 64-bit minimum of the (ordered z_ndc, triangle id) key; the
-// result therefore does not depend on bin order or on how the bin is split.  A tile with one
-// chunk resolves its pixels at once; otherwise each chunk stores its 256 partial keys and the
-// last chunk of the tile to finish (device-scope counter) takes the minimum over all of them.
+// result therefore does not depend on bin order, tile size or how the bin is split.  A tile
+// with one chunk resolves its pixels at once; otherwise each chunk stores its TS*TS partial
+// keys and the last chunk of the tile to finish (device-scope counter) takes the minimum over
+// all of them.
+template <int TS>
 __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
-    __shared__ u64 key[256];
-    __shared__ float spix[2][TILE];
+    constexpr int NP = TS * TS;
+    __shared__ u64 key[NP];
+    __shared__ float spix[2][TS];
     __shared__ u32 s_last;
     const int tid = threadIdx.x;
     const u32 nchunks = a.ws.misc[5];
@@ -603,13 +610,13 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
         const u32 view = (u32)(tile / a.ntiles);
         const int tl = tile % a.ntiles;
         const int tx = tl % a.tilesX, ty = tl / a.tilesX;
-        key[tid] = ~0ull;
-        if (tid < 2 * TILE) {
-            int cc = tid & (TILE - 1);
-            spix[tid >> 4][cc] = tid < TILE ? (float)((tx * TILE + cc + 0.5) * (double)a.xs - 1.0)
-                                            : (float)((ty * TILE + cc + 0.5) * (double)a.ys - 1.0);
+        for (int i = tid; i < NP; i += 256) key[i] = ~0ull;
+        if (tid < 2 * TS) {
+            int cc = tid % TS;
+            spix[tid / TS][cc] = tid < TS ? (float)((tx * TS + cc + 0.5) * (double)a.xs - 1.0)
+                                          : (float)((ty * TS + cc + 0.5) * (double)a.ys - 1.0);
         }
-        const int x0 = tx * TILE, y0 = ty * TILE;
+        const int x0 = tx * TS, y0 = ty * TS;
         const long long X0 = (long long)x0 * 256 + 128;
         const bool scan_all = (ch.count & CHUNK_SCAN_ALL) != 0u;
         const u32 n = ch.count & ~CHUNK_SCAN_ALL;
@@ -618,8 +625,8 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
             u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
             const RRec& r = a.ws.rec[slot];
             u32 id = r.id, vtl = r.view_tl;
-            int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TILE - 1);
-            int yl = max((r.pyr & 0xffff) - y0, 0), yh = min((r.pyr >> 16) - y0, TILE - 1);
+            int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
+            int yl = max((r.pyr & 0xffff) - y0, 0), yh = min((r.pyr >> 16) - y0, TS - 1);
             if (id == INVALID || (vtl >> 8) != view || xl > xh || yl > yh) continue;
             float zp[9];
 #pragma unroll
@@ -648,18 +655,17 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
                     if (!((Wp > 0.0f && Dp > 0.0f) || (Wp < 0.0f && Dp < 0.0f))) continue;
                     float z = Zp / Wp;
                     if (!(z >= -1.0f && z <= 1.0f)) continue;
-                    atomicMin(&key[y * TILE + x], ((u64)ord_f32(z) << 32) | id);
+                    atomicMin(&key[y * TS + x], ((u64)ord_f32(z) << 32) | id);
                 }
             }
         }
         __syncthreads();
-        u64 best = key[tid];
         bool resolve = true;
         if (!scan_all) {
             const u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
-            const u32 nch = max(1u, (end - begin + FINE_CHUNK - 1) / FINE_CHUNK);
+            const u32 nch = max(1u, (end - begin + a.chunk - 1) / a.chunk);
             if (nch > 1) {
-                a.ws.part[(size_t)c * 256 + tid] = best;
+                for (int i = tid; i < NP; i += 256) a.ws.part[(size_t)c * NP + i] = key[i];
                 __threadfence();
                 __syncthreads();
                 if (tid == 0) s_last = atomicAdd(a.ws.done + tile, 1u) == nch - 1 ? 1u : 0u;
@@ -668,16 +674,21 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
                 if (resolve) {
                     __threadfence();
                     const u32 c0 = c - ch.k;
-                    for (u32 i = 0; i < nch; i++) {
-                        if (c0 + i == c) continue;
-                        u64 o = a.ws.part[(size_t)(c0 + i) * 256 + tid];
-                        best = o < best ? o : best;
+                    for (int i = tid; i < NP; i += 256) {
+                        u64 best = key[i];
+                        for (u32 j = 0; j < nch; j++) {
+                            if (c0 + j == c) continue;
+                            u64 o = a.ws.part[(size_t)(c0 + j) * NP + i];
+                            best = o < best ? o : best;
+                        }
+                        key[i] = best;   // each thread rewrites only its own entries
                     }
                 }
             }
         }
         if (resolve)
-            fine_resolve(a, view, x0 + (tid & (TILE - 1)), y0 + (tid >> 4), spix[0][tid & (TILE - 1)], spix[1][tid >> 4], best);
+            for (int i = tid; i < NP; i += 256)
+                fine_resolve(a, view, x0 + i % TS, y0 + i / TS, spix[0][i % TS], spix[1][i / TS], key[i]);
         __syncthreads();
     }
 }
@@ -752,10 +763,15 @@ __global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float x
 
 }  // namespace
 
+static int tile_shift_of(int tile) { return tile == 8 ? 3 : tile == 32 ? 5 : 4; }
+
+// The workspace fits every tile size, so the tuning may change between calls.
 extern "C" size_t vcl_rasterize_workspace_size(int width, int height, int depth, int projNum, int idxNum) {
     (void)projNum;
     if (width <= 0 || height <= 0 || depth <= 0 || idxNum < 0) return 0;
-    return make_layout(width, height, depth, idxNum).total;
+    size_t m = 0;
+    for (int sh = 3; sh <= 5; sh++) m = std::max(m, make_layout(width, height, depth, idxNum, sh).total);
+    return m;
 }
 
 extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t stream) {
@@ -763,7 +779,8 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     VCL_REQUIRE(p->width > 0 && p->height > 0 && p->depth > 0 && p->width <= 16384 && p->height <= 16384);
     VCL_REQUIRE(p->idxNum >= 0 && p->idxNum < (1 << 24) && (p->projNum > 0 || p->idxNum == 0) && p->depth < (1 << 23));
     VCL_REQUIRE(!p->enableDB || p->outDB);
-    Layout L = make_layout(p->width, p->height, p->depth, p->idxNum);
+    const int tshift = tile_shift_of(g_vcl_tuning.raster_tile);
+    Layout L = make_layout(p->width, p->height, p->depth, p->idxNum, tshift);
     if (p->workspace_size < L.total) return VCL_ERR_OOM;
     hipStream_t s = as_hip(stream);
     RastArgs a;
@@ -771,7 +788,7 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     a.enableDB = p->enableDB;
     a.xs = 2.f / (float)p->width; a.ys = 2.f / (float)p->height;
     a.proj = p->proj; a.idx = p->idx; a.out = p->out; a.outDB = p->enableDB ? p->outDB : nullptr;
-    a.tilesX = L.tilesX; a.ntiles = L.ntiles;
+    a.tilesX = L.tilesX; a.ntiles = L.ntiles; a.tshift = tshift; a.chunk = (u32)g_vcl_tuning.bin_chunk;
     a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap; a.bin_cap = L.bin_cap; a.chunk_cap = L.chunk_cap;
     a.ws = ws_ptrs(p->workspace, L);
     int nclear = L.NT > 64 ? L.NT : 64;
@@ -791,7 +808,10 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     }
     hipLaunchKernelGGL(k_bin_large<true>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fine, dim3((unsigned)std::min(L.chunk_cap, (size_t)FINE_MAX_GRID)), dim3(256), 0, s, a);
+    const dim3 fg((unsigned)std::min(L.chunk_cap, (size_t)FINE_MAX_GRID));
+    if (tshift == 3) hipLaunchKernelGGL(k_fine<8>, fg, dim3(256), 0, s, a);
+    else if (tshift == 5) hipLaunchKernelGGL(k_fine<32>, fg, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_fine<16>, fg, dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -800,7 +820,7 @@ extern "C" void vcl_rasterize_set_bin_capacity(size_t entries) { g_bin_cap_overr
 
 extern "C" int vcl_rasterize_status(const vcl_rasterize_params* p, vcl_stream_t stream) {
     VCL_REQUIRE(p && p->workspace);
-    Layout L = make_layout(p->width, p->height, p->depth, p->idxNum);
+    Layout L = make_layout(p->width, p->height, p->depth, p->idxNum, tile_shift_of(g_vcl_tuning.raster_tile));
     u32 misc[3];
     VCL_HIP_CHECK(hipMemcpyAsync(misc, (char*)p->workspace + L.off_misc, sizeof(misc), hipMemcpyDeviceToHost, as_hip(stream)));
     VCL_HIP_CHECK(hipStreamSynchronize(as_hip(stream)));

@@ -13,6 +13,7 @@
 // steady state.
 #include "internal.hpp"
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -100,6 +101,7 @@ struct vcl_server {
     std::vector<Req*> active;               // admitted, stepping
     std::vector<Req*> finished;             // all iterations done (arena held until release)
     std::vector<hipStream_t> streams;       // idle streams for reuse
+    std::map<uint64_t, vcl_tuning> tunings; // autotuned schedule per size class
     double occ_integral = 0, occ_last_t = 0;
 
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
@@ -116,6 +118,14 @@ static void release_req(vcl_server* S, Req* r) {
     if (r->ev) { (void)hipEventDestroy(r->ev); r->ev = nullptr; }
     if (r->stream) { S->streams.push_back(r->stream); r->stream = nullptr; }
     if (r->arena) { occupancy_sample(S); S->pool.put(r->arena); r->arena = nullptr; }
+}
+
+// Size class of a request for the tuning table: pixels over all views in buckets of 4x
+// (256^2, 512^2, 1024^2, 2048^2 ...) and triangles in buckets of 8x; the tuned kernels
+// (raster tiling, texel splats) depend on these, not on the texture size or lobe count.
+static uint64_t size_class(const vcl_fit_desc* d) {
+    auto bucket = [](double x, double base) { return (uint64_t)std::lround(std::log(x < 1.0 ? 1.0 : x) / std::log(base)); };
+    return bucket((double)d->width * d->height * d->views, 4.0) | bucket((double)d->T, 8.0) << 8;
 }
 
 extern "C" int vcl_server_create(const vcl_server_config* c, vcl_server** out) {
@@ -230,6 +240,18 @@ extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
             release_req(S, r);
             continue;
         }
+        if (S->cfg.autotune_steps > 0) {
+            // the schedule of this size class: tuned on its first request, reused afterwards
+            uint64_t key = size_class(&r->d);
+            auto it = S->tunings.find(key);
+            if (it != S->tunings.end()) st = vcl_fit_set_tuning(r->fit, &it->second);
+            else {
+                vcl_tuning t;
+                st = vcl_fit_autotune(r->fit, S->cfg.autotune_steps, &t, nullptr);
+                if (!st) S->tunings[key] = t;
+            }
+            if (st) return st;
+        }
         r->admit = S->now();
         S->active.push_back(r);
     }
@@ -325,3 +347,29 @@ extern "C" int vcl_server_memory(vcl_server* S, size_t* in_use, size_t* peak, si
     if (mean_in_use) *mean_in_use = S->occ_last_t > 0 ? S->occ_integral / S->occ_last_t : 0.0;
     return VCL_OK;
 }
+
+extern "C" int vcl_server_tunings(vcl_server* S, uint64_t* keys, vcl_tuning* out, int max, int* n) {
+    VCL_REQUIRE(S && n && (max == 0 || (keys && out)));
+    int k = 0;
+    for (auto& kv : S->tunings) {
+        if (k >= max) break;
+        keys[k] = kv.first;
+        out[k] = kv.second;
+        k++;
+    }
+    *n = k;
+    return VCL_OK;
+}
+
+extern "C" int vcl_server_load_tunings(vcl_server* S, const uint64_t* keys, const vcl_tuning* t, int n) {
+    VCL_REQUIRE(S && n >= 0 && (n == 0 || (keys && t)));
+    for (int i = 0; i < n; i++) {
+        vcl_tuning saved = g_vcl_tuning;
+        int st = vcl_set_tuning(&t[i]);   // validates
+        g_vcl_tuning = saved;
+        if (st) return st;
+        S->tunings[keys[i]] = t[i];
+    }
+    return VCL_OK;
+}
+

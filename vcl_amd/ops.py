@@ -316,6 +316,18 @@ def fit_desc(scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_qua
     return d, keep
 
 
+# ---------------------------------------------------------------- tuning (vcl_set_tuning)
+def set_tuning(raster_tile=16, bin_chunk=256, splat_rows=16):
+    t = L.Tuning(raster_tile, bin_chunk, splat_rows)
+    L.check(_lib().vcl_set_tuning(C.byref(t)), "set_tuning")
+
+
+def get_tuning():
+    t = L.Tuning()
+    L.check(_lib().vcl_get_tuning(C.byref(t)), "get_tuning")
+    return {"raster_tile": t.raster_tile, "bin_chunk": t.bin_chunk, "splat_rows": t.splat_rows}
+
+
 class Fit:
     """One device-resident fit request (vcl_fit_* of include/vcl.h)."""
 
@@ -343,6 +355,22 @@ class Fit:
 
     def set_target_from_render(self):
         L.check(_lib().vcl_fit_set_target_from_render(self.h, _s()), "fit_set_target")
+
+    def set_tuning(self, raster_tile=16, bin_chunk=256, splat_rows=16):
+        t = L.Tuning(raster_tile, bin_chunk, splat_rows)
+        L.check(_lib().vcl_fit_set_tuning(self.h, C.byref(t)), "fit_set_tuning")
+
+    def tuning(self):
+        t = L.Tuning()
+        L.check(_lib().vcl_fit_get_tuning(self.h, C.byref(t)), "fit_get_tuning")
+        return {"raster_tile": t.raster_tile, "bin_chunk": t.bin_chunk, "splat_rows": t.splat_rows}
+
+    def autotune(self, steps=3):
+        """Pick the fastest kernel schedule for this request (state restored afterwards)."""
+        t = L.Tuning()
+        ms = C.c_double()
+        L.check(_lib().vcl_fit_autotune(self.h, int(steps), C.byref(t), C.byref(ms)), "fit_autotune")
+        return {"raster_tile": t.raster_tile, "bin_chunk": t.bin_chunk, "splat_rows": t.splat_rows, "ms_per_step": ms.value}
 
     def step(self, graph=False):
         if graph:

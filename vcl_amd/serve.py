@@ -28,9 +28,9 @@ class Server:
     FIFO, SRPT = 0, 1
 
     def __init__(self, hbm_budget=200 * GiB, max_active=8, use_graph=True, steps_per_round=2, auto_release=True,
-                 policy=0, max_stepping=0):
+                 policy=0, max_stepping=0, autotune_steps=0):
         self.cfg = L.ServerConfig(int(hbm_budget), int(max_active), 1 if use_graph else 0, int(steps_per_round),
-                                  1 if auto_release else 0, int(policy), int(max_stepping))
+                                  1 if auto_release else 0, int(policy), int(max_stepping), int(autotune_steps))
         h = C.c_void_p()
         L.check(L.load().vcl_server_create(C.byref(self.cfg), C.byref(h)), "server_create")
         self.h = h
@@ -91,6 +91,25 @@ class Server:
 
     def release(self, ticket):
         L.check(L.load().vcl_server_release(self.h, int(ticket)), "server_release")
+
+    def tuning_table(self):
+        """[(size-class key, (raster_tile, bin_chunk, splat_rows))] of the autotuner."""
+        keys = (C.c_uint64 * 256)()
+        ts = (L.Tuning * 256)()
+        n = C.c_int()
+        L.check(L.load().vcl_server_tunings(self.h, keys, ts, 256, C.byref(n)), "server_tunings")
+        return [(keys[i], (ts[i].raster_tile, ts[i].bin_chunk, ts[i].splat_rows)) for i in range(n.value)]
+
+    def load_tuning_table(self, table):
+        n = len(table)
+        keys = (C.c_uint64 * max(n, 1))(*[k for k, _ in table])
+        ts = (L.Tuning * max(n, 1))(*[L.Tuning(*t) for _, t in table])
+        L.check(L.load().vcl_server_load_tunings(self.h, keys, ts, n), "server_load_tunings")
+
+    def tunings(self):
+        """The tuning table keyed by a readable size class."""
+        return {f"px~4^{k & 255} tri~8^{(k >> 8) & 255}": {"raster_tile": t[0], "bin_chunk": t[1], "splat_rows": t[2]}
+                for k, t in self.tuning_table()}
 
     def memory(self):
         a, b, c = C.c_size_t(), C.c_size_t(), C.c_size_t()
