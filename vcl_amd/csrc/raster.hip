@@ -525,6 +525,8 @@ __global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
         if (sl[i] >= 0) bin_write(a, g[i], tab.val[sl[i]] + loc[i], (u32)slot);
 }
 
+constexpr u32 SPAN_EMPTY = 0x00ffu;   // first column 255 > last column 0
+
 // Covered pixel columns of one piece in one row of a tile.  Edge i passes at column x (pixel
 // centre X = X0 + 256 x) iff K_i + dA_i x > 0, with K_i = A_i X0 + B_i Y + C_i + tl_i (the
 // integer form of "E > 0, or E == 0 on a top-left edge") and dA_i = 256 A_i; each edge is
@@ -602,6 +604,13 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
     __shared__ u64 key[NP];
     __shared__ float spix[2][TS];
     __shared__ u32 s_last;
+    constexpr int BATCH = TS == 32 ? 128 : 256;    // pieces per LDS batch
+    __shared__ unsigned short sspan[BATCH][TS];     // per piece: first | last << 8 column per row
+    __shared__ float szp[9][BATCH];
+    __shared__ u32 sid[BATCH];
+    __shared__ u32 sitem[BATCH * TS];               // covered rows, ordered by span length
+    __shared__ u32 sbucket[TS + 1], sbase[TS + 1];
+    __shared__ u32 snitems;
     const int tid = threadIdx.x;
     const u32 nchunks = a.ws.misc[5];
     for (u32 c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -611,6 +620,7 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
         const int tl = tile % a.ntiles;
         const int tx = tl % a.tilesX, ty = tl / a.tilesX;
         for (int i = tid; i < NP; i += 256) key[i] = ~0ull;
+        if (tid <= TS) sbucket[tid] = 0;
         if (tid < 2 * TS) {
             int cc = tid % TS;
             spix[tid / TS][cc] = tid < TS ? (float)((tx * TS + cc + 0.5) * (double)a.xs - 1.0)
@@ -621,45 +631,83 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
         const bool scan_all = (ch.count & CHUNK_SCAN_ALL) != 0u;
         const u32 n = ch.count & ~CHUNK_SCAN_ALL;
         __syncthreads();
-        for (u32 q = tid; q < n; q += 256) {
-            u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
-            const RRec& r = a.ws.rec[slot];
-            u32 id = r.id, vtl = r.view_tl;
-            int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
-            int yl = max((r.pyr & 0xffff) - y0, 0), yh = min((r.pyr >> 16) - y0, TS - 1);
-            if (id == INVALID || (vtl >> 8) != view || xl > xh || yl > yh) continue;
-            float zp[9];
+        for (u32 base = 0; base < n; base += BATCH) {
+            // phase 1: thread = piece; exact row spans and depth planes into LDS; the covered
+            // rows are counted per span length
+            const u32 q = base + tid;
+            int yl = 1, yh = 0;
+            if (tid < BATCH && q < n) {
+                u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
+                const RRec& r = a.ws.rec[slot];
+                u32 id = r.id, vtl = r.view_tl;
+                int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
+                yl = max((r.pyr & 0xffff) - y0, 0);
+                yh = min((r.pyr >> 16) - y0, TS - 1);
+                if (id != INVALID && (vtl >> 8) == view && xl <= xh && yl <= yh) {
+                    long long K[3], dA[3], dB[3];
+                    long long Y = (long long)(y0 + yl) * 256 + 128;
 #pragma unroll
-            for (int k = 0; k < 9; k++) zp[k] = r.zp[k];
-            long long K[3], dA[3], dB[3];
-            long long Y = (long long)(y0 + yl) * 256 + 128;
+                    for (int i = 0; i < 3; i++) {
+                        K[i] = (long long)r.A[i] * X0 + (long long)r.B[i] * Y + r.C[i] + ((vtl >> i) & 1u);
+                        dA[i] = (long long)r.A[i] * 256;
+                        dB[i] = (long long)r.B[i] * 256;
+                    }
+                    for (int y = yl; y <= yh; y++) {
+                        int lo = xl, hi = xh;
+                        row_span(K, dA, lo, hi);
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                K[i] = (long long)r.A[i] * X0 + (long long)r.B[i] * Y + r.C[i] + ((vtl >> i) & 1u);
-                dA[i] = (long long)r.A[i] * 256;
-                dB[i] = (long long)r.B[i] * 256;
+                        for (int i = 0; i < 3; i++) K[i] += dB[i];
+                        sspan[tid][y] = lo <= hi ? (unsigned short)(lo | (hi << 8)) : (unsigned short)SPAN_EMPTY;
+                        if (lo <= hi) atomicAdd(&sbucket[hi - lo + 1], 1u);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 9; k++) szp[k][tid] = r.zp[k];
+                    sid[tid] = id;
+                } else {
+                    yl = 1; yh = 0;
+                }
             }
+            __syncthreads();
+            if (tid == 0) {   // bucket bases, longest spans first
+                u32 acc = 0;
+                for (int i = TS; i >= 1; i--) { u32 c2 = sbucket[i]; sbase[i] = acc; sbucket[i] = 0; acc += c2; }
+                snitems = acc;
+            }
+            __syncthreads();
+            // phase 2: every covered row becomes an item (piece, row, first column, length) at its
+            // length bucket's position
             for (int y = yl; y <= yh; y++) {
-                int lo = xl, hi = xh;
-                row_span(K, dA, lo, hi);
-#pragma unroll
-                for (int i = 0; i < 3; i++) K[i] += dB[i];
+                u32 sp = sspan[tid][y];
+                int lo = (int)(sp & 0xffu), hi = (int)(sp >> 8);
                 if (lo > hi) continue;
+                int len = hi - lo + 1;
+                u32 pos = sbase[len] + atomicAdd(&sbucket[len], 1u);
+                sitem[pos] = (u32)tid | ((u32)y << 8) | ((u32)lo << 16) | ((u32)len << 24);
+            }
+            __syncthreads();
+            if (tid <= TS) sbucket[tid] = 0;   // counters of the next batch (not read below)
+            // phase 3: thread = item; a wave's items have (nearly) the same length
+            const u32 ni = snitems;
+            for (u32 i = tid; i < ni; i += 256) {
+                const u32 it = sitem[i];
+                const int j = (int)(it & 0xffu), y = (int)((it >> 8) & 0xffu), lo = (int)((it >> 16) & 0xffu), len = (int)(it >> 24);
                 const float pixy = spix[1][y];
-                const float zy = zp[1] * pixy, wy = zp[4] * pixy, dy = zp[7] * pixy;
-                for (int x = lo; x <= hi; x++) {
+                const float z0 = szp[0][j], z2 = szp[2][j], w0 = szp[3][j], w2 = szp[5][j], d0 = szp[6][j], d2 = szp[8][j];
+                const float zy = szp[1][j] * pixy, wy = szp[4][j] * pixy, dy = szp[7][j] * pixy;
+                const u32 id = sid[j];
+                for (int x = lo; x < lo + len; x++) {
                     const float pixx = spix[0][x];
-                    float Zp = zp[0] * pixx + zy + zp[2];
-                    float Wp = zp[3] * pixx + wy + zp[5];
-                    float Dp = zp[6] * pixx + dy + zp[8];
+                    float Zp = z0 * pixx + zy + z2;
+                    float Wp = w0 * pixx + wy + w2;
+                    float Dp = d0 * pixx + dy + d2;
                     if (!((Wp > 0.0f && Dp > 0.0f) || (Wp < 0.0f && Dp < 0.0f))) continue;
                     float z = Zp / Wp;
                     if (!(z >= -1.0f && z <= 1.0f)) continue;
                     atomicMin(&key[y * TS + x], ((u64)ord_f32(z) << 32) | id);
                 }
             }
+            __syncthreads();
         }
-        __syncthreads();
         bool resolve = true;
         if (!scan_all) {
             const u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
