@@ -335,6 +335,30 @@ def test_spherical_gaussian(gpu, cfg1):
         assert np.array_equal(a_, b_), "lobe grads must be deterministic"
 
 
+def test_spherical_gaussian_adversarial(gpu, cfg1):
+    # non-unit normals and lobe axes, very sharp specular lobes at some pixels, and NaN / Inf
+    # amplitudes (AddNaNcheck drops those terms per channel): the fast C = 4 forward must still
+    # match the oracle
+    sc, d = cfg1
+    rng = np.random.default_rng(17)
+    nrm = d["nrm"] * rng.uniform(0.3, 3.0, d["nrm"].shape[:-1] + (1,)).astype(F32)
+    axis = (sc.axis * rng.uniform(0.2, 2.0, (sc.K, 1))).astype(F32)
+    ss = d["sSharp"].copy()
+    ss[..., ::7] = 1.0e5
+    for amp_case in ("finite", "nonfinite"):
+        amp = sc.amp.copy()
+        if amp_case == "nonfinite":
+            amp[3, 1] = np.nan
+            amp[5, 2] = np.inf
+        ref = o_sg(d["rast"], axis, sc.sharp, amp, nrm, d["diff"], d["sAxis"], ss, d["sAmp"])
+        out, de, se = gpu.spherical_gaussian(T(d["rast"]), T(axis), T(sc.sharp), T(amp), T(nrm), T(d["diff"]), T(d["sAxis"]), T(ss),
+                                             T(d["sAmp"]))
+        for name, g, r in zip(("out", "diffenv", "specenv"), (out, de, se), ref):
+            g = N(g)
+            assert np.array_equal(np.isfinite(g), np.isfinite(r)), f"sg {name} ({amp_case}): finiteness differs"
+            assert_close(np.nan_to_num(g), np.nan_to_num(r), 1e-5, f"sg {name} ({amp_case})")
+
+
 # ------------------------------------------------------------------ antialias / loss
 def test_antialias(gpu, cfg1):
     import torch

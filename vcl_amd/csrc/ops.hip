@@ -988,6 +988,79 @@ __global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, cons
     }
 }
 
+// C == 4 forward.  Per lobe: two 16-byte LDS reads (axis * sharpness, sharpness; amplitude).
+// AddNaNcheck per channel reduces to one finiteness test per term: the term is bounded
+// (e^{l-c} <= 1 since l <= c, and 1/max(l, 1e-6) <= 1e6), so a finite term times finite
+// amplitudes always passes the per-channel check and a non-finite one always fails it; a block
+// whose lobes hold a non-finite amplitude takes the per-channel path.  1/l and l come from one
+// rsq.  e^{-l-c} is dropped when it is below half an ulp of e^{l-c} for every lane of the wave
+// (l > 8.75 gives e^{-2l} < 2^-25), from l = |w + lam*mu| >= | |lam*mu| - |w| | with |w| the
+// pixel's 2.133*normal or spec-sharpness*spec-axis length (both lengths carry a relative
+// margin for their float rounding).
+struct SgLobe4 { float4 g; float4 a; float len; float pad[3]; };
+
+__device__ __forceinline__ float sg_term_rsq(float wx, float wy, float wz, float base, float lam, bool both) {
+    float d = wx * wx + wy * wy + wz * wz;
+    float il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
+    float l = d * il;
+    float e1 = nexp2(((l - base) - lam) * L2E);
+    float e2 = both ? nexp2(((-l - base) - lam) * L2E) : 0.f;
+    return (e1 - e2) * il;
+}
+
+constexpr float SG_E2_DROP = 8.75f;
+
+__global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
+                                                const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
+                                                const float* sAmp, float* out, float* denv, float* senv) {
+    __shared__ SgLobe4 lobes[SG_MAXK];
+    __shared__ int nonfinite;
+    if (threadIdx.x == 0) nonfinite = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < K; i += blockDim.x) {
+        float lam = sharp[i];
+        float4 a = make_float4(amp[4 * i], amp[4 * i + 1], amp[4 * i + 2], amp[4 * i + 3]);
+        float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
+        lobes[i].g = make_float4(gx, gy, gz, lam);
+        lobes[i].a = a;
+        lobes[i].len = sqrtf(gx * gx + gy * gy + gz * gz);
+        if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(a.w))) nonfinite = 1;
+    }
+    __syncthreads();
+    const bool checked = nonfinite != 0;
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float4 de = make_float4(0.f, 0.f, 0.f, 0.f), se = de;
+    bool cov = (int)rast[4 * p + 3] - 1 >= 0;
+    if (cov) {
+        float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
+        float ss = sSharp[p], sa = sAmp[p];
+        float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
+        const float dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz), slen = sqrtf(snx * snx + sny * sny + snz * snz);
+        for (int i = 0; i < K; i++) {
+            const float4 g = lobes[i].g, a = lobes[i].a;
+            const float gl = lobes[i].len;
+            const bool dboth = !__all(fabsf(gl - dlen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + dlen));
+            const bool sboth = !__all(fabsf(gl - slen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + slen));
+            float dg = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, SG_DS, g.w, dboth) * SG_DA;
+            float sg = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ss, g.w, sboth) * sa;
+            if (!checked) {
+                float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
+                de.x += dv * a.x; de.y += dv * a.y; de.z += dv * a.z; de.w += dv * a.w;
+                se.x += sv * a.x; se.y += sv * a.y; se.z += sv * a.z; se.w += sv * a.w;
+            } else {
+                acc_checked(de.x, dg * a.x); acc_checked(de.y, dg * a.y); acc_checked(de.z, dg * a.z); acc_checked(de.w, dg * a.w);
+                acc_checked(se.x, sg * a.x); acc_checked(se.y, sg * a.y); acc_checked(se.z, sg * a.z); acc_checked(se.w, sg * a.w);
+            }
+        }
+    }
+    float4 df = cov ? ((const float4*)diffuse)[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    ((float4*)out)[p] = cov ? make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+    ((float4*)denv)[p] = de;
+    ((float4*)senv)[p] = se;
+}
+
 // Lobe-parallel backward for the lobe gradients (axis, sharpness, amplitude).  Thread t owns
 // lobe t % KP for pixel sub-group t / KP; covered pixels of a 256-pixel chunk are compacted into
 // LDS (deterministic order), every wave walks them in the same order (LDS broadcast), and the
@@ -1617,7 +1690,12 @@ extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_param
     VCL_REQUIRE(p->sgnum == 0 || (p->axis && p->sharpness && p->amplitude));
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    if (p->sgnum <= SG_MAXK)
+    const bool al16 = ((uintptr_t)p->diffuse % 16 == 0) && ((uintptr_t)p->out % 16 == 0) && ((uintptr_t)p->outDiffenv % 16 == 0) &&
+                      ((uintptr_t)p->outSpecenv % 16 == 0);
+    if (p->sgnum <= SG_MAXK && p->channel == 4 && al16)
+        hipLaunchKernelGGL(k_sg_fwd4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness, p->amplitude,
+                           p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv, p->outSpecenv);
+    else if (p->sgnum <= SG_MAXK)
         hipLaunchKernelGGL(k_sg_fwd_fast, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
                            p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
                            p->outSpecenv);
