@@ -253,6 +253,7 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: one 256 MiB device copy (counter calibration) then --probe-steps fit steps")
     ap.add_argument("--probe-steps", type=int, default=3)
+    ap.add_argument("--probe-batched", action="store_true", help="--pmc-probe on the batched multi-view request instead")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel schedule (no autotuner)")
     ap.add_argument("--served", default="50", help="comma-separated arrival rates (req/s) for the served measurement; '' skips it")
@@ -281,6 +282,19 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.pmc_probe and args.probe_batched:
+        sc = S.config3_scene(1, W=512, views=args.batched_views)
+        tf = ops.Fit(S.config3_scene(2, W=512, views=args.batched_views))
+        tf.set_target_from_render()
+        target = tf.read("image", (args.batched_views, 512, 512, sc.C))
+        tf.close()
+        pf = ops.Fit(sc, target=target)
+        for _ in range(args.probe_steps):
+            pf.step()
+        torch.cuda.synchronize()
+        pf.close()
+        print(json.dumps({"pmc_probe": "done"}))
+        return
     sc = S.config2_scene(1, W=args.res)
     tsc = S.config2_scene(2, W=args.res)
     tf = ops.Fit(tsc)

@@ -12,15 +12,16 @@ __global__ void vcl_fill_u32(u32* p, size_t n, u32 v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-int vcl_fill_async(void* p, size_t bytes, hipStream_t s) {
-    if (!p || bytes == 0) return VCL_OK;
-    size_t n = bytes / 4;
+int vcl_fill_u32_async(u32* p, size_t n, u32 v, hipStream_t s) {
+    if (!p || n == 0) return VCL_OK;
     unsigned g = cdiv(n, 256);
     if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(vcl_fill_u32, dim3(g), dim3(256), 0, s, (u32*)p, n, 0u);
+    hipLaunchKernelGGL(vcl_fill_u32, dim3(g), dim3(256), 0, s, p, n, v);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
+
+int vcl_fill_async(void* p, size_t bytes, hipStream_t s) { return vcl_fill_u32_async((u32*)p, bytes / 4, 0u, s); }
 
 namespace {
 

@@ -67,7 +67,7 @@ struct Chunk {
 struct Layout {
     int tilesX, tilesY, ntiles, NT;
     size_t nrec_direct, ovf_cap, nrec_cap, bin_cap, chunk_cap;
-    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_large, off_chunks, off_part, off_done, total;
+    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_large, off_chunks, off_keys, total;
 };
 
 size_t g_bin_cap_override = 0;   // vcl_rasterize_set_bin_capacity (0: sized from the piece count)
@@ -95,14 +95,13 @@ Layout make_layout(int W, int H, int depth, int T, int tshift) {
     L.off_bins = o; o = al(o + sizeof(u32) * L.bin_cap);
     L.off_large = o; o = al(o + sizeof(u32) * L.nrec_cap);
     L.off_chunks = o; o = al(o + sizeof(Chunk) * L.chunk_cap);
-    L.off_part = o; o = al(o + sizeof(u64) * ts * ts * L.chunk_cap);
-    L.off_done = o; o = al(o + sizeof(u32) * (size_t)L.NT);
+    L.off_keys = o; o = al(o + sizeof(u64) * (size_t)W * H * depth);
     L.total = o;
     return L;
 }
 
 struct Ws {
-    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins; u32* large; Chunk* chunks; u64* part; u32* done;
+    u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins; u32* large; Chunk* chunks; u64* keys;
 };
 
 Ws ws_ptrs(void* base, const Layout& L) {
@@ -115,8 +114,7 @@ Ws ws_ptrs(void* base, const Layout& L) {
     w.bins = (u32*)(b + L.off_bins);
     w.large = (u32*)(b + L.off_large);
     w.chunks = (Chunk*)(b + L.off_chunks);
-    w.part = (u64*)(b + L.off_part);
-    w.done = (u32*)(b + L.off_done);
+    w.keys = (u64*)(b + L.off_keys);
     return w;
 }
 
@@ -362,9 +360,9 @@ __device__ __forceinline__ void bin_or_defer(const RastArgs& a, SetupShared& sh,
     }
 }
 
-__global__ void __launch_bounds__(256) k_clear(u32* counts, u32* done, int NT, u32* misc) {
+__global__ void __launch_bounds__(256) k_clear(u32* counts, int NT, u32* misc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < NT) { counts[i] = 0; done[i] = 0; }
+    if (i < NT) counts[i] = 0;
     if (i < 64) misc[i] = 0;
 }
 
@@ -599,18 +597,16 @@ __device__ void fine_resolve(const RastArgs& a, u32 view, int px, int py, float 
 // keys and the last chunk of the tile to finish (device-scope counter) takes the minimum over
 // all of them.
 template <int TS>
-__global__ void __launch_bounds__(256) k_fine(RastArgs a) {
+__global__ void __launch_bounds__(256, 6) k_fine(RastArgs a) {
     constexpr int NP = TS * TS;
     __shared__ u64 key[NP];
     __shared__ float spix[2][TS];
-    __shared__ u32 s_last;
     constexpr int BATCH = TS == 32 ? 128 : 256;    // pieces per LDS batch
     __shared__ unsigned short sspan[BATCH][TS];     // per piece: first | last << 8 column per row
     __shared__ float szp[9][BATCH];
     __shared__ u32 sid[BATCH];
-    __shared__ u32 sitem[BATCH * TS];               // covered rows, ordered by span length
-    __shared__ u32 sbucket[TS + 1], sbase[TS + 1];
-    __shared__ u32 snitems;
+    __shared__ u32 sitem[BATCH * TS];               // covered rows: piece | row | first column | length
+    __shared__ u32 sbucket[4];                       // per-wave covered-row totals
     const int tid = threadIdx.x;
     const u32 nchunks = a.ws.misc[5];
     for (u32 c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -620,7 +616,6 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
         const int tl = tile % a.ntiles;
         const int tx = tl % a.tilesX, ty = tl / a.tilesX;
         for (int i = tid; i < NP; i += 256) key[i] = ~0ull;
-        if (tid <= TS) sbucket[tid] = 0;
         if (tid < 2 * TS) {
             int cc = tid % TS;
             spix[tid / TS][cc] = tid < TS ? (float)((tx * TS + cc + 0.5) * (double)a.xs - 1.0)
@@ -632,10 +627,11 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
         const u32 n = ch.count & ~CHUNK_SCAN_ALL;
         __syncthreads();
         for (u32 base = 0; base < n; base += BATCH) {
-            // phase 1: thread = piece; exact row spans and depth planes into LDS; the covered
-            // rows are counted per span length
+            // phase 1: thread = piece; exact row spans and depth planes into LDS; covered rows
+            // counted
             const u32 q = base + tid;
             int yl = 1, yh = 0;
+            u32 nrow = 0;
             if (tid < BATCH && q < n) {
                 u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
                 const RRec& r = a.ws.rec[slot];
@@ -658,7 +654,7 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
 #pragma unroll
                         for (int i = 0; i < 3; i++) K[i] += dB[i];
                         sspan[tid][y] = lo <= hi ? (unsigned short)(lo | (hi << 8)) : (unsigned short)SPAN_EMPTY;
-                        if (lo <= hi) atomicAdd(&sbucket[hi - lo + 1], 1u);
+                        nrow += lo <= hi ? 1u : 0u;
                     }
 #pragma unroll
                     for (int k = 0; k < 9; k++) szp[k][tid] = r.zp[k];
@@ -667,27 +663,28 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
                     yl = 1; yh = 0;
                 }
             }
-            __syncthreads();
-            if (tid == 0) {   // bucket bases, longest spans first
-                u32 acc = 0;
-                for (int i = TS; i >= 1; i--) { u32 c2 = sbucket[i]; sbase[i] = acc; sbucket[i] = 0; acc += c2; }
-                snitems = acc;
+            // phase 2: block prefix of the covered-row counts; every covered row becomes an item
+            // (piece, row, first column, length), in piece-then-row order (neighbouring rows of
+            // one piece have similar lengths, so a wave's items are of similar length)
+            u32 incl = nrow;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                u32 yv = __shfl_up(incl, o, 64);
+                if ((tid & 63) >= o) incl += yv;
             }
+            if ((tid & 63) == 63) sbucket[tid >> 6] = incl;
             __syncthreads();
-            // phase 2: every covered row becomes an item (piece, row, first column, length) at its
-            // length bucket's position
+            u32 pos = incl - nrow;
+            for (int w = 0; w < (tid >> 6); w++) pos += sbucket[w];
+            const u32 ni = sbucket[0] + sbucket[1] + sbucket[2] + sbucket[3];
             for (int y = yl; y <= yh; y++) {
                 u32 sp = sspan[tid][y];
                 int lo = (int)(sp & 0xffu), hi = (int)(sp >> 8);
                 if (lo > hi) continue;
-                int len = hi - lo + 1;
-                u32 pos = sbase[len] + atomicAdd(&sbucket[len], 1u);
-                sitem[pos] = (u32)tid | ((u32)y << 8) | ((u32)lo << 16) | ((u32)len << 24);
+                sitem[pos++] = (u32)tid | ((u32)y << 8) | ((u32)lo << 16) | ((u32)(hi - lo + 1) << 24);
             }
             __syncthreads();
-            if (tid <= TS) sbucket[tid] = 0;   // counters of the next batch (not read below)
-            // phase 3: thread = item; a wave's items have (nearly) the same length
-            const u32 ni = snitems;
+            // phase 3: thread = item
             for (u32 i = tid; i < ni; i += 256) {
                 const u32 it = sitem[i];
                 const int j = (int)(it & 0xffu), y = (int)((it >> 8) & 0xffu), lo = (int)((it >> 16) & 0xffu), len = (int)(it >> 24);
@@ -708,37 +705,32 @@ __global__ void __launch_bounds__(256) k_fine(RastArgs a) {
             }
             __syncthreads();
         }
-        bool resolve = true;
-        if (!scan_all) {
-            const u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
-            const u32 nch = max(1u, (end - begin + a.chunk - 1) / a.chunk);
-            if (nch > 1) {
-                for (int i = tid; i < NP; i += 256) a.ws.part[(size_t)c * NP + i] = key[i];
-                __threadfence();
-                __syncthreads();
-                if (tid == 0) s_last = atomicAdd(a.ws.done + tile, 1u) == nch - 1 ? 1u : 0u;
-                __syncthreads();
-                resolve = s_last != 0u;
-                if (resolve) {
-                    __threadfence();
-                    const u32 c0 = c - ch.k;
-                    for (int i = tid; i < NP; i += 256) {
-                        u64 best = key[i];
-                        for (u32 j = 0; j < nch; j++) {
-                            if (c0 + j == c) continue;
-                            u64 o = a.ws.part[(size_t)(c0 + j) * NP + i];
-                            best = o < best ? o : best;
-                        }
-                        key[i] = best;   // each thread rewrites only its own entries
-                    }
-                }
-            }
+        // the tile's winning keys: stored when this chunk is the tile's only one, else merged with
+        // a device-scope 64-bit minimum (the key buffer starts at all-ones)
+        const u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
+        const bool alone = scan_all || (end - begin) <= a.chunk;
+        for (int i = tid; i < NP; i += 256) {
+            const int px = x0 + i % TS, py = y0 + i / TS;
+            if (px >= a.W || py >= a.H) continue;
+            u64* dst = a.ws.keys + ((size_t)view * a.H + py) * a.W + px;
+            const u64 k = key[i];
+            if (alone) *dst = k;
+            else if (k != ~0ull) atomicMin(dst, k);
         }
-        if (resolve)
-            for (int i = tid; i < NP; i += 256)
-                fine_resolve(a, view, x0 + i % TS, y0 + i / TS, spix[0][i % TS], spix[1][i / TS], key[i]);
         __syncthreads();
     }
+}
+
+// One thread per pixel: winning key -> (u, v, z/w, id+1) and DB.
+__global__ void __launch_bounds__(256) k_resolve(RastArgs a) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t P = (size_t)a.W * a.H * a.depth;
+    if (p >= P) return;
+    const int px = (int)(p % a.W), py = (int)((p / a.W) % a.H);
+    const u32 view = (u32)(p / ((size_t)a.W * a.H));
+    const float pixx = (float)((px + 0.5) * (double)a.xs - 1.0);
+    const float pixy = (float)((py + 0.5) * (double)a.ys - 1.0);
+    fine_resolve(a, view, px, py, pixx, pixy, a.ws.keys[p]);
 }
 
 // RasterizeBackwardKernel (rasterize.cu:84-141).  One 16x16 pixel tile per workgroup; the
@@ -840,7 +832,7 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap; a.bin_cap = L.bin_cap; a.chunk_cap = L.chunk_cap;
     a.ws = ws_ptrs(p->workspace, L);
     int nclear = L.NT > 64 ? L.NT : 64;
-    hipLaunchKernelGGL(k_clear, dim3(cdiv(nclear, 256)), dim3(256), 0, s, a.ws.counts, a.ws.done, L.NT, a.ws.misc);
+    hipLaunchKernelGGL(k_clear, dim3(cdiv(nclear, 256)), dim3(256), 0, s, a.ws.counts, L.NT, a.ws.misc);
     VCL_LAUNCH_CHECK();
     if (L.nrec_direct > 0) {
         hipLaunchKernelGGL(k_setup, dim3(cdiv(L.nrec_direct, 256)), dim3(256), 0, s, a);
@@ -856,10 +848,17 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     }
     hipLaunchKernelGGL(k_bin_large<true>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
+    const size_t P = (size_t)a.W * a.H * a.depth;
+    {
+        int st = vcl_fill_u32_async((u32*)a.ws.keys, 2 * P, 0xffffffffu, s);
+        if (st) return st;
+    }
     const dim3 fg((unsigned)std::min(L.chunk_cap, (size_t)FINE_MAX_GRID));
     if (tshift == 3) hipLaunchKernelGGL(k_fine<8>, fg, dim3(256), 0, s, a);
     else if (tshift == 5) hipLaunchKernelGGL(k_fine<32>, fg, dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_fine<16>, fg, dim3(256), 0, s, a);
+    VCL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_resolve, dim3(cdiv(P, 256)), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
