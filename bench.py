@@ -359,6 +359,20 @@ def main():
     value = views * args.steps / (gpu_ms / 1e3)
     loss = float(fit.read("loss", (1,))[0])
     fit.close()
+    # the same request replayed from a HIP graph (one launch per step)
+    gf = ops.Fit(sc, target=target)
+    if tuning:
+        gf.set_tuning(tuning["raster_tile"], tuning["bin_chunk"], tuning["splat_rows"])
+    gf.step(graph=True)
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    g0.record()
+    for _ in range(args.steps - 1):
+        gf.step(graph=True)
+    g1.record()
+    torch.cuda.synchronize()
+    graph_ms = g0.elapsed_time(g1) / max(args.steps - 1, 1)
+    gf.close()
     # per-op breakdown on a second fresh request (all ops event-timed; not the timed value)
     if args.breakdown:
         bf = ops.Fit(sc, target=target)
@@ -411,6 +425,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "tuning": tuning,
+        "graph_ms_per_step": round(graph_ms, 4),
         "default_tuning_ms_per_step": round(default_ms, 4) if default_ms else None,
         "batched": batched,
         "served": served,

@@ -377,8 +377,11 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     hipStream_t hs = as_hip(s);
     int st;
     // output-grad zeroing of the forward(XGradParams&) overloads
-    if ((st = vcl_fill_async(F->buf["gclip"], F->bytes["gclip"], hs))) return st;
-    if ((st = vcl_fill_async(F->buf["gdiff"], F->bytes["gdiff"], hs))) return st;
+    {
+        void* ptrs[2] = {F->buf["gclip"], F->buf["gdiff"]};
+        size_t nb[2] = {F->bytes["gclip"], F->bytes["gdiff"]};
+        if ((st = vcl_fill_multi_async(ptrs, nb, 2, hs))) return st;
+    }
     if ((st = forward(F, s))) return st;
     vcl_loss_params lp = {F->f("image"), F->f("target"), F->f("gimage"), F->f("ws_loss"), W, H, nv, (int)(F->P * C), F->f("loss")};
     { ProfScope ps_(F, "mse_loss", as_hip(s)); st = vcl_mse_loss_backward(&lp, s); }
@@ -447,15 +450,9 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
         {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_axis"), F->f("v_axis")},
         {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_sharp"), F->f("v_sharp")},
         {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_amp"), F->f("v_amp")}};
-    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi_dev(K > 0 ? 4 : 1, op, hp, bt, s); }
+    // the parameter-grad clears (AttributeGrad::clear, SGBufferGrad::clear) happen inside Adam
+    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi_dev(K > 0 ? 4 : 1, op, hp, bt, 1, s); }
     if (st) return st;
-    // parameter-grad clears (AttributeGrad::clear, SGBufferGrad::clear)
-    if ((st = vcl_fill_async(F->buf["grad_pos"], F->bytes["grad_pos"], hs))) return st;
-    if (K > 0) {
-        if ((st = vcl_fill_async(F->buf["grad_axis"], F->bytes["grad_axis"], hs))) return st;
-        if ((st = vcl_fill_async(F->buf["grad_sharpness"], F->bytes["grad_sharpness"], hs))) return st;
-        if ((st = vcl_fill_async(F->buf["grad_amplitude"], F->bytes["grad_amplitude"], hs))) return st;
-    }
     return VCL_OK;
 }
 

@@ -16,7 +16,9 @@ int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, f
 int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
                                   int quarter, int part, const double* bt_dev, vcl_stream_t stream);
 // vcl_adam_step_multi with the bias-correction powers read from the device (bt_dev, may be null).
-int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, vcl_stream_t stream);
+// zero_grad: clear each grad after it is used.
+int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
+                            vcl_stream_t stream);
 // bt_dev[0] *= rhom; bt_dev[1] *= rhov (one thread; the step-start bump of Adam::step).
 int vcl_adam_bump_dev(double* bt_dev, double rhom, double rhov, vcl_stream_t stream);
 }

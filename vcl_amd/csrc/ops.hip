@@ -4,6 +4,7 @@
 // early); reductions that the reference does with racy or atomic float adds are done with
 // fixed-order block partials so results are deterministic.
 #include "internal.hpp"
+#include <algorithm>
 #include <cmath>
 
 __global__ void vcl_fill_u32(u32* p, size_t n, u32 v) {
@@ -22,6 +23,47 @@ int vcl_fill_u32_async(u32* p, size_t n, u32 v, hipStream_t s) {
 }
 
 int vcl_fill_async(void* p, size_t bytes, hipStream_t s) { return vcl_fill_u32_async((u32*)p, bytes / 4, 0u, s); }
+
+// Zero several buffers in one launch (each block clears 4096 words of one buffer).
+constexpr int FILL_MAX = 8;
+struct FillMulti {
+    u32* p[FILL_MAX];
+    size_t n[FILL_MAX];
+    unsigned block_start[FILL_MAX + 1];
+    int count;
+};
+
+__global__ void __launch_bounds__(256) vcl_fill_multi_k(FillMulti f) {
+    unsigned b = blockIdx.x;
+    int t = 0;
+    while (t + 1 < f.count && b >= f.block_start[t + 1]) t++;
+    size_t base = (size_t)(b - f.block_start[t]) * 4096;
+    for (int k = threadIdx.x; k < 4096; k += 256) {
+        size_t i = base + k;
+        if (i < f.n[t]) f.p[t][i] = 0u;
+    }
+}
+
+int vcl_fill_multi_async(void* const* p, const size_t* bytes, int n, hipStream_t s) {
+    for (int base = 0; base < n; base += FILL_MAX) {
+        FillMulti f;
+        f.count = 0;
+        unsigned blocks = 0;
+        for (int i = base; i < n && i < base + FILL_MAX; i++) {
+            if (!p[i] || bytes[i] < 4) continue;
+            f.p[f.count] = (u32*)p[i];
+            f.n[f.count] = bytes[i] / 4;
+            f.block_start[f.count] = blocks;
+            blocks += (unsigned)((f.n[f.count] + 4095) / 4096);
+            f.count++;
+        }
+        f.block_start[f.count] = blocks;
+        if (!blocks) continue;
+        hipLaunchKernelGGL(vcl_fill_multi_k, dim3(blocks), dim3(256), 0, s, f);
+        VCL_LAUNCH_CHECK();
+    }
+    return VCL_OK;
+}
 
 namespace {
 
@@ -525,6 +567,7 @@ struct AdamArgs {
     long long block_start[ADAM_MAX + 1];
     double rhom[ADAM_MAX], rhov[ADAM_MAX], rhomt[ADAM_MAX], rhovt[ADAM_MAX], eta[ADAM_MAX], eps[ADAM_MAX];
     const double* bt;   // device {rhomt, rhovt} shared by every tensor (graph replays), or null
+    int zero_grad;      // clear each grad after use (the caller's grad clear, fused)
 };
 
 __device__ __forceinline__ void adam_elem(float& param, float g, float& m, float& v, double rhom, double rhov, double rhomt, double rhovt,
@@ -553,6 +596,7 @@ __global__ void __launch_bounds__(BS) k_adam(AdamArgs a) {
         float p = P[i], m = M[i], v = Vv[i];
         adam_elem(p, G[i], m, v, a.rhom[t], a.rhov[t], rmt, rvt, a.eta[t], a.eps[t]);
         P[i] = p; M[i] = m; Vv[i] = v;
+        if (a.zero_grad) const_cast<float*>(G)[i] = 0.f;
     }
 }
 
@@ -658,6 +702,50 @@ __global__ void __launch_bounds__(BS) k_downsample4(const float4* src, float4* d
     dst[i] = make_float4((((p00.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p00.y + p01.y) + p10.y) + p11.y) * 0.25f,
                          (((p00.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p00.w + p01.w) + p10.w) + p11.w) * 0.25f);
     if (zero_grad) zero_grad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Several mip levels in one launch: each workgroup reads a 64 x 64 block of level s and writes
+// the block's texels of levels s+1 .. s+nl (32^2 down to 1^2), each from the previous level
+// kept in LDS with the same box-filter arithmetic as k_downsample4 (so the values are
+// identical), clearing the matching coarse grads.
+struct MipArgs {
+    const float4* src;
+    int sw, sh, nl;
+    float4* dst[6];
+    float4* zg[6];
+};
+
+__global__ void __launch_bounds__(BS) k_mip_fused(MipArgs a) {
+#pragma clang fp contract(off)
+    __shared__ float4 buf[2][32 * 32];
+    const int bx = blockIdx.x, by = blockIdx.y;
+    for (int k = 1; k <= a.nl; k++) {
+        const int bs = 64 >> k;                      // block side at level s+k
+        const int w = a.sw >> k, h = a.sh >> k;      // level size
+        const int ox = bx * bs, oy = by * bs;
+        float4* out = buf[k & 1];
+        const float4* in = buf[(k - 1) & 1];
+        for (int i = threadIdx.x; i < bs * bs; i += BS) {
+            const int lx = i % bs, ly = i / bs, x = ox + lx, y = oy + ly;
+            if (x >= w || y >= h) continue;
+            float4 p00, p01, p10, p11;
+            if (k == 1) {
+                const size_t sw = (size_t)a.sw;
+                p00 = a.src[(2 * y) * sw + 2 * x]; p01 = a.src[(2 * y) * sw + 2 * x + 1];
+                p10 = a.src[(2 * y + 1) * sw + 2 * x]; p11 = a.src[(2 * y + 1) * sw + 2 * x + 1];
+            } else {
+                const int pb = 2 * bs;
+                p00 = in[(2 * ly) * pb + 2 * lx]; p01 = in[(2 * ly) * pb + 2 * lx + 1];
+                p10 = in[(2 * ly + 1) * pb + 2 * lx]; p11 = in[(2 * ly + 1) * pb + 2 * lx + 1];
+            }
+            const float4 v = make_float4((((p00.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p00.y + p01.y) + p10.y) + p11.y) * 0.25f,
+                                         (((p00.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p00.w + p01.w) + p10.w) + p11.w) * 0.25f);
+            out[ly * bs + lx] = v;
+            a.dst[k - 1][(size_t)y * w + x] = v;
+            if (a.zg[k - 1]) a.zg[k - 1][(size_t)y * w + x] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------- NormalAxis / ViewAxis
@@ -1502,11 +1590,12 @@ extern "C" int vcl_texture_grad_clear(const vcl_texture* g, vcl_stream_t stream)
 
 // bt_dev: device {rhomt, rhovt} used instead of the per-tensor host values (null: host values).
 extern "C" int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev,
-                                       vcl_stream_t stream) {
+                                       int zero_grad, vcl_stream_t stream) {
     VCL_REQUIRE(n >= 0 && (n == 0 || (p && h)));
     for (int base = 0; base < n; base += ADAM_MAX) {
         AdamArgs a;
         a.bt = bt_dev;
+        a.zero_grad = zero_grad;
         a.n = n - base < ADAM_MAX ? n - base : ADAM_MAX;
         long long blocks = 0;
         for (int i = 0; i < a.n; i++) {
@@ -1528,7 +1617,7 @@ extern "C" int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, cons
 }
 
 extern "C" int vcl_adam_step_multi(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream) {
-    return vcl_adam_step_multi_dev(n, p, h, nullptr, stream);
+    return vcl_adam_step_multi_dev(n, p, h, nullptr, 0, stream);
 }
 
 extern "C" int vcl_adam_bump_dev(double* bt_dev, double rhom, double rhov, vcl_stream_t stream) {
@@ -1568,16 +1657,39 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
             hipLaunchKernelGGL(k_tex_update, dim3(cdiv(n, BS)), dim3(BS), 0, s, a);
         VCL_LAUNCH_CHECK();
     }
-    // rebuild mips from the new level 0; the same pass zeroes the coarse grads
-    for (int l = 1; (part & 2) && l < a.L; l++) {
+    // rebuild mips from the new level 0; the same pass zeroes the coarse grads.  C == 4: level 1
+    // with one thread per texel, then up to six levels per launch (k_mip_fused)
+    if (!(part & 2)) return VCL_OK;
+    if (vec4) {
+        int l = 1;
+        if (l < a.L) {
+            int w = a.tw >> 1, hh = a.th >> 1;
+            hipLaunchKernelGGL(k_downsample4, dim3(cdiv((size_t)w * hh, BS)), dim3(BS), 0, s, (const float4*)tex->texture[0],
+                               (float4*)tex->texture[1], w, hh, (float4*)a.grad[1]);
+            VCL_LAUNCH_CHECK();
+            l = 2;
+        }
+        while (l < a.L) {
+            MipArgs m;
+            m.src = (const float4*)tex->texture[l - 1];
+            m.sw = a.tw >> (l - 1);
+            m.sh = a.th >> (l - 1);
+            m.nl = std::min(6, a.L - l);
+            for (int k = 0; k < 6; k++) {
+                m.dst[k] = k < m.nl ? (float4*)tex->texture[l + k] : nullptr;
+                m.zg[k] = k < m.nl ? (float4*)a.grad[l + k] : nullptr;
+            }
+            hipLaunchKernelGGL(k_mip_fused, dim3(cdiv(m.sw, 64), cdiv(m.sh, 64)), dim3(BS), 0, s, m);
+            VCL_LAUNCH_CHECK();
+            l += m.nl;
+        }
+        return VCL_OK;
+    }
+    for (int l = 1; l < a.L; l++) {
         int w = a.tw >> l, hh = a.th >> l;
         size_t m = (size_t)w * hh * a.C;
         if (!m) continue;
-        if (vec4)
-            hipLaunchKernelGGL(k_downsample4, dim3(cdiv((size_t)w * hh, BS)), dim3(BS), 0, s, (const float4*)tex->texture[l - 1],
-                               (float4*)tex->texture[l], w, hh, (float4*)a.grad[l]);
-        else
-            hipLaunchKernelGGL(k_downsample, dim3(cdiv(m, BS)), dim3(BS), 0, s, tex->texture[l - 1], tex->texture[l], w, hh, a.C, a.grad[l]);
+        hipLaunchKernelGGL(k_downsample, dim3(cdiv(m, BS)), dim3(BS), 0, s, tex->texture[l - 1], tex->texture[l], w, hh, a.C, a.grad[l]);
         VCL_LAUNCH_CHECK();
     }
     return VCL_OK;
