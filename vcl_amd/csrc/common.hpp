@@ -61,10 +61,18 @@ __device__ __forceinline__ void lds_table_clear(LdsTable<SLOTS, NV>& t) {
     }
 }
 
+constexpr int ilog2_c(int n) { return n <= 1 ? 0 : 1 + ilog2_c(n >> 1); }
+
+// Multiplicative (Fibonacci) hash of k into [0, 2^BITS): the top bits of k * 2^32/phi.  The low
+// bits of the product depend only on the low bits of k, so keys that differ by a multiple of a
+// large power of two (texel rows of a wide image) would share them.
+template <int BITS>
+__device__ __forceinline__ u32 fib_hash(u32 k) { return (k * 2654435761u) >> (32 - BITS); }
+
 // Slot of key k (inserted if absent), or -1 when MAXPROBE probes find no room.
 template <int SLOTS, int NV, int MAXPROBE = 24>
 __device__ __forceinline__ int lds_table_slot(LdsTable<SLOTS, NV>& t, u32 k) {
-    u32 h = (k * 2654435761u) & (SLOTS - 1);
+    u32 h = fib_hash<ilog2_c(SLOTS)>(k);
     for (int probe = 0; probe < MAXPROBE; probe++) {
         u32 old = atomicCAS(&t.key[h], LDS_EMPTY, k);
         if (old == LDS_EMPTY || old == k) return (int)h;

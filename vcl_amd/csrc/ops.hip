@@ -400,27 +400,100 @@ __global__ void __launch_bounds__(BS) k_tex_bwd(size_t P, TexArgs t, TexGradArgs
     if (guvDA) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
 }
 
-// C == 4: one 16x16 pixel tile per workgroup; texel grads of the tile are summed in an LDS
-// hash table keyed by (level, texel) and flushed with one set of atomics per touched texel.
-constexpr int TB_SLOTS = 2048;
+// C == 4: texel grads of a workgroup's pixels are summed in an LDS table of NB buckets, one
+// bucket per run of 4 consecutive texels of a level (one 64-byte line of float4 grads), keyed by
+// (level, texel / 4).  The flush compacts the occupied buckets and hands each to 16 consecutive
+// lanes, texel-major and channel-minor, so one wave's atomics cover 4 whole lines instead of 64
+// scattered dwords.  A key that finds no bucket within the probe limit falls back to direct
+// global atomics.
+constexpr u32 TB_LEVEL_SHIFT = 22;  // key = level << 22 | texel >> 2 (texel < 2^24)
 
-__device__ __forceinline__ void tex_splat(LdsTable<TB_SLOTS, 4>& tab, float* G, int level, int texel, float w, float4 go) {
-    u32 key = ((u32)level << 24) | (u32)texel;
-    int sl = lds_table_slot(tab, key);
-    if (sl >= 0) {
-        atomicAdd(&tab.val[0][sl], w * go.x);
-        atomicAdd(&tab.val[1][sl], w * go.y);
-        atomicAdd(&tab.val[2][sl], w * go.z);
-        atomicAdd(&tab.val[3][sl], w * go.w);
+template <int NB>
+struct TexGradTable {
+    float val[NB * 16];  // [bucket][texel & 3][channel]
+    u32 key[NB];
+    u32 used[NB];        // occupied buckets, compacted before the flush
+    u32 wave_count[BS / 64];
+};
+
+template <int NB>
+__device__ __forceinline__ void tex_table_clear(TexGradTable<NB>& t) {
+    for (int i = threadIdx.x; i < NB; i += blockDim.x) t.key[i] = LDS_EMPTY;
+    float4* v = (float4*)t.val;
+    for (int i = threadIdx.x; i < NB * 4; i += blockDim.x) v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Keys only go from empty to their final value, so a plain read that finds k is final; the CAS
+// is issued only for an empty bucket (a read of a stale "empty" is settled by the CAS).
+template <int NB>
+__device__ __forceinline__ int tex_table_bucket(TexGradTable<NB>& t, u32 k) {
+    u32 h = fib_hash<ilog2_c(NB)>(k);
+    for (int probe = 0; probe < 24; probe++) {
+        u32 cur = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == k) return (int)h;
+        if (cur == LDS_EMPTY) {
+            u32 old = atomicCAS(&t.key[h], LDS_EMPTY, k);
+            if (old == LDS_EMPTY || old == k) return (int)h;
+        }
+        h = (h + 1) & (NB - 1);
+    }
+    return -1;
+}
+
+template <int NB>
+__device__ __forceinline__ void tex_splat(TexGradTable<NB>& tab, float* G, int level, int texel, float w, float4 go) {
+    int b = tex_table_bucket(tab, ((u32)level << TB_LEVEL_SHIFT) | ((u32)texel >> 2));
+    if (b >= 0) {
+        float* q = tab.val + b * 16 + (texel & 3) * 4;
+        atomicAdd(q, w * go.x);
+        atomicAdd(q + 1, w * go.y);
+        atomicAdd(q + 2, w * go.z);
+        atomicAdd(q + 3, w * go.w);
     } else {
         float* q = G + (size_t)texel * 4;
         atomicAdd(q, w * go.x); atomicAdd(q + 1, w * go.y); atomicAdd(q + 2, w * go.z); atomicAdd(q + 3, w * go.w);
     }
 }
 
+// Flush of the table (after a barrier; blockDim.x == BS): the occupied buckets are compacted
+// into used[] with a ballot per wave, then lane j of a bucket's 16 adds dword j of its 64-byte
+// run (texel j / 4, channel j % 4).  Untouched texels of a run hold exact zeros and are skipped,
+// which also keeps runs that pass the end of a small level (fewer than 4 texels) inside it.
+template <int NB>
+__device__ __forceinline__ void tex_table_flush(TexGradTable<NB>& tab, const TexGradArgs& g) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    u32 n = 0;
+    for (int c = 0; c < NB; c += BS) {
+        int b = c + threadIdx.x;
+        bool occ = b < NB && tab.key[b] != LDS_EMPTY;
+        u64 m = __ballot(occ);
+        if (lane == 0) tab.wave_count[wave] = (u32)__popcll(m);
+        __syncthreads();
+        u32 off = n, total = 0;
+#pragma unroll
+        for (int w = 0; w < BS / 64; w++) {
+            u32 cw = tab.wave_count[w];
+            off += w < wave ? cw : 0u;
+            total += cw;
+        }
+        if (occ) tab.used[off + (u32)__popcll(m & ((1ull << lane) - 1))] = (u32)b;
+        n += total;
+        __syncthreads();
+    }
+    for (u32 i = threadIdx.x; i < n * 16; i += BS) {
+        u32 b = tab.used[i >> 4];
+        float v = tab.val[b * 16 + (i & 15)];
+        if (v == 0.f) continue;
+        u32 k = tab.key[b];
+        size_t dword = ((size_t)(k & ((1u << TB_LEVEL_SHIFT) - 1)) << 4) | (i & 15);
+        atomicAdd(g.tex[k >> TB_LEVEL_SHIFT] + dword, v);
+    }
+}
+
 // One pixel of the C == 4 texture backward: texel splats into the LDS table, uv / uvDA grads
 // stored.
-__device__ __forceinline__ void tex_bwd4_pixel(LdsTable<TB_SLOTS, 4>& tab, bool want_tex, int W, int H, int view, int px, int py, const TexArgs& t,
+template <int NB>
+__device__ __forceinline__ void tex_bwd4_pixel(TexGradTable<NB>& tab, bool want_tex, int W, int H, int view, int px, int py, const TexArgs& t,
                                                const TexGradArgs& g, const float* rast, const float* uv, const float* uvDA, const float* gout,
                                                float* guv, float* guvDA, int mode) {
     if (px < W && py < H) {
@@ -491,9 +564,9 @@ __device__ __forceinline__ void tex_bwd4_pixel(LdsTable<TB_SLOTS, 4>& tab, bool 
 template <int ROWS>
 __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
                                                        const float* uvDA, const float* gout, float* guv, float* guvDA, int mode) {
-    __shared__ LdsTable<TB_SLOTS, 4> tab;
+    __shared__ TexGradTable<16 * ROWS> tab;
     bool want_tex = g.tex[0] != nullptr;
-    if (want_tex) lds_table_clear(tab);
+    if (want_tex) tex_table_clear(tab);
     __syncthreads();
     int view = blockIdx.y;
     int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
@@ -505,15 +578,7 @@ __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX,
     }
     if (!want_tex) return;
     __syncthreads();
-    for (int i = threadIdx.x; i < TB_SLOTS; i += BS) {
-        u32 k = tab.key[i];
-        if (k == LDS_EMPTY) continue;
-        float* q = g.tex[k >> 24] + (size_t)(k & 0xffffffu) * 4;
-        atomicAdd(q, tab.val[0][i]);
-        atomicAdd(q + 1, tab.val[1][i]);
-        atomicAdd(q + 2, tab.val[2][i]);
-        atomicAdd(q + 3, tab.val[3][i]);
-    }
+    tex_table_flush(tab, g);
 }
 
 // ---------------------------------------------------------------- mip utilities
