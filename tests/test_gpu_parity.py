@@ -375,6 +375,34 @@ def test_antialias(gpu, cfg1):
     assert_close(N(gclip), gclip_ref, 1e-3, "antialias grad clip", atol_frac=1e-4)
 
 
+@pytest.mark.parametrize("W,H,C", [(67, 45, 3), (33, 80, 1), (16, 16, 4)])
+def test_antialias_ragged_views(gpu, W, H, C):
+    # overlapping random triangles in three views, sizes that leave partial 16 x 16 tiles, so
+    # pairs at tile edges, partial tiles and per-view offsets are all exercised
+    import torch
+    rng = np.random.default_rng(W * H + C)
+    views, n = 3, 400
+    ctr = rng.uniform(-0.9, 0.9, (views, n, 1, 2))
+    xy = ctr + rng.uniform(-0.25, 0.25, (views, n, 3, 2))
+    w = rng.uniform(0.8, 1.5, (views, n, 3, 1))
+    z = rng.uniform(-0.9, 0.9, (views, n, 3, 1))
+    clip = np.concatenate([xy * w, z * w, w], -1).reshape(views, 3 * n, 4).astype(F32)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    rast, _ = o_raster(clip, idx, W, H, views)
+    color = rng.uniform(0, 1, (views, H, W, C)).astype(F32)
+    idx_g = T(idx.astype(np.int32)).view(torch.int32)
+    out = gpu.antialias(T(clip), idx_g, T(rast), T(color))
+    ref = o_aa(clip, idx, rast, color)
+    assert not np.array_equal(ref, color)
+    assert_close(N(out), ref, 1e-5, f"antialias fwd {W}x{H}x{C}")
+    gout = rng.normal(size=color.shape).astype(F32)
+    gin_ref, gclip_ref = o_aa_bwd(clip, idx, rast, color, gout)
+    gclip = torch.zeros(views, 3 * n, 4, device="cuda")
+    gin = gpu.antialias_backward(T(clip), idx_g, T(rast), T(color), T(gout), gclip)
+    assert_close(N(gin), gin_ref, 1e-5, f"antialias grad in {W}x{H}x{C}")
+    assert_close(N(gclip), gclip_ref, 1e-3, f"antialias grad clip {W}x{H}x{C}", atol_frac=1e-4)
+
+
 def test_mse(gpu):
     rng = np.random.default_rng(13)
     for n in (1, 1000, 256 * 256 * 4 + 3):

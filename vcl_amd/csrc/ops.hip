@@ -1362,105 +1362,166 @@ __device__ int aa_pair(const AAArgs& a, const float4* clipv, const float4* rastv
     return ne;
 }
 
-__global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, const float* in, float* out) {
+// Tiled antialias: a workgroup owns a 16 x 16 pixel tile and evaluates every pixel pair that
+// touches it once: the left and lower pair of each tile pixel (owned) plus the left pair of the
+// column right of the tile and the lower pair of the row above it (halo, owned by the
+// neighbouring tiles).  Each pair's contributions to its two pixels go to LDS; each pixel then
+// adds the four pairs that touch it in the order (q, left), (q, below), (right, q), (above, q).
+constexpr int AA_T = 16;
+constexpr int AA_OWN = 2 * AA_T * AA_T;
+constexpr int AA_PAIRS = AA_OWN + 2 * AA_T;
+static_assert(AA_T * AA_T == BS, "one thread per tile pixel");
+
+// Pair p of the tile with corner (x0, y0): its current pixel (the one whose left / lower
+// neighbour forms the pair) and direction.
+__device__ __forceinline__ void aa_tile_pair(int p, int x0, int y0, int& cx, int& cy, int& vertical) {
+    if (p < AA_T * AA_T) { cx = x0 + (p % AA_T); cy = y0 + p / AA_T; vertical = 0; }
+    else if (p < AA_OWN) { int r = p - AA_T * AA_T; cx = x0 + (r % AA_T); cy = y0 + r / AA_T; vertical = 1; }
+    else if (p < AA_OWN + AA_T) { cx = x0 + AA_T; cy = y0 + (p - AA_OWN); vertical = 0; }
+    else { cx = x0 + (p - AA_OWN - AA_T); cy = y0 + AA_T; vertical = 1; }
+}
+
+// LDS slot of the pair that touches tile pixel (tx, ty) as the k-th of its four pairs.
+__device__ __forceinline__ int aa_pair_slot(int tx, int ty, int k) {
+    switch (k) {
+        case 0: return ty * AA_T + tx;
+        case 1: return AA_T * AA_T + ty * AA_T + tx;
+        case 2: return tx + 1 < AA_T ? ty * AA_T + tx + 1 : AA_OWN + ty;
+        default: return ty + 1 < AA_T ? AA_T * AA_T + (ty + 1) * AA_T + tx : AA_OWN + AA_T + tx;
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float* in, float* out) {
 #pragma clang fp contract(off)
-    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t P = (size_t)a.W * a.H * a.depth;
-    if (g >= P) return;
-    int px = (int)(g % a.W), py = (int)((g / a.W) % a.H), s = (int)(g / ((size_t)a.W * a.H));
+    __shared__ float4 to_cur[AA_PAIRS], to_nb[AA_PAIRS];
+    const int s = blockIdx.y;
+    const int x0 = (blockIdx.x % tilesX) * AA_T, y0 = (blockIdx.x / tilesX) * AA_T;
     const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
     const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
     const float* inv = in + (size_t)s * a.W * a.H * a.C;
-    int q = py * a.W + px;
-    int C = a.C;
-    float acc[AA_MAXC];
-#pragma unroll
-    for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? inv[(size_t)q * C + c] : 0.f;
-    // pairs touching q, in order: (q, left), (q, below), (right, q), (above, q)
-    for (int k = 0; k < 4; k++) {
-        int cx = px + (k == 2), cy = py + (k == 3);
-        if (cx >= a.W || cy >= a.H) continue;
+    const int C = a.C;
+    for (int p = threadIdx.x; p < AA_PAIRS; p += BS) {
+        int cx, cy, vertical;
+        aa_tile_pair(p, x0, y0, cx, cy, vertical);
+        float tc[AA_MAXC] = {0.f, 0.f, 0.f, 0.f}, tn[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
         int pp = 0, op = 0;
         AAEdge E[3];
-        if (!aa_pair(a, clipv, rastv, cx, cy, k & 1, pp, op, E)) continue;
+        if (cx < a.W && cy < a.H && aa_pair(a, clipv, rastv, cx, cy, vertical, pp, op, E)) {
+            int cur = cy * a.W + cx;
 #pragma unroll
-        for (int e = 0; e < 3; e++) {
-            if (!E[e].valid) continue;
-            int tgt = E[e].target_o ? op : pp;
-            if (tgt != q) continue;
+            for (int e = 0; e < 3; e++) {
+                if (!E[e].valid) continue;
+                bool to_c = (E[e].target_o ? op : pp) == cur;
 #pragma unroll
-            for (int c = 0; c < AA_MAXC; c++)
-                if (c < C) acc[c] += E[e].alpha * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+                for (int c = 0; c < AA_MAXC; c++) {
+                    if (c >= C) continue;
+                    float v = E[e].alpha * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+                    if (to_c) tc[c] += v; else tn[c] += v;
+                }
+            }
         }
+        to_cur[p] = make_float4(tc[0], tc[1], tc[2], tc[3]);
+        to_nb[p] = make_float4(tn[0], tn[1], tn[2], tn[3]);
     }
-    float* o = out + g * C;
+    __syncthreads();
+    const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
+    const int px = x0 + tx, py = y0 + ty;
+    if (px >= a.W || py >= a.H) return;
+    size_t q = (size_t)py * a.W + px;
+    float acc[AA_MAXC];
+#pragma unroll
+    for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? inv[q * C + c] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float4 v = k < 2 ? to_cur[aa_pair_slot(tx, ty, k)] : to_nb[aa_pair_slot(tx, ty, k)];
+        float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int c = 0; c < AA_MAXC; c++) acc[c] += vv[c];
+    }
+    float* o = out + ((size_t)s * a.W * a.H + q) * C;
 #pragma unroll
     for (int c = 0; c < AA_MAXC; c++)
         if (c < C) o[c] = acc[c];
 }
 
-__global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, const float* in, const float* gout, float* gin, float* gproj) {
+// Backward of the tiled antialias.  Per pair, D = sum over its edges of alpha * gout[target]
+// goes to the front pixel pp with + and to the other pixel op with -; the vertex (clip-space)
+// grads of a pair's edges are added by the tile that owns the pair.
+__global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, int tilesX, const float* in, const float* gout, float* gin, float* gproj) {
 #pragma clang fp contract(off)
-    size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t P = (size_t)a.W * a.H * a.depth;
-    if (g >= P) return;
-    int px = (int)(g % a.W), py = (int)((g / a.W) % a.H), s = (int)(g / ((size_t)a.W * a.H));
+    __shared__ float4 to_cur[AA_PAIRS];
+    const int s = blockIdx.y;
+    const int x0 = (blockIdx.x % tilesX) * AA_T, y0 = (blockIdx.x / tilesX) * AA_T;
     const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
     const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
     size_t off = (size_t)s * a.W * a.H * a.C;
     const float* inv = in + off;
     const float* gov = gout + off;
     float* gcs = gproj ? gproj + (size_t)s * a.V * 4 : nullptr;
-    int q = py * a.W + px;
-    int C = a.C;
-    float acc[AA_MAXC];
-#pragma unroll
-    for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? gov[(size_t)q * C + c] : 0.f;
-    for (int k = 0; k < 4; k++) {
-        int cx = px + (k == 2), cy = py + (k == 3);
-        if (cx >= a.W || cy >= a.H) continue;
+    const int C = a.C;
+    for (int p = threadIdx.x; p < AA_PAIRS; p += BS) {
+        int cx, cy, vertical;
+        aa_tile_pair(p, x0, y0, cx, cy, vertical);
+        float D[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
+        bool pp_cur = true;
         int pp = 0, op = 0;
         AAEdge E[3];
-        if (!aa_pair(a, clipv, rastv, cx, cy, k & 1, pp, op, E)) continue;
+        if (cx < a.W && cy < a.H && aa_pair(a, clipv, rastv, cx, cy, vertical, pp, op, E)) {
+            pp_cur = pp == cy * a.W + cx;
 #pragma unroll
-        for (int e = 0; e < 3; e++) {
-            if (!E[e].valid) continue;
-            int tgt = E[e].target_o ? op : pp;
-            float al = E[e].alpha;
-#pragma unroll
-            for (int c = 0; c < AA_MAXC; c++) {
-                if (c >= C) continue;
-                if (pp == q) acc[c] += al * gov[(size_t)tgt * C + c];
-                if (op == q) acc[c] -= al * gov[(size_t)tgt * C + c];
-            }
-            if (gcs && k < 2) {
-                float d = 0.f;
+            for (int e = 0; e < 3; e++) {
+                if (!E[e].valid) continue;
+                int tgt = E[e].target_o ? op : pp;
+                float al = E[e].alpha;
 #pragma unroll
                 for (int c = 0; c < AA_MAXC; c++)
-                    if (c < C) d += gov[(size_t)tgt * C + c] * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
-                const AAEdge& ed = E[e];
-                float ia = ed.ia, av = ed.pax * ed.pby - ed.pay * ed.pbx;
-                float dDax, dDay, dDbx, dDby;
-                if (ed.horiz) { dDax = 0.f; dDay = -ed.sg; dDbx = 0.f; dDby = ed.sg; }
-                else { dDax = ed.sg; dDay = 0.f; dDbx = -ed.sg; dDby = 0.f; }
-                float k2 = av * ia * ia;
-                float gax = d * (ia * ed.pby - k2 * dDax), gay = d * (-ia * ed.pbx - k2 * dDay);
-                float gbx = d * (-ia * ed.pay - k2 * dDbx), gby = d * (ia * ed.pax - k2 * dDby);
-                int vv[2] = {ed.va, ed.vb};
-                float gxs[2] = {gax, gbx}, gys[2] = {gay, gby};
-                for (int j = 0; j < 2; j++) {
-                    float4 c = clipv[vv[j]];
-                    float iw = 1.f / c.w;
-                    float gvx = gxs[j] * a.sx * iw, gvy = gys[j] * a.sy * iw;
-                    float* gp = gcs + 4 * (size_t)vv[j];
-                    atomicAdd(gp, gvx);
-                    atomicAdd(gp + 1, gvy);
-                    atomicAdd(gp + 3, -(gvx * c.x + gvy * c.y) * iw);
+                    if (c < C) D[c] += al * gov[(size_t)tgt * C + c];
+                if (gcs && p < AA_OWN) {
+                    float d = 0.f;
+#pragma unroll
+                    for (int c = 0; c < AA_MAXC; c++)
+                        if (c < C) d += gov[(size_t)tgt * C + c] * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+                    const AAEdge& ed = E[e];
+                    float ia = ed.ia, av = ed.pax * ed.pby - ed.pay * ed.pbx;
+                    float dDax, dDay, dDbx, dDby;
+                    if (ed.horiz) { dDax = 0.f; dDay = -ed.sg; dDbx = 0.f; dDby = ed.sg; }
+                    else { dDax = ed.sg; dDay = 0.f; dDbx = -ed.sg; dDby = 0.f; }
+                    float k2 = av * ia * ia;
+                    float gax = d * (ia * ed.pby - k2 * dDax), gay = d * (-ia * ed.pbx - k2 * dDay);
+                    float gbx = d * (-ia * ed.pay - k2 * dDbx), gby = d * (ia * ed.pax - k2 * dDby);
+                    int vv[2] = {ed.va, ed.vb};
+                    float gxs[2] = {gax, gbx}, gys[2] = {gay, gby};
+                    for (int j = 0; j < 2; j++) {
+                        float4 c = clipv[vv[j]];
+                        float iw = 1.f / c.w;
+                        float gvx = gxs[j] * a.sx * iw, gvy = gys[j] * a.sy * iw;
+                        float* gp = gcs + 4 * (size_t)vv[j];
+                        atomicAdd(gp, gvx);
+                        atomicAdd(gp + 1, gvy);
+                        atomicAdd(gp + 3, -(gvx * c.x + gvy * c.y) * iw);
+                    }
                 }
             }
         }
+        // contribution to the current pixel; the neighbour receives its negation
+        to_cur[p] = pp_cur ? make_float4(D[0], D[1], D[2], D[3]) : make_float4(-D[0], -D[1], -D[2], -D[3]);
     }
-    float* o = gin + g * C;
+    __syncthreads();
+    const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
+    const int px = x0 + tx, py = y0 + ty;
+    if (px >= a.W || py >= a.H) return;
+    size_t q = (size_t)py * a.W + px;
+    float acc[AA_MAXC];
+#pragma unroll
+    for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? gov[q * C + c] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float4 v = to_cur[aa_pair_slot(tx, ty, k)];
+        float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int c = 0; c < AA_MAXC; c++) acc[c] = k < 2 ? acc[c] + vv[c] : acc[c] - vv[c];
+    }
+    float* o = gin + off + q * C;
 #pragma unroll
     for (int c = 0; c < AA_MAXC; c++)
         if (c < C) o[c] = acc[c];
@@ -1934,19 +1995,21 @@ static AAArgs aa_args(const vcl_antialias_params* p) {
 }
 
 extern "C" int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t stream) {
-    VCL_REQUIRE(p && p->proj && p->idx && p->rast && p->in && p->out && p->channel >= 1 && p->channel <= AA_MAXC);
+    VCL_REQUIRE(p && p->proj && p->idx && p->rast && p->in && p->out && p->channel >= 1 && p->channel <= AA_MAXC && p->depth <= 65535);
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_aa_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), aa_args(p), p->in, p->out);
+    int tilesX = (p->width + AA_T - 1) / AA_T, tilesY = (p->height + AA_T - 1) / AA_T;
+    hipLaunchKernelGGL(k_aa_fwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p), tilesX, p->in, p->out);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
 
 extern "C" int vcl_antialias_backward(const vcl_antialias_params* p, const vcl_antialias_grad_params* g, vcl_stream_t stream) {
-    VCL_REQUIRE(p && g && p->proj && p->idx && p->rast && p->in && g->out && g->in && p->channel >= 1 && p->channel <= AA_MAXC);
+    VCL_REQUIRE(p && g && p->proj && p->idx && p->rast && p->in && g->out && g->in && p->channel >= 1 && p->channel <= AA_MAXC && p->depth <= 65535);
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_aa_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), aa_args(p), p->in, g->out, g->in, g->proj);
+    int tilesX = (p->width + AA_T - 1) / AA_T, tilesY = (p->height + AA_T - 1) / AA_T;
+    hipLaunchKernelGGL(k_aa_bwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p), tilesX, p->in, g->out, g->in, g->proj);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
