@@ -69,16 +69,60 @@ constexpr int ilog2_c(int n) { return n <= 1 ? 0 : 1 + ilog2_c(n >> 1); }
 template <int BITS>
 __device__ __forceinline__ u32 fib_hash(u32 k) { return (k * 2654435761u) >> (32 - BITS); }
 
-// Slot of key k (inserted if absent), or -1 when MAXPROBE probes find no room.
+// Slot of key k (inserted if absent), or -1 when MAXPROBE probes find no room.  Keys only go
+// from empty to their final value, so a plain read that finds k is final; the CAS is issued only
+// for an empty slot (a stale "empty" is settled by the CAS).
 template <int SLOTS, int NV, int MAXPROBE = 24>
 __device__ __forceinline__ int lds_table_slot(LdsTable<SLOTS, NV>& t, u32 k) {
     u32 h = fib_hash<ilog2_c(SLOTS)>(k);
     for (int probe = 0; probe < MAXPROBE; probe++) {
-        u32 old = atomicCAS(&t.key[h], LDS_EMPTY, k);
-        if (old == LDS_EMPTY || old == k) return (int)h;
+        u32 cur = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == k) return (int)h;
+        if (cur == LDS_EMPTY) {
+            u32 old = atomicCAS(&t.key[h], LDS_EMPTY, k);
+            if (old == LDS_EMPTY || old == k) return (int)h;
+        }
         h = (h + 1) & (SLOTS - 1);
     }
     return -1;
+}
+
+// Block-wide compaction step: the threads with occ append v to out[] after its first n entries,
+// in thread order, and n advances by the block's count.  Every thread of the block calls it (it
+// holds two barriers); wave_count is an LDS array of blockDim.x / 64 entries.
+__device__ __forceinline__ void block_append(bool occ, u32 v, u32* out, u32* wave_count, u32& n) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    u64 m = __ballot(occ);
+    if (lane == 0) wave_count[wave] = (u32)__popcll(m);
+    __syncthreads();
+    u32 off = n, total = 0;
+    for (int w = 0; w < nw; w++) {
+        u32 c = wave_count[w];
+        off += w < wave ? c : 0u;
+        total += c;
+    }
+    if (occ) out[off + (u32)__popcll(m & ((1ull << lane) - 1))] = v;
+    n += total;
+    __syncthreads();
+}
+
+// Flush of an LDS table of per-vertex clip-space grads (x, y, w) into g[4 v + {0, 1, 3}]: the
+// occupied slots are compacted into used[] (SLOTS entries) and each vertex is handed to four
+// consecutive lanes, so a wave's atomics cover 16 vertices' 16-byte records.  Called by every
+// thread of the block after a barrier that ends the table updates.
+template <int SLOTS>
+__device__ __forceinline__ void lds_vertex_grad_flush(const LdsTable<SLOTS, 3>& t, u32* used, u32* wave_count, float* g) {
+    u32 n = 0;
+    for (int c = 0; c < SLOTS; c += blockDim.x) {
+        int sl = c + threadIdx.x;
+        block_append(sl < SLOTS && t.key[sl] != LDS_EMPTY, (u32)sl, used, wave_count, n);
+    }
+    for (u32 i = threadIdx.x; i < 4 * n; i += blockDim.x) {
+        u32 sl = used[i >> 2], comp = i & 3;
+        if (comp == 2) continue;
+        float v = t.val[comp == 3 ? 2 : comp][sl];
+        if (v != 0.f) atomicAdd(g + 4 * (size_t)t.key[sl] + comp, v);
+    }
 }
 
 // Zero-fill kernel (used instead of hipMemsetAsync so sequences stay graph-safe).

@@ -455,30 +455,16 @@ __device__ __forceinline__ void tex_splat(TexGradTable<NB>& tab, float* G, int l
     }
 }
 
-// Flush of the table (after a barrier; blockDim.x == BS): the occupied buckets are compacted
-// into used[] with a ballot per wave, then lane j of a bucket's 16 adds dword j of its 64-byte
-// run (texel j / 4, channel j % 4).  Untouched texels of a run hold exact zeros and are skipped,
-// which also keeps runs that pass the end of a small level (fewer than 4 texels) inside it.
+// Flush of the table (after a barrier): the occupied buckets are compacted into used[], then
+// lane j of a bucket's 16 adds dword j of its 64-byte run (texel j / 4, channel j % 4).
+// Untouched texels of a run hold exact zeros and are skipped, which also keeps runs that pass the
+// end of a small level (fewer than 4 texels) inside it.
 template <int NB>
 __device__ __forceinline__ void tex_table_flush(TexGradTable<NB>& tab, const TexGradArgs& g) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     u32 n = 0;
     for (int c = 0; c < NB; c += BS) {
         int b = c + threadIdx.x;
-        bool occ = b < NB && tab.key[b] != LDS_EMPTY;
-        u64 m = __ballot(occ);
-        if (lane == 0) tab.wave_count[wave] = (u32)__popcll(m);
-        __syncthreads();
-        u32 off = n, total = 0;
-#pragma unroll
-        for (int w = 0; w < BS / 64; w++) {
-            u32 cw = tab.wave_count[w];
-            off += w < wave ? cw : 0u;
-            total += cw;
-        }
-        if (occ) tab.used[off + (u32)__popcll(m & ((1ull << lane) - 1))] = (u32)b;
-        n += total;
-        __syncthreads();
+        block_append(b < NB && tab.key[b] != LDS_EMPTY, (u32)b, tab.used, tab.wave_count, n);
     }
     for (u32 i = threadIdx.x; i < n * 16; i += BS) {
         u32 b = tab.used[i >> 4];
@@ -1446,10 +1432,30 @@ __global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float
 
 // Backward of the tiled antialias.  Per pair, D = sum over its edges of alpha * gout[target]
 // goes to the front pixel pp with + and to the other pixel op with -; the vertex (clip-space)
-// grads of a pair's edges are added by the tile that owns the pair.
+// grads of a pair's edges are added by the tile that owns the pair, summed per vertex in an LDS
+// table first (neighbouring edge pixels share their edge's vertices) and flushed with four
+// consecutive lanes per vertex (x, y, -, w).
+constexpr int AA_VSLOTS = 256;
+
+__device__ __forceinline__ void aa_vertex_grad(LdsTable<AA_VSLOTS, 3>& vt, float* gcs, int v, float gx, float gy, float gw) {
+    int sl = lds_table_slot(vt, (u32)v);
+    if (sl >= 0) {
+        atomicAdd(&vt.val[0][sl], gx);
+        atomicAdd(&vt.val[1][sl], gy);
+        atomicAdd(&vt.val[2][sl], gw);
+    } else {
+        float* gp = gcs + 4 * (size_t)v;
+        atomicAdd(gp, gx);
+        atomicAdd(gp + 1, gy);
+        atomicAdd(gp + 3, gw);
+    }
+}
+
 __global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, int tilesX, const float* in, const float* gout, float* gin, float* gproj) {
 #pragma clang fp contract(off)
     __shared__ float4 to_cur[AA_PAIRS];
+    __shared__ LdsTable<AA_VSLOTS, 3> vt;
+    __shared__ u32 vused[AA_VSLOTS], wave_count[BS / 64];
     const int s = blockIdx.y;
     const int x0 = (blockIdx.x % tilesX) * AA_T, y0 = (blockIdx.x / tilesX) * AA_T;
     const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
@@ -1459,6 +1465,10 @@ __global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, int tilesX, const float
     const float* gov = gout + off;
     float* gcs = gproj ? gproj + (size_t)s * a.V * 4 : nullptr;
     const int C = a.C;
+    if (gcs) {
+        lds_table_clear(vt);
+        __syncthreads();
+    }
     for (int p = threadIdx.x; p < AA_PAIRS; p += BS) {
         int cx, cy, vertical;
         aa_tile_pair(p, x0, y0, cx, cy, vertical);
@@ -1495,10 +1505,7 @@ __global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, int tilesX, const float
                         float4 c = clipv[vv[j]];
                         float iw = 1.f / c.w;
                         float gvx = gxs[j] * a.sx * iw, gvy = gys[j] * a.sy * iw;
-                        float* gp = gcs + 4 * (size_t)vv[j];
-                        atomicAdd(gp, gvx);
-                        atomicAdd(gp + 1, gvy);
-                        atomicAdd(gp + 3, -(gvx * c.x + gvy * c.y) * iw);
+                        aa_vertex_grad(vt, gcs, vv[j], gvx, gvy, -(gvx * c.x + gvy * c.y) * iw);
                     }
                 }
             }
@@ -1509,22 +1516,25 @@ __global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, int tilesX, const float
     __syncthreads();
     const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
     const int px = x0 + tx, py = y0 + ty;
-    if (px >= a.W || py >= a.H) return;
-    size_t q = (size_t)py * a.W + px;
-    float acc[AA_MAXC];
+    if (px < a.W && py < a.H) {
+        size_t q = (size_t)py * a.W + px;
+        float acc[AA_MAXC];
 #pragma unroll
-    for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? gov[q * C + c] : 0.f;
+        for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? gov[q * C + c] : 0.f;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        float4 v = to_cur[aa_pair_slot(tx, ty, k)];
-        float vv[4] = {v.x, v.y, v.z, v.w};
+        for (int k = 0; k < 4; k++) {
+            float4 v = to_cur[aa_pair_slot(tx, ty, k)];
+            float vv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-        for (int c = 0; c < AA_MAXC; c++) acc[c] = k < 2 ? acc[c] + vv[c] : acc[c] - vv[c];
+            for (int c = 0; c < AA_MAXC; c++) acc[c] = k < 2 ? acc[c] + vv[c] : acc[c] - vv[c];
+        }
+        float* o = gin + off + q * C;
+#pragma unroll
+        for (int c = 0; c < AA_MAXC; c++)
+            if (c < C) o[c] = acc[c];
     }
-    float* o = gin + off + q * C;
-#pragma unroll
-    for (int c = 0; c < AA_MAXC; c++)
-        if (c < C) o[c] = acc[c];
+    // the pair loop's LDS table updates finished at the barrier above
+    if (gcs) lds_vertex_grad_flush(vt, vused, wave_count, gcs);
 }
 
 // ---------------------------------------------------------------- MSE

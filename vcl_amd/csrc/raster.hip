@@ -741,6 +741,7 @@ constexpr int RB_SLOTS = 1024;
 __global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float xs, float ys, const float* proj, const u32* idx,
                                                     const float* rast, const float* grast, float* gproj, int tilesX) {
     __shared__ LdsTable<RB_SLOTS, 3> tab;
+    __shared__ u32 used[RB_SLOTS], wave_count[4];
     lds_table_clear(tab);
     __syncthreads();
     int view = blockIdx.y;
@@ -748,16 +749,21 @@ __global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float x
     int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
     const float4* clip = (const float4*)proj + (size_t)view * V;
     float* gp = gproj + (size_t)view * V * 4;
+    bool act = false;
+    int id = -1;
+    u32 vi[3] = {0u, 0u, 0u};
+    float gv[3][3];
     if (px < W && py < H) {
         size_t p = ((size_t)view * H + py) * W + px;
         float4 r = ((const float4*)rast)[p];
-        int id = (int)r.w - 1;
+        id = (int)r.w - 1;
         float2 g = id >= 0 ? ((const float2*)grast)[p] : make_float2(0.f, 0.f);
         int bitcheck = __float_as_int(g.x) | __float_as_int(g.y);
         if (id >= 0 && (bitcheck << 1) != 0) {
+            act = true;
             float pixx = (float)((px + 0.5) * (double)xs - 1.0);
             float pixy = (float)((py + 0.5) * (double)ys - 1.0);
-            u32 vi[3] = {idx[3 * id], idx[3 * id + 1], idx[3 * id + 2]};
+            vi[0] = idx[3 * id]; vi[1] = idx[3 * id + 1]; vi[2] = idx[3 * id + 2];
             float4 c0 = clip[vi[0]], c1 = clip[vi[1]], c2 = clip[vi[2]];
             float p0x = c0.w * pixx - c0.x, p0y = c0.w * pixy - c0.y;
             float p1x = c1.w * pixx - c1.x, p1y = c1.w * pixy - c1.y;
@@ -766,7 +772,6 @@ __global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float x
             float aa = e0x * e1y - e0y * e1x;
             float ia = 1.f / (aa + (aa > 0 ? 1e-6f : -1e-6f));
             float kuv = r.x * g.x + r.y * g.y;
-            float gv[3][3];
             gv[0][0] = (g.y * p2y + kuv * e1y) * ia;
             gv[0][1] = (-g.y * p2x - kuv * e1x) * ia;
             gv[0][2] = -gv[0][0] * pixx - gv[0][1] * pixy;
@@ -776,29 +781,51 @@ __global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float x
             gv[2][0] = ((-g.y * p0y + g.x * p1y) - kuv * (-p0y + p1y)) * ia;
             gv[2][1] = ((g.y * p0x - g.x * p1x) - kuv * (p0x - p1x)) * ia;
             gv[2][2] = -gv[2][0] * pixx - gv[2][1] * pixy;
+        }
+    }
+    // Neighbouring pixels mostly share a triangle and so the same three vertex slots, whose LDS
+    // atomics would serialise: the grads of a triangle covering RB_GROUP or more of the wave's
+    // lanes are summed over the wave first and kept by its first lane, which alone adds them to
+    // the table; lanes of smaller groups add their own.
+    constexpr int RB_GROUP = 4;
+    const int lane = threadIdx.x & 63;
+    bool add = false;
+    u64 pending = __ballot(act);
+    while (pending) {
+        int leader = __ffsll((unsigned long long)pending) - 1;
+        int lid = __shfl(id, leader, 64);
+        bool mine = act && id == lid;
+        u64 m = __ballot(mine);
+        pending &= ~m;
+        if (__popcll(m) >= RB_GROUP) {
 #pragma unroll
-            for (int k = 0; k < 3; k++) {
-                int sl = lds_table_slot(tab, vi[k]);
-                if (sl >= 0) {
-                    atomicAdd(&tab.val[0][sl], gv[k][0]);
-                    atomicAdd(&tab.val[1][sl], gv[k][1]);
-                    atomicAdd(&tab.val[2][sl], gv[k][2]);
-                } else {
-                    float* q = gp + 4 * (size_t)vi[k];
-                    atomicAdd(q, gv[k][0]); atomicAdd(q + 1, gv[k][1]); atomicAdd(q + 3, gv[k][2]);
+            for (int k = 0; k < 3; k++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    float t = wave_sum(mine ? gv[k][j] : 0.f);
+                    if (lane == leader) gv[k][j] = t;
                 }
+            add |= lane == leader;
+        } else {
+            add |= mine;
+        }
+    }
+    if (add) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            int sl = lds_table_slot(tab, vi[k]);
+            if (sl >= 0) {
+                atomicAdd(&tab.val[0][sl], gv[k][0]);
+                atomicAdd(&tab.val[1][sl], gv[k][1]);
+                atomicAdd(&tab.val[2][sl], gv[k][2]);
+            } else {
+                float* q = gp + 4 * (size_t)vi[k];
+                atomicAdd(q, gv[k][0]); atomicAdd(q + 1, gv[k][1]); atomicAdd(q + 3, gv[k][2]);
             }
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < RB_SLOTS; i += 256) {
-        u32 k = tab.key[i];
-        if (k == LDS_EMPTY) continue;
-        float* q = gp + 4 * (size_t)k;
-        atomicAdd(q, tab.val[0][i]);
-        atomicAdd(q + 1, tab.val[1][i]);
-        atomicAdd(q + 3, tab.val[2][i]);
-    }
+    lds_vertex_grad_flush(tab, used, wave_count, gp);
 }
 
 }  // namespace
