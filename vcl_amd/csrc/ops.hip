@@ -1066,17 +1066,28 @@ __device__ __forceinline__ float nrcp(float x) { return __builtin_amdgcn_rcpf(x)
 // AddNaNcheck for an accumulator that is finite by construction.
 __device__ __forceinline__ void acc_checked(float& a, float b) { float v = a + b; a = isfinite(v) ? v : a; }
 
-struct SgLobe { float gx, gy, gz, lam, mx, my, mz, pad, a[4]; };
+struct SgLobe { float gx, gy, gz, lam, mx, my, mz, lamL2E, a[4]; };  // lamL2E = lam * log2(e)
 
 __device__ __forceinline__ void sg_stage_lobes(SgLobe* L, int K, int C, const float* axis, const float* sharp, const float* amp) {
     for (int i = threadIdx.x; i < K; i += blockDim.x) {
         float lam = sharp[i];
         SgLobe l;
         l.mx = axis[3 * i]; l.my = axis[3 * i + 1]; l.mz = axis[3 * i + 2];
-        l.gx = l.mx * lam; l.gy = l.my * lam; l.gz = l.mz * lam; l.lam = lam; l.pad = 0.f;
+        l.gx = l.mx * lam; l.gy = l.my * lam; l.gz = l.mz * lam; l.lam = lam; l.lamL2E = lam * L2E;
         for (int k = 0; k < 4; k++) l.a[k] = k < C ? amp[i * C + k] : 0.f;
         L[i] = l;
     }
+}
+
+// Backward form of a lobe term with the exponent's constant part c = (base + lam) * log2(e)
+// given: one rsq gives 1/l and l, and each exponent is one fma.
+__device__ __forceinline__ float sg_term_c(float wx, float wy, float wz, float c, float& il, float& e1, float& e2) {
+    float d = wx * wx + wy * wy + wz * wz;
+    il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
+    float l = d * il;
+    e1 = nexp2(fmaf(l, L2E, -c));
+    e2 = nexp2(fmaf(-l, L2E, -c));
+    return (e1 - e2) * il;
 }
 
 // One lobe term: g = (e^{l-c} - e^{-l-c}) / max(l, 1e-6), plus (optionally) dg/dl.
@@ -1150,6 +1161,31 @@ __device__ __forceinline__ float sg_term_rsq(float wx, float wy, float wz, float
 
 constexpr float SG_E2_DROP = 8.75f;
 
+// The lobe loop of one pixel; CHECKED selects the per-channel AddNaNcheck form (a block whose
+// lobes hold a non-finite amplitude).  The choice is uniform per block, so the two forms are
+// separate loops rather than a branch inside one.
+template <bool CHECKED>
+__device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float dnx, float dny, float dnz, float snx, float sny, float snz,
+                                              float ss, float sa, float4& de, float4& se) {
+    const float dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz), slen = sqrtf(snx * snx + sny * sny + snz * snz);
+    for (int i = 0; i < K; i++) {
+        const float4 g = lobes[i].g, a = lobes[i].a;
+        const float gl = lobes[i].len;
+        const bool dboth = !__all(fabsf(gl - dlen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + dlen));
+        const bool sboth = !__all(fabsf(gl - slen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + slen));
+        float dg = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, SG_DS, g.w, dboth) * SG_DA;
+        float sg = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ss, g.w, sboth) * sa;
+        if (!CHECKED) {
+            float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
+            de.x += dv * a.x; de.y += dv * a.y; de.z += dv * a.z; de.w += dv * a.w;
+            se.x += sv * a.x; se.y += sv * a.y; se.z += sv * a.z; se.w += sv * a.w;
+        } else {
+            acc_checked(de.x, dg * a.x); acc_checked(de.y, dg * a.y); acc_checked(de.z, dg * a.z); acc_checked(de.w, dg * a.w);
+            acc_checked(se.x, sg * a.x); acc_checked(se.y, sg * a.y); acc_checked(se.z, sg * a.z); acc_checked(se.w, sg * a.w);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
                                                 const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
                                                 const float* sAmp, float* out, float* denv, float* senv) {
@@ -1176,23 +1212,8 @@ __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* ra
         float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
         float ss = sSharp[p], sa = sAmp[p];
         float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
-        const float dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz), slen = sqrtf(snx * snx + sny * sny + snz * snz);
-        for (int i = 0; i < K; i++) {
-            const float4 g = lobes[i].g, a = lobes[i].a;
-            const float gl = lobes[i].len;
-            const bool dboth = !__all(fabsf(gl - dlen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + dlen));
-            const bool sboth = !__all(fabsf(gl - slen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + slen));
-            float dg = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, SG_DS, g.w, dboth) * SG_DA;
-            float sg = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ss, g.w, sboth) * sa;
-            if (!checked) {
-                float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
-                de.x += dv * a.x; de.y += dv * a.y; de.z += dv * a.z; de.w += dv * a.w;
-                se.x += sv * a.x; se.y += sv * a.y; se.z += sv * a.z; se.w += sv * a.w;
-            } else {
-                acc_checked(de.x, dg * a.x); acc_checked(de.y, dg * a.y); acc_checked(de.z, dg * a.z); acc_checked(de.w, dg * a.w);
-                acc_checked(se.x, sg * a.x); acc_checked(se.y, sg * a.y); acc_checked(se.z, sg * a.z); acc_checked(se.w, sg * a.w);
-            }
-        }
+        if (checked) sg_fwd4_lobes<true>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, de, se);
+        else sg_fwd4_lobes<false>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, de, se);
     }
     float4 df = cov ? ((const float4*)diffuse)[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     ((float4*)out)[p] = cov ? make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w)
@@ -1206,7 +1227,7 @@ __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* ra
 // LDS (deterministic order), every wave walks them in the same order (LDS broadcast), and the
 // eight lobe sums stay in registers across all chunks of the block.  grad.diffuse is
 // gout * diffenv (the forward's lobe-ordered sum) written in the staging pass.
-struct SgPix { float dnx, dny, dnz, ss, snx, sny, snz, sa, X[4], G[4]; };
+struct SgPix { float dnx, dny, dnz, ssL2E, snx, sny, snz, sa, X[4], G[4]; };  // ssL2E = spec sharpness * log2(e)
 
 __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int KP, const float* rast, const float* axis, const float* sharp,
                                                      const float* amp, const float* normal, const float* diffuse, const float* sAxis,
@@ -1232,8 +1253,9 @@ __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int
             if (gdiffuse)
                 for (int k = 0; k < C; k++) add_nan_check(gdiffuse[p * C + k], go[k] * denv[p * C + k]);
             q.dnx = normal[3 * p] * SG_DS; q.dny = normal[3 * p + 1] * SG_DS; q.dnz = normal[3 * p + 2] * SG_DS;
-            q.ss = sSharp[p]; q.sa = sAmp[p];
-            q.snx = sAxis[3 * p] * q.ss; q.sny = sAxis[3 * p + 1] * q.ss; q.snz = sAxis[3 * p + 2] * q.ss;
+            float ss = sSharp[p];
+            q.ssL2E = ss * L2E; q.sa = sAmp[p];
+            q.snx = sAxis[3 * p] * ss; q.sny = sAxis[3 * p + 1] * ss; q.snz = sAxis[3 * p + 2] * ss;
             for (int k = 0; k < 4; k++) { q.X[k] = go[k] * df[k]; q.G[k] = go[k]; }
         }
         if (!want_lobes) continue;
@@ -1254,12 +1276,12 @@ __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int
 #pragma unroll
                 for (int k = 0; k < 4; k++) { sdiff += x.X[k] * L.a[k]; ssum += x.G[k] * L.a[k]; }
                 float wx = x.dnx + L.gx, wy = x.dny + L.gy, wz = x.dnz + L.gz;
-                float l, il, e1, e2;
-                float dgv = sg_term(wx, wy, wz, SG_DS, L.lam, l, il, e1, e2);
+                float il, e1, e2;
+                float dgv = sg_term_c(wx, wy, wz, L.lamL2E + SG_DS * L2E, il, e1, e2);
                 float dgdl = ((1.f - il) * e1 + (1.f + il) * e2) * il;
                 float qx = x.snx + L.gx, qy = x.sny + L.gy, qz = x.snz + L.gz;
-                float sl, isl, f1, f2;
-                float sgv = sg_term(qx, qy, qz, x.ss, L.lam, sl, isl, f1, f2);
+                float isl, f1, f2;
+                float sgv = sg_term_c(qx, qy, qz, L.lamL2E + x.ssL2E, isl, f1, f2);
                 float sgdl = ((1.f - isl) * f1 + (1.f + isl) * f2) * isl;
 #pragma unroll
                 for (int k = 0; k < 4; k++) acc[4 + k] += x.X[k] * (dgv * SG_DA) + x.G[k] * (sgv * x.sa);
