@@ -93,6 +93,38 @@ def test_raster_soup_ties_and_clipping(gpu):
     _check_raster(gpu, clip, idx, 96, 80, what="soup")
 
 
+def test_raster_occlusion_layers(gpu):
+    # heavy overdraw of large perspective triangles (whole-tile covers at many depths, so the
+    # fine raster's occlusion culling is active): near-ties 1e-7 apart in depth, triangles
+    # crossing w = 0 and vertex w varying across each triangle (z not affine in the piece)
+    rng = np.random.default_rng(23)
+    n = 600
+    ctr = rng.uniform(-0.6, 0.6, (n, 1, 2))
+    xy = ctr + rng.uniform(-1.2, 1.2, (n, 3, 2))
+    w = rng.uniform(0.6, 1.8, (n, 3, 1))
+    z = rng.uniform(-0.95, 0.95, (n, 1, 1)) + rng.uniform(-0.05, 0.05, (n, 3, 1))
+    clip = np.concatenate([xy * w, z * w, w], 2).astype(F32)
+    clip[1::7] = clip[0::7][: len(clip[1::7])]
+    clip[1::7, :, 2] += (1e-7 * clip[1::7, :, 3]).astype(F32)
+    clip[3::29, 0, 3] = -clip[3::29, 0, 3]
+    clip = clip.reshape(-1, 4)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    for W, H in ((128, 96), (75, 53)):
+        out, _ = _check_raster(gpu, clip, idx, W, H, what=f"occlusion layers {W}x{H}")
+        assert (out[..., 3] > 0).mean() > 0.9
+
+
+def test_raster_crumpled_mesh(gpu):
+    # a closed mesh whose vertices are scattered as by a diverged fit: long triangles folded over
+    # each other many layers deep
+    unit, faces = S.icosphere(4)
+    rng = np.random.default_rng(29)
+    pos = (unit * (1.0 + rng.normal(0, 0.25, (len(unit), 1)))).astype(F32)
+    cam = S.make_camera((0.0, 0.0, 3.0))
+    clip = o_project(pos, cam.mat)[0]
+    _check_raster(gpu, clip, faces.astype(np.uint32), 200, 152, what="crumpled mesh")
+
+
 def test_raster_deep_bins(gpu):
     # thousands of small overlapping triangles stacked over a few tiles: bins far deeper than
     # one fine-raster chunk (256 pieces), so the partial depth keys of several workgroups merge

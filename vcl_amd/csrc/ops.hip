@@ -1155,7 +1155,13 @@ __device__ __forceinline__ float sg_term_rsq(float wx, float wy, float wz, float
     float il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
     float l = d * il;
     float e1 = nexp2(((l - base) - lam) * L2E);
-    float e2 = both ? nexp2(((-l - base) - lam) * L2E) : 0.f;
+    float e2 = 0.f;
+    if (both) {
+        // both is wave-uniform; the empty asm keeps this a branch the wave skips rather than
+        // an exp computed on every lobe and selected away
+        asm volatile("" :::);
+        e2 = nexp2(((-l - base) - lam) * L2E);
+    }
     return (e1 - e2) * il;
 }
 

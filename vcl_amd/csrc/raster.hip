@@ -59,6 +59,7 @@ constexpr int LARGE_BLOCKS = 512;
 // the partial depth keys.
 constexpr u32 CHUNK_SCAN_ALL = 0x80000000u;   // count flag: scan every piece of every layer
 constexpr int FINE_MAX_GRID = 65536;
+constexpr int CULL_MIN_BOX = 32;   // tile pixels in a piece's box for its depth range to be formed
 
 struct Chunk {
     u32 tile, first, count, k;   // pieces [first, first + count) of the bin; k-th chunk of the tile
@@ -90,7 +91,7 @@ Layout make_layout(int W, int H, int depth, int T, int tshift) {
     size_t o = 0;
     L.off_counts = o; o = al(o + sizeof(u32) * (size_t)L.NT);
     L.off_offsets = o; o = al(o + sizeof(u32) * ((size_t)L.NT + 1));
-    L.off_misc = o; o = al(o + sizeof(u32) * 64);   // [0]=overflow-piece count [2]=overflow flag [3]=large count
+    L.off_misc = o; o = al(o + sizeof(u32) * 64);   // [0]=overflow-piece count [2]=overflow flag [3]=large count [5]=chunks
     L.off_rec = o; o = al(o + sizeof(RRec) * L.nrec_cap);
     L.off_bins = o; o = al(o + sizeof(u32) * L.bin_cap);
     L.off_large = o; o = al(o + sizeof(u32) * L.nrec_cap);
@@ -587,17 +588,77 @@ __device__ void fine_resolve(const RastArgs& a, u32 view, int px, int py, float 
     if (a.outDB) ((float4*)a.outDB)[p] = d;
 }
 
+// ---- tile-level occlusion culling in the fine raster
+// The per-pixel depth is z = Zp / Wp with Zp = (z0 x + z1 y) + z2 (float, uncontracted) and the
+// same for Wp and Dp.  Over a rectangle of pixel centres on which the exact Wp keeps one sign,
+// the exact z (linear-fractional) takes its extremes at the corners.  Any float evaluation of
+// the same expressions (per pixel, or here at the corners) differs from the exact z by at most
+// d = (eZ + |z| eW) / (|Wp| - 2 eW) + ulp(z), with |Wp| the smallest float |Wp| at the corners
+// and eZ = 2 * 2^-23 * (|z0 x| + |z1 y| + |z2|) bounding the four roundings of Zp (likewise eW,
+// eD); the corner values therefore bound every pixel's float z within 2 d, and eZ, eW are taken
+// 4x larger to cover the float evaluation of the bound itself.  A
+// piece that covers every pixel centre of the tile and passes every pixel test there writes a
+// key of depth <= its upper bound at every pixel, so a piece whose lower bound is above that
+// can never win a pixel and is skipped.
+
+// Conservative [lo, hi] of the float per-pixel z of a piece over the rectangle with pixel-centre
+// corners (xa, ya), (xb, yb); false when Wp may vanish in it or a value is not finite.  valid:
+// every pixel of the rectangle passes the Wp / Dp sign test and -1 <= z <= 1.
+__device__ __forceinline__ bool piece_depth_range(const float* zp, float xa, float xb, float ya, float yb, float& lo, float& hi, bool& valid) {
+    float zmin = INFINITY, zmax = -INFINITY, wmin = INFINITY, dmin = INFINITY, SZ = 0.f, SW = 0.f, SD = 0.f;
+    int wpos = 0, wneg = 0, dpos = 0, dneg = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const float x = (c & 1) ? xb : xa, y = (c & 2) ? yb : ya;
+        const float z0 = zp[0] * x, z1 = zp[1] * y, w0 = zp[3] * x, w1 = zp[4] * y, d0 = zp[6] * x, d1 = zp[7] * y;
+        const float Z = z0 + z1 + zp[2], Wv = w0 + w1 + zp[5], Dv = d0 + d1 + zp[8];
+        SZ = fmaxf(SZ, fabsf(z0) + fabsf(z1) + fabsf(zp[2]));
+        SW = fmaxf(SW, fabsf(w0) + fabsf(w1) + fabsf(zp[5]));
+        SD = fmaxf(SD, fabsf(d0) + fabsf(d1) + fabsf(zp[8]));
+        wpos += Wv > 0.f; wneg += Wv < 0.f; dpos += Dv > 0.f; dneg += Dv < 0.f;
+        wmin = fminf(wmin, fabsf(Wv)); dmin = fminf(dmin, fabsf(Dv));
+        const float z = Z / Wv;
+        zmin = fminf(zmin, z); zmax = fmaxf(zmax, z);
+    }
+    if (!(isfinite(SZ) && isfinite(SW) && isfinite(SD))) return false;
+    const float U = 8.0f * 0x1p-23f;
+    const float eZ = U * SZ + 1e-30f, eW = U * SW + 1e-30f, eD = U * SD + 1e-30f;
+    if (!(wpos == 4 || wneg == 4) || !(wmin > 4.f * eW) || !isfinite(zmin) || !isfinite(zmax)) return false;
+    const float za = fmaxf(fabsf(zmin), fabsf(zmax));
+    const float d = 2.f * ((eZ + za * eW) / (wmin - 2.f * eW) + 0x1p-21f * (za + 1.f));
+    lo = zmin - 1.01f * d;
+    hi = zmax + 1.01f * d;
+    valid = ((wpos == 4 && dpos == 4) || (wneg == 4 && dneg == 4)) && dmin > eD && lo >= -1.0f && hi <= 1.0f;
+    return true;
+}
+
+// The piece covers every pixel centre of [px0, px1] x [py0, py1]: its integer edge functions
+// (with the top-left offset, inside <=> > 0) are positive at the four corners, so everywhere.
+__device__ __forceinline__ bool piece_covers_rect(const RRec& r, int px0, int px1, int py0, int py1) {
+    const long long X0 = (long long)px0 * 256 + 128, X1 = (long long)px1 * 256 + 128;
+    const long long Y0 = (long long)py0 * 256 + 128, Y1 = (long long)py1 * 256 + 128;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        const long long c = (long long)r.C[i] + ((r.view_tl >> i) & 1u);
+        const long long ax0 = (long long)r.A[i] * X0, ax1 = (long long)r.A[i] * X1;
+        const long long by0 = (long long)r.B[i] * Y0, by1 = (long long)r.B[i] * Y1;
+        if (ax0 + by0 + c <= 0 || ax1 + by0 + c <= 0 || ax0 + by1 + c <= 0 || ax1 + by1 + c <= 0) return false;
+    }
+    return true;
+}
+
 // One 256-thread workgroup per work-list chunk (up to `chunk` pieces of one TS x TS tile's bin,
-// so a deep bin is spread over several workgroups).  Thread i takes piece i of the chunk and
-// walks its exact covered column span in each of its rows (bisection on the integer edge
-// functions, so no per-pixel edge test), evaluating the piece's depth planes at each covered
-// pixel centre and taking the LDS 64-bit minimum of the (ordered z_ndc, triangle id) key; the
-// result therefore does not depend on bin order, tile size or how the bin is split.  A tile
-// with one chunk resolves its pixels at once; otherwise each chunk stores its TS*TS partial
-// keys and the last chunk of the tile to finish (device-scope counter) takes the minimum over
-// all of them.
+// so a deep bin is spread over several workgroups).  Per batch of pieces: (1a) thread = piece
+// forms the piece's conservative depth range over the tile, and pieces covering the whole tile
+// lower the tile's depth bound; (1b) the pieces not behind that bound find their exact covered
+// column span in each row (bisection on the integer edge functions, so no per-pixel edge test);
+// (2) a block prefix turns the covered rows into items; (3) thread = item evaluates the depth
+// planes at each covered pixel centre and takes the LDS 64-bit minimum of the (ordered z_ndc,
+// triangle id) key.  The result does not depend on bin order, tile size, culling or how the
+// bin is split: a tile's only chunk stores its keys, the chunks of a split tile merge theirs
+// with a device-scope 64-bit minimum.
 template <int TS>
-__global__ void __launch_bounds__(256, 6) k_fine(RastArgs a) {
+__global__ void __launch_bounds__(256, TS == 8 ? 5 : 4) k_fine(RastArgs a) {
     constexpr int NP = TS * TS;
     __shared__ u64 key[NP];
     __shared__ float spix[2][TS];
@@ -607,6 +668,7 @@ __global__ void __launch_bounds__(256, 6) k_fine(RastArgs a) {
     __shared__ u32 sid[BATCH];
     __shared__ u32 sitem[BATCH * TS];               // covered rows: piece | row | first column | length
     __shared__ u32 sbucket[4];                       // per-wave covered-row totals
+    __shared__ u32 sbound;                           // ordered depth bound of the tile (culling)
     const int tid = threadIdx.x;
     const u32 nchunks = a.ws.misc[5];
     for (u32 c = blockIdx.x; c < nchunks; c += gridDim.x) {
@@ -616,6 +678,7 @@ __global__ void __launch_bounds__(256, 6) k_fine(RastArgs a) {
         const int tl = tile % a.ntiles;
         const int tx = tl % a.tilesX, ty = tl / a.tilesX;
         for (int i = tid; i < NP; i += 256) key[i] = ~0ull;
+        if (tid == 0) sbound = 0xffffffffu;
         if (tid < 2 * TS) {
             int cc = tid % TS;
             spix[tid / TS][cc] = tid < TS ? (float)((tx * TS + cc + 0.5) * (double)a.xs - 1.0)
@@ -626,20 +689,42 @@ __global__ void __launch_bounds__(256, 6) k_fine(RastArgs a) {
         const bool scan_all = (ch.count & CHUNK_SCAN_ALL) != 0u;
         const u32 n = ch.count & ~CHUNK_SCAN_ALL;
         __syncthreads();
+        // the tile's pixel rectangle inside the image, for the culling bounds
+        const int rx1 = min(x0 + TS, a.W) - 1 - x0, ry1 = min(y0 + TS, a.H) - 1 - y0;
         for (u32 base = 0; base < n; base += BATCH) {
-            // phase 1: thread = piece; exact row spans and depth planes into LDS; covered rows
-            // counted
+            // phase 1a: thread = piece; conservative depth range over the tile; pieces covering
+            // the whole tile with every pixel test passing lower the tile's depth bound
             const u32 q = base + tid;
             int yl = 1, yh = 0;
             u32 nrow = 0;
+            bool live = false;
+            float zlo = 0.f;
+            bool has_lo = false;
+            RRec r;   // the piece's record, held in registers across the barrier below
             if (tid < BATCH && q < n) {
-                u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
-                const RRec& r = a.ws.rec[slot];
+                const u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
+                r = a.ws.rec[slot];
+                int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
+                int yl0 = max((r.pyr & 0xffff) - y0, 0), yh0 = min((r.pyr >> 16) - y0, TS - 1);
+                live = r.id != INVALID && (r.view_tl >> 8) == view && xl <= xh && yl0 <= yh0;
+                // a piece whose box holds few of the tile's pixels saves little when culled and
+                // cannot cover the tile, so its range is not formed
+                if (live && (xh - xl + 1) * (yh0 - yl0 + 1) >= CULL_MIN_BOX) {
+                    float hi;
+                    bool valid;
+                    has_lo = piece_depth_range(r.zp, spix[0][0], spix[0][rx1], spix[1][0], spix[1][ry1], zlo, hi, valid);
+                    if (has_lo && valid && piece_covers_rect(r, x0, x0 + rx1, y0, y0 + ry1)) atomicMin(&sbound, ord_f32(hi));
+                }
+            }
+            __syncthreads();
+            // phase 1b: pieces not behind the bound: exact row spans and depth planes into LDS;
+            // covered rows counted
+            if (live && !(has_lo && ord_f32(zlo) > sbound)) {
                 u32 id = r.id, vtl = r.view_tl;
                 int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
                 yl = max((r.pyr & 0xffff) - y0, 0);
                 yh = min((r.pyr >> 16) - y0, TS - 1);
-                if (id != INVALID && (vtl >> 8) == view && xl <= xh && yl <= yh) {
+                {
                     long long K[3], dA[3], dB[3];
                     long long Y = (long long)(y0 + yl) * 256 + 128;
 #pragma unroll
@@ -659,8 +744,6 @@ __global__ void __launch_bounds__(256, 6) k_fine(RastArgs a) {
 #pragma unroll
                     for (int k = 0; k < 9; k++) szp[k][tid] = r.zp[k];
                     sid[tid] = id;
-                } else {
-                    yl = 1; yh = 0;
                 }
             }
             // phase 2: block prefix of the covered-row counts; every covered row becomes an item
@@ -692,15 +775,18 @@ __global__ void __launch_bounds__(256, 6) k_fine(RastArgs a) {
                 const float z0 = szp[0][j], z2 = szp[2][j], w0 = szp[3][j], w2 = szp[5][j], d0 = szp[6][j], d2 = szp[8][j];
                 const float zy = szp[1][j] * pixy, wy = szp[4][j] * pixy, dy = szp[7][j] * pixy;
                 const u32 id = sid[j];
+                // branch-free body (the quotient is formed for every pixel, the minimum taken only
+                // where the tests pass); the pixel-centre x is recomputed (the same expression as
+                // spix) rather than read, so the loop holds no LDS read behind the LDS atomics
+#pragma unroll 2
                 for (int x = lo; x < lo + len; x++) {
-                    const float pixx = spix[0][x];
-                    float Zp = z0 * pixx + zy + z2;
-                    float Wp = w0 * pixx + wy + w2;
-                    float Dp = d0 * pixx + dy + d2;
-                    if (!((Wp > 0.0f && Dp > 0.0f) || (Wp < 0.0f && Dp < 0.0f))) continue;
-                    float z = Zp / Wp;
-                    if (!(z >= -1.0f && z <= 1.0f)) continue;
-                    atomicMin(&key[y * TS + x], ((u64)ord_f32(z) << 32) | id);
+                    const float pixx = (float)((x0 + x + 0.5) * (double)a.xs - 1.0);
+                    const float Zp = z0 * pixx + zy + z2;
+                    const float Wp = w0 * pixx + wy + w2;
+                    const float Dp = d0 * pixx + dy + d2;
+                    const float z = Zp / Wp;
+                    const bool ok = ((Wp > 0.0f) & (Dp > 0.0f)) | ((Wp < 0.0f) & (Dp < 0.0f));
+                    if (ok & (z >= -1.0f) & (z <= 1.0f)) atomicMin(&key[y * TS + x], ((u64)ord_f32(z) << 32) | id);
                 }
             }
             __syncthreads();
@@ -831,6 +917,7 @@ __global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float x
 }  // namespace
 
 static int tile_shift_of(int tile) { return tile == 8 ? 3 : tile == 32 ? 5 : 4; }
+
 
 // The workspace fits every tile size, so the tuning may change between calls.
 extern "C" size_t vcl_rasterize_workspace_size(int width, int height, int depth, int projNum, int idxNum) {
