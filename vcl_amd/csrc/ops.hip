@@ -1249,6 +1249,7 @@ __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int
     float acc[SG_COLS];
 #pragma unroll
     for (int k = 0; k < SG_COLS; k++) acc[k] = 0.f;
+    float accW[3] = {0.f, 0.f, 0.f}, accS = 0.f, accX[4] = {0.f, 0.f, 0.f, 0.f}, accG[4] = {0.f, 0.f, 0.f, 0.f};
     for (size_t base = (size_t)blockIdx.x * BS; base < P; base += (size_t)gridDim.x * BS) {
         size_t p = base + tid;
         bool cov = p < P && (int)rast[4 * p + 3] - 1 >= 0;
@@ -1275,31 +1276,49 @@ __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int
         if (cov) pix[off + wpos] = q;
         __syncthreads();
         if (li < K) {
+            // The thread's lobe (axis mu, sharpness lam) is fixed, so the lobe-constant factors
+            // come out of the pixel sum: with cd, cs the diffuse / specular chain factors,
+            //   d/d(axis)      = lam * sum(cd w + cs q)
+            //   d/d(sharpness) = mu . sum(cd w + cs q) - sum(A dg + B sg)
+            //   d/d(amplitude) = DA * sum(X dg) + sum(G sg sa)
+            // (mu . w enters the sharpness term only through cd mu . w), and dg/dl =
+            // il ((e1 + e2) - g) for g = (e1 - e2) il.
             const SgLobe& L = lobes[li];
+            const float cD = L.lamL2E + SG_DS * L2E;
             for (int j = grp; j < ncov; j += G) {
                 const SgPix& x = pix[j];
                 float sdiff = 0.f, ssum = 0.f;
 #pragma unroll
                 for (int k = 0; k < 4; k++) { sdiff += x.X[k] * L.a[k]; ssum += x.G[k] * L.a[k]; }
+                const float A = sdiff * SG_DA, B = ssum * x.sa;
                 float wx = x.dnx + L.gx, wy = x.dny + L.gy, wz = x.dnz + L.gz;
                 float il, e1, e2;
-                float dgv = sg_term_c(wx, wy, wz, L.lamL2E + SG_DS * L2E, il, e1, e2);
-                float dgdl = ((1.f - il) * e1 + (1.f + il) * e2) * il;
+                float dgv = sg_term_c(wx, wy, wz, cD, il, e1, e2);
+                float dgdl = ((e1 + e2) - dgv) * il;
                 float qx = x.snx + L.gx, qy = x.sny + L.gy, qz = x.snz + L.gz;
                 float isl, f1, f2;
                 float sgv = sg_term_c(qx, qy, qz, L.lamL2E + x.ssL2E, isl, f1, f2);
-                float sgdl = ((1.f - isl) * f1 + (1.f + isl) * f2) * isl;
+                float sgdl = ((f1 + f2) - sgv) * isl;
+                const float sgvsa = sgv * x.sa;
 #pragma unroll
-                for (int k = 0; k < 4; k++) acc[4 + k] += x.X[k] * (dgv * SG_DA) + x.G[k] * (sgv * x.sa);
-                float cd = sdiff * SG_DA * dgdl * il, cs = ssum * x.sa * sgdl * isl;
-                acc[0] += (cd * wx + cs * qx) * L.lam;
-                acc[1] += (cd * wy + cs * qy) * L.lam;
-                acc[2] += (cd * wz + cs * qz) * L.lam;
-                float mdw = L.mx * wx + L.my * wy + L.mz * wz, mdq = L.mx * qx + L.my * qy + L.mz * qz;
-                acc[3] += sdiff * SG_DA * (dgdl * mdw * il - dgv) + ssum * x.sa * (sgdl * mdq * isl - sgv);
+                for (int k = 0; k < 4; k++) { accX[k] += x.X[k] * dgv; accG[k] += x.G[k] * sgvsa; }
+                float cd = A * dgdl * il, cs = B * sgdl * isl;
+                accW[0] += cd * wx + cs * qx;
+                accW[1] += cd * wy + cs * qy;
+                accW[2] += cd * wz + cs * qz;
+                accS -= A * dgv + B * sgv;
             }
         }
         __syncthreads();
+    }
+    if (want_lobes && li < K) {
+        const SgLobe& L = lobes[li];
+        acc[0] = accW[0] * L.lam;
+        acc[1] = accW[1] * L.lam;
+        acc[2] = accW[2] * L.lam;
+        acc[3] = (L.mx * accW[0] + L.my * accW[1] + L.mz * accW[2]) + accS;
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[4 + k] = SG_DA * accX[k] + accG[k];
     }
     if (!want_lobes) return;
 #pragma unroll
