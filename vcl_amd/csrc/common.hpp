@@ -130,3 +130,4 @@ __global__ void vcl_fill_u32(u32* p, size_t n, u32 v);
 int vcl_fill_async(void* p, size_t bytes, hipStream_t s);
 int vcl_fill_u32_async(u32* p, size_t n, u32 v, hipStream_t s);
 int vcl_fill_multi_async(void* const* p, const size_t* bytes, int n, hipStream_t s);
+int vcl_fill_multi_bump_async(void* const* p, const size_t* bytes, int n, double* bt, double rhom, double rhov, hipStream_t s);

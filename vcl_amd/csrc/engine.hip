@@ -376,11 +376,14 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     int W = d.width, H = d.height, nv = d.views, C = d.channel, K = d.sgnum;
     hipStream_t hs = as_hip(s);
     int st;
-    // output-grad zeroing of the forward(XGradParams&) overloads
+    // output-grad zeroing of the forward(XGradParams&) overloads; the same launch advances the
+    // Adam bias-correction powers of this step (Adam::step bumps rhomt/rhovt before its
+    // launch, optimize.cu:109-115), which the updates at the end of the step read
+    double* bt = F->t<double>("adam_bt");
     {
         void* ptrs[2] = {F->buf["gclip"], F->buf["gdiff"]};
         size_t nb[2] = {F->bytes["gclip"], F->bytes["gdiff"]};
-        if ((st = vcl_fill_multi_async(ptrs, nb, 2, hs))) return st;
+        if ((st = vcl_fill_multi_bump_async(ptrs, nb, 2, bt, d.beta1, d.beta2, hs))) return st;
     }
     if ((st = forward(F, s))) return st;
     vcl_loss_params lp = {F->f("image"), F->f("target"), F->f("gimage"), F->f("ws_loss"), W, H, nv, (int)(F->P * C), F->f("loss")};
@@ -429,9 +432,7 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     vcl_project_grad_params pg = {F->f("grad_pos"), nullptr, F->f("gclip")};
     { ProfScope ps_(F, "project_bwd", as_hip(s)); st = vcl_project_backward(&pj, &pg, s); }
     if (st) return st;
-    // update (Adam::step bumps rhomt/rhovt before the launch, optimize.cu:109-115)
-    double* bt = F->t<double>("adam_bt");
-    if ((st = vcl_adam_bump_dev(bt, d.beta1, d.beta2, s))) return st;
+    // update
     vcl_adam_hyper_params ht = {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_tex"), F->f("v_tex")};
     vcl_texture tex, gtex;
     fill_texture(F, "texture", tex);

@@ -20,5 +20,4 @@ int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* gra
 int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
                             vcl_stream_t stream);
 // bt_dev[0] *= rhom; bt_dev[1] *= rhov (one thread; the step-start bump of Adam::step).
-int vcl_adam_bump_dev(double* bt_dev, double rhom, double rhov, vcl_stream_t stream);
 }

@@ -24,17 +24,23 @@ int vcl_fill_u32_async(u32* p, size_t n, u32 v, hipStream_t s) {
 
 int vcl_fill_async(void* p, size_t bytes, hipStream_t s) { return vcl_fill_u32_async((u32*)p, bytes / 4, 0u, s); }
 
-// Zero several buffers in one launch (each block clears 4096 words of one buffer).
+// Zero several buffers in one launch (each block clears 4096 words of one buffer).  Optionally
+// block 0 also advances a pair of Adam bias-correction powers (bt[0] *= rhom, bt[1] *= rhov),
+// so a step that starts with this clear needs no launch of its own for them.
 constexpr int FILL_MAX = 8;
 struct FillMulti {
     u32* p[FILL_MAX];
     size_t n[FILL_MAX];
     unsigned block_start[FILL_MAX + 1];
     int count;
+    double* bt;
+    double rhom, rhov;
 };
 
 __global__ void __launch_bounds__(256) vcl_fill_multi_k(FillMulti f) {
     unsigned b = blockIdx.x;
+    if (f.bt && b == 0 && threadIdx.x == 0) { f.bt[0] *= f.rhom; f.bt[1] *= f.rhov; }
+    if (f.count == 0) return;
     int t = 0;
     while (t + 1 < f.count && b >= f.block_start[t + 1]) t++;
     size_t base = (size_t)(b - f.block_start[t]) * 4096;
@@ -45,9 +51,16 @@ __global__ void __launch_bounds__(256) vcl_fill_multi_k(FillMulti f) {
 }
 
 int vcl_fill_multi_async(void* const* p, const size_t* bytes, int n, hipStream_t s) {
-    for (int base = 0; base < n; base += FILL_MAX) {
+    return vcl_fill_multi_bump_async(p, bytes, n, nullptr, 1.0, 1.0, s);
+}
+
+int vcl_fill_multi_bump_async(void* const* p, const size_t* bytes, int n, double* bt, double rhom, double rhov, hipStream_t s) {
+    for (int base = 0; base < n || (base == 0 && bt); base += FILL_MAX) {
         FillMulti f;
         f.count = 0;
+        f.bt = base == 0 ? bt : nullptr;
+        f.rhom = rhom;
+        f.rhov = rhov;
         unsigned blocks = 0;
         for (int i = base; i < n && i < base + FILL_MAX; i++) {
             if (!p[i] || bytes[i] < 4) continue;
@@ -58,8 +71,8 @@ int vcl_fill_multi_async(void* const* p, const size_t* bytes, int n, hipStream_t
             f.count++;
         }
         f.block_start[f.count] = blocks;
-        if (!blocks) continue;
-        hipLaunchKernelGGL(vcl_fill_multi_k, dim3(blocks), dim3(256), 0, s, f);
+        if (!blocks && !f.bt) continue;
+        hipLaunchKernelGGL(vcl_fill_multi_k, dim3(blocks ? blocks : 1), dim3(256), 0, s, f);
         VCL_LAUNCH_CHECK();
     }
     return VCL_OK;
@@ -70,7 +83,8 @@ namespace {
 constexpr int BS = 256;
 
 // ---------------------------------------------------------------- deterministic column sums
-// out[c] += sum_r part[r * cols + c] in a fixed order (two stages of fixed segments).
+// sum_r part[r * cols + c] in a fixed order: segment sums here, the sum over segments by the
+// consumer (k_sg_lobe_final).
 __global__ void __launch_bounds__(256) k_colsum_stage(const float* part, int rows, int cols, int seg_rows, float* seg) {
     // block: 64 columns x 4 row groups; grid.y: segment
     int c = blockIdx.x * 64 + (threadIdx.x & 63);
@@ -85,30 +99,20 @@ __global__ void __launch_bounds__(256) k_colsum_stage(const float* part, int row
     if (rg == 0 && c < cols) seg[(size_t)blockIdx.y * cols + c] = ((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x];
 }
 
-__global__ void __launch_bounds__(256) k_colsum_final(const float* seg, int nseg, int cols, float scale, float* out, int nancheck) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    float s = 0.f;
-    for (int r = 0; r < nseg; r++) s += seg[(size_t)r * cols + c];
-    s *= scale;
-    if (nancheck) add_nan_check(out[c], s);
-    else out[c] += s;
-}
-
 constexpr int COLSUM_SEGS = 32;
 
 size_t colsum_scratch_floats(int cols) { return (size_t)COLSUM_SEGS * cols; }
 
-int colsum(const float* part, int rows, int cols, float* seg_scratch, float* out, float scale, int nancheck, hipStream_t s) {
+// First stage: segment sums of `rows` rows into seg_scratch; returns the segment count, or -1
+// when the launch failed.
+int colsum_stage(const float* part, int rows, int cols, float* seg_scratch, hipStream_t s) {
     int seg_rows = (rows + COLSUM_SEGS - 1) / COLSUM_SEGS;
     if (seg_rows < 1) seg_rows = 1;
     int nseg = (rows + seg_rows - 1) / seg_rows;
     if (nseg < 1) nseg = 1;
     hipLaunchKernelGGL(k_colsum_stage, dim3(cdiv(cols, 64), nseg), dim3(256), 0, s, part, rows, cols, seg_rows, seg_scratch);
-    VCL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_colsum_final, dim3(cdiv(cols, 256)), dim3(256), 0, s, seg_scratch, nseg, cols, scale, out, nancheck);
-    VCL_LAUNCH_CHECK();
-    return VCL_OK;
+    if (hipGetLastError() != hipSuccess) return -1;
+    return nseg;
 }
 
 // ---------------------------------------------------------------- project
@@ -666,10 +670,6 @@ struct TexUpdArgs {
 
 // Adam::step bumps rhomt/rhovt before the launch (optimize.cu:109-115); kept on the device so
 // a captured step advances them on every replay.
-__global__ void k_adam_bump(double* bt, double rhom, double rhov) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) { bt[0] *= rhom; bt[1] *= rhov; }
-}
-
 __global__ void __launch_bounds__(BS) k_tex_update(TexUpdArgs a) {
 #pragma clang fp contract(off)
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1046,14 +1046,18 @@ __global__ void __launch_bounds__(BS) k_sg_bwd(size_t P, int C, int K, const flo
     }
 }
 
-// scatter the reduced [K][SG_COLS] sums into the three lobe-grad arrays
-__global__ void k_sg_scatter(const float* red, int K, int C, float* gaxis, float* gsharp, float* gamp) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= K) return;
-    const float* r = red + (size_t)i * SG_COLS;
-    if (gaxis) for (int k = 0; k < 3; k++) add_nan_check(gaxis[3 * i + k], r[k]);
-    if (gsharp) add_nan_check(gsharp[i], r[3]);
-    if (gamp) for (int k = 0; k < C; k++) add_nan_check(gamp[i * C + k], r[4 + k]);
+// Final stage of the lobe-grad column sums: column i * SG_COLS + k (lobe i) is the sum of the
+// segment partials in segment order, added (AddNaNcheck) into the axis (k < 3), sharpness
+// (k == 3) or amplitude (k >= 4) grad of lobe i.
+__global__ void __launch_bounds__(256) k_sg_lobe_final(const float* seg, int nseg, int K, int C, float* gaxis, float* gsharp, float* gamp) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K * SG_COLS) return;
+    float s = 0.f;
+    for (int r = 0; r < nseg; r++) s += seg[(size_t)r * K * SG_COLS + c];
+    const int i = c / SG_COLS, k = c % SG_COLS;
+    if (k < 3) { if (gaxis) add_nan_check(gaxis[3 * i + k], s); }
+    else if (k == 3) { if (gsharp) add_nan_check(gsharp[i], s); }
+    else if (k - 4 < C && gamp) add_nan_check(gamp[i * C + (k - 4)], s);
 }
 
 // ---- fast SG kernels: native exp2 / sqrt / rcp (1-ulp hardware ops), lobes staged in LDS.
@@ -1585,12 +1589,25 @@ __global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, int tilesX, const float
 }
 
 // ---------------------------------------------------------------- MSE
-__global__ void __launch_bounds__(BS) k_mse(size_t n, const float* pred, const float* target, float* grad, float* part) {
-    size_t i0 = ((size_t)blockIdx.x * BS + threadIdx.x) * 4;
+// grad = pred - target: four elements per thread (one 16-byte access each when `vec`) and a
+// partial sum of grad^2 per block.  (A last-block-finishes reduction would need one device
+// atomic per block on a single counter; those serialise, so the partials are summed by a
+// second launch.)
+__global__ void __launch_bounds__(BS) k_mse(size_t n, const float* pred, const float* target, float* grad, float* part, int vec) {
+    const size_t t = (size_t)blockIdx.x * BS + threadIdx.x;
     float s = 0.f;
-    for (int k = 0; k < 4; k++) {
-        size_t i = i0 + k;
-        if (i < n) { float g = pred[i] - target[i]; grad[i] = g; s += g * g; }
+    if (vec) {
+        if (4 * t < n) {
+            const float4 a = ((const float4*)pred)[t], b = ((const float4*)target)[t];
+            const float4 g = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+            ((float4*)grad)[t] = g;
+            s = ((g.x * g.x + g.y * g.y) + g.z * g.z) + g.w * g.w;
+        }
+    } else {
+        for (int k = 0; k < 4; k++) {
+            const size_t i = 4 * t + k;
+            if (i < n) { float g = pred[i] - target[i]; grad[i] = g; s += g * g; }
+        }
     }
     s = wave_sum(s);
     __shared__ float sh[BS / 64];
@@ -1599,7 +1616,16 @@ __global__ void __launch_bounds__(BS) k_mse(size_t n, const float* pred, const f
     if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-__global__ void k_zero1(float* x) { *x = 0.f; }
+// loss = 0.5 * the sum of the nb block partials, in a fixed order (one workgroup)
+__global__ void __launch_bounds__(BS) k_mse_final(const float* part, size_t nb, float* loss) {
+    float r = 0.f;
+    for (size_t i = threadIdx.x; i < nb; i += BS) r += part[i];
+    r = wave_sum(r);
+    __shared__ float sh[BS / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) *loss = (((sh[0] + sh[1]) + sh[2]) + sh[3]) * 0.5f;
+}
 
 }  // namespace
 
@@ -1803,13 +1829,6 @@ extern "C" int vcl_adam_step_multi(int n, const vcl_optimize_params* p, const vc
     return vcl_adam_step_multi_dev(n, p, h, nullptr, 0, stream);
 }
 
-extern "C" int vcl_adam_bump_dev(double* bt_dev, double rhom, double rhov, vcl_stream_t stream) {
-    VCL_REQUIRE(bt_dev);
-    hipLaunchKernelGGL(k_adam_bump, dim3(1), dim3(64), 0, as_hip(stream), bt_dev, rhom, rhov);
-    VCL_LAUNCH_CHECK();
-    return VCL_OK;
-}
-
 extern "C" int vcl_adam_step(const vcl_optimize_params* p, const vcl_adam_hyper_params* h, vcl_stream_t stream) {
     return vcl_adam_step_multi(1, p, h, stream);
 }
@@ -1977,7 +1996,7 @@ extern "C" size_t vcl_spherical_gaussian_workspace_size(int W, int H, int depth,
     (void)channel;
     size_t P = (size_t)W * H * depth;
     size_t cols = (size_t)sgnum * SG_COLS;
-    return sizeof(float) * ((size_t)sg_bwd_blocks(P) * cols + colsum_scratch_floats((int)cols) + cols) + 256;
+    return sizeof(float) * ((size_t)sg_bwd_blocks(P) * cols + colsum_scratch_floats((int)cols)) + 256;
 }
 
 extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_params* p, vcl_stream_t stream) {
@@ -2033,12 +2052,9 @@ extern "C" int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_para
     VCL_LAUNCH_CHECK();
     if (!want) return VCL_OK;
     float* seg = part + (size_t)nb * cols;
-    float* red = seg + colsum_scratch_floats((int)cols);
-    int st = vcl_fill_async(red, cols * sizeof(float), s);
-    if (st) return st;
-    st = colsum(part, (int)nb, (int)cols, seg, red, 1.f, 0, s);
-    if (st) return st;
-    hipLaunchKernelGGL(k_sg_scatter, dim3(cdiv(K, 64)), dim3(64), 0, s, red, K, C, g->axis, g->sharpness, g->amplitude);
+    const int nseg = colsum_stage(part, (int)nb, (int)cols, seg, s);
+    if (nseg < 0) return VCL_ERR_HIP;
+    hipLaunchKernelGGL(k_sg_lobe_final, dim3(cdiv(cols, 256)), dim3(256), 0, s, seg, nseg, K, C, g->axis, g->sharpness, g->amplitude);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2071,19 +2087,22 @@ extern "C" int vcl_antialias_backward(const vcl_antialias_params* p, const vcl_a
     return VCL_OK;
 }
 
+// workspace: one partial per block
 extern "C" size_t vcl_loss_workspace_size(int size) {
     size_t nb = cdiv((size_t)(size > 0 ? size : 1), (size_t)BS * 4);
-    return sizeof(float) * (nb + colsum_scratch_floats(1)) + 256;
+    return sizeof(float) * nb + 256;
 }
 
 extern "C" int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stream) {
     VCL_REQUIRE(p && p->predict && p->target && p->grad && p->buffer && p->loss && p->size >= 0);
     hipStream_t s = as_hip(stream);
-    hipLaunchKernelGGL(k_zero1, dim3(1), dim3(1), 0, s, p->loss);
+    const size_t nb = p->size > 0 ? cdiv((size_t)p->size, (size_t)BS * 4) : 0;
+    if (nb) {
+        const int vec = ((p->size & 3) == 0 && (((uintptr_t)p->predict | (uintptr_t)p->target | (uintptr_t)p->grad) & 15) == 0) ? 1 : 0;
+        hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(BS), 0, s, (size_t)p->size, p->predict, p->target, p->grad, p->buffer, vec);
+        VCL_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(BS), 0, s, p->buffer, nb, p->loss);
     VCL_LAUNCH_CHECK();
-    if (p->size == 0) return VCL_OK;
-    size_t nb = cdiv((size_t)p->size, (size_t)BS * 4);
-    hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(BS), 0, s, (size_t)p->size, p->predict, p->target, p->grad, p->buffer);
-    VCL_LAUNCH_CHECK();
-    return colsum(p->buffer, (int)nb, 1, p->buffer + nb, p->loss, 0.5f, 0, s);
+    return VCL_OK;
 }

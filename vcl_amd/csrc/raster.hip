@@ -361,10 +361,15 @@ __device__ __forceinline__ void bin_or_defer(const RastArgs& a, SetupShared& sh,
     }
 }
 
-__global__ void __launch_bounds__(256) k_clear(u32* counts, int NT, u32* misc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < NT) counts[i] = 0;
-    if (i < 64) misc[i] = 0;
+// Per-launch workspace reset: tile counters and misc[] to zero, the per-pixel depth keys to
+// all-ones (no winner); grid-stride over the largest of the three.
+__global__ void __launch_bounds__(256) k_clear(u32* counts, int NT, u32* misc, u64* keys, size_t P) {
+    const size_t n = max(max((size_t)NT, P), (size_t)64);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (i < (size_t)NT) counts[i] = 0;
+        if (i < 64) misc[i] = 0;
+        if (i < P) keys[i] = ~0ull;
+    }
 }
 
 __device__ void setup_triangle(const RastArgs& a, SetupShared& sh, size_t gid) {
@@ -945,8 +950,12 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     a.tilesX = L.tilesX; a.ntiles = L.ntiles; a.tshift = tshift; a.chunk = (u32)g_vcl_tuning.bin_chunk;
     a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap; a.bin_cap = L.bin_cap; a.chunk_cap = L.chunk_cap;
     a.ws = ws_ptrs(p->workspace, L);
-    int nclear = L.NT > 64 ? L.NT : 64;
-    hipLaunchKernelGGL(k_clear, dim3(cdiv(nclear, 256)), dim3(256), 0, s, a.ws.counts, L.NT, a.ws.misc);
+    const size_t P = (size_t)a.W * a.H * a.depth;
+    {
+        const size_t nclear = std::max(std::max((size_t)L.NT, P), (size_t)64);
+        hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<size_t>(cdiv(nclear, 256), 4096)), dim3(256), 0, s, a.ws.counts, L.NT, a.ws.misc,
+                           a.ws.keys, P);
+    }
     VCL_LAUNCH_CHECK();
     if (L.nrec_direct > 0) {
         hipLaunchKernelGGL(k_setup, dim3(cdiv(L.nrec_direct, 256)), dim3(256), 0, s, a);
@@ -962,11 +971,6 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     }
     hipLaunchKernelGGL(k_bin_large<true>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
-    const size_t P = (size_t)a.W * a.H * a.depth;
-    {
-        int st = vcl_fill_u32_async((u32*)a.ws.keys, 2 * P, 0xffffffffu, s);
-        if (st) return st;
-    }
     const dim3 fg((unsigned)std::min(L.chunk_cap, (size_t)FINE_MAX_GRID));
     if (tshift == 3) hipLaunchKernelGGL(k_fine<8>, fg, dim3(256), 0, s, a);
     else if (tshift == 5) hipLaunchKernelGGL(k_fine<32>, fg, dim3(256), 0, s, a);
