@@ -52,13 +52,17 @@ struct __align__(16) RRec {
 static_assert(sizeof(RRec) == 112, "record size");
 
 constexpr int SMALL_TILES = 4;  // pieces whose box spans more tiles are binned by a whole wave
-constexpr int LARGE_BLOCKS = 512;
+constexpr int LARGE_BLOCKS = 4096;   // waves for the large-piece list: about one piece each, as each costs two dependent loads
 
 // Fine-raster work unit: up to `chunk` pieces of one tile's bin.  A tile whose bin holds more
 // pieces is split over several chunks (and workgroups); the last of them to finish merges
 // the partial depth keys.
 constexpr u32 CHUNK_SCAN_ALL = 0x80000000u;   // count flag: scan every piece of every layer
 constexpr int FINE_MAX_GRID = 65536;
+// The large pieces of a tile are counted and placed through LSUB sub-counters (piece slot mod
+// LSUB), laid out sub-counter-major so each sits in its own cache line: device atomics on one
+// line serialise, and a deep tile takes ~1000 large pieces.
+constexpr int LSUB = 16;
 constexpr int CULL_MIN_BOX = 32;   // tile pixels in a piece's box for its depth range to be formed
 
 struct Chunk {
@@ -68,7 +72,7 @@ struct Chunk {
 struct Layout {
     int tilesX, tilesY, ntiles, NT;
     size_t nrec_direct, ovf_cap, nrec_cap, bin_cap, chunk_cap;
-    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_large, off_chunks, off_keys, total;
+    size_t off_counts, off_offsets, off_misc, off_rec, off_bins, off_large, off_chunks, off_keys, off_lcounts, off_lcursor, total;
 };
 
 size_t g_bin_cap_override = 0;   // vcl_rasterize_set_bin_capacity (0: sized from the piece count)
@@ -97,12 +101,16 @@ Layout make_layout(int W, int H, int depth, int T, int tshift) {
     L.off_large = o; o = al(o + sizeof(u32) * L.nrec_cap);
     L.off_chunks = o; o = al(o + sizeof(Chunk) * L.chunk_cap);
     L.off_keys = o; o = al(o + sizeof(u64) * (size_t)W * H * depth);
+    L.off_lcounts = o; o = al(o + sizeof(u32) * LSUB * (size_t)L.NT);
+    L.off_lcursor = o; o = al(o + sizeof(u32) * LSUB * (size_t)L.NT);
     L.total = o;
     return L;
 }
 
 struct Ws {
     u32* counts; u32* offsets; u32* misc; RRec* rec; u32* bins; u32* large; Chunk* chunks; u64* keys;
+    u32* lcounts;   // large pieces per (sub-counter, tile): [(slot % LSUB) * NT + tile]
+    u32* lcursor;   // fill cursor of each (sub-counter, tile), starting at its range's first slot
 };
 
 Ws ws_ptrs(void* base, const Layout& L) {
@@ -116,6 +124,8 @@ Ws ws_ptrs(void* base, const Layout& L) {
     w.large = (u32*)(b + L.off_large);
     w.chunks = (Chunk*)(b + L.off_chunks);
     w.keys = (u64*)(b + L.off_keys);
+    w.lcounts = (u32*)(b + L.off_lcounts);
+    w.lcursor = (u32*)(b + L.off_lcursor);
     return w;
 }
 
@@ -333,8 +343,15 @@ __global__ void __launch_bounds__(256) k_bin_large(RastArgs a) {
             int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
             if (!tile_overlaps(r, tx, ty, a.tshift)) continue;
             u32 g = (u32)(view * a.ntiles + ty * a.tilesX + tx);
-            if (!FILL) atomicAdd(a.ws.counts + g, 1u);
-            else bin_fill_direct(a, g, slot);
+            const u32 sub = slot & (LSUB - 1);
+            const size_t NT = (size_t)a.ntiles * a.depth;
+            if (!FILL) {
+                atomicAdd(a.ws.lcounts + sub * NT + g, 1u);
+            } else {
+                // the tile's large pieces take the bottom of its bin (the small ones count down
+                // from the top): sub-counter ranges in order, each filled by its cursor
+                bin_write(a, g, atomicAdd(a.ws.lcursor + sub * NT + g, 1u), slot);
+            }
         }
     }
 }
@@ -361,15 +378,33 @@ __device__ __forceinline__ void bin_or_defer(const RastArgs& a, SetupShared& sh,
     }
 }
 
-// Per-launch workspace reset: tile counters and misc[] to zero, the per-pixel depth keys to
-// all-ones (no winner); grid-stride over the largest of the three.
-__global__ void __launch_bounds__(256) k_clear(u32* counts, int NT, u32* misc, u64* keys, size_t P) {
-    const size_t n = max(max((size_t)NT, P), (size_t)64);
+// Per-launch workspace reset: tile counters, large-piece sub-counters and misc[] to zero, the
+// per-pixel depth keys to all-ones (no winner); grid-stride over the largest.
+__global__ void __launch_bounds__(256) k_clear(Ws w, int NT, size_t P) {
+    const size_t nl = (size_t)LSUB * NT;
+    const size_t n = max(max(nl, P), (size_t)64);
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        if (i < (size_t)NT) counts[i] = 0;
-        if (i < 64) misc[i] = 0;
-        if (i < P) keys[i] = ~0ull;
+        if (i < (size_t)NT) w.counts[i] = 0;
+        if (i < nl) w.lcounts[i] = 0;
+        if (i < 64) w.misc[i] = 0;
+        if (i < P) w.keys[i] = ~0ull;
     }
+}
+
+// Adds each tile's large-piece sub-counts to its count, before the scan, and sets each
+// sub-counter's fill cursor to the start of its range (the exclusive prefix of the sub-counts).
+__global__ void __launch_bounds__(256) k_fold_large(Ws w, int NT) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= NT) return;
+    u32 t = 0;
+#pragma unroll
+    for (int q = 0; q < LSUB; q++) {
+        const size_t i = (size_t)q * NT + g;
+        const u32 c = w.lcounts[i];
+        w.lcursor[i] = t;
+        t += c;
+    }
+    if (t) w.counts[g] += t;
 }
 
 __device__ void setup_triangle(const RastArgs& a, SetupShared& sh, size_t gid) {
@@ -952,9 +987,8 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     a.ws = ws_ptrs(p->workspace, L);
     const size_t P = (size_t)a.W * a.H * a.depth;
     {
-        const size_t nclear = std::max(std::max((size_t)L.NT, P), (size_t)64);
-        hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<size_t>(cdiv(nclear, 256), 4096)), dim3(256), 0, s, a.ws.counts, L.NT, a.ws.misc,
-                           a.ws.keys, P);
+        const size_t nclear = std::max(std::max((size_t)LSUB * L.NT, P), (size_t)64);
+        hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<size_t>(cdiv(nclear, 256), 4096)), dim3(256), 0, s, a.ws, L.NT, P);
     }
     VCL_LAUNCH_CHECK();
     if (L.nrec_direct > 0) {
@@ -962,6 +996,8 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
         VCL_LAUNCH_CHECK();
     }
     hipLaunchKernelGGL(k_bin_large<false>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
+    VCL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fold_large, dim3(cdiv(L.NT, 256)), dim3(256), 0, s, a.ws, L.NT);
     VCL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a, L.NT);
     VCL_LAUNCH_CHECK();
