@@ -1616,15 +1616,25 @@ __global__ void __launch_bounds__(BS) k_mse(size_t n, const float* pred, const f
     if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
 }
 
-// loss = 0.5 * the sum of the nb block partials, in a fixed order (one workgroup)
-__global__ void __launch_bounds__(BS) k_mse_final(const float* part, size_t nb, float* loss) {
-    float r = 0.f;
-    for (size_t i = threadIdx.x; i < nb; i += BS) r += part[i];
+// loss = 0.5 * the sum of the nb block partials, in a fixed order (one 1024-thread workgroup;
+// thread t sums its contiguous share with four interleaved accumulators so the loads overlap)
+__global__ void __launch_bounds__(1024) k_mse_final(const float* part, size_t nb, float* loss) {
+    const size_t per = (nb + 1023) / 1024;
+    const size_t b = threadIdx.x * per, e = min(nb, b + per);
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f, r3 = 0.f;
+    size_t i = b;
+    for (; i + 4 <= e; i += 4) { r0 += part[i]; r1 += part[i + 1]; r2 += part[i + 2]; r3 += part[i + 3]; }
+    for (; i < e; i++) r0 += part[i];
+    float r = (r0 + r1) + (r2 + r3);
     r = wave_sum(r);
-    __shared__ float sh[BS / 64];
+    __shared__ float sh[16];
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
     __syncthreads();
-    if (threadIdx.x == 0) *loss = (((sh[0] + sh[1]) + sh[2]) + sh[3]) * 0.5f;
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+        for (int w = 0; w < 16; w++) t += sh[w];
+        *loss = t * 0.5f;
+    }
 }
 
 }  // namespace
@@ -2102,7 +2112,7 @@ extern "C" int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stre
         hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(BS), 0, s, (size_t)p->size, p->predict, p->target, p->grad, p->buffer, vec);
         VCL_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(BS), 0, s, p->buffer, nb, p->loss);
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(1024), 0, s, p->buffer, nb, p->loss);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
