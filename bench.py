@@ -48,10 +48,8 @@ def algorithmic_bytes(sc, views=1):
         "project": V * 12 + views * V * 16,
         "rasterize": P * 32 + views * V * 16 + T * 12,
         "interpolate": P * 56 + V * 8 + T * 12,
-        "normal_axis": P * 28 + V * 12 + T * 12,
-        "view_axis": P * 12,
+        "shade_prep": P * (16 + 4 + 12 + 12 + 12 + 4 + 4) + V * 12 + T * 12,
         "texture": P * (4 + 8 + 16 + 4 * C),
-        "sg_specular": P * 52,
         "spherical_gaussian": P * 100 + K * 32,
         "antialias": P * (8 + 8 * C) + views * V * 16 + T * 12,
         "mse_loss": P * 12 * C,

@@ -41,7 +41,14 @@ def test_graph_replay_matches_eager(gpu):
         le = float(eager.read("loss", (1,))[0])
         lg = float(graph.read("loss", (1,))[0])
         print(f"  loss eager {le:.6e} graph {lg:.6e}")
-        np.testing.assert_allclose(lg, le, rtol=1e-5)
+        # the first loss is the deterministic forward of the same initial state; later ones
+        # follow gradients summed by float atomics in a run-dependent order, which Adam
+        # compounds, so they agree to the gradient tolerance (a replay that froze or skipped the
+        # bias-correction powers would be off by percents within these steps)
+        if it == 0:
+            assert lg == le
+        else:
+            np.testing.assert_allclose(lg, le, rtol=1e-3)
         ie = eager.read("image", (3, sc.H, sc.W, sc.C))
         ig = graph.read("image", (3, sc.H, sc.W, sc.C))
         assert float(np.abs(ie - ig).max()) <= 1e-4 * max(1.0, float(np.abs(ie).max()))

@@ -320,9 +320,9 @@ static int forward(vcl_fit* F, vcl_stream_t s) {
     std::memset(&np, 0, sizeof(np));
     np.width = W; np.height = H; np.depth = nv; np.rast = F->f("rast"); np.rot = F->f("rot");
     np.normal = F->f("normals"); np.normalidx = F->t<uint32_t>("idx"); np.out = F->f("nrm");
-    { ProfScope ps_(F, "normal_axis", as_hip(s)); st = vcl_normal_axis_forward(&np, s); }
-    if (st) return st;
-    { ProfScope ps_(F, "view_axis", as_hip(s)); st = vcl_view_axis_forward_dev(W, H, nv, F->f("pvinv"), F->f("view"), s); }
+    // NormalAxis -> ViewAxis -> SGSpecular, fused per pixel
+    vcl_sg_specular_params sp = {W, H, nv, F->f("rast"), F->f("nrm"), F->f("view"), F->f("rough"), d.ior, F->f("sAxis"), F->f("sSharp"), F->f("sAmp")};
+    { ProfScope ps_(F, "shade_prep", as_hip(s)); st = vcl_shade_prep_dev(&np, F->f("pvinv"), &sp, s); }
     if (st) return st;
     vcl_texturemap_params tp;
     tp.width = W; tp.height = H; tp.depth = nv; tp.texwidth = d.texwidth; tp.texheight = d.texheight; tp.channel = C; tp.miplevel = d.miplevel;
@@ -331,9 +331,7 @@ static int forward(vcl_fit* F, vcl_stream_t s) {
     tp.out = F->f("diff");
     { ProfScope ps_(F, "texture", as_hip(s)); st = vcl_texturemap_forward(&tp, s); }
     if (st) return st;
-    vcl_sg_specular_params sp = {W, H, nv, F->f("rast"), F->f("nrm"), F->f("view"), F->f("rough"), d.ior, F->f("sAxis"), F->f("sSharp"), F->f("sAmp")};
-    { ProfScope ps_(F, "sg_specular", as_hip(s)); st = vcl_sg_specular_forward(&sp, s); }
-    if (st) return st;
+
     vcl_spherical_gaussian_params gp;
     gp.width = W; gp.height = H; gp.depth = nv; gp.channel = C; gp.rast = F->f("rast");
     gp.sgnum = d.sgnum; gp.axis = F->f("axis"); gp.sharpness = F->f("sharpness"); gp.amplitude = F->f("amplitude");

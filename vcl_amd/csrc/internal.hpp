@@ -10,6 +10,9 @@ extern vcl_tuning g_vcl_tuning;
 extern "C" {
 // ViewAxis forward with the per-view inverse matrices already on the device (depth x 9 floats).
 int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream);
+// NormalAxis, ViewAxis (inverse projection-view blocks on the device) and SGSpecular forward in one
+// launch; sp->normal and sp->view are the outputs of the first two.
+int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp, vcl_stream_t stream);
 // Fused texture update.  part bit 1: fold + Adam + clamp on level 0; bit 2: rebuild the mips and
 // clear the coarse grads.  bt_dev: device {rhomt, rhovt} used instead of h->rhomt/rhovt (null:
 // the host values).

@@ -800,13 +800,11 @@ __global__ void __launch_bounds__(BS) k_mip_fused(MipArgs a) {
 }
 
 // ---------------------------------------------------------------- NormalAxis / ViewAxis
-__global__ void __launch_bounds__(BS) k_normal_axis(size_t P, const float* rast, const float* rot, const float* normal, const u32* nidx, float* out) {
+// NormalAxis of one pixel: interpolated vertex normal, normalised, rotated (zero off-mesh).
+__device__ __forceinline__ void normal_axis_px(float4 r, const float* rot, const float* normal, const u32* nidx, float& o0, float& o1, float& o2) {
 #pragma clang fp contract(off)
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float4 r = ((const float4*)rast)[p];
     int id = (int)r.w - 1;
-    float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+    o0 = 0.f; o1 = 0.f; o2 = 0.f;
     if (id >= 0) {
         const float* n2 = normal + 3 * (size_t)nidx[3 * id + 2];
         const float* n0 = normal + 3 * (size_t)nidx[3 * id];
@@ -820,10 +818,27 @@ __global__ void __launch_bounds__(BS) k_normal_axis(size_t P, const float* rast,
         o1 = rot[1] * N0 + rot[5] * N1 + rot[9] * N2;
         o2 = rot[2] * N0 + rot[6] * N1 + rot[10] * N2;
     }
+}
+
+__global__ void __launch_bounds__(BS) k_normal_axis(size_t P, const float* rast, const float* rot, const float* normal, const u32* nidx, float* out) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float o0, o1, o2;
+    normal_axis_px(((const float4*)rast)[p], rot, normal, nidx, o0, o1, o2);
     out[3 * p] = o0; out[3 * p + 1] = o1; out[3 * p + 2] = o2;
 }
 
 struct Mat3x8 { float m[8][9]; };
+
+// ViewAxis of one pixel from its layer's 3x3 inverse projection-view block m.
+__device__ __forceinline__ void view_axis_px(int W, int H, int px, int py, const float* m, float& o0, float& o1, float& o2) {
+    float sx = (float)px / (float)W * 2.f - 1.f, sy = (float)py / (float)H * 2.f - 1.f;
+    float v0 = m[0] * sx + m[3] * sy + m[6];
+    float v1 = m[1] * sx + m[4] * sy + m[7];
+    float v2 = m[2] * sx + m[5] * sy + m[8];
+    float d = 1.f / sqrtf(v0 * v0 + v1 * v1 + v2 * v2);
+    o0 = v0 * d; o1 = v1 * d; o2 = v2 * d;
+}
 
 __global__ void __launch_bounds__(BS) k_view_axis(int W, int H, int depth, const float* pvinv_dev, Mat3x8 pv_small, float* out) {
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -831,12 +846,7 @@ __global__ void __launch_bounds__(BS) k_view_axis(int W, int H, int depth, const
     if (p >= P) return;
     int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
     const float* m = pvinv_dev ? pvinv_dev + 9 * s : pv_small.m[s];
-    float sx = (float)px / (float)W * 2.f - 1.f, sy = (float)py / (float)H * 2.f - 1.f;
-    float v0 = m[0] * sx + m[3] * sy + m[6];
-    float v1 = m[1] * sx + m[4] * sy + m[7];
-    float v2 = m[2] * sx + m[5] * sy + m[8];
-    float d = 1.f / sqrtf(v0 * v0 + v1 * v1 + v2 * v2);
-    out[3 * p] = v0 * d; out[3 * p + 1] = v1 * d; out[3 * p + 2] = v2 * d;
+    view_axis_px(W, H, px, py, m, out[3 * p], out[3 * p + 1], out[3 * p + 2]);
 }
 
 // ---------------------------------------------------------------- SGSpecular
@@ -878,6 +888,28 @@ __global__ void __launch_bounds__(BS) k_sg_spec_fwd(size_t P, const float* rast,
     if ((int)rast[4 * p + 3] - 1 >= 0)
         spec_terms(normal[3 * p], normal[3 * p + 1], normal[3 * p + 2], view[3 * p], view[3 * p + 1], view[3 * p + 2], rough[p], ior, pa, amp, sh,
                    nullptr, nullptr);
+    oAxis[3 * p] = pa[0]; oAxis[3 * p + 1] = pa[1]; oAxis[3 * p + 2] = pa[2];
+    oSharp[p] = sh; oAmp[p] = amp;
+}
+
+// The fit step's NormalAxis -> ViewAxis -> SGSpecular chain in one pass per pixel: the same
+// per-pixel code as the three ops, without the round trip of the normal and view axes through
+// memory between them.
+__global__ void __launch_bounds__(BS) k_shade_prep(int W, int H, int depth, const float* rast, const float* rot, const float* normal, const u32* nidx,
+                                                   const float* pvinv, const float* rough, float ior, float* onrm, float* oview, float* oAxis,
+                                                   float* oSharp, float* oAmp) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t P = (size_t)W * H * depth;
+    if (p >= P) return;
+    const int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
+    const float4 r = ((const float4*)rast)[p];
+    float n0, n1, n2, v0, v1, v2;
+    normal_axis_px(r, rot, normal, nidx, n0, n1, n2);
+    view_axis_px(W, H, px, py, pvinv + 9 * s, v0, v1, v2);
+    onrm[3 * p] = n0; onrm[3 * p + 1] = n1; onrm[3 * p + 2] = n2;
+    oview[3 * p] = v0; oview[3 * p + 1] = v1; oview[3 * p + 2] = v2;
+    float pa[3] = {0.f, 0.f, 0.f}, amp = 0.f, sh = 0.f;
+    if ((int)r.w - 1 >= 0) spec_terms(n0, n1, n2, v0, v1, v2, rough[p], ior, pa, amp, sh, nullptr, nullptr);
     oAxis[3 * p] = pa[0]; oAxis[3 * p + 1] = pa[1]; oAxis[3 * p + 2] = pa[2];
     oSharp[p] = sh; oAmp[p] = amp;
 }
@@ -1152,19 +1184,22 @@ __global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, cons
 // (l > 8.75 gives e^{-2l} < 2^-25), from l = |w + lam*mu| >= | |lam*mu| - |w| | with |w| the
 // pixel's 2.133*normal or spec-sharpness*spec-axis length (both lengths carry a relative
 // margin for their float rounding).
-struct SgLobe4 { float4 g; float4 a; float len; float pad[3]; };
+struct SgLobe4 { float4 g; float4 a; float len, cD, lamL2E, pad; };   // cD = (SG_DS + lam) log2(e), lamL2E = lam log2(e)
 
-__device__ __forceinline__ float sg_term_rsq(float wx, float wy, float wz, float base, float lam, bool both) {
+// One lobe term with the exponent's constant part c = (base + lam) log2(e) given: one rsq gives
+// 1/l and l, each exponent is one fma (within 1e-6 relative of the reference's (l - c) - lam
+// form), and e^{-l-c} is formed only when `both` (wave-uniform).
+__device__ __forceinline__ float sg_term_rsq(float wx, float wy, float wz, float c, bool both) {
     float d = wx * wx + wy * wy + wz * wz;
     float il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
     float l = d * il;
-    float e1 = nexp2(((l - base) - lam) * L2E);
+    float e1 = nexp2(fmaf(l, L2E, -c));
     float e2 = 0.f;
     if (both) {
-        // both is wave-uniform; the empty asm keeps this a branch the wave skips rather than
-        // an exp computed on every lobe and selected away
+        // the empty asm keeps this a branch the wave skips rather than an exp computed on
+        // every lobe and selected away
         asm volatile("" :::);
-        e2 = nexp2(((-l - base) - lam) * L2E);
+        e2 = nexp2(fmaf(-l, L2E, -c));
     }
     return (e1 - e2) * il;
 }
@@ -1178,13 +1213,14 @@ template <bool CHECKED>
 __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float dnx, float dny, float dnz, float snx, float sny, float snz,
                                               float ss, float sa, float4& de, float4& se) {
     const float dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz), slen = sqrtf(snx * snx + sny * sny + snz * snz);
+    const float ssL2E = ss * L2E;
     for (int i = 0; i < K; i++) {
         const float4 g = lobes[i].g, a = lobes[i].a;
         const float gl = lobes[i].len;
         const bool dboth = !__all(fabsf(gl - dlen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + dlen));
         const bool sboth = !__all(fabsf(gl - slen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + slen));
-        float dg = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, SG_DS, g.w, dboth) * SG_DA;
-        float sg = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ss, g.w, sboth) * sa;
+        float dg = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, lobes[i].cD, dboth) * SG_DA;
+        float sg = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ssL2E + lobes[i].lamL2E, sboth) * sa;
         if (!CHECKED) {
             float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
             de.x += dv * a.x; de.y += dv * a.y; de.z += dv * a.z; de.w += dv * a.w;
@@ -1210,6 +1246,8 @@ __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* ra
         lobes[i].g = make_float4(gx, gy, gz, lam);
         lobes[i].a = a;
         lobes[i].len = sqrtf(gx * gx + gy * gy + gz * gz);
+        lobes[i].cD = (SG_DS + lam) * L2E;
+        lobes[i].lamL2E = lam * L2E;
         if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(a.w))) nonfinite = 1;
     }
     __syncthreads();
@@ -1968,6 +2006,19 @@ extern "C" int vcl_view_axis_forward_dev(int W, int H, int depth, const float* p
     if (!P) return VCL_OK;
     Mat3x8 m;
     hipLaunchKernelGGL(k_view_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), W, H, depth, pvinv_dev, m, out);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp, vcl_stream_t stream) {
+    VCL_REQUIRE(np && sp && pvinv_dev && np->rast && np->rot && np->normal && np->normalidx && np->out && sp->view && sp->roughness &&
+                sp->outAxis && sp->outSharpness && sp->outAmplitude);
+    VCL_REQUIRE(np->width == sp->width && np->height == sp->height && np->depth == sp->depth && sp->normal == np->out && sp->rast == np->rast);
+    size_t P = (size_t)np->width * np->height * np->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_shade_prep, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), np->width, np->height, np->depth, np->rast, np->rot, np->normal,
+                       np->normalidx, pvinv_dev, sp->roughness, sp->ior, np->out, const_cast<float*>(sp->view), sp->outAxis, sp->outSharpness,
+                       sp->outAmplitude);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
