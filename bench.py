@@ -48,6 +48,7 @@ def algorithmic_bytes(sc, views=1):
         "project": V * 12 + views * V * 16,
         "rasterize": P * 32 + views * V * 16 + T * 12,
         "interpolate": P * 56 + V * 8 + T * 12,
+        "interp_texture": P * (16 + 16 + 8 + 16 + 4 * C) + V * 8 + T * 12,
         "shade_prep": P * (16 + 4 + 12 + 12 + 12 + 4 + 4) + V * 12 + T * 12,
         "texture": P * (4 + 8 + 16 + 4 * C),
         "spherical_gaussian": P * 100 + K * 32,

@@ -51,7 +51,13 @@ def test_graph_replay_matches_eager(gpu):
             np.testing.assert_allclose(lg, le, rtol=1e-3)
         ie = eager.read("image", (3, sc.H, sc.W, sc.C))
         ig = graph.read("image", (3, sc.H, sc.W, sc.C))
-        assert float(np.abs(ie - ig).max()) <= 1e-4 * max(1.0, float(np.abs(ie).max()))
+        if it == 0:
+            assert np.array_equal(ie, ig)
+        else:
+            # vertices a last digit apart can move an edge across a pixel centre, which changes
+            # that pixel completely; all but a few pixels agree closely
+            frac = float(np.mean(np.isclose(ie, ig, rtol=1e-4, atol=1e-5)))
+            assert frac > 0.999, f"step {it}: image agrees at only {frac:.5f} of entries"
     pe = eager.read("pos", (sc.V, 3))
     pg = graph.read("pos", (sc.V, 3))
     assert np.mean(np.isclose(pe, pg, rtol=1e-4, atol=1e-6)) > 0.999

@@ -4,9 +4,8 @@
   device, and run by the plain engine (vcl_fit_step) ends in the same state.  The forward is
   deterministic, so the first loss is bit-identical; gradients are summed with float atomics
   whose order differs from run to run (as in the reference), and Adam turns those last-digit
-  differences into diverging trajectories, so after several iterations serving may differ from
-  plain by no more than two plain runs differ from each other (and at most 1e-3 relative when
-  those happen to agree more closely).
+  differences into slowly diverging trajectories: after 3 iterations the losses agree to the
+  gradient tolerance (1e-3), after 10 to 5e-3.
 - Admission respects the HBM budget: with room for one request at a time every request still
   finishes, each admitted only after the previous one released its arena.
 """
@@ -23,6 +22,7 @@ MIX = [{"level": 5, "W": 160, "tex": 1024, "K": 64, "seed": 12},
        {"level": 3, "W": 64, "tex": 256, "K": 128, "seed": 13},
        {"level": 6, "W": 128, "tex": 256, "K": 16, "seed": 14}]
 ITERS = 10
+SHORT_ITERS = 3
 
 
 def _request(spec):
@@ -87,18 +87,19 @@ def test_alone_vs_mix_vs_plain(gpu):
         for a, b, what in ((alone, plain, "alone vs plain"), (mix, alone, "mix vs alone")):
             frac = float(np.mean(np.isclose(a[k], b[k], rtol=1e-5, atol=1e-7)))
             assert frac > 0.999, f"{what}: {k} after one step agrees at only {frac:.5f} of entries"
-    # ten iterations: Adam turns gradient-order differences into steps of up to lr on texels
-    # whose gradient is near zero, so states drift apart run to run; serving (alone or in the
-    # mix) may differ from plain by no more than a second plain run does
+    # a few iterations: the gradient-order differences have compounded little (1e-5 level), so
+    # the losses agree to the gradient tolerance
+    plain, alone, mix = _run_all(sc, tgt, others, SHORT_ITERS)
+    for a, b in ((alone, plain), (mix, alone), (mix, plain)):
+        np.testing.assert_allclose(a["loss"], b["loss"], rtol=1e-3)
+    # ten interleaved iterations: Adam turns the run-dependent summation order of the float
+    # atomics into diverging trajectories (two plain runs differ by 2e-4 to 1.3e-3 relative in
+    # the loss here), so the served results are held to 5e-3 and must keep converging; a
+    # serving fault (wrong buffers, crossed requests, wrong iteration count) is off by far more
     plain, alone, mix = _run_all(sc, tgt, others, ITERS)
-    plain2 = _plain(sc, tgt, ITERS)
-    spread = abs(plain2["loss"] - plain["loss"]) / plain["loss"]
-    tol = max(1e-3, 2.0 * spread)
-    print(f"plain run-to-run spread after {ITERS} iterations: {spread:.2e} (tolerance {tol:.2e})")
-    np.testing.assert_allclose(alone["loss"], plain["loss"], rtol=tol)
-    np.testing.assert_allclose(mix["loss"], alone["loss"], rtol=tol)
-    np.testing.assert_allclose(mix["loss"], plain["loss"], rtol=tol)
-    assert mix["loss"] < plain["first"]
+    for a, b in ((alone, plain), (mix, alone), (mix, plain)):
+        np.testing.assert_allclose(a["loss"], b["loss"], rtol=5e-3)
+    assert mix["loss"] < 0.5 * plain["first"]
 
 
 def test_budget_admission(gpu):

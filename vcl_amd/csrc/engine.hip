@@ -313,9 +313,6 @@ static int forward(vcl_fit* F, vcl_stream_t s) {
     rp.workspace = F->buf["ws_rast"]; rp.workspace_size = F->ws_rast;
     { ProfScope ps_(F, "rasterize", as_hip(s)); st = vcl_rasterize_forward(&rp, s); }
     if (st) return st;
-    vcl_interpolate_params ip = {W, H, nv, 2, 1, F->f("rast"), F->f("db"), F->f("uv"), F->t<uint32_t>("idx"), F->f("uvp"), F->f("uvDA"), d.V};
-    { ProfScope ps_(F, "interpolate", as_hip(s)); st = vcl_interpolate_forward(&ip, s); }
-    if (st) return st;
     vcl_normal_axis_params np;
     std::memset(&np, 0, sizeof(np));
     np.width = W; np.height = H; np.depth = nv; np.rast = F->f("rast"); np.rot = F->f("rot");
@@ -329,9 +326,17 @@ static int forward(vcl_fit* F, vcl_stream_t s) {
     tp.rast = F->f("rast"); tp.uv = F->f("uvp"); tp.uvDA = F->f("uvDA");
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tp.texture[l] = l < d.miplevel ? F->f("texture") + F->lvl_off[l] : nullptr;
     tp.out = F->f("diff");
-    { ProfScope ps_(F, "texture", as_hip(s)); st = vcl_texturemap_forward(&tp, s); }
+    vcl_interpolate_params ip = {W, H, nv, 2, 1, F->f("rast"), F->f("db"), F->f("uv"), F->t<uint32_t>("idx"), F->f("uvp"), F->f("uvDA"), d.V};
+    if (C == 4) {
+        // uv interpolation and texture sample fused per pixel
+        ProfScope ps_(F, "interp_texture", as_hip(s));
+        st = vcl_interp_tex_dev(&ip, &tp, s);
+    } else {
+        { ProfScope ps_(F, "interpolate", as_hip(s)); st = vcl_interpolate_forward(&ip, s); }
+        if (st) return st;
+        { ProfScope ps_(F, "texture", as_hip(s)); st = vcl_texturemap_forward(&tp, s); }
+    }
     if (st) return st;
-
     vcl_spherical_gaussian_params gp;
     gp.width = W; gp.height = H; gp.depth = nv; gp.channel = C; gp.rast = F->f("rast");
     gp.sgnum = d.sgnum; gp.axis = F->f("axis"); gp.sharpness = F->f("sharpness"); gp.amplitude = F->f("amplitude");

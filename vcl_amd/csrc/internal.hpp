@@ -12,6 +12,9 @@ extern "C" {
 int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream);
 // NormalAxis, ViewAxis (inverse projection-view blocks on the device) and SGSpecular forward in one
 // launch; sp->normal and sp->view are the outputs of the first two.
+// Interpolate (uv: 2 attributes with derivatives) and the C == 4 texture forward in one launch;
+// tp->uv and tp->uvDA are ip's outputs.
+int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, vcl_stream_t stream);
 int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp, vcl_stream_t stream);
 // Fused texture update.  part bit 1: fold + Adam + clamp on level 0; bit 2: rebuild the mips and
 // clear the coarse grads.  bt_dev: device {rhomt, rhovt} used instead of h->rhomt/rhovt (null:

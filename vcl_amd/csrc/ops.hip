@@ -158,6 +158,15 @@ __global__ void __launch_bounds__(BS) k_project_bwd(int V, int dim, int views, c
 }
 
 // ---------------------------------------------------------------- interpolate
+// One attribute of one covered pixel (interpolate.cu:38: a0*u + a1*v in float, the (1.0 - u - v)
+// term w2 in double) and its screen derivatives from the barycentric derivatives b.
+__device__ __forceinline__ void interp_attr_px(float4 r, float4 b, double w2, float a0, float a1, float a2, float& o, float2& oda) {
+#pragma clang fp contract(off)
+    o = (float)((double)(a0 * r.x + a1 * r.y) + (double)a2 * w2);
+    float dadu = a0 - a2, dadv = a1 - a2;
+    oda = make_float2(dadu * b.x + dadv * b.z, dadu * b.y + dadv * b.w);
+}
+
 __global__ void __launch_bounds__(BS) k_interp_fwd(size_t P, int D, int enableDA, const float* rast, const float* db, const float* attr,
                                                    const u32* idx, float* out, float* outDA) {
 #pragma clang fp contract(off)
@@ -174,15 +183,13 @@ __global__ void __launch_bounds__(BS) k_interp_fwd(size_t P, int D, int enableDA
     }
     u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
     float4 b = enableDA ? ((const float4*)db)[p] : make_float4(0, 0, 0, 0);
-    // interpolate.cu:38: a0*u + a1*v in float, the (1.0 - u - v) term in double
     double w2 = 1.0 - (double)r.x - (double)r.y;
     for (int i = 0; i < D; i++) {
-        float a0 = attr[(size_t)i0 * D + i], a1 = attr[(size_t)i1 * D + i], a2 = attr[(size_t)i2 * D + i];
-        out[p * D + i] = (float)((double)(a0 * r.x + a1 * r.y) + (double)a2 * w2);
-        if (enableDA) {
-            float dadu = a0 - a2, dadv = a1 - a2;
-            ((float2*)outDA)[p * D + i] = make_float2(dadu * b.x + dadv * b.z, dadu * b.y + dadv * b.w);
-        }
+        float o;
+        float2 oda;
+        interp_attr_px(r, b, w2, attr[(size_t)i0 * D + i], attr[(size_t)i1 * D + i], attr[(size_t)i2 * D + i], o, oda);
+        out[p * D + i] = o;
+        if (enableDA) ((float2*)outDA)[p * D + i] = oda;
     }
 }
 
@@ -282,15 +289,14 @@ __device__ __forceinline__ float4 bilerp4(float4 a, float4 b, float4 c, float4 d
 }
 
 // C == 4 fast path: texels as float4.
-__global__ void __launch_bounds__(BS) k_tex_fwd4(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (rast[4 * p + 3] >= 1.f) {
-        float2 u = ((const float2*)uv)[p];
+// C == 4 texture sample of one covered pixel at uv u with screen derivatives da (used when
+// t.L > 1): trilinear between the two mip levels around the footprint's level.
+__device__ __forceinline__ float4 tex4_px(const TexArgs& t, float2 u, float4 da) {
+    float4 o;
+    {
         if (t.L > 1) {
             int l0, l1; float f, lev;
-            calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, nullptr, 0);
+            calc_level(da, t.tw, t.th, t.L, l0, l1, f, lev, nullptr, 0);
             int4 i0; float fx, fy;
             index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
             const float4* T0 = (const float4*)t.tex[l0];
@@ -308,6 +314,41 @@ __global__ void __launch_bounds__(BS) k_tex_fwd4(size_t P, TexArgs t, const floa
             o = bilerp4(T0[i0.x], T0[i0.y], T0[i0.z], T0[i0.w], fx, fy);
         }
     }
+    return o;
+}
+
+__global__ void __launch_bounds__(BS) k_tex_fwd4(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (rast[4 * p + 3] >= 1.f)
+        o = tex4_px(t, ((const float2*)uv)[p], t.L > 1 ? ((const float4*)uvDA)[p] : make_float4(0.f, 0.f, 0.f, 0.f));
+    ((float4*)out)[p] = o;
+}
+
+// The fit step's uv interpolation (2 attributes, with derivatives) and C == 4 texture forward in
+// one pass per pixel: the same per-pixel code as the two ops, writing uv and its derivatives for
+// the backward and the texture sample.
+__global__ void __launch_bounds__(BS) k_interp_tex4(size_t P, const float* rast, const float* db, const float* uvattr, const u32* idx, float* uvp,
+                                                    float* uvDA, TexArgs t, float* out) {
+    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const float4 r = ((const float4*)rast)[p];
+    const int id = (int)r.w - 1;
+    float2 u = make_float2(0.f, 0.f);
+    float4 da = make_float4(0.f, 0.f, 0.f, 0.f), o = da;
+    if (id >= 0) {
+        const u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
+        const float4 b = ((const float4*)db)[p];
+        const double w2 = 1.0 - (double)r.x - (double)r.y;
+        float2 dau, dav;
+        interp_attr_px(r, b, w2, uvattr[2 * (size_t)i0], uvattr[2 * (size_t)i1], uvattr[2 * (size_t)i2], u.x, dau);
+        interp_attr_px(r, b, w2, uvattr[2 * (size_t)i0 + 1], uvattr[2 * (size_t)i1 + 1], uvattr[2 * (size_t)i2 + 1], u.y, dav);
+        da = make_float4(dau.x, dau.y, dav.x, dav.y);
+    }
+    ((float2*)uvp)[p] = u;
+    ((float4*)uvDA)[p] = da;
+    if (r.w >= 1.f) o = tex4_px(t, u, da);
     ((float4*)out)[p] = o;
 }
 
@@ -1752,6 +1793,21 @@ extern "C" int vcl_texturemap_forward(const vcl_texturemap_params* p, vcl_stream
         hipLaunchKernelGGL(k_tex_fwd4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, p->rast, p->uv, p->uvDA, p->out);
     else
         hipLaunchKernelGGL(k_tex_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, p->rast, p->uv, p->uvDA, p->out);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, vcl_stream_t stream) {
+    VCL_REQUIRE(ip && tp && ip->rast && ip->rastDB && ip->attr && ip->idx && ip->out && ip->outDA && tp->out);
+    VCL_REQUIRE(ip->dimention == 2 && ip->enableDA && tp->channel == 4 && tp->uv == ip->out && tp->uvDA == ip->outDA && tp->rast == ip->rast);
+    VCL_REQUIRE(ip->width == tp->width && ip->height == tp->height && ip->depth == tp->depth);
+    TexArgs t;
+    int st = tex_args(tp, t);
+    if (st) return st;
+    size_t P = (size_t)ip->width * ip->height * ip->depth;
+    if (!P) return VCL_OK;
+    hipLaunchKernelGGL(k_interp_tex4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, ip->rast, ip->rastDB, ip->attr, ip->idx, ip->out, ip->outDA,
+                       t, tp->out);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
