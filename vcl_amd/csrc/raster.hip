@@ -698,7 +698,7 @@ __device__ __forceinline__ bool piece_covers_rect(const RRec& r, int px0, int px
 // bin is split: a tile's only chunk stores its keys, the chunks of a split tile merge theirs
 // with a device-scope 64-bit minimum.
 template <int TS>
-__global__ void __launch_bounds__(256, TS == 8 ? 5 : 4) k_fine(RastArgs a) {
+__global__ void __launch_bounds__(256, TS == 32 ? 4 : 5) k_fine(RastArgs a) {
     constexpr int NP = TS * TS;
     __shared__ u64 key[NP];
     __shared__ float spix[2][TS];
@@ -706,7 +706,8 @@ __global__ void __launch_bounds__(256, TS == 8 ? 5 : 4) k_fine(RastArgs a) {
     __shared__ unsigned short sspan[BATCH][TS];     // per piece: first | last << 8 column per row
     __shared__ float szp[9][BATCH];
     __shared__ u32 sid[BATCH];
-    __shared__ u32 sitem[BATCH * TS];               // covered rows: piece | row | first column | length
+    constexpr int ITEM_CAP = BATCH * TS < 2048 ? BATCH * TS : 2048;
+    __shared__ u32 sitem[ITEM_CAP];                  // covered rows: piece | row | first column | length
     __shared__ u32 sbucket[4];                       // per-wave covered-row totals
     __shared__ u32 sbound;                           // ordered depth bound of the tile (culling)
     const int tid = threadIdx.x;
@@ -797,18 +798,24 @@ __global__ void __launch_bounds__(256, TS == 8 ? 5 : 4) k_fine(RastArgs a) {
             }
             if ((tid & 63) == 63) sbucket[tid >> 6] = incl;
             __syncthreads();
-            u32 pos = incl - nrow;
-            for (int w = 0; w < (tid >> 6); w++) pos += sbucket[w];
+            u32 pos0 = incl - nrow;
+            for (int w = 0; w < (tid >> 6); w++) pos0 += sbucket[w];
             const u32 ni = sbucket[0] + sbucket[1] + sbucket[2] + sbucket[3];
+            // the items go through LDS ITEM_CAP at a time (one round unless the batch is dense)
+            for (u32 r0 = 0; r0 < ni; r0 += ITEM_CAP) {
+            u32 pos = pos0;
             for (int y = yl; y <= yh; y++) {
                 u32 sp = sspan[tid][y];
                 int lo = (int)(sp & 0xffu), hi = (int)(sp >> 8);
                 if (lo > hi) continue;
-                sitem[pos++] = (u32)tid | ((u32)y << 8) | ((u32)lo << 16) | ((u32)(hi - lo + 1) << 24);
+                if (pos >= r0 && pos < r0 + ITEM_CAP)
+                    sitem[pos - r0] = (u32)tid | ((u32)y << 8) | ((u32)lo << 16) | ((u32)(hi - lo + 1) << 24);
+                pos++;
             }
             __syncthreads();
             // phase 3: thread = item
-            for (u32 i = tid; i < ni; i += 256) {
+            const u32 nr = min((u32)ITEM_CAP, ni - r0);
+            for (u32 i = tid; i < nr; i += 256) {
                 const u32 it = sitem[i];
                 const int j = (int)(it & 0xffu), y = (int)((it >> 8) & 0xffu), lo = (int)((it >> 16) & 0xffu), len = (int)(it >> 24);
                 const float pixy = spix[1][y];
@@ -830,6 +837,7 @@ __global__ void __launch_bounds__(256, TS == 8 ? 5 : 4) k_fine(RastArgs a) {
                 }
             }
             __syncthreads();
+            }
         }
         // the tile's winning keys: stored when this chunk is the tile's only one, else merged with
         // a device-scope 64-bit minimum (the key buffer starts at all-ones)
