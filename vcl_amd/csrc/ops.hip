@@ -1581,7 +1581,7 @@ __device__ __forceinline__ void aa_vertex_grad(LdsTable<AA_VSLOTS, 3>& vt, float
     }
 }
 
-__global__ void __launch_bounds__(BS) k_aa_bwd(AAArgs a, int tilesX, const float* in, const float* gout, float* gin, float* gproj) {
+__global__ void __launch_bounds__(BS, 7) k_aa_bwd(AAArgs a, int tilesX, const float* in, const float* gout, float* gin, float* gproj) {
 #pragma clang fp contract(off)
     __shared__ float4 to_cur[AA_PAIRS];
     __shared__ LdsTable<AA_VSLOTS, 3> vt;
