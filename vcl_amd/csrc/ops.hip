@@ -1216,7 +1216,10 @@ __global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, cons
     }
 }
 
-// C == 4 forward.  Per lobe: two 16-byte LDS reads (axis * sharpness, sharpness; amplitude).
+// C == 4 forward.  A workgroup takes 1024 consecutive pixels and compacts the covered ones, so
+// its waves shade covered pixels only (a thread-per-pixel wave at a silhouette runs the whole
+// lobe loop for a few covered lanes).  Per lobe: two 16-byte LDS reads (axis * sharpness,
+// sharpness; amplitude) and one 16-byte read of the lobe constants.
 // AddNaNcheck per channel reduces to one finiteness test per term: the term is bounded
 // (e^{l-c} <= 1 since l <= c, and 1/max(l, 1e-6) <= 1e6), so a finite term times finite
 // amplitudes always passes the per-channel check and a non-finite one always fails it; a block
@@ -1273,11 +1276,15 @@ __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float
     }
 }
 
+constexpr int SG_FWD_PIX = 4 * BS;   // pixels per k_sg_fwd4 workgroup
+
 __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
                                                 const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
                                                 const float* sAmp, float* out, float* denv, float* senv) {
     __shared__ SgLobe4 lobes[SG_MAXK];
     __shared__ int nonfinite;
+    __shared__ u32 plist[SG_FWD_PIX];
+    __shared__ u32 wcount[BS / 64];
     if (threadIdx.x == 0) nonfinite = 0;
     __syncthreads();
     for (int i = threadIdx.x; i < K; i += blockDim.x) {
@@ -1293,22 +1300,32 @@ __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* ra
     }
     __syncthreads();
     const bool checked = nonfinite != 0;
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float4 de = make_float4(0.f, 0.f, 0.f, 0.f), se = de;
-    bool cov = (int)rast[4 * p + 3] - 1 >= 0;
-    if (cov) {
+    // uncovered pixels get their zeros here; the covered ones of the workgroup's SG_FWD_PIX are
+    // compacted (in pixel order) so that every wave shades covered pixels only
+    const size_t base = (size_t)blockIdx.x * SG_FWD_PIX;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    u32 n = 0;
+    for (int k = 0; k < SG_FWD_PIX / BS; k++) {
+        const u32 local = (u32)(k * BS + threadIdx.x);
+        const size_t p = base + local;
+        const bool in = p < P;
+        const bool cov = in && (int)rast[4 * p + 3] - 1 >= 0;
+        if (in && !cov) { ((float4*)out)[p] = z4; ((float4*)denv)[p] = z4; ((float4*)senv)[p] = z4; }
+        block_append(cov, local, plist, wcount, n);
+    }
+    for (u32 i = threadIdx.x; i < n; i += BS) {
+        const size_t p = base + plist[i];
+        float4 de = z4, se = z4;
         float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
         float ss = sSharp[p], sa = sAmp[p];
         float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
         if (checked) sg_fwd4_lobes<true>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, de, se);
         else sg_fwd4_lobes<false>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, de, se);
+        const float4 df = ((const float4*)diffuse)[p];
+        ((float4*)out)[p] = make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w);
+        ((float4*)denv)[p] = de;
+        ((float4*)senv)[p] = se;
     }
-    float4 df = cov ? ((const float4*)diffuse)[p] : make_float4(0.f, 0.f, 0.f, 0.f);
-    ((float4*)out)[p] = cov ? make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w)
-                            : make_float4(0.f, 0.f, 0.f, 0.f);
-    ((float4*)denv)[p] = de;
-    ((float4*)senv)[p] = se;
 }
 
 // Lobe-parallel backward for the lobe gradients (axis, sharpness, amplitude).  Thread t owns
@@ -2125,7 +2142,7 @@ extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_param
     const bool al16 = ((uintptr_t)p->diffuse % 16 == 0) && ((uintptr_t)p->out % 16 == 0) && ((uintptr_t)p->outDiffenv % 16 == 0) &&
                       ((uintptr_t)p->outSpecenv % 16 == 0);
     if (p->sgnum <= SG_MAXK && p->channel == 4 && al16)
-        hipLaunchKernelGGL(k_sg_fwd4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness, p->amplitude,
+        hipLaunchKernelGGL(k_sg_fwd4, dim3(cdiv(P, SG_FWD_PIX)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness, p->amplitude,
                            p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv, p->outSpecenv);
     else if (p->sgnum <= SG_MAXK)
         hipLaunchKernelGGL(k_sg_fwd_fast, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
