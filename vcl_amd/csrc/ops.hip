@@ -1253,25 +1253,58 @@ constexpr float SG_E2_DROP = 8.75f;
 // The lobe loop of one pixel; CHECKED selects the per-channel AddNaNcheck form (a block whose
 // lobes hold a non-finite amplitude).  The choice is uniform per block, so the two forms are
 // separate loops rather than a branch inside one.
+// Which lobes of chunk [c0, c0 + 64) may need e^{-l-c} for some lane of the wave: lane j tests
+// lobe c0 + j against the wave's range [lmin, lmax] of the pixel-side length |w| (the per-lane
+// test |len - |w|| <= SG_E2_DROP + 0.01 + 1e-5 (len + |w|) implies len within that range widened
+// by the largest threshold), and the ballot gives the chunk's mask (a superset of the lobes the
+// per-lane test selects; an exponential formed where it was not needed is below half an ulp and
+// changes nothing).  All 64 lanes must call it.
+__device__ __forceinline__ u64 sg_e2_mask(const SgLobe4* lobes, int K, int c0, float lmin, float lmax) {
+    const int j = c0 + (int)(threadIdx.x & 63);
+    const float gl = j < K ? lobes[j].len : __builtin_nanf("");
+    const float T = SG_E2_DROP + 0.01f + 1e-5f * (gl + lmax);
+    return __ballot(gl >= lmin - T && gl <= lmax + T);
+}
+
+__device__ __forceinline__ float wave_fmin(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_fmax(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// The lobe loop of one pixel; CHECKED selects the per-channel AddNaNcheck form (a block whose
+// lobes hold a non-finite amplitude).  The choice is uniform per block, so the two forms are
+// separate loops rather than a branch inside one.  dmask / smask: per 64-lobe chunk, the lobes
+// whose diffuse / specular e^{-l-c} is formed (sg_e2_mask), tested as scalars.
 template <bool CHECKED>
 __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float dnx, float dny, float dnz, float snx, float sny, float snz,
-                                              float ss, float sa, float4& de, float4& se) {
-    const float dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz), slen = sqrtf(snx * snx + sny * sny + snz * snz);
+                                              float ss, float sa, const u64* dmask, const u64* smask, float4& de, float4& se) {
     const float ssL2E = ss * L2E;
-    for (int i = 0; i < K; i++) {
-        const float4 g = lobes[i].g, a = lobes[i].a;
-        const float gl = lobes[i].len;
-        const bool dboth = !__all(fabsf(gl - dlen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + dlen));
-        const bool sboth = !__all(fabsf(gl - slen) > SG_E2_DROP + 0.01f + 1e-5f * (gl + slen));
-        float dg = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, lobes[i].cD, dboth) * SG_DA;
-        float sg = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ssL2E + lobes[i].lamL2E, sboth) * sa;
-        if (!CHECKED) {
-            float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
-            de.x += dv * a.x; de.y += dv * a.y; de.z += dv * a.z; de.w += dv * a.w;
-            se.x += sv * a.x; se.y += sv * a.y; se.z += sv * a.z; se.w += sv * a.w;
-        } else {
-            acc_checked(de.x, dg * a.x); acc_checked(de.y, dg * a.y); acc_checked(de.z, dg * a.z); acc_checked(de.w, dg * a.w);
-            acc_checked(se.x, sg * a.x); acc_checked(se.y, sg * a.y); acc_checked(se.z, sg * a.z); acc_checked(se.w, sg * a.w);
+#pragma unroll
+    for (int c = 0; c < SG_MAXK / 64; c++) {
+        if (c * 64 >= K) break;
+        const u64 dm = dmask[c], sm = smask[c];
+        const int iend = min(K, c * 64 + 64);
+        for (int i = c * 64; i < iend; i++) {
+            const float4 g = lobes[i].g, a = lobes[i].a;
+            const bool dboth = (dm >> (i - c * 64)) & 1ull;
+            const bool sboth = (sm >> (i - c * 64)) & 1ull;
+            float dg = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, lobes[i].cD, dboth) * SG_DA;
+            float sg = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ssL2E + lobes[i].lamL2E, sboth) * sa;
+            if (!CHECKED) {
+                float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
+                de.x += dv * a.x; de.y += dv * a.y; de.z += dv * a.z; de.w += dv * a.w;
+                se.x += sv * a.x; se.y += sv * a.y; se.z += sv * a.z; se.w += sv * a.w;
+            } else {
+                acc_checked(de.x, dg * a.x); acc_checked(de.y, dg * a.y); acc_checked(de.z, dg * a.z); acc_checked(de.w, dg * a.w);
+                acc_checked(se.x, sg * a.x); acc_checked(se.y, sg * a.y); acc_checked(se.z, sg * a.z); acc_checked(se.w, sg * a.w);
+            }
         }
     }
 }
@@ -1313,18 +1346,35 @@ __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* ra
         if (in && !cov) { ((float4*)out)[p] = z4; ((float4*)denv)[p] = z4; ((float4*)senv)[p] = z4; }
         block_append(cov, local, plist, wcount, n);
     }
-    for (u32 i = threadIdx.x; i < n; i += BS) {
-        const size_t p = base + plist[i];
-        float4 de = z4, se = z4;
-        float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
-        float ss = sSharp[p], sa = sAmp[p];
-        float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
-        if (checked) sg_fwd4_lobes<true>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, de, se);
-        else sg_fwd4_lobes<false>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, de, se);
-        const float4 df = ((const float4*)diffuse)[p];
-        ((float4*)out)[p] = make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w);
-        ((float4*)denv)[p] = de;
-        ((float4*)senv)[p] = se;
+    for (u32 r = 0; r < n; r += BS) {
+        const u32 i = r + threadIdx.x;
+        const bool act = i < n;
+        const size_t p = act ? base + plist[i] : 0;
+        float dnx = 0.f, dny = 0.f, dnz = 0.f, ss = 0.f, sa = 0.f, snx = 0.f, sny = 0.f, snz = 0.f;
+        float dlen = __builtin_nanf(""), slen = __builtin_nanf("");   // lanes without a pixel: ignored by fmin/fmax
+        if (act) {
+            dnx = normal[3 * p] * SG_DS; dny = normal[3 * p + 1] * SG_DS; dnz = normal[3 * p + 2] * SG_DS;
+            ss = sSharp[p]; sa = sAmp[p];
+            snx = sAxis[3 * p] * ss; sny = sAxis[3 * p + 1] * ss; snz = sAxis[3 * p + 2] * ss;
+            dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz);
+            slen = sqrtf(snx * snx + sny * sny + snz * snz);
+        }
+        const float dmin = wave_fmin(dlen), dmax = wave_fmax(dlen), smin = wave_fmin(slen), smax = wave_fmax(slen);
+        u64 dmask[SG_MAXK / 64], smask[SG_MAXK / 64];
+#pragma unroll
+        for (int c = 0; c < SG_MAXK / 64; c++) {
+            dmask[c] = sg_e2_mask(lobes, K, c * 64, dmin, dmax);
+            smask[c] = sg_e2_mask(lobes, K, c * 64, smin, smax);
+        }
+        if (act) {
+            float4 de = z4, se = z4;
+            if (checked) sg_fwd4_lobes<true>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
+            else sg_fwd4_lobes<false>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
+            const float4 df = ((const float4*)diffuse)[p];
+            ((float4*)out)[p] = make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w);
+            ((float4*)denv)[p] = de;
+            ((float4*)senv)[p] = se;
+        }
     }
 }
 
