@@ -223,6 +223,86 @@ def test_texture(gpu, cfg1, mode):
     assert_close(N(gda), gda_ref, 1e-3, "texture grad uvDA", atol_frac=1e-5)
 
 
+def _tex_bwd_case(case, sc, d):
+    """Inputs that steer the texel-grad summation of a 16 x 16 region through each of its forms:
+    a compact patch of texels (fixed-slot window), scattered texels over many levels (hash table),
+    all-zero output grads, output grads spanning 8 decades in one region, and non-finite ones."""
+    rng = np.random.default_rng(11)
+    rast, uv, da = d["rast"], d["uvp"], d["uvDA"].reshape(d["rast"].shape[:3] + (4,))
+    views, H, W = rast.shape[:3]
+    gout = rng.normal(size=(views, H, W, sc.C)).astype(F32)
+    if case == "scattered":
+        H = W = 64
+        rast = np.zeros((2, H, W, 4), F32)
+        rast[..., 3] = 1.0
+        uv = rng.uniform(-0.1, 1.1, (2, H, W, 2)).astype(F32)
+        da = (rng.normal(size=(2, H, W, 4)) * 10.0 ** rng.uniform(-4, -1, (2, H, W, 1))).astype(F32)
+        gout = rng.normal(size=(2, H, W, sc.C)).astype(F32)
+    elif case == "magnified":
+        H = W = 48
+        rast = np.zeros((1, H, W, 4), F32)
+        rast[..., 3] = 1.0
+        yy, xx = np.mgrid[0:H, 0:W].astype(F32)
+        uv = np.stack([0.3 + xx * 1e-3, 0.6 + yy * 7e-4], -1)[None].astype(F32)
+        da = np.broadcast_to(np.array([1e-3, 0.0, 0.0, 7e-4], F32), (1, H, W, 4)).copy()
+        gout = rng.normal(size=(1, H, W, sc.C)).astype(F32)
+    elif case == "zero":
+        gout[:, :, : W // 2] = 0.0
+    elif case == "range":
+        # a smooth uv ramp at 2.5 level-0 texels per pixel (every pixel at level 1.32, away from
+        # the level boundaries where float log2 of host and GPU may pick different levels), with
+        # output grads spanning 8 decades within every region
+        H = W = 64
+        rast = np.zeros((1, H, W, 4), F32)
+        rast[..., 3] = 1.0
+        s = 2.5 / sc.tw
+        yy, xx = np.mgrid[0:H, 0:W].astype(F32)
+        uv = np.stack([0.2 + xx * s, 0.3 + yy * 0.8 * s], -1)[None].astype(F32)
+        da = np.broadcast_to(np.array([s, 0.0, 0.0, 0.8 * s], F32), (1, H, W, 4)).copy()
+        gout = (rng.normal(size=(1, H, W, sc.C)) * 10.0 ** rng.uniform(-8, 0, (1, H, W, 1))).astype(F32)
+    elif case == "nonfinite":
+        cov = np.argwhere(rast[..., 3] >= 1)
+        v, y, x = cov[len(cov) // 3]
+        gout[v, y, x, 1] = np.nan
+        v, y, x = cov[2 * len(cov) // 3]
+        gout[v, y, x, 2] = np.inf
+    return rast, uv, da, gout
+
+
+@pytest.mark.parametrize("case", ["scattered", "magnified", "zero", "range", "nonfinite"])
+def test_texture_backward_summation_forms(gpu, cfg1, case):
+    import torch
+    sc, d = cfg1
+    rast, uv, da, gout = _tex_bwd_case(case, sc, d)
+    views, H, W = rast.shape[:3]
+    levels = gpu.level_views(T(sc.tex), sc.tw, sc.th, sc.C, sc.L)
+    gt_ref, guv_ref, gda_ref = o_tex_bwd(rast, uv, da, sc.tex, sc.tw, sc.th, sc.C, sc.L, gout, 0)
+    gflat = torch.zeros(sc.tex.size, device="cuda")
+    glv = gpu.level_views(gflat, sc.tw, sc.th, sc.C, sc.L)
+    guv = torch.zeros(views, H, W, 2, device="cuda")
+    gda = torch.zeros(views, H, W, 4, device="cuda")
+    gpu.texture_backward(T(rast), T(uv), T(da), levels, T(gout), glv, guv, gda, da_mode=0)
+    for name, got, ref in (("texels", N(gflat), gt_ref), ("uv", N(guv), guv_ref), ("uvDA", N(gda), gda_ref)):
+        # non-finite entries in the same places, with the same signs of infinity
+        assert np.array_equal(np.isnan(got), np.isnan(ref)), f"{case}: NaN pattern of grad {name}"
+        assert np.array_equal(np.isposinf(got), np.isposinf(ref)) and np.array_equal(np.isneginf(got), np.isneginf(ref))
+        fin = np.isfinite(ref)
+        if case == "range" and name == "texels":
+            # contributions spanning 8 decades within each region: every texel within 1e-3 of
+            # its own size plus 1e-6 of the sum of its terms' sizes (the oracle's float sums are
+            # good to about n 2^-24 of that) plus 1e-12 of the largest output grad (the GPU sums
+            # a region in fixed point whose unit is 2^-51 of the region's largest output grad)
+            mag = o_tex_bwd(rast, uv, da, sc.tex, sc.tw, sc.th, sc.C, sc.L, np.abs(gout), 0)[0]
+            bad = np.abs(got - ref) > 1e-3 * np.abs(ref) + 1e-6 * mag + 1e-12 * np.abs(gout).max()
+            assert not bad.any(), f"{case}: {int(bad.sum())} texel grads off, worst {np.max(np.abs(got - ref) / (mag + 1e-30))}"
+            tiny = (mag > 0) & (mag < 1e-6 * mag.max())
+            assert tiny.sum() > 5, "the case should hold texels whose terms are all tiny"
+        else:
+            assert_close(got[fin], ref[fin], 1e-3, f"{case}: grad {name}", atol_frac=1e-5)
+    if case == "nonfinite":
+        assert np.isnan(gt_ref).any()
+
+
 def test_texture_nomip(gpu, cfg1):
     sc, d = cfg1
     lvl0 = sc.tex[: sc.tw * sc.th * sc.C]

@@ -451,21 +451,36 @@ __global__ void __launch_bounds__(BS) k_tex_bwd(size_t P, TexArgs t, TexGradArgs
 // lanes, texel-major and channel-minor, so one wave's atomics cover 4 whole lines instead of 64
 // scattered dwords.  A key that finds no bucket within the probe limit falls back to direct
 // global atomics.
+//
+// The table sums in 64-bit fixed point: an LDS float atomic add costs about ten times an integer
+// one on this GPU (it was most of the kernel's time), so each contribution w * go[c] is taken as
+// the integer w * go[c] * 2^S (truncated) and added with a 64-bit integer atomic.  S = 51 - e,
+// where 2^e <= M < 2^(e+1) and M is the largest |go| component over the workgroup's covered
+// pixels: every contribution is below 2^52 in size, a pixel adds less than 2^52 + 4 to one texel
+// channel (its bilinear weights at one level sum to at most 1), and a sum over the at most 1024
+// pixels of a workgroup stays below 2^63.  The conversion errs by less than 2^-51 M per
+// contribution (float atomics round every partial sum to 2^-24 of its own size), and the integer
+// sums do not depend on the order of the adds.  A workgroup whose go holds a non-finite value
+// sends its splats to global float atomics instead, so NaN and Inf reach the texel grads as float
+// summation would carry them.
 constexpr u32 TB_LEVEL_SHIFT = 22;  // key = level << 22 | texel >> 2 (texel < 2^24)
+constexpr u32 TB_NONFINITE = 0x7f800000u;  // bit pattern of +inf: |go| bit maxima at or above it
+                                           // are a non-finite go
 
 template <int NB>
 struct TexGradTable {
-    float val[NB * 16];  // [bucket][texel & 3][channel]
+    unsigned long long val[NB * 16];  // [bucket][texel & 3][channel], fixed point (2^-S units)
     u32 key[NB];
     u32 used[NB];        // occupied buckets, compacted before the flush
     u32 wave_count[BS / 64];
+    u32 wave_max[BS / 64];  // per wave: the largest |go| component, as its bit pattern
 };
 
 template <int NB>
 __device__ __forceinline__ void tex_table_clear(TexGradTable<NB>& t) {
     for (int i = threadIdx.x; i < NB; i += blockDim.x) t.key[i] = LDS_EMPTY;
-    float4* v = (float4*)t.val;
-    for (int i = threadIdx.x; i < NB * 4; i += blockDim.x) v[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ulonglong2* v = (ulonglong2*)t.val;
+    for (int i = threadIdx.x; i < NB * 8; i += blockDim.x) v[i] = make_ulonglong2(0ull, 0ull);
 }
 
 // Keys only go from empty to their final value, so a plain read that finds k is final; the CAS
@@ -485,15 +500,28 @@ __device__ __forceinline__ int tex_table_bucket(TexGradTable<NB>& t, u32 k) {
     return -1;
 }
 
+// y (|y| < 2^52) truncated to an integer, as a 64-bit two's-complement addend.  |y| is split
+// into floor(|y| / 2^32) and the remainder |y| - 2^32 floor(|y| / 2^32), both exact in float;
+// the remainder's truncation errs by less than 1.
+__device__ __forceinline__ unsigned long long tex_fixed(float y) {
+    const float a = fabsf(y);
+    const float hi = floorf(a * 0x1p-32f);
+    const float r = fminf(fmaf(hi, -0x1p32f, a), 0x1.fffffep31f);
+    const unsigned long long q = ((unsigned long long)(u32)(int)hi << 32) | (u32)r;
+    return y < 0.f ? 0ull - q : q;
+}
+
+// One texel's splat of weight w: into the table (gs = go * 2^S), or, for direct, by global float
+// atomics of w * go.
 template <int NB>
-__device__ __forceinline__ void tex_splat(TexGradTable<NB>& tab, float* G, int level, int texel, float w, float4 go) {
-    int b = tex_table_bucket(tab, ((u32)level << TB_LEVEL_SHIFT) | ((u32)texel >> 2));
+__device__ __forceinline__ void tex_splat(TexGradTable<NB>& tab, float* G, int level, int texel, float w, float4 go, float4 gs, bool direct) {
+    const int b = direct ? -1 : tex_table_bucket(tab, ((u32)level << TB_LEVEL_SHIFT) | ((u32)texel >> 2));
     if (b >= 0) {
-        float* q = tab.val + b * 16 + (texel & 3) * 4;
-        atomicAdd(q, w * go.x);
-        atomicAdd(q + 1, w * go.y);
-        atomicAdd(q + 2, w * go.z);
-        atomicAdd(q + 3, w * go.w);
+        unsigned long long* q = tab.val + b * 16 + (texel & 3) * 4;
+        atomicAdd(q, tex_fixed(w * gs.x));
+        atomicAdd(q + 1, tex_fixed(w * gs.y));
+        atomicAdd(q + 2, tex_fixed(w * gs.z));
+        atomicAdd(q + 3, tex_fixed(w * gs.w));
     } else {
         float* q = G + (size_t)texel * 4;
         atomicAdd(q, w * go.x); atomicAdd(q + 1, w * go.y); atomicAdd(q + 2, w * go.z); atomicAdd(q + 3, w * go.w);
@@ -501,11 +529,11 @@ __device__ __forceinline__ void tex_splat(TexGradTable<NB>& tab, float* G, int l
 }
 
 // Flush of the table (after a barrier): the occupied buckets are compacted into used[], then
-// lane j of a bucket's 16 adds dword j of its 64-byte run (texel j / 4, channel j % 4).
-// Untouched texels of a run hold exact zeros and are skipped, which also keeps runs that pass the
-// end of a small level (fewer than 4 texels) inside it.
+// lane j of a bucket's 16 adds dword j of its 64-byte run (texel j / 4, channel j % 4), converted
+// back from fixed point.  Untouched texels of a run hold exact zeros and are skipped, which also
+// keeps runs that pass the end of a small level (fewer than 4 texels) inside it.
 template <int NB>
-__device__ __forceinline__ void tex_table_flush(TexGradTable<NB>& tab, const TexGradArgs& g) {
+__device__ __forceinline__ void tex_table_flush(TexGradTable<NB>& tab, const TexGradArgs& g, int S) {
     u32 n = 0;
     for (int c = 0; c < NB; c += BS) {
         int b = c + threadIdx.x;
@@ -513,103 +541,134 @@ __device__ __forceinline__ void tex_table_flush(TexGradTable<NB>& tab, const Tex
     }
     for (u32 i = threadIdx.x; i < n * 16; i += BS) {
         u32 b = tab.used[i >> 4];
-        float v = tab.val[b * 16 + (i & 15)];
-        if (v == 0.f) continue;
+        const long long q = (long long)tab.val[b * 16 + (i & 15)];
+        if (q == 0) continue;
         u32 k = tab.key[b];
         size_t dword = ((size_t)(k & ((1u << TB_LEVEL_SHIFT) - 1)) << 4) | (i & 15);
-        atomicAdd(g.tex[k >> TB_LEVEL_SHIFT] + dword, v);
+        atomicAdd(g.tex[k >> TB_LEVEL_SHIFT] + dword, ldexpf((float)q, -S));
     }
 }
 
-// One pixel of the C == 4 texture backward: texel splats into the LDS table, uv / uvDA grads
-// stored.
+// One covered pixel p of the C == 4 texture backward, with its output grad go (gs: go in the
+// table's fixed-point units): texel splats (want_tex), uv / uvDA grads stored.
 template <int NB>
-__device__ __forceinline__ void tex_bwd4_pixel(TexGradTable<NB>& tab, bool want_tex, int W, int H, int view, int px, int py, const TexArgs& t,
-                                               const TexGradArgs& g, const float* rast, const float* uv, const float* uvDA, const float* gout,
-                                               float* guv, float* guvDA, int mode) {
-    if (px < W && py < H) {
-        size_t p = ((size_t)view * H + py) * W + px;
-        if (rast[4 * p + 3] >= 1.f) {
-            float2 u = ((const float2*)uv)[p];
-            float4 go = ((const float4*)gout)[p];
-            int l0 = 0, l1 = 0; float f = 0.f, lev; float4 dl = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, &dl, mode);
-            int4 i0; float fx, fy;
-            index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
-            const float4* T0 = (const float4*)t.tex[l0];
-            float4 a00 = T0[i0.x], a01 = T0[i0.y], a10 = T0[i0.z], a11 = T0[i0.w];
-            float w0 = 1.f - f;
-            if (want_tex) {
-                float* G0 = g.tex[l0];
-                tex_splat(tab, G0, l0, i0.x, w0 * (1.f - fx) * (1.f - fy), go);
-                tex_splat(tab, G0, l0, i0.y, w0 * fx * (1.f - fy), go);
-                tex_splat(tab, G0, l0, i0.z, w0 * (1.f - fx) * fy, go);
-                tex_splat(tab, G0, l0, i0.w, w0 * fx * fy, go);
-            }
-            float s0 = (float)(t.tw >> l0), s1 = (float)(t.th >> l0);
-            float gu = (lerpf_(a01.x - a00.x, a11.x - a10.x, fy) * s0) * go.x + (lerpf_(a01.y - a00.y, a11.y - a10.y, fy) * s0) * go.y +
-                       (lerpf_(a01.z - a00.z, a11.z - a10.z, fy) * s0) * go.z + (lerpf_(a01.w - a00.w, a11.w - a10.w, fy) * s0) * go.w;
-            float gv = (lerpf_(a10.x - a00.x, a11.x - a01.x, fx) * s1) * go.x + (lerpf_(a10.y - a00.y, a11.y - a01.y, fx) * s1) * go.y +
-                       (lerpf_(a10.z - a00.z, a11.z - a01.z, fx) * s1) * go.z + (lerpf_(a10.w - a00.w, a11.w - a01.w, fx) * s1) * go.w;
-            float gl = 0.f;
-            if (t.L > 1 && f > 0) {
-                int4 i1; float hx, hy;
-                index_fetch(t.tw, t.th, l1, u, i1, hx, hy);
-                const float4* T1 = (const float4*)t.tex[l1];
-                float4 b00 = T1[i1.x], b01 = T1[i1.y], b10 = T1[i1.z], b11 = T1[i1.w];
-                if (want_tex) {
-                    float* G1 = g.tex[l1];
-                    tex_splat(tab, G1, l1, i1.x, f * (1.f - hx) * (1.f - hy), go);
-                    tex_splat(tab, G1, l1, i1.y, f * hx * (1.f - hy), go);
-                    tex_splat(tab, G1, l1, i1.z, f * (1.f - hx) * hy, go);
-                    tex_splat(tab, G1, l1, i1.w, f * hx * hy, go);
-                }
-                float r0 = (float)(t.tw >> l1), r1 = (float)(t.th >> l1);
-                // per channel: lerp(level-0 derivative, level-1 derivative, f), as texturemap.cu:273-274
-                float du[4], dv[4];
-                float ax[4] = {a00.x, a00.y, a00.z, a00.w}, bx[4] = {a01.x, a01.y, a01.z, a01.w};
-                float cx[4] = {a10.x, a10.y, a10.z, a10.w}, dx[4] = {a11.x, a11.y, a11.z, a11.w};
-                float ex[4] = {b00.x, b00.y, b00.z, b00.w}, fx1[4] = {b01.x, b01.y, b01.z, b01.w};
-                float gx1[4] = {b10.x, b10.y, b10.z, b10.w}, hx1[4] = {b11.x, b11.y, b11.z, b11.w};
-                float gg[4] = {go.x, go.y, go.z, go.w};
-                gu = 0.f; gv = 0.f;
+__device__ __forceinline__ void tex_bwd4_pixel(TexGradTable<NB>& tab, bool want_tex, bool direct, size_t p, float4 go, float4 gs,
+                                               const TexArgs& t, const TexGradArgs& g, const float* uv, const float* uvDA, float* guv,
+                                               float* guvDA, int mode) {
+    float2 u = ((const float2*)uv)[p];
+    int l0 = 0, l1 = 0; float f = 0.f, lev; float4 dl = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, &dl, mode);
+    int4 i0; float fx, fy;
+    index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
+    const float4* T0 = (const float4*)t.tex[l0];
+    float4 a00 = T0[i0.x], a01 = T0[i0.y], a10 = T0[i0.z], a11 = T0[i0.w];
+    float w0 = 1.f - f;
+    if (want_tex) {
+        float* G0 = g.tex[l0];
+        tex_splat(tab, G0, l0, i0.x, w0 * (1.f - fx) * (1.f - fy), go, gs, direct);
+        tex_splat(tab, G0, l0, i0.y, w0 * fx * (1.f - fy), go, gs, direct);
+        tex_splat(tab, G0, l0, i0.z, w0 * (1.f - fx) * fy, go, gs, direct);
+        tex_splat(tab, G0, l0, i0.w, w0 * fx * fy, go, gs, direct);
+    }
+    float s0 = (float)(t.tw >> l0), s1 = (float)(t.th >> l0);
+    float gu = (lerpf_(a01.x - a00.x, a11.x - a10.x, fy) * s0) * go.x + (lerpf_(a01.y - a00.y, a11.y - a10.y, fy) * s0) * go.y +
+               (lerpf_(a01.z - a00.z, a11.z - a10.z, fy) * s0) * go.z + (lerpf_(a01.w - a00.w, a11.w - a10.w, fy) * s0) * go.w;
+    float gv = (lerpf_(a10.x - a00.x, a11.x - a01.x, fx) * s1) * go.x + (lerpf_(a10.y - a00.y, a11.y - a01.y, fx) * s1) * go.y +
+               (lerpf_(a10.z - a00.z, a11.z - a01.z, fx) * s1) * go.z + (lerpf_(a10.w - a00.w, a11.w - a01.w, fx) * s1) * go.w;
+    float gl = 0.f;
+    if (t.L > 1 && f > 0) {
+        int4 i1; float hx, hy;
+        index_fetch(t.tw, t.th, l1, u, i1, hx, hy);
+        const float4* T1 = (const float4*)t.tex[l1];
+        float4 b00 = T1[i1.x], b01 = T1[i1.y], b10 = T1[i1.z], b11 = T1[i1.w];
+        if (want_tex) {
+            float* G1 = g.tex[l1];
+            tex_splat(tab, G1, l1, i1.x, f * (1.f - hx) * (1.f - hy), go, gs, direct);
+            tex_splat(tab, G1, l1, i1.y, f * hx * (1.f - hy), go, gs, direct);
+            tex_splat(tab, G1, l1, i1.z, f * (1.f - hx) * hy, go, gs, direct);
+            tex_splat(tab, G1, l1, i1.w, f * hx * hy, go, gs, direct);
+        }
+        float r0 = (float)(t.tw >> l1), r1 = (float)(t.th >> l1);
+        // per channel: lerp(level-0 derivative, level-1 derivative, f), as texturemap.cu:273-274
+        float du[4], dv[4];
+        float ax[4] = {a00.x, a00.y, a00.z, a00.w}, bx[4] = {a01.x, a01.y, a01.z, a01.w};
+        float cx[4] = {a10.x, a10.y, a10.z, a10.w}, dx[4] = {a11.x, a11.y, a11.z, a11.w};
+        float ex[4] = {b00.x, b00.y, b00.z, b00.w}, fx1[4] = {b01.x, b01.y, b01.z, b01.w};
+        float gx1[4] = {b10.x, b10.y, b10.z, b10.w}, hx1[4] = {b11.x, b11.y, b11.z, b11.w};
+        float gg[4] = {go.x, go.y, go.z, go.w};
+        gu = 0.f; gv = 0.f;
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    du[c] = lerpf_(lerpf_(bx[c] - ax[c], dx[c] - cx[c], fy) * s0, lerpf_(fx1[c] - ex[c], hx1[c] - gx1[c], hy) * r0, f);
-                    dv[c] = lerpf_(lerpf_(cx[c] - ax[c], dx[c] - bx[c], fx) * s1, lerpf_(gx1[c] - ex[c], hx1[c] - fx1[c], hx) * r1, f);
-                    float lv = lerpf_(lerpf_(ax[c], bx[c], fx), lerpf_(cx[c], dx[c], fx), fy);
-                    gl += (lerpf_(lerpf_(ex[c], fx1[c], hx), lerpf_(gx1[c], hx1[c], hx), hy) - lv) * gg[c];
-                    gu += du[c] * gg[c];
-                    gv += dv[c] * gg[c];
-                }
-            }
-            if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
-            if (guvDA && t.L > 1) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
+        for (int c = 0; c < 4; c++) {
+            du[c] = lerpf_(lerpf_(bx[c] - ax[c], dx[c] - cx[c], fy) * s0, lerpf_(fx1[c] - ex[c], hx1[c] - gx1[c], hy) * r0, f);
+            dv[c] = lerpf_(lerpf_(cx[c] - ax[c], dx[c] - bx[c], fx) * s1, lerpf_(gx1[c] - ex[c], hx1[c] - fx1[c], hx) * r1, f);
+            float lv = lerpf_(lerpf_(ax[c], bx[c], fx), lerpf_(cx[c], dx[c], fx), fy);
+            gl += (lerpf_(lerpf_(ex[c], fx1[c], hx), lerpf_(gx1[c], hx1[c], hx), hy) - lv) * gg[c];
+            gu += du[c] * gg[c];
+            gv += dv[c] * gg[c];
         }
     }
+    if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
+    if (guvDA && t.L > 1) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
 }
+
+__device__ __forceinline__ u32 wave_umax(u32 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (u32)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+// Buckets of the texel table of a 16 x ROWS region: one per pixel up to 256 (35 KB of LDS), 512
+// for larger regions.
+constexpr int tex_table_buckets(int rows) { return rows <= 16 ? 256 : 512; }
 
 // Texel-splat region of 16 x ROWS pixels per workgroup (ROWS / 16 pixels per thread): the
 // texel grads of the region are summed in one LDS hash table keyed by (level, texel) and flushed
-// with one set of atomics per touched texel.
+// with one set of atomics per touched texel.  A first pass over the covered pixels' go finds the
+// region's fixed-point scale.
 template <int ROWS>
 __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
                                                        const float* uvDA, const float* gout, float* guv, float* guvDA, int mode) {
-    __shared__ TexGradTable<16 * ROWS> tab;
-    bool want_tex = g.tex[0] != nullptr;
-    if (want_tex) tex_table_clear(tab);
-    __syncthreads();
-    int view = blockIdx.y;
-    int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
-    int px = tx * 16 + (threadIdx.x & 15);
+    constexpr int NPX = ROWS / 16;
+    __shared__ TexGradTable<tex_table_buckets(ROWS)> tab;
+    const bool want_tex = g.tex[0] != nullptr;
+    const int view = blockIdx.y;
+    const int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
+    const int px = tx * 16 + (threadIdx.x & 15);
+    auto pixel = [&](int k) { return ((size_t)view * H + ty * ROWS + k * 16 + (threadIdx.x >> 4)) * W + px; };
+    auto covered = [&](int k) { return px < W && ty * ROWS + k * 16 + (int)(threadIdx.x >> 4) < H && rast[4 * pixel(k) + 3] >= 1.f; };
+    int S = 0;
+    bool splat = false, direct = false;
+    if (want_tex) {
+        tex_table_clear(tab);
+        u32 mbits = 0;
 #pragma unroll
-    for (int k = 0; k < ROWS / 16; k++) {
-        int py = ty * ROWS + k * 16 + (threadIdx.x >> 4);
-        tex_bwd4_pixel(tab, want_tex, W, H, view, px, py, t, g, rast, uv, uvDA, gout, guv, guvDA, mode);
+        for (int k = 0; k < NPX; k++) {
+            if (!covered(k)) continue;
+            const float4 go = ((const float4*)gout)[pixel(k)];
+            mbits = max(mbits, max(max(__float_as_uint(fabsf(go.x)), __float_as_uint(fabsf(go.y))),
+                                   max(__float_as_uint(fabsf(go.z)), __float_as_uint(fabsf(go.w)))));
+        }
+        mbits = wave_umax(mbits);
+        if ((threadIdx.x & 63) == 0) tab.wave_max[threadIdx.x >> 6] = mbits;
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < BS / 64; w++) mbits = max(mbits, tab.wave_max[w]);
+        // all-zero go adds nothing; a non-finite one takes the float path
+        splat = mbits != 0;
+        direct = mbits >= TB_NONFINITE;
+        S = 51 - ((int)(mbits >> 23) - 127);
     }
-    if (!want_tex) return;
+#pragma unroll 1
+    for (int k = 0; k < NPX; k++) {
+        if (!covered(k)) continue;
+        const size_t p = pixel(k);
+        const float4 go = ((const float4*)gout)[p];
+        const float4 gs = make_float4(ldexpf(go.x, S), ldexpf(go.y, S), ldexpf(go.z, S), ldexpf(go.w, S));
+        tex_bwd4_pixel(tab, splat, direct, p, go, gs, t, g, uv, uvDA, guv, guvDA, mode);
+    }
+    if (!splat || direct) return;
     __syncthreads();
-    tex_table_flush(tab, g);
+    tex_table_flush(tab, g, S);
 }
 
 // ---------------------------------------------------------------- mip utilities
