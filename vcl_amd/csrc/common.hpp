@@ -42,22 +42,52 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// Open-addressing hash table in LDS: key -> NV float sums (ds_add_f32).  Used to pre-aggregate
-// gradient scatter (vertex / texel grads) within a workgroup before the global atomics.
+__device__ __forceinline__ u32 wave_umax(u32 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (u32)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+// Fixed-point sums for LDS tables.  LDS float atomic adds run at about a tenth of the rate of
+// integer ones on this GPU, so gradient sums that many lanes feed are held as 64-bit integers in
+// units of 2^-S, with S chosen per workgroup from the largest |value| it adds: with
+// 2^e <= max|value| < 2^(e+1), S = 51 - e puts every value times 2^S below 2^52, so up to 2^11
+// adds to one sum cannot overflow, and each value is truncated by less than 2^-51 max|value|.
+// The integer sums do not depend on the order of the adds.  A workgroup whose largest |value| is
+// not finite (bit pattern FIXED_NONFINITE or above) keeps float atomics, so NaN and Inf propagate
+// as float summation would carry them.
+constexpr u32 FIXED_NONFINITE = 0x7f800000u;
+
+__device__ __forceinline__ int fixed_scale(u32 max_abs_bits) { return 51 - ((int)(max_abs_bits >> 23) - 127); }
+
+// y (|y| < 2^52) truncated to an integer, as a 64-bit two's-complement addend.  |y| is split
+// into floor(|y| / 2^32) and the remainder |y| - 2^32 floor(|y| / 2^32), both exact in float;
+// the remainder's truncation errs by less than 1.
+__device__ __forceinline__ unsigned long long fixed64(float y) {
+    const float a = fabsf(y);
+    const float hi = floorf(a * 0x1p-32f);
+    const float r = fminf(fmaf(hi, -0x1p32f, a), 0x1.fffffep31f);
+    const unsigned long long q = ((unsigned long long)(u32)(int)hi << 32) | (u32)r;
+    return y < 0.f ? 0ull - q : q;
+}
+
+// Open-addressing hash table in LDS: key -> NV sums (float, or 64-bit fixed point).  Used to
+// pre-aggregate gradient scatter (vertex / texel grads) within a workgroup before the global
+// atomics.
 constexpr u32 LDS_EMPTY = 0xffffffffu;
 
-template <int SLOTS, int NV>
+template <int SLOTS, int NV, typename VT = float>
 struct LdsTable {
     u32 key[SLOTS];
-    float val[NV][SLOTS];
+    VT val[NV][SLOTS];
 };
 
-template <int SLOTS, int NV>
-__device__ __forceinline__ void lds_table_clear(LdsTable<SLOTS, NV>& t) {
+template <int SLOTS, int NV, typename VT>
+__device__ __forceinline__ void lds_table_clear(LdsTable<SLOTS, NV, VT>& t) {
     for (int i = threadIdx.x; i < SLOTS; i += blockDim.x) {
         t.key[i] = LDS_EMPTY;
 #pragma unroll
-        for (int k = 0; k < NV; k++) t.val[k][i] = 0.f;
+        for (int k = 0; k < NV; k++) t.val[k][i] = VT(0);
     }
 }
 
@@ -72,8 +102,8 @@ __device__ __forceinline__ u32 fib_hash(u32 k) { return (k * 2654435761u) >> (32
 // Slot of key k (inserted if absent), or -1 when MAXPROBE probes find no room.  Keys only go
 // from empty to their final value, so a plain read that finds k is final; the CAS is issued only
 // for an empty slot (a stale "empty" is settled by the CAS).
-template <int SLOTS, int NV, int MAXPROBE = 24>
-__device__ __forceinline__ int lds_table_slot(LdsTable<SLOTS, NV>& t, u32 k) {
+template <int SLOTS, int NV, typename VT, int MAXPROBE = 24>
+__device__ __forceinline__ int lds_table_slot(LdsTable<SLOTS, NV, VT>& t, u32 k) {
     u32 h = fib_hash<ilog2_c(SLOTS)>(k);
     for (int probe = 0; probe < MAXPROBE; probe++) {
         u32 cur = __hip_atomic_load(&t.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -122,6 +152,23 @@ __device__ __forceinline__ void lds_vertex_grad_flush(const LdsTable<SLOTS, 3>& 
         if (comp == 2) continue;
         float v = t.val[comp == 3 ? 2 : comp][sl];
         if (v != 0.f) atomicAdd(g + 4 * (size_t)t.key[sl] + comp, v);
+    }
+}
+
+// The same for a fixed-point table in units of 2^-S.
+template <int SLOTS>
+__device__ __forceinline__ void lds_vertex_grad_flush(const LdsTable<SLOTS, 3, unsigned long long>& t, u32* used, u32* wave_count, float* g,
+                                                      int S) {
+    u32 n = 0;
+    for (int c = 0; c < SLOTS; c += blockDim.x) {
+        int sl = c + threadIdx.x;
+        block_append(sl < SLOTS && t.key[sl] != LDS_EMPTY, (u32)sl, used, wave_count, n);
+    }
+    for (u32 i = threadIdx.x; i < 4 * n; i += blockDim.x) {
+        u32 sl = used[i >> 2], comp = i & 3;
+        if (comp == 2) continue;
+        const long long q = (long long)t.val[comp == 3 ? 2 : comp][sl];
+        if (q != 0) atomicAdd(g + 4 * (size_t)t.key[sl] + comp, ldexpf((float)q, -S));
     }
 }
 

@@ -452,20 +452,13 @@ __global__ void __launch_bounds__(BS) k_tex_bwd(size_t P, TexArgs t, TexGradArgs
 // scattered dwords.  A key that finds no bucket within the probe limit falls back to direct
 // global atomics.
 //
-// The table sums in 64-bit fixed point: an LDS float atomic add costs about ten times an integer
-// one on this GPU (it was most of the kernel's time), so each contribution w * go[c] is taken as
-// the integer w * go[c] * 2^S (truncated) and added with a 64-bit integer atomic.  S = 51 - e,
-// where 2^e <= M < 2^(e+1) and M is the largest |go| component over the workgroup's covered
-// pixels: every contribution is below 2^52 in size, a pixel adds less than 2^52 + 4 to one texel
-// channel (its bilinear weights at one level sum to at most 1), and a sum over the at most 1024
-// pixels of a workgroup stays below 2^63.  The conversion errs by less than 2^-51 M per
-// contribution (float atomics round every partial sum to 2^-24 of its own size), and the integer
-// sums do not depend on the order of the adds.  A workgroup whose go holds a non-finite value
-// sends its splats to global float atomics instead, so NaN and Inf reach the texel grads as float
-// summation would carry them.
+// The table sums in 64-bit fixed point (fixed64), scaled by M, the largest |go| component over
+// the workgroup's covered pixels: every contribution w * go[c] is at most M in size, a pixel adds
+// less than 2^52 + 4 to one texel channel (its bilinear weights at one level sum to at most 1),
+// and a sum over the at most 1024 pixels of a workgroup stays below 2^63.  Each contribution is
+// truncated by less than 2^-51 M (float atomics round every partial sum to 2^-24 of its own
+// size).  A workgroup whose go holds a non-finite value sends its splats to global float atomics.
 constexpr u32 TB_LEVEL_SHIFT = 22;  // key = level << 22 | texel >> 2 (texel < 2^24)
-constexpr u32 TB_NONFINITE = 0x7f800000u;  // bit pattern of +inf: |go| bit maxima at or above it
-                                           // are a non-finite go
 
 template <int NB>
 struct TexGradTable {
@@ -500,17 +493,6 @@ __device__ __forceinline__ int tex_table_bucket(TexGradTable<NB>& t, u32 k) {
     return -1;
 }
 
-// y (|y| < 2^52) truncated to an integer, as a 64-bit two's-complement addend.  |y| is split
-// into floor(|y| / 2^32) and the remainder |y| - 2^32 floor(|y| / 2^32), both exact in float;
-// the remainder's truncation errs by less than 1.
-__device__ __forceinline__ unsigned long long tex_fixed(float y) {
-    const float a = fabsf(y);
-    const float hi = floorf(a * 0x1p-32f);
-    const float r = fminf(fmaf(hi, -0x1p32f, a), 0x1.fffffep31f);
-    const unsigned long long q = ((unsigned long long)(u32)(int)hi << 32) | (u32)r;
-    return y < 0.f ? 0ull - q : q;
-}
-
 // One texel's splat of weight w: into the table (gs = go * 2^S), or, for direct, by global float
 // atomics of w * go.
 template <int NB>
@@ -518,10 +500,10 @@ __device__ __forceinline__ void tex_splat(TexGradTable<NB>& tab, float* G, int l
     const int b = direct ? -1 : tex_table_bucket(tab, ((u32)level << TB_LEVEL_SHIFT) | ((u32)texel >> 2));
     if (b >= 0) {
         unsigned long long* q = tab.val + b * 16 + (texel & 3) * 4;
-        atomicAdd(q, tex_fixed(w * gs.x));
-        atomicAdd(q + 1, tex_fixed(w * gs.y));
-        atomicAdd(q + 2, tex_fixed(w * gs.z));
-        atomicAdd(q + 3, tex_fixed(w * gs.w));
+        atomicAdd(q, fixed64(w * gs.x));
+        atomicAdd(q + 1, fixed64(w * gs.y));
+        atomicAdd(q + 2, fixed64(w * gs.z));
+        atomicAdd(q + 3, fixed64(w * gs.w));
     } else {
         float* q = G + (size_t)texel * 4;
         atomicAdd(q, w * go.x); atomicAdd(q + 1, w * go.y); atomicAdd(q + 2, w * go.z); atomicAdd(q + 3, w * go.w);
@@ -611,12 +593,6 @@ __device__ __forceinline__ void tex_bwd4_pixel(TexGradTable<NB>& tab, bool want_
     if (guvDA && t.L > 1) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
 }
 
-__device__ __forceinline__ u32 wave_umax(u32 v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, (u32)__shfl_xor((int)v, o, 64));
-    return v;
-}
-
 // Buckets of the texel table of a 16 x ROWS region: one per pixel up to 256 (35 KB of LDS), 512
 // for larger regions.
 constexpr int tex_table_buckets(int rows) { return rows <= 16 ? 256 : 512; }
@@ -655,8 +631,8 @@ __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX,
         for (int w = 0; w < BS / 64; w++) mbits = max(mbits, tab.wave_max[w]);
         // all-zero go adds nothing; a non-finite one takes the float path
         splat = mbits != 0;
-        direct = mbits >= TB_NONFINITE;
-        S = 51 - ((int)(mbits >> 23) - 127);
+        direct = mbits >= FIXED_NONFINITE;
+        S = fixed_scale(mbits);
     }
 #pragma unroll 1
     for (int k = 0; k < NPX; k++) {
