@@ -609,9 +609,13 @@ __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX,
     const bool want_tex = g.tex[0] != nullptr;
     const int view = blockIdx.y;
     const int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
-    const int px = tx * 16 + (threadIdx.x & 15);
-    auto pixel = [&](int k) { return ((size_t)view * H + ty * ROWS + k * 16 + (threadIdx.x >> 4)) * W + px; };
-    auto covered = [&](int k) { return px < W && ty * ROWS + k * 16 + (int)(threadIdx.x >> 4) < H && rast[4 * pixel(k) + 3] >= 1.f; };
+    // wave w takes the pixels of parity class (w & 1, w >> 1) of each 16 x 16 band, so its lanes
+    // sit two pixels apart and fewer of them add to the same texel in one LDS instruction
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    const int lx = 2 * (ln & 7) + (wv & 1), ly = 2 * (ln >> 3) + (wv >> 1);
+    const int px = tx * 16 + lx;
+    auto pixel = [&](int k) { return ((size_t)view * H + ty * ROWS + k * 16 + ly) * W + px; };
+    auto covered = [&](int k) { return px < W && ty * ROWS + k * 16 + ly < H && rast[4 * pixel(k) + 3] >= 1.f; };
     int S = 0;
     bool splat = false, direct = false;
     if (want_tex) {
