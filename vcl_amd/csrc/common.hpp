@@ -42,6 +42,11 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
+// Sum over the wave's active lanes by DPP row operations and lane reads (no LDS traffic, unlike
+// the cross-lane shuffles of wave_sum), returned to every lane.
+extern "C" __device__ float __ockl_wfred_add_f32(float);
+__device__ __forceinline__ float wave_sum_dpp(float v) { return __ockl_wfred_add_f32(v); }
+
 __device__ __forceinline__ u32 wave_umax(u32 v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, (u32)__shfl_xor((int)v, o, 64));

@@ -927,7 +927,7 @@ __global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float x
     u64 pending = __ballot(act);
     while (pending) {
         int leader = __ffsll((unsigned long long)pending) - 1;
-        int lid = __shfl(id, leader, 64);
+        int lid = __builtin_amdgcn_readlane(id, leader);
         bool mine = act && id == lid;
         u64 m = __ballot(mine);
         pending &= ~m;
@@ -936,7 +936,7 @@ __global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float x
             for (int k = 0; k < 3; k++)
 #pragma unroll
                 for (int j = 0; j < 3; j++) {
-                    float t = wave_sum(mine ? gv[k][j] : 0.f);
+                    float t = wave_sum_dpp(mine ? gv[k][j] : 0.f);
                     if (lane == leader) gv[k][j] = t;
                 }
             add |= lane == leader;
