@@ -1527,28 +1527,50 @@ constexpr int AA_MAXC = 4;
 
 struct AAArgs {
     int W, H, depth, C, V;
+    int v4;  // C == 4 and the per-pixel buffers 16-byte aligned: one float4 load per pixel
     float sx, sy;
     const float* proj;
     const u32* idx;
     const float* rast;
 };
 
-// Evaluate pair (current pixel (cx,cy), neighbour left/below); returns edge count written.
-__device__ int aa_pair(const AAArgs& a, const float4* clipv, const float4* rastv, int cx, int cy, int vertical, int& pp, int& op, AAEdge* E) {
-#pragma clang fp contract(off)
+// The C (<= AA_MAXC) values of pixel q of a per-pixel buffer, zero-padded.
+__device__ __forceinline__ void aa_load_px(const float* base, size_t q, int C, bool v4, float (&o)[AA_MAXC]) {
+    if (v4) {
+        const float4 t = ((const float4*)base)[q];
+        o[0] = t.x; o[1] = t.y; o[2] = t.z; o[3] = t.w;
+    } else {
+#pragma unroll
+        for (int c = 0; c < AA_MAXC; c++) o[c] = c < C ? base[q * C + c] : 0.f;
+    }
+}
+
+// The pixels of pair (current pixel (cx,cy), neighbour left/below): false when the neighbour is
+// outside the image or both pixels hold the same triangle, else the front pixel pp, the other
+// pixel op and the front pixel's rast record.
+__device__ __forceinline__ bool aa_pair_pixels(const AAArgs& a, const float4* rastv, int cx, int cy, int vertical, int& pp, int& op,
+                                               float4& front) {
     int pid = cy * a.W + cx;
     int nid;
-    E[0].valid = E[1].valid = E[2].valid = 0;
-    if (!vertical) { if (cx <= 0) return 0; nid = pid - 1; } else { if (cy <= 0) return 0; nid = pid - a.W; }
+    if (!vertical) { if (cx <= 0) return false; nid = pid - 1; } else { if (cy <= 0) return false; nid = pid - a.W; }
     float4 t = rastv[pid], nb = rastv[nid];
-    if (nb.w == t.w) return 0;
+    if (nb.w == t.w) return false;
     pp = (int)t.w ? pid : nid;
     op = (int)nb.w ? pid : nid;
     if (pp == op) { if (t.z > nb.z) op = nid; else pp = nid; }
+    front = pp == pid ? t : nb;
+    return true;
+}
+
+// The edges of a pair whose pixels aa_pair_pixels found; returns the edge count written.
+__device__ __forceinline__ int aa_pair_edges(const AAArgs& a, const float4* clipv, float4 front, int cx, int cy, int vertical, int pp, int op,
+                                             AAEdge* E) {
+#pragma clang fp contract(off)
+    E[0].valid = E[1].valid = E[2].valid = 0;
     float fx, fy, ox, oy;
     if (!vertical) { fx = (float)cx + (pp > op ? .5f : -.5f); fy = (float)cy + .5f; ox = pp > op ? -1.f : 1.f; oy = 0.f; }
     else { fx = (float)cx + .5f; fy = (float)cy + (pp > op ? .5f : -.5f); ox = 0.f; oy = pp > op ? -1.f : 1.f; }
-    int id = (int)rastv[pp].w - 1;
+    int id = (int)front.w - 1;
     u32 vi[3] = {a.idx[3 * id], a.idx[3 * id + 1], a.idx[3 * id + 2]};
     float vx[3], vy[3];
 #pragma unroll
@@ -1627,8 +1649,15 @@ __global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float
         aa_tile_pair(p, x0, y0, cx, cy, vertical);
         float tc[AA_MAXC] = {0.f, 0.f, 0.f, 0.f}, tn[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
         int pp = 0, op = 0;
+        float4 front;
         AAEdge E[3];
-        if (cx < a.W && cy < a.H && aa_pair(a, clipv, rastv, cx, cy, vertical, pp, op, E)) {
+        if (cx < a.W && cy < a.H && aa_pair_pixels(a, rastv, cx, cy, vertical, pp, op, front)) {
+            // the two pixels' colours are loaded before the triangle's indices and vertices, so
+            // the loads overlap that dependent chain
+            float ip[AA_MAXC], io[AA_MAXC];
+            aa_load_px(inv, (size_t)pp, C, a.v4, ip);
+            aa_load_px(inv, (size_t)op, C, a.v4, io);
+            aa_pair_edges(a, clipv, front, cx, cy, vertical, pp, op, E);
             int cur = cy * a.W + cx;
 #pragma unroll
             for (int e = 0; e < 3; e++) {
@@ -1637,7 +1666,7 @@ __global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float
 #pragma unroll
                 for (int c = 0; c < AA_MAXC; c++) {
                     if (c >= C) continue;
-                    float v = E[e].alpha * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+                    float v = E[e].alpha * (ip[c] - io[c]);
                     if (to_c) tc[c] += v; else tn[c] += v;
                 }
             }
@@ -1651,8 +1680,7 @@ __global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float
     if (px >= a.W || py >= a.H) return;
     size_t q = (size_t)py * a.W + px;
     float acc[AA_MAXC];
-#pragma unroll
-    for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? inv[q * C + c] : 0.f;
+    aa_load_px(inv, q, C, a.v4, acc);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         float4 v = k < 2 ? to_cur[aa_pair_slot(tx, ty, k)] : to_nb[aa_pair_slot(tx, ty, k)];
@@ -1711,22 +1739,37 @@ __global__ void __launch_bounds__(BS, 7) k_aa_bwd(AAArgs a, int tilesX, const fl
         float D[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
         bool pp_cur = true;
         int pp = 0, op = 0;
+        float4 front;
         AAEdge E[3];
-        if (cx < a.W && cy < a.H && aa_pair(a, clipv, rastv, cx, cy, vertical, pp, op, E)) {
+        if (cx < a.W && cy < a.H && aa_pair_pixels(a, rastv, cx, cy, vertical, pp, op, front)) {
             pp_cur = pp == cy * a.W + cx;
+            // both pixels' output grads (and colours, for the vertex grads) are loaded before the
+            // triangle's indices and vertices, so the loads overlap that dependent chain
+            const bool vgrad = gcs && p < AA_OWN;
+            float gp_[AA_MAXC], go_[AA_MAXC], dc[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
+            aa_load_px(gov, (size_t)pp, C, a.v4, gp_);
+            aa_load_px(gov, (size_t)op, C, a.v4, go_);
+            if (vgrad) {
+                float ip[AA_MAXC], io[AA_MAXC];
+                aa_load_px(inv, (size_t)pp, C, a.v4, ip);
+                aa_load_px(inv, (size_t)op, C, a.v4, io);
+#pragma unroll
+                for (int c = 0; c < AA_MAXC; c++) dc[c] = ip[c] - io[c];
+            }
+            aa_pair_edges(a, clipv, front, cx, cy, vertical, pp, op, E);
 #pragma unroll
             for (int e = 0; e < 3; e++) {
                 if (!E[e].valid) continue;
-                int tgt = E[e].target_o ? op : pp;
+                const bool to_o = E[e].target_o;
                 float al = E[e].alpha;
 #pragma unroll
                 for (int c = 0; c < AA_MAXC; c++)
-                    if (c < C) D[c] += al * gov[(size_t)tgt * C + c];
-                if (gcs && p < AA_OWN) {
+                    if (c < C) D[c] += al * (to_o ? go_[c] : gp_[c]);
+                if (vgrad) {
                     float d = 0.f;
 #pragma unroll
                     for (int c = 0; c < AA_MAXC; c++)
-                        if (c < C) d += gov[(size_t)tgt * C + c] * (inv[(size_t)pp * C + c] - inv[(size_t)op * C + c]);
+                        if (c < C) d += (to_o ? go_[c] : gp_[c]) * dc[c];
                     const AAEdge& ed = E[e];
                     float ia = ed.ia, av = ed.pax * ed.pby - ed.pay * ed.pbx;
                     float dDax, dDay, dDbx, dDby;
@@ -1755,8 +1798,7 @@ __global__ void __launch_bounds__(BS, 7) k_aa_bwd(AAArgs a, int tilesX, const fl
     if (px < a.W && py < a.H) {
         size_t q = (size_t)py * a.W + px;
         float acc[AA_MAXC];
-#pragma unroll
-        for (int c = 0; c < AA_MAXC; c++) acc[c] = c < C ? gov[q * C + c] : 0.f;
+        aa_load_px(gov, q, C, a.v4, acc);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             float4 v = to_cur[aa_pair_slot(tx, ty, k)];
@@ -2282,9 +2324,11 @@ extern "C" int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_para
     return VCL_OK;
 }
 
-static AAArgs aa_args(const vcl_antialias_params* p) {
+// gout: the backward's output grads (16-byte alignment checked with the input's), or null
+static AAArgs aa_args(const vcl_antialias_params* p, const float* gout) {
     AAArgs a;
     a.W = p->width; a.H = p->height; a.depth = p->depth; a.C = p->channel; a.V = p->projNum;
+    a.v4 = p->channel == 4 && ((((uintptr_t)p->in) | (uintptr_t)gout) & 15) == 0;
     a.sx = p->s[0]; a.sy = p->s[1];
     a.proj = p->proj; a.idx = p->idx; a.rast = p->rast;
     return a;
@@ -2295,7 +2339,7 @@ extern "C" int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
     int tilesX = (p->width + AA_T - 1) / AA_T, tilesY = (p->height + AA_T - 1) / AA_T;
-    hipLaunchKernelGGL(k_aa_fwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p), tilesX, p->in, p->out);
+    hipLaunchKernelGGL(k_aa_fwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p, nullptr), tilesX, p->in, p->out);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2305,7 +2349,7 @@ extern "C" int vcl_antialias_backward(const vcl_antialias_params* p, const vcl_a
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
     int tilesX = (p->width + AA_T - 1) / AA_T, tilesY = (p->height + AA_T - 1) / AA_T;
-    hipLaunchKernelGGL(k_aa_bwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p), tilesX, p->in, g->out, g->in, g->proj);
+    hipLaunchKernelGGL(k_aa_bwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p, g->out), tilesX, p->in, g->out, g->in, g->proj);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
