@@ -47,6 +47,19 @@ __device__ __forceinline__ float wave_sum(float v) {
 extern "C" __device__ float __ockl_wfred_add_f32(float);
 __device__ __forceinline__ float wave_sum_dpp(float v) { return __ockl_wfred_add_f32(v); }
 
+// Inclusive prefix sum over the wave's 64 lanes (all active) by DPP: row shifts 1, 2, 4, 8 scan
+// each 16-lane row, then lane 15 of rows 0 and 2 is broadcast into rows 1 and 3 and lane 31
+// into rows 2 and 3.  No LDS traffic, unlike scans built on __shfl_up.
+__device__ __forceinline__ u32 wave_incl_scan(u32 x) {
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += (u32)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
 __device__ __forceinline__ u32 wave_umax(u32 v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, (u32)__shfl_xor((int)v, o, 64));

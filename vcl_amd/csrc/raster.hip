@@ -470,21 +470,11 @@ __global__ void __launch_bounds__(256) k_setup(RastArgs a) {
 // (which is advanced by the workgroup's total).
 __device__ u32 scan1024(u32 v, u32* wsum, u32* carry) {
     int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    u32 x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        u32 y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    u32 x = wave_incl_scan(v);
     if (lane == 63) wsum[wid] = x;
     __syncthreads();
     if (wid == 0) {
-        u32 s = lane < 16 ? wsum[lane] : 0u;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            u32 y = __shfl_up(s, o, 64);
-            if (lane >= o) s += y;
-        }
+        u32 s = wave_incl_scan(lane < 16 ? wsum[lane] : 0u);
         if (lane < 16) wsum[lane] = s;
     }
     __syncthreads();
@@ -790,12 +780,7 @@ __global__ void __launch_bounds__(256, TS == 32 ? 4 : 5) k_fine(RastArgs a) {
             // phase 2: block prefix of the covered-row counts; every covered row becomes an item
             // (piece, row, first column, length), in piece-then-row order (neighbouring rows of
             // one piece have similar lengths, so a wave's items are of similar length)
-            u32 incl = nrow;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                u32 yv = __shfl_up(incl, o, 64);
-                if ((tid & 63) >= o) incl += yv;
-            }
+            const u32 incl = wave_incl_scan(nrow);
             if ((tid & 63) == 63) sbucket[tid >> 6] = incl;
             __syncthreads();
             u32 pos0 = incl - nrow;
