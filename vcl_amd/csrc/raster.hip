@@ -495,23 +495,45 @@ __global__ void __launch_bounds__(1024) k_scan(RastArgs a, int NT) {
     if (tid == 0) { carry[0] = 0; carry[1] = 0; }
     __syncthreads();
     const u32 nrec = (u32)(a.nrec_direct + min(a.ws.misc[0], (u32)a.ovf_cap));
-    for (int base = 0; base < NT; base += 1024) {
-        int i = base + tid;
-        u32 v = i < NT ? a.ws.counts[i] : 0u;
-        u32 pre = scan1024(v, wsum, &carry[0]);
-        if (i < NT) a.ws.offsets[i] = pre;
-        bool all = (size_t)pre + v > a.bin_cap;
-        u32 nch = i >= NT ? 0u : all ? 1u : max(1u, (v + a.chunk - 1) / a.chunk);
-        u32 cpre = scan1024(nch, wsum, &carry[1]);
-        for (u32 k = 0; k < nch; k++) {
-            size_t c = (size_t)cpre + k;
-            if (c >= a.chunk_cap) break;
-            Chunk ch;
-            ch.tile = (u32)i;
-            ch.first = all ? 0u : pre + k * a.chunk;
-            ch.count = all ? (nrec | CHUNK_SCAN_ALL) : min(a.chunk, v - k * a.chunk);
-            ch.k = k;
-            a.ws.chunks[c] = ch;
+    // each thread takes TPT consecutive tiles: their local prefix, then one block scan of the
+    // threads' totals (TPT times fewer block scans and barriers than one tile per thread)
+    constexpr int TPT = 4;
+    for (int base = 0; base < NT; base += 1024 * TPT) {
+        const int i0 = base + TPT * tid;
+        u32 v[TPT], loc[TPT], t = 0;
+#pragma unroll
+        for (int j = 0; j < TPT; j++) {
+            v[j] = i0 + j < NT ? a.ws.counts[i0 + j] : 0u;
+            loc[j] = t;
+            t += v[j];
+        }
+        const u32 pre0 = scan1024(t, wsum, &carry[0]);
+        u32 nch[TPT], cloc[TPT], tc = 0;
+        bool all[TPT];
+#pragma unroll
+        for (int j = 0; j < TPT; j++) {
+            const int i = i0 + j;
+            const u32 pre = pre0 + loc[j];
+            if (i < NT) a.ws.offsets[i] = pre;
+            all[j] = (size_t)pre + v[j] > a.bin_cap;
+            nch[j] = i >= NT ? 0u : all[j] ? 1u : max(1u, (v[j] + a.chunk - 1) / a.chunk);
+            cloc[j] = tc;
+            tc += nch[j];
+        }
+        const u32 cpre0 = scan1024(tc, wsum, &carry[1]);
+#pragma unroll
+        for (int j = 0; j < TPT; j++) {
+            const u32 pre = pre0 + loc[j];
+            for (u32 k = 0; k < nch[j]; k++) {
+                size_t c = (size_t)cpre0 + cloc[j] + k;
+                if (c >= a.chunk_cap) break;
+                Chunk ch;
+                ch.tile = (u32)(i0 + j);
+                ch.first = all[j] ? 0u : pre + k * a.chunk;
+                ch.count = all[j] ? (nrec | CHUNK_SCAN_ALL) : min(a.chunk, v[j] - k * a.chunk);
+                ch.k = k;
+                a.ws.chunks[c] = ch;
+            }
         }
     }
     if (tid == 0) {
