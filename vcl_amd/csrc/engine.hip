@@ -85,7 +85,7 @@ static Plan plan(const vcl_fit_desc* d) {
     L.add("image", P * C * f4); L.add("gimage", P * C * f4); L.add("gcolor", P * C * f4);
     L.add("gdiff", P * C * f4);
     L.add("gtex", ntex * f4);
-    L.add("guv", P * 2 * f4); L.add("guvDA", P * 4 * f4);
+    L.add("guv", P * 2 * f4);
     L.add("grast", P * 4 * f4);
     L.add("gclip", (size_t)nv * V * 4 * f4);
     L.add("grad_pos", (size_t)V * 3 * f4);
@@ -415,7 +415,9 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tp.texture[l] = l < d.miplevel ? F->f("texture") + F->lvl_off[l] : nullptr;
     tp.out = F->f("diff");
     vcl_texturemap_grad_params tg;
-    tg.out = F->f("gdiff"); tg.uv = F->f("guv"); tg.uvDA = F->f("guvDA"); tg.daMode = VCL_TEX_GRAD_DA_REF;
+    // the uvDA grad is not requested: the interpolate backward ignores it, as the reference's does
+    // (interpolate.cu:114-143), so nothing downstream would read it
+    tg.out = F->f("gdiff"); tg.uv = F->f("guv"); tg.uvDA = nullptr; tg.daMode = VCL_TEX_GRAD_DA_REF;
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tg.texture[l] = l < d.miplevel ? F->f("gtex") + F->lvl_off[l] : nullptr;
     { ProfScope ps_(F, "texture_bwd", as_hip(s)); st = vcl_texturemap_backward(&tp, &tg, s); }
     if (st) return st;

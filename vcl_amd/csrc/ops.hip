@@ -539,7 +539,7 @@ __device__ __forceinline__ void tex_bwd4_pixel(TexGradTable<NB>& tab, bool want_
                                                float* guvDA, int mode) {
     float2 u = ((const float2*)uv)[p];
     int l0 = 0, l1 = 0; float f = 0.f, lev; float4 dl = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, &dl, mode);
+    if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, guvDA ? &dl : nullptr, mode);
     int4 i0; float fx, fy;
     index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
     const float4* T0 = (const float4*)t.tex[l0];
