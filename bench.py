@@ -56,8 +56,7 @@ def algorithmic_bytes(sc, views=1):
         "mse_loss": P * 12 * C,
         "antialias_bwd": P * (8 + 12 * C),
         "spherical_gaussian_bwd": P * (52 + 8 * C) + K * 64,
-        "texture_bwd": P * (4 + 8 + 16 + 4 * C + 8 + 16),
-        "interpolate_bwd": P * 32,
+        "texture_interp_bwd": P * (4 + 8 + 16 + 4 * C + 8) + V * 8 + T * 12,
         "rasterize_bwd": P * 24 + views * V * 40 + T * 12,
         "project_bwd": V * 40,
         "texture_adam": n0 * 32 + ncoarse * 4,

@@ -419,11 +419,9 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     // (interpolate.cu:114-143), so nothing downstream would read it
     tg.out = F->f("gdiff"); tg.uv = F->f("guv"); tg.uvDA = nullptr; tg.daMode = VCL_TEX_GRAD_DA_REF;
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tg.texture[l] = l < d.miplevel ? F->f("gtex") + F->lvl_off[l] : nullptr;
-    { ProfScope ps_(F, "texture_bwd", as_hip(s)); st = vcl_texturemap_backward(&tp, &tg, s); }
-    if (st) return st;
     vcl_interpolate_params ip = {W, H, nv, 2, 1, F->f("rast"), F->f("db"), F->f("uv"), F->t<uint32_t>("idx"), F->f("uvp"), F->f("uvDA"), d.V};
     vcl_interpolate_grad_params ig = {F->f("guv"), nullptr, F->f("grast"), nullptr, nullptr};
-    { ProfScope ps_(F, "interpolate_bwd", as_hip(s)); st = vcl_interpolate_backward(&ip, &ig, s); }
+    { ProfScope ps_(F, "texture_interp_bwd", as_hip(s)); st = vcl_texture_interp_backward_dev(&tp, &tg, &ip, &ig, s); }
     if (st) return st;
     vcl_rasterize_params rp;
     std::memset(&rp, 0, sizeof(rp));

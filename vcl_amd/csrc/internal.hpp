@@ -16,6 +16,11 @@ int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, f
 // tp->uv and tp->uvDA are ip's outputs.
 int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, vcl_stream_t stream);
 int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp, vcl_stream_t stream);
+// Texture backward then interpolate backward (the latter's output grad being the former's uv
+// grad): fused into one kernel when the tiled C == 4 texture backward applies, a 2-channel
+// attribute feeds the uv and only the rast grad is wanted; otherwise the two ops in turn.
+int vcl_texture_interp_backward_dev(const vcl_texturemap_params* tp, const vcl_texturemap_grad_params* tg, const vcl_interpolate_params* ip,
+                                    const vcl_interpolate_grad_params* ig, vcl_stream_t stream);
 // Fused texture update.  part bit 1: fold + Adam + clamp on level 0; bit 2: rebuild the mips and
 // clear the coarse grads.  bt_dev: device {rhomt, rhovt} used instead of h->rhomt/rhovt (null:
 // the host values).

@@ -531,12 +531,32 @@ __device__ __forceinline__ void tex_table_flush(TexGradTable<NB>& tab, const Tex
     }
 }
 
+// The interpolate backward of a 2-channel attribute fused into the texture backward that produces
+// its output grad (grast null: not fused).
+struct InterpBwdArgs {
+    const float* rast;
+    const u32* idx;
+    const float* attr;
+    float* grast;
+};
+
 // One covered pixel p of the C == 4 texture backward, with its output grad go (gs: go in the
-// table's fixed-point units): texel splats (want_tex), uv / uvDA grads stored.
+// table's fixed-point units): texel splats (want_tex), uv / uvDA grads stored, and, when fused,
+// the interpolate backward's rast grad from the uv grad.
 template <int NB>
 __device__ __forceinline__ void tex_bwd4_pixel(TexGradTable<NB>& tab, bool want_tex, bool direct, size_t p, float4 go, float4 gs,
                                                const TexArgs& t, const TexGradArgs& g, const float* uv, const float* uvDA, float* guv,
-                                               float* guvDA, int mode) {
+                                               float* guvDA, int mode, const InterpBwdArgs& ib) {
+    // the fused interpolate backward's triangle attributes, loaded first so that their chain of
+    // loads overlaps the texture work
+    float a0x = 0.f, a0y = 0.f, a1x = 0.f, a1y = 0.f, a2x = 0.f, a2y = 0.f;
+    if (ib.grast) {
+        const int id = (int)ib.rast[4 * p + 3] - 1;
+        const u32 i0 = ib.idx[3 * id], i1 = ib.idx[3 * id + 1], i2 = ib.idx[3 * id + 2];
+        a0x = ib.attr[2 * (size_t)i0]; a0y = ib.attr[2 * (size_t)i0 + 1];
+        a1x = ib.attr[2 * (size_t)i1]; a1y = ib.attr[2 * (size_t)i1 + 1];
+        a2x = ib.attr[2 * (size_t)i2]; a2y = ib.attr[2 * (size_t)i2 + 1];
+    }
     float2 u = ((const float2*)uv)[p];
     int l0 = 0, l1 = 0; float f = 0.f, lev; float4 dl = make_float4(0.f, 0.f, 0.f, 0.f);
     if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, guvDA ? &dl : nullptr, mode);
@@ -591,6 +611,14 @@ __device__ __forceinline__ void tex_bwd4_pixel(TexGradTable<NB>& tab, bool want_
     }
     if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
     if (guvDA && t.L > 1) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
+    if (ib.grast) {
+        // InterpolateBackward (interpolate.cu:114-143) for D = 2 with output grad (gu, gv), in
+        // its channel order
+        float ru = 0.f, rv = 0.f;
+        ru += gu * (a0x - a2x); rv += gu * (a1x - a2x);
+        ru += gv * (a0y - a2y); rv += gv * (a1y - a2y);
+        ((float2*)ib.grast)[p] = make_float2(ru, rv);
+    }
 }
 
 // Buckets of the texel table of a 16 x ROWS region: one per pixel up to 256 (35 KB of LDS), 512
@@ -603,7 +631,8 @@ constexpr int tex_table_buckets(int rows) { return rows <= 16 ? 256 : 512; }
 // region's fixed-point scale.
 template <int ROWS>
 __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
-                                                       const float* uvDA, const float* gout, float* guv, float* guvDA, int mode) {
+                                                       const float* uvDA, const float* gout, float* guv, float* guvDA, int mode,
+                                                       InterpBwdArgs ib) {
     constexpr int NPX = ROWS / 16;
     __shared__ TexGradTable<tex_table_buckets(ROWS)> tab;
     const bool want_tex = g.tex[0] != nullptr;
@@ -644,7 +673,7 @@ __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX,
         const size_t p = pixel(k);
         const float4 go = ((const float4*)gout)[p];
         const float4 gs = make_float4(ldexpf(go.x, S), ldexpf(go.y, S), ldexpf(go.z, S), ldexpf(go.w, S));
-        tex_bwd4_pixel(tab, splat, direct, p, go, gs, t, g, uv, uvDA, guv, guvDA, mode);
+        tex_bwd4_pixel(tab, splat, direct, p, go, gs, t, g, uv, uvDA, guv, guvDA, mode, ib);
     }
     if (!splat || direct) return;
     __syncthreads();
@@ -1960,7 +1989,12 @@ int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_pa
     return VCL_OK;
 }
 
-extern "C" int vcl_texturemap_backward(const vcl_texturemap_params* p, const vcl_texturemap_grad_params* g, vcl_stream_t stream) {
+// ib.grast not null: the interpolate backward to run inside the tiled kernel.  *fused_done tells
+// whether it ran; when the tiled kernel does not apply it stays false and the caller runs the
+// interpolate backward itself.
+static int texturemap_backward(const vcl_texturemap_params* p, const vcl_texturemap_grad_params* g, const InterpBwdArgs& ib, bool* fused_done,
+                               vcl_stream_t stream) {
+    if (fused_done) *fused_done = false;
     VCL_REQUIRE(p && g && p->rast && p->uv && g->out);
     TexArgs t;
     int st = tex_args(p, t);
@@ -1976,21 +2010,42 @@ extern "C" int vcl_texturemap_backward(const vcl_texturemap_params* p, const vcl
         int tilesX = (p->width + 15) / 16, tilesY = (p->height + rows - 1) / rows;
         dim3 grid(tilesX * tilesY, p->depth);
         float* gda = t.L > 1 ? g->uvDA : nullptr;
+        // fused: the uv grad is consumed in the kernel by the interpolate backward, not stored
+        float* guv = ib.grast ? nullptr : g->uv;
         if (rows == 64)
             hipLaunchKernelGGL(k_tex_bwd4_tiled<64>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
-                               g->out, g->uv, gda, g->daMode);
+                               g->out, guv, gda, g->daMode, ib);
         else if (rows == 32)
             hipLaunchKernelGGL(k_tex_bwd4_tiled<32>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
-                               g->out, g->uv, gda, g->daMode);
+                               g->out, guv, gda, g->daMode, ib);
         else
             hipLaunchKernelGGL(k_tex_bwd4_tiled<16>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
-                               g->out, g->uv, gda, g->daMode);
+                               g->out, guv, gda, g->daMode, ib);
+        if (fused_done) *fused_done = ib.grast != nullptr;
     } else {
         hipLaunchKernelGGL(k_tex_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, ga, p->rast, p->uv, p->uvDA, g->out, g->uv,
                            t.L > 1 ? g->uvDA : nullptr, g->daMode);
     }
     VCL_LAUNCH_CHECK();
     return VCL_OK;
+}
+
+extern "C" int vcl_texturemap_backward(const vcl_texturemap_params* p, const vcl_texturemap_grad_params* g, vcl_stream_t stream) {
+    return texturemap_backward(p, g, InterpBwdArgs{nullptr, nullptr, nullptr, nullptr}, nullptr, stream);
+}
+
+int vcl_texture_interp_backward_dev(const vcl_texturemap_params* tp, const vcl_texturemap_grad_params* tg, const vcl_interpolate_params* ip,
+                                    const vcl_interpolate_grad_params* ig, vcl_stream_t stream) {
+    VCL_REQUIRE(tp && tg && ip && ig && ip->rast && ip->attr && ip->idx && ig->out);
+    // fusable: the interpolate backward's output grad is this texture backward's uv grad, over
+    // the same pixels, with a 2-channel attribute and only the rast grad wanted
+    const bool fusable = ip->dimention == 2 && ig->out == tg->uv && !ig->attr && ig->rast && ip->rast == tp->rast && ip->width == tp->width &&
+                         ip->height == tp->height && ip->depth == tp->depth;
+    bool done = false;
+    int st = texturemap_backward(tp, tg, fusable ? InterpBwdArgs{ip->rast, ip->idx, ip->attr, ig->rast} : InterpBwdArgs{nullptr, nullptr, nullptr, nullptr},
+                                 &done, stream);
+    if (st || done) return st;
+    return vcl_interpolate_backward(ip, ig, stream);
 }
 
 extern "C" int vcl_texture_mip_levels(int w, int h, int requested) {
