@@ -297,12 +297,17 @@ static void fill_texture(vcl_fit* F, const char* name, vcl_texture& t) {
 }
 
 // Forward chain up to the antialiased image.
-static int forward(vcl_fit* F, vcl_stream_t s) {
+// step: also the step's start work, folded into the forward's first passes: the clip-space and
+// texture-output grad clears of the forward(XGradParams&) overloads, and the Adam
+// bias-correction bump (Adam::step bumps rhomt/rhovt before its launch, optimize.cu:109-115),
+// which the updates at the end of the step read.
+static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     const vcl_fit_desc& d = F->d;
     int W = d.width, H = d.height, nv = d.views, C = d.channel;
     int st;
     vcl_project_params pj = {d.V, 4, F->f("pos"), F->f("mats"), F->f("clip"), nv};
-    { ProfScope ps_(F, "project", as_hip(s)); st = vcl_project_forward(&pj, s); }
+    { ProfScope ps_(F, "project", as_hip(s));
+      st = step ? vcl_project_forward_dev(&pj, F->f("gclip"), F->t<double>("adam_bt"), d.beta1, d.beta2, s) : vcl_project_forward(&pj, s); }
     if (st) return st;
     vcl_rasterize_params rp;
     rp.width = W; rp.height = H; rp.depth = nv; rp.enableDB = 1;
@@ -330,11 +335,12 @@ static int forward(vcl_fit* F, vcl_stream_t s) {
     if (C == 4) {
         // uv interpolation and texture sample fused per pixel
         ProfScope ps_(F, "interp_texture", as_hip(s));
-        st = vcl_interp_tex_dev(&ip, &tp, s);
+        st = vcl_interp_tex_dev(&ip, &tp, step ? F->f("gdiff") : nullptr, s);
     } else {
         { ProfScope ps_(F, "interpolate", as_hip(s)); st = vcl_interpolate_forward(&ip, s); }
         if (st) return st;
         { ProfScope ps_(F, "texture", as_hip(s)); st = vcl_texturemap_forward(&tp, s); }
+        if (!st && step) st = vcl_fill_async(F->buf["gdiff"], F->bytes["gdiff"], as_hip(s));
     }
     if (st) return st;
     vcl_spherical_gaussian_params gp;
@@ -377,20 +383,14 @@ static int step_body(vcl_fit* F, vcl_stream_t s) {
 static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     const vcl_fit_desc& d = F->d;
     int W = d.width, H = d.height, nv = d.views, C = d.channel, K = d.sgnum;
-    hipStream_t hs = as_hip(s);
     int st;
-    // output-grad zeroing of the forward(XGradParams&) overloads; the same launch advances the
-    // Adam bias-correction powers of this step (Adam::step bumps rhomt/rhovt before its
-    // launch, optimize.cu:109-115), which the updates at the end of the step read
+    // the step's output-grad clears and the Adam bias-correction bump run inside the forward's
+    // first passes (forward(..., true))
     double* bt = F->t<double>("adam_bt");
-    {
-        void* ptrs[2] = {F->buf["gclip"], F->buf["gdiff"]};
-        size_t nb[2] = {F->bytes["gclip"], F->bytes["gdiff"]};
-        if ((st = vcl_fill_multi_bump_async(ptrs, nb, 2, bt, d.beta1, d.beta2, hs))) return st;
-    }
-    if ((st = forward(F, s))) return st;
+    if ((st = forward(F, s, true))) return st;
     vcl_loss_params lp = {F->f("image"), F->f("target"), F->f("gimage"), F->f("ws_loss"), W, H, nv, (int)(F->P * C), F->f("loss")};
-    { ProfScope ps_(F, "mse_loss", as_hip(s)); st = vcl_mse_loss_backward(&lp, s); }
+    // the loss's final sum runs in the Adam launch at the end of the step
+    { ProfScope ps_(F, "mse_loss", as_hip(s)); st = vcl_mse_loss_grad_dev(&lp, s); }
     if (st) return st;
     // backward
     vcl_antialias_params ap = {W, H, nv, C, {W / 2.f, H / 2.f}, F->f("clip"), F->t<uint32_t>("idx"), F->f("rast"), F->f("color"), F->f("image"), d.V};
@@ -431,10 +431,18 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     vcl_rasterize_grad_params rg = {F->f("gclip"), F->f("grast"), nullptr};
     { ProfScope ps_(F, "rasterize_bwd", as_hip(s)); st = vcl_rasterize_backward(&rp, &rg, s); }
     if (st) return st;
-    vcl_project_params pj = {d.V, 4, F->f("pos"), F->f("mats"), F->f("clip"), nv};
-    vcl_project_grad_params pg = {F->f("grad_pos"), nullptr, F->f("gclip")};
-    { ProfScope ps_(F, "project_bwd", as_hip(s)); st = vcl_project_backward(&pj, &pg, s); }
-    if (st) return st;
+    // the project backward (positions' grad from gclip) runs inside the Adam launch below for a
+    // few views; with many views each position element's loop over the views is better spread
+    // over the project backward's one thread per vertex
+    constexpr int FOLD_PROJECT_MAX_VIEWS = 4;
+    vcl_project_params pj = {d.V, 4, F->f("pos"), F->f("mats"), F->f("gclip"), nv};
+    const bool fold_project = nv <= FOLD_PROJECT_MAX_VIEWS;
+    if (!fold_project) {
+        vcl_project_params pf = {d.V, 4, F->f("pos"), F->f("mats"), F->f("clip"), nv};
+        vcl_project_grad_params pg = {F->f("grad_pos"), nullptr, F->f("gclip")};
+        { ProfScope ps_(F, "project_bwd", as_hip(s)); st = vcl_project_backward(&pf, &pg, s); }
+        if (st) return st;
+    }
     // update
     vcl_adam_hyper_params ht = {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_tex"), F->f("v_tex")};
     vcl_texture tex, gtex;
@@ -455,7 +463,8 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
         {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_sharp"), F->f("v_sharp")},
         {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_amp"), F->f("v_amp")}};
     // the parameter-grad clears (AttributeGrad::clear, SGBufferGrad::clear) happen inside Adam
-    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi_dev(K > 0 ? 4 : 1, op, hp, bt, 1, s); }
+    // pj.out: the clip-space grads the folded project backward takes
+    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi_proj_dev(K > 0 ? 4 : 1, op, hp, bt, 1, fold_project ? &pj : nullptr, &lp, s); }
     if (st) return st;
     return VCL_OK;
 }

@@ -14,7 +14,11 @@ int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, f
 // launch; sp->normal and sp->view are the outputs of the first two.
 // Interpolate (uv: 2 attributes with derivatives) and the C == 4 texture forward in one launch;
 // tp->uv and tp->uvDA are ip's outputs.
-int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, vcl_stream_t stream);
+// zero (may be null): a 4-float-per-pixel buffer cleared in the same pass.
+int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, float* zero, vcl_stream_t stream);
+// vcl_project_forward that also clears zero (shaped like p->out, may be null) and, when bt is not
+// null, applies the step-start bump bt[0] *= rhom, bt[1] *= rhov once.
+int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt, double rhom, double rhov, vcl_stream_t stream);
 int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp, vcl_stream_t stream);
 // Texture backward then interpolate backward (the latter's output grad being the former's uv
 // grad): fused into one kernel when the tiled C == 4 texture backward applies, a 2-channel
@@ -30,5 +34,14 @@ int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* gra
 // zero_grad: clear each grad after it is used.
 int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
                             vcl_stream_t stream);
+// The same with the project backward folded in: tensor 0 is proj's positions (3 floats per
+// vertex), and its grad, before the update, also takes the vec grad of vcl_project_backward
+// with proj->out as the output grad (clip-space, 4 floats per vertex and view).
+// loss (may be null): the launch also sums the loss partials that vcl_mse_loss_grad_dev left in
+// loss->buffer into *loss->loss (one more workgroup; n <= 8 tensors).
+int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
+                                 const vcl_project_params* proj, const vcl_loss_params* loss, vcl_stream_t stream);
+// vcl_mse_loss_backward without the final sum: the grad and the per-block partials only.
+int vcl_mse_loss_grad_dev(const vcl_loss_params* p, vcl_stream_t stream);
 // bt_dev[0] *= rhom; bt_dev[1] *= rhov (one thread; the step-start bump of Adam::step).
 }

@@ -116,9 +116,13 @@ int colsum_stage(const float* part, int rows, int cols, float* seg_scratch, hipS
 }
 
 // ---------------------------------------------------------------- project
-__global__ void __launch_bounds__(BS) k_project_fwd(int V, int dim, int views, const float* vec, const float* mat, float* out) {
+// zero (optional): a buffer shaped like out cleared in the same pass (the step's clip-space grad
+// clear); bt (optional): the step-start bump of Adam's bias-correction powers, done once.
+__global__ void __launch_bounds__(BS) k_project_fwd(int V, int dim, int views, const float* vec, const float* mat, float* out, float* zero,
+                                                    double* bt, double rhom, double rhov) {
 #pragma clang fp contract(off)
     size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (bt && gid == 0) { bt[0] *= rhom; bt[1] *= rhov; }
     if (gid >= (size_t)V * views) return;
     int s = (int)(gid / V), v = (int)(gid % V);
     const float* m = mat + 16 * s;
@@ -128,6 +132,7 @@ __global__ void __launch_bounds__(BS) k_project_fwd(int V, int dim, int views, c
     for (int i = 0; i < 4; i++) r[i] = m[i] * x + m[4 + i] * y + m[8 + i] * z + m[12 + i];
     if (dim == 4) ((float4*)out)[gid] = make_float4(r[0], r[1], r[2], r[3]);
     else for (int i = 0; i < dim; i++) out[gid * dim + i] = r[i];
+    if (zero) for (int i = 0; i < dim; i++) zero[gid * dim + i] = 0.f;
 }
 
 // gvec[v] += sum over views of M^T g (views in order).  gmat (camera-matrix grad, off the
@@ -329,8 +334,10 @@ __global__ void __launch_bounds__(BS) k_tex_fwd4(size_t P, TexArgs t, const floa
 // The fit step's uv interpolation (2 attributes, with derivatives) and C == 4 texture forward in
 // one pass per pixel: the same per-pixel code as the two ops, writing uv and its derivatives for
 // the backward and the texture sample.
+// zero (optional): a buffer shaped like out (4 floats per pixel) cleared in the same pass (the
+// step's texture-output grad clear).
 __global__ void __launch_bounds__(BS) k_interp_tex4(size_t P, const float* rast, const float* db, const float* uvattr, const u32* idx, float* uvp,
-                                                    float* uvDA, TexArgs t, float* out) {
+                                                    float* uvDA, TexArgs t, float* out, float* zero) {
     size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
     const float4 r = ((const float4*)rast)[p];
@@ -350,6 +357,7 @@ __global__ void __launch_bounds__(BS) k_interp_tex4(size_t P, const float* rast,
     ((float4*)uvDA)[p] = da;
     if (r.w >= 1.f) o = tex4_px(t, u, da);
     ((float4*)out)[p] = o;
+    if (zero) ((float4*)zero)[p] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 __global__ void __launch_bounds__(BS) k_tex_fwd(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) {
@@ -721,6 +729,29 @@ __global__ void __launch_bounds__(BS) k_clamp(float* x, size_t n, float lo, floa
 
 // ---------------------------------------------------------------- Adam
 constexpr int ADAM_MAX = 8;
+// The loss from the nb per-block partials, by one workgroup of NT threads in a fixed order:
+// thread k sums a contiguous run with four interleaved accumulators, then the waves, then the
+// wave totals in order.
+template <int NT>
+__device__ __forceinline__ void mse_final_block(const float* part, size_t nb, float* loss) {
+    const size_t per = (nb + NT - 1) / NT;
+    const size_t b = threadIdx.x * per, e = min(nb, b + per);
+    float r0 = 0.f, r1 = 0.f, r2 = 0.f, r3 = 0.f;
+    size_t i = b;
+    for (; i + 4 <= e; i += 4) { r0 += part[i]; r1 += part[i + 1]; r2 += part[i + 2]; r3 += part[i + 3]; }
+    for (; i < e; i++) r0 += part[i];
+    float r = (r0 + r1) + (r2 + r3);
+    r = wave_sum(r);
+    __shared__ float sh[NT / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+        for (int w = 0; w < NT / 64; w++) t += sh[w];
+        *loss = t * 0.5f;
+    }
+}
+
 struct AdamArgs {
     int n;
     float* param[ADAM_MAX];
@@ -732,7 +763,29 @@ struct AdamArgs {
     double rhom[ADAM_MAX], rhov[ADAM_MAX], rhomt[ADAM_MAX], rhovt[ADAM_MAX], eta[ADAM_MAX], eps[ADAM_MAX];
     const double* bt;   // device {rhomt, rhovt} shared by every tensor (graph replays), or null
     int zero_grad;      // clear each grad after use (the caller's grad clear, fused)
+    // tensor 0's grad (positions, 3 floats per vertex) also takes the project backward of pgclip
+    // (pviews x pV clip-space grads) through pmats, when pgclip is not null
+    const float* pgclip;
+    const float* pmats;
+    int pviews, pV;
+    // the loss's final sum from loss_nb partials, by the launch's last workgroup, when loss is
+    // not null
+    const float* loss_part;
+    size_t loss_nb;
+    float* loss;
 };
+
+// ProjectBackward's vec grad (project_bwd above) of element i = 3 v + j of the positions.
+__device__ __forceinline__ float adam_proj_grad(const AdamArgs& a, long long i) {
+    const int v = (int)(i / 3), j = (int)(i % 3);
+    float acc = 0.f;
+    for (int s = 0; s < a.pviews; s++) {
+        const float* m = a.pmats + 16 * s + 4 * j;
+        const float4 g = ((const float4*)a.pgclip)[(size_t)s * a.pV + v];
+        acc += m[0] * g.x; acc += m[1] * g.y; acc += m[2] * g.z; acc += m[3] * g.w;
+    }
+    return acc;
+}
 
 __device__ __forceinline__ void adam_elem(float& param, float g, float& m, float& v, double rhom, double rhov, double rhomt, double rhovt,
                                           double eta, double eps) {
@@ -747,6 +800,10 @@ __device__ __forceinline__ void adam_elem(float& param, float g, float& m, float
 constexpr int ADAM_PER_BLOCK = BS * 4;
 
 __global__ void __launch_bounds__(BS) k_adam(AdamArgs a) {
+    if (a.loss && blockIdx.x == gridDim.x - 1) {
+        mse_final_block<BS>(a.loss_part, a.loss_nb, a.loss);
+        return;
+    }
     long long b = blockIdx.x;
     int t = 0;
     while (t + 1 < a.n && b >= a.block_start[t + 1]) t++;
@@ -758,7 +815,9 @@ __global__ void __launch_bounds__(BS) k_adam(AdamArgs a) {
         long long i = base + k * BS + threadIdx.x;
         if (i >= n) break;
         float p = P[i], m = M[i], v = Vv[i];
-        adam_elem(p, G[i], m, v, a.rhom[t], a.rhov[t], rmt, rvt, a.eta[t], a.eps[t]);
+        float g = G[i];
+        if (t == 0 && a.pgclip) g += adam_proj_grad(a, i);
+        adam_elem(p, g, m, v, a.rhom[t], a.rhov[t], rmt, rvt, a.eta[t], a.eps[t]);
         P[i] = p; M[i] = m; Vv[i] = v;
         if (a.zero_grad) const_cast<float*>(G)[i] = 0.f;
     }
@@ -1874,24 +1933,7 @@ __global__ void __launch_bounds__(BS) k_mse(size_t n, const float* pred, const f
 
 // loss = 0.5 * the sum of the nb block partials, in a fixed order (one 1024-thread workgroup;
 // thread t sums its contiguous share with four interleaved accumulators so the loads overlap)
-__global__ void __launch_bounds__(1024) k_mse_final(const float* part, size_t nb, float* loss) {
-    const size_t per = (nb + 1023) / 1024;
-    const size_t b = threadIdx.x * per, e = min(nb, b + per);
-    float r0 = 0.f, r1 = 0.f, r2 = 0.f, r3 = 0.f;
-    size_t i = b;
-    for (; i + 4 <= e; i += 4) { r0 += part[i]; r1 += part[i + 1]; r2 += part[i + 2]; r3 += part[i + 3]; }
-    for (; i < e; i++) r0 += part[i];
-    float r = (r0 + r1) + (r2 + r3);
-    r = wave_sum(r);
-    __shared__ float sh[16];
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = r;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.f;
-        for (int w = 0; w < 16; w++) t += sh[w];
-        *loss = t * 0.5f;
-    }
-}
+__global__ void __launch_bounds__(1024) k_mse_final(const float* part, size_t nb, float* loss) { mse_final_block<1024>(part, nb, loss); }
 
 }  // namespace
 
@@ -1909,14 +1951,19 @@ extern "C" const char* vcl_status_string(int s) {
 
 extern "C" const char* vcl_build_info(void) { return "vcl_amd 0.1 gfx950"; }
 
-extern "C" int vcl_project_forward(const vcl_project_params* p, vcl_stream_t stream) {
+int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt, double rhom, double rhov, vcl_stream_t stream) {
     VCL_REQUIRE(p && p->vec && p->mat && p->out && p->vecNum >= 0 && (p->dimention == 3 || p->dimention == 4));
     int views = p->views < 1 ? 1 : p->views;
     size_t n = (size_t)p->vecNum * views;
-    if (!n) return VCL_OK;
-    hipLaunchKernelGGL(k_project_fwd, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat, p->out);
+    if (!n && !bt) return VCL_OK;
+    hipLaunchKernelGGL(k_project_fwd, dim3(n ? cdiv(n, BS) : 1), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat, p->out,
+                       zero, bt, rhom, rhov);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
+}
+
+extern "C" int vcl_project_forward(const vcl_project_params* p, vcl_stream_t stream) {
+    return vcl_project_forward_dev(p, nullptr, nullptr, 1.0, 1.0, stream);
 }
 
 extern "C" int vcl_project_backward(const vcl_project_params* p, const vcl_project_grad_params* g, vcl_stream_t stream) {
@@ -1974,7 +2021,7 @@ extern "C" int vcl_texturemap_forward(const vcl_texturemap_params* p, vcl_stream
     return VCL_OK;
 }
 
-int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, vcl_stream_t stream) {
+int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, float* zero, vcl_stream_t stream) {
     VCL_REQUIRE(ip && tp && ip->rast && ip->rastDB && ip->attr && ip->idx && ip->out && ip->outDA && tp->out);
     VCL_REQUIRE(ip->dimention == 2 && ip->enableDA && tp->channel == 4 && tp->uv == ip->out && tp->uvDA == ip->outDA && tp->rast == ip->rast);
     VCL_REQUIRE(ip->width == tp->width && ip->height == tp->height && ip->depth == tp->depth);
@@ -1984,7 +2031,7 @@ int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_pa
     size_t P = (size_t)ip->width * ip->height * ip->depth;
     if (!P) return VCL_OK;
     hipLaunchKernelGGL(k_interp_tex4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, ip->rast, ip->rastDB, ip->attr, ip->idx, ip->out, ip->outDA,
-                       t, tp->out);
+                       t, tp->out, zero);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2107,11 +2154,29 @@ extern "C" int vcl_texture_grad_clear(const vcl_texture* g, vcl_stream_t stream)
 // bt_dev: device {rhomt, rhovt} used instead of the per-tensor host values (null: host values).
 extern "C" int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev,
                                        int zero_grad, vcl_stream_t stream) {
+    return vcl_adam_step_multi_proj_dev(n, p, h, bt_dev, zero_grad, nullptr, nullptr, stream);
+}
+
+int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
+                                 const vcl_project_params* proj, const vcl_loss_params* loss, vcl_stream_t stream) {
+    if (loss) VCL_REQUIRE(loss->buffer && loss->loss && loss->size >= 0 && n <= ADAM_MAX);
     VCL_REQUIRE(n >= 0 && (n == 0 || (p && h)));
+    if (proj) {
+        // tensor 0 must be the projected positions, 3 floats per vertex, with 4-float clip grads
+        VCL_REQUIRE(n >= 1 && proj->dimention == 4 && proj->vec == p[0].param && p[0].size == 3 * (long long)proj->vecNum && proj->mat &&
+                    proj->out);
+    }
     for (int base = 0; base < n; base += ADAM_MAX) {
         AdamArgs a;
         a.bt = bt_dev;
         a.zero_grad = zero_grad;
+        a.pgclip = base == 0 && proj ? proj->out : nullptr;
+        a.pmats = proj ? proj->mat : nullptr;
+        a.pviews = proj ? (proj->views < 1 ? 1 : proj->views) : 0;
+        a.pV = proj ? proj->vecNum : 0;
+        a.loss = base == 0 && loss ? loss->loss : nullptr;
+        a.loss_part = loss ? loss->buffer : nullptr;
+        a.loss_nb = loss && loss->size > 0 ? cdiv((size_t)loss->size, (size_t)BS * 4) : 0;
         a.n = n - base < ADAM_MAX ? n - base : ADAM_MAX;
         long long blocks = 0;
         for (int i = 0; i < a.n; i++) {
@@ -2124,8 +2189,8 @@ extern "C" int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, cons
             blocks += (q.size + ADAM_PER_BLOCK - 1) / ADAM_PER_BLOCK;
         }
         a.block_start[a.n] = blocks;
-        if (blocks > 0) {
-            hipLaunchKernelGGL(k_adam, dim3((unsigned)blocks), dim3(BS), 0, as_hip(stream), a);
+        if (blocks > 0 || a.loss) {
+            hipLaunchKernelGGL(k_adam, dim3((unsigned)(blocks + (a.loss ? 1 : 0))), dim3(BS), 0, as_hip(stream), a);
             VCL_LAUNCH_CHECK();
         }
     }
@@ -2415,7 +2480,7 @@ extern "C" size_t vcl_loss_workspace_size(int size) {
     return sizeof(float) * nb + 256;
 }
 
-extern "C" int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stream) {
+int vcl_mse_loss_grad_dev(const vcl_loss_params* p, vcl_stream_t stream) {
     VCL_REQUIRE(p && p->predict && p->target && p->grad && p->buffer && p->loss && p->size >= 0);
     hipStream_t s = as_hip(stream);
     const size_t nb = p->size > 0 ? cdiv((size_t)p->size, (size_t)BS * 4) : 0;
@@ -2424,7 +2489,14 @@ extern "C" int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stre
         hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(BS), 0, s, (size_t)p->size, p->predict, p->target, p->grad, p->buffer, vec);
         VCL_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(1024), 0, s, p->buffer, nb, p->loss);
+    return VCL_OK;
+}
+
+extern "C" int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stream) {
+    int st = vcl_mse_loss_grad_dev(p, stream);
+    if (st) return st;
+    const size_t nb = p->size > 0 ? cdiv((size_t)p->size, (size_t)BS * 4) : 0;
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(1024), 0, as_hip(stream), p->buffer, nb, p->loss);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
