@@ -59,8 +59,8 @@ def algorithmic_bytes(sc, views=1):
         "texture_interp_bwd": P * (4 + 8 + 16 + 4 * C + 8) + V * 8 + T * 12,
         "rasterize_bwd": P * 24 + views * V * 40 + T * 12,
         "project_bwd": V * 40,
-        "texture_adam": n0 * 32 + ncoarse * 4,
-        "texture_mip": (n0 + ncoarse) * 4 + ncoarse * 8,
+        "texture_adam": n0 * 32 + ncoarse * 4 + (n0 // 4) * 8,
+        "texture_mip": ncoarse * 4 + (ncoarse - n0 // 4) * 8,
         "adam": (V * 3 + K * (4 + C)) * 28,
     }
 

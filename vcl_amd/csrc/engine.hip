@@ -448,9 +448,9 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     vcl_texture tex, gtex;
     fill_texture(F, "texture", tex);
     fill_texture(F, "gtex", gtex);
-    { ProfScope ps_(F, "texture_adam", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 1, bt, s); }
+    { ProfScope ps_(F, "texture_adam", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 1 | 4, bt, s); }
     if (st) return st;
-    { ProfScope ps_(F, "texture_mip", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 2, bt, s); }
+    { ProfScope ps_(F, "texture_mip", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 2 | 4, bt, s); }
     if (st) return st;
     vcl_optimize_params op[4] = {
         {F->f("pos"), F->f("grad_pos"), d.V * 3, d.V, 3, 1},

@@ -26,8 +26,10 @@ int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev,
 int vcl_texture_interp_backward_dev(const vcl_texturemap_params* tp, const vcl_texturemap_grad_params* tg, const vcl_interpolate_params* ip,
                                     const vcl_interpolate_grad_params* ig, vcl_stream_t stream);
 // Fused texture update.  part bit 1: fold + Adam + clamp on level 0; bit 2: rebuild the mips and
-// clear the coarse grads.  bt_dev: device {rhomt, rhovt} used instead of h->rhomt/rhovt (null:
-// the host values).
+// clear the coarse grads; bit 4 (C == 4): level 1 and its grad clear move from the mip rebuild
+// into the level-0 update (one 2 x 2 quad per thread), so the two calls of a split update must
+// both pass it.  bt_dev: device {rhomt, rhovt} used instead of h->rhomt/rhovt (null: the host
+// values).
 int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
                                   int quarter, int part, const double* bt_dev, vcl_stream_t stream);
 // vcl_adam_step_multi with the bias-correction powers read from the device (bt_dev, may be null).

@@ -828,6 +828,7 @@ __global__ void __launch_bounds__(BS) k_adam(AdamArgs a) {
 struct TexUpdArgs {
     int tw, th, C, L;
     float* tex0;
+    float* tex1;   // level 1 (k_tex_update4m writes it)
     float* grad[VCL_TEX_MAX_MIP_LEVEL];
     float* m;
     float* v;
@@ -907,6 +908,62 @@ __global__ void __launch_bounds__(BS) k_tex_update4(TexUpdArgs a) {
     ((float4*)a.m)[i] = m;
     ((float4*)a.v)[i] = v;
     ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// k_tex_update4 that also builds level 1 (L > 1, even level-0 sizes): one texel per lane as
+// there, with each wave on a 32 x 2 texel patch (a workgroup on 128 x 2), so the four texels of
+// a level-1 quad sit in lanes l, l ^ 1, l ^ 32, l ^ 33.  After the updates the quad's first lane
+// gathers the other three new values and writes their mean in k_downsample4's order, and clears
+// the level-1 grad (which all four lanes read for the folded grad before).
+__global__ void __launch_bounds__(BS) k_tex_update4m(TexUpdArgs a) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int x = blockIdx.x * 128 + (threadIdx.x >> 6) * 32 + (lane & 31);
+    const int y = blockIdx.y * 2 + (lane >> 5);
+    const bool in = x < a.tw;   // a.tw and a.th are even: a quad is all in or all out
+    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in) {
+        const size_t i = (size_t)y * a.tw + x;
+        float4 anc[VCL_TEX_MAX_MIP_LEVEL];
+#pragma unroll
+        for (int l = 1; l < VCL_TEX_MAX_MIP_LEVEL; l++)
+            if (l < a.L) anc[l] = ((const float4*)a.grad[l])[(size_t)(y >> l) * (a.tw >> l) + (x >> l)];
+        float4 g = ((const float4*)a.grad[0])[i];
+        float4 m = ((const float4*)a.m)[i], v = ((const float4*)a.v)[i];
+        p = ((const float4*)a.tex0)[i];
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int l = VCL_TEX_MAX_MIP_LEVEL - 1; l >= 1; l--) {
+            if (l == a.L - 1) acc = anc[l];
+            else if (l < a.L - 1) {
+                float4 t = anc[l];
+                add_nan_check4(t, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
+                acc = t;
+            }
+        }
+        add_nan_check4(g, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
+        const double rmt = a.bt ? a.bt[0] : a.rhomt, rvt = a.bt ? a.bt[1] : a.rhovt;
+        adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+        adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+        adam_elem(p.z, g.z, m.z, v.z, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+        adam_elem(p.w, g.w, m.w, v.w, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+        p.x = p.x < a.lo ? a.lo : p.x > a.hi ? a.hi : p.x;
+        p.y = p.y < a.lo ? a.lo : p.y > a.hi ? a.hi : p.y;
+        p.z = p.z < a.lo ? a.lo : p.z > a.hi ? a.hi : p.z;
+        p.w = p.w < a.lo ? a.lo : p.w > a.hi ? a.hi : p.w;
+        ((float4*)a.tex0)[i] = p;
+        ((float4*)a.m)[i] = m;
+        ((float4*)a.v)[i] = v;
+        ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    auto xor4 = [](float4 t, int o) { return make_float4(__shfl_xor(t.x, o, 64), __shfl_xor(t.y, o, 64), __shfl_xor(t.z, o, 64), __shfl_xor(t.w, o, 64)); };
+    const float4 p01 = xor4(p, 1), p10 = xor4(p, 32), p11 = xor4(p, 33);
+    if (in && (lane & 33) == 0) {
+        const size_t j = (size_t)(y >> 1) * (a.tw >> 1) + (x >> 1);
+        ((float4*)a.tex1)[j] = make_float4((((p.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p.y + p01.y) + p10.y) + p11.y) * 0.25f,
+                                           (((p.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p.w + p01.w) + p10.w) + p11.w) * 0.25f);
+        ((float4*)a.grad[1])[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
 }
 
 // C == 4 mip level from the level above (float4 per texel), clearing that level's grads.
@@ -2213,6 +2270,7 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
     TexUpdArgs a;
     a.tw = tex->width; a.th = tex->height; a.C = tex->channel; a.L = tex->miplevel;
     a.tex0 = tex->texture[0];
+    a.tex1 = a.L > 1 ? tex->texture[1] : nullptr;
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) a.grad[l] = l < a.L ? grad->texture[l] : nullptr;
     for (int l = 0; l < a.L; l++) VCL_REQUIRE(a.grad[l] && tex->texture[l]);
     a.m = h->m; a.v = h->v;
@@ -2224,8 +2282,13 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
     bool vec4 = a.C == 4;
     for (int l = 0; l < a.L; l++) vec4 &= ((uintptr_t)a.grad[l] % 16 == 0) && ((uintptr_t)tex->texture[l] % 16 == 0);
     vec4 &= ((uintptr_t)a.m % 16 == 0) && ((uintptr_t)a.v % 16 == 0);
+    // part & 4: level 1 is built by the C == 4 update (k_tex_update4m) instead of the mip pass;
+    // both calls of a split update must pass it
+    const bool quad = vec4 && a.L > 1 && (part & 4) && a.tw % 2 == 0 && a.th % 2 == 0 && a.th / 2 <= 65535;
     if (part & 1) {
-        if (vec4)
+        if (quad)
+            hipLaunchKernelGGL(k_tex_update4m, dim3((unsigned)cdiv((size_t)a.tw, 128), (unsigned)(a.th / 2)), dim3(BS), 0, s, a);
+        else if (vec4)
             hipLaunchKernelGGL(k_tex_update4, dim3(cdiv((size_t)a.tw * a.th, BS)), dim3(BS), 0, s, a);
         else
             hipLaunchKernelGGL(k_tex_update, dim3(cdiv(n, BS)), dim3(BS), 0, s, a);
@@ -2235,8 +2298,8 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
     // with one thread per texel, then up to six levels per launch (k_mip_fused)
     if (!(part & 2)) return VCL_OK;
     if (vec4) {
-        int l = 1;
-        if (l < a.L) {
+        int l = quad ? 2 : 1;
+        if (l == 1 && l < a.L) {
             int w = a.tw >> 1, hh = a.th >> 1;
             hipLaunchKernelGGL(k_downsample4, dim3(cdiv((size_t)w * hh, BS)), dim3(BS), 0, s, (const float4*)tex->texture[0],
                                (float4*)tex->texture[1], w, hh, (float4*)a.grad[1]);
@@ -2271,7 +2334,7 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
 
 extern "C" int vcl_texture_adam_update(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
                                        int quarter, vcl_stream_t stream) {
-    return vcl_texture_adam_update_parts(tex, grad, h, lo, hi, quarter, 3, nullptr, stream);
+    return vcl_texture_adam_update_parts(tex, grad, h, lo, hi, quarter, 3 | 4, nullptr, stream);
 }
 
 extern "C" int vcl_normal_axis_forward(const vcl_normal_axis_params* p, vcl_stream_t stream) {
