@@ -20,7 +20,7 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per k_tex_update4 launch at the default workload, from two rocprofv3 --pmc runs
+# HBM bytes per k_tex_update4m launch at the default workload, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE) scaled by what the counters read for a 256 MiB device copy
 TRAFFIC_FILE = "profiles/fit_step_1024_100k_hbm_traffic.md"
 
@@ -29,7 +29,7 @@ def measured_traffic():
     try:
         with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
             for line in f:
-                if line.startswith("k_tex_update4 traffic_bytes_per_launch:"):
+                if line.startswith("k_tex_update4m traffic_bytes_per_launch:"):
                     return int(line.split(":")[1])
     except OSError:
         pass
@@ -393,7 +393,7 @@ def main():
     rk = kernels.get(ROOF_OP)
     achieved = rk["GBps"] if rk else None
     traffic = measured_traffic() if args.res == 1024 else None
-    roofline = {"bound": "hbm", "kernel": "k_tex_update4 (" + ROOF_OP + ")", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+    roofline = {"bound": "hbm", "kernel": "k_tex_update4m (" + ROOF_OP + ")", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None, "traffic": traffic,
                 "traffic_source": TRAFFIC_FILE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)" if traffic else None,
                 "alg_bytes_per_launch": ab[ROOF_OP], "avg_ms": rk["avg_ms"] if rk else None}
