@@ -407,7 +407,9 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     std::memset(&gg, 0, sizeof(gg));
     gg.diffuse = F->f("gdiff"); gg.axis = F->f("grad_axis"); gg.sharpness = F->f("grad_sharpness"); gg.amplitude = F->f("grad_amplitude");
     gg.out = F->f("gcolor"); gg.workspace = F->buf["ws_sg"]; gg.workspace_size = F->ws_sg;
-    { ProfScope ps_(F, "spherical_gaussian_bwd", as_hip(s)); st = vcl_spherical_gaussian_backward(&gp, &gg, s); }
+    // the lobe grads' final sums run in the Adam launch at the end of the step
+    vcl_sg_lobe_sums lobes = {nullptr, 0, 0, 0};
+    { ProfScope ps_(F, "spherical_gaussian_bwd", as_hip(s)); st = vcl_spherical_gaussian_backward_dev(&gp, &gg, &lobes, s); }
     if (st) return st;
     vcl_texturemap_params tp;
     tp.width = W; tp.height = H; tp.depth = nv; tp.texwidth = d.texwidth; tp.texheight = d.texheight; tp.channel = C; tp.miplevel = d.miplevel;
@@ -464,7 +466,7 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
         {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_amp"), F->f("v_amp")}};
     // the parameter-grad clears (AttributeGrad::clear, SGBufferGrad::clear) happen inside Adam
     // pj.out: the clip-space grads the folded project backward takes
-    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi_proj_dev(K > 0 ? 4 : 1, op, hp, bt, 1, fold_project ? &pj : nullptr, &lp, s); }
+    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi_proj_dev(K > 0 ? 4 : 1, op, hp, bt, 1, fold_project ? &pj : nullptr, &lp, lobes.seg ? &lobes : nullptr, s); }
     if (st) return st;
     return VCL_OK;
 }

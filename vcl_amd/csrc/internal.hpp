@@ -39,10 +39,23 @@ int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_
 // The same with the project backward folded in: tensor 0 is proj's positions (3 floats per
 // vertex), and its grad, before the update, also takes the vec grad of vcl_project_backward
 // with proj->out as the output grad (clip-space, 4 floats per vertex and view).
+// The SG backward's per-lobe sums left for the caller: seg holds nseg rows of K x 8 columns
+// (axis 3, sharpness 1, amplitude up to 4), summed over the rows in order into the lobe grads.
+struct vcl_sg_lobe_sums {
+    const float* seg;
+    int nseg, K, C;
+};
+// vcl_spherical_gaussian_backward that, when lobes is not null and all three lobe grads are
+// requested, leaves the lobe sums to the caller (lobes->seg set) instead of adding them.
+int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
+                                        vcl_sg_lobe_sums* lobes, vcl_stream_t stream);
 // loss (may be null): the launch also sums the loss partials that vcl_mse_loss_grad_dev left in
-// loss->buffer into *loss->loss (one more workgroup; n <= 8 tensors).
+// loss->buffer into *loss->loss (one more workgroup; n <= 8 tensors).  lobes (may be null, or
+// with seg null): tensors 1..3 are the lobe axis, sharpness and amplitude, whose grads first
+// take those sums.
 int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
-                                 const vcl_project_params* proj, const vcl_loss_params* loss, vcl_stream_t stream);
+                                 const vcl_project_params* proj, const vcl_loss_params* loss, const vcl_sg_lobe_sums* lobes,
+                                 vcl_stream_t stream);
 // vcl_mse_loss_backward without the final sum: the grad and the per-block partials only.
 int vcl_mse_loss_grad_dev(const vcl_loss_params* p, vcl_stream_t stream);
 // bt_dev[0] *= rhom; bt_dev[1] *= rhov (one thread; the step-start bump of Adam::step).

@@ -760,6 +760,8 @@ struct AdamArgs {
     float* v[ADAM_MAX];
     long long size[ADAM_MAX];
     long long block_start[ADAM_MAX + 1];
+    int per_thread[ADAM_MAX];   // elements per thread: ADAM_PER_BLOCK / BS, or 1 for tensors whose
+                                // elements each sum a column of lobe partials
     double rhom[ADAM_MAX], rhov[ADAM_MAX], rhomt[ADAM_MAX], rhovt[ADAM_MAX], eta[ADAM_MAX], eps[ADAM_MAX];
     const double* bt;   // device {rhomt, rhovt} shared by every tensor (graph replays), or null
     int zero_grad;      // clear each grad after use (the caller's grad clear, fused)
@@ -773,7 +775,27 @@ struct AdamArgs {
     const float* loss_part;
     size_t loss_nb;
     float* loss;
+    // tensors 1..3 (lobe axis, sharpness, amplitude) also take the SG backward's lobe sums:
+    // column i * SG_COLS + k of lseg, summed over lnseg segment rows (k_sg_lobe_final's work)
+    const float* lseg;
+    int lnseg, lK, lC;
 };
+
+constexpr int ADAM_SG_COLS = 3 + 1 + 4;  // the SG backward's per-lobe columns (axis, sharpness, C <= 4 amplitudes)
+
+// k_sg_lobe_final's sum for element e of lobe tensor t (1 axis, 2 sharpness, 3 amplitude).
+__device__ __forceinline__ float adam_lobe_sum(const AdamArgs& a, int t, long long e) {
+    int i, k;
+    if (t == 1) { i = (int)(e / 3); k = (int)(e % 3); }
+    else if (t == 2) { i = (int)e; k = 3; }
+    else { i = (int)(e / a.lC); k = 4 + (int)(e % a.lC); }
+    const size_t c = (size_t)i * ADAM_SG_COLS + k, stride = (size_t)a.lK * ADAM_SG_COLS;
+    float s = 0.f;
+    // unrolled so that the loads run ahead of the (in-order) adds
+#pragma unroll 8
+    for (int r = 0; r < a.lnseg; r++) s += a.lseg[(size_t)r * stride + c];
+    return s;
+}
 
 // ProjectBackward's vec grad (project_bwd above) of element i = 3 v + j of the positions.
 __device__ __forceinline__ float adam_proj_grad(const AdamArgs& a, long long i) {
@@ -807,16 +829,18 @@ __global__ void __launch_bounds__(BS) k_adam(AdamArgs a) {
     long long b = blockIdx.x;
     int t = 0;
     while (t + 1 < a.n && b >= a.block_start[t + 1]) t++;
-    long long base = (b - a.block_start[t]) * ADAM_PER_BLOCK;
+    const int ept = a.per_thread[t];
+    long long base = (b - a.block_start[t]) * BS * ept;
     float* P = a.param[t]; const float* G = a.grad[t]; float* M = a.m[t]; float* Vv = a.v[t];
     long long n = a.size[t];
     const double rmt = a.bt ? a.bt[0] : a.rhomt[t], rvt = a.bt ? a.bt[1] : a.rhovt[t];
-    for (int k = 0; k < 4; k++) {
+    for (int k = 0; k < ept; k++) {
         long long i = base + k * BS + threadIdx.x;
         if (i >= n) break;
         float p = P[i], m = M[i], v = Vv[i];
         float g = G[i];
         if (t == 0 && a.pgclip) g += adam_proj_grad(a, i);
+        if (t >= 1 && t <= 3 && a.lseg) add_nan_check(g, adam_lobe_sum(a, t, i));
         adam_elem(p, g, m, v, a.rhom[t], a.rhov[t], rmt, rvt, a.eta[t], a.eps[t]);
         P[i] = p; M[i] = m; Vv[i] = v;
         if (a.zero_grad) const_cast<float*>(G)[i] = 0.f;
@@ -1200,6 +1224,7 @@ __global__ void __launch_bounds__(BS) k_sg_fwd(size_t P, int C, int K, const flo
 // one row of partials; rows are summed in a fixed order afterwards.
 constexpr int SG_NPIX = 2;
 constexpr int SG_COLS = 3 + 1 + SG_MAXC;  // axis, sharpness, amplitude
+static_assert(SG_COLS == ADAM_SG_COLS, "the Adam launch's lobe-sum columns are the SG backward's");
 
 __global__ void __launch_bounds__(BS) k_sg_bwd(size_t P, int C, int K, const float* rast, const float* __restrict__ axis,
                                                const float* __restrict__ sharp, const float* __restrict__ amp, const float* normal,
@@ -1310,6 +1335,7 @@ __global__ void __launch_bounds__(256) k_sg_lobe_final(const float* seg, int nse
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= K * SG_COLS) return;
     float s = 0.f;
+#pragma unroll 8
     for (int r = 0; r < nseg; r++) s += seg[(size_t)r * K * SG_COLS + c];
     const int i = c / SG_COLS, k = c % SG_COLS;
     if (k < 3) { if (gaxis) add_nan_check(gaxis[3 * i + k], s); }
@@ -2211,12 +2237,18 @@ extern "C" int vcl_texture_grad_clear(const vcl_texture* g, vcl_stream_t stream)
 // bt_dev: device {rhomt, rhovt} used instead of the per-tensor host values (null: host values).
 extern "C" int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev,
                                        int zero_grad, vcl_stream_t stream) {
-    return vcl_adam_step_multi_proj_dev(n, p, h, bt_dev, zero_grad, nullptr, nullptr, stream);
+    return vcl_adam_step_multi_proj_dev(n, p, h, bt_dev, zero_grad, nullptr, nullptr, nullptr, stream);
 }
 
 int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
-                                 const vcl_project_params* proj, const vcl_loss_params* loss, vcl_stream_t stream) {
+                                 const vcl_project_params* proj, const vcl_loss_params* loss, const vcl_sg_lobe_sums* lobes,
+                                 vcl_stream_t stream) {
     if (loss) VCL_REQUIRE(loss->buffer && loss->loss && loss->size >= 0 && n <= ADAM_MAX);
+    if (lobes) {
+        // tensors 1..3: the lobe axis (K x 3), sharpness (K) and amplitude (K x C) grads
+        VCL_REQUIRE(lobes->seg && lobes->nseg >= 0 && lobes->C >= 1 && lobes->C <= 4 && n >= 4 && n <= ADAM_MAX && p[1].size == 3LL * lobes->K &&
+                    p[2].size == (long long)lobes->K && p[3].size == (long long)lobes->K * lobes->C);
+    }
     VCL_REQUIRE(n >= 0 && (n == 0 || (p && h)));
     if (proj) {
         // tensor 0 must be the projected positions, 3 floats per vertex, with 4-float clip grads
@@ -2234,6 +2266,10 @@ int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_
         a.loss = base == 0 && loss ? loss->loss : nullptr;
         a.loss_part = loss ? loss->buffer : nullptr;
         a.loss_nb = loss && loss->size > 0 ? cdiv((size_t)loss->size, (size_t)BS * 4) : 0;
+        a.lseg = base == 0 && lobes ? lobes->seg : nullptr;
+        a.lnseg = lobes ? lobes->nseg : 0;
+        a.lK = lobes ? lobes->K : 0;
+        a.lC = lobes ? lobes->C : 1;
         a.n = n - base < ADAM_MAX ? n - base : ADAM_MAX;
         long long blocks = 0;
         for (int i = 0; i < a.n; i++) {
@@ -2243,7 +2279,8 @@ int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_
             a.param[i] = q.param; a.grad[i] = q.grad; a.m[i] = hh.m; a.v[i] = hh.v; a.size[i] = q.size;
             a.rhom[i] = hh.rhom; a.rhov[i] = hh.rhov; a.rhomt[i] = hh.rhomt; a.rhovt[i] = hh.rhovt; a.eta[i] = hh.eta; a.eps[i] = hh.eps;
             a.block_start[i] = blocks;
-            blocks += (q.size + ADAM_PER_BLOCK - 1) / ADAM_PER_BLOCK;
+            a.per_thread[i] = base == 0 && lobes && i >= 1 && i <= 3 ? 1 : ADAM_PER_BLOCK / BS;
+            blocks += (q.size + (long long)BS * a.per_thread[i] - 1) / ((long long)BS * a.per_thread[i]);
         }
         a.block_start[a.n] = blocks;
         if (blocks > 0 || a.loss) {
@@ -2470,8 +2507,9 @@ extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_param
     return VCL_OK;
 }
 
-extern "C" int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
-                                               vcl_stream_t stream) {
+int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
+                                        vcl_sg_lobe_sums* lobes, vcl_stream_t stream) {
+    if (lobes) lobes->seg = nullptr;
     VCL_REQUIRE(p && g && p->rast && p->normal && p->diffuse && p->specAxis && p->specSharpness && p->specAmplitude && g->out &&
                 p->channel >= 1 && p->channel <= SG_MAXC && p->sgnum >= 0);
     size_t P = (size_t)p->width * p->height * p->depth;
@@ -2502,9 +2540,19 @@ extern "C" int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_para
     float* seg = part + (size_t)nb * cols;
     const int nseg = colsum_stage(part, (int)nb, (int)cols, seg, s);
     if (nseg < 0) return VCL_ERR_HIP;
+    if (lobes && g->axis && g->sharpness && g->amplitude) {
+        // the caller's Adam launch adds the sums into the lobe grads
+        lobes->seg = seg; lobes->nseg = nseg; lobes->K = K; lobes->C = C;
+        return VCL_OK;
+    }
     hipLaunchKernelGGL(k_sg_lobe_final, dim3(cdiv(cols, 256)), dim3(256), 0, s, seg, nseg, K, C, g->axis, g->sharpness, g->amplitude);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
+}
+
+extern "C" int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
+                                               vcl_stream_t stream) {
+    return vcl_spherical_gaussian_backward_dev(p, g, nullptr, stream);
 }
 
 // gout: the backward's output grads (16-byte alignment checked with the input's), or null
