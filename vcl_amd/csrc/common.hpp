@@ -190,6 +190,30 @@ __device__ __forceinline__ void lds_vertex_grad_flush(const LdsTable<SLOTS, 3, u
     }
 }
 
+// The rasterizer's per-launch workspace reset as a list of ranges (tile counters, large-piece
+// sub-counters and misc[] to zero, per-pixel depth keys to all ones), so that another pass can
+// do it: element i of ws_clear_elem covers index i of every range; n() elements in all.
+struct WsClear {
+    u32* counts;
+    size_t nt;
+    u32* lcounts;
+    size_t nl;
+    u32* misc;   // 64 entries
+    unsigned long long* keys;
+    size_t np;
+    __host__ __device__ size_t n() const {
+        const size_t a = nt > nl ? nt : nl, b = np > 64 ? np : 64;
+        return counts ? (a > b ? a : b) : 0;
+    }
+};
+
+__device__ __forceinline__ void ws_clear_elem(const WsClear& c, size_t i) {
+    if (i < c.nt) c.counts[i] = 0;
+    if (i < c.nl) c.lcounts[i] = 0;
+    if (i < 64) c.misc[i] = 0;
+    if (i < c.np) c.keys[i] = ~0ull;
+}
+
 // Zero-fill kernel (used instead of hipMemsetAsync so sequences stay graph-safe).
 __global__ void vcl_fill_u32(u32* p, size_t n, u32 v);
 int vcl_fill_async(void* p, size_t bytes, hipStream_t s);

@@ -305,10 +305,6 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     const vcl_fit_desc& d = F->d;
     int W = d.width, H = d.height, nv = d.views, C = d.channel;
     int st;
-    vcl_project_params pj = {d.V, 4, F->f("pos"), F->f("mats"), F->f("clip"), nv};
-    { ProfScope ps_(F, "project", as_hip(s));
-      st = step ? vcl_project_forward_dev(&pj, F->f("gclip"), F->t<double>("adam_bt"), d.beta1, d.beta2, s) : vcl_project_forward(&pj, s); }
-    if (st) return st;
     vcl_rasterize_params rp;
     rp.width = W; rp.height = H; rp.depth = nv; rp.enableDB = 1;
     rp.xs = 2.f / W; rp.ys = 2.f / H;
@@ -316,7 +312,14 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     rp.out = F->f("rast"); rp.outDB = F->f("db");
     rp.projNum = d.V; rp.idxNum = d.T;
     rp.workspace = F->buf["ws_rast"]; rp.workspace_size = F->ws_rast;
-    { ProfScope ps_(F, "rasterize", as_hip(s)); st = vcl_rasterize_forward(&rp, s); }
+    // in a step, the projection pass also does the rasterizer's workspace reset
+    WsClear rc;
+    if (step && (st = vcl_rasterize_clear_spec(&rp, &rc))) return st;
+    vcl_project_params pj = {d.V, 4, F->f("pos"), F->f("mats"), F->f("clip"), nv};
+    { ProfScope ps_(F, "project", as_hip(s));
+      st = step ? vcl_project_forward_dev(&pj, F->f("gclip"), F->t<double>("adam_bt"), d.beta1, d.beta2, &rc, s) : vcl_project_forward(&pj, s); }
+    if (st) return st;
+    { ProfScope ps_(F, "rasterize", as_hip(s)); st = vcl_rasterize_forward_dev(&rp, step ? 1 : 0, s); }
     if (st) return st;
     vcl_normal_axis_params np;
     std::memset(&np, 0, sizeof(np));

@@ -16,9 +16,15 @@ int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, f
 // tp->uv and tp->uvDA are ip's outputs.
 // zero (may be null): a 4-float-per-pixel buffer cleared in the same pass.
 int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, float* zero, vcl_stream_t stream);
-// vcl_project_forward that also clears zero (shaped like p->out, may be null) and, when bt is not
-// null, applies the step-start bump bt[0] *= rhom, bt[1] *= rhov once.
-int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt, double rhom, double rhov, vcl_stream_t stream);
+// vcl_project_forward that also clears zero (shaped like p->out, may be null), when bt is not
+// null applies the step-start bump bt[0] *= rhom, bt[1] *= rhov once, and, when rc is not null,
+// does the rasterizer's workspace reset that vcl_rasterize_clear_spec describes.
+int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt, double rhom, double rhov, const WsClear* rc,
+                            vcl_stream_t stream);
+// The workspace reset of the next vcl_rasterize_forward(_dev) with these params (same tuning).
+int vcl_rasterize_clear_spec(const vcl_rasterize_params* p, WsClear* c);
+// vcl_rasterize_forward; cleared: the workspace reset was already done (vcl_rasterize_clear_spec).
+int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_stream_t stream);
 int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp, vcl_stream_t stream);
 // Texture backward then interpolate backward (the latter's output grad being the former's uv
 // grad): fused into one kernel when the tiled C == 4 texture backward applies, a 2-channel

@@ -5,6 +5,7 @@
 // fixed-order block partials so results are deterministic.
 #include "internal.hpp"
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 
 __global__ void vcl_fill_u32(u32* p, size_t n, u32 v) {
@@ -119,10 +120,11 @@ int colsum_stage(const float* part, int rows, int cols, float* seg_scratch, hipS
 // zero (optional): a buffer shaped like out cleared in the same pass (the step's clip-space grad
 // clear); bt (optional): the step-start bump of Adam's bias-correction powers, done once.
 __global__ void __launch_bounds__(BS) k_project_fwd(int V, int dim, int views, const float* vec, const float* mat, float* out, float* zero,
-                                                    double* bt, double rhom, double rhov) {
+                                                    double* bt, double rhom, double rhov, WsClear rc) {
 #pragma clang fp contract(off)
     size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (bt && gid == 0) { bt[0] *= rhom; bt[1] *= rhov; }
+    if (rc.counts && gid < rc.n()) ws_clear_elem(rc, gid);
     if (gid >= (size_t)V * views) return;
     int s = (int)(gid / V), v = (int)(gid % V);
     const float* m = mat + 16 * s;
@@ -2034,19 +2036,24 @@ extern "C" const char* vcl_status_string(int s) {
 
 extern "C" const char* vcl_build_info(void) { return "vcl_amd 0.1 gfx950"; }
 
-int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt, double rhom, double rhov, vcl_stream_t stream) {
+int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt, double rhom, double rhov, const WsClear* rc,
+                            vcl_stream_t stream) {
     VCL_REQUIRE(p && p->vec && p->mat && p->out && p->vecNum >= 0 && (p->dimention == 3 || p->dimention == 4));
     int views = p->views < 1 ? 1 : p->views;
     size_t n = (size_t)p->vecNum * views;
-    if (!n && !bt) return VCL_OK;
-    hipLaunchKernelGGL(k_project_fwd, dim3(n ? cdiv(n, BS) : 1), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat, p->out,
-                       zero, bt, rhom, rhov);
+    WsClear c;
+    std::memset(&c, 0, sizeof(c));
+    if (rc) c = *rc;
+    const size_t total = std::max(n, c.n());
+    if (!total && !bt) return VCL_OK;
+    hipLaunchKernelGGL(k_project_fwd, dim3(total ? cdiv(total, BS) : 1), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat,
+                       p->out, zero, bt, rhom, rhov, c);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
 
 extern "C" int vcl_project_forward(const vcl_project_params* p, vcl_stream_t stream) {
-    return vcl_project_forward_dev(p, nullptr, nullptr, 1.0, 1.0, stream);
+    return vcl_project_forward_dev(p, nullptr, nullptr, 1.0, 1.0, nullptr, stream);
 }
 
 extern "C" int vcl_project_backward(const vcl_project_params* p, const vcl_project_grad_params* g, vcl_stream_t stream) {

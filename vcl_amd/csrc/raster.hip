@@ -380,15 +380,18 @@ __device__ __forceinline__ void bin_or_defer(const RastArgs& a, SetupShared& sh,
 
 // Per-launch workspace reset: tile counters, large-piece sub-counters and misc[] to zero, the
 // per-pixel depth keys to all-ones (no winner); grid-stride over the largest.
-__global__ void __launch_bounds__(256) k_clear(Ws w, int NT, size_t P) {
-    const size_t nl = (size_t)LSUB * NT;
-    const size_t n = max(max(nl, P), (size_t)64);
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        if (i < (size_t)NT) w.counts[i] = 0;
-        if (i < nl) w.lcounts[i] = 0;
-        if (i < 64) w.misc[i] = 0;
-        if (i < P) w.keys[i] = ~0ull;
-    }
+__global__ void __launch_bounds__(256) k_clear(WsClear c) {
+    const size_t n = c.n();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) ws_clear_elem(c, i);
+}
+
+WsClear ws_clear_of(const Ws& w, const Layout& L, size_t P) {
+    WsClear c;
+    c.counts = w.counts; c.nt = L.NT;
+    c.lcounts = w.lcounts; c.nl = (size_t)LSUB * L.NT;
+    c.misc = w.misc;
+    c.keys = (unsigned long long*)w.keys; c.np = P;
+    return c;
 }
 
 // Adds each tile's large-piece sub-counts to its count, before the scan, and sets each
@@ -983,7 +986,7 @@ extern "C" size_t vcl_rasterize_workspace_size(int width, int height, int depth,
     return m;
 }
 
-extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t stream) {
+int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_stream_t stream) {
     VCL_REQUIRE(p && p->out && p->workspace && (p->idxNum == 0 || (p->proj && p->idx)));
     VCL_REQUIRE(p->width > 0 && p->height > 0 && p->depth > 0 && p->width <= 16384 && p->height <= 16384);
     VCL_REQUIRE(p->idxNum >= 0 && p->idxNum < (1 << 24) && (p->projNum > 0 || p->idxNum == 0) && p->depth < (1 << 23));
@@ -1001,11 +1004,11 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     a.nrec_direct = L.nrec_direct; a.ovf_cap = L.ovf_cap; a.bin_cap = L.bin_cap; a.chunk_cap = L.chunk_cap;
     a.ws = ws_ptrs(p->workspace, L);
     const size_t P = (size_t)a.W * a.H * a.depth;
-    {
-        const size_t nclear = std::max(std::max((size_t)LSUB * L.NT, P), (size_t)64);
-        hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<size_t>(cdiv(nclear, 256), 4096)), dim3(256), 0, s, a.ws, L.NT, P);
+    if (!cleared) {
+        const WsClear c = ws_clear_of(a.ws, L, P);
+        hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<size_t>(cdiv(c.n(), 256), 4096)), dim3(256), 0, s, c);
+        VCL_LAUNCH_CHECK();
     }
-    VCL_LAUNCH_CHECK();
     if (L.nrec_direct > 0) {
         hipLaunchKernelGGL(k_setup, dim3(cdiv(L.nrec_direct, 256)), dim3(256), 0, s, a);
         VCL_LAUNCH_CHECK();
@@ -1029,6 +1032,17 @@ extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t
     VCL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_resolve, dim3(cdiv(P, 256)), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+extern "C" int vcl_rasterize_forward(const vcl_rasterize_params* p, vcl_stream_t stream) { return vcl_rasterize_forward_dev(p, 0, stream); }
+
+int vcl_rasterize_clear_spec(const vcl_rasterize_params* p, WsClear* c) {
+    VCL_REQUIRE(p && c && p->workspace && p->width > 0 && p->height > 0 && p->depth > 0 && p->idxNum >= 0);
+    const int tshift = tile_shift_of(g_vcl_tuning.raster_tile);
+    Layout L = make_layout(p->width, p->height, p->depth, p->idxNum, tshift);
+    if (p->workspace_size < L.total) return VCL_ERR_OOM;
+    *c = ws_clear_of(ws_ptrs(p->workspace, L), L, (size_t)p->width * p->height * p->depth);
     return VCL_OK;
 }
 
