@@ -936,38 +936,53 @@ __global__ void __launch_bounds__(BS) k_tex_update4(TexUpdArgs a) {
     ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Lane S's v in every lane of its quad (lanes 4k .. 4k + 3; DPP quad_perm, no LDS).
+template <int S>
+__device__ __forceinline__ float quad_lane(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), S * 0x55, 0xf, 0xf, false));
+}
+
 // k_tex_update4 that also builds level 1 (L > 1, even level-0 sizes): one texel per lane as
-// there, with each wave on a 32 x 2 texel patch (a workgroup on 128 x 2), so the four texels of
-// a level-1 quad sit in lanes l, l ^ 1, l ^ 32, l ^ 33.  After the updates the quad's first lane
-// gathers the other three new values and writes their mean in k_downsample4's order, and clears
-// the level-1 grad (which all four lanes read for the folded grad before).
+// there, with each wave on a 32 x 2 texel patch (a workgroup on 128 x 2) and the four texels of
+// a level-1 quad in the lanes of one DPP quad (lane & 1: x offset, lane & 2: y offset).  The
+// coarse grads that fold into a texel are the same for the whole level-1 quad, so the quad's
+// lane c folds channel c only (one float per level instead of a float4 in every lane) and the
+// four results are broadcast in the quad.  After the updates the quad's first lane gathers the
+// other three new values and writes their mean in k_downsample4's order, and clears the level-1
+// grad (which the quad read for the fold before).
 __global__ void __launch_bounds__(BS) k_tex_update4m(TexUpdArgs a) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
-    const int x = blockIdx.x * 128 + (threadIdx.x >> 6) * 32 + (lane & 31);
-    const int y = blockIdx.y * 2 + (lane >> 5);
+    const int c = lane & 3;
+    const int x = blockIdx.x * 128 + (threadIdx.x >> 6) * 32 + 2 * (lane >> 2) + (lane & 1);
+    const int y = blockIdx.y * 2 + ((lane >> 1) & 1);
     const bool in = x < a.tw;   // a.tw and a.th are even: a quad is all in or all out
-    float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 p = z4, g = z4, m = z4, v = z4;
+    float acc = 0.f;
+    const size_t i = (size_t)y * a.tw + x;
     if (in) {
-        const size_t i = (size_t)y * a.tw + x;
-        float4 anc[VCL_TEX_MAX_MIP_LEVEL];
+        float anc[VCL_TEX_MAX_MIP_LEVEL];
 #pragma unroll
         for (int l = 1; l < VCL_TEX_MAX_MIP_LEVEL; l++)
-            if (l < a.L) anc[l] = ((const float4*)a.grad[l])[(size_t)(y >> l) * (a.tw >> l) + (x >> l)];
-        float4 g = ((const float4*)a.grad[0])[i];
-        float4 m = ((const float4*)a.m)[i], v = ((const float4*)a.v)[i];
+            if (l < a.L) anc[l] = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * 4 + c];
+        g = ((const float4*)a.grad[0])[i];
+        m = ((const float4*)a.m)[i];
+        v = ((const float4*)a.v)[i];
         p = ((const float4*)a.tex0)[i];
-        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int l = VCL_TEX_MAX_MIP_LEVEL - 1; l >= 1; l--) {
             if (l == a.L - 1) acc = anc[l];
             else if (l < a.L - 1) {
-                float4 t = anc[l];
-                add_nan_check4(t, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
+                float t = anc[l];
+                add_nan_check(t, acc * a.scale);
                 acc = t;
             }
         }
-        add_nan_check4(g, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
+    }
+    const float4 acc4 = make_float4(quad_lane<0>(acc), quad_lane<1>(acc), quad_lane<2>(acc), quad_lane<3>(acc));
+    if (in) {
+        add_nan_check4(g, make_float4(acc4.x * a.scale, acc4.y * a.scale, acc4.z * a.scale, acc4.w * a.scale));
         const double rmt = a.bt ? a.bt[0] : a.rhomt, rvt = a.bt ? a.bt[1] : a.rhovt;
         adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
         adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
@@ -982,9 +997,10 @@ __global__ void __launch_bounds__(BS) k_tex_update4m(TexUpdArgs a) {
         ((float4*)a.v)[i] = v;
         ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    auto xor4 = [](float4 t, int o) { return make_float4(__shfl_xor(t.x, o, 64), __shfl_xor(t.y, o, 64), __shfl_xor(t.z, o, 64), __shfl_xor(t.w, o, 64)); };
-    const float4 p01 = xor4(p, 1), p10 = xor4(p, 32), p11 = xor4(p, 33);
-    if (in && (lane & 33) == 0) {
+    const float4 p01 = make_float4(quad_lane<1>(p.x), quad_lane<1>(p.y), quad_lane<1>(p.z), quad_lane<1>(p.w));
+    const float4 p10 = make_float4(quad_lane<2>(p.x), quad_lane<2>(p.y), quad_lane<2>(p.z), quad_lane<2>(p.w));
+    const float4 p11 = make_float4(quad_lane<3>(p.x), quad_lane<3>(p.y), quad_lane<3>(p.z), quad_lane<3>(p.w));
+    if (in && c == 0) {
         const size_t j = (size_t)(y >> 1) * (a.tw >> 1) + (x >> 1);
         ((float4*)a.tex1)[j] = make_float4((((p.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p.y + p01.y) + p10.y) + p11.y) * 0.25f,
                                            (((p.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p.w + p01.w) + p10.w) + p11.w) * 0.25f);
