@@ -471,6 +471,37 @@ def test_spherical_gaussian_adversarial(gpu, cfg1):
             assert_close(np.nan_to_num(g), np.nan_to_num(r), 1e-5, f"sg {name} ({amp_case})")
 
 
+@pytest.mark.parametrize("K", [2, 3, 100, 200])
+def test_spherical_gaussian_lobe_waves(gpu, cfg1, K):
+    # more lobes than one wave of lobe lanes holds (two and four waves), sharpness over [0.5, 60]
+    # with non-unit axes, very sharp specular lobes at some pixels and very wide ones at others:
+    # the lobe-parallel backward stages the lobes by length and leaves out e^{-l-c} for a wave
+    # and pixel chunk where it is below half an ulp; its lobe grads must match the oracle and
+    # repeat exactly
+    import torch
+    sc, d = cfg1
+    rng = np.random.default_rng(K)
+    axis, sharp, amp = S.sg_lobes(K, sc.C, rng, 0.5, 60.0)
+    axis = (axis * rng.uniform(0.5, 1.5, (K, 1))).astype(F32)
+    ss = d["sSharp"].copy()
+    ss[..., ::7] = 1.0e5
+    ss[..., 3::11] = 0.5
+    args = [T(d["rast"]), T(axis), T(sharp), T(amp), T(d["nrm"]), T(d["diff"]), T(d["sAxis"]), T(ss), T(d["sAmp"])]
+    _, de, _ = gpu.spherical_gaussian(*args)
+    gout = rng.normal(size=d["color"].shape).astype(F32)
+    ref = o_sg_bwd(d["rast"], axis, sharp, amp, d["nrm"], d["diff"], d["sAxis"], ss, d["sAmp"], gout)
+    res = []
+    for _ in range(2):
+        gd = torch.zeros_like(args[5])
+        ga, gs, gm = torch.zeros(K, 3, device="cuda"), torch.zeros(K, device="cuda"), torch.zeros(K, sc.C, device="cuda")
+        gpu.spherical_gaussian_backward(*args, T(gout), gd, ga, gs, gm, denv=de)
+        res.append([N(x) for x in (gd, ga, gs, gm)])
+    for name, g, r in zip(["diffuse", "axis", "sharpness", "amplitude"], res[0], ref[:4]):
+        assert_close(g, r, 1e-3, f"sg lobe-wave grad {name} K={K}", atol_frac=1e-4)
+    for a_, b_ in zip(res[0], res[1]):
+        assert np.array_equal(a_, b_), "lobe grads must be deterministic"
+
+
 # ------------------------------------------------------------------ antialias / loss
 def test_antialias(gpu, cfg1):
     import torch

@@ -1613,6 +1613,83 @@ __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* ra
 // gout * diffenv (the forward's lobe-ordered sum) written in the staging pass.
 struct SgPix { float dnx, dny, dnz, ssL2E, snx, sny, snz, sa, X[4], G[4]; };  // ssL2E = spec sharpness * log2(e)
 
+// One lobe term of the backward: g = (e1 - e2) il and dg/dl = ((e1 + e2) - g) il, with e2 =
+// e^{-l-c} formed only when BOTH (otherwise e2 is below half an ulp of e1 and both sums equal e1).
+template <bool BOTH>
+__device__ __forceinline__ float sg_term_bwd(float wx, float wy, float wz, float c, float& il, float& dgdl) {
+    float d = wx * wx + wy * wy + wz * wz;
+    il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
+    float l = d * il;
+    float e1 = nexp2(fmaf(l, L2E, -c));
+    if (BOTH) {
+        float e2 = nexp2(fmaf(-l, L2E, -c));
+        float g = (e1 - e2) * il;
+        dgdl = ((e1 + e2) - g) * il;
+        return g;
+    }
+    float g = e1 * il;
+    dgdl = (e1 - g) * il;
+    return g;
+}
+
+// The pixel loop of one lobe lane (see k_sg_bwd_lobes); DB / SB: the diffuse / specular e2 is
+// needed by some pixel of the chunk for some lobe of the wave.
+template <bool DB, bool SB>
+__device__ __forceinline__ void sg_bwd_lobe_loop(const SgLobe& L, const SgPix* pix, int j0, int ncov, int G, float accW[3], float& accS,
+                                                 float accX[4], float accG[4]) {
+    const float cD = L.lamL2E + SG_DS * L2E;
+    for (int j = j0; j < ncov; j += G) {
+        const SgPix& x = pix[j];
+        float sdiff = 0.f, ssum = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) { sdiff += x.X[k] * L.a[k]; ssum += x.G[k] * L.a[k]; }
+        const float A = sdiff * SG_DA, B = ssum * x.sa;
+        float wx = x.dnx + L.gx, wy = x.dny + L.gy, wz = x.dnz + L.gz;
+        float il, dgdl;
+        float dgv = sg_term_bwd<DB>(wx, wy, wz, cD, il, dgdl);
+        float qx = x.snx + L.gx, qy = x.sny + L.gy, qz = x.snz + L.gz;
+        float isl, sgdl;
+        float sgv = sg_term_bwd<SB>(qx, qy, qz, L.lamL2E + x.ssL2E, isl, sgdl);
+        const float sgvsa = sgv * x.sa;
+#pragma unroll
+        for (int k = 0; k < 4; k++) { accX[k] += x.X[k] * dgv; accG[k] += x.G[k] * sgvsa; }
+        float cd = A * dgdl * il, cs = B * sgdl * isl;
+        accW[0] += cd * wx + cs * qx;
+        accW[1] += cd * wy + cs * qy;
+        accW[2] += cd * wz + cs * qz;
+        accS -= A * dgv + B * sgv;
+    }
+}
+
+// One pixel's inputs to the lobe backward (channels beyond C are 0; gd: the current diffuse grad,
+// de: diffenv, loaded when the diffuse grad is wanted).
+struct SgRaw { float go[4], df[4], de[4], gd[4], nx, ny, nz, ss, sa, ax, ay, az; bool cov; };
+
+__device__ __forceinline__ void sg_load_raw(SgRaw& r, size_t p, bool cov, int C, const float* gout, const float* diffuse, const float* denv,
+                                            const float* gdiffuse, const float* normal, const float* sAxis, const float* sSharp,
+                                            const float* sAmp) {
+    r.cov = cov;
+    if (!cov) return;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        r.go[k] = k < C ? gout[p * C + k] : 0.f;
+        r.df[k] = k < C ? diffuse[p * C + k] : 0.f;
+        r.de[k] = k < C && gdiffuse ? denv[p * C + k] : 0.f;
+        r.gd[k] = k < C && gdiffuse ? gdiffuse[p * C + k] : 0.f;
+    }
+    r.nx = normal[3 * p]; r.ny = normal[3 * p + 1]; r.nz = normal[3 * p + 2];
+    r.ss = sSharp[p]; r.sa = sAmp[p];
+    r.ax = sAxis[3 * p]; r.ay = sAxis[3 * p + 1]; r.az = sAxis[3 * p + 2];
+}
+
+// Does a pixel-side vector of length w need e^{-l-c} for some lobe of length in [lo, hi]?
+// l = |w_vec + lobe| >= |len - w|, and e2 drops when l > SG_E2_DROP; the margin covers the
+// float rounding of l.  NaN lengths answer yes.
+__device__ __forceinline__ bool sg_needs_e2(float w, float lo, float hi) {
+    const float T = SG_E2_DROP + 0.01f + 1e-5f * (hi + w);
+    return !(w + T < lo) && !(w - T > hi);
+}
+
 __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int KP, const float* rast, const float* axis, const float* sharp,
                                                      const float* amp, const float* normal, const float* diffuse, const float* sAxis,
                                                      const float* sSharp, const float* sAmp, const float* gout, const float* denv, float* gdiffuse,
@@ -1620,37 +1697,110 @@ __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int
     __shared__ SgLobe lobes[SG_MAXK];
     __shared__ SgPix pix[BS];
     __shared__ int wcount[BS / 64];
+    __shared__ u32 wneed[BS / 64];
     __shared__ float red[BS][SG_COLS];
-    sg_stage_lobes(lobes, K, C, axis, sharp, amp);
+    __shared__ unsigned long long skey[SG_MAXK];
+    __shared__ int lorig[SG_MAXK];
+    __shared__ float2 lrange[BS / 64];
     int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    int li = tid % KP, grp = tid / KP, G = BS / KP;
+    // Lane layout: with KP >= 4, wave w holds the KP / 4 lobes [w KP / 4, (w + 1) KP / 4) and
+    // 256 / KP pixel groups (lane = group * KP / 4 + lobe offset), so a wave's lobes are a quarter
+    // of the staged order; with KP < 4 every wave holds all lobes.  Either way a lobe's pixels
+    // go to groups 0 .. G - 1 (G = BS / KP) in the same order.
+    const int LPW = KP >= 4 ? KP / 4 : KP;
+    int li = KP >= 4 ? wid * LPW + lane % LPW : tid % KP, grp = KP >= 4 ? lane / LPW : tid / KP, G = BS / KP;
+    // Lobes are staged in order of |axis * sharpness| (ties by index), so that a wave's 64 lobe
+    // lanes hold similar lengths and the e2 test below can pass for a whole wave; the per-lobe
+    // sums do not depend on which lane holds a lobe.
+    for (int i = tid; i < K; i += BS) {
+        const float lam = sharp[i];
+        const float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
+        const float len = sqrtf(gx * gx + gy * gy + gz * gz);
+        const u32 kb = isnan(len) ? 0xffffffffu : __float_as_uint(len);
+        skey[i] = ((unsigned long long)kb << 32) | (u32)i;
+    }
+    __syncthreads();
+    for (int i = tid; i < K; i += BS) {
+        const unsigned long long ki = skey[i];
+        int r = 0;
+        for (int j = 0; j < K; j++) r += skey[j] < ki;
+        const float lam = sharp[i];
+        SgLobe l;
+        l.mx = axis[3 * i]; l.my = axis[3 * i + 1]; l.mz = axis[3 * i + 2];
+        l.gx = l.mx * lam; l.gy = l.my * lam; l.gz = l.mz * lam; l.lam = lam; l.lamL2E = lam * L2E;
+        for (int k = 0; k < 4; k++) l.a[k] = k < C ? amp[i * C + k] : 0.f;
+        lobes[r] = l;
+        lorig[r] = i;
+    }
+    __syncthreads();
+    // lobe-length range of each wave's lobes (one range, lw = 0, when KP < 4)
+    const int nlw = KP >= 4 ? BS / 64 : 1, lw = KP >= 4 ? wid : 0;
+    {
+        float len = __builtin_nanf("");
+        if (li < K) { const SgLobe& L = lobes[li]; len = sqrtf(L.gx * L.gx + L.gy * L.gy + L.gz * L.gz); }
+        const float lo = wave_fmin(len), hi = wave_fmax(len);
+        if (lane == 0 && wid < nlw) lrange[lw] = make_float2(lo, hi);
+    }
+    __syncthreads();
     float acc[SG_COLS];
 #pragma unroll
     for (int k = 0; k < SG_COLS; k++) acc[k] = 0.f;
     float accW[3] = {0.f, 0.f, 0.f}, accS = 0.f, accX[4] = {0.f, 0.f, 0.f, 0.f}, accG[4] = {0.f, 0.f, 0.f, 0.f};
-    for (size_t base = (size_t)blockIdx.x * BS; base < P; base += (size_t)gridDim.x * BS) {
-        size_t p = base + tid;
-        bool cov = p < P && (int)rast[4 * p + 3] - 1 >= 0;
-        SgPix q;
-        if (cov) {
-            float go[4] = {0.f, 0.f, 0.f, 0.f}, df[4] = {0.f, 0.f, 0.f, 0.f};
-            for (int k = 0; k < C; k++) { go[k] = gout[p * C + k]; df[k] = diffuse[p * C + k]; }
-            if (gdiffuse)
-                for (int k = 0; k < C; k++) add_nan_check(gdiffuse[p * C + k], go[k] * denv[p * C + k]);
-            q.dnx = normal[3 * p] * SG_DS; q.dny = normal[3 * p + 1] * SG_DS; q.dnz = normal[3 * p + 2] * SG_DS;
-            float ss = sSharp[p];
-            q.ssL2E = ss * L2E; q.sa = sAmp[p];
-            q.snx = sAxis[3 * p] * ss; q.sny = sAxis[3 * p + 1] * ss; q.snz = sAxis[3 * p + 2] * ss;
-            for (int k = 0; k < 4; k++) { q.X[k] = go[k] * df[k]; q.G[k] = go[k]; }
+    // The pixel inputs of the next chunk are loaded while this chunk's lobe loop runs (and the
+    // coverage word of the chunk after that), so the chunks' loads overlap the lobe work instead
+    // of every workgroup waiting on them between chunks.
+    const size_t stride = (size_t)gridDim.x * BS;
+    SgRaw cur, nxt;
+    float rwn;
+    {
+        const size_t p0 = (size_t)blockIdx.x * BS + tid, p1 = p0 + stride;
+        sg_load_raw(cur, p0, p0 < P && (int)rast[4 * p0 + 3] - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp);
+        rwn = p1 < P ? rast[4 * p1 + 3] : 0.f;
+    }
+    for (size_t base = (size_t)blockIdx.x * BS; base < P; base += stride) {
+        const size_t p = base + tid;
+        const bool cov = cur.cov;
+        {
+            const size_t pn = p + stride, p2 = pn + stride;
+            sg_load_raw(nxt, pn, pn < P && (int)rwn - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp);
+            rwn = p2 < P ? rast[4 * p2 + 3] : 0.f;
         }
-        if (!want_lobes) continue;
+        SgPix q;
+        u32 need = 0;   // bit 2 lw: diffuse e2 for lobe wave lw, bit 2 lw + 1: specular
+        if (cov) {
+            if (gdiffuse)
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (k < C) {
+                        float t = cur.gd[k];
+                        add_nan_check(t, cur.go[k] * cur.de[k]);
+                        gdiffuse[p * C + k] = t;
+                    }
+            q.dnx = cur.nx * SG_DS; q.dny = cur.ny * SG_DS; q.dnz = cur.nz * SG_DS;
+            const float ss = cur.ss;
+            q.ssL2E = ss * L2E; q.sa = cur.sa;
+            q.snx = cur.ax * ss; q.sny = cur.ay * ss; q.snz = cur.az * ss;
+#pragma unroll
+            for (int k = 0; k < 4; k++) { q.X[k] = cur.go[k] * cur.df[k]; q.G[k] = cur.go[k]; }
+            const float dlen = sqrtf(q.dnx * q.dnx + q.dny * q.dny + q.dnz * q.dnz);
+            const float slen = sqrtf(q.snx * q.snx + q.sny * q.sny + q.snz * q.snz);
+            for (int w = 0; w < nlw; w++) {
+                const float2 r = lrange[w];
+                need |= (sg_needs_e2(dlen, r.x, r.y) ? 1u : 0u) << (2 * w);
+                need |= (sg_needs_e2(slen, r.x, r.y) ? 2u : 0u) << (2 * w);
+            }
+        }
+        if (!want_lobes) { cur = nxt; continue; }
         // deterministic compaction: wave ballot + prefix over waves
         unsigned long long m = __ballot(cov);
         int wpos = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) wcount[wid] = __popcll(m);
+        u32 wn = 0;
+        for (int b = 0; b < 2 * nlw; b++) wn |= __ballot((need >> b) & 1u) ? 1u << b : 0u;
+        if (lane == 0) { wcount[wid] = __popcll(m); wneed[wid] = wn; }
         __syncthreads();
         int off = 0, ncov = 0;
-        for (int w = 0; w < BS / 64; w++) { if (w < wid) off += wcount[w]; ncov += wcount[w]; }
+        u32 cneed = 0;
+        for (int w = 0; w < BS / 64; w++) { if (w < wid) off += wcount[w]; ncov += wcount[w]; cneed |= wneed[w]; }
         if (cov) pix[off + wpos] = q;
         __syncthreads();
         if (li < K) {
@@ -1660,34 +1810,16 @@ __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int
             //   d/d(sharpness) = mu . sum(cd w + cs q) - sum(A dg + B sg)
             //   d/d(amplitude) = DA * sum(X dg) + sum(G sg sa)
             // (mu . w enters the sharpness term only through cd mu . w), and dg/dl =
-            // il ((e1 + e2) - g) for g = (e1 - e2) il.
+            // il ((e1 + e2) - g) for g = (e1 - e2) il.  The e2 choice is uniform per wave.
             const SgLobe& L = lobes[li];
-            const float cD = L.lamL2E + SG_DS * L2E;
-            for (int j = grp; j < ncov; j += G) {
-                const SgPix& x = pix[j];
-                float sdiff = 0.f, ssum = 0.f;
-#pragma unroll
-                for (int k = 0; k < 4; k++) { sdiff += x.X[k] * L.a[k]; ssum += x.G[k] * L.a[k]; }
-                const float A = sdiff * SG_DA, B = ssum * x.sa;
-                float wx = x.dnx + L.gx, wy = x.dny + L.gy, wz = x.dnz + L.gz;
-                float il, e1, e2;
-                float dgv = sg_term_c(wx, wy, wz, cD, il, e1, e2);
-                float dgdl = ((e1 + e2) - dgv) * il;
-                float qx = x.snx + L.gx, qy = x.sny + L.gy, qz = x.snz + L.gz;
-                float isl, f1, f2;
-                float sgv = sg_term_c(qx, qy, qz, L.lamL2E + x.ssL2E, isl, f1, f2);
-                float sgdl = ((f1 + f2) - sgv) * isl;
-                const float sgvsa = sgv * x.sa;
-#pragma unroll
-                for (int k = 0; k < 4; k++) { accX[k] += x.X[k] * dgv; accG[k] += x.G[k] * sgvsa; }
-                float cd = A * dgdl * il, cs = B * sgdl * isl;
-                accW[0] += cd * wx + cs * qx;
-                accW[1] += cd * wy + cs * qy;
-                accW[2] += cd * wz + cs * qz;
-                accS -= A * dgv + B * sgv;
-            }
+            const bool db = (cneed >> (2 * lw)) & 1u, sb = (cneed >> (2 * lw + 1)) & 1u;
+            if (db && sb) sg_bwd_lobe_loop<true, true>(L, pix, grp, ncov, G, accW, accS, accX, accG);
+            else if (db) sg_bwd_lobe_loop<true, false>(L, pix, grp, ncov, G, accW, accS, accX, accG);
+            else if (sb) sg_bwd_lobe_loop<false, true>(L, pix, grp, ncov, G, accW, accS, accX, accG);
+            else sg_bwd_lobe_loop<false, false>(L, pix, grp, ncov, G, accW, accS, accX, accG);
         }
         __syncthreads();
+        cur = nxt;
     }
     if (want_lobes && li < K) {
         const SgLobe& L = lobes[li];
@@ -1703,10 +1835,10 @@ __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int
     for (int k = 0; k < SG_COLS; k++) red[tid][k] = acc[k];
     __syncthreads();
     for (int i = tid; i < K * SG_COLS; i += BS) {
-        int lobe = i / SG_COLS, col = i % SG_COLS;
+        int s = i / SG_COLS, col = i % SG_COLS;   // s: the lobe's staged (sorted) position
         float sum = 0.f;
-        for (int g2 = 0; g2 < G; g2++) sum += red[g2 * KP + lobe][col];
-        part[(size_t)blockIdx.x * K * SG_COLS + i] = col < 4 + C ? sum : 0.f;
+        for (int g2 = 0; g2 < G; g2++) sum += red[KP >= 4 ? (s / LPW) * 64 + g2 * LPW + s % LPW : g2 * KP + s][col];
+        part[(size_t)blockIdx.x * K * SG_COLS + (size_t)lorig[s] * SG_COLS + col] = col < 4 + C ? sum : 0.f;
     }
 }
 
