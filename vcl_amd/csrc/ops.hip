@@ -1537,8 +1537,11 @@ __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float
     }
 }
 
-constexpr int SG_FWD_PIX = 4 * BS;   // pixels per k_sg_fwd4 workgroup
-
+// Pixels per k_sg_fwd4 workgroup: 1024, or 512 when that would leave fewer than 4,096
+// workgroups.  Covered pixels are unevenly spread over the image, and with few workgroups per
+// CU (all resident at once) the CUs holding interior rows finish last; halving the workgroup
+// lets the dispatcher balance them.
+template <int SG_FWD_PIX>
 __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
                                                 const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
                                                 const float* sAmp, float* out, float* denv, float* senv) {
@@ -2624,10 +2627,13 @@ extern "C" int vcl_sg_specular_backward(const vcl_sg_specular_params* p, const v
     return VCL_OK;
 }
 
+// Workgroups of the lobe backward: one per two 256-pixel chunks, at most 2,048 (twice what can
+// be resident), so that workgroups whose chunks are mostly covered do not set the end of the
+// launch while other CUs idle; each adds a row of K x 8 partial sums to the column sum.
 static unsigned sg_bwd_blocks(size_t P) {
     size_t sweep = (size_t)BS * SG_NPIX;
     size_t nb = (P + sweep - 1) / sweep;
-    if (nb > 1024) nb = 1024;
+    if (nb > 2048) nb = 2048;
     if (nb < 1) nb = 1;
     return (unsigned)nb;
 }
@@ -2647,9 +2653,16 @@ extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_param
     if (!P) return VCL_OK;
     const bool al16 = ((uintptr_t)p->diffuse % 16 == 0) && ((uintptr_t)p->out % 16 == 0) && ((uintptr_t)p->outDiffenv % 16 == 0) &&
                       ((uintptr_t)p->outSpecenv % 16 == 0);
-    if (p->sgnum <= SG_MAXK && p->channel == 4 && al16)
-        hipLaunchKernelGGL(k_sg_fwd4, dim3(cdiv(P, SG_FWD_PIX)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness, p->amplitude,
-                           p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv, p->outSpecenv);
+    if (p->sgnum <= SG_MAXK && p->channel == 4 && al16) {
+        if (cdiv(P, 4 * BS) >= 4096)
+            hipLaunchKernelGGL(k_sg_fwd4<4 * BS>, dim3(cdiv(P, 4 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
+                               p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
+                               p->outSpecenv);
+        else
+            hipLaunchKernelGGL(k_sg_fwd4<2 * BS>, dim3(cdiv(P, 2 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
+                               p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
+                               p->outSpecenv);
+    }
     else if (p->sgnum <= SG_MAXK)
         hipLaunchKernelGGL(k_sg_fwd_fast, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
                            p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
