@@ -2627,13 +2627,13 @@ extern "C" int vcl_sg_specular_backward(const vcl_sg_specular_params* p, const v
     return VCL_OK;
 }
 
-// Workgroups of the lobe backward: one per two 256-pixel chunks, at most 2,048 (twice what can
-// be resident), so that workgroups whose chunks are mostly covered do not set the end of the
-// launch while other CUs idle; each adds a row of K x 8 partial sums to the column sum.
+// Workgroups of the lobe backward: one per two 256-pixel chunks, at most 4,096 (four times what
+// can be resident), so that workgroups whose chunks are mostly covered do not set the end of
+// the launch while other CUs idle; each adds a row of K x 8 partial sums to the column sum.
 static unsigned sg_bwd_blocks(size_t P) {
     size_t sweep = (size_t)BS * SG_NPIX;
     size_t nb = (P + sweep - 1) / sweep;
-    if (nb > 2048) nb = 2048;
+    if (nb > 4096) nb = 4096;
     if (nb < 1) nb = 1;
     return (unsigned)nb;
 }
