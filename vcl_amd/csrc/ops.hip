@@ -1384,17 +1384,6 @@ __device__ __forceinline__ void sg_stage_lobes(SgLobe* L, int K, int C, const fl
     }
 }
 
-// Backward form of a lobe term with the exponent's constant part c = (base + lam) * log2(e)
-// given: one rsq gives 1/l and l, and each exponent is one fma.
-__device__ __forceinline__ float sg_term_c(float wx, float wy, float wz, float c, float& il, float& e1, float& e2) {
-    float d = wx * wx + wy * wy + wz * wz;
-    il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
-    float l = d * il;
-    e1 = nexp2(fmaf(l, L2E, -c));
-    e2 = nexp2(fmaf(-l, L2E, -c));
-    return (e1 - e2) * il;
-}
-
 // One lobe term: g = (e^{l-c} - e^{-l-c}) / max(l, 1e-6), plus (optionally) dg/dl.
 __device__ __forceinline__ float sg_term(float wx, float wy, float wz, float base, float lam, float& l, float& il, float& e1, float& e2) {
     l = nsqrt(wx * wx + wy * wy + wz * wz);
@@ -1476,11 +1465,9 @@ __device__ __forceinline__ float sg_term_rsq(float wx, float wy, float wz, float
     return (e1 - e2) * il;
 }
 
+// e^{-l-c} is below half an ulp of e^{l-c} once l > SG_E2_DROP (e^{-17.5} < 2^-25).
 constexpr float SG_E2_DROP = 8.75f;
 
-// The lobe loop of one pixel; CHECKED selects the per-channel AddNaNcheck form (a block whose
-// lobes hold a non-finite amplitude).  The choice is uniform per block, so the two forms are
-// separate loops rather than a branch inside one.
 // Which lobes of chunk [c0, c0 + 64) may need e^{-l-c} for some lane of the wave: lane j tests
 // lobe c0 + j against the wave's range [lmin, lmax] of the pixel-side length |w| (the per-lane
 // test |len - |w|| <= SG_E2_DROP + 0.01 + 1e-5 (len + |w|) implies len within that range widened
