@@ -335,15 +335,25 @@ def test_mip_build_fold_bitexact(gpu, orc):
     del torch
 
 
-def test_texture_adam_update_bitexact(gpu, orc):
+@pytest.mark.parametrize("tw,th,nonfinite", [(32, 32, False), (96, 64, True), (200, 66, False), (130, 34, True)])
+def test_texture_adam_update_bitexact(gpu, orc, tw, th, nonfinite):
+    # widths that leave partial 128-texel workgroups, 2 to 6 levels, and NaN / Inf grads in the
+    # coarse levels (the fold's AddNaNcheck per channel, which the level-1 quad splits over lanes)
     import torch
-    rng = np.random.default_rng(8)
-    tw, th, C = 32, 32, 4
+    rng = np.random.default_rng(8 + tw)
+    C = 4
     L = S.mip_levels(tw, th, 16)
     n = sum((tw >> l) * (th >> l) * C for l in range(L))
     n0 = tw * th * C
     tex = S.build_mips_np(rng.uniform(0, 1, (th, tw, C)).astype(F32), L)
     g = rng.normal(size=n).astype(F32) * 0.1
+    if nonfinite:
+        n1 = (tw >> 1) * (th >> 1) * C
+        g[n0 + 5] = np.nan
+        g[n0 + n1 // 2 + 2] = np.inf
+        if L > 2:
+            g[n0 + n1 + 3] = -np.inf
+        g[7] = np.nan
     m = rng.normal(size=n0).astype(F32) * 0.01
     v = rng.uniform(0, 0.01, n0).astype(F32)
     hyper = (0.9, 0.999, 0.9 ** 3, 0.999 ** 3, 1e-2, 1e-8)
