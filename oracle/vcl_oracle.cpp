@@ -79,7 +79,7 @@ static inline double dmax(double a, double b) { return a > b ? a : b; }
 // ------------------------------------------------------------------------------------
 // Project (project.cu:26-34 forward, :71-88 backward).  clip[v][i] = sum_j mat[4j+i]*[p,1]_j,
 // mat column-major.  views > 1: one matrix and one clip slice per view.
-// pinned: closed form (tests/test_oracle.py::test_project_closed_form), adjoint identity.
+// pinned: closed form and adjoint identity (tests/test_oracle.py::test_project_closed_form_and_adjoint).
 template <class T>
 static void project_fwd(int V, int views, const T* pos, const T* mats, T* clip) {
     for (int s = 0; s < views; s++) {
@@ -129,9 +129,13 @@ static void project_bwd(int V, int views, const T* mats, const T* gclip, T* gpos
 //    the closed form of rasterize.cu:114-121; rast.z is the key's z_ndc.
 //  * Output rast = (u, v, z_ndc, id + 1) (rasterize.cpp:175), background 0 (glClear).
 //    DB = (du/dX, du/dY, dv/dX, dv/dY) per pixel, analytic (rasterize.cpp:100-164).
-// pinned: brute force over all triangles per pixel (test_raster_bruteforce), watertight
-// shared edges and full-screen quad coverage (test_raster_watertight), closed-form affine
-// barycentrics (test_raster_affine), DB vs central differences (test_raster_db_fd).
+// pinned: brute force over all triangles per pixel (test_raster_matches_bruteforce_config1,
+// test_raster_matches_bruteforce_soup_with_clipping); an independent float64 point-in-triangle
+// and screen-plane depth test for "nearer triangle wins" and id ties
+// (test_raster_nearer_triangle_wins_independent); watertight shared edges
+// (test_raster_planar_grid_is_watertight, test_raster_closed_mesh_even_coverage); closed-form
+// affine barycentrics (test_raster_affine_barycentrics_closed_form); DB vs central differences
+// (test_raster_db_matches_finite_differences).
 
 static const float RAST_GB = 16.0f;
 static const float RAST_WEPS = 1e-6f;
@@ -441,7 +445,8 @@ static void raster_fwd(int W, int H, int views, int V, const T* clip, int T_, co
 // RasterizeBackwardKernel (rasterize.cu:84-141): (dL/du, dL/dv) -> clip x, y, w of the three
 // vertices.  grast is P x 2 (float2 per pixel, rasterize.cu:94).  z and DB grads ignored
 // (rasterize.cu:96-98,134-136).
-// pinned: central differences of the closed-form u,v w.r.t. clip coords (test_raster_bwd_fd).
+// pinned: central differences of the closed-form u,v w.r.t. clip coords
+// (test_raster_bwd_matches_finite_differences).
 template <class T>
 static void raster_bwd(int W, int H, int views, int V, const T* clip, const u32* idx, const T* rast, const T* grast, T* gclip) {
     T xs = (T)(2.0f / (float)W), ys = (T)(2.0f / (float)H);
@@ -492,8 +497,8 @@ static void raster_bwd(int W, int H, int views, int V, const T* clip, const u32*
 // Interpolate (interpolate.cu:20-49 forward, :114-143 backward).
 // out = a0*u + a1*v + a2*(1.0-u-v) with the double literal of interpolate.cu:38 (A-14);
 // DA_i = ((a0-a2)*db.x + (a1-a2)*db.z, (a0-a2)*db.y + (a1-a2)*db.w).  Background stays 0.
-// pinned: closed form (test_interp_closed_form), linear attributes reproduce exactly,
-// adjoint identity of the backward (test_interp_adjoint).
+// pinned: linear attributes reproduce exactly, adjoint identity of the backward
+// (test_interpolate_linear_attributes_and_adjoint).
 template <class T>
 static void interp_fwd(int W, int H, int views, int D, const T* rast, const T* db, const T* attr, const u32* idx, T* out, T* outDA) {
     parallel_for(views * H, [&](int r0, int r1, int) {
@@ -604,8 +609,9 @@ static void index_fetch(int tw, int th, int level, T uvx, T uvy, int idx4[4], T&
 template <class T> static inline T lerp_(T a, T b, T t) { return a + t * (b - a); }
 template <class T> static inline T bilerp_(T a, T b, T c, T d, T tx, T ty) { return lerp_(lerp_(a, b, tx), lerp_(c, d, tx), ty); }
 
-// pinned: brute-force trilinear on small textures with explicit weights
-// (test_tex_fwd_closed_form), constant texture invariance, exact 2^k footprint -> level k.
+// pinned: affine textures, whose box mips stay affine, give a closed-form trilinear sample at
+// any uv and isotropic footprint (test_texture_affine_trilinear_closed_form); constant texture
+// invariance and an exact 2^k footprint -> level k (test_texture_constant_and_exact_level).
 template <class T>
 static void tex_fwd(int W, int H, int views, int tw, int th, int C, int L, const T* levels, const T* rast, const T* uv, const T* uvDA, T* out) {
     std::vector<size_t> off(L);
@@ -646,8 +652,8 @@ static void tex_fwd(int W, int H, int views, int tw, int th, int C, int L, const
 // formula of calculateLevelWithJacob (:160-197, A-2).  mode 1 ("full") writes the exact
 // d level / d uvDA chain (zero outside the blended range), for finite-difference agreement.
 // Texel grads accumulate (atomics of :249-267), guv/guvDA are stored (:281-282).
-// pinned: central differences w.r.t. texels and uv (test_tex_bwd_fd), adjoint identity for
-// texel grads (test_tex_adjoint), full-mode uvDA grad vs central differences.
+// pinned: central differences w.r.t. texels and uv, adjoint identity for texel grads, full-mode
+// uvDA grad vs central differences (test_texture_bwd_adjoint_and_fd).
 template <class T>
 static void tex_bwd(int W, int H, int views, int tw, int th, int C, int L, const T* levels, const T* rast, const T* uv, const T* uvDA,
                     const T* gout, T* glevels, T* guv, T* guvDA, int mode) {
@@ -760,7 +766,7 @@ static void tex_bwd(int W, int H, int views, int tw, int th, int C, int L, const
 }
 
 // buildMIP/downSampling (buffer.cu:449-484): 2x2 box mean, levels 1..L-1.
-// pinned: constant invariance and closed-form block means (test_mip_build).
+// pinned: constant invariance and closed-form block means (test_mip_build_and_fold_adjoint).
 template <class T>
 static void mip_build(int tw, int th, int C, int L, T* levels) {
     for (int l = 1; l < L; l++) {
@@ -780,7 +786,7 @@ static void mip_build(int tw, int th, int C, int L, T* levels) {
 // gradSumup/gardAddThrough (buffer.cu:615-648): coarse-to-fine, each coarse grad added to its
 // four children with AddNaNcheck.  quarter=1 scales by 1/4 (exact adjoint of mip_build, A-3);
 // quarter=0 is the reference's x1.
-// pinned: adjoint identity <build(x), g> = <x, fold(g)> (test_mip_fold_adjoint).
+// pinned: adjoint identity <build(x), g> = <x, fold(g)> (test_mip_build_and_fold_adjoint).
 template <class T>
 static void mip_fold(int tw, int th, int C, int L, T* glevels, int quarter) {
     for (int l = L - 1; l > 0; l--) {
@@ -803,7 +809,7 @@ static void mip_fold(int tw, int th, int C, int L, T* glevels, int quarter) {
 // ------------------------------------------------------------------------------------
 // NormalAxis (material.cu:25-69): out = rot3x3 * normalize(n0' u + n1' v + n2), the effective
 // semantics of the reference (A-8).  rot column-major 4x4 (16 values).
-// pinned: closed form (test_normal_axis).
+// pinned: closed form (test_normal_and_view_axis_closed_form).
 template <class T>
 static void normal_axis(int W, int H, int views, const T* rast, const T* normals, const u32* nidx, const T* rot, T* out) {
     parallel_for(views * H, [&](int r0, int r1, int) {
@@ -830,7 +836,7 @@ static void normal_axis(int W, int H, int views, const T* rast, const T* normals
 // ViewAxis (material.cu:92-118): out = normalize(pvinv[0]*sx + pvinv[1]*sy + pvinv[2]) with
 // sx = px/W*2-1 (no half-pixel offset, A-9); pvinv column-major 3x3, one per view.  Written
 // for every pixel.
-// pinned: closed form (test_view_axis).
+// pinned: closed form (test_normal_and_view_axis_closed_form).
 template <class T>
 static void view_axis(int W, int H, int views, const T* pvinv, T* out) {
     parallel_for(views * H, [&](int r0, int r1, int) {
@@ -850,7 +856,9 @@ static void view_axis(int W, int H, int views, const T* pvinv, T* out) {
 }
 
 // SGSpecular forward (material.cu:145-183) and backward (:203-249, roughness only).
-// pinned: closed form; roughness grad vs central differences (test_sg_specular_fd).
+// pinned: forward by textbook optics -- Householder reflection axis, the Fresnel equations and
+// GGX Smith G1 (test_sg_specular_forward_closed_forms); roughness grad vs central differences
+// (test_sg_specular_roughness_fd).
 template <class T>
 static void sg_spec_terms(const T* n, const T* view, T m, T ior, T* p, T& amp, T& sharp,
                           T* dAmp_dm, T* dSharp_dm) {
@@ -920,7 +928,7 @@ static void sg_spec_bwd(int W, int H, int views, const T* rast, const T* normal,
 //   diffenv[k] += g(|2.133 n + l_i mu_i|, 2.133 + l_i) * 2.34 * a_i[k]
 //   specenv[k] += g(|l_s p + l_i mu_i|, l_s + l_i) * A_s * a_i[k]     (AddNaNcheck, lobe order)
 //   out[k] = diffuse[k] * diffenv[k] + specenv[k]
-// pinned: aligned-axis closed form g = (1 - e^{-2l})/l (test_sg_closed_form).
+// pinned: aligned-axis closed form g = (1 - e^{-2l})/l (test_sg_aligned_closed_form).
 static const float SG_DIFF_SHARP = 2.133f, SG_DIFF_AMP = 2.34f;
 
 template <class T>
@@ -966,7 +974,7 @@ static void sg_fwd(int W, int H, int views, int C, int K, const T* rast, const T
 // thread order.  Per pixel (w = l_x x + l_i mu_i, l = |w|, e1 = e^{l-c}, e2 = e^{-l-c}):
 //   dg/dl = ((1 - 1/l) e1 + (1 + 1/l) e2) / l,  dg/dc = -g
 // gdiffuse[k] += gout[k] * diffenv[k] (accumulated), grad.normal/spec grads per pixel.
-// pinned: central differences for every parameter (test_sg_bwd_fd); grad.diffuse equals the
+// pinned: central differences for every parameter (test_sg_bwd_finite_differences); grad.diffuse equals the
 // reference's race-free line material.cu:429.
 template <class T>
 static void sg_bwd(int W, int H, int views, int C, int K, const T* rast, const T* axis, const T* sharp, const T* amp,
@@ -1062,8 +1070,9 @@ static void sg_bwd(int W, int H, int views, int C, int K, const T* rast, const T
 // The backward mirrors the forward's p/o selection (A-5 fixed) and uses the exact derivative
 // of alpha, including the w term (the reference's w grad uses the pixel-space offset instead
 // of x/w, antialias.cu:237-238).
-// pinned: single straight edge closed form (test_aa_closed_form), mirror symmetry, central
-// differences w.r.t. clip coords and input colours (test_aa_bwd_fd).
+// pinned: single straight edge closed form (test_aa_straight_edge_closed_form), mirror symmetry
+// (test_aa_mirror_symmetry), central differences w.r.t. clip coords and input colours
+// (test_aa_bwd_finite_differences).
 struct AAEdge { int valid; double alpha; int target_is_o; int va, vb; double ia; double pax, pay, pbx, pby; int horiz; float sg; };
 
 template <class T>
@@ -1229,7 +1238,7 @@ static double mse(size_t n, const T* pred, const T* target, T* grad) {
 
 // Adam (optimize.cu:95-115): host bumps rhomt/rhovt before the launch, so step t uses
 // beta^(t+1) (A-10); m, v updated in double and stored as T; update via AddNaNcheck.
-// pinned: brute-force FP64 recurrence (test_adam_recurrence).
+// pinned: brute-force FP64 recurrence (test_adam_recurrence_fp64).
 template <class T>
 static void adam(size_t n, T* param, const T* grad, T* m, T* v, double rhom, double rhov, double rhomt, double rhovt, double eta, double eps) {
     for (size_t i = 0; i < n; i++) {

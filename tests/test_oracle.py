@@ -239,6 +239,39 @@ def test_texture_constant_and_exact_level():
         np.testing.assert_allclose(o[0, 0, 0], texel, rtol=1e-9)
 
 
+def test_texture_affine_trilinear_closed_form():
+    """Bilinear filtering reproduces affine functions, and the 2x2 box mip of an affine texture
+    is affine: level l of T0[y][x] = a + b x + c y is T_l[y][x] = a + b (2^l x + (2^l-1)/2)
+    + c (2^l y + (2^l-1)/2).  With the reference's tap rule (sample position t = uv * size, no
+    half-texel offset, texturemap.cu:51-63) the level-l sample at uv is therefore
+    a + (b u + c v) S + (2^l - 1)/2 (b + c), and an isotropic footprint of s texels per pixel
+    gives level log2(s) (texturemap.cu:36-41), blended linearly between its two levels."""
+    tw = th = 64
+    C = 1
+    L = S.mip_levels(tw, th, 16)
+    a, b, c = 0.25, 1.0 / 64, -1.0 / 128  # dyadic: every level is exact in float32
+    ys, xs = np.mgrid[0:th, 0:tw]
+    lvl0 = (a + b * xs + c * ys)[..., None]
+    tex = S.build_mips_np(lvl0.astype(F32), L).astype(D64)
+    rng = np.random.default_rng(17)
+    P = 40
+    rast = np.zeros((1, 1, P, 4))
+    rast[..., 3] = 1
+    uv = rng.uniform(0.2, 0.7, (1, 1, P, 2))
+    lam = rng.uniform(0.05, 2.95, P)
+    s = 2.0 ** lam
+    da = np.zeros((1, 1, P, 4))
+    da[0, 0, :, 0] = s / tw
+    da[0, 0, :, 3] = s / th
+    out = o_tex(rast, uv, da, tex, tw, th, C, L, dt=D64)[0, 0, :, 0]
+    l0 = np.floor(lam)
+    f = lam - l0
+    base = a + (b * uv[0, 0, :, 0] * tw + c * uv[0, 0, :, 1] * th)
+    lev = lambda l: base + (2.0 ** l - 1) / 2 * (b + c)
+    want = lev(l0) + f * (lev(l0 + 1) - lev(l0))
+    np.testing.assert_allclose(out, want, rtol=1e-9, atol=1e-12)
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_texture_bwd_adjoint_and_fd(mode):
     tw, th, C = 32, 16, 4
@@ -412,6 +445,161 @@ def test_sg_specular_roughness_fd():
     fd = (f(rough + h) - f(rough - h)) / (2 * h)
     # the reference's d amplitude / d m (material.cu:247) is exact: it must agree with FD
     np.testing.assert_allclose(gr, fd, rtol=1e-4, atol=1e-6)
+
+
+def _fresnel_unpolarized(cos_i, ior):
+    """Textbook dielectric Fresnel reflectance (mean of the s and p terms), Snell's law for the
+    transmitted angle; air (n1 = 1) to a medium of index ior."""
+    sin_t = np.sqrt(np.maximum(1.0 - cos_i ** 2, 0.0)) / ior
+    cos_t = np.sqrt(1.0 - sin_t ** 2)
+    rs = ((cos_i - ior * cos_t) / (cos_i + ior * cos_t)) ** 2
+    rp = ((cos_t - ior * cos_i) / (cos_t + ior * cos_i)) ** 2
+    return 0.5 * (rs + rp)
+
+
+def _smith_ggx_g1(c, alpha):
+    """Smith masking of the GGX distribution: G1 = 2c / (c + sqrt(alpha^2 + (1 - alpha^2) c^2))."""
+    return 2.0 * c / (c + np.sqrt(alpha ** 2 + (1.0 - alpha ** 2) * c ** 2))
+
+
+def _spec_fwd64(n, view, rough, ior):
+    P = n.shape[0]
+    rast = np.zeros((1, 1, P, 4))
+    rast[..., 3] = 1
+    return o_spec(rast, n.reshape(1, 1, P, 3), view.reshape(1, 1, P, 3), rough.reshape(1, 1, P), ior, dt=D64)
+
+
+def test_sg_specular_forward_closed_forms():
+    """SGSpecular forward (material.cu:157-182) against textbook optics, not its own formula:
+    - axis p is the mirror image of v = -view about n: p = -(I - 2 n n^T) v (Householder), so
+      |p| = |v| and p.n = v.n;
+    - its F is twice the unpolarized dielectric Fresnel reflectance of the Fresnel equations
+      (the reference drops the 1/2, material.cu:176), its G terms are GGX Smith G1(c)/(2c) at
+      c = v.n = p.n, so amplitude = 2 F_unpol G1(c)^2 / (4 c m^2) and sharpness = 1/(2 m^2 c);
+    - normal incidence gives axis = n, amplitude = 2 R0 / (4 m^2) with R0 = ((ior-1)/(ior+1))^2;
+    - grazing incidence drives the amplitude to 0; back-facing views give amplitude 0."""
+    rng = np.random.default_rng(21)
+    P, ior = 64, 1.45
+    n = rng.normal(size=(P, 3))
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    # views with v.n in (0.05, 1): mix a random tangent direction into n
+    t = rng.normal(size=(P, 3))
+    t -= np.sum(t * n, axis=1, keepdims=True) * n
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    c = rng.uniform(0.05, 1.0, P)
+    v = c[:, None] * n + np.sqrt(1 - c ** 2)[:, None] * t
+    view = -v
+    m = rng.uniform(0.1, 0.9, P)
+    ax, sh, am = _spec_fwd64(n, view, m, ior)
+    ax, sh, am = ax.reshape(P, 3), sh.reshape(P), am.reshape(P)
+    for i in range(P):
+        H = np.eye(3) - 2.0 * np.outer(n[i], n[i])
+        np.testing.assert_allclose(ax[i], -H @ v[i], rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(np.linalg.norm(ax, axis=1), np.linalg.norm(v, axis=1), rtol=1e-12)
+    np.testing.assert_allclose(np.sum(ax * n, axis=1), c, rtol=1e-12, atol=1e-14)
+    F = 2.0 * _fresnel_unpolarized(c, ior)
+    G1 = _smith_ggx_g1(c, m)
+    np.testing.assert_allclose(am, F * G1 ** 2 / (4.0 * c * m ** 2), rtol=1e-10)
+    np.testing.assert_allclose(sh, 0.5 / (m ** 2 * c), rtol=1e-12)
+
+    # normal incidence: v = n
+    ax, sh, am = _spec_fwd64(n, -n, m, ior)
+    r0 = ((ior - 1) / (ior + 1)) ** 2
+    np.testing.assert_allclose(ax.reshape(P, 3), n, rtol=1e-12, atol=1e-14)
+    np.testing.assert_allclose(am.reshape(P), 2 * r0 / (4 * m ** 2), rtol=1e-12)
+    np.testing.assert_allclose(sh.reshape(P), 0.5 / m ** 2, rtol=1e-12)
+
+    # grazing: amplitude -> 0 linearly in c (G1 ~ 2c/m), Fresnel -> 1 (so F -> 2)
+    for cg in (1e-2, 1e-3, 1e-4):
+        vg = cg * n + np.sqrt(1 - cg ** 2) * t
+        _, _, ag = _spec_fwd64(n, -vg, m, ior)
+        bound = 2.0 * (2 * cg / m) ** 2 / (4 * cg * m ** 2) * 1.01
+        assert np.all(np.abs(ag.reshape(P)) <= bound), cg
+
+    # back-facing: v.n < 0 -> p.n < 0 -> amplitude exactly 0
+    _, _, ab = _spec_fwd64(n, v, m, ior)
+    assert np.all(ab == 0.0)
+
+
+# ------------------------------------------------------------------ depth: nearer triangle wins
+def _independent_ids(tris, W, H, margin=1e-4):
+    """Triangle ids by a separate route from the oracle's fixed-point rasterizer: per pixel
+    centre, float64 point-in-triangle by the signs of the screen-space edge cross products,
+    and depth z/w from the screen-space plane through the three projected vertices (z/w is
+    affine in screen space), minimum (z/w, id) wins.  Pixels whose centre lies within `margin`
+    of an edge, or whose best two depths are within `margin`, are marked -1 (not judged)."""
+    ys, xs = np.mgrid[0:H, 0:W]
+    X = (xs + 0.5) * 2.0 / W - 1.0
+    Y = (ys + 0.5) * 2.0 / H - 1.0
+    best = np.full((H, W), np.inf)
+    second = np.full((H, W), np.inf)
+    ids = np.zeros((H, W), np.int64)
+    amb = np.zeros((H, W), bool)
+    for t, v in enumerate(tris):
+        s = v[:, :3] / v[:, 3:4]  # screen-space x, y, z
+        area = (s[1, 0] - s[0, 0]) * (s[2, 1] - s[0, 1]) - (s[1, 1] - s[0, 1]) * (s[2, 0] - s[0, 0])
+        e = []
+        for a, b in ((0, 1), (1, 2), (2, 0)):
+            ex = (s[b, 0] - s[a, 0]) * (Y - s[a, 1]) - (s[b, 1] - s[a, 1]) * (X - s[a, 0])
+            ln = np.hypot(s[b, 0] - s[a, 0], s[b, 1] - s[a, 1])
+            e.append(ex * np.sign(area) / ln)  # signed distance, > 0 inside
+        e = np.stack(e)
+        inside = np.all(e > 0, axis=0)
+        amb |= np.any(np.abs(e) < margin, axis=0)
+        A = np.c_[s[:, :2], np.ones(3)]
+        cz = np.linalg.solve(A, s[:, 2])
+        z = cz[0] * X + cz[1] * Y + cz[2]
+        inside &= (z >= -1.0) & (z <= 1.0)
+        upd = inside & (z < best)
+        second = np.where(upd, best, np.where(inside, np.minimum(second, z), second))
+        ids = np.where(upd, t + 1, ids)
+        best = np.where(upd, z, best)
+    with np.errstate(invalid="ignore"):
+        amb |= (second - best) < margin  # inf - inf where no triangle covers: not ambiguous
+    ids[amb] = -1
+    return ids
+
+
+def test_raster_nearer_triangle_wins_independent():
+    """Overlapping soups with w in [0.5, 3] (no clipping): the oracle's ids equal an
+    independent float64 point-in-triangle + screen-plane depth test wherever the answer is not
+    within 1e-4 of a tie or an edge.  Also two full-screen planes at z = 0.3 and 0.1: the
+    nearer (second) one owns every pixel whatever the order of the index list."""
+    rng = np.random.default_rng(5)
+    W, H = 64, 48
+    judged = 0
+    for trial in range(6):
+        T = 40
+        tris = []
+        for _ in range(T):
+            c = rng.uniform(-0.8, 0.8, 2)
+            xy = c + rng.normal(scale=0.35, size=(3, 2))
+            w = rng.uniform(0.5, 3.0, 3)
+            z = rng.uniform(-0.9, 0.9, 3)
+            tris.append(np.c_[xy * w[:, None], z * w, w])
+        tris = np.array(tris)
+        clip = tris.reshape(-1, 4).astype(np.float32)
+        idx = np.arange(3 * T, dtype=np.uint32).reshape(T, 3)
+        got = _ids(clip, idx, W, H).astype(np.int64)
+        want = _independent_ids(tris.astype(np.float32).astype(np.float64), W, H)
+        ok = want >= 0
+        judged += int(ok.sum())
+        assert np.array_equal(got[ok], want[ok]), (trial, int(np.sum(got[ok] != want[ok])))
+    assert judged > 0.8 * 6 * W * H
+    # two full-screen quads (4 triangles), far first then near, and the reverse order
+    quad = lambda z: np.array([[-2, -2, z, 1], [2, -2, z, 1], [2, 2, z, 1], [-2, 2, z, 1]], np.float32)
+    clip = np.r_[quad(0.3), quad(0.1)]
+    for order in ([0, 1], [1, 0]):
+        tri = [[0, 1, 2], [0, 2, 3]]
+        idx = np.array([[a + 4 * q for a in t] for q in order for t in tri], np.uint32)
+        ids = _ids(clip, idx, W, H)
+        near_first = order.index(1) * 2  # triangle index of the near quad's first triangle
+        assert np.all((ids == near_first + 1) | (ids == near_first + 2))
+    # equal depth: the lower triangle id wins (GL_LESS keeps the first drawn)
+    clip = np.r_[quad(0.2), quad(0.2)]
+    idx = np.array([[0, 1, 2], [0, 2, 3], [4, 5, 6], [4, 6, 7]], np.uint32)
+    ids = _ids(clip, idx, W, H)
+    assert np.all((ids == 1) | (ids == 2))
 
 
 # ------------------------------------------------------------------ antialias
