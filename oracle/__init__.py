@@ -78,6 +78,12 @@ def _declare(L):
     L.orc_fit_step_f32.argtypes = [vp, vp]
     L.orc_work_buffer.argtypes = [vp, C.c_char_p]
     L.orc_work_buffer.restype = C.POINTER(C.c_float)
+    L.orc_work64_create.restype = vp
+    L.orc_work64_destroy.argtypes = [vp]
+    L.orc_fit_grads_f64.argtypes = [vp, vp]
+    L.orc_fit_grads_f64.restype = cd
+    L.orc_work64_buffer.argtypes = [vp, C.c_char_p]
+    L.orc_work64_buffer.restype = C.POINTER(C.c_double)
 
 
 def P(a):
@@ -104,4 +110,4 @@ class OrcScene(C.Structure):
         ("m_pos", vp), ("v_pos", vp), ("m_tex", vp), ("v_tex", vp), ("m_axis", vp), ("v_axis", vp),
         ("m_sharp", vp), ("v_sharp", vp), ("m_amp", vp), ("v_amp", vp),
         ("lr", cd), ("b1", cd), ("b2", cd), ("eps", cd), ("b1t", cd), ("b2t", cd),
-        ("out_image", vp), ("loss", cd)]
+        ("out_image", vp), ("loss", cd), ("views", C.c_int)]

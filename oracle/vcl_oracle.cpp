@@ -430,13 +430,18 @@ extern "C" void orc_raster_coverage_count(int W, int H, const float* clip, int T
     }
 }
 
+// cov_clip (optional): float clip coordinates that decide coverage and visibility instead of
+// float(clip).  The FP64 yardstick passes the float path's own projection, so it resolves the
+// same winners as the float path (a double projection rounded to float can differ from the
+// float projection in the last bit and flip ties) and evaluates everything continuous in double.
 template <class T>
-static void raster_fwd(int W, int H, int views, int V, const T* clip, int T_, const u32* idx, T* rast, T* db) {
+static void raster_fwd(int W, int H, int views, int V, const T* clip, int T_, const u32* idx, T* rast, T* db, const float* cov_clip = nullptr) {
     std::vector<float> cf((size_t)V * 4);
     std::vector<u64> key;
     for (int s = 0; s < views; s++) {
         const T* cs = clip + (size_t)s * V * 4;
-        for (size_t i = 0; i < cf.size(); i++) cf[i] = (float)cs[i];
+        if (cov_clip) std::memcpy(cf.data(), cov_clip + (size_t)s * V * 4, sizeof(float) * cf.size());
+        else for (size_t i = 0; i < cf.size(); i++) cf[i] = (float)cs[i];
         raster_keys(W, H, cf.data(), T_, idx, key);
         raster_resolve<T>(W, H, cs, idx, key, rast + (size_t)s * W * H * 4, db ? db + (size_t)s * W * H * 4 : nullptr);
     }
@@ -1289,90 +1294,130 @@ ORC_BOTH(ORC_WRAP)
 // ViewAxis -> mip texture -> SGSpecular -> SphericalGaussian -> antialias -> MSE, then the
 // backward chain antialias -> SG -> texture -> interpolate -> rasterize -> project, then
 // gradSumup -> Adam(texture L0) -> clear -> clamp+buildMIP, and Adam on positions and lobes.
-// Float only: this is the CPU baseline that bench.py times.
+// `views` cameras share the mesh, texture and lobes: per-view arrays (mat, pvinv, rough,
+// target, out_image) hold `views` consecutive slices, the loss and every parameter gradient
+// sum over the views.  orc_fit_step_f32 is the CPU baseline that bench.py times;
+// orc_fit_grads_f64 evaluates the same forward and backward in double (the error yardstick).
 struct OrcScene {
     int W, H, V, T, C, K, tw, th, L;
     float* pos; const float* normals; const float* uv; const u32* idx;
     const float* mat; const float* rot; const float* pvinv;
     float* tex;            // all levels
     float* axis; float* sharp; float* amp;
-    const float* rough;    // P roughness (constant map)
+    const float* rough;    // views x P roughness (constant map)
     float ior;
-    const float* target;   // P x C
+    const float* target;   // views x P x C
     int fold_quarter;      // 1: exact adjoint fold
     // Adam state
     float *m_pos, *v_pos, *m_tex, *v_tex, *m_axis, *v_axis, *m_sharp, *v_sharp, *m_amp, *v_amp;
     double lr, b1, b2, eps, b1t, b2t;
     // outputs
     float* out_image; double loss;
+    int views;             // 0 or 1: one view
 };
 
-struct OrcWork {
-    std::vector<float> clip, rast, db, uvp, uvDA, nrm, view, diff, sAxis, sSharp, sAmp, color, denv, senv, img, gimg, gcolor,
+template <class T>
+struct FitWork {
+    std::vector<T> clip, rast, db, uvp, uvDA, nrm, view, diff, sAxis, sSharp, sAmp, color, denv, senv, img, gimg, gcolor,
         gdiff, gtex, guv, guvDA, grast, gclip, gpos, gaxis, gsharp, gamp;
+    // double copies of the float inputs (orc_fit_grads_f64 only)
+    std::vector<T> in_pos, in_normals, in_uv, in_mat, in_rot, in_pvinv, in_tex, in_axis, in_sharp, in_amp, in_rough, in_target;
 };
+typedef FitWork<float> OrcWork;
+
+// Forward and backward of one iteration; leaves every parameter gradient in the work buffers
+// (texture grads per level, not folded).  Returns the loss.
+// cov_clip: see raster_fwd (NULL in the float step).
+template <class T>
+static double fit_fwd_bwd(const OrcScene* s, FitWork<T>& w, const T* pos, const T* normals, const T* uv, const T* mat, const T* rot,
+                          const T* pvinv, const T* tex, const T* axis, const T* sharp, const T* amp, const T* rough, const T* target,
+                          const float* cov_clip = nullptr) {
+    int nv = s->views > 1 ? s->views : 1;
+    size_t P = (size_t)s->W * s->H * nv;
+    int C = s->C;
+    size_t ntex = level_offset(s->tw, s->th, C, s->L);
+    auto sz = [](std::vector<T>& v, size_t n) { v.assign(n, T(0)); };
+    sz(w.clip, (size_t)nv * s->V * 4); sz(w.rast, P * 4); sz(w.db, P * 4); sz(w.uvp, P * 2); sz(w.uvDA, P * 4);
+    sz(w.nrm, P * 3); sz(w.view, P * 3); sz(w.diff, P * C); sz(w.sAxis, P * 3); sz(w.sSharp, P); sz(w.sAmp, P);
+    sz(w.color, P * C); sz(w.denv, P * C); sz(w.senv, P * C); sz(w.img, P * C); sz(w.gimg, P * C); sz(w.gcolor, P * C);
+    sz(w.gdiff, P * C); sz(w.gtex, ntex); sz(w.guv, P * 2); sz(w.guvDA, P * 4); sz(w.grast, P * 2); sz(w.gclip, (size_t)nv * s->V * 4);
+    sz(w.gpos, (size_t)s->V * 3); sz(w.gaxis, (size_t)s->K * 3); sz(w.gsharp, s->K); sz(w.gamp, (size_t)s->K * C);
+    // forward
+    project_fwd<T>(s->V, nv, pos, mat, w.clip.data());
+    raster_fwd<T>(s->W, s->H, nv, s->V, w.clip.data(), s->T, s->idx, w.rast.data(), w.db.data(), cov_clip);
+    interp_fwd<T>(s->W, s->H, nv, 2, w.rast.data(), w.db.data(), uv, s->idx, w.uvp.data(), w.uvDA.data());
+    normal_axis<T>(s->W, s->H, nv, w.rast.data(), normals, s->idx, rot, w.nrm.data());
+    view_axis<T>(s->W, s->H, nv, pvinv, w.view.data());
+    tex_fwd<T>(s->W, s->H, nv, s->tw, s->th, C, s->L, tex, w.rast.data(), w.uvp.data(), w.uvDA.data(), w.diff.data());
+    sg_spec_fwd<T>(s->W, s->H, nv, w.rast.data(), w.nrm.data(), w.view.data(), rough, (T)s->ior, w.sAxis.data(), w.sSharp.data(), w.sAmp.data());
+    sg_fwd<T>(s->W, s->H, nv, C, s->K, w.rast.data(), axis, sharp, amp, w.nrm.data(), w.diff.data(), w.sAxis.data(), w.sSharp.data(), w.sAmp.data(), w.color.data(), w.denv.data(), w.senv.data());
+    aa_fwd<T>(s->W, s->H, nv, C, s->V, w.clip.data(), s->idx, w.rast.data(), w.color.data(), w.img.data());
+    double loss = mse<T>(P * C, w.img.data(), target, w.gimg.data());
+    // backward
+    aa_bwd<T>(s->W, s->H, nv, C, s->V, w.clip.data(), s->idx, w.rast.data(), w.color.data(), w.gimg.data(), w.gcolor.data(), w.gclip.data());
+    sg_bwd<T>(s->W, s->H, nv, C, s->K, w.rast.data(), axis, sharp, amp, w.nrm.data(), w.diff.data(), w.sAxis.data(), w.sSharp.data(), w.sAmp.data(),
+              w.gcolor.data(), w.gdiff.data(), w.gaxis.data(), w.gsharp.data(), w.gamp.data(), nullptr, nullptr, nullptr, nullptr);
+    tex_bwd<T>(s->W, s->H, nv, s->tw, s->th, C, s->L, tex, w.rast.data(), w.uvp.data(), w.uvDA.data(), w.gdiff.data(), w.gtex.data(), w.guv.data(), w.guvDA.data(), 0);
+    interp_bwd<T>(s->W, s->H, nv, 2, w.rast.data(), uv, s->idx, w.guv.data(), nullptr, w.grast.data(), s->V);
+    raster_bwd<T>(s->W, s->H, nv, s->V, w.clip.data(), s->idx, w.rast.data(), w.grast.data(), w.gclip.data());
+    project_bwd<T>(s->V, nv, mat, w.gclip.data(), w.gpos.data());
+    return loss;
+}
 
 extern "C" void* orc_work_create(void) { return new OrcWork(); }
 extern "C" void orc_work_destroy(void* w) { delete (OrcWork*)w; }
+extern "C" void* orc_work64_create(void) { return new FitWork<double>(); }
+extern "C" void orc_work64_destroy(void* w) { delete (FitWork<double>*)w; }
 
 extern "C" void orc_fit_step_f32(OrcScene* s, void* work) {
     OrcWork& w = *(OrcWork*)work;
-    size_t P = (size_t)s->W * s->H;
     int C = s->C;
-    size_t ntex = level_offset(s->tw, s->th, C, s->L);
-    auto sz = [](std::vector<float>& v, size_t n) { if (v.size() != n) v.assign(n, 0.f); };
-    sz(w.clip, (size_t)s->V * 4); sz(w.rast, P * 4); sz(w.db, P * 4); sz(w.uvp, P * 2); sz(w.uvDA, P * 4);
-    sz(w.nrm, P * 3); sz(w.view, P * 3); sz(w.diff, P * C); sz(w.sAxis, P * 3); sz(w.sSharp, P); sz(w.sAmp, P);
-    sz(w.color, P * C); sz(w.denv, P * C); sz(w.senv, P * C); sz(w.img, P * C); sz(w.gimg, P * C); sz(w.gcolor, P * C);
-    sz(w.gdiff, P * C); sz(w.gtex, ntex); sz(w.guv, P * 2); sz(w.guvDA, P * 4); sz(w.grast, P * 2); sz(w.gclip, (size_t)s->V * 4);
-    sz(w.gpos, (size_t)s->V * 3); sz(w.gaxis, (size_t)s->K * 3); sz(w.gsharp, s->K); sz(w.gamp, (size_t)s->K * C);
-    // forward
-    project_fwd<float>(s->V, 1, s->pos, s->mat, w.clip.data());
-    raster_fwd<float>(s->W, s->H, 1, s->V, w.clip.data(), s->T, s->idx, w.rast.data(), w.db.data());
-    interp_fwd<float>(s->W, s->H, 1, 2, w.rast.data(), w.db.data(), s->uv, s->idx, w.uvp.data(), w.uvDA.data());
-    normal_axis<float>(s->W, s->H, 1, w.rast.data(), s->normals, s->idx, s->rot, w.nrm.data());
-    view_axis<float>(s->W, s->H, 1, s->pvinv, w.view.data());
-    tex_fwd<float>(s->W, s->H, 1, s->tw, s->th, C, s->L, s->tex, w.rast.data(), w.uvp.data(), w.uvDA.data(), w.diff.data());
-    sg_spec_fwd<float>(s->W, s->H, 1, w.rast.data(), w.nrm.data(), w.view.data(), s->rough, s->ior, w.sAxis.data(), w.sSharp.data(), w.sAmp.data());
-    sg_fwd<float>(s->W, s->H, 1, C, s->K, w.rast.data(), s->axis, s->sharp, s->amp, w.nrm.data(), w.diff.data(), w.sAxis.data(), w.sSharp.data(), w.sAmp.data(), w.color.data(), w.denv.data(), w.senv.data());
-    aa_fwd<float>(s->W, s->H, 1, C, s->V, w.clip.data(), s->idx, w.rast.data(), w.color.data(), w.img.data());
-    s->loss = mse<float>(P * C, w.img.data(), s->target, w.gimg.data());
-    // backward
-    std::fill(w.gclip.begin(), w.gclip.end(), 0.f);
-    aa_bwd<float>(s->W, s->H, 1, C, s->V, w.clip.data(), s->idx, w.rast.data(), w.color.data(), w.gimg.data(), w.gcolor.data(), w.gclip.data());
-    std::fill(w.gdiff.begin(), w.gdiff.end(), 0.f);
-    sg_bwd<float>(s->W, s->H, 1, C, s->K, w.rast.data(), s->axis, s->sharp, s->amp, w.nrm.data(), w.diff.data(), w.sAxis.data(), w.sSharp.data(), w.sAmp.data(),
-                  w.gcolor.data(), w.gdiff.data(), w.gaxis.data(), w.gsharp.data(), w.gamp.data(), nullptr, nullptr, nullptr, nullptr);
-    std::fill(w.guv.begin(), w.guv.end(), 0.f);
-    tex_bwd<float>(s->W, s->H, 1, s->tw, s->th, C, s->L, s->tex, w.rast.data(), w.uvp.data(), w.uvDA.data(), w.gdiff.data(), w.gtex.data(), w.guv.data(), w.guvDA.data(), 0);
-    std::fill(w.grast.begin(), w.grast.end(), 0.f);
-    interp_bwd<float>(s->W, s->H, 1, 2, w.rast.data(), s->uv, s->idx, w.guv.data(), nullptr, w.grast.data(), s->V);
-    raster_bwd<float>(s->W, s->H, 1, s->V, w.clip.data(), s->idx, w.rast.data(), w.grast.data(), w.gclip.data());
-    project_bwd<float>(s->V, 1, s->mat, w.gclip.data(), w.gpos.data());
+    int nv = s->views > 1 ? s->views : 1;
+    size_t P = (size_t)s->W * s->H * nv;
+    s->loss = fit_fwd_bwd<float>(s, w, s->pos, s->normals, s->uv, s->mat, s->rot, s->pvinv, s->tex, s->axis, s->sharp, s->amp, s->rough, s->target);
     // update
     s->b1t *= s->b1; s->b2t *= s->b2;
     mip_fold<float>(s->tw, s->th, C, s->L, w.gtex.data(), s->fold_quarter);
     adam<float>((size_t)s->tw * s->th * C, s->tex, w.gtex.data(), s->m_tex, s->v_tex, s->b1, s->b2, s->b1t, s->b2t, s->lr, s->eps);
-    std::fill(w.gtex.begin(), w.gtex.end(), 0.f);
     clamp_<float>((size_t)s->tw * s->th * C, s->tex, 0.f, 1.f);
     mip_build<float>(s->tw, s->th, C, s->L, s->tex);
     adam<float>((size_t)s->V * 3, s->pos, w.gpos.data(), s->m_pos, s->v_pos, s->b1, s->b2, s->b1t, s->b2t, s->lr, s->eps);
     adam<float>((size_t)s->K * 3, s->axis, w.gaxis.data(), s->m_axis, s->v_axis, s->b1, s->b2, s->b1t, s->b2t, s->lr, s->eps);
     adam<float>((size_t)s->K, s->sharp, w.gsharp.data(), s->m_sharp, s->v_sharp, s->b1, s->b2, s->b1t, s->b2t, s->lr, s->eps);
     adam<float>((size_t)s->K * C, s->amp, w.gamp.data(), s->m_amp, s->v_amp, s->b1, s->b2, s->b1t, s->b2t, s->lr, s->eps);
-    std::fill(w.gpos.begin(), w.gpos.end(), 0.f);
-    std::fill(w.gaxis.begin(), w.gaxis.end(), 0.f);
-    std::fill(w.gsharp.begin(), w.gsharp.end(), 0.f);
-    std::fill(w.gamp.begin(), w.gamp.end(), 0.f);
     if (s->out_image) std::memcpy(s->out_image, w.img.data(), sizeof(float) * P * C);
 }
 
+// The same forward and backward in double from the scene's float inputs (no update); the
+// texture gradient is folded into level 0 as the step folds it.  Returns the loss.
+extern "C" double orc_fit_grads_f64(const OrcScene* s, void* work) {
+    FitWork<double>& w = *(FitWork<double>*)work;
+    int nv = s->views > 1 ? s->views : 1;
+    size_t P = (size_t)s->W * s->H * nv;
+    int C = s->C;
+    auto cp = [](std::vector<double>& d, const float* src, size_t n) { d.assign(src, src + n); };
+    cp(w.in_pos, s->pos, (size_t)s->V * 3); cp(w.in_normals, s->normals, (size_t)s->V * 3); cp(w.in_uv, s->uv, (size_t)s->V * 2);
+    cp(w.in_mat, s->mat, (size_t)nv * 16); cp(w.in_rot, s->rot, 16); cp(w.in_pvinv, s->pvinv, (size_t)nv * 9);
+    cp(w.in_tex, s->tex, level_offset(s->tw, s->th, C, s->L)); cp(w.in_axis, s->axis, (size_t)s->K * 3);
+    cp(w.in_sharp, s->sharp, s->K); cp(w.in_amp, s->amp, (size_t)s->K * C); cp(w.in_rough, s->rough, P);
+    cp(w.in_target, s->target, P * C);
+    std::vector<float> cov((size_t)nv * s->V * 4);
+    project_fwd<float>(s->V, nv, s->pos, s->mat, cov.data());
+    double loss = fit_fwd_bwd<double>(s, w, w.in_pos.data(), w.in_normals.data(), w.in_uv.data(), w.in_mat.data(), w.in_rot.data(),
+                                      w.in_pvinv.data(), w.in_tex.data(), w.in_axis.data(), w.in_sharp.data(), w.in_amp.data(),
+                                      w.in_rough.data(), w.in_target.data(), cov.data());
+    mip_fold<double>(s->tw, s->th, C, s->L, w.gtex.data(), s->fold_quarter);
+    return loss;
+}
+
 // Accessors for the work buffers of the last step (tests compare intermediates).
-extern "C" const float* orc_work_buffer(void* work, const char* name) {
-    OrcWork& w = *(OrcWork*)work;
+template <class T>
+static const T* work_buffer(FitWork<T>& w, const char* name) {
 #define B(x) if (!std::strcmp(name, #x)) return w.x.data();
-    B(clip) B(rast) B(db) B(uvp) B(uvDA) B(nrm) B(view) B(diff) B(sAxis) B(sSharp) B(sAmp) B(color) B(img) B(gimg) B(gcolor)
-    B(gdiff) B(guv) B(grast) B(gclip)
+    B(clip) B(rast) B(db) B(uvp) B(uvDA) B(nrm) B(view) B(diff) B(sAxis) B(sSharp) B(sAmp) B(color) B(denv) B(senv) B(img) B(gimg) B(gcolor)
+    B(gdiff) B(guv) B(grast) B(gclip) B(gtex) B(gpos) B(gaxis) B(gsharp) B(gamp)
 #undef B
     return nullptr;
 }
+extern "C" const float* orc_work_buffer(void* work, const char* name) { return work_buffer(*(OrcWork*)work, name); }
+extern "C" const double* orc_work64_buffer(void* work, const char* name) { return work_buffer(*(FitWork<double>*)work, name); }

@@ -216,3 +216,89 @@ def grid_mesh(n, z=0.5, w=1.0, jitter=0.0, seed=0):
             b, c, d = a + 1, a + n + 1, a + n + 2
             tris += [(a, b, d), (a, d, c)]
     return clip, np.array(tris, np.uint32)
+
+
+# ------------------------------------------------------------------ whole-step oracle
+class OracleStep:
+    """The oracle's fwd+bwd+Adam step (orc_fit_step_f32) of scene `sc` against `target`
+    (views x H x W x C), over every camera of sc.cams; optionally the same forward and backward
+    in float64 (orc_fit_grads_f64).  Keeps the parameter copies it updated and the work
+    buffers of both runs."""
+
+    def __init__(self, sc, target, lr=1e-2, fold_quarter=1, fp64=False):
+        import ctypes as C
+        self.sc = sc
+        nv = len(sc.cams)
+        self.views = nv
+        L = O.lib()
+        s = O.OrcScene()
+        keep = {}
+
+        def p(name, arr):
+            keep[name] = np.ascontiguousarray(arr)
+            return keep[name].ctypes.data_as(C.c_void_p)
+        s.W, s.H, s.V, s.T, s.C, s.K, s.tw, s.th, s.L = sc.W, sc.H, sc.V, sc.T, sc.C, sc.K, sc.tw, sc.th, sc.L
+        s.views = nv
+        s.pos = p("pos", sc.pos.copy()); s.normals = p("n", sc.normals); s.uv = p("uv", sc.uv); s.idx = p("idx", sc.idx)
+        s.mat = p("mat", np.stack([c.mat for c in sc.cams]).astype(F32)); s.rot = p("rot", np.eye(4, dtype=F32).reshape(16))
+        s.pvinv = p("pv", np.stack([c.pvinv.reshape(9) for c in sc.cams]).astype(F32))
+        s.tex = p("tex", sc.tex.copy()); s.axis = p("axis", sc.axis.copy()); s.sharp = p("sharp", sc.sharp.copy()); s.amp = p("amp", sc.amp.copy())
+        s.rough = p("rough", np.full(nv * sc.W * sc.H, sc.rough, F32)); s.ior = sc.ior
+        s.target = p("tgt", np.ascontiguousarray(target, F32).reshape(-1)); s.fold_quarter = int(fold_quarter)
+        for nm, n in (("pos", sc.V * 3), ("tex", sc.tw * sc.th * sc.C), ("axis", sc.K * 3), ("sharp", sc.K), ("amp", sc.K * sc.C)):
+            setattr(s, "m_" + nm, p("m_" + nm, np.zeros(n, F32)))
+            setattr(s, "v_" + nm, p("v_" + nm, np.zeros(n, F32)))
+        s.lr, s.b1, s.b2, s.eps, s.b1t, s.b2t = lr, 0.9, 0.999, 1e-8, 0.9, 0.999
+        self.image = np.zeros((nv, sc.H, sc.W, sc.C), F32)
+        s.out_image = p("img", self.image)
+        self.s, self.keep = s, keep
+        self.loss64 = None
+        self.w64 = None
+        if fp64:
+            # the double pass reads the parameters before the float step updates them
+            self.w64 = L.orc_work64_create()
+            self.loss64 = L.orc_fit_grads_f64(C.byref(s), self.w64)
+        self.w = L.orc_work_create()
+        L.orc_fit_step_f32(C.byref(s), self.w)
+        self.loss = s.loss
+
+    def buf(self, name, shape):
+        return np.ctypeslib.as_array(O.lib().orc_work_buffer(self.w, name.encode()), shape=(int(np.prod(shape)),)).reshape(shape).copy()
+
+    def buf64(self, name, shape):
+        return np.ctypeslib.as_array(O.lib().orc_work64_buffer(self.w64, name.encode()), shape=(int(np.prod(shape)),)).reshape(shape).copy()
+
+    def close(self):
+        O.lib().orc_work_destroy(self.w)
+        if self.w64:
+            O.lib().orc_work64_destroy(self.w64)
+        self.w = self.w64 = None
+
+
+def fp64_report(what, gpu, f32, f64, tol=1e-5):
+    """Errors of the GPU result and of the float oracle against the FP64 oracle, printed so
+    every op's error is stated against FP64: the normwise relative error max|a-b| / max|b|,
+    its 99.9th percentile over elements, and the fraction of elements within tol * max|b|.
+    Ops with discrete decisions (mip level, antialias edge tests) and sliver triangles have
+    isolated elements where float and double take different branches; the percentile and the
+    fraction show the bulk.  Returns (gpu, float oracle) tuples of (max, p99.9, fraction)."""
+    b = np.asarray(f64, np.float64)
+    scale = max(float(np.max(np.abs(b))) if b.size else 0.0, 1e-30)
+    out = []
+    for a in (gpu, f32):
+        d = np.abs(np.asarray(a, np.float64) - b) / scale
+        out.append((float(d.max()) if d.size else 0.0, float(np.percentile(d, 99.9)) if d.size else 0.0,
+                    float(np.mean(d <= tol)) if d.size else 1.0))
+    (gm, gp, gf), (om, op, of) = out
+    print(f"  {what:<22} GPU vs FP64: max {gm:.2e} p99.9 {gp:.2e} within {tol:g}: {gf:.5f}   "
+          f"float oracle vs FP64: max {om:.2e} p99.9 {op:.2e} within: {of:.5f}", flush=True)
+    return out[0], out[1]
+
+
+def assert_fp64_no_worse(what, g, o, tol):
+    """The GPU is as accurate as the reference-semantics float path: its FP64 error (max and
+    99.9th percentile) is at most twice the float oracle's (plus a floor of tol / 10), and no
+    fewer of its elements are within tol of FP64 (up to 1e-4 of them)."""
+    assert g[0] <= 2 * o[0] + tol / 10, (what, "max", g, o)
+    assert g[1] <= 2 * o[1] + tol / 10, (what, "p99.9", g, o)
+    assert g[2] >= o[2] - 1e-4, (what, "fraction within tol", g, o)
