@@ -381,6 +381,40 @@ typedef struct vcl_loss_params {
 size_t vcl_loss_workspace_size(int size);
 int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stream);
 
+/* ---------------------------------------------------------------- Rotation / Camera (host)
+ * Host math of Rotation and Camera (matrix.h:1-96, matrix.cu:3-239), usable without a GPU.
+ * Quaternions are 4 floats in glm's storage order (x, y, z, w); matrices are 16 floats
+ * column-major (glm::mat4).  include/vcl_matrix.hpp holds the same math header-only for C++
+ * callers; include/vcl_layers.hpp builds the Rotation / Camera classes on it. */
+/* getRotation (matrix.cu:43-53) */
+int vcl_rotation_matrix(const float* q, float* out16);
+/* getView (matrix.cu:175-180) */
+int vcl_view_matrix(const float* q, const float* eye, float* out16);
+/* Camera::setCam's look-at quaternion (matrix.cu:182-192) */
+int vcl_look_at_quat(const float* eye, const float* center, const float* up, float* q_out);
+/* Camera::forward's projection (matrix.cu:211-219) */
+int vcl_projection_matrix(float size, float aspect, float znear, float zfar, float* out16);
+/* Camera::forward (matrix.cu:209-222): view, projection and mat = projection * view * rotation. */
+int vcl_camera_matrices(const float* q, const float* eye, float size, float aspect, float znear, float zfar,
+                        const float* rotation16, float* view16, float* projection16, float* mat16);
+/* Camera::backward (matrix.cu:234-239).  exact = 0: the reference's products
+ * (G V^T P^T, R^T G P^T, R^T V^T G); exact = 1: the derivatives of mat = P V R. */
+int vcl_camera_backward_host(const float* gmat16, const float* rotation16, const float* view16, const float* projection16,
+                             int exact, float* grot16, float* gview16, float* gproj16);
+/* Rotation::backward (matrix.cu:85-99).  exact = 0: the reference's sums (half the derivative);
+ * exact = 1: the derivative of getRotation w.r.t. q. */
+int vcl_rotation_backward_host(const float* q, const float* grot16, int exact, float* gq);
+/* Rotation::init(..., eta, rhom, rhov, eps) state and Rotation::step (matrix.cu:101-137):
+ * Adam on the quaternion in double with the reference's rhom^(t+1) bias correction, then
+ * renormalisation.  q is updated in place. */
+typedef struct vcl_rotation_adam {
+    int it;
+    double rhom, rhov, rhomt, rhovt, eta, eps;
+    float m[4];
+    float v[4];
+} vcl_rotation_adam;
+int vcl_rotation_adam_step(vcl_rotation_adam* a, float* q, const float* gq);
+
 /* ---------------------------------------------------------------- Fit engine
  * One inverse-rendering request (the chain of preset_primitives.cpp:160-207 generalised to
  * vertex positions, mip texture and SG lobes): project -> rasterize(+DB) -> interpolate(uv)

@@ -113,6 +113,11 @@ class LossParams(C.Structure):
                 ("size", ci), ("loss", vp)]
 
 
+class RotationAdam(C.Structure):
+    _fields_ = [("it", ci), ("rhom", cd), ("rhov", cd), ("rhomt", cd), ("rhovt", cd), ("eta", cd), ("eps", cd),
+                ("m", C.c_float * 4), ("v", C.c_float * 4)]
+
+
 class FitDesc(C.Structure):
     _fields_ = [("width", ci), ("height", ci), ("views", ci), ("V", ci), ("T", ci), ("texwidth", ci), ("texheight", ci),
                 ("channel", ci), ("miplevel", ci), ("sgnum", ci),
@@ -171,6 +176,14 @@ SIGNATURES = {
     "vcl_spherical_gaussian_backward": ([P_(SphericalGaussianParams), P_(SphericalGaussianGradParams), vp], ci),
     "vcl_loss_workspace_size": ([ci], csz),
     "vcl_mse_loss_backward": ([P_(LossParams), vp], ci),
+    "vcl_rotation_matrix": ([vp, vp], ci),
+    "vcl_view_matrix": ([vp, vp, vp], ci),
+    "vcl_look_at_quat": ([vp, vp, vp, vp], ci),
+    "vcl_projection_matrix": ([cf, cf, cf, cf, vp], ci),
+    "vcl_camera_matrices": ([vp, vp, cf, cf, cf, cf, vp, vp, vp, vp], ci),
+    "vcl_camera_backward_host": ([vp, vp, vp, vp, ci, vp, vp, vp], ci),
+    "vcl_rotation_backward_host": ([vp, vp, ci, vp], ci),
+    "vcl_rotation_adam_step": ([P_(RotationAdam), vp, vp], ci),
     "vcl_fit_create": ([P_(FitDesc), P_(vp)], ci),
     "vcl_fit_arena_bytes": ([P_(FitDesc)], csz),
     "vcl_fit_create_in": ([P_(FitDesc), vp, csz, vp, P_(vp)], ci),

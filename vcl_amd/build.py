@@ -32,7 +32,7 @@ FLAGS = [
 
 def _compile(src: str) -> str:
     obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
-    deps = [src] + glob.glob(os.path.join(CSRC, "*.hpp")) + [os.path.join(ROOT, "include", "vcl.h")]
+    deps = [src] + glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(ROOT, "include", "*"))
     if os.path.exists(obj) and all(os.path.getmtime(obj) >= os.path.getmtime(d) for d in deps):
         return obj
     cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
@@ -44,7 +44,7 @@ def _compile(src: str) -> str:
 
 def build(force: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
-    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")) + glob.glob(os.path.join(CSRC, "*.cpp")))
     if force:
         for o in glob.glob(os.path.join(OUT_DIR, "*.o")):
             os.remove(o)
