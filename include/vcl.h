@@ -380,6 +380,15 @@ typedef struct vcl_loss_params {
 
 size_t vcl_loss_workspace_size(int size);
 int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stream);
+/* The same MSE loss without allocation or workspace argument (no counterpart in the
+ * reference): for callers that cannot call hipMalloc (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid it).  p->buffer is ignored; the block partials live in
+ * a 1 MiB device array of the library, one per GPU, held until the process ends, so sizes up
+ * to vcl_mse_loss_noalloc_max_size() elements (2^28) are accepted and larger ones return
+ * VCL_ERR_OOM.  Results equal vcl_mse_loss_backward's.  Calls that overlap on one GPU share
+ * that array and race. */
+size_t vcl_mse_loss_noalloc_max_size(void);
+int vcl_mse_loss_backward_noalloc(const vcl_loss_params* p, vcl_stream_t stream);
 
 /* ---------------------------------------------------------------- Rotation / Camera (host)
  * Host math of Rotation and Camera (matrix.h:1-96, matrix.cu:3-239), usable without a GPU.

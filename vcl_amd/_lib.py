@@ -176,6 +176,8 @@ SIGNATURES = {
     "vcl_spherical_gaussian_backward": ([P_(SphericalGaussianParams), P_(SphericalGaussianGradParams), vp], ci),
     "vcl_loss_workspace_size": ([ci], csz),
     "vcl_mse_loss_backward": ([P_(LossParams), vp], ci),
+    "vcl_mse_loss_noalloc_max_size": ([], csz),
+    "vcl_mse_loss_backward_noalloc": ([P_(LossParams), vp], ci),
     "vcl_rotation_matrix": ([vp, vp], ci),
     "vcl_view_matrix": ([vp, vp, vp], ci),
     "vcl_look_at_quat": ([vp, vp, vp, vp], ci),

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstring>
 #include <cmath>
+#include <mutex>
 
 __global__ void vcl_fill_u32(u32* p, size_t n, u32 v) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -2763,6 +2764,54 @@ extern "C" int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stre
     if (st) return st;
     const size_t nb = p->size > 0 ? cdiv((size_t)p->size, (size_t)BS * 4) : 0;
     hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(1024), 0, as_hip(stream), p->buffer, nb, p->loss);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+// ---------------------------------------------------------------- MSE without allocation
+// The block partials of vcl_mse_loss_backward_noalloc live in this file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never uses the array's name as an
+// address (on the host it is only a stand-in for the symbol): each device's address comes
+// from hipGetSymbolAddress, taken once per device with that device current.  k_mse writes
+// every partial that k_mse_final reads, so nothing needs clearing between calls.  Calls that
+// overlap on one GPU share the array and race.
+#define VCL_MSE_NOALLOC_PARTIALS (1u << 18)  /* 1 MiB of float partials */
+static __device__ __align__(256) float scratch[VCL_MSE_NOALLOC_PARTIALS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(float** out) {
+    int dev = 0;
+    VCL_HIP_CHECK(hipGetDevice(&dev));
+    VCL_REQUIRE(dev >= 0 && dev < kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        VCL_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        VCL_REQUIRE(ptr);
+        g_scratch_ptr[dev] = (float*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return VCL_OK;
+}
+}  // namespace
+
+extern "C" size_t vcl_mse_loss_noalloc_max_size(void) { return (size_t)VCL_MSE_NOALLOC_PARTIALS * BS * 4; }
+
+extern "C" int vcl_mse_loss_backward_noalloc(const vcl_loss_params* p, vcl_stream_t stream) {
+    VCL_REQUIRE(p && p->predict && p->target && p->grad && p->loss && p->size >= 0);
+    if ((size_t)p->size > vcl_mse_loss_noalloc_max_size()) return VCL_ERR_OOM;
+    float* part = nullptr;
+    int st = scratch_for_current_device(&part);
+    if (st) return st;
+    vcl_loss_params q = *p;
+    q.buffer = part;
+    if ((st = vcl_mse_loss_grad_dev(&q, stream))) return st;
+    const size_t nb = q.size > 0 ? cdiv((size_t)q.size, (size_t)BS * 4) : 0;
+    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(1024), 0, as_hip(stream), part, nb, q.loss);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
