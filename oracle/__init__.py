@@ -17,8 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libvcl_oracle.so")
 
 
 def build(force: bool = False) -> str:
-    src = os.path.join(_HERE, "vcl_oracle.cpp")
-    if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
+    srcs = [os.path.join(_HERE, f) for f in ("vcl_oracle.cpp", "vcl_oracle_extra.cpp", "oracle_common.hpp")]
+    if force or not os.path.exists(LIB_PATH) or any(os.path.getmtime(LIB_PATH) < os.path.getmtime(f) for f in srcs):
         subprocess.check_call(["make", "-s", "-C", _HERE, "-B" if force else "all"])
     return LIB_PATH
 
@@ -68,6 +68,43 @@ def _declare(L):
             f = getattr(L, f"orc_{name}_{s}")
             f.argtypes = args
             f.restype = cd if name == "mse" else None
+    for s_, ct in (("f32", C.c_float), ("f64", C.c_double)):
+        extra = {
+            "ssim": ([ci, ci, ci, vp, vp, vp, vp, ci], cd),
+            "filter_fwd": ([ci, ci, ci, ci, ct, vp, vp], None),
+            "filter_bwd": ([ci, ci, ci, ci, ct, vp, vp, vp], cd),
+            "compute_normal_fwd": ([ci, vp, ci, vp, ci, vp], None),
+            "compute_normal_bwd": ([ci, vp, ci, vp, ci, vp, vp], None),
+        }
+        for name, (args, res) in extra.items():
+            f = getattr(L, f"orc_{name}_{s_}")
+            f.argtypes = args
+            f.restype = res
+    u32 = C.c_uint32
+    cf = C.c_float
+    for name, args, res in (
+            ("orc_uniform", [csz, vp, u32, u32, vp], None),
+            ("orc_noise", [ci, ci, ci, ci, cf, vp, vp, u32], None),
+            ("orc_sgd", [csz, vp, vp, cd], None),
+            ("orc_nadam", [csz, vp, vp, vp, vp] + [cd] * 9, None),
+            ("orc_random_params", [csz, vp, u32, cf, cf], None),
+            ("orc_linear", [csz, vp, vp, cf, cf], None),
+            ("orc_add_random", [csz, vp, cf, cf, u32], None),
+            ("orc_clamp_buf", [csz, vp, vp, cf, cf], None),
+            ("orc_step_buf", [csz, vp, vp, cf], None),
+            ("orc_normalize_rows", [ci, ci, vp, vp], None),
+            ("orc_spherical_random", [ci, vp, u32], None),
+            ("orc_distance_error", [ci, ci, vp, vp], cd),
+            ("orc_bilinear_downsampling", [ci, ci, ci, ci, vp], None),
+            ("orc_sg_bake", [ci, ci, vp, vp, vp, ci, ci, vp], None),
+            ("orc_sg_normalize", [ci, ci, vp, vp, vp], None),
+            ("orc_sg_disperse", [ci, ci, vp, vp, vp, vp], None),
+            ("orc_texture_loss", [ci, ci, ci, vp, vp, vp], cd),
+            ("orc_raster_idhash", [csz, vp, vp], None),
+            ("orc_raster_wireframe", [csz, vp, vp, vp], None)):
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
     L.orc_raster_brute.argtypes = [ci, ci, vp, ci, vp, vp]
     L.orc_raster_coverage_count.argtypes = [ci, ci, vp, ci, vp, vp]
     L.orc_mip_levels.argtypes = [ci, ci, ci]

@@ -310,6 +310,18 @@ def test_texture_nomip(gpu, cfg1):
     out = gpu.texture(T(d["rast"]), T(d["uvp"]), T(d["uvDA"]), levels)
     ref = o_tex(d["rast"], d["uvp"], d["uvDA"], lvl0, sc.tw, sc.th, sc.C, 1)
     assert_close(N(out), ref, 1e-5, "texture no-mip fwd")
+    # backward (TexturemapNoMipBackwardKernel, texturemap.cu:286-319): bilinear texel grads of
+    # level 0 and the uv grad; no level, so no uvDA grad
+    import torch
+    rng = np.random.default_rng(6)
+    gout = rng.normal(size=(1, sc.H, sc.W, sc.C)).astype(F32)
+    gt_ref, guv_ref, _ = o_tex_bwd(d["rast"], d["uvp"], d["uvDA"], lvl0, sc.tw, sc.th, sc.C, 1, gout)
+    gflat = torch.zeros(lvl0.size, device="cuda")
+    glv = gpu.level_views(gflat, sc.tw, sc.th, sc.C, 1)
+    guv = torch.zeros(1, sc.H, sc.W, 2, device="cuda")
+    gpu.texture_backward(T(d["rast"]), T(d["uvp"]), T(d["uvDA"]), levels, T(gout), glv, guv, None)
+    assert_close(N(gflat), gt_ref, 1e-3, "texture no-mip grad texels", atol_frac=1e-5)
+    assert_close(N(guv), guv_ref, 1e-3, "texture no-mip grad uv", atol_frac=1e-5)
 
 
 def test_mip_build_fold_bitexact(gpu, orc):
