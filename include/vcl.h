@@ -390,6 +390,153 @@ int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stream);
 size_t vcl_mse_loss_noalloc_max_size(void);
 int vcl_mse_loss_backward_noalloc(const vcl_loss_params* p, vcl_stream_t stream);
 
+/* ---------------------------------------------------------------- Further losses
+ * SSIMLoss::backward (loss.h:18-20, loss.cu:93-174) on vcl_loss_params: per element (HWC with
+ * `depth` values per pixel, pidx = (px + width py) depth + pz) the SSIM of its (2k+1)^2 window,
+ * k = 5, clipped to the image, means and (co)variances over k^2 = 25 (loss.cu:155),
+ * c1 = 1e-4, c2 = 1e-3.  buffer: device scratch of vcl_ssim_workspace_size bytes whose first
+ * width*height*depth floats hold 1 - SSIM on return (the reference's loss.buffer);
+ * *loss = sum(1 - SSIM) / (4 size) (the reference divides by the byte size, loss.cu:173).
+ * gradMode VCL_SSIM_GRAD_REF: loss.cu:146-149 as written (the centre element's term of its own
+ * window, with 1 - SSIM where the derivative has SSIM); VCL_SSIM_GRAD_EXACT: the derivative of
+ * sum(1 - SSIM) over every window that holds the element.  grad is stored. */
+enum { VCL_SSIM_GRAD_REF = 0, VCL_SSIM_GRAD_EXACT = 1 };
+size_t vcl_ssim_workspace_size(int width, int height, int depth);
+int vcl_ssim_loss_backward(const vcl_loss_params* p, int gradMode, vcl_stream_t stream);
+/* MSELoss::textureloss (loss.cu:59-91): squared differences outside the masked block
+ * ((px < W/4 or W/2 <= px) and (py < W/4 or W/2 <= py) are zero), *loss = sum / (size / 2);
+ * no gradient.  buffer: vcl_loss_workspace_size(size) bytes plus size floats. */
+size_t vcl_texture_loss_workspace_size(int size);
+int vcl_mse_texture_loss(const vcl_loss_params* p, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Further optimizers
+ * SGD (optimize.h:36-48, optimize.cu:64-82): param += -eta * grad (AddNaNcheck).
+ * Nadam (optimize.h:71-98, optimize.cu:118-167): the kernel of Nadam::step with the host state
+ * as given (the caller advances it as Nadam::step does, optimize.cu:157-163).
+ * Optimizer::randomParams / clampParams (optimize.cu:30-60); the reference draws the seed with
+ * rand(), here it is an argument. */
+typedef struct vcl_sgd_hyper_params {
+    double eta;
+} vcl_sgd_hyper_params;
+typedef struct vcl_nadam_hyper_params {
+    double mupow, mupowt, mu, mu0, mu1, rho, alpha, mut0, mut1, rhot, eps;
+    float* m;
+    float* v;
+} vcl_nadam_hyper_params;
+int vcl_sgd_step(const vcl_optimize_params* p, const vcl_sgd_hyper_params* h, vcl_stream_t stream);
+int vcl_nadam_step(const vcl_optimize_params* p, const vcl_nadam_hyper_params* h, vcl_stream_t stream);
+int vcl_optimize_random_params(const vcl_optimize_params* p, uint32_t seed, float min, float max, vcl_stream_t stream);
+int vcl_optimize_clamp_params(const vcl_optimize_params* p, float min, float max, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Buffer utilities
+ * The kernels behind Buffer / Attribute / Texture / SGBuffer helpers (buffer.cu:3-110,
+ * :313-361, :410-447, :543-594, :669-811).  n counts floats; dst may equal src. */
+int vcl_buffer_linear(float* dst, const float* src, size_t n, float w, float b, vcl_stream_t stream);        /* linerKernel */
+int vcl_buffer_add_random(float* dst, size_t n, float min, float max, uint32_t seed, vcl_stream_t stream);    /* randomKernel */
+int vcl_buffer_clamp(float* dst, const float* src, size_t n, float min, float max, vcl_stream_t stream);    /* clampKernel */
+int vcl_buffer_step(float* dst, const float* src, size_t n, float threshold, vcl_stream_t stream);          /* stepKernel */
+/* normalizeKernel: each of `rows` rows of `dimention` floats times 1/|src row| (zero rows kept) */
+int vcl_buffer_normalize(float* dst, const float* src, int rows, int dimention, vcl_stream_t stream);
+/* setValueKernel: every row of `dimention` (<= 16) floats set to value (host array) */
+int vcl_buffer_set_value(float* dst, int rows, int dimention, const float* value, vcl_stream_t stream);
+/* sphericalRandomKernel: count unit directions (3 floats each) */
+int vcl_buffer_spherical_random(float* dst, int count, uint32_t seed, vcl_stream_t stream);
+/* Attribute::distanceError (buffer.cu:321-344): mean |predict - target| over num rows of
+ * `dimention`, into *out (device float), in a fixed order. */
+int vcl_attribute_distance_error(const float* predict, const float* target, int num, int dimention, float* out, vcl_stream_t stream);
+/* Texture::bilinearDownsampling (buffer.cu:410-447): levels 1.. from level 0 by the [1 3 3 1]/8
+ * separable filter, channels 0..min(channel, 3)-1 (the reference writes three). */
+int vcl_texture_bilinear_downsampling(const vcl_texture* t, vcl_stream_t stream);
+/* SGBuffer::bake (buffer.cu:734-761): level 0 of t = sum of the lobes over equirectangular
+ * directions (t->channel = the lobes' channel count); the other levels are cleared first, as
+ * Texture::liner(texture, 0, 0) does. */
+int vcl_sg_bake(const float* axis, const float* sharpness, const float* amplitude, int num, int channel, const vcl_texture* t,
+                vcl_stream_t stream);
+/* SGBuffer::normalize (buffer.cu:685-704): unit axes, sharpness in [0, 8], amplitude in [0, 1]. */
+int vcl_sg_normalize(float* axis, float* sharpness, float* amplitude, int num, int channel, vcl_stream_t stream);
+/* SGBufferGrad::disperse (buffer.cu:771-804): axes pushed apart; buffer holds num x 3 floats.
+ * Amplitude differences are taken from the amplitudes (the reference indexes sharpness). */
+int vcl_sg_disperse(float* axis, const float* sharpness, const float* amplitude, int num, int channel, float* buffer,
+                    vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Noise
+ * NoiseKernelParams (extention.h:5-14), Noise::forward (extention.cu:13-30):
+ * out = in + intensity (U(pidx, seed + i, 0xcafef00d) - in); the seed is an argument (the
+ * reference draws it with rand()). */
+typedef struct vcl_noise_params {
+    int width, height, depth, channel;
+    float intensity;
+    const float* in;
+    float* out;
+} vcl_noise_params;
+int vcl_noise_forward(const vcl_noise_params* p, uint32_t seed, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Filter
+ * FilterKernelParams / FilterKernelGradParams (filter.h:5-31), Filter::forward/backward
+ * (filter.cu:62-186): separable Gaussian blur, horizontal then vertical, taps outside the image
+ * dropped, k = min(ceil(4 sigma), 127).  sigma stays on the device and the weights are formed
+ * there (the reference copies sigma to the host each call), so no call synchronises.  The
+ * backward stores grad.in (the same blur of grad.out) and *grad.sigma = d loss / d sigma,
+ * computed exactly (the reference's FilterBackwardKernelFinal adds each pixel's term once per
+ * vertical tap and uses 2 / sigma for 2 m2 / sigma^3, filter.cu:141-157,182).
+ * filter: 256 floats of scratch; buf, bufx, bufy: width*height*depth*channel floats each. */
+typedef struct vcl_filter_params {
+    int width, height, depth, channel;
+    int k;                  /* unused: derived from sigma on the device */
+    const float* sigma;     /* device, 1 float */
+    float* filter;          /* device scratch, 256 floats */
+    const float* in;
+    float* buf;             /* device scratch */
+    float* out;
+} vcl_filter_params;
+typedef struct vcl_filter_grad_params {
+    float* in;              /* stored */
+    float* sigma;           /* device, 1 float, stored */
+    float* filter;          /* device scratch, 256 floats */
+    float* buf;
+    float* bufx;
+    float* bufy;
+    const float* out;
+} vcl_filter_grad_params;
+int vcl_filter_forward(const vcl_filter_params* p, vcl_stream_t stream);
+int vcl_filter_backward(const vcl_filter_params* p, const vcl_filter_grad_params* g, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- ComputeNormal
+ * ComputeNormalKernelParams (computenormal.h:5-19), ComputeNormal::forward (computenormal.cu:
+ * 17-43): per triangle the unit face normal added to its vertices with the weight acos(e1.e2)
+ * of each corner's edges -- the raw edges as written (VCL_NORMAL_ANGLE_REF,
+ * computenormal.cu:26-28) or the unit edges, i.e. the interior angle (VCL_NORMAL_ANGLE_EXACT) --
+ * then normalised.  The backward (a stub in the reference, computenormal.cu:56-65) is the exact
+ * derivative w.r.t. the positions.  sum: posNum x 3 floats of device scratch that keep the
+ * unnormalised sums for the backward. */
+enum { VCL_NORMAL_ANGLE_REF = 0, VCL_NORMAL_ANGLE_EXACT = 1 };
+typedef struct vcl_compute_normal_params {
+    int posNum;
+    const float* pos;
+    const uint32_t* vao;
+    int vaoNum;
+    const float* mat;       /* unused, kept for layout */
+    float* out;             /* posNum x 3 */
+    float* sum;             /* posNum x 3 device scratch */
+    int angleMode;
+} vcl_compute_normal_params;
+typedef struct vcl_compute_normal_grad_params {
+    float* pos;             /* posNum x 3, accumulated */
+    const float* out;       /* posNum x 3 */
+} vcl_compute_normal_grad_params;
+int vcl_compute_normal_forward(const vcl_compute_normal_params* p, vcl_stream_t stream);
+int vcl_compute_normal_backward(const vcl_compute_normal_params* p, const vcl_compute_normal_grad_params* g, vcl_stream_t stream);
+
+/* ---------------------------------------------------------------- Debug raster shadings
+ * Rasterize::wireframeinit / idhashinit + drawforward (rasterize.cpp:292-512), from a forward's
+ * output (rast, and DB for the wireframe) into out (P x 4):
+ * wireframe: e = 1 - min_i 1 / (1 + exp(16 (d_i - b_i))) over the barycentrics b = (u, v, 1-u-v)
+ * with d = (|db/dX| + |db/dY|) / 2 from the analytic DB (GL's fwidth differences 2 x 2 quads),
+ * colour (e, e, e, 1); idhash: the triangle id mixed with 0xbeefcade, 0xdeadcafe by the
+ * reference's integer mix, colour (a, b, c) & 0x7fffff / 0x7fffff, alpha 1.  Background 0. */
+int vcl_rasterize_wireframe(int width, int height, int depth, const float* rast, const float* rastDB, float* out, vcl_stream_t stream);
+int vcl_rasterize_idhash(int width, int height, int depth, const float* rast, float* out, vcl_stream_t stream);
+
 /* ---------------------------------------------------------------- Rotation / Camera (host)
  * Host math of Rotation and Camera (matrix.h:1-96, matrix.cu:3-239), usable without a GPU.
  * Quaternions are 4 floats in glm's storage order (x, y, z, w); matrices are 16 floats

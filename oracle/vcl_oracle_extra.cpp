@@ -149,7 +149,6 @@ extern "C" void orc_bilinear_downsampling(int tw, int th, int C, int L, float* l
         const float* src = levels + off;
         off += (size_t)w * h * C;
         float* dst = levels + off;
-        int sw = w;
         w >>= 1; h >>= 1;
         int nc = C < 3 ? C : 3;
         for (int py = 0; py < h; py++)
@@ -162,7 +161,7 @@ extern "C" void orc_bilinear_downsampling(int tw, int th, int C, int L, float* l
                 for (int x = xs; x <= xe; x++)
                     for (int y = ys; y <= ye; y++) {
                         float f = filt[x + 1] * filt[y + 1];
-                        size_t q = (size_t)(x2 + x) + (size_t)sw * (y2 + y);
+                        size_t q = (size_t)(x2 + x) + (size_t)W2 * (y2 + y);  // the reference strides by 2w (buffer.cu:429)
                         for (int i = 0; i < nc; i++) dst[p * C + i] += src[q * C + i] * f;
                     }
             }

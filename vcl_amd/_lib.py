@@ -118,6 +118,36 @@ class RotationAdam(C.Structure):
                 ("m", C.c_float * 4), ("v", C.c_float * 4)]
 
 
+class SgdHyperParams(C.Structure):
+    _fields_ = [("eta", cd)]
+
+
+class NadamHyperParams(C.Structure):
+    _fields_ = [(n, cd) for n in ("mupow", "mupowt", "mu", "mu0", "mu1", "rho", "alpha", "mut0", "mut1", "rhot", "eps")] + [
+        ("m", vp), ("v", vp)]
+
+
+class NoiseParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("channel", ci), ("intensity", cf), ("in_", vp), ("out", vp)]
+
+
+class FilterParams(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("depth", ci), ("channel", ci), ("k", ci), ("sigma", vp), ("filter", vp), ("in_", vp),
+                ("buf", vp), ("out", vp)]
+
+
+class FilterGradParams(C.Structure):
+    _fields_ = [("in_", vp), ("sigma", vp), ("filter", vp), ("buf", vp), ("bufx", vp), ("bufy", vp), ("out", vp)]
+
+
+class ComputeNormalParams(C.Structure):
+    _fields_ = [("posNum", ci), ("pos", vp), ("vao", vp), ("vaoNum", ci), ("mat", vp), ("out", vp), ("sum", vp), ("angleMode", ci)]
+
+
+class ComputeNormalGradParams(C.Structure):
+    _fields_ = [("pos", vp), ("out", vp)]
+
+
 class FitDesc(C.Structure):
     _fields_ = [("width", ci), ("height", ci), ("views", ci), ("V", ci), ("T", ci), ("texwidth", ci), ("texheight", ci),
                 ("channel", ci), ("miplevel", ci), ("sgnum", ci),
@@ -178,6 +208,33 @@ SIGNATURES = {
     "vcl_mse_loss_backward": ([P_(LossParams), vp], ci),
     "vcl_mse_loss_noalloc_max_size": ([], csz),
     "vcl_mse_loss_backward_noalloc": ([P_(LossParams), vp], ci),
+    "vcl_ssim_workspace_size": ([ci, ci, ci], csz),
+    "vcl_ssim_loss_backward": ([P_(LossParams), ci, vp], ci),
+    "vcl_texture_loss_workspace_size": ([ci], csz),
+    "vcl_mse_texture_loss": ([P_(LossParams), vp], ci),
+    "vcl_sgd_step": ([P_(OptimizeParams), P_(SgdHyperParams), vp], ci),
+    "vcl_nadam_step": ([P_(OptimizeParams), P_(NadamHyperParams), vp], ci),
+    "vcl_optimize_random_params": ([P_(OptimizeParams), C.c_uint32, cf, cf, vp], ci),
+    "vcl_optimize_clamp_params": ([P_(OptimizeParams), cf, cf, vp], ci),
+    "vcl_buffer_linear": ([vp, vp, csz, cf, cf, vp], ci),
+    "vcl_buffer_add_random": ([vp, csz, cf, cf, C.c_uint32, vp], ci),
+    "vcl_buffer_clamp": ([vp, vp, csz, cf, cf, vp], ci),
+    "vcl_buffer_step": ([vp, vp, csz, cf, vp], ci),
+    "vcl_buffer_normalize": ([vp, vp, ci, ci, vp], ci),
+    "vcl_buffer_set_value": ([vp, ci, ci, vp, vp], ci),
+    "vcl_buffer_spherical_random": ([vp, ci, C.c_uint32, vp], ci),
+    "vcl_attribute_distance_error": ([vp, vp, ci, ci, vp, vp], ci),
+    "vcl_texture_bilinear_downsampling": ([P_(Texture), vp], ci),
+    "vcl_sg_bake": ([vp, vp, vp, ci, ci, P_(Texture), vp], ci),
+    "vcl_sg_normalize": ([vp, vp, vp, ci, ci, vp], ci),
+    "vcl_sg_disperse": ([vp, vp, vp, ci, ci, vp, vp], ci),
+    "vcl_noise_forward": ([P_(NoiseParams), C.c_uint32, vp], ci),
+    "vcl_filter_forward": ([P_(FilterParams), vp], ci),
+    "vcl_filter_backward": ([P_(FilterParams), P_(FilterGradParams), vp], ci),
+    "vcl_compute_normal_forward": ([P_(ComputeNormalParams), vp], ci),
+    "vcl_compute_normal_backward": ([P_(ComputeNormalParams), P_(ComputeNormalGradParams), vp], ci),
+    "vcl_rasterize_wireframe": ([ci, ci, ci, vp, vp, vp, vp], ci),
+    "vcl_rasterize_idhash": ([ci, ci, ci, vp, vp, vp], ci),
     "vcl_rotation_matrix": ([vp, vp], ci),
     "vcl_view_matrix": ([vp, vp, vp], ci),
     "vcl_look_at_quat": ([vp, vp, vp, vp], ci),

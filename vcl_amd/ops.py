@@ -285,6 +285,179 @@ def mse_loss(pred, target):
     return grad, loss
 
 
+# ---------------------------------------------------------------- further losses (loss.h)
+def ssim_loss(pred, target, grad_mode=0):
+    """SSIMLoss::backward over HWC tensors (H, W, D): returns (grad, loss, 1 - SSIM map).
+    grad_mode 0: the reference's gradient formula; 1: the exact gradient of sum(1 - SSIM)."""
+    H, W, D = pred.shape
+    n = pred.numel()
+    grad = torch.empty_like(pred)
+    ws = torch.empty(int(_lib().vcl_ssim_workspace_size(W, H, D)), dtype=torch.uint8, device="cuda")
+    loss = empty(1)
+    p = L.LossParams(_p(pred).value, _p(target).value, _p(grad).value, _p(ws).value, W, H, D, n, _p(loss).value)
+    L.check(_lib().vcl_ssim_loss_backward(C.byref(p), int(grad_mode), _s()), "ssim")
+    return grad, loss, ws[: 4 * n].view(torch.float32).view(H, W, D)
+
+
+def texture_loss(pred, target):
+    """MSELoss::textureloss over (H, W, D): returns (loss, per-element map)."""
+    H, W, D = pred.shape
+    n = pred.numel()
+    ws = torch.empty(int(_lib().vcl_texture_loss_workspace_size(n)), dtype=torch.uint8, device="cuda")
+    loss = empty(1)
+    p = L.LossParams(_p(pred).value, _p(target).value, None, _p(ws).value, W, H, D, n, _p(loss).value)
+    L.check(_lib().vcl_mse_texture_loss(C.byref(p), _s()), "texture_loss")
+    return loss, ws[: 4 * n].view(torch.float32).view(H, W, D)
+
+
+# ---------------------------------------------------------------- further optimizers (optimize.h)
+def _opt(param, grad=None):
+    return L.OptimizeParams(_p(param).value, _p(grad).value if grad is not None else None, param.numel(), param.numel(), 1, 1)
+
+
+def sgd_step(param, grad, eta):
+    p = _opt(param, grad)
+    L.check(_lib().vcl_sgd_step(C.byref(p), C.byref(L.SgdHyperParams(eta)), _s()), "sgd")
+
+
+def nadam_step(param, grad, m, v, mu, mu0, mu1, rho, alpha, mut0, mut1, rhot, eps):
+    p = _opt(param, grad)
+    h = L.NadamHyperParams(0.0, 0.0, mu, mu0, mu1, rho, alpha, mut0, mut1, rhot, eps, _p(m).value, _p(v).value)
+    L.check(_lib().vcl_nadam_step(C.byref(p), C.byref(h), _s()), "nadam")
+
+
+def random_params(param, seed, lo, hi):
+    p = _opt(param)
+    L.check(_lib().vcl_optimize_random_params(C.byref(p), int(seed), lo, hi, _s()), "random_params")
+
+
+def clamp_params(param, lo, hi):
+    p = _opt(param)
+    L.check(_lib().vcl_optimize_clamp_params(C.byref(p), lo, hi, _s()), "clamp_params")
+
+
+# ---------------------------------------------------------------- buffer utilities (buffer.h)
+def buffer_linear(dst, src, w, b):
+    L.check(_lib().vcl_buffer_linear(_p(dst), _p(src), dst.numel(), w, b, _s()), "linear")
+
+
+def buffer_add_random(dst, lo, hi, seed):
+    L.check(_lib().vcl_buffer_add_random(_p(dst), dst.numel(), lo, hi, int(seed), _s()), "add_random")
+
+
+def buffer_clamp(dst, src, lo, hi):
+    L.check(_lib().vcl_buffer_clamp(_p(dst), _p(src), dst.numel(), lo, hi, _s()), "clamp")
+
+
+def buffer_step(dst, src, thr):
+    L.check(_lib().vcl_buffer_step(_p(dst), _p(src), dst.numel(), thr, _s()), "step")
+
+
+def buffer_normalize(dst, src, dim):
+    L.check(_lib().vcl_buffer_normalize(_p(dst), _p(src), dst.numel() // dim, dim, _s()), "normalize")
+
+
+def buffer_set_value(dst, value):
+    v = np.ascontiguousarray(value, np.float32)
+    L.check(_lib().vcl_buffer_set_value(_p(dst), dst.numel() // v.size, v.size, v.ctypes.data, _s()), "set_value")
+
+
+def buffer_spherical_random(dst, seed):
+    L.check(_lib().vcl_buffer_spherical_random(_p(dst), dst.numel() // 3, int(seed), _s()), "spherical_random")
+
+
+def distance_error(pred, target):
+    out = empty(1)
+    L.check(_lib().vcl_attribute_distance_error(_p(pred), _p(target), pred.shape[0], pred.shape[1], _p(out), _s()), "distance_error")
+    return out
+
+
+def bilinear_downsampling(levels):
+    t = _texture_struct(levels)
+    L.check(_lib().vcl_texture_bilinear_downsampling(C.byref(t), _s()), "bilinear_downsampling")
+
+
+def sg_bake(axis, sharp, amp, levels):
+    t = _texture_struct(levels)
+    L.check(_lib().vcl_sg_bake(_p(axis), _p(sharp), _p(amp), sharp.numel(), amp.shape[1], C.byref(t), _s()), "sg_bake")
+
+
+def sg_normalize(axis, sharp, amp):
+    L.check(_lib().vcl_sg_normalize(_p(axis), _p(sharp), _p(amp), sharp.numel(), amp.shape[1], _s()), "sg_normalize")
+
+
+def sg_disperse(axis, sharp, amp):
+    buf = empty(axis.shape)
+    L.check(_lib().vcl_sg_disperse(_p(axis), _p(sharp), _p(amp), sharp.numel(), amp.shape[1], _p(buf), _s()), "sg_disperse")
+    return buf
+
+
+# ---------------------------------------------------------------- Noise (extention.h), Filter (filter.h)
+def noise(inp, intensity, seed):
+    depth, H, W, C_ = inp.shape
+    out = torch.empty_like(inp)
+    p = L.NoiseParams(W, H, depth, C_, intensity, _p(inp).value, _p(out).value)
+    L.check(_lib().vcl_noise_forward(C.byref(p), int(seed), _s()), "noise")
+    return out
+
+
+def _filter_params(inp, sigma_dev, out):
+    depth, H, W, C_ = inp.shape
+    filt = empty(256)
+    buf = torch.empty_like(inp)
+    return L.FilterParams(W, H, depth, C_, 0, _p(sigma_dev).value, _p(filt).value, _p(inp).value, _p(buf).value, _p(out).value), (filt, buf)
+
+
+def filter_forward(inp, sigma_dev):
+    out = torch.empty_like(inp)
+    p, keep = _filter_params(inp, sigma_dev, out)
+    L.check(_lib().vcl_filter_forward(C.byref(p), _s()), "filter")
+    return out
+
+
+def filter_backward(inp, sigma_dev, gout):
+    """Returns (grad of the input, d loss / d sigma as a 1-element tensor)."""
+    p, keep = _filter_params(inp, sigma_dev, torch.empty_like(inp))
+    gin = torch.empty_like(inp)
+    gsig = empty(1)
+    gf, b0, b1, b2 = empty(256), torch.empty_like(inp), torch.empty_like(inp), torch.empty_like(inp)
+    g = L.FilterGradParams(_p(gin).value, _p(gsig).value, _p(gf).value, _p(b0).value, _p(b1).value, _p(b2).value, _p(gout).value)
+    L.check(_lib().vcl_filter_backward(C.byref(p), C.byref(g), _s()), "filter_backward")
+    torch.cuda.current_stream().synchronize()
+    return gin, gsig
+
+
+# ---------------------------------------------------------------- ComputeNormal (computenormal.h)
+def compute_normal(pos, idx, mode=0, gout=None):
+    """Vertex normals; with gout also returns the position grad."""
+    V = pos.shape[0]
+    out = empty(V, 3)
+    ssum = empty(V, 3)
+    p = L.ComputeNormalParams(V, _p(pos).value, _p(idx).value, idx.shape[0], None, _p(out).value, _p(ssum).value, int(mode))
+    L.check(_lib().vcl_compute_normal_forward(C.byref(p), _s()), "compute_normal")
+    if gout is None:
+        return out
+    gpos = zeros(V, 3)
+    g = L.ComputeNormalGradParams(_p(gpos).value, _p(gout).value)
+    L.check(_lib().vcl_compute_normal_backward(C.byref(p), C.byref(g), _s()), "compute_normal_backward")
+    return out, gpos
+
+
+# ---------------------------------------------------------------- debug raster shadings (rasterize.h)
+def rasterize_wireframe(rast, db):
+    depth, H, W = rast.shape[:3]
+    out = empty(depth, H, W, 4)
+    L.check(_lib().vcl_rasterize_wireframe(W, H, depth, _p(rast), _p(db), _p(out), _s()), "wireframe")
+    return out
+
+
+def rasterize_idhash(rast):
+    depth, H, W = rast.shape[:3]
+    out = empty(depth, H, W, 4)
+    L.check(_lib().vcl_rasterize_idhash(W, H, depth, _p(rast), _p(out), _s()), "idhash")
+    return out
+
+
 # ---------------------------------------------------------------- Fit engine
 def fit_desc(scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_quarter=True, fused=False, views=None):
     """vcl_fit_desc of a Scene; returns (desc, keep) where `keep` holds the host arrays the
