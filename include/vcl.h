@@ -537,6 +537,41 @@ int vcl_compute_normal_backward(const vcl_compute_normal_params* p, const vcl_co
 int vcl_rasterize_wireframe(int width, int height, int depth, const float* rast, const float* rastDB, float* out, vcl_stream_t stream);
 int vcl_rasterize_idhash(int width, int height, int depth, const float* rast, float* out, vcl_stream_t stream);
 
+/* ---------------------------------------------------------------- On-disk formats (host)
+ * Host-only readers and writers, usable without a GPU; returned arrays are malloc'd and freed
+ * by vcl_obj_free / vcl_image_free.  include/vcl_layers.hpp uploads them for
+ * Attribute::loadOBJ, Texture::loadBMP and SGBuffer::loadTXT. */
+typedef struct vcl_obj_mesh {
+    int posNum, texelNum, normalNum, indexNum;
+    float* pos;              /* posNum x 3 */
+    float* texel;            /* texelNum x 2, or NULL */
+    float* normal;           /* normalNum x 3, or NULL */
+    uint32_t* posIndex;      /* indexNum x 3 (0-based) */
+    uint32_t* texelIndex;    /* indexNum x 3, or NULL */
+    uint32_t* normalIndex;   /* indexNum x 3, or NULL */
+} vcl_obj_mesh;
+/* Attribute::loadOBJ (buffer.cu:203-306): token-driven v / vt / vn / f records, faces in the
+ * form fixed by the records read so far (v/vt/vn, v/vt, v//vn or v); VCL_ERR_INVALID when the
+ * file cannot be opened or a record does not parse (the reference prints and returns). */
+int vcl_load_obj(const char* path, vcl_obj_mesh* out);
+void vcl_obj_free(vcl_obj_mesh* m);
+typedef struct vcl_image {
+    int width, height, channel;
+    float* data;             /* width x height x channel, row 0 at the bottom */
+} vcl_image;
+/* Texture::loadBMP's decode (buffer.cu:486-541): header fields at 0x0A, 0x12, 0x16, 0x1C, 0x22;
+ * pixel bytes read contiguously (no row padding), per pixel in reversed order (BGR -> RGB),
+ * divided by 255. */
+int vcl_load_bmp(const char* path, vcl_image* out);
+void vcl_image_free(vcl_image* im);
+/* The exact inverse of vcl_load_bmp's decode (values clamped to [0, 1], times 255, rounded);
+ * channel 1, 3 or 4. */
+int vcl_save_bmp(const char* path, const float* data, int width, int height, int channel);
+/* Portable float map ("Pf" for 1 channel, "PF" otherwise), rows bottom first. */
+int vcl_save_pfm(const char* path, const float* data, int width, int height, int channel);
+/* SGBuffer::loadTXT (buffer.cu:706-732): num lines of axis (3), sharpness, amplitude (3). */
+int vcl_load_sg_txt(const char* path, int num, float* axis, float* sharpness, float* amplitude);
+
 /* ---------------------------------------------------------------- Rotation / Camera (host)
  * Host math of Rotation and Camera (matrix.h:1-96, matrix.cu:3-239), usable without a GPU.
  * Quaternions are 4 floats in glm's storage order (x, y, z, w); matrices are 16 floats

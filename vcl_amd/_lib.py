@@ -148,6 +148,15 @@ class ComputeNormalGradParams(C.Structure):
     _fields_ = [("pos", vp), ("out", vp)]
 
 
+class ObjMesh(C.Structure):
+    _fields_ = [("posNum", ci), ("texelNum", ci), ("normalNum", ci), ("indexNum", ci), ("pos", C.POINTER(cf)), ("texel", C.POINTER(cf)),
+                ("normal", C.POINTER(cf)), ("posIndex", C.POINTER(C.c_uint32)), ("texelIndex", C.POINTER(C.c_uint32)), ("normalIndex", C.POINTER(C.c_uint32))]
+
+
+class Image(C.Structure):
+    _fields_ = [("width", ci), ("height", ci), ("channel", ci), ("data", C.POINTER(cf))]
+
+
 class FitDesc(C.Structure):
     _fields_ = [("width", ci), ("height", ci), ("views", ci), ("V", ci), ("T", ci), ("texwidth", ci), ("texheight", ci),
                 ("channel", ci), ("miplevel", ci), ("sgnum", ci),
@@ -235,6 +244,13 @@ SIGNATURES = {
     "vcl_compute_normal_backward": ([P_(ComputeNormalParams), P_(ComputeNormalGradParams), vp], ci),
     "vcl_rasterize_wireframe": ([ci, ci, ci, vp, vp, vp, vp], ci),
     "vcl_rasterize_idhash": ([ci, ci, ci, vp, vp, vp], ci),
+    "vcl_load_obj": ([C.c_char_p, P_(ObjMesh)], ci),
+    "vcl_obj_free": ([P_(ObjMesh)], None),
+    "vcl_load_bmp": ([C.c_char_p, P_(Image)], ci),
+    "vcl_image_free": ([P_(Image)], None),
+    "vcl_save_bmp": ([C.c_char_p, vp, ci, ci, ci], ci),
+    "vcl_save_pfm": ([C.c_char_p, vp, ci, ci, ci], ci),
+    "vcl_load_sg_txt": ([C.c_char_p, ci, vp, vp, vp], ci),
     "vcl_rotation_matrix": ([vp, vp], ci),
     "vcl_view_matrix": ([vp, vp, vp], ci),
     "vcl_look_at_quat": ([vp, vp, vp, vp], ci),
