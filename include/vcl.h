@@ -631,7 +631,10 @@ typedef struct vcl_fit_desc {
     float roughness, ior;
     double lr, beta1, beta2, eps;
     int fold_quarter;                 /* 1: exact mip-grad fold (default) */
-    int fused;                        /* 0: reference op-by-op sequence; 1: fused kernels */
+    int op_by_op;                     /* 0: fused passes (interpolate + texture, NormalAxis +
+                                         ViewAxis + SGSpecular, texture + interpolate backward);
+                                         1: the reference's op-by-op sequence of separate launches.
+                                         Both give the same results. */
 } vcl_fit_desc;
 
 typedef struct vcl_fit vcl_fit;       /* opaque */
@@ -689,8 +692,15 @@ int vcl_fit_get_tuning(const vcl_fit* f, vcl_tuning* t);
  * then raster tile x bin chunk, then texel-splat rows; every candidate from the same snapshot
  * of the request's parameters and Adam state, which is restored afterwards).  A candidate
  * replaces the incumbent only when at least 3 % faster.  Keeps the winner on the request and
- * returns it with its ms per step.  Allocates a temporary snapshot of the persistent state. */
+ * returns it with its ms per step.  Allocates a temporary snapshot of the persistent state and
+ * synchronises the device first. */
 int vcl_fit_autotune(vcl_fit* f, int steps, vcl_tuning* best, double* best_ms);
+/* The same without allocation or device-wide synchronisation: the snapshot lives in caller
+ * memory of at least vcl_fit_autotune_scratch_bytes(f) bytes, the trials run on `stream` (after
+ * the work already queued on it), and only that stream is synchronised at the end.  The
+ * server uses it with memory from its budgeted pool. */
+size_t vcl_fit_autotune_scratch_bytes(const vcl_fit* f);
+int vcl_fit_autotune_in(vcl_fit* f, int steps, void* scratch, size_t scratch_bytes, vcl_stream_t stream, vcl_tuning* best, double* best_ms);
 
 /* ---------------------------------------------------------------- Serving
  * Continuous batching of fit requests on one GPU (no counterpart in the reference, which runs

@@ -4,9 +4,11 @@
 //
 // usage: preset_fit <scene.bin> <out.bin> <steps>
 // scene.bin (little endian): int32 W,H,V,T,tw,th,C,L,K; float32 pos[V*3], normals[V*3], uv[V*2];
-//   uint32 idx[T*3]; float32 tex0[tw*th*C], axis[K*3], sharp[K], amp[K*C], mat[16], view[16],
-//   proj[16], rot[16], target[H*W*C]; float32 rough, ior
-// out.bin: float32 loss per step, image[H*W*C] of the last step, pos[V*3] after the last step
+//   uint32 idx[T*3]; float32 tex0[tw*th*C], axis[K*3], sharp[K], amp[K*C], camera[10] (eye 3,
+//   center 3, size, aspect, znear, zfar), target[H*W*C]; float32 rough, ior
+// out.bin: float32 loss per step, then from the first step: image[H*W*C], the gradients before
+//   the updates -- positions [V*3], texture level 0 after gradSumup [tw*th*C], lobe axis [K*3],
+//   sharpness [K], amplitude [K*C] -- then pos[V*3] after the last step
 #include <cstdio>
 #include <vector>
 
@@ -30,7 +32,7 @@ int main(int argc, char** argv) {
     auto pos = rd<float>(f, (size_t)V * 3), normals = rd<float>(f, (size_t)V * 3), uv = rd<float>(f, (size_t)V * 2);
     auto idx = rd<unsigned>(f, (size_t)T * 3);
     auto tex0 = rd<float>(f, (size_t)tw * th * C), axis = rd<float>(f, (size_t)K * 3), sharp = rd<float>(f, K), amp = rd<float>(f, (size_t)K * C);
-    auto mat = rd<float>(f, 16), view = rd<float>(f, 16), proj = rd<float>(f, 16), rotm = rd<float>(f, 16);
+    auto camera = rd<float>(f, 10);
     auto target = rd<float>(f, (size_t)W * H * C);
     auto tail = rd<float>(f, 2);
     std::fclose(f);
@@ -57,10 +59,14 @@ int main(int argc, char** argv) {
     SGBufferGrad::init(sgbuf, K, C);
     SGBuffer::copy(sgbuf, axis.data(), sharp.data(), amp.data());
 
+    // model rotation and camera (matrix.h), as preset_primitives.cpp:56-58 sets them up
     RotationParams rot;
-    RotationParams::init(rot, rotm.data());
+    Rotation::init(rot);
     CameraParams cam;
-    CameraParams::init(cam, view.data(), proj.data(), mat.data());
+    Camera::init(cam, rot, vec3(camera[0], camera[1], camera[2]), vec3(camera[3], camera[4], camera[5]), vec3(0.f, 1.f, 0.f), camera[6],
+                 camera[7], camera[8], camera[9]);
+    Rotation::forward(rot);
+    Camera::forward(cam);
 
     ProjectGradParams proj_p;
     Project::init(proj_p, cam.kernel.out, pos_a, true);
@@ -105,7 +111,11 @@ int main(int argc, char** argv) {
     Optimizer::init(adam_amp, ((SGBuffer&)sgbuf).amplitude, sgbuf.amplitude, K * C, K, C, 1);
     Adam::setHyperParams(adam_amp, 1e-2, .9, .999, 1e-8);
 
-    std::vector<float> losses;
+    std::vector<float> losses, first_img((size_t)W * H * C), g_pos((size_t)V * 3), g_tex((size_t)tw * th * C), g_axis((size_t)K * 3),
+        g_sharp(K), g_amp((size_t)K * C);
+    auto down = [](std::vector<float>& h, const float* d) {
+        VCL_HIP_LAYER_CHECK(hipMemcpy(h.data(), d, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    };
     for (int s = 0; s < steps; s++) {
         // ---- update (preset_primitives.cpp:167-207 call order)
         Project::forward(proj_p);
@@ -125,6 +135,10 @@ int main(int argc, char** argv) {
         Rasterize::backward(rast);
         Project::backward(proj_p);
         TextureGrad::gradSumup(diffusemap);
+        if (s == 0) {
+            down(first_img, aa.kernel.out); down(g_pos, pos_a.grad); down(g_tex, diffusemap.grad[0]);
+            down(g_axis, sgbuf.axis); down(g_sharp, sgbuf.sharpness); down(g_amp, sgbuf.amplitude);
+        }
         Adam::step(adam_tex);
         TextureGrad::clear(diffusemap);
         Texture::clamp(diffusemap, 0.f, 1.f);
@@ -136,14 +150,12 @@ int main(int argc, char** argv) {
         SGBufferGrad::clear(sgbuf);
         losses.push_back(Loss::loss(loss));
     }
-    std::vector<float> img((size_t)W * H * C), posout((size_t)V * 3);
-    VCL_HIP_LAYER_CHECK(hipMemcpy(img.data(), aa.kernel.out, img.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::vector<float> posout((size_t)V * 3);
     VCL_HIP_LAYER_CHECK(hipMemcpy(posout.data(), pos_a.vbo, posout.size() * sizeof(float), hipMemcpyDeviceToHost));
     FILE* o = std::fopen(argv[2], "wb");
     if (!o) { std::perror("out"); return 2; }
     std::fwrite(losses.data(), sizeof(float), losses.size(), o);
-    std::fwrite(img.data(), sizeof(float), img.size(), o);
-    std::fwrite(posout.data(), sizeof(float), posout.size(), o);
+    for (auto* v : {&first_img, &g_pos, &g_tex, &g_axis, &g_sharp, &g_amp, &posout}) std::fwrite(v->data(), sizeof(float), v->size(), o);
     std::fclose(o);
     std::printf("preset_fit: %d steps, first loss %.6e last loss %.6e\n", steps, losses.empty() ? 0.f : losses.front(),
                 losses.empty() ? 0.f : losses.back());

@@ -1,60 +1,162 @@
-"""The C++ drop-in layer (include/vcl_layers.hpp) driven in the reference preset's call order
-(tests/cpp/preset_fit.cpp) agrees with the fit engine and the CPU oracle."""
+"""The C++ drop-in layer (include/vcl_layers.hpp) driven by host programs in the reference
+presets' call order, on the GPU, against the CPU oracle.
+
+- tests/cpp/preset_fit: the preset_primitives.cpp call order over a trainable mesh, mip texture
+  and SG lobes, with the Rotation / Camera classes and the reference's gradSumup (x1 fold).  Its
+  first step matches the oracle's step: loss and image within 1e-5, the gradients of positions,
+  texture (level 0 after the fold) and lobes within 1e-3.
+- tests/cpp/preset_pose: preset_pose.cpp (OBJ mesh coloured by position, SSIM loss, the
+  rotation's gradient through Project -> Camera -> Rotation, headless display).  Its first loss
+  and target image match the oracle's chain (project, rasterize, interpolate, antialias, SSIM)
+  for the rotations it drew, within 1e-5; the headless window holds the two images side by side;
+  with the exact Camera / Rotation gradients the pose fit lowers the loss."""
+import ctypes as C
+import math
 import os
 import subprocess
-import sys
 
 import numpy as np
 import pytest
 
-from helpers import F32, forward_inputs
+import oracle as O
+from helpers import F32, OracleStep, assert_close, o_aa, o_interp, o_project, o_raster
+from vcl_amd import _lib as L
 from vcl_amd import scene as S
 
 pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "tests", "cpp", "preset_fit")
+FIT_EXE = os.path.join(ROOT, "tests", "cpp", "preset_fit")
+POSE_EXE = os.path.join(ROOT, "tests", "cpp", "preset_pose")
 
 
-def write_scene(path, sc, target):
-    cam = sc.cams[0]
-    with open(path, "wb") as f:
+def camera_from_abi(q_xyzw, eye, size, aspect, znear, zfar, R16=None):
+    """Matrices by the library's own host math (the same code the C++ classes run)."""
+    lib = L.load()
+    q = np.ascontiguousarray(q_xyzw, F32)
+    e = np.ascontiguousarray(eye, F32)
+    R = np.ascontiguousarray(np.eye(4, dtype=F32).reshape(16) if R16 is None else R16, F32)
+    V, P, M = (np.zeros(16, F32) for _ in range(3))
+    assert lib.vcl_camera_matrices(q.ctypes.data, e.ctypes.data, size, aspect, znear, zfar, R.ctypes.data, V.ctypes.data, P.ctypes.data,
+                                   M.ctypes.data) == 0
+    p3 = P.reshape(4, 4)[:3, :3].astype(np.float64)
+    v3 = V.reshape(4, 4)[:3, :3].astype(np.float64)
+    pvinv = np.linalg.inv((v3 @ p3).T).T  # inverse(mat3(P) * mat3(V)) in column-major storage
+    return S.Camera(M, pvinv.reshape(9).astype(F32), P.reshape(4, 4), V.reshape(4, 4))
+
+
+def look_at(eye, center, up=(0, 1, 0)):
+    q = np.zeros(4, F32)
+    e, c, u = (np.ascontiguousarray(a, F32) for a in (eye, center, up))
+    assert L.load().vcl_look_at_quat(e.ctypes.data, c.ctypes.data, u.ctypes.data, q.ctypes.data) == 0
+    return q
+
+
+def test_fit_program_first_step_matches_oracle(gpu, orc, tmp_path):
+    if not os.path.exists(FIT_EXE):
+        pytest.skip("tests/cpp/preset_fit not built (run __graft_entry__.build())")
+    size, eye = float(F32(0.1) * F32(math.tan(math.radians(22.5)))), (0.0, 0.0, 3.0)
+    cam = camera_from_abi(look_at(eye, (0, 0, 0)), eye, size, 1.0, 0.1, 10.0)
+    sc = S.config1_scene(1, W=96)
+    sc.cams = [cam]
+    tsc = S.config1_scene(2, W=96)
+    tsc.cams = [cam]
+    target = OracleStep(tsc, np.zeros((1, 96, 96, 4), F32)).image[0].copy()
+    scene_bin, out_bin = tmp_path / "scene.bin", tmp_path / "out.bin"
+    with open(scene_bin, "wb") as f:
         np.array([sc.W, sc.H, sc.V, sc.T, sc.tw, sc.th, sc.C, sc.L, sc.K], np.int32).tofile(f)
         for a in (sc.pos, sc.normals, sc.uv):
             np.ascontiguousarray(a, F32).tofile(f)
         np.ascontiguousarray(sc.idx, np.uint32).tofile(f)
         np.ascontiguousarray(sc.tex[: sc.tw * sc.th * sc.C], F32).tofile(f)
-        for a in (sc.axis, sc.sharp, sc.amp, cam.mat, cam.view.reshape(16), cam.proj.reshape(16), np.eye(4, dtype=F32).reshape(16), target):
+        for a in (sc.axis, sc.sharp, sc.amp, np.array(list(eye) + [0, 0, 0, size, 1.0, 0.1, 10.0]), target):
             np.ascontiguousarray(a, F32).tofile(f)
         np.array([sc.rough, sc.ior], F32).tofile(f)
-
-
-def test_layers_program_matches_engine(gpu, tmp_path):
-    if not os.path.exists(EXE):
-        pytest.skip("tests/cpp/preset_fit not built (run __graft_entry__.build())")
-    sc = S.config1_scene(1, W=96)
-    target = forward_inputs(S.config1_scene(2, W=96))["image"][0].copy()
-    scene_bin, out_bin = tmp_path / "scene.bin", tmp_path / "out.bin"
-    write_scene(scene_bin, sc, target)
     steps = 3
-    print("running", EXE, flush=True)
-    r = subprocess.run([EXE, str(scene_bin), str(out_bin), str(steps)], capture_output=True, text=True, timeout=120)
+    print("running", FIT_EXE, flush=True)
+    r = subprocess.run([FIT_EXE, str(scene_bin), str(out_bin), str(steps)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-3000:]
     print(r.stdout.strip())
     out = np.fromfile(out_bin, F32)
-    losses = out[:steps]
-    img = out[steps: steps + sc.H * sc.W * sc.C].reshape(sc.H, sc.W, sc.C)
-    # the engine (fused texture update) on the same request
-    fit = gpu.Fit(sc, target=target)
-    elosses = []
-    for _ in range(steps):
-        fit.step()
-        elosses.append(float(fit.read("loss", (1,))[0]))
-    eimg = fit.read("image", img.shape)
-    fit.close()
-    print("layers losses", losses, "engine losses", elosses)
-    np.testing.assert_allclose(losses[0], elosses[0], rtol=1e-5)
-    np.testing.assert_allclose(losses, elosses, rtol=2e-3)
+    sizes = [steps, sc.H * sc.W * sc.C, sc.V * 3, sc.tw * sc.th * sc.C, sc.K * 3, sc.K, sc.K * sc.C, sc.V * 3]
+    parts = np.split(out, np.cumsum(sizes)[:-1])
+    losses, img, g_pos, g_tex, g_axis, g_sharp, g_amp, _ = parts
+    # the oracle's step with the reference's x1 mip-grad fold (the drop-in gradSumup default)
+    orc_ = OracleStep(sc, target[None], fold_quarter=0)
+    print("program losses", losses, "oracle first loss", orc_.loss)
+    assert abs(losses[0] - orc_.loss) <= 1e-5 * abs(orc_.loss)
+    assert_close(img.reshape(sc.H, sc.W, sc.C), orc_.image[0], 1e-5, "first image")
+    n_tex = sc.tw * sc.th * sc.C
+    for what, got, name, n in (("positions", g_pos, "gpos", sc.V * 3), ("texture level 0", g_tex, "gtex", n_tex),
+                               ("lobe axes", g_axis, "gaxis", sc.K * 3), ("lobe sharpness", g_sharp, "gsharp", sc.K),
+                               ("lobe amplitude", g_amp, "gamp", sc.K * sc.C)):
+        total = sum((sc.tw >> l) * (sc.th >> l) * sc.C for l in range(sc.L)) if name == "gtex" else n
+        ref = orc_.buf(name, (total,))[:n]
+        assert_close(got, ref, 1e-3, f"grad {what}", atol_frac=1e-4)
     assert losses[-1] < losses[0]
-    assert float(np.abs(img - eimg).max()) < 0.05 * float(np.abs(eimg).max())
-    sys.stdout.flush()
+    orc_.close()
+
+
+def _cube_obj(path):
+    v = [(-1, -1, -1), (1, -1, -1), (1, 1, -1), (-1, 1, -1), (-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)]
+    f = [(1, 3, 2), (1, 4, 3), (5, 6, 7), (5, 7, 8), (1, 2, 6), (1, 6, 5), (2, 3, 7), (2, 7, 6), (3, 4, 8), (3, 8, 7), (4, 1, 5), (4, 5, 8)]
+    lines = ["# cube", "o Cube"] + [f"v {a} {b} {c}" for a, b, c in v] + ["s off"] + [f"f {a} {b} {c}" for a, b, c in f]
+    path.write_text("\n".join(lines) + "\n")
+    return np.array(v, F32), np.array(f, np.uint32) - 1
+
+
+def _pose_chain(pos, idx, q, eye, size, aspect, znear, zfar, W):
+    """project -> rasterize -> interpolate(colour = pos + 0.5) -> antialias, by the oracle, with
+    the model rotation R(q) and the camera of preset_pose.cpp:20,35."""
+    lib = L.load()
+    R = np.zeros(16, F32)
+    assert lib.vcl_rotation_matrix(np.ascontiguousarray(q, F32).ctypes.data, R.ctypes.data) == 0
+    cam = camera_from_abi(look_at(eye, (0, 0, -1)), eye, size, aspect, znear, zfar, R)
+    clip = o_project(pos, cam.mat)
+    rast, db = o_raster(clip[0], idx, W, W)
+    color, _ = o_interp(rast, db, pos + F32(0.5), idx)
+    return o_aa(clip[0], idx, rast, color)[0]
+
+
+def test_pose_program(gpu, orc, tmp_path):
+    if not os.path.exists(POSE_EXE):
+        pytest.skip("tests/cpp/preset_pose not built (run __graft_entry__.build())")
+    O.set_threads(16)
+    obj = tmp_path / "cube.obj"
+    pos, idx = _cube_obj(obj)
+    W = 512
+    results = {}
+    for exact, steps in ((0, 8), (1, 150)):
+        out_bin, bmp = tmp_path / f"pose{exact}.bin", tmp_path / f"pose{exact}.bmp"
+        r = subprocess.run([POSE_EXE, str(obj), str(out_bin), str(steps), str(exact), str(bmp)], capture_output=True, text=True, timeout=180)
+        assert r.returncode == 0, r.stderr[-3000:]
+        print(r.stdout.strip(), flush=True)
+        out = np.fromfile(out_bin, F32)
+        losses = out[:steps]
+        quats = out[steps: steps + 8]
+        img = out[steps + 8: steps + 8 + W * W * 3].reshape(W, W, 3)
+        timg = out[steps + 8 + W * W * 3:].reshape(W, W, 3)
+        results[exact] = (losses, quats, img, timg, bmp)
+    losses, quats, img, timg, bmp = results[0]
+    eye, size, aspect, znear, zfar = (0.0, 0.0, 3.5), 0.5, 1.0, 1.0, 10.0
+    target = _pose_chain(pos, idx, quats[:4], eye, size, aspect, znear, zfar, W)
+    assert_close(timg, target, 1e-5, "pose target image")
+    first = _pose_chain(pos, idx, quats[4:], eye, size, aspect, znear, zfar, W)
+    g = np.zeros_like(first)
+    buf = np.zeros_like(first)
+    ref_loss = O.lib().orc_ssim_f32(W, W, 3, O.P(first), O.P(target), O.P(g), O.P(buf), 0)
+    print("first SSIM loss: program", losses[0], "oracle", ref_loss)
+    assert abs(losses[0] - ref_loss) <= 1e-5 * abs(ref_loss)
+    # headless window: the fitted image on the left half, the target on the right
+    im = L.Image()
+    assert L.load().vcl_load_bmp(str(bmp).encode(), C.byref(im)) == 0
+    win = np.ctypeslib.as_array(im.data, shape=(im.width * im.height * 3,)).reshape(im.height, im.width, 3).copy()
+    L.load().vcl_image_free(C.byref(im))
+    assert win.shape == (512, 1024, 3)
+    q8 = lambda a: np.floor(np.clip(a, 0, 1).astype(F32) * F32(255) + F32(0.5)) / F32(255)  # the BMP writer's rounding
+    np.testing.assert_allclose(win[:, :512], q8(img), atol=1e-6)
+    np.testing.assert_allclose(win[:, 512:], q8(timg), atol=1e-6)
+    # the exact gradients fit the pose: the loss falls
+    l1 = results[1][0]
+    print("exact-gradient pose fit: loss", l1[0], "->", l1[-1])
+    assert np.mean(l1[-10:]) < np.mean(l1[:10])

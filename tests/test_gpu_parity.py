@@ -627,3 +627,33 @@ def test_fit_step_config1(gpu, orc):
         agree = np.mean(np.isclose(got, ref, rtol=1e-5, atol=1e-6))
         assert agree > 0.999, f"{nm}: only {agree:.5f} of the updated values agree"
     fit.close()
+
+
+def test_fit_step_op_by_op_matches_fused(gpu, orc):
+    """vcl_fit_desc.op_by_op = 1 runs the reference's separate launches (NormalAxis, ViewAxis,
+    SGSpecular, Interpolate, Texturemap, their backwards in turn) instead of the fused passes;
+    the per-pixel code is shared, so the first forward is identical and the gradients agree to
+    the float-atomic summation order (which also makes the second step's parameters, hence its
+    image, differ in the last bits)."""
+    sc = S.config1_scene(1, W=128)
+    target = forward_inputs(S.config1_scene(2, W=128))["image"].reshape(sc.H, sc.W, sc.C).copy()
+    out = []
+    for op_by_op in (False, True):
+        fit = gpu.Fit(sc, target=target, op_by_op=op_by_op)
+        fit.step()
+        r = {"loss": float(fit.read("loss", (1,))[0]), "image": fit.read("image", (sc.H, sc.W, sc.C)),
+             "m_pos": fit.read("m_pos", (sc.V * 3,)), "m_tex": fit.read("m_tex", (sc.tw * sc.th * sc.C,))}
+        fit.step()
+        r["loss2"] = float(fit.read("loss", (1,))[0])
+        r["image2"] = fit.read("image", (sc.H, sc.W, sc.C))
+        out.append(r)
+        fit.close()
+    f, o = out
+    assert_close(o["image"], f["image"], 1e-6, "op-by-op image (step 1)")
+    assert abs(o["loss"] - f["loss"]) <= 1e-6 * abs(f["loss"])
+    # after step 1 the first moments are (1 - beta1) * gradient
+    assert_close(o["m_pos"], f["m_pos"], 1e-3, "op-by-op position moments", atol_frac=1e-4)
+    assert_close(o["m_tex"], f["m_tex"], 1e-3, "op-by-op texture moments", atol_frac=1e-4)
+    # normwise: a silhouette pixel may move with the last-bit parameter differences
+    assert rel_err(o["image2"], f["image2"]) < 1e-4
+    assert abs(o["loss2"] - f["loss2"]) <= 1e-4 * abs(f["loss2"])

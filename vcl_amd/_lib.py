@@ -163,7 +163,7 @@ class FitDesc(C.Structure):
                 ("pos", vp), ("normals", vp), ("uv", vp), ("idx", vp), ("texture", vp), ("axis", vp), ("sharpness", vp),
                 ("amplitude", vp), ("mats", vp), ("pvinv", vp), ("rot", vp), ("target", vp),
                 ("roughness", cf), ("ior", cf), ("lr", cd), ("beta1", cd), ("beta2", cd), ("eps", cd),
-                ("fold_quarter", ci), ("fused", ci)]
+                ("fold_quarter", ci), ("op_by_op", ci)]
 
 
 class Tuning(C.Structure):
@@ -280,6 +280,8 @@ SIGNATURES = {
     "vcl_fit_set_tuning": ([vp, P_(Tuning)], ci),
     "vcl_fit_get_tuning": ([vp, P_(Tuning)], ci),
     "vcl_fit_autotune": ([vp, ci, P_(Tuning), P_(cd)], ci),
+    "vcl_fit_autotune_scratch_bytes": ([vp], csz),
+    "vcl_fit_autotune_in": ([vp, ci, vp, csz, vp, P_(Tuning), P_(cd)], ci),
     "vcl_fit_destroy": ([vp], ci),
     "vcl_fit_set_target_from_render": ([vp, vp], ci),
     "vcl_fit_step": ([vp, vp], ci),

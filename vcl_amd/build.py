@@ -63,6 +63,8 @@ def build(force: bool = False) -> str:
 TOOLS = {
     # tests/cpp/preset_fit: host program on the drop-in layer header (include/vcl_layers.hpp)
     os.path.join(ROOT, "tests", "cpp", "preset_fit"): os.path.join(ROOT, "tests", "cpp", "preset_fit.cpp"),
+    # tests/cpp/preset_pose: the reference's pose preset (OBJ, SSIM, Rotation / Camera grads)
+    os.path.join(ROOT, "tests", "cpp", "preset_pose"): os.path.join(ROOT, "tests", "cpp", "preset_pose.cpp"),
 }
 
 
@@ -70,7 +72,7 @@ def build_tools(force: bool = False) -> list:
     """Host programs linked against libvcl_amd.so (rpath to vcl_amd/)."""
     out = []
     for exe, src in TOOLS.items():
-        deps = [src, LIB, os.path.join(ROOT, "include", "vcl_layers.hpp"), os.path.join(ROOT, "include", "vcl.h")]
+        deps = [src, LIB] + glob.glob(os.path.join(ROOT, "include", "*"))
         if force or not os.path.exists(exe) or any(os.path.getmtime(d) > os.path.getmtime(exe) for d in deps):
             rel = os.path.relpath(HERE, os.path.dirname(exe))
             cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"), src, "-o", exe,

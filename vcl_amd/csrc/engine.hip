@@ -325,9 +325,17 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     std::memset(&np, 0, sizeof(np));
     np.width = W; np.height = H; np.depth = nv; np.rast = F->f("rast"); np.rot = F->f("rot");
     np.normal = F->f("normals"); np.normalidx = F->t<uint32_t>("idx"); np.out = F->f("nrm");
-    // NormalAxis -> ViewAxis -> SGSpecular, fused per pixel
     vcl_sg_specular_params sp = {W, H, nv, F->f("rast"), F->f("nrm"), F->f("view"), F->f("rough"), d.ior, F->f("sAxis"), F->f("sSharp"), F->f("sAmp")};
-    { ProfScope ps_(F, "shade_prep", as_hip(s)); st = vcl_shade_prep_dev(&np, F->f("pvinv"), &sp, s); }
+    if (d.op_by_op) {
+        // the reference's sequence: NormalAxis, ViewAxis, SGSpecular as separate launches
+        { ProfScope ps_(F, "normal_axis", as_hip(s)); st = vcl_normal_axis_forward(&np, s); }
+        if (!st) { ProfScope ps_(F, "view_axis", as_hip(s)); st = vcl_view_axis_forward_dev(W, H, nv, F->f("pvinv"), F->f("view"), s); }
+        if (!st) { ProfScope ps_(F, "sg_specular", as_hip(s)); st = vcl_sg_specular_forward(&sp, s); }
+    } else {
+        // NormalAxis -> ViewAxis -> SGSpecular, fused per pixel
+        ProfScope ps_(F, "shade_prep", as_hip(s));
+        st = vcl_shade_prep_dev(&np, F->f("pvinv"), &sp, s);
+    }
     if (st) return st;
     vcl_texturemap_params tp;
     tp.width = W; tp.height = H; tp.depth = nv; tp.texwidth = d.texwidth; tp.texheight = d.texheight; tp.channel = C; tp.miplevel = d.miplevel;
@@ -335,7 +343,7 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tp.texture[l] = l < d.miplevel ? F->f("texture") + F->lvl_off[l] : nullptr;
     tp.out = F->f("diff");
     vcl_interpolate_params ip = {W, H, nv, 2, 1, F->f("rast"), F->f("db"), F->f("uv"), F->t<uint32_t>("idx"), F->f("uvp"), F->f("uvDA"), d.V};
-    if (C == 4) {
+    if (C == 4 && !d.op_by_op) {
         // uv interpolation and texture sample fused per pixel
         ProfScope ps_(F, "interp_texture", as_hip(s));
         st = vcl_interp_tex_dev(&ip, &tp, step ? F->f("gdiff") : nullptr, s);
@@ -426,7 +434,15 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tg.texture[l] = l < d.miplevel ? F->f("gtex") + F->lvl_off[l] : nullptr;
     vcl_interpolate_params ip = {W, H, nv, 2, 1, F->f("rast"), F->f("db"), F->f("uv"), F->t<uint32_t>("idx"), F->f("uvp"), F->f("uvDA"), d.V};
     vcl_interpolate_grad_params ig = {F->f("guv"), nullptr, F->f("grast"), nullptr, nullptr};
-    { ProfScope ps_(F, "texture_interp_bwd", as_hip(s)); st = vcl_texture_interp_backward_dev(&tp, &tg, &ip, &ig, s); }
+    if (d.op_by_op) {
+        // the reference's sequence: the texture backward stores the uv grad, the interpolate
+        // backward turns it into the rast grad
+        { ProfScope ps_(F, "texture_bwd", as_hip(s)); st = vcl_texturemap_backward(&tp, &tg, s); }
+        if (!st) { ProfScope ps_(F, "interpolate_bwd", as_hip(s)); st = vcl_interpolate_backward(&ip, &ig, s); }
+    } else {
+        ProfScope ps_(F, "texture_interp_bwd", as_hip(s));
+        st = vcl_texture_interp_backward_dev(&tp, &tg, &ip, &ig, s);
+    }
     if (st) return st;
     vcl_rasterize_params rp;
     std::memset(&rp, 0, sizeof(rp));
@@ -541,20 +557,27 @@ extern "C" int vcl_fit_get_tuning(const vcl_fit* F, vcl_tuning* t) {
 // snapshot of the request's persistent state (parameters, Adam moments and bias powers), then
 // restores the snapshot and keeps the fastest candidate.  Coordinate descent: raster tile x bin
 // chunk first, then the texel-splat rows.
-extern "C" int vcl_fit_autotune(vcl_fit* F, int steps, vcl_tuning* best, double* best_ms) {
-    VCL_REQUIRE(F && steps >= 1);
-    static const char* keep[] = {"pos", "texture", "axis", "sharpness", "amplitude", "m_pos", "v_pos", "m_tex", "v_tex",
-                                 "m_axis", "v_axis", "m_sharp", "v_sharp", "m_amp", "v_amp", "adam_bt"};
+static const char* const kTuneKeep[] = {"pos", "texture", "axis", "sharpness", "amplitude", "m_pos", "v_pos", "m_tex", "v_tex",
+                                        "m_axis", "v_axis", "m_sharp", "v_sharp", "m_amp", "v_amp", "adam_bt"};
+
+extern "C" size_t vcl_fit_autotune_scratch_bytes(const vcl_fit* F) {
+    if (!F) return 0;
     size_t total = 0;
-    for (const char* n : keep) total += (F->bytes.at(n) + 255) & ~(size_t)255;
-    char* snap = nullptr;
-    if (hipMalloc((void**)&snap, total) != hipSuccess) return VCL_ERR_OOM;
-    hipStream_t s = nullptr;
+    for (const char* n : kTuneKeep) total += (F->bytes.at(n) + 255) & ~(size_t)255;
+    return total;
+}
+
+extern "C" int vcl_fit_autotune_in(vcl_fit* F, int steps, void* scratch, size_t scratch_bytes, vcl_stream_t stream, vcl_tuning* best,
+                                   double* best_ms) {
+    VCL_REQUIRE(F && steps >= 1 && scratch);
+    if (scratch_bytes < vcl_fit_autotune_scratch_bytes(F)) return VCL_ERR_OOM;
+    char* snap = (char*)scratch;
+    hipStream_t s = as_hip(stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int st = VCL_OK;
     auto copy = [&](bool save) {
         size_t o = 0;
-        for (const char* n : keep) {
+        for (const char* n : kTuneKeep) {
             size_t b = F->bytes.at(n);
             void* dev = F->buf.at(n);
             if (hipMemcpyAsync(save ? (void*)(snap + o) : dev, save ? dev : (void*)(snap + o), b, hipMemcpyDeviceToDevice, s) != hipSuccess)
@@ -587,7 +610,6 @@ extern "C" int vcl_fit_autotune(vcl_fit* F, int steps, vcl_tuning* best, double*
         if (!st && (best_v >= 1e30 || ms / steps < MARGIN * best_v)) { best_v = ms / steps; best_t = cand; }
     };
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = VCL_ERR_HIP;
-    if (!st && hipDeviceSynchronize() != hipSuccess) st = VCL_ERR_HIP;
     copy(true);
     static const int tiles[] = {8, 16, 32}, chunks[] = {64, 128, 256}, rows[] = {16, 32, 64};
     trial(start);   // the incumbent is timed first
@@ -601,7 +623,6 @@ extern "C" int vcl_fit_autotune(vcl_fit* F, int steps, vcl_tuning* best, double*
     if (!st && hipStreamSynchronize(s) != hipSuccess) st = VCL_ERR_HIP;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    (void)hipFree(snap);
     if (st) { F->tune = start; F->has_tune = had; return st; }
     F->tune = best_t;
     F->has_tune = true;
@@ -609,5 +630,18 @@ extern "C" int vcl_fit_autotune(vcl_fit* F, int steps, vcl_tuning* best, double*
     if (best) *best = best_t;
     if (best_ms) *best_ms = best_v;
     return VCL_OK;
+}
+
+// Standalone form: its own snapshot memory (hipMalloc / hipFree) and the null stream after a
+// device synchronisation; vcl_fit_autotune_in is the allocation-free form on a caller stream.
+extern "C" int vcl_fit_autotune(vcl_fit* F, int steps, vcl_tuning* best, double* best_ms) {
+    VCL_REQUIRE(F && steps >= 1);
+    size_t bytes = vcl_fit_autotune_scratch_bytes(F);
+    void* snap = nullptr;
+    if (hipMalloc(&snap, bytes) != hipSuccess) return VCL_ERR_OOM;
+    int st = hipDeviceSynchronize() == hipSuccess ? VCL_OK : VCL_ERR_HIP;
+    if (!st) st = vcl_fit_autotune_in(F, steps, snap, bytes, nullptr, best, best_ms);
+    (void)hipFree(snap);
+    return st;
 }
 

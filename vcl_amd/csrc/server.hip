@@ -246,8 +246,18 @@ extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
             auto it = S->tunings.find(key);
             if (it != S->tunings.end()) st = vcl_fit_set_tuning(r->fit, &it->second);
             else {
+                // the snapshot of the tuning trials comes from the pool, counted against the
+                // budget, and the trials run on the request's own stream
                 vcl_tuning t;
-                st = vcl_fit_autotune(r->fit, S->cfg.autotune_steps, &t, nullptr);
+                size_t sb = vcl_fit_autotune_scratch_bytes(r->fit);
+                void* snap = nullptr;
+                int g2 = S->pool.get(sb, &snap);
+                if (g2 < 0) return VCL_ERR_HIP;
+                if (g2 > 0) st = VCL_ERR_OOM;
+                else {
+                    st = vcl_fit_autotune_in(r->fit, S->cfg.autotune_steps, snap, sb, (vcl_stream_t)r->stream, &t, nullptr);
+                    S->pool.put(snap);
+                }
                 if (!st) S->tunings[key] = t;
             }
             if (st) return st;

@@ -459,7 +459,7 @@ def rasterize_idhash(rast):
 
 
 # ---------------------------------------------------------------- Fit engine
-def fit_desc(scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_quarter=True, fused=False, views=None):
+def fit_desc(scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_quarter=True, op_by_op=False, views=None):
     """vcl_fit_desc of a Scene; returns (desc, keep) where `keep` holds the host arrays the
     descriptor points into (they must outlive the upload)."""
     cams = scene.cams if views is None else views
@@ -485,7 +485,7 @@ def fit_desc(scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_qua
     d.roughness, d.ior = scene.rough, scene.ior
     d.lr, d.beta1, d.beta2, d.eps = lr, betas[0], betas[1], eps
     d.fold_quarter = 1 if fold_quarter else 0
-    d.fused = 1 if fused else 0
+    d.op_by_op = 1 if op_by_op else 0
     return d, keep
 
 
@@ -504,9 +504,9 @@ def get_tuning():
 class Fit:
     """One device-resident fit request (vcl_fit_* of include/vcl.h)."""
 
-    def __init__(self, scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_quarter=True, fused=False, views=None):
+    def __init__(self, scene, target=None, lr=1e-2, betas=(0.9, 0.999), eps=1e-8, fold_quarter=True, op_by_op=False, views=None):
         cams = scene.cams if views is None else views
-        d, self._keep = fit_desc(scene, target, lr, betas, eps, fold_quarter, fused, cams)
+        d, self._keep = fit_desc(scene, target, lr, betas, eps, fold_quarter, op_by_op, cams)
         self.desc = d
         self.scene = scene
         self.views = len(cams)
