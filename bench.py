@@ -120,13 +120,15 @@ def _served_specs(SV, n, rate, seed):
     return built
 
 
-def _serve_once(SV, built, rate_scale, max_active, use_graph, steps_per_round, budget, policy=0, max_stepping=0, tunings=None):
+def _serve_once(SV, built, rate_scale, max_active, use_graph, steps_per_round, budget, policy=0, max_stepping=0, tunings=None,
+                grouped=False):
     """Run one arrival stream through a server; arrival times scaled by rate_scale (0: all at
     t=0).  Returns latency / throughput / occupancy figures.  tunings: None (no autotuning) or a
     list of (size-class key, tuning) that pre-fills the server's table and receives it back
     (classes missing from it are autotuned on admission)."""
     srv = SV.Server(hbm_budget=budget, max_active=max_active, use_graph=use_graph, steps_per_round=steps_per_round,
-                    auto_release=True, policy=policy, max_stepping=max_stepping, autotune_steps=3 if tunings is not None else 0)
+                    auto_release=True, policy=policy, max_stepping=max_stepping, autotune_steps=3 if tunings is not None else 0,
+                    grouped=grouped)
     if tunings:
         srv.load_tuning_table(tunings)
     for sp, sc, tgt in built:
@@ -164,6 +166,7 @@ def run_served(torch, rates, n, seed=3, budget_gb=200, max_active=16):
            "rates": {}}
     for rate in rates:
         cont = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget)
+        grp = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget, grouped=True)
         srpt = _serve_once(SV, base, 1.0 / rate, max_active, True, 2, budget, policy=SV.Server.SRPT, max_stepping=4)
         # autotuner: an untimed pass fills the per-size-class table, the timed pass reuses it
         table = []
@@ -172,7 +175,7 @@ def run_served(torch, rates, n, seed=3, budget_gb=200, max_active=16):
                                  tunings=table)
         srpt_tuned["size_classes"] = {f"px~4^{k & 255} tri~8^{(k >> 8) & 255}": t for k, t in table}
         seq = _serve_once(SV, base, 1.0 / rate, 1, False, 1, budget)
-        pol = {"continuous_fifo": cont, "continuous_srpt": srpt, "continuous_srpt_autotuned": srpt_tuned}
+        pol = {"continuous_fifo": cont, "continuous_fifo_grouped": grp, "continuous_srpt": srpt, "continuous_srpt_autotuned": srpt_tuned}
         out["rates"][str(rate)] = dict(pol, sequential=seq, vs_sequential={
             k: {"throughput": round(v["eq_frames_per_s"] / seq["eq_frames_per_s"], 3), "p50": round(v["p50_ms"] / seq["p50_ms"], 3),
                 "p99": round(v["p99_ms"] / seq["p99_ms"], 3)} for k, v in pol.items()})
@@ -181,7 +184,8 @@ def run_served(torch, rates, n, seed=3, budget_gb=200, max_active=16):
 
 def run_uneven(torch, n=32, iters=20, seed=5, max_active=32):
     """Batched uneven scenes: n independent requests of uneven size, all admitted at once and
-    stepped concurrently (graph replays on per-request streams), against the same requests run
+    stepped concurrently -- in shared launches over device-side work tables (grouped), and with
+    each request replaying its own graph on its own stream -- against the same requests run
     one after another with eager steps."""
     from vcl_amd import serve as SV
     specs = SV.uneven_scenes(n, seed)
@@ -191,15 +195,22 @@ def run_uneven(torch, n=32, iters=20, seed=5, max_active=32):
         sp = dict(sp, arrival_s=0.0, iterations=iters)
         built.append((sp, sc, SV.render_target(tsc)))
     _serve_once(SV, built[:1], 0.0, 1, True, 2, int(200e9))
+    _serve_once(SV, built[:2], 0.0, 2, True, 2, int(200e9), grouped=True)
+    grp = _serve_once(SV, built, 0.0, max_active, True, 2, int(200e9), grouped=True)
     cont = _serve_once(SV, built, 0.0, max_active, True, 2, int(200e9))
     seq = _serve_once(SV, built, 0.0, 1, False, 1, int(200e9))
-    return {"scenes": n, "iterations": iters, "batched": cont, "sequential": seq,
-            "value": cont["eq_frames_per_s"], "unit": "1024^2-equivalent frames/s",
-            "vs_sequential": round(cont["eq_frames_per_s"] / seq["eq_frames_per_s"], 3)}
+    return {"scenes": n, "iterations": iters, "grouped": grp, "streams": cont, "sequential": seq,
+            "value": grp["eq_frames_per_s"], "unit": "1024^2-equivalent frames/s",
+            "vs_sequential": round(grp["eq_frames_per_s"] / seq["eq_frames_per_s"], 3),
+            "streams_vs_sequential": round(cont["eq_frames_per_s"] / seq["eq_frames_per_s"], 3),
+            "workload": "32 requests: displaced icospheres 1,280-327,680 T, 256^2-2048^2, textures 256^2-4096^2 x4 with full "
+                        "mips, 16/64/128 lobes, 20 Adam iterations each, all admitted at t=0"}
 
 
 def cpu_baseline(sc, target, steps, threads):
-    """Time the CPU oracle's float fit step (a C++ restatement of the reference semantics)."""
+    """Time the CPU oracle's float fit step (a C++ restatement of the reference semantics) on
+    `threads` worker threads: one untimed step, then `steps` timed steps (iterations 2..steps+1
+    of the request).  Returns the per-step wall times in seconds."""
     import ctypes as C
 
     import numpy as np
@@ -221,7 +232,7 @@ def cpu_baseline(sc, target, steps, threads):
     s.mat, s.rot, s.pvinv = p(cam.mat), p(np.eye(4, dtype=np.float32).reshape(16)), p(cam.pvinv)
     s.tex, s.axis, s.sharp, s.amp = p(sc.tex.copy()), p(sc.axis.copy()), p(sc.sharp.copy()), p(sc.amp.copy())
     s.rough = p(np.full(sc.W * sc.H, sc.rough, np.float32))
-    s.ior, s.target, s.fold_quarter = sc.ior, p(target), 1
+    s.ior, s.target, s.fold_quarter, s.views = sc.ior, p(target), 1, 1
     for nm, n in (("pos", sc.V * 3), ("tex", sc.tw * sc.th * sc.C), ("axis", sc.K * 3), ("sharp", sc.K), ("amp", sc.K * sc.C)):
         setattr(s, "m_" + nm, p(np.zeros(n, np.float32)))
         setattr(s, "v_" + nm, p(np.zeros(n, np.float32)))
@@ -229,12 +240,23 @@ def cpu_baseline(sc, target, steps, threads):
     s.out_image = None
     w = L.orc_work_create()
     L.orc_fit_step_f32(C.byref(s), w)  # warm-up (allocations)
-    t0 = time.perf_counter()
+    times = []
     for _ in range(steps):
+        t0 = time.perf_counter()
         L.orc_fit_step_f32(C.byref(s), w)
-    dt = time.perf_counter() - t0
+        times.append(time.perf_counter() - t0)
     L.orc_work_destroy(w)
-    return steps / dt, dt
+    return times
+
+
+def cpu_threads():
+    """Worker threads for the CPU baseline: the CPUs this process may run on, capped at the 16
+    host threads a one-GPU share of the machine has."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def main():
@@ -245,7 +267,8 @@ def main():
     ap.add_argument("--mode", default="plain", choices=["plain"])
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=5, help="timed oracle steps on all threads (median reported)")
+    ap.add_argument("--cpu-single-steps", type=int, default=1, help="timed oracle steps on one thread (0: skip)")
     ap.add_argument("--breakdown", action="store_true", help="print a per-op table to stderr")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="for rocprofv3 --pmc runs: one 256 MiB device copy (counter calibration) then --probe-steps fit steps")
@@ -253,9 +276,9 @@ def main():
     ap.add_argument("--probe-batched", action="store_true", help="--pmc-probe on the batched multi-view request instead")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view measurement")
     ap.add_argument("--no-tune", action="store_true", help="keep the default kernel schedule (no autotuner)")
-    ap.add_argument("--served", default="50", help="comma-separated arrival rates (req/s) for the served measurement; '' skips it")
+    ap.add_argument("--served", default="10,50,200", help="comma-separated arrival rates (req/s) for the served measurement; '' skips it")
     ap.add_argument("--served-requests", type=int, default=24)
-    ap.add_argument("--uneven", type=int, default=0, help="run the uneven-scene batch with this many scenes (0: skip)")
+    ap.add_argument("--uneven", type=int, default=32, help="run the uneven-scene batch with this many scenes (0: skip)")
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--batched-views", type=int, default=64)
     args = ap.parse_args()
@@ -399,10 +422,17 @@ def main():
                 "alg_bytes_per_launch": ab[ROOF_OP], "avg_ms": rk["avg_ms"] if rk else None}
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        fps, dt = cpu_baseline(sc, target, args.cpu_steps, threads)
-        cpu = {"value": round(fps, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_steps} fit steps of the same workload ({sc.W}x{sc.H}, {sc.T} tris, K={sc.K}) on the oracle, {dt:.1f} s"}
+        threads = cpu_threads()
+        times = cpu_baseline(sc, target, args.cpu_steps, threads)
+        med = float(np.median(times))
+        cpu = {"value": round(1.0 / med, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+               "sample": f"median of {len(times)} fit steps (iterations 2..{len(times) + 1}) of the same workload ({sc.W}x{sc.H}, "
+                         f"{sc.T} tris, K={sc.K}) on the oracle with {threads} threads, {sum(times):.1f} s",
+               "step_s": [round(t, 3) for t in times]}
+        if args.cpu_single_steps > 0:
+            t1 = cpu_baseline(sc, target, args.cpu_single_steps, 1)
+            cpu["single_thread"] = {"value": round(1.0 / float(np.median(t1)), 4), "unit": "frames/s", "cores": 1,
+                                    "sample": f"median of {len(t1)} fit step(s) on one thread, {sum(t1):.1f} s"}
     served = None
     if args.served:
         served = run_served(torch, [float(r) for r in args.served.split(",")], args.served_requests)

@@ -702,6 +702,40 @@ int vcl_fit_autotune(vcl_fit* f, int steps, vcl_tuning* best, double* best_ms);
 size_t vcl_fit_autotune_scratch_bytes(const vcl_fit* f);
 int vcl_fit_autotune_in(vcl_fit* f, int steps, void* scratch, size_t scratch_bytes, vcl_stream_t stream, vcl_tuning* best, double* best_ms);
 
+/* ---------------------------------------------------------------- Groups
+ * One step of several requests in shared launches (no counterpart in the reference, whose only
+ * batch dimension is the layer count of one draw: one mesh, one texture, one resolution,
+ * rasterize.h:42).  Requests of any size (mesh, resolution, texture, lobe count) whose steps
+ * launch the same kernels with the same workgroup shape (equal vcl_fit_step_signature) form a
+ * group.  A group step is one launch per kernel of the step for all its requests: each launch
+ * reads a device-side work table (every request's workgroup range, grid and arguments), and
+ * each workgroup runs the kernel for its own request.  Results equal each request's own
+ * vcl_fit_step.  The tables hold pointers into the requests' arenas: the requests must outlive
+ * the group, and the group's work must be complete before a request is destroyed. */
+typedef struct vcl_group vcl_group;   /* opaque */
+
+/* Hash of the kernels, workgroup shapes and argument sizes of f's step (records the step's
+ * launches; runs nothing). */
+int vcl_fit_step_signature(vcl_fit* f, uint64_t* sig);
+/* Device bytes of the work tables of a group of these requests; 0 if they cannot form one. */
+size_t vcl_group_table_bytes(vcl_fit* const* fits, int n);
+/* Group of n distinct requests with its work tables in caller device memory (uploaded on
+ * `stream`; the memory stays the caller's and must not be reused before the group's last step
+ * has completed).  VCL_ERR_INVALID if the requests' steps differ in their kernels or shapes. */
+int vcl_group_create_in(vcl_fit* const* fits, int n, void* table, size_t table_bytes, vcl_stream_t stream, vcl_group** out);
+/* The same with the tables in memory the group allocates (and frees in vcl_group_destroy). */
+int vcl_group_create(vcl_fit* const* fits, int n, vcl_stream_t stream, vcl_group** out);
+/* One fwd+bwd+Adam step of every request of the group on `stream`. */
+int vcl_group_step(vcl_group* g, vcl_stream_t stream);
+/* The same step replayed from a HIP graph (captured on the first call, on a stream of the
+ * group's own, ordered with `stream` by events). */
+int vcl_group_step_graph(vcl_group* g, vcl_stream_t stream);
+int vcl_group_size(const vcl_group* g);       /* requests */
+int vcl_group_launches(const vcl_group* g);   /* launches per group step */
+/* Releases the group (waits for its graph stream; eager steps on a caller stream must have
+ * completed). */
+int vcl_group_destroy(vcl_group* g);
+
 /* ---------------------------------------------------------------- Serving
  * Continuous batching of fit requests on one GPU (no counterpart in the reference, which runs
  * one preset in a GLUT loop, main.cpp:39-49).  Requests queue with an arrival time on the
@@ -730,6 +764,9 @@ typedef struct vcl_server_config {
                              all views in buckets of 4x, triangles in buckets of 8x) with this
                              many timed steps per candidate, and reuse its tuning for the class;
                              0: no tuning */
+    int grouped;          /* 1: the requests stepping in a round run in shared launches
+                             (vcl_group), one group per step signature, each on its own
+                             stream; 0: each request steps on its own stream */
 } vcl_server_config;
 
 typedef struct vcl_request_stats {

@@ -6,6 +6,8 @@
   whose order differs from run to run (as in the reference), and Adam turns those last-digit
   differences into slowly diverging trajectories: after 3 iterations the losses agree to the
   gradient tolerance (1e-3), after 10 to 5e-3.
+- The same at the mix's largest draws (327,680 triangles, 2048^2, 4096^2 texture), with the
+  requests stepping on their own streams and in shared launches (grouped).
 - Admission respects the HBM budget: with room for one request at a time every request still
   finishes, each admitted only after the previous one released its arena.
 """
@@ -120,3 +122,41 @@ def test_budget_admission(gpu):
     for a, b in zip(st, st[1:]):
         assert b["admit_s"] >= a["finish_s"], "admitted before the budget had room"
     assert mem["peak"] <= biggest + (biggest >> 3)
+
+
+# the largest draws of the uneven mix: 327,680 triangles, 2048^2 image, 4096^2 x 4 texture
+BIG_A = {"level": 7, "W": 2048, "tex": 4096, "K": 128, "seed": 21}
+BIG_MIX = [{"level": 7, "W": 1024, "tex": 4096, "K": 64, "seed": 22},
+           {"level": 6, "W": 2048, "tex": 1024, "K": 16, "seed": 23},
+           {"level": 3, "W": 256, "tex": 256, "K": 128, "seed": 24}]
+
+
+@pytest.mark.parametrize("grouped", [False, True])
+def test_alone_vs_mix_largest_draws(gpu, grouped):
+    """The alone / mix / plain equivalence at the mix's largest sizes, per-stream and grouped:
+    first loss bit-identical, after three iterations within the gradient tolerance."""
+    sc, tgt = _request(BIG_A)
+    others = [_request(s) for s in BIG_MIX]
+    plain = _plain(sc, tgt, SHORT_ITERS)
+    for mix in (False, True):
+        srv = SV.Server(max_active=4, auto_release=False, steps_per_round=1, grouped=grouped)
+        if mix:
+            for i, (o_sc, o_tgt) in enumerate(others):
+                srv.submit(o_sc, o_tgt, SHORT_ITERS + i + 1)
+        t = srv.submit(sc, tgt, SHORT_ITERS)
+        srv.drain()
+        st = {s["ticket"]: s for s in srv.finished()}
+        assert all(s["status"] == 0 for s in st.values()) and st[t]["iterations"] == SHORT_ITERS
+        state = _state(srv, t, sc)
+        srv.close()
+        print(f"grouped={grouped} mix={mix}: loss {state['loss']:.7e} (plain {plain['loss']:.7e})", flush=True)
+        np.testing.assert_allclose(state["loss"], plain["loss"], rtol=1e-3)
+    # one iteration: the forward of the initial state is deterministic
+    one = _plain(sc, tgt, 1)
+    srv = SV.Server(max_active=4, auto_release=False, steps_per_round=1, grouped=grouped)
+    for i, (o_sc, o_tgt) in enumerate(others):
+        srv.submit(o_sc, o_tgt, 2 + i)
+    t = srv.submit(sc, tgt, 1)
+    srv.drain()
+    assert float(srv.read(t, "loss", (1,))[0]) == one["first"]
+    srv.close()

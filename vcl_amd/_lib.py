@@ -172,7 +172,7 @@ class Tuning(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", csz), ("max_active", ci), ("use_graph", ci), ("steps_per_round", ci), ("auto_release", ci),
-                ("policy", ci), ("max_stepping", ci), ("autotune_steps", ci)]
+                ("policy", ci), ("max_stepping", ci), ("autotune_steps", ci), ("grouped", ci)]
 
 
 class RequestStats(C.Structure):
@@ -262,6 +262,15 @@ SIGNATURES = {
     "vcl_fit_create": ([P_(FitDesc), P_(vp)], ci),
     "vcl_fit_arena_bytes": ([P_(FitDesc)], csz),
     "vcl_fit_create_in": ([P_(FitDesc), vp, csz, vp, P_(vp)], ci),
+    "vcl_fit_step_signature": ([vp, P_(C.c_uint64)], ci),
+    "vcl_group_table_bytes": ([P_(vp), ci], csz),
+    "vcl_group_create_in": ([P_(vp), ci, vp, csz, vp, P_(vp)], ci),
+    "vcl_group_create": ([P_(vp), ci, vp, P_(vp)], ci),
+    "vcl_group_step": ([vp, vp], ci),
+    "vcl_group_step_graph": ([vp, vp], ci),
+    "vcl_group_size": ([vp], ci),
+    "vcl_group_launches": ([vp], ci),
+    "vcl_group_destroy": ([vp], ci),
     "vcl_server_create": ([P_(ServerConfig), P_(vp)], ci),
     "vcl_server_destroy": ([vp], ci),
     "vcl_server_reset_clock": ([vp], ci),

@@ -497,6 +497,30 @@ extern "C" int vcl_fit_step(vcl_fit* F, vcl_stream_t s) {
     return st;
 }
 
+// Records the launches of one step into rec without running anything.  The step runs with
+// the recorder installed, inside a capture of a private stream: groupable launches go to the
+// recorder, and any other device work would land in the captured graph, which must stay empty.
+int vcl_fit_record_step(vcl_fit* F, VRecorder* rec) {
+    VCL_REQUIRE(F && rec && !g_vcl_recorder);
+    if (F->prof.on) return VCL_ERR_INVALID;
+    thread_local hipStream_t cs = nullptr;
+    if (!cs) VCL_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    VCL_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    g_vcl_recorder = rec;
+    int st = step_body(F, (vcl_stream_t)cs);
+    g_vcl_recorder = nullptr;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    size_t nodes = 0;
+    if (e == hipSuccess && g) e = hipGraphGetNodes(g, nullptr, &nodes);
+    if (g) (void)hipGraphDestroy(g);
+    if (st) return st;
+    VCL_HIP_CHECK(e);
+    return nodes ? VCL_ERR_INVALID : VCL_OK;
+}
+
+void vcl_fit_count_steps(vcl_fit* F, int k) { F->it += k; }
+
 // Capture one step (same launches as vcl_fit_step) into a HIP graph on a private stream the
 // first time, then replay it.  The caller's stream and the replay are ordered with events.
 extern "C" int vcl_fit_step_graph(vcl_fit* F, vcl_stream_t s) {

@@ -2,6 +2,7 @@
 // the public C ABI in include/vcl.h).
 #pragma once
 #include "common.hpp"
+#include "group.hpp"
 
 // Process-wide kernel tuning (vcl_set_tuning); read by the launch code of the raster and
 // texture-backward entry points.
@@ -66,3 +67,9 @@ int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_
 int vcl_mse_loss_grad_dev(const vcl_loss_params* p, vcl_stream_t stream);
 // bt_dev[0] *= rhom; bt_dev[1] *= rhov (one thread; the step-start bump of Adam::step).
 }
+
+// Shared launches over several requests (group.hip): the launches of one step of F, recorded
+// (nothing runs); VCL_ERR_INVALID if the step holds device work that cannot be grouped.
+int vcl_fit_record_step(vcl_fit* F, VRecorder* rec);
+// Adds k to F's step count (steps taken through a group).
+void vcl_fit_count_steps(vcl_fit* F, int k);

@@ -9,17 +9,21 @@
 #include <cmath>
 #include <mutex>
 
-__global__ void vcl_fill_u32(u32* p, size_t n, u32 v) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = v;
-}
+struct VclFillU32F {
+    static constexpr int kBS = 1024, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, u32* p, size_t n, u32 v) const {
+        size_t i = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        size_t stride = (size_t)vb.g.x * blockDim.x;
+        for (; i < n; i += stride) p[i] = v;
+    }
+};
+__global__ void vcl_fill_u32(u32* p, size_t n, u32 v) { VclFillU32F{}(vb_hw(), p, n, v); }
 
 int vcl_fill_u32_async(u32* p, size_t n, u32 v, hipStream_t s) {
     if (!p || n == 0) return VCL_OK;
     unsigned g = cdiv(n, 256);
     if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(vcl_fill_u32, dim3(g), dim3(256), 0, s, p, n, v);
+    vlaunch<VclFillU32F>(vcl_fill_u32, dim3(g), dim3(256), 0, s, p, n, v);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -39,18 +43,22 @@ struct FillMulti {
     double rhom, rhov;
 };
 
-__global__ void __launch_bounds__(256) vcl_fill_multi_k(FillMulti f) {
-    unsigned b = blockIdx.x;
-    if (f.bt && b == 0 && threadIdx.x == 0) { f.bt[0] *= f.rhom; f.bt[1] *= f.rhov; }
-    if (f.count == 0) return;
-    int t = 0;
-    while (t + 1 < f.count && b >= f.block_start[t + 1]) t++;
-    size_t base = (size_t)(b - f.block_start[t]) * 4096;
-    for (int k = threadIdx.x; k < 4096; k += 256) {
-        size_t i = base + k;
-        if (i < f.n[t]) f.p[t][i] = 0u;
+struct VclFillMultiKF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, FillMulti f) const {
+        unsigned b = vb.b.x;
+        if (f.bt && b == 0 && threadIdx.x == 0) { f.bt[0] *= f.rhom; f.bt[1] *= f.rhov; }
+        if (f.count == 0) return;
+        int t = 0;
+        while (t + 1 < f.count && b >= f.block_start[t + 1]) t++;
+        size_t base = (size_t)(b - f.block_start[t]) * 4096;
+        for (int k = threadIdx.x; k < 4096; k += 256) {
+            size_t i = base + k;
+            if (i < f.n[t]) f.p[t][i] = 0u;
+        }
     }
-}
+};
+__global__ void __launch_bounds__(256) vcl_fill_multi_k(FillMulti f) { VclFillMultiKF{}(vb_hw(), f); }
 
 int vcl_fill_multi_async(void* const* p, const size_t* bytes, int n, hipStream_t s) {
     return vcl_fill_multi_bump_async(p, bytes, n, nullptr, 1.0, 1.0, s);
@@ -74,7 +82,7 @@ int vcl_fill_multi_bump_async(void* const* p, const size_t* bytes, int n, double
         }
         f.block_start[f.count] = blocks;
         if (!blocks && !f.bt) continue;
-        hipLaunchKernelGGL(vcl_fill_multi_k, dim3(blocks ? blocks : 1), dim3(256), 0, s, f);
+        vlaunch<VclFillMultiKF>(vcl_fill_multi_k, dim3(blocks ? blocks : 1), dim3(256), 0, s, f);
         VCL_LAUNCH_CHECK();
     }
     return VCL_OK;
@@ -87,19 +95,23 @@ constexpr int BS = 256;
 // ---------------------------------------------------------------- deterministic column sums
 // sum_r part[r * cols + c] in a fixed order: segment sums here, the sum over segments by the
 // consumer (k_sg_lobe_final).
-__global__ void __launch_bounds__(256) k_colsum_stage(const float* part, int rows, int cols, int seg_rows, float* seg) {
-    // block: 64 columns x 4 row groups; grid.y: segment
-    int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    int rg = threadIdx.x >> 6;
-    int r0 = blockIdx.y * seg_rows, r1 = min(rows, r0 + seg_rows);
-    float s = 0.f;
-    if (c < cols)
-        for (int r = r0 + rg; r < r1; r += 4) s += part[(size_t)r * cols + c];
-    __shared__ float sh[4][64];
-    sh[rg][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (rg == 0 && c < cols) seg[(size_t)blockIdx.y * cols + c] = ((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x];
-}
+struct KColsumStageF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, const float* part, int rows, int cols, int seg_rows, float* seg) const {
+        // block: 64 columns x 4 row groups; grid.y: segment
+        int c = vb.b.x * 64 + (threadIdx.x & 63);
+        int rg = threadIdx.x >> 6;
+        int r0 = vb.b.y * seg_rows, r1 = min(rows, r0 + seg_rows);
+        float s = 0.f;
+        if (c < cols)
+            for (int r = r0 + rg; r < r1; r += 4) s += part[(size_t)r * cols + c];
+        __shared__ float sh[4][64];
+        sh[rg][threadIdx.x & 63] = s;
+        __syncthreads();
+        if (rg == 0 && c < cols) seg[(size_t)vb.b.y * cols + c] = ((sh[0][threadIdx.x] + sh[1][threadIdx.x]) + sh[2][threadIdx.x]) + sh[3][threadIdx.x];
+    }
+};
+__global__ void __launch_bounds__(256) k_colsum_stage(const float* part, int rows, int cols, int seg_rows, float* seg) { KColsumStageF{}(vb_hw(), part, rows, cols, seg_rows, seg); }
 
 constexpr int COLSUM_SEGS = 32;
 
@@ -112,7 +124,7 @@ int colsum_stage(const float* part, int rows, int cols, float* seg_scratch, hipS
     if (seg_rows < 1) seg_rows = 1;
     int nseg = (rows + seg_rows - 1) / seg_rows;
     if (nseg < 1) nseg = 1;
-    hipLaunchKernelGGL(k_colsum_stage, dim3(cdiv(cols, 64), nseg), dim3(256), 0, s, part, rows, cols, seg_rows, seg_scratch);
+    vlaunch<KColsumStageF>(k_colsum_stage, dim3(cdiv(cols, 64), nseg), dim3(256), 0, s, part, rows, cols, seg_rows, seg_scratch);
     if (hipGetLastError() != hipSuccess) return -1;
     return nseg;
 }
@@ -120,50 +132,60 @@ int colsum_stage(const float* part, int rows, int cols, float* seg_scratch, hipS
 // ---------------------------------------------------------------- project
 // zero (optional): a buffer shaped like out cleared in the same pass (the step's clip-space grad
 // clear); bt (optional): the step-start bump of Adam's bias-correction powers, done once.
-__global__ void __launch_bounds__(BS) k_project_fwd(int V, int dim, int views, const float* vec, const float* mat, float* out, float* zero,
-                                                    double* bt, double rhom, double rhov, WsClear rc) {
+struct KProjectFwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, int V, int dim, int views, const float* vec, const float* mat, float* out, float* zero,
+                                                    double* bt, double rhom, double rhov, WsClear rc) const {
 #pragma clang fp contract(off)
-    size_t gid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (bt && gid == 0) { bt[0] *= rhom; bt[1] *= rhov; }
-    if (rc.counts && gid < rc.n()) ws_clear_elem(rc, gid);
-    if (gid >= (size_t)V * views) return;
-    int s = (int)(gid / V), v = (int)(gid % V);
-    const float* m = mat + 16 * s;
-    float x = vec[3 * v], y = vec[3 * v + 1], z = vec[3 * v + 2];
-    float r[4];
+        size_t gid = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (bt && gid == 0) { bt[0] *= rhom; bt[1] *= rhov; }
+        if (rc.counts && gid < rc.n()) ws_clear_elem(rc, gid);
+        if (gid >= (size_t)V * views) return;
+        int s = (int)(gid / V), v = (int)(gid % V);
+        const float* m = mat + 16 * s;
+        float x = vec[3 * v], y = vec[3 * v + 1], z = vec[3 * v + 2];
+        float r[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) r[i] = m[i] * x + m[4 + i] * y + m[8 + i] * z + m[12 + i];
-    if (dim == 4) ((float4*)out)[gid] = make_float4(r[0], r[1], r[2], r[3]);
-    else for (int i = 0; i < dim; i++) out[gid * dim + i] = r[i];
-    if (zero) for (int i = 0; i < dim; i++) zero[gid * dim + i] = 0.f;
-}
+        for (int i = 0; i < 4; i++) r[i] = m[i] * x + m[4 + i] * y + m[8 + i] * z + m[12 + i];
+        if (dim == 4) ((float4*)out)[gid] = make_float4(r[0], r[1], r[2], r[3]);
+        else for (int i = 0; i < dim; i++) out[gid * dim + i] = r[i];
+        if (zero) for (int i = 0; i < dim; i++) zero[gid * dim + i] = 0.f;
+    }
+};
+__global__ void __launch_bounds__(BS) k_project_fwd(int V, int dim, int views, const float* vec, const float* mat, float* out, float* zero,
+                                                    double* bt, double rhom, double rhov, WsClear rc) { KProjectFwdF{}(vb_hw(), V, dim, views, vec, mat, out, zero, bt, rhom, rhov, rc); }
 
 // gvec[v] += sum over views of M^T g (views in order).  gmat (camera-matrix grad, off the
 // fit path) is reduced per wave and added with one atomic per wave and entry.
-__global__ void __launch_bounds__(BS) k_project_bwd(int V, int dim, int views, const float* vec, const float* mat, const float* gout,
-                                                    float* gvec, float* gmat) {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    bool ok = v < V;
-    float gx = 0, gy = 0, gz = 0;
-    float pv[4] = {ok ? vec[3 * v] : 0.f, ok ? vec[3 * v + 1] : 0.f, ok ? vec[3 * v + 2] : 0.f, ok ? 1.f : 0.f};
-    for (int s = 0; s < views; s++) {
-        const float* m = mat + 16 * s;
-        float g4[4];
-        for (int i = 0; i < 4; i++) {
-            float g = (ok && i < dim) ? gout[((size_t)s * V + v) * dim + i] : 0.f;
-            g4[i] = g;
-            gx += m[i] * g; gy += m[4 + i] * g; gz += m[8 + i] * g;
+struct KProjectBwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, int V, int dim, int views, const float* vec, const float* mat, const float* gout,
+                                                    float* gvec, float* gmat) const {
+        int v = vb.b.x * blockDim.x + threadIdx.x;
+        bool ok = v < V;
+        float gx = 0, gy = 0, gz = 0;
+        float pv[4] = {ok ? vec[3 * v] : 0.f, ok ? vec[3 * v + 1] : 0.f, ok ? vec[3 * v + 2] : 0.f, ok ? 1.f : 0.f};
+        for (int s = 0; s < views; s++) {
+            const float* m = mat + 16 * s;
+            float g4[4];
+            for (int i = 0; i < 4; i++) {
+                float g = (ok && i < dim) ? gout[((size_t)s * V + v) * dim + i] : 0.f;
+                g4[i] = g;
+                gx += m[i] * g; gy += m[4 + i] * g; gz += m[8 + i] * g;
+            }
+            if (gmat) {
+                for (int j = 0; j < 4; j++)
+                    for (int i = 0; i < 4; i++) {
+                        float val = wave_sum(pv[j] * g4[i]);
+                        if ((threadIdx.x & 63) == 0 && val != 0.f) atomicAdd(gmat + 16 * s + 4 * j + i, val);
+                    }
+            }
         }
-        if (gmat) {
-            for (int j = 0; j < 4; j++)
-                for (int i = 0; i < 4; i++) {
-                    float val = wave_sum(pv[j] * g4[i]);
-                    if ((threadIdx.x & 63) == 0 && val != 0.f) atomicAdd(gmat + 16 * s + 4 * j + i, val);
-                }
-        }
+        if (ok && gvec) { gvec[3 * v] += gx; gvec[3 * v + 1] += gy; gvec[3 * v + 2] += gz; }
     }
-    if (ok && gvec) { gvec[3 * v] += gx; gvec[3 * v + 1] += gy; gvec[3 * v + 2] += gz; }
-}
+};
+__global__ void __launch_bounds__(BS) k_project_bwd(int V, int dim, int views, const float* vec, const float* mat, const float* gout,
+                                                    float* gvec, float* gmat) { KProjectBwdF{}(vb_hw(), V, dim, views, vec, mat, gout, gvec, gmat); }
 
 // ---------------------------------------------------------------- interpolate
 // One attribute of one covered pixel (interpolate.cu:38: a0*u + a1*v in float, the (1.0 - u - v)
@@ -175,55 +197,65 @@ __device__ __forceinline__ void interp_attr_px(float4 r, float4 b, double w2, fl
     oda = make_float2(dadu * b.x + dadv * b.z, dadu * b.y + dadv * b.w);
 }
 
-__global__ void __launch_bounds__(BS) k_interp_fwd(size_t P, int D, int enableDA, const float* rast, const float* db, const float* attr,
-                                                   const u32* idx, float* out, float* outDA) {
+struct KInterpFwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, int D, int enableDA, const float* rast, const float* db, const float* attr,
+                                                   const u32* idx, float* out, float* outDA) const {
 #pragma clang fp contract(off)
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float4 r = ((const float4*)rast)[p];
-    int id = (int)r.w - 1;
-    if (id < 0) {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        float4 r = ((const float4*)rast)[p];
+        int id = (int)r.w - 1;
+        if (id < 0) {
+            for (int i = 0; i < D; i++) {
+                out[p * D + i] = 0.f;
+                if (enableDA) ((float2*)outDA)[p * D + i] = make_float2(0.f, 0.f);
+            }
+            return;
+        }
+        u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
+        float4 b = enableDA ? ((const float4*)db)[p] : make_float4(0, 0, 0, 0);
+        double w2 = 1.0 - (double)r.x - (double)r.y;
         for (int i = 0; i < D; i++) {
-            out[p * D + i] = 0.f;
-            if (enableDA) ((float2*)outDA)[p * D + i] = make_float2(0.f, 0.f);
+            float o;
+            float2 oda;
+            interp_attr_px(r, b, w2, attr[(size_t)i0 * D + i], attr[(size_t)i1 * D + i], attr[(size_t)i2 * D + i], o, oda);
+            out[p * D + i] = o;
+            if (enableDA) ((float2*)outDA)[p * D + i] = oda;
         }
-        return;
     }
-    u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
-    float4 b = enableDA ? ((const float4*)db)[p] : make_float4(0, 0, 0, 0);
-    double w2 = 1.0 - (double)r.x - (double)r.y;
-    for (int i = 0; i < D; i++) {
-        float o;
-        float2 oda;
-        interp_attr_px(r, b, w2, attr[(size_t)i0 * D + i], attr[(size_t)i1 * D + i], attr[(size_t)i2 * D + i], o, oda);
-        out[p * D + i] = o;
-        if (enableDA) ((float2*)outDA)[p * D + i] = oda;
-    }
-}
+};
+__global__ void __launch_bounds__(BS) k_interp_fwd(size_t P, int D, int enableDA, const float* rast, const float* db, const float* attr,
+                                                   const u32* idx, float* out, float* outDA) { KInterpFwdF{}(vb_hw(), P, D, enableDA, rast, db, attr, idx, out, outDA); }
 
-__global__ void __launch_bounds__(BS) k_interp_bwd(size_t P, int D, const float* rast, const float* attr, const u32* idx, const float* gout,
-                                                   float* gattr, float* grast) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float4 r = ((const float4*)rast)[p];
-    int id = (int)r.w - 1;
-    if (id < 0) return;
-    u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
-    float gu = 0.f, gv = 0.f;
-    float w2 = 1.0f - r.x - r.y;
-    for (int i = 0; i < D; i++) {
-        float g = gout[p * D + i];
-        if (gattr) {
-            atomicAdd(gattr + (size_t)i0 * D + i, g * r.x);
-            atomicAdd(gattr + (size_t)i1 * D + i, g * r.y);
-            atomicAdd(gattr + (size_t)i2 * D + i, g * w2);
+struct KInterpBwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, int D, const float* rast, const float* attr, const u32* idx, const float* gout,
+                                                   float* gattr, float* grast) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        float4 r = ((const float4*)rast)[p];
+        int id = (int)r.w - 1;
+        if (id < 0) return;
+        u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
+        float gu = 0.f, gv = 0.f;
+        float w2 = 1.0f - r.x - r.y;
+        for (int i = 0; i < D; i++) {
+            float g = gout[p * D + i];
+            if (gattr) {
+                atomicAdd(gattr + (size_t)i0 * D + i, g * r.x);
+                atomicAdd(gattr + (size_t)i1 * D + i, g * r.y);
+                atomicAdd(gattr + (size_t)i2 * D + i, g * w2);
+            }
+            float a2 = attr[(size_t)i2 * D + i];
+            gu += g * (attr[(size_t)i0 * D + i] - a2);
+            gv += g * (attr[(size_t)i1 * D + i] - a2);
         }
-        float a2 = attr[(size_t)i2 * D + i];
-        gu += g * (attr[(size_t)i0 * D + i] - a2);
-        gv += g * (attr[(size_t)i1 * D + i] - a2);
+        if (grast) ((float2*)grast)[p] = make_float2(gu, gv);
     }
-    if (grast) ((float2*)grast)[p] = make_float2(gu, gv);
-}
+};
+__global__ void __launch_bounds__(BS) k_interp_bwd(size_t P, int D, const float* rast, const float* attr, const u32* idx, const float* gout,
+                                                   float* gattr, float* grast) { KInterpBwdF{}(vb_hw(), P, D, rast, attr, idx, gout, gattr, grast); }
 
 // ---------------------------------------------------------------- mip texture
 struct TexArgs {
@@ -325,136 +357,154 @@ __device__ __forceinline__ float4 tex4_px(const TexArgs& t, float2 u, float4 da)
     return o;
 }
 
-__global__ void __launch_bounds__(BS) k_tex_fwd4(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (rast[4 * p + 3] >= 1.f)
-        o = tex4_px(t, ((const float2*)uv)[p], t.L > 1 ? ((const float4*)uvDA)[p] : make_float4(0.f, 0.f, 0.f, 0.f));
-    ((float4*)out)[p] = o;
-}
+struct KTexFwd4F {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (rast[4 * p + 3] >= 1.f)
+            o = tex4_px(t, ((const float2*)uv)[p], t.L > 1 ? ((const float4*)uvDA)[p] : make_float4(0.f, 0.f, 0.f, 0.f));
+        ((float4*)out)[p] = o;
+    }
+};
+__global__ void __launch_bounds__(BS) k_tex_fwd4(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) { KTexFwd4F{}(vb_hw(), P, t, rast, uv, uvDA, out); }
 
 // The fit step's uv interpolation (2 attributes, with derivatives) and C == 4 texture forward in
 // one pass per pixel: the same per-pixel code as the two ops, writing uv and its derivatives for
 // the backward and the texture sample.
 // zero (optional): a buffer shaped like out (4 floats per pixel) cleared in the same pass (the
 // step's texture-output grad clear).
-__global__ void __launch_bounds__(BS) k_interp_tex4(size_t P, const float* rast, const float* db, const float* uvattr, const u32* idx, float* uvp,
-                                                    float* uvDA, TexArgs t, float* out, float* zero) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const float4 r = ((const float4*)rast)[p];
-    const int id = (int)r.w - 1;
-    float2 u = make_float2(0.f, 0.f);
-    float4 da = make_float4(0.f, 0.f, 0.f, 0.f), o = da;
-    if (id >= 0) {
-        const u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
-        const float4 b = ((const float4*)db)[p];
-        const double w2 = 1.0 - (double)r.x - (double)r.y;
-        float2 dau, dav;
-        interp_attr_px(r, b, w2, uvattr[2 * (size_t)i0], uvattr[2 * (size_t)i1], uvattr[2 * (size_t)i2], u.x, dau);
-        interp_attr_px(r, b, w2, uvattr[2 * (size_t)i0 + 1], uvattr[2 * (size_t)i1 + 1], uvattr[2 * (size_t)i2 + 1], u.y, dav);
-        da = make_float4(dau.x, dau.y, dav.x, dav.y);
-    }
-    ((float2*)uvp)[p] = u;
-    ((float4*)uvDA)[p] = da;
-    if (r.w >= 1.f) o = tex4_px(t, u, da);
-    ((float4*)out)[p] = o;
-    if (zero) ((float4*)zero)[p] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
-
-__global__ void __launch_bounds__(BS) k_tex_fwd(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    int C = t.C;
-    if (rast[4 * p + 3] < 1.f) { for (int c = 0; c < C; c++) out[p * C + c] = 0.f; return; }
-    float2 u = ((const float2*)uv)[p];
-    int l0 = 0, l1 = 0; float f = 0.f, lev;
-    if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, nullptr, 0);
-    int4 i0, i1; float fx, fy, gx = 0, gy = 0;
-    index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
-    if (t.L > 1) index_fetch(t.tw, t.th, l1, u, i1, gx, gy);
-    const float* T0 = t.tex[l0];
-    const float* T1 = t.tex[l1];
-    for (int c = 0; c < C; c++) {
-        float v = lerpf_(lerpf_(T0[i0.x * C + c], T0[i0.y * C + c], fx), lerpf_(T0[i0.z * C + c], T0[i0.w * C + c], fx), fy);
-        if (t.L > 1 && f > 0) {
-            float v1 = lerpf_(lerpf_(T1[i1.x * C + c], T1[i1.y * C + c], gx), lerpf_(T1[i1.z * C + c], T1[i1.w * C + c], gx), gy);
-            v = lerpf_(v, v1, f);
+struct KInterpTex4F {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, const float* rast, const float* db, const float* uvattr, const u32* idx, float* uvp,
+                                                    float* uvDA, TexArgs t, float* out, float* zero) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        const float4 r = ((const float4*)rast)[p];
+        const int id = (int)r.w - 1;
+        float2 u = make_float2(0.f, 0.f);
+        float4 da = make_float4(0.f, 0.f, 0.f, 0.f), o = da;
+        if (id >= 0) {
+            const u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
+            const float4 b = ((const float4*)db)[p];
+            const double w2 = 1.0 - (double)r.x - (double)r.y;
+            float2 dau, dav;
+            interp_attr_px(r, b, w2, uvattr[2 * (size_t)i0], uvattr[2 * (size_t)i1], uvattr[2 * (size_t)i2], u.x, dau);
+            interp_attr_px(r, b, w2, uvattr[2 * (size_t)i0 + 1], uvattr[2 * (size_t)i1 + 1], uvattr[2 * (size_t)i2 + 1], u.y, dav);
+            da = make_float4(dau.x, dau.y, dav.x, dav.y);
         }
-        out[p * C + c] = v;
+        ((float2*)uvp)[p] = u;
+        ((float4*)uvDA)[p] = da;
+        if (r.w >= 1.f) o = tex4_px(t, u, da);
+        ((float4*)out)[p] = o;
+        if (zero) ((float4*)zero)[p] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-}
+};
+__global__ void __launch_bounds__(BS) k_interp_tex4(size_t P, const float* rast, const float* db, const float* uvattr, const u32* idx, float* uvp,
+                                                    float* uvDA, TexArgs t, float* out, float* zero) { KInterpTex4F{}(vb_hw(), P, rast, db, uvattr, idx, uvp, uvDA, t, out, zero); }
 
-__global__ void __launch_bounds__(BS) k_tex_bwd(size_t P, TexArgs t, TexGradArgs g, const float* rast, const float* uv, const float* uvDA,
-                                                const float* gout, float* guv, float* guvDA, int mode) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    if (rast[4 * p + 3] < 1.f) return;
-    int C = t.C;
-    float2 u = ((const float2*)uv)[p];
-    if (t.L <= 1) {
-        int4 i0; float fx, fy;
-        index_fetch(t.tw, t.th, 0, u, i0, fx, fy);
-        float gu = 0.f, gv = 0.f;
-        const float* T0 = t.tex[0];
-        float* G0 = g.tex[0];
+struct KTexFwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        int C = t.C;
+        if (rast[4 * p + 3] < 1.f) { for (int c = 0; c < C; c++) out[p * C + c] = 0.f; return; }
+        float2 u = ((const float2*)uv)[p];
+        int l0 = 0, l1 = 0; float f = 0.f, lev;
+        if (t.L > 1) calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, nullptr, 0);
+        int4 i0, i1; float fx, fy, gx = 0, gy = 0;
+        index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
+        if (t.L > 1) index_fetch(t.tw, t.th, l1, u, i1, gx, gy);
+        const float* T0 = t.tex[l0];
+        const float* T1 = t.tex[l1];
+        for (int c = 0; c < C; c++) {
+            float v = lerpf_(lerpf_(T0[i0.x * C + c], T0[i0.y * C + c], fx), lerpf_(T0[i0.z * C + c], T0[i0.w * C + c], fx), fy);
+            if (t.L > 1 && f > 0) {
+                float v1 = lerpf_(lerpf_(T1[i1.x * C + c], T1[i1.y * C + c], gx), lerpf_(T1[i1.z * C + c], T1[i1.w * C + c], gx), gy);
+                v = lerpf_(v, v1, f);
+            }
+            out[p * C + c] = v;
+        }
+    }
+};
+__global__ void __launch_bounds__(BS) k_tex_fwd(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) { KTexFwdF{}(vb_hw(), P, t, rast, uv, uvDA, out); }
+
+struct KTexBwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, TexArgs t, TexGradArgs g, const float* rast, const float* uv, const float* uvDA,
+                                                const float* gout, float* guv, float* guvDA, int mode) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        if (rast[4 * p + 3] < 1.f) return;
+        int C = t.C;
+        float2 u = ((const float2*)uv)[p];
+        if (t.L <= 1) {
+            int4 i0; float fx, fy;
+            index_fetch(t.tw, t.th, 0, u, i0, fx, fy);
+            float gu = 0.f, gv = 0.f;
+            const float* T0 = t.tex[0];
+            float* G0 = g.tex[0];
+            for (int c = 0; c < C; c++) {
+                float go = gout[p * C + c];
+                if (G0) {
+                    atomicAdd(G0 + i0.x * C + c, (1.f - fx) * (1.f - fy) * go);
+                    atomicAdd(G0 + i0.y * C + c, fx * (1.f - fy) * go);
+                    atomicAdd(G0 + i0.z * C + c, (1.f - fx) * fy * go);
+                    atomicAdd(G0 + i0.w * C + c, fx * fy * go);
+                }
+                float t00 = T0[i0.x * C + c], t01 = T0[i0.y * C + c], t10 = T0[i0.z * C + c], t11 = T0[i0.w * C + c];
+                gu += lerpf_(t01 - t00, t11 - t10, fy) * (float)t.tw * go;
+                gv += lerpf_(t10 - t00, t11 - t01, fx) * (float)t.th * go;
+            }
+            if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
+            return;
+        }
+        int l0, l1; float f, lev; float4 dl;
+        calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, &dl, mode);
+        int4 i0, i1; float fx, fy, hx, hy;
+        index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
+        index_fetch(t.tw, t.th, l1, u, i1, hx, hy);
+        const float* T0 = t.tex[l0];
+        const float* T1 = t.tex[l1];
+        float* G0 = g.tex[l0];
+        float* G1 = g.tex[l1];
+        float gu = 0.f, gv = 0.f, gl = 0.f;
+        float w0 = 1.f - f;
         for (int c = 0; c < C; c++) {
             float go = gout[p * C + c];
             if (G0) {
-                atomicAdd(G0 + i0.x * C + c, (1.f - fx) * (1.f - fy) * go);
-                atomicAdd(G0 + i0.y * C + c, fx * (1.f - fy) * go);
-                atomicAdd(G0 + i0.z * C + c, (1.f - fx) * fy * go);
-                atomicAdd(G0 + i0.w * C + c, fx * fy * go);
+                atomicAdd(G0 + i0.x * C + c, w0 * (1.f - fx) * (1.f - fy) * go);
+                atomicAdd(G0 + i0.y * C + c, w0 * fx * (1.f - fy) * go);
+                atomicAdd(G0 + i0.z * C + c, w0 * (1.f - fx) * fy * go);
+                atomicAdd(G0 + i0.w * C + c, w0 * fx * fy * go);
             }
             float t00 = T0[i0.x * C + c], t01 = T0[i0.y * C + c], t10 = T0[i0.z * C + c], t11 = T0[i0.w * C + c];
-            gu += lerpf_(t01 - t00, t11 - t10, fy) * (float)t.tw * go;
-            gv += lerpf_(t10 - t00, t11 - t01, fx) * (float)t.th * go;
+            float du = lerpf_(t01 - t00, t11 - t10, fy) * (float)(t.tw >> l0);
+            float dv = lerpf_(t10 - t00, t11 - t01, fx) * (float)(t.th >> l0);
+            if (f > 0) {
+                float lv = lerpf_(lerpf_(t00, t01, fx), lerpf_(t10, t11, fx), fy);
+                if (G1) {
+                    atomicAdd(G1 + i1.x * C + c, f * (1.f - hx) * (1.f - hy) * go);
+                    atomicAdd(G1 + i1.y * C + c, f * hx * (1.f - hy) * go);
+                    atomicAdd(G1 + i1.z * C + c, f * (1.f - hx) * hy * go);
+                    atomicAdd(G1 + i1.w * C + c, f * hx * hy * go);
+                }
+                float s00 = T1[i1.x * C + c], s01 = T1[i1.y * C + c], s10 = T1[i1.z * C + c], s11 = T1[i1.w * C + c];
+                du = lerpf_(du, lerpf_(s01 - s00, s11 - s10, hy) * (float)(t.tw >> l1), f);
+                dv = lerpf_(dv, lerpf_(s10 - s00, s11 - s01, hx) * (float)(t.th >> l1), f);
+                gl += (lerpf_(lerpf_(s00, s01, hx), lerpf_(s10, s11, hx), hy) - lv) * go;
+            }
+            gu += du * go;
+            gv += dv * go;
         }
         if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
-        return;
+        if (guvDA) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
     }
-    int l0, l1; float f, lev; float4 dl;
-    calc_level(((const float4*)uvDA)[p], t.tw, t.th, t.L, l0, l1, f, lev, &dl, mode);
-    int4 i0, i1; float fx, fy, hx, hy;
-    index_fetch(t.tw, t.th, l0, u, i0, fx, fy);
-    index_fetch(t.tw, t.th, l1, u, i1, hx, hy);
-    const float* T0 = t.tex[l0];
-    const float* T1 = t.tex[l1];
-    float* G0 = g.tex[l0];
-    float* G1 = g.tex[l1];
-    float gu = 0.f, gv = 0.f, gl = 0.f;
-    float w0 = 1.f - f;
-    for (int c = 0; c < C; c++) {
-        float go = gout[p * C + c];
-        if (G0) {
-            atomicAdd(G0 + i0.x * C + c, w0 * (1.f - fx) * (1.f - fy) * go);
-            atomicAdd(G0 + i0.y * C + c, w0 * fx * (1.f - fy) * go);
-            atomicAdd(G0 + i0.z * C + c, w0 * (1.f - fx) * fy * go);
-            atomicAdd(G0 + i0.w * C + c, w0 * fx * fy * go);
-        }
-        float t00 = T0[i0.x * C + c], t01 = T0[i0.y * C + c], t10 = T0[i0.z * C + c], t11 = T0[i0.w * C + c];
-        float du = lerpf_(t01 - t00, t11 - t10, fy) * (float)(t.tw >> l0);
-        float dv = lerpf_(t10 - t00, t11 - t01, fx) * (float)(t.th >> l0);
-        if (f > 0) {
-            float lv = lerpf_(lerpf_(t00, t01, fx), lerpf_(t10, t11, fx), fy);
-            if (G1) {
-                atomicAdd(G1 + i1.x * C + c, f * (1.f - hx) * (1.f - hy) * go);
-                atomicAdd(G1 + i1.y * C + c, f * hx * (1.f - hy) * go);
-                atomicAdd(G1 + i1.z * C + c, f * (1.f - hx) * hy * go);
-                atomicAdd(G1 + i1.w * C + c, f * hx * hy * go);
-            }
-            float s00 = T1[i1.x * C + c], s01 = T1[i1.y * C + c], s10 = T1[i1.z * C + c], s11 = T1[i1.w * C + c];
-            du = lerpf_(du, lerpf_(s01 - s00, s11 - s10, hy) * (float)(t.tw >> l1), f);
-            dv = lerpf_(dv, lerpf_(s10 - s00, s11 - s01, hx) * (float)(t.th >> l1), f);
-            gl += (lerpf_(lerpf_(s00, s01, hx), lerpf_(s10, s11, hx), hy) - lv) * go;
-        }
-        gu += du * go;
-        gv += dv * go;
-    }
-    if (guv) ((float2*)guv)[p] = make_float2(gu, gv);
-    if (guvDA) ((float4*)guvDA)[p] = make_float4(gl * dl.x, gl * dl.y, gl * dl.z, gl * dl.w);
-}
+};
+__global__ void __launch_bounds__(BS) k_tex_bwd(size_t P, TexArgs t, TexGradArgs g, const float* rast, const float* uv, const float* uvDA,
+                                                const float* gout, float* guv, float* guvDA, int mode) { KTexBwdF{}(vb_hw(), P, t, g, rast, uv, uvDA, gout, guv, guvDA, mode); }
 
 // C == 4: texel grads of a workgroup's pixels are summed in an LDS table of NB buckets, one
 // bucket per run of 4 consecutive texels of a level (one 64-byte line of float4 grads), keyed by
@@ -641,94 +691,113 @@ constexpr int tex_table_buckets(int rows) { return rows <= 16 ? 256 : 512; }
 // with one set of atomics per touched texel.  A first pass over the covered pixels' go finds the
 // region's fixed-point scale.
 template <int ROWS>
-__global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
+struct KTexBwd4TiledF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
                                                        const float* uvDA, const float* gout, float* guv, float* guvDA, int mode,
-                                                       InterpBwdArgs ib) {
-    constexpr int NPX = ROWS / 16;
-    __shared__ TexGradTable<tex_table_buckets(ROWS)> tab;
-    const bool want_tex = g.tex[0] != nullptr;
-    const int view = blockIdx.y;
-    const int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
-    // wave w takes the pixels of parity class (w & 1, w >> 1) of each 16 x 16 band, so its lanes
-    // sit two pixels apart and fewer of them add to the same texel in one LDS instruction
-    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
-    const int lx = 2 * (ln & 7) + (wv & 1), ly = 2 * (ln >> 3) + (wv >> 1);
-    const int px = tx * 16 + lx;
-    auto pixel = [&](int k) { return ((size_t)view * H + ty * ROWS + k * 16 + ly) * W + px; };
-    auto covered = [&](int k) { return px < W && ty * ROWS + k * 16 + ly < H && rast[4 * pixel(k) + 3] >= 1.f; };
-    int S = 0;
-    bool splat = false, direct = false;
-    if (want_tex) {
-        tex_table_clear(tab);
-        u32 mbits = 0;
+                                                       InterpBwdArgs ib) const {
+        constexpr int NPX = ROWS / 16;
+        __shared__ TexGradTable<tex_table_buckets(ROWS)> tab;
+        const bool want_tex = g.tex[0] != nullptr;
+        const int view = vb.b.y;
+        const int tx = vb.b.x % tilesX, ty = vb.b.x / tilesX;
+        // wave w takes the pixels of parity class (w & 1, w >> 1) of each 16 x 16 band, so its lanes
+        // sit two pixels apart and fewer of them add to the same texel in one LDS instruction
+        const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+        const int lx = 2 * (ln & 7) + (wv & 1), ly = 2 * (ln >> 3) + (wv >> 1);
+        const int px = tx * 16 + lx;
+        auto pixel = [&](int k) { return ((size_t)view * H + ty * ROWS + k * 16 + ly) * W + px; };
+        auto covered = [&](int k) { return px < W && ty * ROWS + k * 16 + ly < H && rast[4 * pixel(k) + 3] >= 1.f; };
+        int S = 0;
+        bool splat = false, direct = false;
+        if (want_tex) {
+            tex_table_clear(tab);
+            u32 mbits = 0;
 #pragma unroll
+            for (int k = 0; k < NPX; k++) {
+                if (!covered(k)) continue;
+                const float4 go = ((const float4*)gout)[pixel(k)];
+                mbits = max(mbits, max(max(__float_as_uint(fabsf(go.x)), __float_as_uint(fabsf(go.y))),
+                                       max(__float_as_uint(fabsf(go.z)), __float_as_uint(fabsf(go.w)))));
+            }
+            mbits = wave_umax(mbits);
+            if ((threadIdx.x & 63) == 0) tab.wave_max[threadIdx.x >> 6] = mbits;
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < BS / 64; w++) mbits = max(mbits, tab.wave_max[w]);
+            // all-zero go adds nothing; a non-finite one takes the float path
+            splat = mbits != 0;
+            direct = mbits >= FIXED_NONFINITE;
+            S = fixed_scale(mbits);
+        }
+#pragma unroll 1
         for (int k = 0; k < NPX; k++) {
             if (!covered(k)) continue;
-            const float4 go = ((const float4*)gout)[pixel(k)];
-            mbits = max(mbits, max(max(__float_as_uint(fabsf(go.x)), __float_as_uint(fabsf(go.y))),
-                                   max(__float_as_uint(fabsf(go.z)), __float_as_uint(fabsf(go.w)))));
+            const size_t p = pixel(k);
+            const float4 go = ((const float4*)gout)[p];
+            const float4 gs = make_float4(ldexpf(go.x, S), ldexpf(go.y, S), ldexpf(go.z, S), ldexpf(go.w, S));
+            tex_bwd4_pixel(tab, splat, direct, p, go, gs, t, g, uv, uvDA, guv, guvDA, mode, ib);
         }
-        mbits = wave_umax(mbits);
-        if ((threadIdx.x & 63) == 0) tab.wave_max[threadIdx.x >> 6] = mbits;
+        if (!splat || direct) return;
         __syncthreads();
-#pragma unroll
-        for (int w = 0; w < BS / 64; w++) mbits = max(mbits, tab.wave_max[w]);
-        // all-zero go adds nothing; a non-finite one takes the float path
-        splat = mbits != 0;
-        direct = mbits >= FIXED_NONFINITE;
-        S = fixed_scale(mbits);
+        tex_table_flush(tab, g, S);
     }
-#pragma unroll 1
-    for (int k = 0; k < NPX; k++) {
-        if (!covered(k)) continue;
-        const size_t p = pixel(k);
-        const float4 go = ((const float4*)gout)[p];
-        const float4 gs = make_float4(ldexpf(go.x, S), ldexpf(go.y, S), ldexpf(go.z, S), ldexpf(go.w, S));
-        tex_bwd4_pixel(tab, splat, direct, p, go, gs, t, g, uv, uvDA, guv, guvDA, mode, ib);
-    }
-    if (!splat || direct) return;
-    __syncthreads();
-    tex_table_flush(tab, g, S);
-}
+};
+template <int ROWS>
+__global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
+                                                       const float* uvDA, const float* gout, float* guv, float* guvDA, int mode,
+                                                       InterpBwdArgs ib) { KTexBwd4TiledF<ROWS>{}(vb_hw(), W, H, tilesX, t, g, rast, uv, uvDA, gout, guv, guvDA, mode, ib); }
 
 // ---------------------------------------------------------------- mip utilities
-__global__ void __launch_bounds__(BS) k_downsample(const float* src, float* dst, int w, int h, int C, float* zero_grad) {
+struct KDownsampleF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, const float* src, float* dst, int w, int h, int C, float* zero_grad) const {
 #pragma clang fp contract(off)
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)w * h * C;
-    if (i >= n) return;
-    int c = (int)(i % C);
-    size_t q = i / C;
-    int x = (int)(q % w), y = (int)(q / w);
-    int sw = 2 * w;
-    float p00 = src[((size_t)(2 * y) * sw + 2 * x) * C + c], p01 = src[((size_t)(2 * y) * sw + 2 * x + 1) * C + c];
-    float p10 = src[((size_t)(2 * y + 1) * sw + 2 * x) * C + c], p11 = src[((size_t)(2 * y + 1) * sw + 2 * x + 1) * C + c];
-    dst[i] = (p00 + p01 + p10 + p11) * 0.25f;
-    if (zero_grad) zero_grad[i] = 0.f;
-}
+        size_t i = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        size_t n = (size_t)w * h * C;
+        if (i >= n) return;
+        int c = (int)(i % C);
+        size_t q = i / C;
+        int x = (int)(q % w), y = (int)(q / w);
+        int sw = 2 * w;
+        float p00 = src[((size_t)(2 * y) * sw + 2 * x) * C + c], p01 = src[((size_t)(2 * y) * sw + 2 * x + 1) * C + c];
+        float p10 = src[((size_t)(2 * y + 1) * sw + 2 * x) * C + c], p11 = src[((size_t)(2 * y + 1) * sw + 2 * x + 1) * C + c];
+        dst[i] = (p00 + p01 + p10 + p11) * 0.25f;
+        if (zero_grad) zero_grad[i] = 0.f;
+    }
+};
+__global__ void __launch_bounds__(BS) k_downsample(const float* src, float* dst, int w, int h, int C, float* zero_grad) { KDownsampleF{}(vb_hw(), src, dst, w, h, C, zero_grad); }
 
-__global__ void __launch_bounds__(BS) k_fold(const float* src, float* dst, int w, int h, int C, float scale) {
+struct KFoldF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, const float* src, float* dst, int w, int h, int C, float scale) const {
 #pragma clang fp contract(off)
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)w * h * C;
-    if (i >= n) return;
-    int c = (int)(i % C);
-    size_t q = i / C;
-    int x = (int)(q % w), y = (int)(q / w);
-    int dw = 2 * w;
-    float g = src[i] * scale;
-    add_nan_check(dst[((size_t)(2 * y) * dw + 2 * x) * C + c], g);
-    add_nan_check(dst[((size_t)(2 * y) * dw + 2 * x + 1) * C + c], g);
-    add_nan_check(dst[((size_t)(2 * y + 1) * dw + 2 * x) * C + c], g);
-    add_nan_check(dst[((size_t)(2 * y + 1) * dw + 2 * x + 1) * C + c], g);
-}
+        size_t i = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        size_t n = (size_t)w * h * C;
+        if (i >= n) return;
+        int c = (int)(i % C);
+        size_t q = i / C;
+        int x = (int)(q % w), y = (int)(q / w);
+        int dw = 2 * w;
+        float g = src[i] * scale;
+        add_nan_check(dst[((size_t)(2 * y) * dw + 2 * x) * C + c], g);
+        add_nan_check(dst[((size_t)(2 * y) * dw + 2 * x + 1) * C + c], g);
+        add_nan_check(dst[((size_t)(2 * y + 1) * dw + 2 * x) * C + c], g);
+        add_nan_check(dst[((size_t)(2 * y + 1) * dw + 2 * x + 1) * C + c], g);
+    }
+};
+__global__ void __launch_bounds__(BS) k_fold(const float* src, float* dst, int w, int h, int C, float scale) { KFoldF{}(vb_hw(), src, dst, w, h, C, scale); }
 
-__global__ void __launch_bounds__(BS) k_clamp(float* x, size_t n, float lo, float hi) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float v = x[i];
-    x[i] = v < lo ? lo : v > hi ? hi : v;
-}
+struct KClampF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, float* x, size_t n, float lo, float hi) const {
+        size_t i = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (i >= n) return;
+        float v = x[i];
+        x[i] = v < lo ? lo : v > hi ? hi : v;
+    }
+};
+__global__ void __launch_bounds__(BS) k_clamp(float* x, size_t n, float lo, float hi) { KClampF{}(vb_hw(), x, n, lo, hi); }
 
 // ---------------------------------------------------------------- Adam
 constexpr int ADAM_MAX = 8;
@@ -824,31 +893,35 @@ __device__ __forceinline__ void adam_elem(float& param, float g, float& m, float
 
 constexpr int ADAM_PER_BLOCK = BS * 4;
 
-__global__ void __launch_bounds__(BS) k_adam(AdamArgs a) {
-    if (a.loss && blockIdx.x == gridDim.x - 1) {
-        mse_final_block<BS>(a.loss_part, a.loss_nb, a.loss);
-        return;
+struct KAdamF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, AdamArgs a) const {
+        if (a.loss && vb.b.x == vb.g.x - 1) {
+            mse_final_block<BS>(a.loss_part, a.loss_nb, a.loss);
+            return;
+        }
+        long long b = vb.b.x;
+        int t = 0;
+        while (t + 1 < a.n && b >= a.block_start[t + 1]) t++;
+        const int ept = a.per_thread[t];
+        long long base = (b - a.block_start[t]) * BS * ept;
+        float* P = a.param[t]; const float* G = a.grad[t]; float* M = a.m[t]; float* Vv = a.v[t];
+        long long n = a.size[t];
+        const double rmt = a.bt ? a.bt[0] : a.rhomt[t], rvt = a.bt ? a.bt[1] : a.rhovt[t];
+        for (int k = 0; k < ept; k++) {
+            long long i = base + k * BS + threadIdx.x;
+            if (i >= n) break;
+            float p = P[i], m = M[i], v = Vv[i];
+            float g = G[i];
+            if (t == 0 && a.pgclip) g += adam_proj_grad(a, i);
+            if (t >= 1 && t <= 3 && a.lseg) add_nan_check(g, adam_lobe_sum(a, t, i));
+            adam_elem(p, g, m, v, a.rhom[t], a.rhov[t], rmt, rvt, a.eta[t], a.eps[t]);
+            P[i] = p; M[i] = m; Vv[i] = v;
+            if (a.zero_grad) const_cast<float*>(G)[i] = 0.f;
+        }
     }
-    long long b = blockIdx.x;
-    int t = 0;
-    while (t + 1 < a.n && b >= a.block_start[t + 1]) t++;
-    const int ept = a.per_thread[t];
-    long long base = (b - a.block_start[t]) * BS * ept;
-    float* P = a.param[t]; const float* G = a.grad[t]; float* M = a.m[t]; float* Vv = a.v[t];
-    long long n = a.size[t];
-    const double rmt = a.bt ? a.bt[0] : a.rhomt[t], rvt = a.bt ? a.bt[1] : a.rhovt[t];
-    for (int k = 0; k < ept; k++) {
-        long long i = base + k * BS + threadIdx.x;
-        if (i >= n) break;
-        float p = P[i], m = M[i], v = Vv[i];
-        float g = G[i];
-        if (t == 0 && a.pgclip) g += adam_proj_grad(a, i);
-        if (t >= 1 && t <= 3 && a.lseg) add_nan_check(g, adam_lobe_sum(a, t, i));
-        adam_elem(p, g, m, v, a.rhom[t], a.rhov[t], rmt, rvt, a.eta[t], a.eps[t]);
-        P[i] = p; M[i] = m; Vv[i] = v;
-        if (a.zero_grad) const_cast<float*>(G)[i] = 0.f;
-    }
-}
+};
+__global__ void __launch_bounds__(BS) k_adam(AdamArgs a) { KAdamF{}(vb_hw(), a); }
 
 // Fused texture update on level 0: folded grad (coarse to fine, AddNaNcheck chain of
 // gardAddThrough, buffer.cu:615-636) -> Adam -> clamp; level-0 grad zeroed.
@@ -866,32 +939,36 @@ struct TexUpdArgs {
 
 // Adam::step bumps rhomt/rhovt before the launch (optimize.cu:109-115); kept on the device so
 // a captured step advances them on every replay.
-__global__ void __launch_bounds__(BS) k_tex_update(TexUpdArgs a) {
+struct KTexUpdateF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, TexUpdArgs a) const {
 #pragma clang fp contract(off)
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)a.tw * a.th * a.C;
-    if (i >= n) return;
-    int c = (int)(i % a.C);
-    size_t q = i / a.C;
-    int x = (int)(q % a.tw), y = (int)(q / a.tw);
-    float acc = 0.f;
-    if (a.L > 1) {
-        int l = a.L - 1;
-        acc = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * a.C + c];
-        for (l = a.L - 2; l >= 1; l--) {
-            float tmp = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * a.C + c];
-            add_nan_check(tmp, acc * a.scale);
-            acc = tmp;
+        size_t i = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        size_t n = (size_t)a.tw * a.th * a.C;
+        if (i >= n) return;
+        int c = (int)(i % a.C);
+        size_t q = i / a.C;
+        int x = (int)(q % a.tw), y = (int)(q / a.tw);
+        float acc = 0.f;
+        if (a.L > 1) {
+            int l = a.L - 1;
+            acc = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * a.C + c];
+            for (l = a.L - 2; l >= 1; l--) {
+                float tmp = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * a.C + c];
+                add_nan_check(tmp, acc * a.scale);
+                acc = tmp;
+            }
         }
+        float g = a.grad[0][i];
+        if (a.L > 1) add_nan_check(g, acc * a.scale);
+        float p = a.tex0[i], m = a.m[i], v = a.v[i];
+        adam_elem(p, g, m, v, a.rhom, a.rhov, a.bt ? a.bt[0] : a.rhomt, a.bt ? a.bt[1] : a.rhovt, a.eta, a.eps);
+        p = p < a.lo ? a.lo : p > a.hi ? a.hi : p;
+        a.tex0[i] = p; a.m[i] = m; a.v[i] = v;
+        a.grad[0][i] = 0.f;
     }
-    float g = a.grad[0][i];
-    if (a.L > 1) add_nan_check(g, acc * a.scale);
-    float p = a.tex0[i], m = a.m[i], v = a.v[i];
-    adam_elem(p, g, m, v, a.rhom, a.rhov, a.bt ? a.bt[0] : a.rhomt, a.bt ? a.bt[1] : a.rhovt, a.eta, a.eps);
-    p = p < a.lo ? a.lo : p > a.hi ? a.hi : p;
-    a.tex0[i] = p; a.m[i] = m; a.v[i] = v;
-    a.grad[0][i] = 0.f;
-}
+};
+__global__ void __launch_bounds__(BS) k_tex_update(TexUpdArgs a) { KTexUpdateF{}(vb_hw(), a); }
 
 // C == 4: one thread per level-0 texel, float4 traffic; every ancestor grad is loaded up front
 // (compile-time-unrolled levels) so the fold costs one memory latency, not one per level.
@@ -899,43 +976,47 @@ __device__ __forceinline__ void add_nan_check4(float4& a, float4 b) {
     add_nan_check(a.x, b.x); add_nan_check(a.y, b.y); add_nan_check(a.z, b.z); add_nan_check(a.w, b.w);
 }
 
-__global__ void __launch_bounds__(BS) k_tex_update4(TexUpdArgs a) {
+struct KTexUpdate4F {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, TexUpdArgs a) const {
 #pragma clang fp contract(off)
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t n = (size_t)a.tw * a.th;
-    if (i >= n) return;
-    int x = (int)(i % a.tw), y = (int)(i / a.tw);
-    float4 anc[VCL_TEX_MAX_MIP_LEVEL];
+        size_t i = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        size_t n = (size_t)a.tw * a.th;
+        if (i >= n) return;
+        int x = (int)(i % a.tw), y = (int)(i / a.tw);
+        float4 anc[VCL_TEX_MAX_MIP_LEVEL];
 #pragma unroll
-    for (int l = 1; l < VCL_TEX_MAX_MIP_LEVEL; l++)
-        if (l < a.L) anc[l] = ((const float4*)a.grad[l])[(size_t)(y >> l) * (a.tw >> l) + (x >> l)];
-    float4 g = ((const float4*)a.grad[0])[i];
-    float4 p = ((const float4*)a.tex0)[i], m = ((const float4*)a.m)[i], v = ((const float4*)a.v)[i];
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int l = 1; l < VCL_TEX_MAX_MIP_LEVEL; l++)
+            if (l < a.L) anc[l] = ((const float4*)a.grad[l])[(size_t)(y >> l) * (a.tw >> l) + (x >> l)];
+        float4 g = ((const float4*)a.grad[0])[i];
+        float4 p = ((const float4*)a.tex0)[i], m = ((const float4*)a.m)[i], v = ((const float4*)a.v)[i];
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-    for (int l = VCL_TEX_MAX_MIP_LEVEL - 1; l >= 1; l--) {
-        if (l == a.L - 1) acc = anc[l];
-        else if (l < a.L - 1) {
-            float4 t = anc[l];
-            add_nan_check4(t, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
-            acc = t;
+        for (int l = VCL_TEX_MAX_MIP_LEVEL - 1; l >= 1; l--) {
+            if (l == a.L - 1) acc = anc[l];
+            else if (l < a.L - 1) {
+                float4 t = anc[l];
+                add_nan_check4(t, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
+                acc = t;
+            }
         }
+        if (a.L > 1) add_nan_check4(g, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
+        const double rmt = a.bt ? a.bt[0] : a.rhomt, rvt = a.bt ? a.bt[1] : a.rhovt;
+        adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+        adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+        adam_elem(p.z, g.z, m.z, v.z, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+        adam_elem(p.w, g.w, m.w, v.w, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+        p.x = p.x < a.lo ? a.lo : p.x > a.hi ? a.hi : p.x;
+        p.y = p.y < a.lo ? a.lo : p.y > a.hi ? a.hi : p.y;
+        p.z = p.z < a.lo ? a.lo : p.z > a.hi ? a.hi : p.z;
+        p.w = p.w < a.lo ? a.lo : p.w > a.hi ? a.hi : p.w;
+        ((float4*)a.tex0)[i] = p;
+        ((float4*)a.m)[i] = m;
+        ((float4*)a.v)[i] = v;
+        ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    if (a.L > 1) add_nan_check4(g, make_float4(acc.x * a.scale, acc.y * a.scale, acc.z * a.scale, acc.w * a.scale));
-    const double rmt = a.bt ? a.bt[0] : a.rhomt, rvt = a.bt ? a.bt[1] : a.rhovt;
-    adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
-    adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
-    adam_elem(p.z, g.z, m.z, v.z, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
-    adam_elem(p.w, g.w, m.w, v.w, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
-    p.x = p.x < a.lo ? a.lo : p.x > a.hi ? a.hi : p.x;
-    p.y = p.y < a.lo ? a.lo : p.y > a.hi ? a.hi : p.y;
-    p.z = p.z < a.lo ? a.lo : p.z > a.hi ? a.hi : p.z;
-    p.w = p.w < a.lo ? a.lo : p.w > a.hi ? a.hi : p.w;
-    ((float4*)a.tex0)[i] = p;
-    ((float4*)a.m)[i] = m;
-    ((float4*)a.v)[i] = v;
-    ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
+};
+__global__ void __launch_bounds__(BS) k_tex_update4(TexUpdArgs a) { KTexUpdate4F{}(vb_hw(), a); }
 
 // Lane S's v in every lane of its quad (lanes 4k .. 4k + 3; DPP quad_perm, no LDS).
 template <int S>
@@ -951,77 +1032,85 @@ __device__ __forceinline__ float quad_lane(float v) {
 // four results are broadcast in the quad.  After the updates the quad's first lane gathers the
 // other three new values and writes their mean in k_downsample4's order, and clears the level-1
 // grad (which the quad read for the fold before).
-__global__ void __launch_bounds__(BS) k_tex_update4m(TexUpdArgs a) {
+struct KTexUpdate4mF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, TexUpdArgs a) const {
 #pragma clang fp contract(off)
-    const int lane = threadIdx.x & 63;
-    const int c = lane & 3;
-    const int x = blockIdx.x * 128 + (threadIdx.x >> 6) * 32 + 2 * (lane >> 2) + (lane & 1);
-    const int y = blockIdx.y * 2 + ((lane >> 1) & 1);
-    const bool in = x < a.tw;   // a.tw and a.th are even: a quad is all in or all out
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 p = z4, g = z4, m = z4, v = z4;
-    float acc = 0.f;
-    const size_t i = (size_t)y * a.tw + x;
-    if (in) {
-        float anc[VCL_TEX_MAX_MIP_LEVEL];
+        const int lane = threadIdx.x & 63;
+        const int c = lane & 3;
+        const int x = vb.b.x * 128 + (threadIdx.x >> 6) * 32 + 2 * (lane >> 2) + (lane & 1);
+        const int y = vb.b.y * 2 + ((lane >> 1) & 1);
+        const bool in = x < a.tw;   // a.tw and a.th are even: a quad is all in or all out
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 p = z4, g = z4, m = z4, v = z4;
+        float acc = 0.f;
+        const size_t i = (size_t)y * a.tw + x;
+        if (in) {
+            float anc[VCL_TEX_MAX_MIP_LEVEL];
 #pragma unroll
-        for (int l = 1; l < VCL_TEX_MAX_MIP_LEVEL; l++)
-            if (l < a.L) anc[l] = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * 4 + c];
-        g = ((const float4*)a.grad[0])[i];
-        m = ((const float4*)a.m)[i];
-        v = ((const float4*)a.v)[i];
-        p = ((const float4*)a.tex0)[i];
+            for (int l = 1; l < VCL_TEX_MAX_MIP_LEVEL; l++)
+                if (l < a.L) anc[l] = a.grad[l][((size_t)(y >> l) * (a.tw >> l) + (x >> l)) * 4 + c];
+            g = ((const float4*)a.grad[0])[i];
+            m = ((const float4*)a.m)[i];
+            v = ((const float4*)a.v)[i];
+            p = ((const float4*)a.tex0)[i];
 #pragma unroll
-        for (int l = VCL_TEX_MAX_MIP_LEVEL - 1; l >= 1; l--) {
-            if (l == a.L - 1) acc = anc[l];
-            else if (l < a.L - 1) {
-                float t = anc[l];
-                add_nan_check(t, acc * a.scale);
-                acc = t;
+            for (int l = VCL_TEX_MAX_MIP_LEVEL - 1; l >= 1; l--) {
+                if (l == a.L - 1) acc = anc[l];
+                else if (l < a.L - 1) {
+                    float t = anc[l];
+                    add_nan_check(t, acc * a.scale);
+                    acc = t;
+                }
             }
         }
+        const float4 acc4 = make_float4(quad_lane<0>(acc), quad_lane<1>(acc), quad_lane<2>(acc), quad_lane<3>(acc));
+        if (in) {
+            add_nan_check4(g, make_float4(acc4.x * a.scale, acc4.y * a.scale, acc4.z * a.scale, acc4.w * a.scale));
+            const double rmt = a.bt ? a.bt[0] : a.rhomt, rvt = a.bt ? a.bt[1] : a.rhovt;
+            adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+            adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+            adam_elem(p.z, g.z, m.z, v.z, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+            adam_elem(p.w, g.w, m.w, v.w, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
+            p.x = p.x < a.lo ? a.lo : p.x > a.hi ? a.hi : p.x;
+            p.y = p.y < a.lo ? a.lo : p.y > a.hi ? a.hi : p.y;
+            p.z = p.z < a.lo ? a.lo : p.z > a.hi ? a.hi : p.z;
+            p.w = p.w < a.lo ? a.lo : p.w > a.hi ? a.hi : p.w;
+            ((float4*)a.tex0)[i] = p;
+            ((float4*)a.m)[i] = m;
+            ((float4*)a.v)[i] = v;
+            ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        const float4 p01 = make_float4(quad_lane<1>(p.x), quad_lane<1>(p.y), quad_lane<1>(p.z), quad_lane<1>(p.w));
+        const float4 p10 = make_float4(quad_lane<2>(p.x), quad_lane<2>(p.y), quad_lane<2>(p.z), quad_lane<2>(p.w));
+        const float4 p11 = make_float4(quad_lane<3>(p.x), quad_lane<3>(p.y), quad_lane<3>(p.z), quad_lane<3>(p.w));
+        if (in && c == 0) {
+            const size_t j = (size_t)(y >> 1) * (a.tw >> 1) + (x >> 1);
+            ((float4*)a.tex1)[j] = make_float4((((p.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p.y + p01.y) + p10.y) + p11.y) * 0.25f,
+                                               (((p.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p.w + p01.w) + p10.w) + p11.w) * 0.25f);
+            ((float4*)a.grad[1])[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
-    const float4 acc4 = make_float4(quad_lane<0>(acc), quad_lane<1>(acc), quad_lane<2>(acc), quad_lane<3>(acc));
-    if (in) {
-        add_nan_check4(g, make_float4(acc4.x * a.scale, acc4.y * a.scale, acc4.z * a.scale, acc4.w * a.scale));
-        const double rmt = a.bt ? a.bt[0] : a.rhomt, rvt = a.bt ? a.bt[1] : a.rhovt;
-        adam_elem(p.x, g.x, m.x, v.x, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
-        adam_elem(p.y, g.y, m.y, v.y, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
-        adam_elem(p.z, g.z, m.z, v.z, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
-        adam_elem(p.w, g.w, m.w, v.w, a.rhom, a.rhov, rmt, rvt, a.eta, a.eps);
-        p.x = p.x < a.lo ? a.lo : p.x > a.hi ? a.hi : p.x;
-        p.y = p.y < a.lo ? a.lo : p.y > a.hi ? a.hi : p.y;
-        p.z = p.z < a.lo ? a.lo : p.z > a.hi ? a.hi : p.z;
-        p.w = p.w < a.lo ? a.lo : p.w > a.hi ? a.hi : p.w;
-        ((float4*)a.tex0)[i] = p;
-        ((float4*)a.m)[i] = m;
-        ((float4*)a.v)[i] = v;
-        ((float4*)a.grad[0])[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-    const float4 p01 = make_float4(quad_lane<1>(p.x), quad_lane<1>(p.y), quad_lane<1>(p.z), quad_lane<1>(p.w));
-    const float4 p10 = make_float4(quad_lane<2>(p.x), quad_lane<2>(p.y), quad_lane<2>(p.z), quad_lane<2>(p.w));
-    const float4 p11 = make_float4(quad_lane<3>(p.x), quad_lane<3>(p.y), quad_lane<3>(p.z), quad_lane<3>(p.w));
-    if (in && c == 0) {
-        const size_t j = (size_t)(y >> 1) * (a.tw >> 1) + (x >> 1);
-        ((float4*)a.tex1)[j] = make_float4((((p.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p.y + p01.y) + p10.y) + p11.y) * 0.25f,
-                                           (((p.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p.w + p01.w) + p10.w) + p11.w) * 0.25f);
-        ((float4*)a.grad[1])[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-    }
-}
+};
+__global__ void __launch_bounds__(BS) k_tex_update4m(TexUpdArgs a) { KTexUpdate4mF{}(vb_hw(), a); }
 
 // C == 4 mip level from the level above (float4 per texel), clearing that level's grads.
-__global__ void __launch_bounds__(BS) k_downsample4(const float4* src, float4* dst, int w, int h, float4* zero_grad) {
+struct KDownsample4F {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, const float4* src, float4* dst, int w, int h, float4* zero_grad) const {
 #pragma clang fp contract(off)
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)w * h) return;
-    int x = (int)(i % w), y = (int)(i / w);
-    size_t sw = 2 * (size_t)w;
-    float4 p00 = src[(2 * y) * sw + 2 * x], p01 = src[(2 * y) * sw + 2 * x + 1];
-    float4 p10 = src[(2 * y + 1) * sw + 2 * x], p11 = src[(2 * y + 1) * sw + 2 * x + 1];
-    dst[i] = make_float4((((p00.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p00.y + p01.y) + p10.y) + p11.y) * 0.25f,
-                         (((p00.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p00.w + p01.w) + p10.w) + p11.w) * 0.25f);
-    if (zero_grad) zero_grad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-}
+        size_t i = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (i >= (size_t)w * h) return;
+        int x = (int)(i % w), y = (int)(i / w);
+        size_t sw = 2 * (size_t)w;
+        float4 p00 = src[(2 * y) * sw + 2 * x], p01 = src[(2 * y) * sw + 2 * x + 1];
+        float4 p10 = src[(2 * y + 1) * sw + 2 * x], p11 = src[(2 * y + 1) * sw + 2 * x + 1];
+        dst[i] = make_float4((((p00.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p00.y + p01.y) + p10.y) + p11.y) * 0.25f,
+                             (((p00.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p00.w + p01.w) + p10.w) + p11.w) * 0.25f);
+        if (zero_grad) zero_grad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+};
+__global__ void __launch_bounds__(BS) k_downsample4(const float4* src, float4* dst, int w, int h, float4* zero_grad) { KDownsample4F{}(vb_hw(), src, dst, w, h, zero_grad); }
 
 // Several mip levels in one launch: each workgroup reads a 64 x 64 block of level s and writes
 // the block's texels of levels s+1 .. s+nl (32^2 down to 1^2), each from the previous level
@@ -1034,38 +1123,42 @@ struct MipArgs {
     float4* zg[6];
 };
 
-__global__ void __launch_bounds__(BS) k_mip_fused(MipArgs a) {
+struct KMipFusedF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, MipArgs a) const {
 #pragma clang fp contract(off)
-    __shared__ float4 buf[2][32 * 32];
-    const int bx = blockIdx.x, by = blockIdx.y;
-    for (int k = 1; k <= a.nl; k++) {
-        const int bs = 64 >> k;                      // block side at level s+k
-        const int w = a.sw >> k, h = a.sh >> k;      // level size
-        const int ox = bx * bs, oy = by * bs;
-        float4* out = buf[k & 1];
-        const float4* in = buf[(k - 1) & 1];
-        for (int i = threadIdx.x; i < bs * bs; i += BS) {
-            const int lx = i % bs, ly = i / bs, x = ox + lx, y = oy + ly;
-            if (x >= w || y >= h) continue;
-            float4 p00, p01, p10, p11;
-            if (k == 1) {
-                const size_t sw = (size_t)a.sw;
-                p00 = a.src[(2 * y) * sw + 2 * x]; p01 = a.src[(2 * y) * sw + 2 * x + 1];
-                p10 = a.src[(2 * y + 1) * sw + 2 * x]; p11 = a.src[(2 * y + 1) * sw + 2 * x + 1];
-            } else {
-                const int pb = 2 * bs;
-                p00 = in[(2 * ly) * pb + 2 * lx]; p01 = in[(2 * ly) * pb + 2 * lx + 1];
-                p10 = in[(2 * ly + 1) * pb + 2 * lx]; p11 = in[(2 * ly + 1) * pb + 2 * lx + 1];
+        __shared__ float4 buf[2][32 * 32];
+        const int bx = vb.b.x, by = vb.b.y;
+        for (int k = 1; k <= a.nl; k++) {
+            const int bs = 64 >> k;                      // block side at level s+k
+            const int w = a.sw >> k, h = a.sh >> k;      // level size
+            const int ox = bx * bs, oy = by * bs;
+            float4* out = buf[k & 1];
+            const float4* in = buf[(k - 1) & 1];
+            for (int i = threadIdx.x; i < bs * bs; i += BS) {
+                const int lx = i % bs, ly = i / bs, x = ox + lx, y = oy + ly;
+                if (x >= w || y >= h) continue;
+                float4 p00, p01, p10, p11;
+                if (k == 1) {
+                    const size_t sw = (size_t)a.sw;
+                    p00 = a.src[(2 * y) * sw + 2 * x]; p01 = a.src[(2 * y) * sw + 2 * x + 1];
+                    p10 = a.src[(2 * y + 1) * sw + 2 * x]; p11 = a.src[(2 * y + 1) * sw + 2 * x + 1];
+                } else {
+                    const int pb = 2 * bs;
+                    p00 = in[(2 * ly) * pb + 2 * lx]; p01 = in[(2 * ly) * pb + 2 * lx + 1];
+                    p10 = in[(2 * ly + 1) * pb + 2 * lx]; p11 = in[(2 * ly + 1) * pb + 2 * lx + 1];
+                }
+                const float4 v = make_float4((((p00.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p00.y + p01.y) + p10.y) + p11.y) * 0.25f,
+                                             (((p00.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p00.w + p01.w) + p10.w) + p11.w) * 0.25f);
+                out[ly * bs + lx] = v;
+                a.dst[k - 1][(size_t)y * w + x] = v;
+                if (a.zg[k - 1]) a.zg[k - 1][(size_t)y * w + x] = make_float4(0.f, 0.f, 0.f, 0.f);
             }
-            const float4 v = make_float4((((p00.x + p01.x) + p10.x) + p11.x) * 0.25f, (((p00.y + p01.y) + p10.y) + p11.y) * 0.25f,
-                                         (((p00.z + p01.z) + p10.z) + p11.z) * 0.25f, (((p00.w + p01.w) + p10.w) + p11.w) * 0.25f);
-            out[ly * bs + lx] = v;
-            a.dst[k - 1][(size_t)y * w + x] = v;
-            if (a.zg[k - 1]) a.zg[k - 1][(size_t)y * w + x] = make_float4(0.f, 0.f, 0.f, 0.f);
+            __syncthreads();
         }
-        __syncthreads();
     }
-}
+};
+__global__ void __launch_bounds__(BS) k_mip_fused(MipArgs a) { KMipFusedF{}(vb_hw(), a); }
 
 // ---------------------------------------------------------------- NormalAxis / ViewAxis
 // NormalAxis of one pixel: interpolated vertex normal, normalised, rotated (zero off-mesh).
@@ -1088,13 +1181,17 @@ __device__ __forceinline__ void normal_axis_px(float4 r, const float* rot, const
     }
 }
 
-__global__ void __launch_bounds__(BS) k_normal_axis(size_t P, const float* rast, const float* rot, const float* normal, const u32* nidx, float* out) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float o0, o1, o2;
-    normal_axis_px(((const float4*)rast)[p], rot, normal, nidx, o0, o1, o2);
-    out[3 * p] = o0; out[3 * p + 1] = o1; out[3 * p + 2] = o2;
-}
+struct KNormalAxisF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, const float* rast, const float* rot, const float* normal, const u32* nidx, float* out) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        float o0, o1, o2;
+        normal_axis_px(((const float4*)rast)[p], rot, normal, nidx, o0, o1, o2);
+        out[3 * p] = o0; out[3 * p + 1] = o1; out[3 * p + 2] = o2;
+    }
+};
+__global__ void __launch_bounds__(BS) k_normal_axis(size_t P, const float* rast, const float* rot, const float* normal, const u32* nidx, float* out) { KNormalAxisF{}(vb_hw(), P, rast, rot, normal, nidx, out); }
 
 struct Mat3x8 { float m[8][9]; };
 
@@ -1108,14 +1205,18 @@ __device__ __forceinline__ void view_axis_px(int W, int H, int px, int py, const
     o0 = v0 * d; o1 = v1 * d; o2 = v2 * d;
 }
 
-__global__ void __launch_bounds__(BS) k_view_axis(int W, int H, int depth, const float* pvinv_dev, Mat3x8 pv_small, float* out) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t P = (size_t)W * H * depth;
-    if (p >= P) return;
-    int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
-    const float* m = pvinv_dev ? pvinv_dev + 9 * s : pv_small.m[s];
-    view_axis_px(W, H, px, py, m, out[3 * p], out[3 * p + 1], out[3 * p + 2]);
-}
+struct KViewAxisF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, int W, int H, int depth, const float* pvinv_dev, Mat3x8 pv_small, float* out) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        size_t P = (size_t)W * H * depth;
+        if (p >= P) return;
+        int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
+        const float* m = pvinv_dev ? pvinv_dev + 9 * s : pv_small.m[s];
+        view_axis_px(W, H, px, py, m, out[3 * p], out[3 * p + 1], out[3 * p + 2]);
+    }
+};
+__global__ void __launch_bounds__(BS) k_view_axis(int W, int H, int depth, const float* pvinv_dev, Mat3x8 pv_small, float* out) { KViewAxisF{}(vb_hw(), W, H, depth, pvinv_dev, pv_small, out); }
 
 // ---------------------------------------------------------------- SGSpecular
 __device__ __forceinline__ void spec_terms(float nx, float ny, float nz, float vx_, float vy_, float vz_, float m, float ior, float* p, float& amp,
@@ -1148,53 +1249,69 @@ __device__ __forceinline__ void spec_terms(float nx, float ny, float nz, float v
     }
 }
 
+struct KSgSpecFwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, const float* rast, const float* normal, const float* view, const float* rough,
+                                                    float ior, float* oAxis, float* oSharp, float* oAmp) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        float pa[3] = {0.f, 0.f, 0.f}, amp = 0.f, sh = 0.f;
+        if ((int)rast[4 * p + 3] - 1 >= 0)
+            spec_terms(normal[3 * p], normal[3 * p + 1], normal[3 * p + 2], view[3 * p], view[3 * p + 1], view[3 * p + 2], rough[p], ior, pa, amp, sh,
+                       nullptr, nullptr);
+        oAxis[3 * p] = pa[0]; oAxis[3 * p + 1] = pa[1]; oAxis[3 * p + 2] = pa[2];
+        oSharp[p] = sh; oAmp[p] = amp;
+    }
+};
 __global__ void __launch_bounds__(BS) k_sg_spec_fwd(size_t P, const float* rast, const float* normal, const float* view, const float* rough,
-                                                    float ior, float* oAxis, float* oSharp, float* oAmp) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float pa[3] = {0.f, 0.f, 0.f}, amp = 0.f, sh = 0.f;
-    if ((int)rast[4 * p + 3] - 1 >= 0)
-        spec_terms(normal[3 * p], normal[3 * p + 1], normal[3 * p + 2], view[3 * p], view[3 * p + 1], view[3 * p + 2], rough[p], ior, pa, amp, sh,
-                   nullptr, nullptr);
-    oAxis[3 * p] = pa[0]; oAxis[3 * p + 1] = pa[1]; oAxis[3 * p + 2] = pa[2];
-    oSharp[p] = sh; oAmp[p] = amp;
-}
+                                                    float ior, float* oAxis, float* oSharp, float* oAmp) { KSgSpecFwdF{}(vb_hw(), P, rast, normal, view, rough, ior, oAxis, oSharp, oAmp); }
 
 // The fit step's NormalAxis -> ViewAxis -> SGSpecular chain in one pass per pixel: the same
 // per-pixel code as the three ops, without the round trip of the normal and view axes through
 // memory between them.
+struct KShadePrepF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, int W, int H, int depth, const float* rast, const float* rot, const float* normal, const u32* nidx,
+                                                   const float* pvinv, const float* rough, float ior, float* onrm, float* oview, float* oAxis,
+                                                   float* oSharp, float* oAmp) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        size_t P = (size_t)W * H * depth;
+        if (p >= P) return;
+        const int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
+        const float4 r = ((const float4*)rast)[p];
+        float n0, n1, n2, v0, v1, v2;
+        normal_axis_px(r, rot, normal, nidx, n0, n1, n2);
+        view_axis_px(W, H, px, py, pvinv + 9 * s, v0, v1, v2);
+        onrm[3 * p] = n0; onrm[3 * p + 1] = n1; onrm[3 * p + 2] = n2;
+        oview[3 * p] = v0; oview[3 * p + 1] = v1; oview[3 * p + 2] = v2;
+        float pa[3] = {0.f, 0.f, 0.f}, amp = 0.f, sh = 0.f;
+        if ((int)r.w - 1 >= 0) spec_terms(n0, n1, n2, v0, v1, v2, rough[p], ior, pa, amp, sh, nullptr, nullptr);
+        oAxis[3 * p] = pa[0]; oAxis[3 * p + 1] = pa[1]; oAxis[3 * p + 2] = pa[2];
+        oSharp[p] = sh; oAmp[p] = amp;
+    }
+};
 __global__ void __launch_bounds__(BS) k_shade_prep(int W, int H, int depth, const float* rast, const float* rot, const float* normal, const u32* nidx,
                                                    const float* pvinv, const float* rough, float ior, float* onrm, float* oview, float* oAxis,
-                                                   float* oSharp, float* oAmp) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t P = (size_t)W * H * depth;
-    if (p >= P) return;
-    const int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
-    const float4 r = ((const float4*)rast)[p];
-    float n0, n1, n2, v0, v1, v2;
-    normal_axis_px(r, rot, normal, nidx, n0, n1, n2);
-    view_axis_px(W, H, px, py, pvinv + 9 * s, v0, v1, v2);
-    onrm[3 * p] = n0; onrm[3 * p + 1] = n1; onrm[3 * p + 2] = n2;
-    oview[3 * p] = v0; oview[3 * p + 1] = v1; oview[3 * p + 2] = v2;
-    float pa[3] = {0.f, 0.f, 0.f}, amp = 0.f, sh = 0.f;
-    if ((int)r.w - 1 >= 0) spec_terms(n0, n1, n2, v0, v1, v2, rough[p], ior, pa, amp, sh, nullptr, nullptr);
-    oAxis[3 * p] = pa[0]; oAxis[3 * p + 1] = pa[1]; oAxis[3 * p + 2] = pa[2];
-    oSharp[p] = sh; oAmp[p] = amp;
-}
+                                                   float* oSharp, float* oAmp) { KShadePrepF{}(vb_hw(), W, H, depth, rast, rot, normal, nidx, pvinv, rough, ior, onrm, oview, oAxis, oSharp, oAmp); }
 
+struct KSgSpecBwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, const float* rast, const float* normal, const float* view, const float* rough,
+                                                    float ior, const float* gAmp, const float* gSharp, float* grough) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        if ((int)rast[4 * p + 3] - 1 < 0) return;
+        float pa[3], amp, sh, dA, dS;
+        spec_terms(normal[3 * p], normal[3 * p + 1], normal[3 * p + 2], view[3 * p], view[3 * p + 1], view[3 * p + 2], rough[p], ior, pa, amp, sh, &dA,
+                   &dS);
+        float g = grough[p];
+        add_nan_check(g, gAmp[p] * dA);
+        add_nan_check(g, gSharp[p] * dS);
+        grough[p] = g;
+    }
+};
 __global__ void __launch_bounds__(BS) k_sg_spec_bwd(size_t P, const float* rast, const float* normal, const float* view, const float* rough,
-                                                    float ior, const float* gAmp, const float* gSharp, float* grough) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    if ((int)rast[4 * p + 3] - 1 < 0) return;
-    float pa[3], amp, sh, dA, dS;
-    spec_terms(normal[3 * p], normal[3 * p + 1], normal[3 * p + 2], view[3 * p], view[3 * p + 1], view[3 * p + 2], rough[p], ior, pa, amp, sh, &dA,
-               &dS);
-    float g = grough[p];
-    add_nan_check(g, gAmp[p] * dA);
-    add_nan_check(g, gSharp[p] * dS);
-    grough[p] = g;
-}
+                                                    float ior, const float* gAmp, const float* gSharp, float* grough) { KSgSpecBwdF{}(vb_hw(), P, rast, normal, view, rough, ior, gAmp, gSharp, grough); }
 
 // ---------------------------------------------------------------- SphericalGaussian
 constexpr float SG_DS = 2.133f, SG_DA = 2.34f;
@@ -1204,39 +1321,45 @@ __device__ __forceinline__ float sg_g(float l, float c, float amp) {
     return (float)((expf(l - c) - expf(-l - c)) / fmax((double)l, 1e-6) * amp);
 }
 
-__global__ void __launch_bounds__(BS) k_sg_fwd(size_t P, int C, int K, const float* rast, const float* __restrict__ axis, const float* __restrict__ sharp,
+struct KSgFwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, int C, int K, const float* rast, const float* __restrict__ axis, const float* __restrict__ sharp,
                                                const float* __restrict__ amp, const float* normal, const float* diffuse, const float* sAxis,
-                                               const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float de[SG_MAXC] = {0.f, 0.f, 0.f, 0.f}, se[SG_MAXC] = {0.f, 0.f, 0.f, 0.f};
-    bool cov = (int)rast[4 * p + 3] - 1 >= 0;
-    if (cov) {
-        float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
-        float ss = sSharp[p], sa = sAmp[p];
-        float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
-        for (int i = 0; i < K; i++) {
-            float lam = sharp[i];
-            float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
-            float wx = dnx + gx, wy = dny + gy, wz = dnz + gz;
-            float dl = sqrtf(wx * wx + wy * wy + wz * wz);
-            float dg = sg_g(dl, SG_DS + lam, SG_DA);
-            float qx = snx + gx, qy = sny + gy, qz = snz + gz;
-            float sl = sqrtf(qx * qx + qy * qy + qz * qz);
-            float sg = sg_g(sl, ss + lam, sa);
-            for (int k = 0; k < C; k++) {
-                float a = amp[i * C + k];
-                add_nan_check(de[k], dg * a);
-                add_nan_check(se[k], sg * a);
+                                               const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        float de[SG_MAXC] = {0.f, 0.f, 0.f, 0.f}, se[SG_MAXC] = {0.f, 0.f, 0.f, 0.f};
+        bool cov = (int)rast[4 * p + 3] - 1 >= 0;
+        if (cov) {
+            float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
+            float ss = sSharp[p], sa = sAmp[p];
+            float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
+            for (int i = 0; i < K; i++) {
+                float lam = sharp[i];
+                float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
+                float wx = dnx + gx, wy = dny + gy, wz = dnz + gz;
+                float dl = sqrtf(wx * wx + wy * wy + wz * wz);
+                float dg = sg_g(dl, SG_DS + lam, SG_DA);
+                float qx = snx + gx, qy = sny + gy, qz = snz + gz;
+                float sl = sqrtf(qx * qx + qy * qy + qz * qz);
+                float sg = sg_g(sl, ss + lam, sa);
+                for (int k = 0; k < C; k++) {
+                    float a = amp[i * C + k];
+                    add_nan_check(de[k], dg * a);
+                    add_nan_check(se[k], sg * a);
+                }
             }
         }
+        for (int k = 0; k < C; k++) {
+            out[p * C + k] = cov ? diffuse[p * C + k] * de[k] + se[k] : 0.f;
+            denv[p * C + k] = de[k];
+            senv[p * C + k] = se[k];
+        }
     }
-    for (int k = 0; k < C; k++) {
-        out[p * C + k] = cov ? diffuse[p * C + k] * de[k] + se[k] : 0.f;
-        denv[p * C + k] = de[k];
-        senv[p * C + k] = se[k];
-    }
-}
+};
+__global__ void __launch_bounds__(BS) k_sg_fwd(size_t P, int C, int K, const float* rast, const float* __restrict__ axis, const float* __restrict__ sharp,
+                                               const float* __restrict__ amp, const float* normal, const float* diffuse, const float* sAxis,
+                                               const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) { KSgFwdF{}(vb_hw(), P, C, K, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out, denv, senv); }
 
 // Backward: each block owns SG_PIX_PER_THREAD * 256 pixels per sweep and a fixed set of
 // sweeps; lobe grads are summed over its pixels (wave reductions, then LDS) and written as
@@ -1350,17 +1473,21 @@ __global__ void __launch_bounds__(BS) k_sg_bwd(size_t P, int C, int K, const flo
 // Final stage of the lobe-grad column sums: column i * SG_COLS + k (lobe i) is the sum of the
 // segment partials in segment order, added (AddNaNcheck) into the axis (k < 3), sharpness
 // (k == 3) or amplitude (k >= 4) grad of lobe i.
-__global__ void __launch_bounds__(256) k_sg_lobe_final(const float* seg, int nseg, int K, int C, float* gaxis, float* gsharp, float* gamp) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= K * SG_COLS) return;
-    float s = 0.f;
+struct KSgLobeFinalF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, const float* seg, int nseg, int K, int C, float* gaxis, float* gsharp, float* gamp) const {
+        const int c = vb.b.x * blockDim.x + threadIdx.x;
+        if (c >= K * SG_COLS) return;
+        float s = 0.f;
 #pragma unroll 8
-    for (int r = 0; r < nseg; r++) s += seg[(size_t)r * K * SG_COLS + c];
-    const int i = c / SG_COLS, k = c % SG_COLS;
-    if (k < 3) { if (gaxis) add_nan_check(gaxis[3 * i + k], s); }
-    else if (k == 3) { if (gsharp) add_nan_check(gsharp[i], s); }
-    else if (k - 4 < C && gamp) add_nan_check(gamp[i * C + (k - 4)], s);
-}
+        for (int r = 0; r < nseg; r++) s += seg[(size_t)r * K * SG_COLS + c];
+        const int i = c / SG_COLS, k = c % SG_COLS;
+        if (k < 3) { if (gaxis) add_nan_check(gaxis[3 * i + k], s); }
+        else if (k == 3) { if (gsharp) add_nan_check(gsharp[i], s); }
+        else if (k - 4 < C && gamp) add_nan_check(gamp[i * C + (k - 4)], s);
+    }
+};
+__global__ void __launch_bounds__(256) k_sg_lobe_final(const float* seg, int nseg, int K, int C, float* gaxis, float* gsharp, float* gamp) { KSgLobeFinalF{}(vb_hw(), seg, nseg, K, C, gaxis, gsharp, gamp); }
 
 // ---- fast SG kernels: native exp2 / sqrt / rcp (1-ulp hardware ops), lobes staged in LDS.
 constexpr float L2E = 1.4426950408889634f;
@@ -1394,45 +1521,51 @@ __device__ __forceinline__ float sg_term(float wx, float wy, float wz, float bas
     return (e1 - e2) * il;
 }
 
-__global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, const float* rast, const float* axis, const float* sharp,
+struct KSgFwdFastF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, int C, int K, const float* rast, const float* axis, const float* sharp,
                                                     const float* amp, const float* normal, const float* diffuse, const float* sAxis,
-                                                    const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) {
-    __shared__ SgLobe lobes[SG_MAXK];
-    sg_stage_lobes(lobes, K, C, axis, sharp, amp);
-    __syncthreads();
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    float de[4] = {0.f, 0.f, 0.f, 0.f}, se[4] = {0.f, 0.f, 0.f, 0.f};
-    bool cov = (int)rast[4 * p + 3] - 1 >= 0;
-    if (cov) {
-        float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
-        float ss = sSharp[p], sa = sAmp[p];
-        float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
-        for (int i = 0; i < K; i++) {
-            const SgLobe& L = lobes[i];
-            float l, il, e1, e2;
-            float dg = sg_term(dnx + L.gx, dny + L.gy, dnz + L.gz, SG_DS, L.lam, l, il, e1, e2) * SG_DA;
-            float sg = sg_term(snx + L.gx, sny + L.gy, snz + L.gz, ss, L.lam, l, il, e1, e2) * sa;
+                                                    const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) const {
+        __shared__ SgLobe lobes[SG_MAXK];
+        sg_stage_lobes(lobes, K, C, axis, sharp, amp);
+        __syncthreads();
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        if (p >= P) return;
+        float de[4] = {0.f, 0.f, 0.f, 0.f}, se[4] = {0.f, 0.f, 0.f, 0.f};
+        bool cov = (int)rast[4 * p + 3] - 1 >= 0;
+        if (cov) {
+            float dnx = normal[3 * p] * SG_DS, dny = normal[3 * p + 1] * SG_DS, dnz = normal[3 * p + 2] * SG_DS;
+            float ss = sSharp[p], sa = sAmp[p];
+            float snx = sAxis[3 * p] * ss, sny = sAxis[3 * p + 1] * ss, snz = sAxis[3 * p + 2] * ss;
+            for (int i = 0; i < K; i++) {
+                const SgLobe& L = lobes[i];
+                float l, il, e1, e2;
+                float dg = sg_term(dnx + L.gx, dny + L.gy, dnz + L.gz, SG_DS, L.lam, l, il, e1, e2) * SG_DA;
+                float sg = sg_term(snx + L.gx, sny + L.gy, snz + L.gz, ss, L.lam, l, il, e1, e2) * sa;
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (k < C) { acc_checked(de[k], dg * L.a[k]); acc_checked(se[k], sg * L.a[k]); }
+                for (int k = 0; k < 4; k++) {
+                    if (k < C) { acc_checked(de[k], dg * L.a[k]); acc_checked(se[k], sg * L.a[k]); }
+                }
+            }
+        }
+        if (C == 4) {
+            ((float4*)out)[p] = cov ? make_float4(diffuse[4 * p] * de[0] + se[0], diffuse[4 * p + 1] * de[1] + se[1], diffuse[4 * p + 2] * de[2] + se[2],
+                                                  diffuse[4 * p + 3] * de[3] + se[3])
+                                    : make_float4(0.f, 0.f, 0.f, 0.f);
+            ((float4*)denv)[p] = make_float4(de[0], de[1], de[2], de[3]);
+            ((float4*)senv)[p] = make_float4(se[0], se[1], se[2], se[3]);
+        } else {
+            for (int k = 0; k < C; k++) {
+                out[p * C + k] = cov ? diffuse[p * C + k] * de[k] + se[k] : 0.f;
+                denv[p * C + k] = de[k];
+                senv[p * C + k] = se[k];
             }
         }
     }
-    if (C == 4) {
-        ((float4*)out)[p] = cov ? make_float4(diffuse[4 * p] * de[0] + se[0], diffuse[4 * p + 1] * de[1] + se[1], diffuse[4 * p + 2] * de[2] + se[2],
-                                              diffuse[4 * p + 3] * de[3] + se[3])
-                                : make_float4(0.f, 0.f, 0.f, 0.f);
-        ((float4*)denv)[p] = make_float4(de[0], de[1], de[2], de[3]);
-        ((float4*)senv)[p] = make_float4(se[0], se[1], se[2], se[3]);
-    } else {
-        for (int k = 0; k < C; k++) {
-            out[p * C + k] = cov ? diffuse[p * C + k] * de[k] + se[k] : 0.f;
-            denv[p * C + k] = de[k];
-            senv[p * C + k] = se[k];
-        }
-    }
-}
+};
+__global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, const float* rast, const float* axis, const float* sharp,
+                                                    const float* amp, const float* normal, const float* diffuse, const float* sAxis,
+                                                    const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) { KSgFwdFastF{}(vb_hw(), P, C, K, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out, denv, senv); }
 
 // C == 4 forward.  A workgroup takes 1024 consecutive pixels and compacts the covered ones, so
 // its waves shade covered pixels only (a thread-per-pixel wave at a silhouette runs the whole
@@ -1530,72 +1663,79 @@ __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float
 // CU (all resident at once) the CUs holding interior rows finish last; halving the workgroup
 // lets the dispatcher balance them.
 template <int SG_FWD_PIX>
+struct KSgFwd4F {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
+                                                const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
+                                                const float* sAmp, float* out, float* denv, float* senv) const {
+        __shared__ SgLobe4 lobes[SG_MAXK];
+        __shared__ int nonfinite;
+        __shared__ u32 plist[SG_FWD_PIX];
+        __shared__ u32 wcount[BS / 64];
+        if (threadIdx.x == 0) nonfinite = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < K; i += blockDim.x) {
+            float lam = sharp[i];
+            float4 a = make_float4(amp[4 * i], amp[4 * i + 1], amp[4 * i + 2], amp[4 * i + 3]);
+            float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
+            lobes[i].g = make_float4(gx, gy, gz, lam);
+            lobes[i].a = a;
+            lobes[i].len = sqrtf(gx * gx + gy * gy + gz * gz);
+            lobes[i].cD = (SG_DS + lam) * L2E;
+            lobes[i].lamL2E = lam * L2E;
+            if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(a.w))) nonfinite = 1;
+        }
+        __syncthreads();
+        const bool checked = nonfinite != 0;
+        // uncovered pixels get their zeros here; the covered ones of the workgroup's SG_FWD_PIX are
+        // compacted (in pixel order) so that every wave shades covered pixels only
+        const size_t base = (size_t)vb.b.x * SG_FWD_PIX;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        u32 n = 0;
+        for (int k = 0; k < SG_FWD_PIX / BS; k++) {
+            const u32 local = (u32)(k * BS + threadIdx.x);
+            const size_t p = base + local;
+            const bool in = p < P;
+            const bool cov = in && (int)rast[4 * p + 3] - 1 >= 0;
+            if (in && !cov) { ((float4*)out)[p] = z4; ((float4*)denv)[p] = z4; ((float4*)senv)[p] = z4; }
+            block_append(cov, local, plist, wcount, n);
+        }
+        for (u32 r = 0; r < n; r += BS) {
+            const u32 i = r + threadIdx.x;
+            const bool act = i < n;
+            const size_t p = act ? base + plist[i] : 0;
+            float dnx = 0.f, dny = 0.f, dnz = 0.f, ss = 0.f, sa = 0.f, snx = 0.f, sny = 0.f, snz = 0.f;
+            float dlen = __builtin_nanf(""), slen = __builtin_nanf("");   // lanes without a pixel: ignored by fmin/fmax
+            if (act) {
+                dnx = normal[3 * p] * SG_DS; dny = normal[3 * p + 1] * SG_DS; dnz = normal[3 * p + 2] * SG_DS;
+                ss = sSharp[p]; sa = sAmp[p];
+                snx = sAxis[3 * p] * ss; sny = sAxis[3 * p + 1] * ss; snz = sAxis[3 * p + 2] * ss;
+                dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz);
+                slen = sqrtf(snx * snx + sny * sny + snz * snz);
+            }
+            const float dmin = wave_fmin(dlen), dmax = wave_fmax(dlen), smin = wave_fmin(slen), smax = wave_fmax(slen);
+            u64 dmask[SG_MAXK / 64], smask[SG_MAXK / 64];
+#pragma unroll
+            for (int c = 0; c < SG_MAXK / 64; c++) {
+                dmask[c] = sg_e2_mask(lobes, K, c * 64, dmin, dmax);
+                smask[c] = sg_e2_mask(lobes, K, c * 64, smin, smax);
+            }
+            if (act) {
+                float4 de = z4, se = z4;
+                if (checked) sg_fwd4_lobes<true>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
+                else sg_fwd4_lobes<false>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
+                const float4 df = ((const float4*)diffuse)[p];
+                ((float4*)out)[p] = make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w);
+                ((float4*)denv)[p] = de;
+                ((float4*)senv)[p] = se;
+            }
+        }
+    }
+};
+template <int SG_FWD_PIX>
 __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
                                                 const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
-                                                const float* sAmp, float* out, float* denv, float* senv) {
-    __shared__ SgLobe4 lobes[SG_MAXK];
-    __shared__ int nonfinite;
-    __shared__ u32 plist[SG_FWD_PIX];
-    __shared__ u32 wcount[BS / 64];
-    if (threadIdx.x == 0) nonfinite = 0;
-    __syncthreads();
-    for (int i = threadIdx.x; i < K; i += blockDim.x) {
-        float lam = sharp[i];
-        float4 a = make_float4(amp[4 * i], amp[4 * i + 1], amp[4 * i + 2], amp[4 * i + 3]);
-        float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
-        lobes[i].g = make_float4(gx, gy, gz, lam);
-        lobes[i].a = a;
-        lobes[i].len = sqrtf(gx * gx + gy * gy + gz * gz);
-        lobes[i].cD = (SG_DS + lam) * L2E;
-        lobes[i].lamL2E = lam * L2E;
-        if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(a.w))) nonfinite = 1;
-    }
-    __syncthreads();
-    const bool checked = nonfinite != 0;
-    // uncovered pixels get their zeros here; the covered ones of the workgroup's SG_FWD_PIX are
-    // compacted (in pixel order) so that every wave shades covered pixels only
-    const size_t base = (size_t)blockIdx.x * SG_FWD_PIX;
-    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    u32 n = 0;
-    for (int k = 0; k < SG_FWD_PIX / BS; k++) {
-        const u32 local = (u32)(k * BS + threadIdx.x);
-        const size_t p = base + local;
-        const bool in = p < P;
-        const bool cov = in && (int)rast[4 * p + 3] - 1 >= 0;
-        if (in && !cov) { ((float4*)out)[p] = z4; ((float4*)denv)[p] = z4; ((float4*)senv)[p] = z4; }
-        block_append(cov, local, plist, wcount, n);
-    }
-    for (u32 r = 0; r < n; r += BS) {
-        const u32 i = r + threadIdx.x;
-        const bool act = i < n;
-        const size_t p = act ? base + plist[i] : 0;
-        float dnx = 0.f, dny = 0.f, dnz = 0.f, ss = 0.f, sa = 0.f, snx = 0.f, sny = 0.f, snz = 0.f;
-        float dlen = __builtin_nanf(""), slen = __builtin_nanf("");   // lanes without a pixel: ignored by fmin/fmax
-        if (act) {
-            dnx = normal[3 * p] * SG_DS; dny = normal[3 * p + 1] * SG_DS; dnz = normal[3 * p + 2] * SG_DS;
-            ss = sSharp[p]; sa = sAmp[p];
-            snx = sAxis[3 * p] * ss; sny = sAxis[3 * p + 1] * ss; snz = sAxis[3 * p + 2] * ss;
-            dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz);
-            slen = sqrtf(snx * snx + sny * sny + snz * snz);
-        }
-        const float dmin = wave_fmin(dlen), dmax = wave_fmax(dlen), smin = wave_fmin(slen), smax = wave_fmax(slen);
-        u64 dmask[SG_MAXK / 64], smask[SG_MAXK / 64];
-#pragma unroll
-        for (int c = 0; c < SG_MAXK / 64; c++) {
-            dmask[c] = sg_e2_mask(lobes, K, c * 64, dmin, dmax);
-            smask[c] = sg_e2_mask(lobes, K, c * 64, smin, smax);
-        }
-        if (act) {
-            float4 de = z4, se = z4;
-            if (checked) sg_fwd4_lobes<true>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
-            else sg_fwd4_lobes<false>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
-            const float4 df = ((const float4*)diffuse)[p];
-            ((float4*)out)[p] = make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w);
-            ((float4*)denv)[p] = de;
-            ((float4*)senv)[p] = se;
-        }
-    }
-}
+                                                const float* sAmp, float* out, float* denv, float* senv) { KSgFwd4F<SG_FWD_PIX>{}(vb_hw(), P, K, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out, denv, senv); }
 
 // Lobe-parallel backward for the lobe gradients (axis, sharpness, amplitude).  Thread t owns
 // lobe t % KP for pixel sub-group t / KP; covered pixels of a 256-pixel chunk are compacted into
@@ -1681,157 +1821,164 @@ __device__ __forceinline__ bool sg_needs_e2(float w, float lo, float hi) {
     return !(w + T < lo) && !(w - T > hi);
 }
 
+struct KSgBwdLobesF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t P, int C, int K, int KP, const float* rast, const float* axis, const float* sharp,
+                                                     const float* amp, const float* normal, const float* diffuse, const float* sAxis,
+                                                     const float* sSharp, const float* sAmp, const float* gout, const float* denv, float* gdiffuse,
+                                                     float* part, int want_lobes) const {
+        __shared__ SgLobe lobes[SG_MAXK];
+        __shared__ SgPix pix[BS];
+        __shared__ int wcount[BS / 64];
+        __shared__ u32 wneed[BS / 64];
+        __shared__ float red[BS][SG_COLS];
+        __shared__ unsigned long long skey[SG_MAXK];
+        __shared__ int lorig[SG_MAXK];
+        __shared__ float2 lrange[BS / 64];
+        int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+        // Lane layout: with KP >= 4, wave w holds the KP / 4 lobes [w KP / 4, (w + 1) KP / 4) and
+        // 256 / KP pixel groups (lane = group * KP / 4 + lobe offset), so a wave's lobes are a quarter
+        // of the staged order; with KP < 4 every wave holds all lobes.  Either way a lobe's pixels
+        // go to groups 0 .. G - 1 (G = BS / KP) in the same order.
+        const int LPW = KP >= 4 ? KP / 4 : KP;
+        int li = KP >= 4 ? wid * LPW + lane % LPW : tid % KP, grp = KP >= 4 ? lane / LPW : tid / KP, G = BS / KP;
+        // Lobes are staged in order of |axis * sharpness| (ties by index), so that a wave's 64 lobe
+        // lanes hold similar lengths and the e2 test below can pass for a whole wave; the per-lobe
+        // sums do not depend on which lane holds a lobe.
+        for (int i = tid; i < K; i += BS) {
+            const float lam = sharp[i];
+            const float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
+            const float len = sqrtf(gx * gx + gy * gy + gz * gz);
+            const u32 kb = isnan(len) ? 0xffffffffu : __float_as_uint(len);
+            skey[i] = ((unsigned long long)kb << 32) | (u32)i;
+        }
+        __syncthreads();
+        for (int i = tid; i < K; i += BS) {
+            const unsigned long long ki = skey[i];
+            int r = 0;
+            for (int j = 0; j < K; j++) r += skey[j] < ki;
+            const float lam = sharp[i];
+            SgLobe l;
+            l.mx = axis[3 * i]; l.my = axis[3 * i + 1]; l.mz = axis[3 * i + 2];
+            l.gx = l.mx * lam; l.gy = l.my * lam; l.gz = l.mz * lam; l.lam = lam; l.lamL2E = lam * L2E;
+            for (int k = 0; k < 4; k++) l.a[k] = k < C ? amp[i * C + k] : 0.f;
+            lobes[r] = l;
+            lorig[r] = i;
+        }
+        __syncthreads();
+        // lobe-length range of each wave's lobes (one range, lw = 0, when KP < 4)
+        const int nlw = KP >= 4 ? BS / 64 : 1, lw = KP >= 4 ? wid : 0;
+        {
+            float len = __builtin_nanf("");
+            if (li < K) { const SgLobe& L = lobes[li]; len = sqrtf(L.gx * L.gx + L.gy * L.gy + L.gz * L.gz); }
+            const float lo = wave_fmin(len), hi = wave_fmax(len);
+            if (lane == 0 && wid < nlw) lrange[lw] = make_float2(lo, hi);
+        }
+        __syncthreads();
+        float acc[SG_COLS];
+#pragma unroll
+        for (int k = 0; k < SG_COLS; k++) acc[k] = 0.f;
+        float accW[3] = {0.f, 0.f, 0.f}, accS = 0.f, accX[4] = {0.f, 0.f, 0.f, 0.f}, accG[4] = {0.f, 0.f, 0.f, 0.f};
+        // The pixel inputs of the next chunk are loaded while this chunk's lobe loop runs (and the
+        // coverage word of the chunk after that), so the chunks' loads overlap the lobe work instead
+        // of every workgroup waiting on them between chunks.
+        const size_t stride = (size_t)vb.g.x * BS;
+        SgRaw cur, nxt;
+        float rwn;
+        {
+            const size_t p0 = (size_t)vb.b.x * BS + tid, p1 = p0 + stride;
+            sg_load_raw(cur, p0, p0 < P && (int)rast[4 * p0 + 3] - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp);
+            rwn = p1 < P ? rast[4 * p1 + 3] : 0.f;
+        }
+        for (size_t base = (size_t)vb.b.x * BS; base < P; base += stride) {
+            const size_t p = base + tid;
+            const bool cov = cur.cov;
+            {
+                const size_t pn = p + stride, p2 = pn + stride;
+                sg_load_raw(nxt, pn, pn < P && (int)rwn - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp);
+                rwn = p2 < P ? rast[4 * p2 + 3] : 0.f;
+            }
+            SgPix q;
+            u32 need = 0;   // bit 2 lw: diffuse e2 for lobe wave lw, bit 2 lw + 1: specular
+            if (cov) {
+                if (gdiffuse)
+#pragma unroll
+                    for (int k = 0; k < 4; k++)
+                        if (k < C) {
+                            float t = cur.gd[k];
+                            add_nan_check(t, cur.go[k] * cur.de[k]);
+                            gdiffuse[p * C + k] = t;
+                        }
+                q.dnx = cur.nx * SG_DS; q.dny = cur.ny * SG_DS; q.dnz = cur.nz * SG_DS;
+                const float ss = cur.ss;
+                q.ssL2E = ss * L2E; q.sa = cur.sa;
+                q.snx = cur.ax * ss; q.sny = cur.ay * ss; q.snz = cur.az * ss;
+#pragma unroll
+                for (int k = 0; k < 4; k++) { q.X[k] = cur.go[k] * cur.df[k]; q.G[k] = cur.go[k]; }
+                const float dlen = sqrtf(q.dnx * q.dnx + q.dny * q.dny + q.dnz * q.dnz);
+                const float slen = sqrtf(q.snx * q.snx + q.sny * q.sny + q.snz * q.snz);
+                for (int w = 0; w < nlw; w++) {
+                    const float2 r = lrange[w];
+                    need |= (sg_needs_e2(dlen, r.x, r.y) ? 1u : 0u) << (2 * w);
+                    need |= (sg_needs_e2(slen, r.x, r.y) ? 2u : 0u) << (2 * w);
+                }
+            }
+            if (!want_lobes) { cur = nxt; continue; }
+            // deterministic compaction: wave ballot + prefix over waves
+            unsigned long long m = __ballot(cov);
+            int wpos = __popcll(m & ((1ull << lane) - 1ull));
+            u32 wn = 0;
+            for (int b = 0; b < 2 * nlw; b++) wn |= __ballot((need >> b) & 1u) ? 1u << b : 0u;
+            if (lane == 0) { wcount[wid] = __popcll(m); wneed[wid] = wn; }
+            __syncthreads();
+            int off = 0, ncov = 0;
+            u32 cneed = 0;
+            for (int w = 0; w < BS / 64; w++) { if (w < wid) off += wcount[w]; ncov += wcount[w]; cneed |= wneed[w]; }
+            if (cov) pix[off + wpos] = q;
+            __syncthreads();
+            if (li < K) {
+                // The thread's lobe (axis mu, sharpness lam) is fixed, so the lobe-constant factors
+                // come out of the pixel sum: with cd, cs the diffuse / specular chain factors,
+                //   d/d(axis)      = lam * sum(cd w + cs q)
+                //   d/d(sharpness) = mu . sum(cd w + cs q) - sum(A dg + B sg)
+                //   d/d(amplitude) = DA * sum(X dg) + sum(G sg sa)
+                // (mu . w enters the sharpness term only through cd mu . w), and dg/dl =
+                // il ((e1 + e2) - g) for g = (e1 - e2) il.  The e2 choice is uniform per wave.
+                const SgLobe& L = lobes[li];
+                const bool db = (cneed >> (2 * lw)) & 1u, sb = (cneed >> (2 * lw + 1)) & 1u;
+                if (db && sb) sg_bwd_lobe_loop<true, true>(L, pix, grp, ncov, G, accW, accS, accX, accG);
+                else if (db) sg_bwd_lobe_loop<true, false>(L, pix, grp, ncov, G, accW, accS, accX, accG);
+                else if (sb) sg_bwd_lobe_loop<false, true>(L, pix, grp, ncov, G, accW, accS, accX, accG);
+                else sg_bwd_lobe_loop<false, false>(L, pix, grp, ncov, G, accW, accS, accX, accG);
+            }
+            __syncthreads();
+            cur = nxt;
+        }
+        if (want_lobes && li < K) {
+            const SgLobe& L = lobes[li];
+            acc[0] = accW[0] * L.lam;
+            acc[1] = accW[1] * L.lam;
+            acc[2] = accW[2] * L.lam;
+            acc[3] = (L.mx * accW[0] + L.my * accW[1] + L.mz * accW[2]) + accS;
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc[4 + k] = SG_DA * accX[k] + accG[k];
+        }
+        if (!want_lobes) return;
+#pragma unroll
+        for (int k = 0; k < SG_COLS; k++) red[tid][k] = acc[k];
+        __syncthreads();
+        for (int i = tid; i < K * SG_COLS; i += BS) {
+            int s = i / SG_COLS, col = i % SG_COLS;   // s: the lobe's staged (sorted) position
+            float sum = 0.f;
+            for (int g2 = 0; g2 < G; g2++) sum += red[KP >= 4 ? (s / LPW) * 64 + g2 * LPW + s % LPW : g2 * KP + s][col];
+            part[(size_t)vb.b.x * K * SG_COLS + (size_t)lorig[s] * SG_COLS + col] = col < 4 + C ? sum : 0.f;
+        }
+    }
+};
 __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int KP, const float* rast, const float* axis, const float* sharp,
                                                      const float* amp, const float* normal, const float* diffuse, const float* sAxis,
                                                      const float* sSharp, const float* sAmp, const float* gout, const float* denv, float* gdiffuse,
-                                                     float* part, int want_lobes) {
-    __shared__ SgLobe lobes[SG_MAXK];
-    __shared__ SgPix pix[BS];
-    __shared__ int wcount[BS / 64];
-    __shared__ u32 wneed[BS / 64];
-    __shared__ float red[BS][SG_COLS];
-    __shared__ unsigned long long skey[SG_MAXK];
-    __shared__ int lorig[SG_MAXK];
-    __shared__ float2 lrange[BS / 64];
-    int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    // Lane layout: with KP >= 4, wave w holds the KP / 4 lobes [w KP / 4, (w + 1) KP / 4) and
-    // 256 / KP pixel groups (lane = group * KP / 4 + lobe offset), so a wave's lobes are a quarter
-    // of the staged order; with KP < 4 every wave holds all lobes.  Either way a lobe's pixels
-    // go to groups 0 .. G - 1 (G = BS / KP) in the same order.
-    const int LPW = KP >= 4 ? KP / 4 : KP;
-    int li = KP >= 4 ? wid * LPW + lane % LPW : tid % KP, grp = KP >= 4 ? lane / LPW : tid / KP, G = BS / KP;
-    // Lobes are staged in order of |axis * sharpness| (ties by index), so that a wave's 64 lobe
-    // lanes hold similar lengths and the e2 test below can pass for a whole wave; the per-lobe
-    // sums do not depend on which lane holds a lobe.
-    for (int i = tid; i < K; i += BS) {
-        const float lam = sharp[i];
-        const float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
-        const float len = sqrtf(gx * gx + gy * gy + gz * gz);
-        const u32 kb = isnan(len) ? 0xffffffffu : __float_as_uint(len);
-        skey[i] = ((unsigned long long)kb << 32) | (u32)i;
-    }
-    __syncthreads();
-    for (int i = tid; i < K; i += BS) {
-        const unsigned long long ki = skey[i];
-        int r = 0;
-        for (int j = 0; j < K; j++) r += skey[j] < ki;
-        const float lam = sharp[i];
-        SgLobe l;
-        l.mx = axis[3 * i]; l.my = axis[3 * i + 1]; l.mz = axis[3 * i + 2];
-        l.gx = l.mx * lam; l.gy = l.my * lam; l.gz = l.mz * lam; l.lam = lam; l.lamL2E = lam * L2E;
-        for (int k = 0; k < 4; k++) l.a[k] = k < C ? amp[i * C + k] : 0.f;
-        lobes[r] = l;
-        lorig[r] = i;
-    }
-    __syncthreads();
-    // lobe-length range of each wave's lobes (one range, lw = 0, when KP < 4)
-    const int nlw = KP >= 4 ? BS / 64 : 1, lw = KP >= 4 ? wid : 0;
-    {
-        float len = __builtin_nanf("");
-        if (li < K) { const SgLobe& L = lobes[li]; len = sqrtf(L.gx * L.gx + L.gy * L.gy + L.gz * L.gz); }
-        const float lo = wave_fmin(len), hi = wave_fmax(len);
-        if (lane == 0 && wid < nlw) lrange[lw] = make_float2(lo, hi);
-    }
-    __syncthreads();
-    float acc[SG_COLS];
-#pragma unroll
-    for (int k = 0; k < SG_COLS; k++) acc[k] = 0.f;
-    float accW[3] = {0.f, 0.f, 0.f}, accS = 0.f, accX[4] = {0.f, 0.f, 0.f, 0.f}, accG[4] = {0.f, 0.f, 0.f, 0.f};
-    // The pixel inputs of the next chunk are loaded while this chunk's lobe loop runs (and the
-    // coverage word of the chunk after that), so the chunks' loads overlap the lobe work instead
-    // of every workgroup waiting on them between chunks.
-    const size_t stride = (size_t)gridDim.x * BS;
-    SgRaw cur, nxt;
-    float rwn;
-    {
-        const size_t p0 = (size_t)blockIdx.x * BS + tid, p1 = p0 + stride;
-        sg_load_raw(cur, p0, p0 < P && (int)rast[4 * p0 + 3] - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp);
-        rwn = p1 < P ? rast[4 * p1 + 3] : 0.f;
-    }
-    for (size_t base = (size_t)blockIdx.x * BS; base < P; base += stride) {
-        const size_t p = base + tid;
-        const bool cov = cur.cov;
-        {
-            const size_t pn = p + stride, p2 = pn + stride;
-            sg_load_raw(nxt, pn, pn < P && (int)rwn - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp);
-            rwn = p2 < P ? rast[4 * p2 + 3] : 0.f;
-        }
-        SgPix q;
-        u32 need = 0;   // bit 2 lw: diffuse e2 for lobe wave lw, bit 2 lw + 1: specular
-        if (cov) {
-            if (gdiffuse)
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (k < C) {
-                        float t = cur.gd[k];
-                        add_nan_check(t, cur.go[k] * cur.de[k]);
-                        gdiffuse[p * C + k] = t;
-                    }
-            q.dnx = cur.nx * SG_DS; q.dny = cur.ny * SG_DS; q.dnz = cur.nz * SG_DS;
-            const float ss = cur.ss;
-            q.ssL2E = ss * L2E; q.sa = cur.sa;
-            q.snx = cur.ax * ss; q.sny = cur.ay * ss; q.snz = cur.az * ss;
-#pragma unroll
-            for (int k = 0; k < 4; k++) { q.X[k] = cur.go[k] * cur.df[k]; q.G[k] = cur.go[k]; }
-            const float dlen = sqrtf(q.dnx * q.dnx + q.dny * q.dny + q.dnz * q.dnz);
-            const float slen = sqrtf(q.snx * q.snx + q.sny * q.sny + q.snz * q.snz);
-            for (int w = 0; w < nlw; w++) {
-                const float2 r = lrange[w];
-                need |= (sg_needs_e2(dlen, r.x, r.y) ? 1u : 0u) << (2 * w);
-                need |= (sg_needs_e2(slen, r.x, r.y) ? 2u : 0u) << (2 * w);
-            }
-        }
-        if (!want_lobes) { cur = nxt; continue; }
-        // deterministic compaction: wave ballot + prefix over waves
-        unsigned long long m = __ballot(cov);
-        int wpos = __popcll(m & ((1ull << lane) - 1ull));
-        u32 wn = 0;
-        for (int b = 0; b < 2 * nlw; b++) wn |= __ballot((need >> b) & 1u) ? 1u << b : 0u;
-        if (lane == 0) { wcount[wid] = __popcll(m); wneed[wid] = wn; }
-        __syncthreads();
-        int off = 0, ncov = 0;
-        u32 cneed = 0;
-        for (int w = 0; w < BS / 64; w++) { if (w < wid) off += wcount[w]; ncov += wcount[w]; cneed |= wneed[w]; }
-        if (cov) pix[off + wpos] = q;
-        __syncthreads();
-        if (li < K) {
-            // The thread's lobe (axis mu, sharpness lam) is fixed, so the lobe-constant factors
-            // come out of the pixel sum: with cd, cs the diffuse / specular chain factors,
-            //   d/d(axis)      = lam * sum(cd w + cs q)
-            //   d/d(sharpness) = mu . sum(cd w + cs q) - sum(A dg + B sg)
-            //   d/d(amplitude) = DA * sum(X dg) + sum(G sg sa)
-            // (mu . w enters the sharpness term only through cd mu . w), and dg/dl =
-            // il ((e1 + e2) - g) for g = (e1 - e2) il.  The e2 choice is uniform per wave.
-            const SgLobe& L = lobes[li];
-            const bool db = (cneed >> (2 * lw)) & 1u, sb = (cneed >> (2 * lw + 1)) & 1u;
-            if (db && sb) sg_bwd_lobe_loop<true, true>(L, pix, grp, ncov, G, accW, accS, accX, accG);
-            else if (db) sg_bwd_lobe_loop<true, false>(L, pix, grp, ncov, G, accW, accS, accX, accG);
-            else if (sb) sg_bwd_lobe_loop<false, true>(L, pix, grp, ncov, G, accW, accS, accX, accG);
-            else sg_bwd_lobe_loop<false, false>(L, pix, grp, ncov, G, accW, accS, accX, accG);
-        }
-        __syncthreads();
-        cur = nxt;
-    }
-    if (want_lobes && li < K) {
-        const SgLobe& L = lobes[li];
-        acc[0] = accW[0] * L.lam;
-        acc[1] = accW[1] * L.lam;
-        acc[2] = accW[2] * L.lam;
-        acc[3] = (L.mx * accW[0] + L.my * accW[1] + L.mz * accW[2]) + accS;
-#pragma unroll
-        for (int k = 0; k < 4; k++) acc[4 + k] = SG_DA * accX[k] + accG[k];
-    }
-    if (!want_lobes) return;
-#pragma unroll
-    for (int k = 0; k < SG_COLS; k++) red[tid][k] = acc[k];
-    __syncthreads();
-    for (int i = tid; i < K * SG_COLS; i += BS) {
-        int s = i / SG_COLS, col = i % SG_COLS;   // s: the lobe's staged (sorted) position
-        float sum = 0.f;
-        for (int g2 = 0; g2 < G; g2++) sum += red[KP >= 4 ? (s / LPW) * 64 + g2 * LPW + s % LPW : g2 * KP + s][col];
-        part[(size_t)blockIdx.x * K * SG_COLS + (size_t)lorig[s] * SG_COLS + col] = col < 4 + C ? sum : 0.f;
-    }
-}
+                                                     float* part, int want_lobes) { KSgBwdLobesF{}(vb_hw(), P, C, K, KP, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, gout, denv, gdiffuse, part, want_lobes); }
 
 // ---------------------------------------------------------------- antialias
 struct AAEdge { float alpha, ia, pax, pay, pbx, pby, sg; int va, vb, horiz, target_o, valid; };
@@ -1947,64 +2094,68 @@ __device__ __forceinline__ int aa_pair_slot(int tx, int ty, int k) {
     }
 }
 
-__global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float* in, float* out) {
+struct KAaFwdF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, AAArgs a, int tilesX, const float* in, float* out) const {
 #pragma clang fp contract(off)
-    __shared__ float4 to_cur[AA_PAIRS], to_nb[AA_PAIRS];
-    const int s = blockIdx.y;
-    const int x0 = (blockIdx.x % tilesX) * AA_T, y0 = (blockIdx.x / tilesX) * AA_T;
-    const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
-    const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
-    const float* inv = in + (size_t)s * a.W * a.H * a.C;
-    const int C = a.C;
-    for (int p = threadIdx.x; p < AA_PAIRS; p += BS) {
-        int cx, cy, vertical;
-        aa_tile_pair(p, x0, y0, cx, cy, vertical);
-        float tc[AA_MAXC] = {0.f, 0.f, 0.f, 0.f}, tn[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
-        int pp = 0, op = 0;
-        float4 front;
-        AAEdge E[3];
-        if (cx < a.W && cy < a.H && aa_pair_pixels(a, rastv, cx, cy, vertical, pp, op, front)) {
-            // the two pixels' colours are loaded before the triangle's indices and vertices, so
-            // the loads overlap that dependent chain
-            float ip[AA_MAXC], io[AA_MAXC];
-            aa_load_px(inv, (size_t)pp, C, a.v4, ip);
-            aa_load_px(inv, (size_t)op, C, a.v4, io);
-            aa_pair_edges(a, clipv, front, cx, cy, vertical, pp, op, E);
-            int cur = cy * a.W + cx;
+        __shared__ float4 to_cur[AA_PAIRS], to_nb[AA_PAIRS];
+        const int s = vb.b.y;
+        const int x0 = (vb.b.x % tilesX) * AA_T, y0 = (vb.b.x / tilesX) * AA_T;
+        const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
+        const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
+        const float* inv = in + (size_t)s * a.W * a.H * a.C;
+        const int C = a.C;
+        for (int p = threadIdx.x; p < AA_PAIRS; p += BS) {
+            int cx, cy, vertical;
+            aa_tile_pair(p, x0, y0, cx, cy, vertical);
+            float tc[AA_MAXC] = {0.f, 0.f, 0.f, 0.f}, tn[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
+            int pp = 0, op = 0;
+            float4 front;
+            AAEdge E[3];
+            if (cx < a.W && cy < a.H && aa_pair_pixels(a, rastv, cx, cy, vertical, pp, op, front)) {
+                // the two pixels' colours are loaded before the triangle's indices and vertices, so
+                // the loads overlap that dependent chain
+                float ip[AA_MAXC], io[AA_MAXC];
+                aa_load_px(inv, (size_t)pp, C, a.v4, ip);
+                aa_load_px(inv, (size_t)op, C, a.v4, io);
+                aa_pair_edges(a, clipv, front, cx, cy, vertical, pp, op, E);
+                int cur = cy * a.W + cx;
 #pragma unroll
-            for (int e = 0; e < 3; e++) {
-                if (!E[e].valid) continue;
-                bool to_c = (E[e].target_o ? op : pp) == cur;
+                for (int e = 0; e < 3; e++) {
+                    if (!E[e].valid) continue;
+                    bool to_c = (E[e].target_o ? op : pp) == cur;
 #pragma unroll
-                for (int c = 0; c < AA_MAXC; c++) {
-                    if (c >= C) continue;
-                    float v = E[e].alpha * (ip[c] - io[c]);
-                    if (to_c) tc[c] += v; else tn[c] += v;
+                    for (int c = 0; c < AA_MAXC; c++) {
+                        if (c >= C) continue;
+                        float v = E[e].alpha * (ip[c] - io[c]);
+                        if (to_c) tc[c] += v; else tn[c] += v;
+                    }
                 }
             }
+            to_cur[p] = make_float4(tc[0], tc[1], tc[2], tc[3]);
+            to_nb[p] = make_float4(tn[0], tn[1], tn[2], tn[3]);
         }
-        to_cur[p] = make_float4(tc[0], tc[1], tc[2], tc[3]);
-        to_nb[p] = make_float4(tn[0], tn[1], tn[2], tn[3]);
+        __syncthreads();
+        const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
+        const int px = x0 + tx, py = y0 + ty;
+        if (px >= a.W || py >= a.H) return;
+        size_t q = (size_t)py * a.W + px;
+        float acc[AA_MAXC];
+        aa_load_px(inv, q, C, a.v4, acc);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float4 v = k < 2 ? to_cur[aa_pair_slot(tx, ty, k)] : to_nb[aa_pair_slot(tx, ty, k)];
+            float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int c = 0; c < AA_MAXC; c++) acc[c] += vv[c];
+        }
+        float* o = out + ((size_t)s * a.W * a.H + q) * C;
+#pragma unroll
+        for (int c = 0; c < AA_MAXC; c++)
+            if (c < C) o[c] = acc[c];
     }
-    __syncthreads();
-    const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
-    const int px = x0 + tx, py = y0 + ty;
-    if (px >= a.W || py >= a.H) return;
-    size_t q = (size_t)py * a.W + px;
-    float acc[AA_MAXC];
-    aa_load_px(inv, q, C, a.v4, acc);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        float4 v = k < 2 ? to_cur[aa_pair_slot(tx, ty, k)] : to_nb[aa_pair_slot(tx, ty, k)];
-        float vv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int c = 0; c < AA_MAXC; c++) acc[c] += vv[c];
-    }
-    float* o = out + ((size_t)s * a.W * a.H + q) * C;
-#pragma unroll
-    for (int c = 0; c < AA_MAXC; c++)
-        if (c < C) o[c] = acc[c];
-}
+};
+__global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float* in, float* out) { KAaFwdF{}(vb_hw(), a, tilesX, in, out); }
 
 // Backward of the tiled antialias.  Per pair, D = sum over its edges of alpha * gout[target]
 // goes to the front pixel pp with + and to the other pixel op with -; the vertex (clip-space)
@@ -2027,137 +2178,149 @@ __device__ __forceinline__ void aa_vertex_grad(LdsTable<AA_VSLOTS, 3>& vt, float
     }
 }
 
-__global__ void __launch_bounds__(BS, 7) k_aa_bwd(AAArgs a, int tilesX, const float* in, const float* gout, float* gin, float* gproj) {
+struct KAaBwdF {
+    static constexpr int kBS = BS, kMINW = 7;
+    __device__ __forceinline__ void operator()(const VB& vb, AAArgs a, int tilesX, const float* in, const float* gout, float* gin, float* gproj) const {
 #pragma clang fp contract(off)
-    __shared__ float4 to_cur[AA_PAIRS];
-    __shared__ LdsTable<AA_VSLOTS, 3> vt;
-    __shared__ u32 vused[AA_VSLOTS], wave_count[BS / 64];
-    const int s = blockIdx.y;
-    const int x0 = (blockIdx.x % tilesX) * AA_T, y0 = (blockIdx.x / tilesX) * AA_T;
-    const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
-    const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
-    size_t off = (size_t)s * a.W * a.H * a.C;
-    const float* inv = in + off;
-    const float* gov = gout + off;
-    float* gcs = gproj ? gproj + (size_t)s * a.V * 4 : nullptr;
-    const int C = a.C;
-    if (gcs) {
-        lds_table_clear(vt);
-        __syncthreads();
-    }
-    for (int p = threadIdx.x; p < AA_PAIRS; p += BS) {
-        int cx, cy, vertical;
-        aa_tile_pair(p, x0, y0, cx, cy, vertical);
-        float D[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
-        bool pp_cur = true;
-        int pp = 0, op = 0;
-        float4 front;
-        AAEdge E[3];
-        if (cx < a.W && cy < a.H && aa_pair_pixels(a, rastv, cx, cy, vertical, pp, op, front)) {
-            pp_cur = pp == cy * a.W + cx;
-            // both pixels' output grads (and colours, for the vertex grads) are loaded before the
-            // triangle's indices and vertices, so the loads overlap that dependent chain
-            const bool vgrad = gcs && p < AA_OWN;
-            float gp_[AA_MAXC], go_[AA_MAXC], dc[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
-            aa_load_px(gov, (size_t)pp, C, a.v4, gp_);
-            aa_load_px(gov, (size_t)op, C, a.v4, go_);
-            if (vgrad) {
-                float ip[AA_MAXC], io[AA_MAXC];
-                aa_load_px(inv, (size_t)pp, C, a.v4, ip);
-                aa_load_px(inv, (size_t)op, C, a.v4, io);
-#pragma unroll
-                for (int c = 0; c < AA_MAXC; c++) dc[c] = ip[c] - io[c];
-            }
-            aa_pair_edges(a, clipv, front, cx, cy, vertical, pp, op, E);
-#pragma unroll
-            for (int e = 0; e < 3; e++) {
-                if (!E[e].valid) continue;
-                const bool to_o = E[e].target_o;
-                float al = E[e].alpha;
-#pragma unroll
-                for (int c = 0; c < AA_MAXC; c++)
-                    if (c < C) D[c] += al * (to_o ? go_[c] : gp_[c]);
+        __shared__ float4 to_cur[AA_PAIRS];
+        __shared__ LdsTable<AA_VSLOTS, 3> vt;
+        __shared__ u32 vused[AA_VSLOTS], wave_count[BS / 64];
+        const int s = vb.b.y;
+        const int x0 = (vb.b.x % tilesX) * AA_T, y0 = (vb.b.x / tilesX) * AA_T;
+        const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
+        const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
+        size_t off = (size_t)s * a.W * a.H * a.C;
+        const float* inv = in + off;
+        const float* gov = gout + off;
+        float* gcs = gproj ? gproj + (size_t)s * a.V * 4 : nullptr;
+        const int C = a.C;
+        if (gcs) {
+            lds_table_clear(vt);
+            __syncthreads();
+        }
+        for (int p = threadIdx.x; p < AA_PAIRS; p += BS) {
+            int cx, cy, vertical;
+            aa_tile_pair(p, x0, y0, cx, cy, vertical);
+            float D[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
+            bool pp_cur = true;
+            int pp = 0, op = 0;
+            float4 front;
+            AAEdge E[3];
+            if (cx < a.W && cy < a.H && aa_pair_pixels(a, rastv, cx, cy, vertical, pp, op, front)) {
+                pp_cur = pp == cy * a.W + cx;
+                // both pixels' output grads (and colours, for the vertex grads) are loaded before the
+                // triangle's indices and vertices, so the loads overlap that dependent chain
+                const bool vgrad = gcs && p < AA_OWN;
+                float gp_[AA_MAXC], go_[AA_MAXC], dc[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
+                aa_load_px(gov, (size_t)pp, C, a.v4, gp_);
+                aa_load_px(gov, (size_t)op, C, a.v4, go_);
                 if (vgrad) {
-                    float d = 0.f;
+                    float ip[AA_MAXC], io[AA_MAXC];
+                    aa_load_px(inv, (size_t)pp, C, a.v4, ip);
+                    aa_load_px(inv, (size_t)op, C, a.v4, io);
+#pragma unroll
+                    for (int c = 0; c < AA_MAXC; c++) dc[c] = ip[c] - io[c];
+                }
+                aa_pair_edges(a, clipv, front, cx, cy, vertical, pp, op, E);
+#pragma unroll
+                for (int e = 0; e < 3; e++) {
+                    if (!E[e].valid) continue;
+                    const bool to_o = E[e].target_o;
+                    float al = E[e].alpha;
 #pragma unroll
                     for (int c = 0; c < AA_MAXC; c++)
-                        if (c < C) d += (to_o ? go_[c] : gp_[c]) * dc[c];
-                    const AAEdge& ed = E[e];
-                    float ia = ed.ia, av = ed.pax * ed.pby - ed.pay * ed.pbx;
-                    float dDax, dDay, dDbx, dDby;
-                    if (ed.horiz) { dDax = 0.f; dDay = -ed.sg; dDbx = 0.f; dDby = ed.sg; }
-                    else { dDax = ed.sg; dDay = 0.f; dDbx = -ed.sg; dDby = 0.f; }
-                    float k2 = av * ia * ia;
-                    float gax = d * (ia * ed.pby - k2 * dDax), gay = d * (-ia * ed.pbx - k2 * dDay);
-                    float gbx = d * (-ia * ed.pay - k2 * dDbx), gby = d * (ia * ed.pax - k2 * dDby);
-                    int vv[2] = {ed.va, ed.vb};
-                    float gxs[2] = {gax, gbx}, gys[2] = {gay, gby};
-                    for (int j = 0; j < 2; j++) {
-                        float4 c = clipv[vv[j]];
-                        float iw = 1.f / c.w;
-                        float gvx = gxs[j] * a.sx * iw, gvy = gys[j] * a.sy * iw;
-                        aa_vertex_grad(vt, gcs, vv[j], gvx, gvy, -(gvx * c.x + gvy * c.y) * iw);
+                        if (c < C) D[c] += al * (to_o ? go_[c] : gp_[c]);
+                    if (vgrad) {
+                        float d = 0.f;
+#pragma unroll
+                        for (int c = 0; c < AA_MAXC; c++)
+                            if (c < C) d += (to_o ? go_[c] : gp_[c]) * dc[c];
+                        const AAEdge& ed = E[e];
+                        float ia = ed.ia, av = ed.pax * ed.pby - ed.pay * ed.pbx;
+                        float dDax, dDay, dDbx, dDby;
+                        if (ed.horiz) { dDax = 0.f; dDay = -ed.sg; dDbx = 0.f; dDby = ed.sg; }
+                        else { dDax = ed.sg; dDay = 0.f; dDbx = -ed.sg; dDby = 0.f; }
+                        float k2 = av * ia * ia;
+                        float gax = d * (ia * ed.pby - k2 * dDax), gay = d * (-ia * ed.pbx - k2 * dDay);
+                        float gbx = d * (-ia * ed.pay - k2 * dDbx), gby = d * (ia * ed.pax - k2 * dDby);
+                        int vv[2] = {ed.va, ed.vb};
+                        float gxs[2] = {gax, gbx}, gys[2] = {gay, gby};
+                        for (int j = 0; j < 2; j++) {
+                            float4 c = clipv[vv[j]];
+                            float iw = 1.f / c.w;
+                            float gvx = gxs[j] * a.sx * iw, gvy = gys[j] * a.sy * iw;
+                            aa_vertex_grad(vt, gcs, vv[j], gvx, gvy, -(gvx * c.x + gvy * c.y) * iw);
+                        }
                     }
                 }
             }
+            // contribution to the current pixel; the neighbour receives its negation
+            to_cur[p] = pp_cur ? make_float4(D[0], D[1], D[2], D[3]) : make_float4(-D[0], -D[1], -D[2], -D[3]);
         }
-        // contribution to the current pixel; the neighbour receives its negation
-        to_cur[p] = pp_cur ? make_float4(D[0], D[1], D[2], D[3]) : make_float4(-D[0], -D[1], -D[2], -D[3]);
-    }
-    __syncthreads();
-    const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
-    const int px = x0 + tx, py = y0 + ty;
-    if (px < a.W && py < a.H) {
-        size_t q = (size_t)py * a.W + px;
-        float acc[AA_MAXC];
-        aa_load_px(gov, q, C, a.v4, acc);
+        __syncthreads();
+        const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
+        const int px = x0 + tx, py = y0 + ty;
+        if (px < a.W && py < a.H) {
+            size_t q = (size_t)py * a.W + px;
+            float acc[AA_MAXC];
+            aa_load_px(gov, q, C, a.v4, acc);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float4 v = to_cur[aa_pair_slot(tx, ty, k)];
-            float vv[4] = {v.x, v.y, v.z, v.w};
+            for (int k = 0; k < 4; k++) {
+                float4 v = to_cur[aa_pair_slot(tx, ty, k)];
+                float vv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int c = 0; c < AA_MAXC; c++) acc[c] = k < 2 ? acc[c] + vv[c] : acc[c] - vv[c];
+                for (int c = 0; c < AA_MAXC; c++) acc[c] = k < 2 ? acc[c] + vv[c] : acc[c] - vv[c];
+            }
+            float* o = gin + off + q * C;
+#pragma unroll
+            for (int c = 0; c < AA_MAXC; c++)
+                if (c < C) o[c] = acc[c];
         }
-        float* o = gin + off + q * C;
-#pragma unroll
-        for (int c = 0; c < AA_MAXC; c++)
-            if (c < C) o[c] = acc[c];
+        // the pair loop's LDS table updates finished at the barrier above
+        if (gcs) lds_vertex_grad_flush(vt, vused, wave_count, gcs);
     }
-    // the pair loop's LDS table updates finished at the barrier above
-    if (gcs) lds_vertex_grad_flush(vt, vused, wave_count, gcs);
-}
+};
+__global__ void __launch_bounds__(BS, 7) k_aa_bwd(AAArgs a, int tilesX, const float* in, const float* gout, float* gin, float* gproj) { KAaBwdF{}(vb_hw(), a, tilesX, in, gout, gin, gproj); }
 
 // ---------------------------------------------------------------- MSE
 // grad = pred - target: four elements per thread (one 16-byte access each when `vec`) and a
 // partial sum of grad^2 per block.  (A last-block-finishes reduction would need one device
 // atomic per block on a single counter; those serialise, so the partials are summed by a
 // second launch.)
-__global__ void __launch_bounds__(BS) k_mse(size_t n, const float* pred, const float* target, float* grad, float* part, int vec) {
-    const size_t t = (size_t)blockIdx.x * BS + threadIdx.x;
-    float s = 0.f;
-    if (vec) {
-        if (4 * t < n) {
-            const float4 a = ((const float4*)pred)[t], b = ((const float4*)target)[t];
-            const float4 g = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
-            ((float4*)grad)[t] = g;
-            s = ((g.x * g.x + g.y * g.y) + g.z * g.z) + g.w * g.w;
+struct KMseF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, size_t n, const float* pred, const float* target, float* grad, float* part, int vec) const {
+        const size_t t = (size_t)vb.b.x * BS + threadIdx.x;
+        float s = 0.f;
+        if (vec) {
+            if (4 * t < n) {
+                const float4 a = ((const float4*)pred)[t], b = ((const float4*)target)[t];
+                const float4 g = make_float4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w);
+                ((float4*)grad)[t] = g;
+                s = ((g.x * g.x + g.y * g.y) + g.z * g.z) + g.w * g.w;
+            }
+        } else {
+            for (int k = 0; k < 4; k++) {
+                const size_t i = 4 * t + k;
+                if (i < n) { float g = pred[i] - target[i]; grad[i] = g; s += g * g; }
+            }
         }
-    } else {
-        for (int k = 0; k < 4; k++) {
-            const size_t i = 4 * t + k;
-            if (i < n) { float g = pred[i] - target[i]; grad[i] = g; s += g * g; }
-        }
+        s = wave_sum(s);
+        __shared__ float sh[BS / 64];
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) part[vb.b.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
     }
-    s = wave_sum(s);
-    __shared__ float sh[BS / 64];
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = ((sh[0] + sh[1]) + sh[2]) + sh[3];
-}
+};
+__global__ void __launch_bounds__(BS) k_mse(size_t n, const float* pred, const float* target, float* grad, float* part, int vec) { KMseF{}(vb_hw(), n, pred, target, grad, part, vec); }
 
 // loss = 0.5 * the sum of the nb block partials, in a fixed order (one 1024-thread workgroup;
 // thread t sums its contiguous share with four interleaved accumulators so the loads overlap)
-__global__ void __launch_bounds__(1024) k_mse_final(const float* part, size_t nb, float* loss) { mse_final_block<1024>(part, nb, loss); }
+struct KMseFinalF {
+    static constexpr int kBS = 1024, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, const float* part, size_t nb, float* loss) const { mse_final_block<1024>(part, nb, loss); }
+};
+__global__ void __launch_bounds__(1024) k_mse_final(const float* part, size_t nb, float* loss) { KMseFinalF{}(vb_hw(), part, nb, loss); }
 
 }  // namespace
 
@@ -2185,7 +2348,7 @@ int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt
     if (rc) c = *rc;
     const size_t total = std::max(n, c.n());
     if (!total && !bt) return VCL_OK;
-    hipLaunchKernelGGL(k_project_fwd, dim3(total ? cdiv(total, BS) : 1), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat,
+    vlaunch<KProjectFwdF>(k_project_fwd, dim3(total ? cdiv(total, BS) : 1), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat,
                        p->out, zero, bt, rhom, rhov, c);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2199,7 +2362,7 @@ extern "C" int vcl_project_backward(const vcl_project_params* p, const vcl_proje
     VCL_REQUIRE(p && g && p->vec && p->mat && g->out && (p->dimention == 3 || p->dimention == 4));
     int views = p->views < 1 ? 1 : p->views;
     if (p->vecNum <= 0 || (!g->vec && !g->mat)) return VCL_OK;
-    hipLaunchKernelGGL(k_project_bwd, dim3(cdiv(p->vecNum, BS)), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat, g->out,
+    vlaunch<KProjectBwdF>(k_project_bwd, dim3(cdiv(p->vecNum, BS)), dim3(BS), 0, as_hip(stream), p->vecNum, p->dimention, views, p->vec, p->mat, g->out,
                        g->vec, g->mat);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2210,7 +2373,7 @@ extern "C" int vcl_interpolate_forward(const vcl_interpolate_params* p, vcl_stre
     VCL_REQUIRE(!p->enableDA || (p->rastDB && p->outDA));
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_interp_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->dimention, p->enableDA, p->rast, p->rastDB, p->attr,
+    vlaunch<KInterpFwdF>(k_interp_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->dimention, p->enableDA, p->rast, p->rastDB, p->attr,
                        p->idx, p->out, p->outDA);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2220,7 +2383,7 @@ extern "C" int vcl_interpolate_backward(const vcl_interpolate_params* p, const v
     VCL_REQUIRE(p && g && p->rast && p->attr && p->idx && g->out);
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P || (!g->attr && !g->rast)) return VCL_OK;
-    hipLaunchKernelGGL(k_interp_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->dimention, p->rast, p->attr, p->idx, g->out, g->attr,
+    vlaunch<KInterpBwdF>(k_interp_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->dimention, p->rast, p->attr, p->idx, g->out, g->attr,
                        g->rast);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2243,9 +2406,9 @@ extern "C" int vcl_texturemap_forward(const vcl_texturemap_params* p, vcl_stream
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
     if (t.C == 4)
-        hipLaunchKernelGGL(k_tex_fwd4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, p->rast, p->uv, p->uvDA, p->out);
+        vlaunch<KTexFwd4F>(k_tex_fwd4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, p->rast, p->uv, p->uvDA, p->out);
     else
-        hipLaunchKernelGGL(k_tex_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, p->rast, p->uv, p->uvDA, p->out);
+        vlaunch<KTexFwdF>(k_tex_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, p->rast, p->uv, p->uvDA, p->out);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2259,7 +2422,7 @@ int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_pa
     if (st) return st;
     size_t P = (size_t)ip->width * ip->height * ip->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_interp_tex4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, ip->rast, ip->rastDB, ip->attr, ip->idx, ip->out, ip->outDA,
+    vlaunch<KInterpTex4F>(k_interp_tex4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, ip->rast, ip->rastDB, ip->attr, ip->idx, ip->out, ip->outDA,
                        t, tp->out, zero);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2289,17 +2452,17 @@ static int texturemap_backward(const vcl_texturemap_params* p, const vcl_texture
         // fused: the uv grad is consumed in the kernel by the interpolate backward, not stored
         float* guv = ib.grast ? nullptr : g->uv;
         if (rows == 64)
-            hipLaunchKernelGGL(k_tex_bwd4_tiled<64>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
+            vlaunch<KTexBwd4TiledF<64>>(k_tex_bwd4_tiled<64>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
                                g->out, guv, gda, g->daMode, ib);
         else if (rows == 32)
-            hipLaunchKernelGGL(k_tex_bwd4_tiled<32>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
+            vlaunch<KTexBwd4TiledF<32>>(k_tex_bwd4_tiled<32>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
                                g->out, guv, gda, g->daMode, ib);
         else
-            hipLaunchKernelGGL(k_tex_bwd4_tiled<16>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
+            vlaunch<KTexBwd4TiledF<16>>(k_tex_bwd4_tiled<16>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
                                g->out, guv, gda, g->daMode, ib);
         if (fused_done) *fused_done = ib.grast != nullptr;
     } else {
-        hipLaunchKernelGGL(k_tex_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, ga, p->rast, p->uv, p->uvDA, g->out, g->uv,
+        vlaunch<KTexBwdF>(k_tex_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, t, ga, p->rast, p->uv, p->uvDA, g->out, g->uv,
                            t.L > 1 ? g->uvDA : nullptr, g->daMode);
     }
     VCL_LAUNCH_CHECK();
@@ -2339,7 +2502,7 @@ extern "C" int vcl_texture_build_mip(const vcl_texture* t, vcl_stream_t stream) 
         int w = t->width >> l, h = t->height >> l;
         size_t n = (size_t)w * h * t->channel;
         if (!n) continue;
-        hipLaunchKernelGGL(k_downsample, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), t->texture[l - 1], t->texture[l], w, h, t->channel,
+        vlaunch<KDownsampleF>(k_downsample, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), t->texture[l - 1], t->texture[l], w, h, t->channel,
                            (float*)nullptr);
         VCL_LAUNCH_CHECK();
     }
@@ -2350,7 +2513,7 @@ extern "C" int vcl_texture_clamp(const vcl_texture* t, float lo, float hi, vcl_s
     VCL_REQUIRE(t && t->texture[0]);
     size_t n = (size_t)t->width * t->height * t->channel;
     if (n) {
-        hipLaunchKernelGGL(k_clamp, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), t->texture[0], n, lo, hi);
+        vlaunch<KClampF>(k_clamp, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), t->texture[0], n, lo, hi);
         VCL_LAUNCH_CHECK();
     }
     return vcl_texture_build_mip(t, stream);
@@ -2363,7 +2526,7 @@ extern "C" int vcl_texture_grad_sumup(const vcl_texture* g, int quarter, vcl_str
         int w = g->width >> l, h = g->height >> l;
         size_t n = (size_t)w * h * g->channel;
         if (!n) continue;
-        hipLaunchKernelGGL(k_fold, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), g->texture[l], g->texture[l - 1], w, h, g->channel,
+        vlaunch<KFoldF>(k_fold, dim3(cdiv(n, BS)), dim3(BS), 0, as_hip(stream), g->texture[l], g->texture[l - 1], w, h, g->channel,
                            quarter ? 0.25f : 1.f);
         VCL_LAUNCH_CHECK();
     }
@@ -2430,7 +2593,7 @@ int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_
         }
         a.block_start[a.n] = blocks;
         if (blocks > 0 || a.loss) {
-            hipLaunchKernelGGL(k_adam, dim3((unsigned)(blocks + (a.loss ? 1 : 0))), dim3(BS), 0, as_hip(stream), a);
+            vlaunch<KAdamF>(k_adam, dim3((unsigned)(blocks + (a.loss ? 1 : 0))), dim3(BS), 0, as_hip(stream), a);
             VCL_LAUNCH_CHECK();
         }
     }
@@ -2470,11 +2633,11 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
     const bool quad = vec4 && a.L > 1 && (part & 4) && a.tw % 2 == 0 && a.th % 2 == 0 && a.th / 2 <= 65535;
     if (part & 1) {
         if (quad)
-            hipLaunchKernelGGL(k_tex_update4m, dim3((unsigned)cdiv((size_t)a.tw, 128), (unsigned)(a.th / 2)), dim3(BS), 0, s, a);
+            vlaunch<KTexUpdate4mF>(k_tex_update4m, dim3((unsigned)cdiv((size_t)a.tw, 128), (unsigned)(a.th / 2)), dim3(BS), 0, s, a);
         else if (vec4)
-            hipLaunchKernelGGL(k_tex_update4, dim3(cdiv((size_t)a.tw * a.th, BS)), dim3(BS), 0, s, a);
+            vlaunch<KTexUpdate4F>(k_tex_update4, dim3(cdiv((size_t)a.tw * a.th, BS)), dim3(BS), 0, s, a);
         else
-            hipLaunchKernelGGL(k_tex_update, dim3(cdiv(n, BS)), dim3(BS), 0, s, a);
+            vlaunch<KTexUpdateF>(k_tex_update, dim3(cdiv(n, BS)), dim3(BS), 0, s, a);
         VCL_LAUNCH_CHECK();
     }
     // rebuild mips from the new level 0; the same pass zeroes the coarse grads.  C == 4: level 1
@@ -2484,7 +2647,7 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
         int l = quad ? 2 : 1;
         if (l == 1 && l < a.L) {
             int w = a.tw >> 1, hh = a.th >> 1;
-            hipLaunchKernelGGL(k_downsample4, dim3(cdiv((size_t)w * hh, BS)), dim3(BS), 0, s, (const float4*)tex->texture[0],
+            vlaunch<KDownsample4F>(k_downsample4, dim3(cdiv((size_t)w * hh, BS)), dim3(BS), 0, s, (const float4*)tex->texture[0],
                                (float4*)tex->texture[1], w, hh, (float4*)a.grad[1]);
             VCL_LAUNCH_CHECK();
             l = 2;
@@ -2499,7 +2662,7 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
                 m.dst[k] = k < m.nl ? (float4*)tex->texture[l + k] : nullptr;
                 m.zg[k] = k < m.nl ? (float4*)a.grad[l + k] : nullptr;
             }
-            hipLaunchKernelGGL(k_mip_fused, dim3(cdiv(m.sw, 64), cdiv(m.sh, 64)), dim3(BS), 0, s, m);
+            vlaunch<KMipFusedF>(k_mip_fused, dim3(cdiv(m.sw, 64), cdiv(m.sh, 64)), dim3(BS), 0, s, m);
             VCL_LAUNCH_CHECK();
             l += m.nl;
         }
@@ -2509,7 +2672,7 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
         int w = a.tw >> l, hh = a.th >> l;
         size_t m = (size_t)w * hh * a.C;
         if (!m) continue;
-        hipLaunchKernelGGL(k_downsample, dim3(cdiv(m, BS)), dim3(BS), 0, s, tex->texture[l - 1], tex->texture[l], w, hh, a.C, a.grad[l]);
+        vlaunch<KDownsampleF>(k_downsample, dim3(cdiv(m, BS)), dim3(BS), 0, s, tex->texture[l - 1], tex->texture[l], w, hh, a.C, a.grad[l]);
         VCL_LAUNCH_CHECK();
     }
     return VCL_OK;
@@ -2524,7 +2687,7 @@ extern "C" int vcl_normal_axis_forward(const vcl_normal_axis_params* p, vcl_stre
     VCL_REQUIRE(p && p->rast && p->rot && p->normal && p->normalidx && p->out);
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_normal_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->rot, p->normal, p->normalidx, p->out);
+    vlaunch<KNormalAxisF>(k_normal_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->rot, p->normal, p->normalidx, p->out);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2563,7 +2726,7 @@ extern "C" int vcl_view_axis_forward(const vcl_view_axis_params* p, vcl_stream_t
     VCL_REQUIRE(p->depth <= 8);
     Mat3x8 m;
     for (int s = 0; s < p->depth; s++) pv_inverse(p->projection + 16 * s, p->view + 16 * s, m.m[s]);
-    hipLaunchKernelGGL(k_view_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), p->width, p->height, p->depth, (const float*)nullptr, m,
+    vlaunch<KViewAxisF>(k_view_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), p->width, p->height, p->depth, (const float*)nullptr, m,
                        p->out);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2575,7 +2738,7 @@ extern "C" int vcl_view_axis_forward_dev(int W, int H, int depth, const float* p
     size_t P = (size_t)W * H * depth;
     if (!P) return VCL_OK;
     Mat3x8 m;
-    hipLaunchKernelGGL(k_view_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), W, H, depth, pvinv_dev, m, out);
+    vlaunch<KViewAxisF>(k_view_axis, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), W, H, depth, pvinv_dev, m, out);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2586,7 +2749,7 @@ int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev,
     VCL_REQUIRE(np->width == sp->width && np->height == sp->height && np->depth == sp->depth && sp->normal == np->out && sp->rast == np->rast);
     size_t P = (size_t)np->width * np->height * np->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_shade_prep, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), np->width, np->height, np->depth, np->rast, np->rot, np->normal,
+    vlaunch<KShadePrepF>(k_shade_prep, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), np->width, np->height, np->depth, np->rast, np->rot, np->normal,
                        np->normalidx, pvinv_dev, sp->roughness, sp->ior, np->out, const_cast<float*>(sp->view), sp->outAxis, sp->outSharpness,
                        sp->outAmplitude);
     VCL_LAUNCH_CHECK();
@@ -2597,7 +2760,7 @@ extern "C" int vcl_sg_specular_forward(const vcl_sg_specular_params* p, vcl_stre
     VCL_REQUIRE(p && p->rast && p->normal && p->view && p->roughness && p->outAxis && p->outSharpness && p->outAmplitude);
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_sg_spec_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->normal, p->view, p->roughness, p->ior,
+    vlaunch<KSgSpecFwdF>(k_sg_spec_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->normal, p->view, p->roughness, p->ior,
                        p->outAxis, p->outSharpness, p->outAmplitude);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2609,7 +2772,7 @@ extern "C" int vcl_sg_specular_backward(const vcl_sg_specular_params* p, const v
     VCL_REQUIRE(g->outAmplitude && g->outSharpness);
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
-    hipLaunchKernelGGL(k_sg_spec_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->normal, p->view, p->roughness, p->ior,
+    vlaunch<KSgSpecBwdF>(k_sg_spec_bwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->rast, p->normal, p->view, p->roughness, p->ior,
                        g->outAmplitude, g->outSharpness, g->roughness);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2643,20 +2806,20 @@ extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_param
                       ((uintptr_t)p->outSpecenv % 16 == 0);
     if (p->sgnum <= SG_MAXK && p->channel == 4 && al16) {
         if (cdiv(P, 4 * BS) >= 4096)
-            hipLaunchKernelGGL(k_sg_fwd4<4 * BS>, dim3(cdiv(P, 4 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
+            vlaunch<KSgFwd4F<4 * BS>>(k_sg_fwd4<4 * BS>, dim3(cdiv(P, 4 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
                                p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
                                p->outSpecenv);
         else
-            hipLaunchKernelGGL(k_sg_fwd4<2 * BS>, dim3(cdiv(P, 2 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
+            vlaunch<KSgFwd4F<2 * BS>>(k_sg_fwd4<2 * BS>, dim3(cdiv(P, 2 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
                                p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
                                p->outSpecenv);
     }
     else if (p->sgnum <= SG_MAXK)
-        hipLaunchKernelGGL(k_sg_fwd_fast, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
+        vlaunch<KSgFwdFastF>(k_sg_fwd_fast, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
                            p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
                            p->outSpecenv);
     else
-        hipLaunchKernelGGL(k_sg_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
+        vlaunch<KSgFwdF>(k_sg_fwd, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, p->channel, p->sgnum, p->rast, p->axis, p->sharpness,
                            p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
                            p->outSpecenv);
     VCL_LAUNCH_CHECK();
@@ -2682,11 +2845,12 @@ int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, 
     if (!per_pixel && K <= SG_MAXK && p->outDiffenv) {
         int KP = 1;
         while (KP < K) KP <<= 1;
-        hipLaunchKernelGGL(k_sg_bwd_lobes, dim3(nb), dim3(BS), 0, s, P, C, K, KP, p->rast, p->axis, p->sharpness, p->amplitude, p->normal,
+        vlaunch<KSgBwdLobesF>(k_sg_bwd_lobes, dim3(nb), dim3(BS), 0, s, P, C, K, KP, p->rast, p->axis, p->sharpness, p->amplitude, p->normal,
                            p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, g->out, p->outDiffenv, g->diffuse, part, want);
     } else {
         size_t lds = sizeof(float) * 4 * (size_t)K * SG_COLS;
         VCL_REQUIRE(lds <= 160 * 1024);
+        if (g_vcl_recorder) return VCL_ERR_INVALID;   // per-pixel lobe grads: not a groupable launch
         hipLaunchKernelGGL(k_sg_bwd, dim3(nb), dim3(BS), want ? lds : 0, s, P, C, K, p->rast, p->axis, p->sharpness, p->amplitude, p->normal,
                            p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, g->out, g->diffuse, g->normal, g->specAxis,
                            g->specSharpness, g->specAmplitude, part, want);
@@ -2701,7 +2865,7 @@ int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, 
         lobes->seg = seg; lobes->nseg = nseg; lobes->K = K; lobes->C = C;
         return VCL_OK;
     }
-    hipLaunchKernelGGL(k_sg_lobe_final, dim3(cdiv(cols, 256)), dim3(256), 0, s, seg, nseg, K, C, g->axis, g->sharpness, g->amplitude);
+    vlaunch<KSgLobeFinalF>(k_sg_lobe_final, dim3(cdiv(cols, 256)), dim3(256), 0, s, seg, nseg, K, C, g->axis, g->sharpness, g->amplitude);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2726,7 +2890,7 @@ extern "C" int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
     int tilesX = (p->width + AA_T - 1) / AA_T, tilesY = (p->height + AA_T - 1) / AA_T;
-    hipLaunchKernelGGL(k_aa_fwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p, nullptr), tilesX, p->in, p->out);
+    vlaunch<KAaFwdF>(k_aa_fwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p, nullptr), tilesX, p->in, p->out);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2736,7 +2900,7 @@ extern "C" int vcl_antialias_backward(const vcl_antialias_params* p, const vcl_a
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
     int tilesX = (p->width + AA_T - 1) / AA_T, tilesY = (p->height + AA_T - 1) / AA_T;
-    hipLaunchKernelGGL(k_aa_bwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p, g->out), tilesX, p->in, g->out, g->in, g->proj);
+    vlaunch<KAaBwdF>(k_aa_bwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p, g->out), tilesX, p->in, g->out, g->in, g->proj);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2753,7 +2917,7 @@ int vcl_mse_loss_grad_dev(const vcl_loss_params* p, vcl_stream_t stream) {
     const size_t nb = p->size > 0 ? cdiv((size_t)p->size, (size_t)BS * 4) : 0;
     if (nb) {
         const int vec = ((p->size & 3) == 0 && (((uintptr_t)p->predict | (uintptr_t)p->target | (uintptr_t)p->grad) & 15) == 0) ? 1 : 0;
-        hipLaunchKernelGGL(k_mse, dim3((unsigned)nb), dim3(BS), 0, s, (size_t)p->size, p->predict, p->target, p->grad, p->buffer, vec);
+        vlaunch<KMseF>(k_mse, dim3((unsigned)nb), dim3(BS), 0, s, (size_t)p->size, p->predict, p->target, p->grad, p->buffer, vec);
         VCL_LAUNCH_CHECK();
     }
     return VCL_OK;
@@ -2763,7 +2927,7 @@ extern "C" int vcl_mse_loss_backward(const vcl_loss_params* p, vcl_stream_t stre
     int st = vcl_mse_loss_grad_dev(p, stream);
     if (st) return st;
     const size_t nb = p->size > 0 ? cdiv((size_t)p->size, (size_t)BS * 4) : 0;
-    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(1024), 0, as_hip(stream), p->buffer, nb, p->loss);
+    vlaunch<KMseFinalF>(k_mse_final, dim3(1), dim3(1024), 0, as_hip(stream), p->buffer, nb, p->loss);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -2811,7 +2975,7 @@ extern "C" int vcl_mse_loss_backward_noalloc(const vcl_loss_params* p, vcl_strea
     q.buffer = part;
     if ((st = vcl_mse_loss_grad_dev(&q, stream))) return st;
     const size_t nb = q.size > 0 ? cdiv((size_t)q.size, (size_t)BS * 4) : 0;
-    hipLaunchKernelGGL(k_mse_final, dim3(1), dim3(1024), 0, as_hip(stream), part, nb, q.loss);
+    vlaunch<KMseFinalF>(k_mse_final, dim3(1), dim3(1024), 0, as_hip(stream), part, nb, q.loss);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }

@@ -327,34 +327,39 @@ __device__ __forceinline__ int small_tiles(const RastArgs& a, const RRec& r, u32
 // single lane.  These entries spread over many tiles per workgroup, so they go straight to
 // the global counters (LDS aggregation measured slower here).
 template <bool FILL>
-__global__ void __launch_bounds__(256) k_bin_large(RastArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int nwaves = (gridDim.x * blockDim.x) >> 6;
-    const u32 nlarge = a.ws.misc[3];
-    for (u32 i = wave; i < nlarge; i += nwaves) {
-        u32 slot = a.ws.large[i];
-        const RRec& r = a.ws.rec[slot];
-        int view = (int)(r.view_tl >> 8);
-        int tx0 = (r.pxr & 0xffff) >> a.tshift, tx1 = (r.pxr >> 16) >> a.tshift;
-        int ty0 = (r.pyr & 0xffff) >> a.tshift, ty1 = (r.pyr >> 16) >> a.tshift;
-        int ntx = tx1 - tx0 + 1, n = ntx * (ty1 - ty0 + 1);
-        for (int k = lane; k < n; k += 64) {
-            int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
-            if (!tile_overlaps(r, tx, ty, a.tshift)) continue;
-            u32 g = (u32)(view * a.ntiles + ty * a.tilesX + tx);
-            const u32 sub = slot & (LSUB - 1);
-            const size_t NT = (size_t)a.ntiles * a.depth;
-            if (!FILL) {
-                atomicAdd(a.ws.lcounts + sub * NT + g, 1u);
-            } else {
-                // the tile's large pieces take the bottom of its bin (the small ones count down
-                // from the top): sub-counter ranges in order, each filled by its cursor
-                bin_write(a, g, atomicAdd(a.ws.lcursor + sub * NT + g, 1u), slot);
+struct KBinLargeF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, RastArgs a) const {
+        const int lane = threadIdx.x & 63;
+        const int wave = (vb.b.x * blockDim.x + threadIdx.x) >> 6;
+        const int nwaves = (vb.g.x * blockDim.x) >> 6;
+        const u32 nlarge = a.ws.misc[3];
+        for (u32 i = wave; i < nlarge; i += nwaves) {
+            u32 slot = a.ws.large[i];
+            const RRec& r = a.ws.rec[slot];
+            int view = (int)(r.view_tl >> 8);
+            int tx0 = (r.pxr & 0xffff) >> a.tshift, tx1 = (r.pxr >> 16) >> a.tshift;
+            int ty0 = (r.pyr & 0xffff) >> a.tshift, ty1 = (r.pyr >> 16) >> a.tshift;
+            int ntx = tx1 - tx0 + 1, n = ntx * (ty1 - ty0 + 1);
+            for (int k = lane; k < n; k += 64) {
+                int tx = tx0 + k % ntx, ty = ty0 + k / ntx;
+                if (!tile_overlaps(r, tx, ty, a.tshift)) continue;
+                u32 g = (u32)(view * a.ntiles + ty * a.tilesX + tx);
+                const u32 sub = slot & (LSUB - 1);
+                const size_t NT = (size_t)a.ntiles * a.depth;
+                if (!FILL) {
+                    atomicAdd(a.ws.lcounts + sub * NT + g, 1u);
+                } else {
+                    // the tile's large pieces take the bottom of its bin (the small ones count down
+                    // from the top): sub-counter ranges in order, each filled by its cursor
+                    bin_write(a, g, atomicAdd(a.ws.lcursor + sub * NT + g, 1u), slot);
+                }
             }
         }
     }
-}
+};
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_bin_large(RastArgs a) { KBinLargeF<FILL>{}(vb_hw(), a); }
 
 // Workgroup-local list of the large pieces found by setup, appended to the global list with
 // one atomic per workgroup.
@@ -380,10 +385,14 @@ __device__ __forceinline__ void bin_or_defer(const RastArgs& a, SetupShared& sh,
 
 // Per-launch workspace reset: tile counters, large-piece sub-counters and misc[] to zero, the
 // per-pixel depth keys to all-ones (no winner); grid-stride over the largest.
-__global__ void __launch_bounds__(256) k_clear(WsClear c) {
-    const size_t n = c.n();
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) ws_clear_elem(c, i);
-}
+struct KClearF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, WsClear c) const {
+        const size_t n = c.n();
+        for (size_t i = (size_t)vb.b.x * blockDim.x + threadIdx.x; i < n; i += (size_t)vb.g.x * blockDim.x) ws_clear_elem(c, i);
+    }
+};
+__global__ void __launch_bounds__(256) k_clear(WsClear c) { KClearF{}(vb_hw(), c); }
 
 WsClear ws_clear_of(const Ws& w, const Layout& L, size_t P) {
     WsClear c;
@@ -396,19 +405,23 @@ WsClear ws_clear_of(const Ws& w, const Layout& L, size_t P) {
 
 // Adds each tile's large-piece sub-counts to its count, before the scan, and sets each
 // sub-counter's fill cursor to the start of its range (the exclusive prefix of the sub-counts).
-__global__ void __launch_bounds__(256) k_fold_large(Ws w, int NT) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= NT) return;
-    u32 t = 0;
+struct KFoldLargeF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, Ws w, int NT) const {
+        const int g = vb.b.x * blockDim.x + threadIdx.x;
+        if (g >= NT) return;
+        u32 t = 0;
 #pragma unroll
-    for (int q = 0; q < LSUB; q++) {
-        const size_t i = (size_t)q * NT + g;
-        const u32 c = w.lcounts[i];
-        w.lcursor[i] = t;
-        t += c;
+        for (int q = 0; q < LSUB; q++) {
+            const size_t i = (size_t)q * NT + g;
+            const u32 c = w.lcounts[i];
+            w.lcursor[i] = t;
+            t += c;
+        }
+        if (t) w.counts[g] += t;
     }
-    if (t) w.counts[g] += t;
-}
+};
+__global__ void __launch_bounds__(256) k_fold_large(Ws w, int NT) { KFoldLargeF{}(vb_hw(), w, NT); }
 
 __device__ void setup_triangle(const RastArgs& a, SetupShared& sh, size_t gid) {
     if (gid >= a.nrec_direct) return;
@@ -455,19 +468,23 @@ __device__ void setup_triangle(const RastArgs& a, SetupShared& sh, size_t gid) {
 // One thread per (layer, triangle): guard-band test, clipping when needed, snapping and edge
 // setup, one record per piece; small pieces are counted into their tiles' bins, large pieces
 // are listed for the wave-per-piece pass.
-__global__ void __launch_bounds__(256) k_setup(RastArgs a) {
-    __shared__ SetupShared sh;
-    bin_table_clear(sh.tab);
-    if (threadIdx.x == 0) sh.nlq = 0;
-    __syncthreads();
-    setup_triangle(a, sh, (size_t)blockIdx.x * blockDim.x + threadIdx.x);
-    __syncthreads();
-    bin_count_flush(a, sh.tab.key, sh.tab.val, BIN_SLOTS);
-    u32 nl = min(sh.nlq, (u32)SETUP_LQ);
-    if (threadIdx.x == 0 && nl) sh.lbase = atomicAdd(a.ws.misc + 3, nl);
-    __syncthreads();
-    for (u32 i = threadIdx.x; i < nl; i += blockDim.x) a.ws.large[sh.lbase + i] = sh.lq[i];
-}
+struct KSetupF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, RastArgs a) const {
+        __shared__ SetupShared sh;
+        bin_table_clear(sh.tab);
+        if (threadIdx.x == 0) sh.nlq = 0;
+        __syncthreads();
+        setup_triangle(a, sh, (size_t)vb.b.x * blockDim.x + threadIdx.x);
+        __syncthreads();
+        bin_count_flush(a, sh.tab.key, sh.tab.val, BIN_SLOTS);
+        u32 nl = min(sh.nlq, (u32)SETUP_LQ);
+        if (threadIdx.x == 0 && nl) sh.lbase = atomicAdd(a.ws.misc + 3, nl);
+        __syncthreads();
+        for (u32 i = threadIdx.x; i < nl; i += blockDim.x) a.ws.large[sh.lbase + i] = sh.lq[i];
+    }
+};
+__global__ void __launch_bounds__(256) k_setup(RastArgs a) { KSetupF{}(vb_hw(), a); }
 
 // Exclusive prefix of one value per thread over a 1024-lane workgroup, continuing from *carry
 // (which is advanced by the workgroup's total).
@@ -491,93 +508,101 @@ __device__ u32 scan1024(u32 v, u32* wsum, u32* carry) {
 // Single workgroup: exclusive scan of the NT tile counters into bin offsets, then the fine-raster
 // work list: each tile's bin cut into chunks of `chunk` pieces (one chunk for an empty tile,
 // and one scan-everything chunk for a tile whose bin would pass the bin capacity).
-__global__ void __launch_bounds__(1024) k_scan(RastArgs a, int NT) {
-    __shared__ u32 wsum[16];
-    __shared__ u32 carry[2];
-    int tid = threadIdx.x;
-    if (tid == 0) { carry[0] = 0; carry[1] = 0; }
-    __syncthreads();
-    const u32 nrec = (u32)(a.nrec_direct + min(a.ws.misc[0], (u32)a.ovf_cap));
-    // each thread takes TPT consecutive tiles: their local prefix, then one block scan of the
-    // threads' totals (TPT times fewer block scans and barriers than one tile per thread)
-    constexpr int TPT = 4;
-    for (int base = 0; base < NT; base += 1024 * TPT) {
-        const int i0 = base + TPT * tid;
-        u32 v[TPT], loc[TPT], t = 0;
+struct KScanF {
+    static constexpr int kBS = 1024, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, RastArgs a, int NT) const {
+        __shared__ u32 wsum[16];
+        __shared__ u32 carry[2];
+        int tid = threadIdx.x;
+        if (tid == 0) { carry[0] = 0; carry[1] = 0; }
+        __syncthreads();
+        const u32 nrec = (u32)(a.nrec_direct + min(a.ws.misc[0], (u32)a.ovf_cap));
+        // each thread takes TPT consecutive tiles: their local prefix, then one block scan of the
+        // threads' totals (TPT times fewer block scans and barriers than one tile per thread)
+        constexpr int TPT = 4;
+        for (int base = 0; base < NT; base += 1024 * TPT) {
+            const int i0 = base + TPT * tid;
+            u32 v[TPT], loc[TPT], t = 0;
 #pragma unroll
-        for (int j = 0; j < TPT; j++) {
-            v[j] = i0 + j < NT ? a.ws.counts[i0 + j] : 0u;
-            loc[j] = t;
-            t += v[j];
-        }
-        const u32 pre0 = scan1024(t, wsum, &carry[0]);
-        u32 nch[TPT], cloc[TPT], tc = 0;
-        bool all[TPT];
+            for (int j = 0; j < TPT; j++) {
+                v[j] = i0 + j < NT ? a.ws.counts[i0 + j] : 0u;
+                loc[j] = t;
+                t += v[j];
+            }
+            const u32 pre0 = scan1024(t, wsum, &carry[0]);
+            u32 nch[TPT], cloc[TPT], tc = 0;
+            bool all[TPT];
 #pragma unroll
-        for (int j = 0; j < TPT; j++) {
-            const int i = i0 + j;
-            const u32 pre = pre0 + loc[j];
-            if (i < NT) a.ws.offsets[i] = pre;
-            all[j] = (size_t)pre + v[j] > a.bin_cap;
-            nch[j] = i >= NT ? 0u : all[j] ? 1u : max(1u, (v[j] + a.chunk - 1) / a.chunk);
-            cloc[j] = tc;
-            tc += nch[j];
-        }
-        const u32 cpre0 = scan1024(tc, wsum, &carry[1]);
+            for (int j = 0; j < TPT; j++) {
+                const int i = i0 + j;
+                const u32 pre = pre0 + loc[j];
+                if (i < NT) a.ws.offsets[i] = pre;
+                all[j] = (size_t)pre + v[j] > a.bin_cap;
+                nch[j] = i >= NT ? 0u : all[j] ? 1u : max(1u, (v[j] + a.chunk - 1) / a.chunk);
+                cloc[j] = tc;
+                tc += nch[j];
+            }
+            const u32 cpre0 = scan1024(tc, wsum, &carry[1]);
 #pragma unroll
-        for (int j = 0; j < TPT; j++) {
-            const u32 pre = pre0 + loc[j];
-            for (u32 k = 0; k < nch[j]; k++) {
-                size_t c = (size_t)cpre0 + cloc[j] + k;
-                if (c >= a.chunk_cap) break;
-                Chunk ch;
-                ch.tile = (u32)(i0 + j);
-                ch.first = all[j] ? 0u : pre + k * a.chunk;
-                ch.count = all[j] ? (nrec | CHUNK_SCAN_ALL) : min(a.chunk, v[j] - k * a.chunk);
-                ch.k = k;
-                a.ws.chunks[c] = ch;
+            for (int j = 0; j < TPT; j++) {
+                const u32 pre = pre0 + loc[j];
+                for (u32 k = 0; k < nch[j]; k++) {
+                    size_t c = (size_t)cpre0 + cloc[j] + k;
+                    if (c >= a.chunk_cap) break;
+                    Chunk ch;
+                    ch.tile = (u32)(i0 + j);
+                    ch.first = all[j] ? 0u : pre + k * a.chunk;
+                    ch.count = all[j] ? (nrec | CHUNK_SCAN_ALL) : min(a.chunk, v[j] - k * a.chunk);
+                    ch.k = k;
+                    a.ws.chunks[c] = ch;
+                }
             }
         }
+        if (tid == 0) {
+            a.ws.offsets[NT] = carry[0];
+            a.ws.misc[5] = (u32)min((size_t)carry[1], a.chunk_cap);
+        }
     }
-    if (tid == 0) {
-        a.ws.offsets[NT] = carry[0];
-        a.ws.misc[5] = (u32)min((size_t)carry[1], a.chunk_cap);
-    }
-}
+};
+__global__ void __launch_bounds__(1024) k_scan(RastArgs a, int NT) { KScanF{}(vb_hw(), a, NT); }
 
 // Small pieces into their tiles' bins: entries counted into LDS (local position from the LDS
 // atomic), each tile's run reserved with one global atomic, then written.
-__global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) {
-    __shared__ BinTable tab;
-    bin_table_clear(tab);
-    __syncthreads();
-    size_t slot = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    u32 g[SMALL_TILES], loc[SMALL_TILES];
-    int sl[SMALL_TILES];
-    int n = 0;
-    bool ok = slot < nslots;
-    if (ok && slot >= a.nrec_direct) {
-        u32 novf = min(a.ws.misc[0], (u32)a.ovf_cap);
-        ok = slot - a.nrec_direct < novf;
-    }
-    if (ok) {
-        const RRec& r = a.ws.rec[slot];
-        if (r.id != INVALID) {
-            RRec rr = r;
-            if (piece_small(rr, a.tshift)) n = small_tiles(a, rr, g);
+struct KFillF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, RastArgs a, size_t nslots) const {
+        __shared__ BinTable tab;
+        bin_table_clear(tab);
+        __syncthreads();
+        size_t slot = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        u32 g[SMALL_TILES], loc[SMALL_TILES];
+        int sl[SMALL_TILES];
+        int n = 0;
+        bool ok = slot < nslots;
+        if (ok && slot >= a.nrec_direct) {
+            u32 novf = min(a.ws.misc[0], (u32)a.ovf_cap);
+            ok = slot - a.nrec_direct < novf;
         }
+        if (ok) {
+            const RRec& r = a.ws.rec[slot];
+            if (r.id != INVALID) {
+                RRec rr = r;
+                if (piece_small(rr, a.tshift)) n = small_tiles(a, rr, g);
+            }
+        }
+        for (int i = 0; i < n; i++) {
+            sl[i] = bin_slot(tab, g[i]);
+            if (sl[i] >= 0) loc[i] = atomicAdd(&tab.val[sl[i]], 1u);
+            else bin_fill_direct(a, g[i], (u32)slot);
+        }
+        __syncthreads();
+        bin_fill_reserve(a, tab.key, tab.val, BIN_SLOTS);
+        __syncthreads();
+        for (int i = 0; i < n; i++)
+            if (sl[i] >= 0) bin_write(a, g[i], tab.val[sl[i]] + loc[i], (u32)slot);
     }
-    for (int i = 0; i < n; i++) {
-        sl[i] = bin_slot(tab, g[i]);
-        if (sl[i] >= 0) loc[i] = atomicAdd(&tab.val[sl[i]], 1u);
-        else bin_fill_direct(a, g[i], (u32)slot);
-    }
-    __syncthreads();
-    bin_fill_reserve(a, tab.key, tab.val, BIN_SLOTS);
-    __syncthreads();
-    for (int i = 0; i < n; i++)
-        if (sl[i] >= 0) bin_write(a, g[i], tab.val[sl[i]] + loc[i], (u32)slot);
-}
+};
+__global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) { KFillF{}(vb_hw(), a, nslots); }
 
 constexpr u32 SPAN_EMPTY = 0x00ffu;   // first column 255 > last column 0
 
@@ -713,264 +738,278 @@ __device__ __forceinline__ bool piece_covers_rect(const RRec& r, int px0, int px
 // bin is split: a tile's only chunk stores its keys, the chunks of a split tile merge theirs
 // with a device-scope 64-bit minimum.
 template <int TS>
-__global__ void __launch_bounds__(256, TS == 32 ? 4 : 5) k_fine(RastArgs a) {
-    constexpr int NP = TS * TS;
-    __shared__ u64 key[NP];
-    __shared__ float spix[2][TS];
-    constexpr int BATCH = TS == 32 ? 128 : 256;    // pieces per LDS batch
-    __shared__ unsigned short sspan[BATCH][TS];     // per piece: first | last << 8 column per row
-    __shared__ float szp[9][BATCH];
-    __shared__ u32 sid[BATCH];
-    constexpr int ITEM_CAP = BATCH * TS < 2048 ? BATCH * TS : 2048;
-    __shared__ u32 sitem[ITEM_CAP];                  // covered rows: piece | row | first column | length
-    __shared__ u32 sbucket[4];                       // per-wave covered-row totals
-    __shared__ u32 sbound;                           // ordered depth bound of the tile (culling)
-    const int tid = threadIdx.x;
-    const u32 nchunks = a.ws.misc[5];
-    for (u32 c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const Chunk ch = a.ws.chunks[c];
-        const int tile = (int)ch.tile;
-        const u32 view = (u32)(tile / a.ntiles);
-        const int tl = tile % a.ntiles;
-        const int tx = tl % a.tilesX, ty = tl / a.tilesX;
-        for (int i = tid; i < NP; i += 256) key[i] = ~0ull;
-        if (tid == 0) sbound = 0xffffffffu;
-        if (tid < 2 * TS) {
-            int cc = tid % TS;
-            spix[tid / TS][cc] = tid < TS ? (float)((tx * TS + cc + 0.5) * (double)a.xs - 1.0)
-                                          : (float)((ty * TS + cc + 0.5) * (double)a.ys - 1.0);
-        }
-        const int x0 = tx * TS, y0 = ty * TS;
-        const long long X0 = (long long)x0 * 256 + 128;
-        const bool scan_all = (ch.count & CHUNK_SCAN_ALL) != 0u;
-        const u32 n = ch.count & ~CHUNK_SCAN_ALL;
-        __syncthreads();
-        // the tile's pixel rectangle inside the image, for the culling bounds
-        const int rx1 = min(x0 + TS, a.W) - 1 - x0, ry1 = min(y0 + TS, a.H) - 1 - y0;
-        for (u32 base = 0; base < n; base += BATCH) {
-            // phase 1a: thread = piece; conservative depth range over the tile; pieces covering
-            // the whole tile with every pixel test passing lower the tile's depth bound
-            const u32 q = base + tid;
-            int yl = 1, yh = 0;
-            u32 nrow = 0;
-            bool live = false;
-            float zlo = 0.f;
-            bool has_lo = false;
-            RRec r;   // the piece's record, held in registers across the barrier below
-            if (tid < BATCH && q < n) {
-                const u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
-                r = a.ws.rec[slot];
-                int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
-                int yl0 = max((r.pyr & 0xffff) - y0, 0), yh0 = min((r.pyr >> 16) - y0, TS - 1);
-                live = r.id != INVALID && (r.view_tl >> 8) == view && xl <= xh && yl0 <= yh0;
-                // a piece whose box holds few of the tile's pixels saves little when culled and
-                // cannot cover the tile, so its range is not formed
-                if (live && (xh - xl + 1) * (yh0 - yl0 + 1) >= CULL_MIN_BOX) {
-                    float hi;
-                    bool valid;
-                    has_lo = piece_depth_range(r.zp, spix[0][0], spix[0][rx1], spix[1][0], spix[1][ry1], zlo, hi, valid);
-                    if (has_lo && valid && piece_covers_rect(r, x0, x0 + rx1, y0, y0 + ry1)) atomicMin(&sbound, ord_f32(hi));
-                }
+struct KFineF {
+    static constexpr int kBS = 256, kMINW = TS == 32 ? 4 : 5;
+    __device__ __forceinline__ void operator()(const VB& vb, RastArgs a) const {
+        constexpr int NP = TS * TS;
+        __shared__ u64 key[NP];
+        __shared__ float spix[2][TS];
+        constexpr int BATCH = TS == 32 ? 128 : 256;    // pieces per LDS batch
+        __shared__ unsigned short sspan[BATCH][TS];     // per piece: first | last << 8 column per row
+        __shared__ float szp[9][BATCH];
+        __shared__ u32 sid[BATCH];
+        constexpr int ITEM_CAP = BATCH * TS < 2048 ? BATCH * TS : 2048;
+        __shared__ u32 sitem[ITEM_CAP];                  // covered rows: piece | row | first column | length
+        __shared__ u32 sbucket[4];                       // per-wave covered-row totals
+        __shared__ u32 sbound;                           // ordered depth bound of the tile (culling)
+        const int tid = threadIdx.x;
+        const u32 nchunks = a.ws.misc[5];
+        for (u32 c = vb.b.x; c < nchunks; c += vb.g.x) {
+            const Chunk ch = a.ws.chunks[c];
+            const int tile = (int)ch.tile;
+            const u32 view = (u32)(tile / a.ntiles);
+            const int tl = tile % a.ntiles;
+            const int tx = tl % a.tilesX, ty = tl / a.tilesX;
+            for (int i = tid; i < NP; i += 256) key[i] = ~0ull;
+            if (tid == 0) sbound = 0xffffffffu;
+            if (tid < 2 * TS) {
+                int cc = tid % TS;
+                spix[tid / TS][cc] = tid < TS ? (float)((tx * TS + cc + 0.5) * (double)a.xs - 1.0)
+                                              : (float)((ty * TS + cc + 0.5) * (double)a.ys - 1.0);
             }
+            const int x0 = tx * TS, y0 = ty * TS;
+            const long long X0 = (long long)x0 * 256 + 128;
+            const bool scan_all = (ch.count & CHUNK_SCAN_ALL) != 0u;
+            const u32 n = ch.count & ~CHUNK_SCAN_ALL;
             __syncthreads();
-            // phase 1b: pieces not behind the bound: exact row spans and depth planes into LDS;
-            // covered rows counted
-            if (live && !(has_lo && ord_f32(zlo) > sbound)) {
-                u32 id = r.id, vtl = r.view_tl;
-                int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
-                yl = max((r.pyr & 0xffff) - y0, 0);
-                yh = min((r.pyr >> 16) - y0, TS - 1);
-                {
-                    long long K[3], dA[3], dB[3];
-                    long long Y = (long long)(y0 + yl) * 256 + 128;
-#pragma unroll
-                    for (int i = 0; i < 3; i++) {
-                        K[i] = (long long)r.A[i] * X0 + (long long)r.B[i] * Y + r.C[i] + ((vtl >> i) & 1u);
-                        dA[i] = (long long)r.A[i] * 256;
-                        dB[i] = (long long)r.B[i] * 256;
+            // the tile's pixel rectangle inside the image, for the culling bounds
+            const int rx1 = min(x0 + TS, a.W) - 1 - x0, ry1 = min(y0 + TS, a.H) - 1 - y0;
+            for (u32 base = 0; base < n; base += BATCH) {
+                // phase 1a: thread = piece; conservative depth range over the tile; pieces covering
+                // the whole tile with every pixel test passing lower the tile's depth bound
+                const u32 q = base + tid;
+                int yl = 1, yh = 0;
+                u32 nrow = 0;
+                bool live = false;
+                float zlo = 0.f;
+                bool has_lo = false;
+                RRec r;   // the piece's record, held in registers across the barrier below
+                if (tid < BATCH && q < n) {
+                    const u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
+                    r = a.ws.rec[slot];
+                    int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
+                    int yl0 = max((r.pyr & 0xffff) - y0, 0), yh0 = min((r.pyr >> 16) - y0, TS - 1);
+                    live = r.id != INVALID && (r.view_tl >> 8) == view && xl <= xh && yl0 <= yh0;
+                    // a piece whose box holds few of the tile's pixels saves little when culled and
+                    // cannot cover the tile, so its range is not formed
+                    if (live && (xh - xl + 1) * (yh0 - yl0 + 1) >= CULL_MIN_BOX) {
+                        float hi;
+                        bool valid;
+                        has_lo = piece_depth_range(r.zp, spix[0][0], spix[0][rx1], spix[1][0], spix[1][ry1], zlo, hi, valid);
+                        if (has_lo && valid && piece_covers_rect(r, x0, x0 + rx1, y0, y0 + ry1)) atomicMin(&sbound, ord_f32(hi));
                     }
-                    for (int y = yl; y <= yh; y++) {
-                        int lo = xl, hi = xh;
-                        row_span(K, dA, lo, hi);
-#pragma unroll
-                        for (int i = 0; i < 3; i++) K[i] += dB[i];
-                        sspan[tid][y] = lo <= hi ? (unsigned short)(lo | (hi << 8)) : (unsigned short)SPAN_EMPTY;
-                        nrow += lo <= hi ? 1u : 0u;
-                    }
-#pragma unroll
-                    for (int k = 0; k < 9; k++) szp[k][tid] = r.zp[k];
-                    sid[tid] = id;
                 }
-            }
-            // phase 2: block prefix of the covered-row counts; every covered row becomes an item
-            // (piece, row, first column, length), in piece-then-row order (neighbouring rows of
-            // one piece have similar lengths, so a wave's items are of similar length)
-            const u32 incl = wave_incl_scan(nrow);
-            if ((tid & 63) == 63) sbucket[tid >> 6] = incl;
-            __syncthreads();
-            u32 pos0 = incl - nrow;
-            for (int w = 0; w < (tid >> 6); w++) pos0 += sbucket[w];
-            const u32 ni = sbucket[0] + sbucket[1] + sbucket[2] + sbucket[3];
-            // the items go through LDS ITEM_CAP at a time (one round unless the batch is dense)
-            for (u32 r0 = 0; r0 < ni; r0 += ITEM_CAP) {
-            u32 pos = pos0;
-            for (int y = yl; y <= yh; y++) {
-                u32 sp = sspan[tid][y];
-                int lo = (int)(sp & 0xffu), hi = (int)(sp >> 8);
-                if (lo > hi) continue;
-                if (pos >= r0 && pos < r0 + ITEM_CAP)
-                    sitem[pos - r0] = (u32)tid | ((u32)y << 8) | ((u32)lo << 16) | ((u32)(hi - lo + 1) << 24);
-                pos++;
-            }
-            __syncthreads();
-            // phase 3: thread = item
-            const u32 nr = min((u32)ITEM_CAP, ni - r0);
-            for (u32 i = tid; i < nr; i += 256) {
-                const u32 it = sitem[i];
-                const int j = (int)(it & 0xffu), y = (int)((it >> 8) & 0xffu), lo = (int)((it >> 16) & 0xffu), len = (int)(it >> 24);
-                const float pixy = spix[1][y];
-                const float z0 = szp[0][j], z2 = szp[2][j], w0 = szp[3][j], w2 = szp[5][j], d0 = szp[6][j], d2 = szp[8][j];
-                const float zy = szp[1][j] * pixy, wy = szp[4][j] * pixy, dy = szp[7][j] * pixy;
-                const u32 id = sid[j];
-                // branch-free body (the quotient is formed for every pixel, the minimum taken only
-                // where the tests pass); the pixel-centre x is recomputed (the same expression as
-                // spix) rather than read, so the loop holds no LDS read behind the LDS atomics
+                __syncthreads();
+                // phase 1b: pieces not behind the bound: exact row spans and depth planes into LDS;
+                // covered rows counted
+                if (live && !(has_lo && ord_f32(zlo) > sbound)) {
+                    u32 id = r.id, vtl = r.view_tl;
+                    int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
+                    yl = max((r.pyr & 0xffff) - y0, 0);
+                    yh = min((r.pyr >> 16) - y0, TS - 1);
+                    {
+                        long long K[3], dA[3], dB[3];
+                        long long Y = (long long)(y0 + yl) * 256 + 128;
+#pragma unroll
+                        for (int i = 0; i < 3; i++) {
+                            K[i] = (long long)r.A[i] * X0 + (long long)r.B[i] * Y + r.C[i] + ((vtl >> i) & 1u);
+                            dA[i] = (long long)r.A[i] * 256;
+                            dB[i] = (long long)r.B[i] * 256;
+                        }
+                        for (int y = yl; y <= yh; y++) {
+                            int lo = xl, hi = xh;
+                            row_span(K, dA, lo, hi);
+#pragma unroll
+                            for (int i = 0; i < 3; i++) K[i] += dB[i];
+                            sspan[tid][y] = lo <= hi ? (unsigned short)(lo | (hi << 8)) : (unsigned short)SPAN_EMPTY;
+                            nrow += lo <= hi ? 1u : 0u;
+                        }
+#pragma unroll
+                        for (int k = 0; k < 9; k++) szp[k][tid] = r.zp[k];
+                        sid[tid] = id;
+                    }
+                }
+                // phase 2: block prefix of the covered-row counts; every covered row becomes an item
+                // (piece, row, first column, length), in piece-then-row order (neighbouring rows of
+                // one piece have similar lengths, so a wave's items are of similar length)
+                const u32 incl = wave_incl_scan(nrow);
+                if ((tid & 63) == 63) sbucket[tid >> 6] = incl;
+                __syncthreads();
+                u32 pos0 = incl - nrow;
+                for (int w = 0; w < (tid >> 6); w++) pos0 += sbucket[w];
+                const u32 ni = sbucket[0] + sbucket[1] + sbucket[2] + sbucket[3];
+                // the items go through LDS ITEM_CAP at a time (one round unless the batch is dense)
+                for (u32 r0 = 0; r0 < ni; r0 += ITEM_CAP) {
+                u32 pos = pos0;
+                for (int y = yl; y <= yh; y++) {
+                    u32 sp = sspan[tid][y];
+                    int lo = (int)(sp & 0xffu), hi = (int)(sp >> 8);
+                    if (lo > hi) continue;
+                    if (pos >= r0 && pos < r0 + ITEM_CAP)
+                        sitem[pos - r0] = (u32)tid | ((u32)y << 8) | ((u32)lo << 16) | ((u32)(hi - lo + 1) << 24);
+                    pos++;
+                }
+                __syncthreads();
+                // phase 3: thread = item
+                const u32 nr = min((u32)ITEM_CAP, ni - r0);
+                for (u32 i = tid; i < nr; i += 256) {
+                    const u32 it = sitem[i];
+                    const int j = (int)(it & 0xffu), y = (int)((it >> 8) & 0xffu), lo = (int)((it >> 16) & 0xffu), len = (int)(it >> 24);
+                    const float pixy = spix[1][y];
+                    const float z0 = szp[0][j], z2 = szp[2][j], w0 = szp[3][j], w2 = szp[5][j], d0 = szp[6][j], d2 = szp[8][j];
+                    const float zy = szp[1][j] * pixy, wy = szp[4][j] * pixy, dy = szp[7][j] * pixy;
+                    const u32 id = sid[j];
+                    // branch-free body (the quotient is formed for every pixel, the minimum taken only
+                    // where the tests pass); the pixel-centre x is recomputed (the same expression as
+                    // spix) rather than read, so the loop holds no LDS read behind the LDS atomics
 #pragma unroll 2
-                for (int x = lo; x < lo + len; x++) {
-                    const float pixx = (float)((x0 + x + 0.5) * (double)a.xs - 1.0);
-                    const float Zp = z0 * pixx + zy + z2;
-                    const float Wp = w0 * pixx + wy + w2;
-                    const float Dp = d0 * pixx + dy + d2;
-                    const float z = Zp / Wp;
-                    const bool ok = ((Wp > 0.0f) & (Dp > 0.0f)) | ((Wp < 0.0f) & (Dp < 0.0f));
-                    if (ok & (z >= -1.0f) & (z <= 1.0f)) atomicMin(&key[y * TS + x], ((u64)ord_f32(z) << 32) | id);
+                    for (int x = lo; x < lo + len; x++) {
+                        const float pixx = (float)((x0 + x + 0.5) * (double)a.xs - 1.0);
+                        const float Zp = z0 * pixx + zy + z2;
+                        const float Wp = w0 * pixx + wy + w2;
+                        const float Dp = d0 * pixx + dy + d2;
+                        const float z = Zp / Wp;
+                        const bool ok = ((Wp > 0.0f) & (Dp > 0.0f)) | ((Wp < 0.0f) & (Dp < 0.0f));
+                        if (ok & (z >= -1.0f) & (z <= 1.0f)) atomicMin(&key[y * TS + x], ((u64)ord_f32(z) << 32) | id);
+                    }
+                }
+                __syncthreads();
                 }
             }
-            __syncthreads();
+            // the tile's winning keys: stored when this chunk is the tile's only one, else merged with
+            // a device-scope 64-bit minimum (the key buffer starts at all-ones)
+            const u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
+            const bool alone = scan_all || (end - begin) <= a.chunk;
+            for (int i = tid; i < NP; i += 256) {
+                const int px = x0 + i % TS, py = y0 + i / TS;
+                if (px >= a.W || py >= a.H) continue;
+                u64* dst = a.ws.keys + ((size_t)view * a.H + py) * a.W + px;
+                const u64 k = key[i];
+                if (alone) *dst = k;
+                else if (k != ~0ull) atomicMin(dst, k);
             }
+            __syncthreads();
         }
-        // the tile's winning keys: stored when this chunk is the tile's only one, else merged with
-        // a device-scope 64-bit minimum (the key buffer starts at all-ones)
-        const u32 begin = a.ws.offsets[tile], end = a.ws.offsets[tile + 1];
-        const bool alone = scan_all || (end - begin) <= a.chunk;
-        for (int i = tid; i < NP; i += 256) {
-            const int px = x0 + i % TS, py = y0 + i / TS;
-            if (px >= a.W || py >= a.H) continue;
-            u64* dst = a.ws.keys + ((size_t)view * a.H + py) * a.W + px;
-            const u64 k = key[i];
-            if (alone) *dst = k;
-            else if (k != ~0ull) atomicMin(dst, k);
-        }
-        __syncthreads();
     }
-}
+};
+template <int TS>
+__global__ void __launch_bounds__(256, TS == 32 ? 4 : 5) k_fine(RastArgs a) { KFineF<TS>{}(vb_hw(), a); }
 
 // One thread per pixel: winning key -> (u, v, z/w, id+1) and DB.
-__global__ void __launch_bounds__(256) k_resolve(RastArgs a) {
-    size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t P = (size_t)a.W * a.H * a.depth;
-    if (p >= P) return;
-    const int px = (int)(p % a.W), py = (int)((p / a.W) % a.H);
-    const u32 view = (u32)(p / ((size_t)a.W * a.H));
-    const float pixx = (float)((px + 0.5) * (double)a.xs - 1.0);
-    const float pixy = (float)((py + 0.5) * (double)a.ys - 1.0);
-    fine_resolve(a, view, px, py, pixx, pixy, a.ws.keys[p]);
-}
+struct KResolveF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, RastArgs a) const {
+        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
+        size_t P = (size_t)a.W * a.H * a.depth;
+        if (p >= P) return;
+        const int px = (int)(p % a.W), py = (int)((p / a.W) % a.H);
+        const u32 view = (u32)(p / ((size_t)a.W * a.H));
+        const float pixx = (float)((px + 0.5) * (double)a.xs - 1.0);
+        const float pixy = (float)((py + 0.5) * (double)a.ys - 1.0);
+        fine_resolve(a, view, px, py, pixx, pixy, a.ws.keys[p]);
+    }
+};
+__global__ void __launch_bounds__(256) k_resolve(RastArgs a) { KResolveF{}(vb_hw(), a); }
 
 // RasterizeBackwardKernel (rasterize.cu:84-141).  One 16x16 pixel tile per workgroup; the
 // nine (x, y, w) vertex terms of every covered pixel are summed per vertex in an LDS hash
 // table, then each touched vertex gets one set of global atomics per tile.
 constexpr int RB_SLOTS = 1024;
 
-__global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float xs, float ys, const float* proj, const u32* idx,
-                                                    const float* rast, const float* grast, float* gproj, int tilesX) {
-    __shared__ LdsTable<RB_SLOTS, 3> tab;
-    __shared__ u32 used[RB_SLOTS], wave_count[4];
-    lds_table_clear(tab);
-    __syncthreads();
-    int view = blockIdx.y;
-    int tx = blockIdx.x % tilesX, ty = blockIdx.x / tilesX;
-    int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
-    const float4* clip = (const float4*)proj + (size_t)view * V;
-    float* gp = gproj + (size_t)view * V * 4;
-    bool act = false;
-    int id = -1;
-    u32 vi[3] = {0u, 0u, 0u};
-    float gv[3][3];
-    if (px < W && py < H) {
-        size_t p = ((size_t)view * H + py) * W + px;
-        float4 r = ((const float4*)rast)[p];
-        id = (int)r.w - 1;
-        float2 g = id >= 0 ? ((const float2*)grast)[p] : make_float2(0.f, 0.f);
-        int bitcheck = __float_as_int(g.x) | __float_as_int(g.y);
-        if (id >= 0 && (bitcheck << 1) != 0) {
-            act = true;
-            float pixx = (float)((px + 0.5) * (double)xs - 1.0);
-            float pixy = (float)((py + 0.5) * (double)ys - 1.0);
-            vi[0] = idx[3 * id]; vi[1] = idx[3 * id + 1]; vi[2] = idx[3 * id + 2];
-            float4 c0 = clip[vi[0]], c1 = clip[vi[1]], c2 = clip[vi[2]];
-            float p0x = c0.w * pixx - c0.x, p0y = c0.w * pixy - c0.y;
-            float p1x = c1.w * pixx - c1.x, p1y = c1.w * pixy - c1.y;
-            float p2x = c2.w * pixx - c2.x, p2y = c2.w * pixy - c2.y;
-            float e0x = p0x - p2x, e0y = p0y - p2y, e1x = p1x - p2x, e1y = p1y - p2y;
-            float aa = e0x * e1y - e0y * e1x;
-            float ia = 1.f / (aa + (aa > 0 ? 1e-6f : -1e-6f));
-            float kuv = r.x * g.x + r.y * g.y;
-            gv[0][0] = (g.y * p2y + kuv * e1y) * ia;
-            gv[0][1] = (-g.y * p2x - kuv * e1x) * ia;
-            gv[0][2] = -gv[0][0] * pixx - gv[0][1] * pixy;
-            gv[1][0] = (-g.x * p2y - kuv * e0y) * ia;
-            gv[1][1] = (g.x * p2x + kuv * e0x) * ia;
-            gv[1][2] = -gv[1][0] * pixx - gv[1][1] * pixy;
-            gv[2][0] = ((-g.y * p0y + g.x * p1y) - kuv * (-p0y + p1y)) * ia;
-            gv[2][1] = ((g.y * p0x - g.x * p1x) - kuv * (p0x - p1x)) * ia;
-            gv[2][2] = -gv[2][0] * pixx - gv[2][1] * pixy;
-        }
-    }
-    // Neighbouring pixels mostly share a triangle and so the same three vertex slots, whose LDS
-    // atomics would serialise: the grads of a triangle covering RB_GROUP or more of the wave's
-    // lanes are summed over the wave first and kept by its first lane, which alone adds them to
-    // the table; lanes of smaller groups add their own.
-    constexpr int RB_GROUP = 4;
-    const int lane = threadIdx.x & 63;
-    bool add = false;
-    u64 pending = __ballot(act);
-    while (pending) {
-        int leader = __ffsll((unsigned long long)pending) - 1;
-        int lid = __builtin_amdgcn_readlane(id, leader);
-        bool mine = act && id == lid;
-        u64 m = __ballot(mine);
-        pending &= ~m;
-        if (__popcll(m) >= RB_GROUP) {
-#pragma unroll
-            for (int k = 0; k < 3; k++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    float t = wave_sum_dpp(mine ? gv[k][j] : 0.f);
-                    if (lane == leader) gv[k][j] = t;
-                }
-            add |= lane == leader;
-        } else {
-            add |= mine;
-        }
-    }
-    if (add) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            int sl = lds_table_slot(tab, vi[k]);
-            if (sl >= 0) {
-                atomicAdd(&tab.val[0][sl], gv[k][0]);
-                atomicAdd(&tab.val[1][sl], gv[k][1]);
-                atomicAdd(&tab.val[2][sl], gv[k][2]);
-            } else {
-                float* q = gp + 4 * (size_t)vi[k];
-                atomicAdd(q, gv[k][0]); atomicAdd(q + 1, gv[k][1]); atomicAdd(q + 3, gv[k][2]);
+struct KRasterBwdF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, int W, int H, int V, float xs, float ys, const float* proj, const u32* idx,
+                                                    const float* rast, const float* grast, float* gproj, int tilesX) const {
+        __shared__ LdsTable<RB_SLOTS, 3> tab;
+        __shared__ u32 used[RB_SLOTS], wave_count[4];
+        lds_table_clear(tab);
+        __syncthreads();
+        int view = vb.b.y;
+        int tx = vb.b.x % tilesX, ty = vb.b.x / tilesX;
+        int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
+        const float4* clip = (const float4*)proj + (size_t)view * V;
+        float* gp = gproj + (size_t)view * V * 4;
+        bool act = false;
+        int id = -1;
+        u32 vi[3] = {0u, 0u, 0u};
+        float gv[3][3];
+        if (px < W && py < H) {
+            size_t p = ((size_t)view * H + py) * W + px;
+            float4 r = ((const float4*)rast)[p];
+            id = (int)r.w - 1;
+            float2 g = id >= 0 ? ((const float2*)grast)[p] : make_float2(0.f, 0.f);
+            int bitcheck = __float_as_int(g.x) | __float_as_int(g.y);
+            if (id >= 0 && (bitcheck << 1) != 0) {
+                act = true;
+                float pixx = (float)((px + 0.5) * (double)xs - 1.0);
+                float pixy = (float)((py + 0.5) * (double)ys - 1.0);
+                vi[0] = idx[3 * id]; vi[1] = idx[3 * id + 1]; vi[2] = idx[3 * id + 2];
+                float4 c0 = clip[vi[0]], c1 = clip[vi[1]], c2 = clip[vi[2]];
+                float p0x = c0.w * pixx - c0.x, p0y = c0.w * pixy - c0.y;
+                float p1x = c1.w * pixx - c1.x, p1y = c1.w * pixy - c1.y;
+                float p2x = c2.w * pixx - c2.x, p2y = c2.w * pixy - c2.y;
+                float e0x = p0x - p2x, e0y = p0y - p2y, e1x = p1x - p2x, e1y = p1y - p2y;
+                float aa = e0x * e1y - e0y * e1x;
+                float ia = 1.f / (aa + (aa > 0 ? 1e-6f : -1e-6f));
+                float kuv = r.x * g.x + r.y * g.y;
+                gv[0][0] = (g.y * p2y + kuv * e1y) * ia;
+                gv[0][1] = (-g.y * p2x - kuv * e1x) * ia;
+                gv[0][2] = -gv[0][0] * pixx - gv[0][1] * pixy;
+                gv[1][0] = (-g.x * p2y - kuv * e0y) * ia;
+                gv[1][1] = (g.x * p2x + kuv * e0x) * ia;
+                gv[1][2] = -gv[1][0] * pixx - gv[1][1] * pixy;
+                gv[2][0] = ((-g.y * p0y + g.x * p1y) - kuv * (-p0y + p1y)) * ia;
+                gv[2][1] = ((g.y * p0x - g.x * p1x) - kuv * (p0x - p1x)) * ia;
+                gv[2][2] = -gv[2][0] * pixx - gv[2][1] * pixy;
             }
         }
+        // Neighbouring pixels mostly share a triangle and so the same three vertex slots, whose LDS
+        // atomics would serialise: the grads of a triangle covering RB_GROUP or more of the wave's
+        // lanes are summed over the wave first and kept by its first lane, which alone adds them to
+        // the table; lanes of smaller groups add their own.
+        constexpr int RB_GROUP = 4;
+        const int lane = threadIdx.x & 63;
+        bool add = false;
+        u64 pending = __ballot(act);
+        while (pending) {
+            int leader = __ffsll((unsigned long long)pending) - 1;
+            int lid = __builtin_amdgcn_readlane(id, leader);
+            bool mine = act && id == lid;
+            u64 m = __ballot(mine);
+            pending &= ~m;
+            if (__popcll(m) >= RB_GROUP) {
+#pragma unroll
+                for (int k = 0; k < 3; k++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) {
+                        float t = wave_sum_dpp(mine ? gv[k][j] : 0.f);
+                        if (lane == leader) gv[k][j] = t;
+                    }
+                add |= lane == leader;
+            } else {
+                add |= mine;
+            }
+        }
+        if (add) {
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                int sl = lds_table_slot(tab, vi[k]);
+                if (sl >= 0) {
+                    atomicAdd(&tab.val[0][sl], gv[k][0]);
+                    atomicAdd(&tab.val[1][sl], gv[k][1]);
+                    atomicAdd(&tab.val[2][sl], gv[k][2]);
+                } else {
+                    float* q = gp + 4 * (size_t)vi[k];
+                    atomicAdd(q, gv[k][0]); atomicAdd(q + 1, gv[k][1]); atomicAdd(q + 3, gv[k][2]);
+                }
+            }
+        }
+        __syncthreads();
+        lds_vertex_grad_flush(tab, used, wave_count, gp);
     }
-    __syncthreads();
-    lds_vertex_grad_flush(tab, used, wave_count, gp);
-}
+};
+__global__ void __launch_bounds__(256) k_raster_bwd(int W, int H, int V, float xs, float ys, const float* proj, const u32* idx,
+                                                    const float* rast, const float* grast, float* gproj, int tilesX) { KRasterBwdF{}(vb_hw(), W, H, V, xs, ys, proj, idx, rast, grast, gproj, tilesX); }
 
 }  // namespace
 
@@ -1006,31 +1045,31 @@ int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_st
     const size_t P = (size_t)a.W * a.H * a.depth;
     if (!cleared) {
         const WsClear c = ws_clear_of(a.ws, L, P);
-        hipLaunchKernelGGL(k_clear, dim3((unsigned)std::min<size_t>(cdiv(c.n(), 256), 4096)), dim3(256), 0, s, c);
+        vlaunch<KClearF>(k_clear, dim3((unsigned)std::min<size_t>(cdiv(c.n(), 256), 4096)), dim3(256), 0, s, c);
         VCL_LAUNCH_CHECK();
     }
     if (L.nrec_direct > 0) {
-        hipLaunchKernelGGL(k_setup, dim3(cdiv(L.nrec_direct, 256)), dim3(256), 0, s, a);
+        vlaunch<KSetupF>(k_setup, dim3(cdiv(L.nrec_direct, 256)), dim3(256), 0, s, a);
         VCL_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_bin_large<false>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
+    vlaunch<KBinLargeF<false>>(k_bin_large<false>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_fold_large, dim3(cdiv(L.NT, 256)), dim3(256), 0, s, a.ws, L.NT);
+    vlaunch<KFoldLargeF>(k_fold_large, dim3(cdiv(L.NT, 256)), dim3(256), 0, s, a.ws, L.NT);
     VCL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a, L.NT);
+    vlaunch<KScanF>(k_scan, dim3(1), dim3(1024), 0, s, a, L.NT);
     VCL_LAUNCH_CHECK();
     if (L.nrec_cap > 0) {
-        hipLaunchKernelGGL(k_fill, dim3(cdiv(L.nrec_cap, 256)), dim3(256), 0, s, a, L.nrec_cap);
+        vlaunch<KFillF>(k_fill, dim3(cdiv(L.nrec_cap, 256)), dim3(256), 0, s, a, L.nrec_cap);
         VCL_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_bin_large<true>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
+    vlaunch<KBinLargeF<true>>(k_bin_large<true>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
     const dim3 fg((unsigned)std::min(L.chunk_cap, (size_t)FINE_MAX_GRID));
-    if (tshift == 3) hipLaunchKernelGGL(k_fine<8>, fg, dim3(256), 0, s, a);
-    else if (tshift == 5) hipLaunchKernelGGL(k_fine<32>, fg, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_fine<16>, fg, dim3(256), 0, s, a);
+    if (tshift == 3) vlaunch<KFineF<8>>(k_fine<8>, fg, dim3(256), 0, s, a);
+    else if (tshift == 5) vlaunch<KFineF<32>>(k_fine<32>, fg, dim3(256), 0, s, a);
+    else vlaunch<KFineF<16>>(k_fine<16>, fg, dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_resolve, dim3(cdiv(P, 256)), dim3(256), 0, s, a);
+    vlaunch<KResolveF>(k_resolve, dim3(cdiv(P, 256)), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
@@ -1063,7 +1102,7 @@ extern "C" int vcl_rasterize_backward(const vcl_rasterize_params* p, const vcl_r
     VCL_REQUIRE(g->out);
     if ((size_t)p->width * p->height * p->depth == 0) return VCL_OK;
     int tilesX = (p->width + 15) / 16, tilesY = (p->height + 15) / 16;
-    hipLaunchKernelGGL(k_raster_bwd, dim3(tilesX * tilesY, p->depth), dim3(256), 0, as_hip(stream), p->width, p->height, p->projNum,
+    vlaunch<KRasterBwdF>(k_raster_bwd, dim3(tilesX * tilesY, p->depth), dim3(256), 0, as_hip(stream), p->width, p->height, p->projNum,
                        2.f / (float)p->width, 2.f / (float)p->height, p->proj, p->idx, p->out, g->out, g->proj, tilesX);
     VCL_LAUNCH_CHECK();
     return VCL_OK;

@@ -4,7 +4,10 @@
 // (vcl_server_progress) retires completed step batches, finishes requests that ran all their
 // iterations, admits arrived requests while a slot is free and the request's arena fits the
 // HBM budget, and queues the next steps of active requests, each on its own stream, so the
-// steps of different requests overlap on the device.  Policy FIFO admits in arrival order and
+// steps of different requests overlap on the device.  In grouped mode the requests due to step
+// in a round are split by step signature and each part steps as one vcl_group (shared launches
+// over a device-side work table) on a stream of its own; a group is kept while its requests
+// keep stepping together and rebuilt when they change.  Policy FIFO admits in arrival order and
 // steps every active request; SRPT admits the arrived request with the least work first and
 // steps only the few active requests with the least remaining work.  Every
 // request's state (mesh, mip texture, lobes, Adam moments, intermediates) stays resident in one
@@ -84,10 +87,23 @@ struct Req {
     hipStream_t stream = nullptr;
     hipEvent_t ev = nullptr;
     int launched = 0, done = 0;
+    uint64_t sig = 0;    // step signature (grouped mode)
     double cost = 0;     // work per iteration (pixels + 8 per triangle, over all views)
     float loss = 0.f;
     int status = VCL_OK;
     bool reported = false;
+};
+
+// A built group: its requests (by ticket, ascending), work tables from the pool, stream and the
+// event after its last queued steps.
+struct GroupSlot {
+    std::vector<uint64_t> key;
+    vcl_group* g = nullptr;
+    void* table = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    int uses = 0;
+    bool mark = false;
 };
 
 }  // namespace
@@ -102,6 +118,7 @@ struct vcl_server {
     std::vector<Req*> finished;             // all iterations done (arena held until release)
     std::vector<hipStream_t> streams;       // idle streams for reuse
     std::map<uint64_t, vcl_tuning> tunings; // autotuned schedule per size class
+    std::vector<GroupSlot*> groups;         // grouped mode: built groups
     double occ_integral = 0, occ_last_t = 0;
 
     double now() const { return std::chrono::duration<double>(clk::now() - t0).count(); }
@@ -138,8 +155,19 @@ extern "C" int vcl_server_create(const vcl_server_config* c, vcl_server** out) {
     return VCL_OK;
 }
 
+static void drop_group(vcl_server* S, GroupSlot* G) {
+    if (G->stream) (void)hipStreamSynchronize(G->stream);
+    if (G->g) vcl_group_destroy(G->g);
+    if (G->ev) (void)hipEventDestroy(G->ev);
+    if (G->stream) S->streams.push_back(G->stream);
+    if (G->table) S->pool.put(G->table);
+    delete G;
+}
+
 extern "C" int vcl_server_destroy(vcl_server* S) {
     if (!S) return VCL_OK;
+    for (GroupSlot* G : S->groups) drop_group(S, G);
+    S->groups.clear();
     for (Req* r : S->active) if (r->stream) (void)hipStreamSynchronize(r->stream);
     for (auto* v : {&S->active, &S->finished}) {
         for (Req* r : *v) { release_req(S, r); delete r; }
@@ -188,6 +216,80 @@ static int launch_steps(vcl_server* S, Req* r) {
     }
     r->launched += k;
     VCL_HIP_CHECK(hipEventRecord(r->ev, r->stream));
+    return VCL_OK;
+}
+
+// Grouped mode: steps the due requests (idle, with iterations left) as one group per step
+// signature.  A group whose requests are exactly one part is reused; groups not reused this
+// round are dropped once their queued work has completed.
+static int launch_grouped(vcl_server* S, const std::vector<Req*>& due) {
+    std::map<uint64_t, std::vector<Req*>> parts;
+    for (Req* r : due) parts[r->sig].push_back(r);
+    for (GroupSlot* G : S->groups) G->mark = false;
+    std::vector<std::pair<GroupSlot*, std::vector<Req*>*>> plan;
+    for (auto& kv : parts) {
+        std::vector<Req*>& m = kv.second;
+        std::sort(m.begin(), m.end(), [](const Req* a, const Req* b) { return a->ticket < b->ticket; });
+        std::vector<uint64_t> key;
+        for (Req* r : m) key.push_back(r->ticket);
+        GroupSlot* found = nullptr;
+        for (GroupSlot* G : S->groups)
+            if (!G->mark && G->key == key) { found = G; break; }
+        if (found) found->mark = true;
+        plan.push_back({found, &m});
+    }
+    // drop the groups that were not reused and have nothing in flight
+    for (size_t i = 0; i < S->groups.size();) {
+        GroupSlot* G = S->groups[i];
+        if (!G->mark && (!G->ev || hipEventQuery(G->ev) == hipSuccess)) {
+            drop_group(S, G);
+            S->groups.erase(S->groups.begin() + i);
+        } else {
+            i++;
+        }
+    }
+    for (auto& pm : plan) {
+        std::vector<Req*>& m = *pm.second;
+        GroupSlot* G = pm.first;
+        if (!G) {
+            std::vector<vcl_fit*> fits;
+            for (Req* r : m) fits.push_back(r->fit);
+            size_t bytes = vcl_group_table_bytes(fits.data(), (int)fits.size());
+            if (!bytes) return VCL_ERR_INVALID;
+            G = new GroupSlot();
+            for (Req* r : m) G->key.push_back(r->ticket);
+            occupancy_sample(S);
+            int g = S->pool.get(bytes, &G->table);
+            if (g != 0) {
+                delete G;
+                if (g < 0) return VCL_ERR_HIP;
+                // the tables do not fit the budget now: these requests step on their own
+                for (Req* r : m) { int st = launch_steps(S, r); if (st) return st; }
+                continue;
+            }
+            if (!S->streams.empty()) { G->stream = S->streams.back(); S->streams.pop_back(); }
+            else if (hipStreamCreateWithFlags(&G->stream, hipStreamNonBlocking) != hipSuccess) { drop_group(S, G); return VCL_ERR_HIP; }
+            if (hipEventCreateWithFlags(&G->ev, hipEventDisableTiming) != hipSuccess) { drop_group(S, G); return VCL_ERR_HIP; }
+            int st = vcl_group_create_in(fits.data(), (int)fits.size(), G->table, bytes, (vcl_stream_t)G->stream, &G->g);
+            if (st) { drop_group(S, G); return st; }
+            G->mark = true;
+            S->groups.push_back(G);
+        }
+        int k = S->cfg.steps_per_round;
+        for (Req* r : m) k = std::min(k, r->iterations - r->launched);
+        for (int i = 0; i < k; i++) {
+            // eager on a group's first use (a one-off group is not worth a capture), then replayed
+            int st = S->cfg.use_graph && G->uses > 0 ? vcl_group_step_graph(G->g, (vcl_stream_t)G->stream)
+                                                     : vcl_group_step(G->g, (vcl_stream_t)G->stream);
+            if (st) return st;
+        }
+        G->uses++;
+        VCL_HIP_CHECK(hipEventRecord(G->ev, G->stream));
+        for (Req* r : m) {
+            r->launched += k;
+            VCL_HIP_CHECK(hipEventRecord(r->ev, G->stream));
+        }
+    }
     return VCL_OK;
 }
 
@@ -262,12 +364,29 @@ extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
             }
             if (st) return st;
         }
+        if (S->cfg.grouped && (st = vcl_fit_step_signature(r->fit, &r->sig))) return st;
         r->admit = S->now();
         S->active.push_back(r);
     }
     // 3. queue the next steps: every idle active request (FIFO), or under VCL_SCHED_SRPT only the
     //    max_stepping requests with the least remaining work (the others stay resident, paused)
-    if (srpt) {
+    if (S->cfg.grouped) {
+        std::vector<Req*> due;
+        if (srpt) {
+            std::vector<Req*> order(S->active);
+            std::stable_sort(order.begin(), order.end(), [](const Req* a, const Req* b) {
+                return a->cost * (a->iterations - a->done) < b->cost * (b->iterations - b->done);
+            });
+            int lim = S->cfg.max_stepping > 0 ? S->cfg.max_stepping : (int)order.size();
+            for (int i = 0; i < (int)order.size() && i < lim; i++)
+                if (order[i]->launched == order[i]->done && order[i]->launched < order[i]->iterations) due.push_back(order[i]);
+        } else {
+            for (Req* r : S->active)
+                if (r->launched == r->done && r->launched < r->iterations) due.push_back(r);
+        }
+        int st = launch_grouped(S, due);
+        if (st) return st;
+    } else if (srpt) {
         std::vector<Req*> order(S->active);
         std::stable_sort(order.begin(), order.end(), [](const Req* a, const Req* b) {
             return a->cost * (a->iterations - a->done) < b->cost * (b->iterations - b->done);

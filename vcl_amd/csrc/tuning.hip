@@ -3,6 +3,8 @@
 
 vcl_tuning g_vcl_tuning = {16, 256, 16};
 
+thread_local VRecorder* g_vcl_recorder = nullptr;
+
 static bool tuning_ok(const vcl_tuning& t) {
     bool tile = t.raster_tile == 8 || t.raster_tile == 16 || t.raster_tile == 32;
     bool chunk = t.bin_chunk == 64 || t.bin_chunk == 128 || t.bin_chunk == 256;

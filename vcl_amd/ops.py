@@ -585,3 +585,40 @@ class Fit:
 
     def device_bytes(self):
         return int(_lib().vcl_fit_device_bytes(self.h))
+
+
+def step_signature(fit):
+    """Hash of the kernels and launch shapes of one step of `fit` (equal: they can be grouped)."""
+    s = C.c_uint64()
+    L.check(_lib().vcl_fit_step_signature(fit.h, C.byref(s)), "fit_step_signature")
+    return s.value
+
+
+class Group:
+    """One step of several Fit requests in shared launches (vcl_group_* of include/vcl.h)."""
+
+    def __init__(self, fits):
+        self.fits = list(fits)
+        arr = (C.c_void_p * len(self.fits))(*[f.h.value for f in self.fits])
+        h_ = C.c_void_p()
+        L.check(_lib().vcl_group_create(arr, len(self.fits), _s(), C.byref(h_)), "group_create")
+        self.h = h_
+
+    def step(self, graph=False):
+        fn = _lib().vcl_group_step_graph if graph else _lib().vcl_group_step
+        L.check(fn(self.h, _s()), "group_step")
+
+    def launches(self):
+        return int(_lib().vcl_group_launches(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            torch.cuda.synchronize()
+            _lib().vcl_group_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
