@@ -121,14 +121,14 @@ def _served_specs(SV, n, rate, seed):
 
 
 def _serve_once(SV, built, rate_scale, max_active, use_graph, steps_per_round, budget, policy=0, max_stepping=0, tunings=None,
-                grouped=False):
+                grouped=False, group_max_pixels=0):
     """Run one arrival stream through a server; arrival times scaled by rate_scale (0: all at
     t=0).  Returns latency / throughput / occupancy figures.  tunings: None (no autotuning) or a
     list of (size-class key, tuning) that pre-fills the server's table and receives it back
     (classes missing from it are autotuned on admission)."""
     srv = SV.Server(hbm_budget=budget, max_active=max_active, use_graph=use_graph, steps_per_round=steps_per_round,
                     auto_release=True, policy=policy, max_stepping=max_stepping, autotune_steps=3 if tunings is not None else 0,
-                    grouped=grouped)
+                    grouped=grouped, group_max_pixels=group_max_pixels)
     if tunings:
         srv.load_tuning_table(tunings)
     for sp, sc, tgt in built:

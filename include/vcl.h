@@ -765,8 +765,11 @@ typedef struct vcl_server_config {
                              many timed steps per candidate, and reuse its tuning for the class;
                              0: no tuning */
     int grouped;          /* 1: the requests stepping in a round run in shared launches
-                             (vcl_group), one group per step signature, each on its own
-                             stream; 0: each request steps on its own stream */
+                             (vcl_group), one group per step signature and pixel-count bucket
+                             (4x), each on its own stream; 0: each request steps on its own
+                             stream */
+    int group_max_pixels; /* grouped mode: requests with more pixels (over all views) step on
+                             their own stream (0: no limit) */
 } vcl_server_config;
 
 typedef struct vcl_request_stats {

@@ -172,7 +172,8 @@ class Tuning(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", csz), ("max_active", ci), ("use_graph", ci), ("steps_per_round", ci), ("auto_release", ci),
-                ("policy", ci), ("max_stepping", ci), ("autotune_steps", ci), ("grouped", ci)]
+                ("policy", ci), ("max_stepping", ci), ("autotune_steps", ci), ("grouped", ci),
+                ("group_max_pixels", ci)]
 
 
 class RequestStats(C.Structure):

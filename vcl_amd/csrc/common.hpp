@@ -220,3 +220,17 @@ int vcl_fill_async(void* p, size_t bytes, hipStream_t s);
 int vcl_fill_u32_async(u32* p, size_t n, u32 v, hipStream_t s);
 int vcl_fill_multi_async(void* const* p, const size_t* bytes, int n, hipStream_t s);
 int vcl_fill_multi_bump_async(void* const* p, const size_t* bytes, int n, double* bt, double rhom, double rhov, hipStream_t s);
+
+// The dispatcher hands workgroup i of a launch to XCD i % 8, and each XCD has its own L2.
+// xcd_swizzle(i, n, c) maps hardware block i of n to a logical block such that XCD x serves
+// logical chunks x, x + 8, x + 16, ... of c consecutive blocks: with c = tiles per image row,
+// each tile row sits in one L2, so the halo column a tile reads from its right neighbour (a
+// full cache line for one pixel) is already there, while consecutive tile rows still spread over
+// all XCDs (covered rows cost far more than background ones).  Needs n % (8 c) == 0, else the
+// identity.  Any bijection gives the same results.
+constexpr unsigned VCL_XCDS = 8;
+__host__ __device__ __forceinline__ unsigned xcd_swizzle(unsigned i, unsigned n, unsigned c) {
+    if (c == 0 || n % (VCL_XCDS * c)) return i;
+    const unsigned x = i % VCL_XCDS, k = i / VCL_XCDS;
+    return ((k / c) * VCL_XCDS + x) * c + k % c;
+}

@@ -297,10 +297,12 @@ static void fill_texture(vcl_fit* F, const char* name, vcl_texture& t) {
 }
 
 // Forward chain up to the antialiased image.
-// step: also the step's start work, folded into the forward's first passes: the clip-space and
-// texture-output grad clears of the forward(XGradParams&) overloads, and the Adam
-// bias-correction bump (Adam::step bumps rhomt/rhovt before its launch, optimize.cu:109-115),
-// which the updates at the end of the step read.
+// step: also the step's start work, folded into the forward's first passes: the clip-space grad
+// clear of the forward(XGradParams&) overloads, and the Adam bias-correction bump (Adam::step
+// bumps rhomt/rhovt before its launch, optimize.cu:109-115), which the updates at the end of
+// the step read.  The texture-output grad needs no clear in the fused step: the SG backward
+// writes it for every covered pixel, the only pixels the texture backward reads (the op-by-op
+// sequence clears it, as the reference does).
 static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     const vcl_fit_desc& d = F->d;
     int W = d.width, H = d.height, nv = d.views, C = d.channel;
@@ -346,7 +348,7 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     if (C == 4 && !d.op_by_op) {
         // uv interpolation and texture sample fused per pixel
         ProfScope ps_(F, "interp_texture", as_hip(s));
-        st = vcl_interp_tex_dev(&ip, &tp, step ? F->f("gdiff") : nullptr, s);
+        st = vcl_interp_tex_dev(&ip, &tp, nullptr, s);
     } else {
         { ProfScope ps_(F, "interpolate", as_hip(s)); st = vcl_interpolate_forward(&ip, s); }
         if (st) return st;
@@ -420,7 +422,10 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     gg.out = F->f("gcolor"); gg.workspace = F->buf["ws_sg"]; gg.workspace_size = F->ws_sg;
     // the lobe grads' final sums run in the Adam launch at the end of the step
     vcl_sg_lobe_sums lobes = {nullptr, 0, 0, 0};
-    { ProfScope ps_(F, "spherical_gaussian_bwd", as_hip(s)); st = vcl_spherical_gaussian_backward_dev(&gp, &gg, &lobes, s); }
+    // the fused forward leaves the texture-output grad uncleared: the SG backward writes it
+    // (the op-by-op forward clears it, as the reference's forward(GradParams&) does)
+    { ProfScope ps_(F, "spherical_gaussian_bwd", as_hip(s));
+      st = vcl_spherical_gaussian_backward_dev(&gp, &gg, &lobes, (C == 4 && !d.op_by_op) ? 1 : 0, s); }
     if (st) return st;
     vcl_texturemap_params tp;
     tp.width = W; tp.height = H; tp.depth = nv; tp.texwidth = d.texwidth; tp.texheight = d.texheight; tp.channel = C; tp.miplevel = d.miplevel;

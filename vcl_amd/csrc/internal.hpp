@@ -54,8 +54,10 @@ struct vcl_sg_lobe_sums {
 };
 // vcl_spherical_gaussian_backward that, when lobes is not null and all three lobe grads are
 // requested, leaves the lobe sums to the caller (lobes->seg set) instead of adding them.
+// gd_fresh: g->diffuse holds nothing yet (the pass writes it as if it had been cleared, so the
+// caller needs no clear of its own).
 int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
-                                        vcl_sg_lobe_sums* lobes, vcl_stream_t stream);
+                                        vcl_sg_lobe_sums* lobes, int gd_fresh, vcl_stream_t stream);
 // loss (may be null): the launch also sums the loss partials that vcl_mse_loss_grad_dev left in
 // loss->buffer into *loss->loss (one more workgroup; n <= 8 tensors).  lobes (may be null, or
 // with seg null): tensors 1..3 are the lobe axis, sharpness and amplitude, whose grads first

@@ -370,39 +370,125 @@ struct KTexFwd4F {
 };
 __global__ void __launch_bounds__(BS) k_tex_fwd4(size_t P, TexArgs t, const float* rast, const float* uv, const float* uvDA, float* out) { KTexFwd4F{}(vb_hw(), P, t, rast, uv, uvDA, out); }
 
+// Texel addresses of one C == 4 sample (tex4_px split so that the loads of several pixels can be
+// issued before any is used): level l0 at i0 with weights fx, fy; when two, also level l1 at i1
+// (gx, gy) blended by f.
+struct Tex4Taps {
+    const float4* T0;
+    const float4* T1;
+    int4 i0, i1;
+    float fx, fy, gx, gy, f;
+    bool two;
+};
+
+__device__ __forceinline__ Tex4Taps tex4_taps(const TexArgs& t, float2 u, float4 da) {
+    Tex4Taps q;
+    q.two = false;
+    q.f = 0.f;
+    q.gx = q.gy = 0.f;
+    if (t.L > 1) {
+        int l0, l1; float lev;
+        calc_level(da, t.tw, t.th, t.L, l0, l1, q.f, lev, nullptr, 0);
+        index_fetch(t.tw, t.th, l0, u, q.i0, q.fx, q.fy);
+        q.T0 = (const float4*)t.tex[l0];
+        q.two = q.f > 0;
+        index_fetch(t.tw, t.th, l1, u, q.i1, q.gx, q.gy);
+        q.T1 = (const float4*)t.tex[l1];
+    } else {
+        index_fetch(t.tw, t.th, 0, u, q.i0, q.fx, q.fy);
+        q.T0 = (const float4*)t.tex[0];
+        q.T1 = q.T0;
+        q.i1 = q.i0;
+    }
+    return q;
+}
+
 // The fit step's uv interpolation (2 attributes, with derivatives) and C == 4 texture forward in
-// one pass per pixel: the same per-pixel code as the two ops, writing uv and its derivatives for
-// the backward and the texture sample.
-// zero (optional): a buffer shaped like out (4 floats per pixel) cleared in the same pass (the
-// step's texture-output grad clear).
+// one pass: the same per-pixel arithmetic as the two ops (interp_attr_px, tex4_px), writing uv and
+// its derivatives for the backward and the texture sample.  Each thread takes NPX pixels one
+// workgroup-width apart and goes through the dependent loads in stages (rast; triangle indices
+// and DB; uv attributes; texels), each stage's loads for all its pixels issued together, so a
+// wave keeps NPX chains in flight instead of one.
+// zero (optional): a buffer shaped like out (4 floats per pixel) cleared in the same pass.
+template <int NPX>
 struct KInterpTex4F {
     static constexpr int kBS = BS, kMINW = 1;
     __device__ __forceinline__ void operator()(const VB& vb, size_t P, const float* rast, const float* db, const float* uvattr, const u32* idx, float* uvp,
-                                                    float* uvDA, TexArgs t, float* out, float* zero) const {
-        size_t p = (size_t)vb.b.x * blockDim.x + threadIdx.x;
-        if (p >= P) return;
-        const float4 r = ((const float4*)rast)[p];
-        const int id = (int)r.w - 1;
-        float2 u = make_float2(0.f, 0.f);
-        float4 da = make_float4(0.f, 0.f, 0.f, 0.f), o = da;
-        if (id >= 0) {
-            const u32 i0 = idx[3 * id], i1 = idx[3 * id + 1], i2 = idx[3 * id + 2];
-            const float4 b = ((const float4*)db)[p];
-            const double w2 = 1.0 - (double)r.x - (double)r.y;
-            float2 dau, dav;
-            interp_attr_px(r, b, w2, uvattr[2 * (size_t)i0], uvattr[2 * (size_t)i1], uvattr[2 * (size_t)i2], u.x, dau);
-            interp_attr_px(r, b, w2, uvattr[2 * (size_t)i0 + 1], uvattr[2 * (size_t)i1 + 1], uvattr[2 * (size_t)i2 + 1], u.y, dav);
-            da = make_float4(dau.x, dau.y, dav.x, dav.y);
+                                               float* uvDA, TexArgs t, float* out, float* zero) const {
+        const size_t p0 = (size_t)vb.b.x * (BS * NPX) + threadIdx.x;
+        float4 r[NPX];
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const size_t p = p0 + (size_t)j * BS;
+            r[j] = p < P ? ((const float4*)rast)[p] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        ((float2*)uvp)[p] = u;
-        ((float4*)uvDA)[p] = da;
-        if (r.w >= 1.f) o = tex4_px(t, u, da);
-        ((float4*)out)[p] = o;
-        if (zero) ((float4*)zero)[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        u32 i0[NPX], i1[NPX], i2[NPX];
+        float4 b[NPX];
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const int id = (int)r[j].w - 1;
+            i0[j] = i1[j] = i2[j] = 0;
+            b[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (id >= 0) {
+                i0[j] = idx[3 * id]; i1[j] = idx[3 * id + 1]; i2[j] = idx[3 * id + 2];
+                b[j] = ((const float4*)db)[p0 + (size_t)j * BS];
+            }
+        }
+        float2 a0[NPX], a1[NPX], a2[NPX];
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            a0[j] = a1[j] = a2[j] = make_float2(0.f, 0.f);
+            if ((int)r[j].w - 1 >= 0) {
+                a0[j] = ((const float2*)uvattr)[i0[j]];
+                a1[j] = ((const float2*)uvattr)[i1[j]];
+                a2[j] = ((const float2*)uvattr)[i2[j]];
+            }
+        }
+        Tex4Taps q[NPX];
+        float4 tx[NPX][8];
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const size_t p = p0 + (size_t)j * BS;
+            float2 u = make_float2(0.f, 0.f);
+            float4 da = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((int)r[j].w - 1 >= 0) {
+                const double w2 = 1.0 - (double)r[j].x - (double)r[j].y;
+                float2 dau, dav;
+                interp_attr_px(r[j], b[j], w2, a0[j].x, a1[j].x, a2[j].x, u.x, dau);
+                interp_attr_px(r[j], b[j], w2, a0[j].y, a1[j].y, a2[j].y, u.y, dav);
+                da = make_float4(dau.x, dau.y, dav.x, dav.y);
+            }
+            if (p < P) {
+                ((float2*)uvp)[p] = u;
+                ((float4*)uvDA)[p] = da;
+            }
+            if (r[j].w >= 1.f) {
+                q[j] = tex4_taps(t, u, da);
+                tx[j][0] = q[j].T0[q[j].i0.x]; tx[j][1] = q[j].T0[q[j].i0.y]; tx[j][2] = q[j].T0[q[j].i0.z]; tx[j][3] = q[j].T0[q[j].i0.w];
+                if (q[j].two) {
+                    tx[j][4] = q[j].T1[q[j].i1.x]; tx[j][5] = q[j].T1[q[j].i1.y]; tx[j][6] = q[j].T1[q[j].i1.z]; tx[j][7] = q[j].T1[q[j].i1.w];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const size_t p = p0 + (size_t)j * BS;
+            if (p >= P) continue;
+            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r[j].w >= 1.f) {
+                o = bilerp4(tx[j][0], tx[j][1], tx[j][2], tx[j][3], q[j].fx, q[j].fy);
+                if (q[j].two) o = lerp4(o, bilerp4(tx[j][4], tx[j][5], tx[j][6], tx[j][7], q[j].gx, q[j].gy), q[j].f);
+            }
+            ((float4*)out)[p] = o;
+            if (zero) ((float4*)zero)[p] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
 };
+constexpr int INTERP_TEX_NPX = 2;
 __global__ void __launch_bounds__(BS) k_interp_tex4(size_t P, const float* rast, const float* db, const float* uvattr, const u32* idx, float* uvp,
-                                                    float* uvDA, TexArgs t, float* out, float* zero) { KInterpTex4F{}(vb_hw(), P, rast, db, uvattr, idx, uvp, uvDA, t, out, zero); }
+                                                    float* uvDA, TexArgs t, float* out, float* zero) {
+    KInterpTex4F<INTERP_TEX_NPX>{}(vb_hw(), P, rast, db, uvattr, idx, uvp, uvDA, t, out, zero);
+}
 
 struct KTexFwdF {
     static constexpr int kBS = BS, kMINW = 1;
@@ -1793,12 +1879,13 @@ __device__ __forceinline__ void sg_bwd_lobe_loop(const SgLobe& L, const SgPix* p
 }
 
 // One pixel's inputs to the lobe backward (channels beyond C are 0; gd: the current diffuse grad,
-// de: diffenv, loaded when the diffuse grad is wanted).
+// de: diffenv, loaded when the diffuse grad is wanted; gd_fresh: the diffuse grad holds nothing
+// yet, so gd is 0 without a load -- the value a cleared buffer would give).
 struct SgRaw { float go[4], df[4], de[4], gd[4], nx, ny, nz, ss, sa, ax, ay, az; bool cov; };
 
 __device__ __forceinline__ void sg_load_raw(SgRaw& r, size_t p, bool cov, int C, const float* gout, const float* diffuse, const float* denv,
                                             const float* gdiffuse, const float* normal, const float* sAxis, const float* sSharp,
-                                            const float* sAmp) {
+                                            const float* sAmp, bool gd_fresh) {
     r.cov = cov;
     if (!cov) return;
 #pragma unroll
@@ -1806,7 +1893,7 @@ __device__ __forceinline__ void sg_load_raw(SgRaw& r, size_t p, bool cov, int C,
         r.go[k] = k < C ? gout[p * C + k] : 0.f;
         r.df[k] = k < C ? diffuse[p * C + k] : 0.f;
         r.de[k] = k < C && gdiffuse ? denv[p * C + k] : 0.f;
-        r.gd[k] = k < C && gdiffuse ? gdiffuse[p * C + k] : 0.f;
+        r.gd[k] = k < C && gdiffuse && !gd_fresh ? gdiffuse[p * C + k] : 0.f;
     }
     r.nx = normal[3 * p]; r.ny = normal[3 * p + 1]; r.nz = normal[3 * p + 2];
     r.ss = sSharp[p]; r.sa = sAmp[p];
@@ -1826,7 +1913,7 @@ struct KSgBwdLobesF {
     __device__ __forceinline__ void operator()(const VB& vb, size_t P, int C, int K, int KP, const float* rast, const float* axis, const float* sharp,
                                                      const float* amp, const float* normal, const float* diffuse, const float* sAxis,
                                                      const float* sSharp, const float* sAmp, const float* gout, const float* denv, float* gdiffuse,
-                                                     float* part, int want_lobes) const {
+                                                     float* part, int want_lobes, int gd_fresh) const {
         __shared__ SgLobe lobes[SG_MAXK];
         __shared__ SgPix pix[BS];
         __shared__ int wcount[BS / 64];
@@ -1887,7 +1974,8 @@ struct KSgBwdLobesF {
         float rwn;
         {
             const size_t p0 = (size_t)vb.b.x * BS + tid, p1 = p0 + stride;
-            sg_load_raw(cur, p0, p0 < P && (int)rast[4 * p0 + 3] - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp);
+            sg_load_raw(cur, p0, p0 < P && (int)rast[4 * p0 + 3] - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp,
+                        gd_fresh != 0);
             rwn = p1 < P ? rast[4 * p1 + 3] : 0.f;
         }
         for (size_t base = (size_t)vb.b.x * BS; base < P; base += stride) {
@@ -1895,7 +1983,8 @@ struct KSgBwdLobesF {
             const bool cov = cur.cov;
             {
                 const size_t pn = p + stride, p2 = pn + stride;
-                sg_load_raw(nxt, pn, pn < P && (int)rwn - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp);
+                sg_load_raw(nxt, pn, pn < P && (int)rwn - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp,
+                            gd_fresh != 0);
                 rwn = p2 < P ? rast[4 * p2 + 3] : 0.f;
             }
             SgPix q;
@@ -1978,7 +2067,9 @@ struct KSgBwdLobesF {
 __global__ void __launch_bounds__(BS) k_sg_bwd_lobes(size_t P, int C, int K, int KP, const float* rast, const float* axis, const float* sharp,
                                                      const float* amp, const float* normal, const float* diffuse, const float* sAxis,
                                                      const float* sSharp, const float* sAmp, const float* gout, const float* denv, float* gdiffuse,
-                                                     float* part, int want_lobes) { KSgBwdLobesF{}(vb_hw(), P, C, K, KP, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, gout, denv, gdiffuse, part, want_lobes); }
+                                                     float* part, int want_lobes, int gd_fresh) {
+    KSgBwdLobesF{}(vb_hw(), P, C, K, KP, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, gout, denv, gdiffuse, part, want_lobes, gd_fresh);
+}
 
 // ---------------------------------------------------------------- antialias
 struct AAEdge { float alpha, ia, pax, pay, pbx, pby, sg; int va, vb, horiz, target_o, valid; };
@@ -2100,7 +2191,8 @@ struct KAaFwdF {
 #pragma clang fp contract(off)
         __shared__ float4 to_cur[AA_PAIRS], to_nb[AA_PAIRS];
         const int s = vb.b.y;
-        const int x0 = (vb.b.x % tilesX) * AA_T, y0 = (vb.b.x / tilesX) * AA_T;
+        const unsigned tile = xcd_swizzle(vb.b.x, vb.g.x, (unsigned)tilesX);
+        const int x0 = (tile % tilesX) * AA_T, y0 = (tile / tilesX) * AA_T;
         const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
         const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
         const float* inv = in + (size_t)s * a.W * a.H * a.C;
@@ -2186,7 +2278,8 @@ struct KAaBwdF {
         __shared__ LdsTable<AA_VSLOTS, 3> vt;
         __shared__ u32 vused[AA_VSLOTS], wave_count[BS / 64];
         const int s = vb.b.y;
-        const int x0 = (vb.b.x % tilesX) * AA_T, y0 = (vb.b.x / tilesX) * AA_T;
+        const unsigned tile = xcd_swizzle(vb.b.x, vb.g.x, (unsigned)tilesX);
+        const int x0 = (tile % tilesX) * AA_T, y0 = (tile / tilesX) * AA_T;
         const float4* clipv = (const float4*)a.proj + (size_t)s * a.V;
         const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
         size_t off = (size_t)s * a.W * a.H * a.C;
@@ -2422,7 +2515,7 @@ int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_pa
     if (st) return st;
     size_t P = (size_t)ip->width * ip->height * ip->depth;
     if (!P) return VCL_OK;
-    vlaunch<KInterpTex4F>(k_interp_tex4, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), P, ip->rast, ip->rastDB, ip->attr, ip->idx, ip->out, ip->outDA,
+    vlaunch<KInterpTex4F<INTERP_TEX_NPX>>(k_interp_tex4, dim3(cdiv(P, (size_t)BS * INTERP_TEX_NPX)), dim3(BS), 0, as_hip(stream), P, ip->rast, ip->rastDB, ip->attr, ip->idx, ip->out, ip->outDA,
                        t, tp->out, zero);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
@@ -2827,7 +2920,7 @@ extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_param
 }
 
 int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
-                                        vcl_sg_lobe_sums* lobes, vcl_stream_t stream) {
+                                        vcl_sg_lobe_sums* lobes, int gd_fresh, vcl_stream_t stream) {
     if (lobes) lobes->seg = nullptr;
     VCL_REQUIRE(p && g && p->rast && p->normal && p->diffuse && p->specAxis && p->specSharpness && p->specAmplitude && g->out &&
                 p->channel >= 1 && p->channel <= SG_MAXC && p->sgnum >= 0);
@@ -2846,7 +2939,7 @@ int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, 
         int KP = 1;
         while (KP < K) KP <<= 1;
         vlaunch<KSgBwdLobesF>(k_sg_bwd_lobes, dim3(nb), dim3(BS), 0, s, P, C, K, KP, p->rast, p->axis, p->sharpness, p->amplitude, p->normal,
-                           p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, g->out, p->outDiffenv, g->diffuse, part, want);
+                           p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, g->out, p->outDiffenv, g->diffuse, part, want, gd_fresh);
     } else {
         size_t lds = sizeof(float) * 4 * (size_t)K * SG_COLS;
         VCL_REQUIRE(lds <= 160 * 1024);
@@ -2872,7 +2965,7 @@ int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, 
 
 extern "C" int vcl_spherical_gaussian_backward(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,
                                                vcl_stream_t stream) {
-    return vcl_spherical_gaussian_backward_dev(p, g, nullptr, stream);
+    return vcl_spherical_gaussian_backward_dev(p, g, nullptr, 0, stream);
 }
 
 // gout: the backward's output grads (16-byte alignment checked with the input's), or null

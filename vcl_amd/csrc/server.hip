@@ -223,8 +223,19 @@ static int launch_steps(vcl_server* S, Req* r) {
 // signature.  A group whose requests are exactly one part is reused; groups not reused this
 // round are dropped once their queued work has completed.
 static int launch_grouped(vcl_server* S, const std::vector<Req*>& due) {
-    std::map<uint64_t, std::vector<Req*>> parts;
-    for (Req* r : due) parts[r->sig].push_back(r);
+    // one part per (step signature, pixel bucket of 4x): requests of like size step in lock
+    // step well, while a group of a large and a small request would hold the small one to the
+    // large one's pace at every launch
+    std::map<std::pair<uint64_t, uint64_t>, std::vector<Req*>> parts;
+    const double max_px = S->cfg.group_max_pixels > 0 ? (double)S->cfg.group_max_pixels : 1e300;
+    for (Req* r : due) {
+        if ((double)r->d.width * r->d.height * r->d.views > max_px) {
+            int st = launch_steps(S, r);
+            if (st) return st;
+            continue;
+        }
+        parts[{r->sig, size_class(&r->d) & 255}].push_back(r);
+    }
     for (GroupSlot* G : S->groups) G->mark = false;
     std::vector<std::pair<GroupSlot*, std::vector<Req*>*>> plan;
     for (auto& kv : parts) {

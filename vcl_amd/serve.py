@@ -28,10 +28,10 @@ class Server:
     FIFO, SRPT = 0, 1
 
     def __init__(self, hbm_budget=200 * GiB, max_active=8, use_graph=True, steps_per_round=2, auto_release=True,
-                 policy=0, max_stepping=0, autotune_steps=0, grouped=False):
+                 policy=0, max_stepping=0, autotune_steps=0, grouped=False, group_max_pixels=0):
         self.cfg = L.ServerConfig(int(hbm_budget), int(max_active), 1 if use_graph else 0, int(steps_per_round),
                                   1 if auto_release else 0, int(policy), int(max_stepping), int(autotune_steps),
-                                  1 if grouped else 0)
+                                  1 if grouped else 0, int(group_max_pixels))
         h = C.c_void_p()
         L.check(L.load().vcl_server_create(C.byref(self.cfg), C.byref(h)), "server_create")
         self.h = h
