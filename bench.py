@@ -25,15 +25,80 @@ PEAK_HBM_GBS = 8000.0
 TRAFFIC_FILE = "profiles/fit_step_1024_100k_hbm_traffic.md"
 
 
-def measured_traffic():
+# VALU issue rates, L2 atomic counts and the device's measured peaks (tests/cpp/device_rates), from
+# rocprofv3 --pmc runs of the same workload
+COUNTERS_FILE = "profiles/fit_step_counters_issue_lds_atomics.md"
+
+
+def _profile_values(path):
+    """'key: value' lines of a profiles/ summary."""
+    out = {}
     try:
-        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+        with open(os.path.join(ROOT, path)) as f:
             for line in f:
-                if line.startswith("k_tex_update4m traffic_bytes_per_launch:"):
-                    return int(line.split(":")[1])
+                if ":" in line and not line.startswith(("|", "#", "`")):
+                    k, v = line.rsplit(":", 1)
+                    try:
+                        out[k.strip()] = float(v)
+                    except ValueError:
+                        pass
     except OSError:
         pass
-    return None
+    return out
+
+
+def measured_traffic(kernel="k_tex_update4m"):
+    v = _profile_values(TRAFFIC_FILE).get(f"{kernel} traffic_bytes_per_launch")
+    return int(v) if v is not None else None
+
+
+def unique_texels(rast, uv, uvda, tw, th, L):
+    """Texels (over all mip levels) that one frame's texture lookups touch: the trilinear taps
+    of every covered pixel (level choice and tap rule of the texture op), counted once each."""
+    import numpy as np
+    cov = rast.reshape(-1, 4)[:, 3] >= 1.0
+    u = uv.reshape(-1, 2)[cov].astype(np.float32)
+    d = uvda.reshape(-1, 4)[cov].astype(np.float32)
+    keys = []
+
+    def taps(level, sel):
+        sx, sy = (tw >> level), (th >> level)
+        tx, ty = u[sel, 0] * np.float32(sx), u[sel, 1] * np.float32(sy)
+        u0 = np.where(tx < 0, 0, np.where(tx > sx - 1, sx - 1, tx.astype(np.int64)))
+        u1 = np.where(tx < 1, 0, np.where(tx > sx - 2, sx - 1, tx.astype(np.int64) + 1))
+        v0 = np.where(ty < 0, 0, np.where(ty > sy - 1, sy - 1, ty.astype(np.int64)))
+        v1 = np.where(ty < 1, 0, np.where(ty > sy - 2, sy - 1, ty.astype(np.int64) + 1))
+        base = level.astype(np.int64) << 40 if hasattr(level, "astype") else np.int64(level) << 40
+        for yy, xx in ((v0, u0), (v0, u1), (v1, u0), (v1, u1)):
+            keys.append(base + yy * sx + xx)
+
+    if L <= 1:
+        taps(0, np.ones(len(u), bool))
+    else:
+        dsdx, dsdy, dtdx, dtdy = d[:, 0] * tw, d[:, 1] * tw, d[:, 2] * th, d[:, 3] * th
+        b = np.float32(0.5) * ((dsdx * dsdx + dsdy * dsdy) + (dtdx * dtdx + dtdy * dtdy))
+        pm, qm, pp, qp = dsdx - dtdy, dsdy + dtdx, dsdx + dtdy, dsdy - dtdx
+        c = np.sqrt(np.float32(0.25) * ((pm * pm + qm * qm) * (pp * pp + qp * qp)))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            lev = np.float32(0.5) * np.log2(b + c)
+        fl = np.floor(lev)
+        l0 = np.where(lev <= 0, 0, np.where(L - 2 <= lev, L - 2, fl)).astype(np.int64)
+        l1 = np.where(lev <= 1, 1, np.where(L - 1 <= lev, L - 1, fl + 1)).astype(np.int64)
+        f = np.where(lev <= 0, 0, np.where(L - 1 <= lev, 1, lev - fl))
+        for lv in np.unique(l0):
+            taps(int(lv), l0 == lv)
+        for lv in np.unique(l1[f > 0]):
+            taps(int(lv), (l1 == lv) & (f > 0))
+    keys = [k for k in keys if k.size]
+    return int(np.unique(np.concatenate(keys)).size) if keys else 0
+
+
+def unfused_bytes(sc, views, U):
+    """HBM bytes per step of the unfused op graph: each op's surfaces read or written once, per
+    pixel 712 B (forward 428, backward 284), texture update 240 B per level-0 texel of 4
+    channels, vertex / triangle / lobe terms, and 64 B per touched texel (16 forward, 48 backward)."""
+    P = sc.W * sc.H * views
+    return 712 * P + 240 * sc.tw * sc.th * sc.C // 4 + 216 * sc.V + 60 * sc.T + 320 * sc.K + 64 * U
 
 
 def algorithmic_bytes(sc, views=1):
@@ -65,13 +130,13 @@ def algorithmic_bytes(sc, views=1):
     }
 
 
-def run_batched(ops, S, torch, steps, warmup, views=64, W=512, tune=True):
+def run_batched(ops, S, torch, steps, warmup, views=64, W=512, tune=True, K=64):
     """Batched multi-view request: the 100k-triangle mesh from `views` Fibonacci-sphere cameras
     at W x W, shared 2048^2 x 4 texture, 64 lobes; one fwd+bwd+Adam step over all views per
     iteration, replayed from a HIP graph.  Returns the result object (1024^2-equivalent frames/s
     = pixels per second / 1024^2)."""
-    sc = S.config3_scene(1, W=W, views=views)
-    tf = ops.Fit(S.config3_scene(2, W=W, views=views))
+    sc = S.config3_scene(1, W=W, views=views, K=K)
+    tf = ops.Fit(S.config3_scene(2, W=W, views=views, K=K))
     tf.set_target_from_render()
     target = tf.read("image", (views, W, W, sc.C))
     tf.close()
@@ -182,11 +247,24 @@ def run_served(torch, rates, n, seed=3, budget_gb=200, max_active=16):
     return out
 
 
+def _uneven_modes(SV, built, max_active):
+    """The same batch (all requests admitted at t=0): grouped shared launches, per-request
+    streams, and one request after another."""
+    _serve_once(SV, built[:1], 0.0, 1, True, 2, int(200e9))
+    _serve_once(SV, built[:2], 0.0, 2, True, 2, int(200e9), grouped=True)
+    grp = _serve_once(SV, built, 0.0, max_active, True, 4, int(200e9), grouped=True)
+    cont = _serve_once(SV, built, 0.0, max_active, True, 4, int(200e9))
+    seq = _serve_once(SV, built, 0.0, 1, False, 1, int(200e9))
+    return grp, cont, seq
+
+
 def run_uneven(torch, n=32, iters=20, seed=5, max_active=32):
     """Batched uneven scenes: n independent requests of uneven size, all admitted at once and
-    stepped concurrently -- in shared launches over device-side work tables (grouped), and with
-    each request replaying its own graph on its own stream -- against the same requests run
-    one after another with eager steps."""
+    stepped concurrently -- each request replaying its own graph on its own stream (the
+    server's default), and in shared launches over device-side work tables (grouped) -- against
+    the same requests run one after another with eager steps.  Also a batch of 64 small
+    requests (128^2 / 256^2, 1,280-20,480 triangles), where the shared launches pay most."""
+    import numpy as np
     from vcl_amd import serve as SV
     specs = SV.uneven_scenes(n, seed)
     built = []
@@ -194,17 +272,26 @@ def run_uneven(torch, n=32, iters=20, seed=5, max_active=32):
         sc, tsc = SV.build(sp)
         sp = dict(sp, arrival_s=0.0, iterations=iters)
         built.append((sp, sc, SV.render_target(tsc)))
-    _serve_once(SV, built[:1], 0.0, 1, True, 2, int(200e9))
-    _serve_once(SV, built[:2], 0.0, 2, True, 2, int(200e9), grouped=True)
-    grp = _serve_once(SV, built, 0.0, max_active, True, 2, int(200e9), grouped=True)
-    cont = _serve_once(SV, built, 0.0, max_active, True, 2, int(200e9))
-    seq = _serve_once(SV, built, 0.0, 1, False, 1, int(200e9))
-    return {"scenes": n, "iterations": iters, "grouped": grp, "streams": cont, "sequential": seq,
-            "value": grp["eq_frames_per_s"], "unit": "1024^2-equivalent frames/s",
-            "vs_sequential": round(grp["eq_frames_per_s"] / seq["eq_frames_per_s"], 3),
-            "streams_vs_sequential": round(cont["eq_frames_per_s"] / seq["eq_frames_per_s"], 3),
-            "workload": "32 requests: displaced icospheres 1,280-327,680 T, 256^2-2048^2, textures 256^2-4096^2 x4 with full "
-                        "mips, 16/64/128 lobes, 20 Adam iterations each, all admitted at t=0"}
+    grp, cont, seq = _uneven_modes(SV, built, max_active)
+    rng = np.random.default_rng(seed + 1)
+    small = []
+    for i in range(64):
+        sp = {"level": int(rng.choice([3, 4, 5])), "W": int(rng.choice([128, 256])), "tex": 256, "K": int(rng.choice([16, 64])),
+              "seed": 3000 + i}
+        sc, tsc = SV.build(sp)
+        small.append((dict(sp, arrival_s=0.0, iterations=iters), sc, SV.render_target(tsc)))
+    sgrp, scont, sseq = _uneven_modes(SV, small, 64)
+    return {"scenes": n, "iterations": iters, "streams": cont, "grouped": grp, "sequential": seq,
+            "value": cont["eq_frames_per_s"], "unit": "1024^2-equivalent frames/s",
+            "vs_sequential": round(cont["eq_frames_per_s"] / seq["eq_frames_per_s"], 3),
+            "grouped_vs_sequential": round(grp["eq_frames_per_s"] / seq["eq_frames_per_s"], 3),
+            "workload": f"{n} requests: displaced icospheres 1,280-327,680 T, 256^2-2048^2, textures 256^2-4096^2 x4 with full "
+                        f"mips, 16/64/128 lobes, {iters} Adam iterations each, all admitted at t=0",
+            "small_requests": {"requests": 64, "streams": scont, "grouped": sgrp, "sequential": sseq,
+                               "grouped_vs_sequential": round(sgrp["eq_frames_per_s"] / sseq["eq_frames_per_s"], 3),
+                               "streams_vs_sequential": round(scont["eq_frames_per_s"] / sseq["eq_frames_per_s"], 3),
+                               "workload": f"64 requests: icospheres 1,280-20,480 T, 128^2 / 256^2, 256^2 x4 textures, 16/64 lobes, "
+                                           f"{iters} iterations each, all admitted at t=0"}}
 
 
 def cpu_baseline(sc, target, steps, threads):
@@ -257,6 +344,50 @@ def cpu_threads():
     except (AttributeError, OSError):
         n = os.cpu_count() or 1
     return max(1, min(16, n))
+
+
+def kernel_roofline_entries(sc, views, kernels, ab, value, U):
+    """Roofline of the step's main kernels: live per-op times of this run against HBM bytes,
+    VALU issue or L2 atomic rates (counter figures of the same workload from profiles/), and the
+    end-to-end frames/s against the unfused op graph's HBM bound."""
+    cv = _profile_values(COUNTERS_FILE)
+    fma = cv.get("valu_fma_peak_per_cu_cycle")
+    apeak = cv.get("atomic_f32_peak_per_s")
+    out = []
+
+    def t(op):
+        k = kernels.get(op)
+        return k["avg_ms"] if k else None
+
+    for op, kern in (("texture_adam", "k_tex_update4m"), ("interp_texture", "k_interp_tex4"), ("antialias_bwd", "k_aa_bwd"),
+                     ("antialias", "k_aa_fwd")):
+        ms = t(op)
+        if not ms:
+            continue
+        alg = ab[op] + (16 * U if op == "interp_texture" else 0)
+        tr = measured_traffic(kern)
+        out.append({"kernel": kern, "op": op, "bound": "hbm", "avg_ms": ms, "alg_bytes": alg,
+                    "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": tr,
+                    "traffic_frac": round(tr / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if tr else None,
+                    "note": "alg_bytes include 16 B per touched texel" if op == "interp_texture" else None})
+    ms = t("texture_interp_bwd")
+    n_at = cv.get("plain k_tex_bwd4_tiled<16> l2_atomics_per_launch")
+    if ms and n_at and apeak:
+        out.append({"kernel": "k_tex_bwd4_tiled<16>", "op": "texture_interp_bwd", "bound": "atomics", "avg_ms": ms,
+                    "l2_atomics_per_launch": int(n_at), "achieved": round(n_at / (ms * 1e-3) / 1e9, 2), "peak": round(apeak / 1e9, 2),
+                    "unit": "G atomics/s", "frac": round(n_at / (ms * 1e-3) / apeak, 4), "traffic": measured_traffic("k_tex_bwd4_tiled<16>")})
+    for op, kern in (("spherical_gaussian", "k_sg_fwd4<512>"), ("spherical_gaussian_bwd", "k_sg_bwd_lobes")):
+        ms = t(op)
+        vi = cv.get(f"plain {kern} valu_issue_per_cu_cycle")
+        if ms and vi and fma:
+            out.append({"kernel": kern, "op": op, "bound": "valu", "avg_ms": ms, "achieved": round(vi, 4), "peak": round(fma, 4),
+                        "unit": "VALU wave instructions per CU per cycle (fp32, no packed math)", "frac": round(vi / fma, 4)})
+    B = unfused_bytes(sc, views, U)
+    bound_fps = PEAK_HBM_GBS * 1e9 / B * views
+    out.append({"kernel": "whole step", "bound": "hbm (unfused op graph)", "unfused_bytes_per_step": int(B), "touched_texels_first_frame": U,
+                "peak_frames_per_s": round(bound_fps, 1), "achieved": value, "unit": "frames/s", "frac": round(value / bound_fps, 4)})
+    return out
 
 
 def main():
@@ -393,16 +524,20 @@ def main():
     torch.cuda.synchronize()
     graph_ms = g0.elapsed_time(g1) / max(args.steps - 1, 1)
     gf.close()
-    # per-op breakdown on a second fresh request (all ops event-timed; not the timed value)
-    if args.breakdown:
-        bf = ops.Fit(sc, target=target)
-        bf.profile_begin(max_launches=args.steps + 1)
-        for _ in range(args.steps):
-            bf.step()
-        allprof = bf.profile_end()
-        bf.close()
-    else:
-        allprof = {}
+    # per-op breakdown on a second fresh request (all ops event-timed; not the timed value), and
+    # the texels its last frame touched
+    bf = ops.Fit(sc, target=target)
+    if tuning:
+        bf.set_tuning(tuning["raster_tile"], tuning["bin_chunk"], tuning["splat_rows"])
+    bf.profile_begin(max_launches=args.steps + 1)
+    P1 = sc.W * sc.H * views
+    U = None
+    for i in range(args.steps):
+        bf.step()
+        if i == 0:   # the texels of the first frame (later frames of the degrading mesh sample coarser levels)
+            U = unique_texels(bf.read("rast", (P1, 4)), bf.read("uvp", (P1, 2)), bf.read("uvDA", (P1, 4)), sc.tw, sc.th, sc.L)
+    allprof = bf.profile_end()
+    bf.close()
     allprof.update(prof)
     ab = algorithmic_bytes(sc, views)
     kernels = {}
@@ -420,6 +555,7 @@ def main():
                 "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None, "traffic": traffic,
                 "traffic_source": TRAFFIC_FILE + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)" if traffic else None,
                 "alg_bytes_per_launch": ab[ROOF_OP], "avg_ms": rk["avg_ms"] if rk else None}
+    kernel_rooflines = kernel_roofline_entries(sc, views, kernels, ab, value, U) if args.res == 1024 else None
     cpu = None
     if not args.no_cpu:
         threads = cpu_threads()
@@ -441,6 +577,9 @@ def main():
     if not args.no_batched:
         batched = run_batched(ops, S, torch, args.batched_steps, 3, views=args.batched_views, tune=not args.no_tune)
         batched["vs_plain"] = round(batched["value"] / value, 3)
+        # the same batch with the plain request's lobe count, so the ratio compares equal SG work per pixel
+        eq = run_batched(ops, S, torch, args.batched_steps, 3, views=args.batched_views, tune=False, K=sc.K)
+        batched["equal_k"] = {"K": sc.K, "value": eq["value"], "ms_per_step": eq["ms_per_step"], "vs_plain": round(eq["value"] / value, 3)}
     out = {
         "metric": "fwd+bwd+Adam frames/s at 1024^2, 100k tris (plain per-request path)",
         "value": round(value, 2), "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -450,6 +589,7 @@ def main():
                                f"L={sc.L}, {sc.K} SG lobes, antialias, L2, Adam (pos, texture, lobes)",
                    "mode": args.mode, "views": views},
         "roofline": roofline,
+        "kernel_rooflines": kernel_rooflines,
         "cpu_baseline": cpu,
         "tuning": tuning,
         "graph_ms_per_step": round(graph_ms, 4),

@@ -65,6 +65,8 @@ TOOLS = {
     os.path.join(ROOT, "tests", "cpp", "preset_fit"): os.path.join(ROOT, "tests", "cpp", "preset_fit.cpp"),
     # tests/cpp/preset_pose: the reference's pose preset (OBJ, SSIM, Rotation / Camera grads)
     os.path.join(ROOT, "tests", "cpp", "preset_pose"): os.path.join(ROOT, "tests", "cpp", "preset_pose.cpp"),
+    # tests/cpp/device_rates: measured atomic and VALU peak rates (the scatter / shading kernels' ceilings)
+    os.path.join(ROOT, "tests", "cpp", "device_rates"): os.path.join(ROOT, "tests", "cpp", "device_rates.hip"),
 }
 
 
@@ -75,8 +77,8 @@ def build_tools(force: bool = False) -> list:
         deps = [src, LIB] + glob.glob(os.path.join(ROOT, "include", "*"))
         if force or not os.path.exists(exe) or any(os.path.getmtime(d) > os.path.getmtime(exe) for d in deps):
             rel = os.path.relpath(HERE, os.path.dirname(exe))
-            cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"), src, "-o", exe,
-                   "-L", HERE, "-lvcl_amd", f"-Wl,-rpath,$ORIGIN/{rel}"]
+            cmd = [HIPCC, "-O2", "-std=c++17", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                   "-I", os.path.join(ROOT, "include"), src, "-o", exe, "-L", HERE, "-lvcl_amd", f"-Wl,-rpath,$ORIGIN/{rel}"]
             r = subprocess.run(cmd, capture_output=True, text=True)
             if r.returncode != 0:
                 raise RuntimeError(f"tool build failed for {src}:\n{r.stderr[-6000:]}")
