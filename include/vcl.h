@@ -759,7 +759,8 @@ typedef struct vcl_server_config {
     int steps_per_round;  /* steps queued per request per scheduling round (>= 1) */
     int auto_release;     /* 1: free a request's arena as soon as it finishes */
     int policy;           /* VCL_SCHED_FIFO or VCL_SCHED_SRPT */
-    int max_stepping;     /* SRPT: requests stepping at once (0: all active) */
+    int max_stepping;     /* requests stepping at once (0: all active): under FIFO the earliest
+                             admitted, under SRPT those with the least remaining work */
     int autotune_steps;   /* > 0: autotune the first request of each size class (pixels over
                              all views in buckets of 4x, triangles in buckets of 8x) with this
                              many timed steps per candidate, and reuse its tuning for the class;

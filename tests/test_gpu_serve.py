@@ -8,6 +8,7 @@
   gradient tolerance (1e-3), after 10 to 5e-3.
 - The same at the mix's largest draws (327,680 triangles, 2048^2, 4096^2 texture), with the
   requests stepping on their own streams and in shared launches (grouped).
+- FIFO with a stepping limit steps only the earliest-admitted requests.
 - Admission respects the HBM budget: with room for one request at a time every request still
   finishes, each admitted only after the previous one released its arena.
 """
@@ -160,3 +161,21 @@ def test_alone_vs_mix_largest_draws(gpu, grouped):
     srv.drain()
     assert float(srv.read(t, "loss", (1,))[0]) == one["first"]
     srv.close()
+
+
+def test_fifo_stepping_limit(gpu):
+    """FIFO with max_stepping = 1 steps only the earliest-admitted active request: the requests
+    finish in admission order, each with all its iterations and the loss of its own plain run's
+    first step when it has one iteration."""
+    reqs = [_request(s) for s in [A_SPEC] + MIX]
+    srv = SV.Server(max_active=4, auto_release=False, steps_per_round=2, max_stepping=1)
+    tickets = [srv.submit(sc, tgt, 1 + 2 * i) for i, (sc, tgt) in enumerate(reqs)]
+    srv.drain()
+    st = {s["ticket"]: s for s in srv.finished()}
+    assert all(st[t]["status"] == 0 and st[t]["iterations"] == 1 + 2 * i for i, t in enumerate(tickets))
+    order = sorted(tickets, key=lambda t: st[t]["admit_s"])
+    fin = [st[t]["finish_s"] for t in order]
+    assert fin == sorted(fin), "a later-admitted request finished first"
+    first = float(srv.read(tickets[0], "loss", (1,))[0])
+    srv.close()
+    assert first == _plain(*reqs[0], 1)["first"]

@@ -379,8 +379,9 @@ extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
         r->admit = S->now();
         S->active.push_back(r);
     }
-    // 3. queue the next steps: every idle active request (FIFO), or under VCL_SCHED_SRPT only the
-    //    max_stepping requests with the least remaining work (the others stay resident, paused)
+    // 3. queue the next steps: under FIFO the idle ones of the max_stepping earliest-admitted
+    //    requests (all when 0), under VCL_SCHED_SRPT those of the max_stepping requests with the
+    //    least remaining work (the others stay resident, paused)
     if (S->cfg.grouped) {
         std::vector<Req*> due;
         if (srpt) {
@@ -392,8 +393,11 @@ extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
             for (int i = 0; i < (int)order.size() && i < lim; i++)
                 if (order[i]->launched == order[i]->done && order[i]->launched < order[i]->iterations) due.push_back(order[i]);
         } else {
-            for (Req* r : S->active)
+            const int lim = S->cfg.max_stepping > 0 ? S->cfg.max_stepping : (int)S->active.size();
+            for (int i = 0; i < (int)S->active.size() && i < lim; i++) {
+                Req* r = S->active[i];
                 if (r->launched == r->done && r->launched < r->iterations) due.push_back(r);
+            }
         }
         int st = launch_grouped(S, due);
         if (st) return st;
@@ -411,7 +415,10 @@ extern "C" int vcl_server_progress(vcl_server* S, int* pending) {
             }
         }
     } else {
-        for (Req* r : S->active) {
+        // FIFO: the max_stepping earliest-admitted requests (all when 0)
+        const int lim = S->cfg.max_stepping > 0 ? S->cfg.max_stepping : (int)S->active.size();
+        for (int i = 0; i < (int)S->active.size() && i < lim; i++) {
+            Req* r = S->active[i];
             if (r->launched == r->done && r->launched < r->iterations) {
                 int st = launch_steps(S, r);
                 if (st) return st;
