@@ -4,9 +4,13 @@
 // loss, the rotation's gradient flowing through Project -> Camera -> Rotation.  The GLUT window
 // becomes the headless vcl::Window (GLbuffer::draw composites, Window::save writes a BMP).
 //
-// usage: preset_pose <mesh.obj> <out.bin> <steps> <exact 0|1> <image.bmp>
+// usage: preset_pose <mesh.obj> <out.bin> <steps> <exact 0|1> <image.bmp> [offset]
+// offset (radians, optional): the fitted rotation starts at the target's rotated by this angle
+//   about a fixed axis instead of at a second random draw (a start the fit can reach whatever
+//   rand() returns; the runtime's own threads may draw from the same generator).
 // out.bin: float32 loss per step; then the first step's quaternions (x, y, z, w) of the target
-//   and of the fitted rotation; then the last rendered image [512*512*3] and the target image.
+//   and of the fitted rotation; then the last rendered image [512*512*3] and the target image;
+//   then the fitted rotation after the last step.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -16,10 +20,11 @@
 using namespace vcl;
 
 int main(int argc, char** argv) {
-    if (argc < 6) { std::fprintf(stderr, "usage: %s mesh.obj out.bin steps exact image.bmp\n", argv[0]); return 2; }
+    if (argc < 6) { std::fprintf(stderr, "usage: %s mesh.obj out.bin steps exact image.bmp [offset]\n", argv[0]); return 2; }
     const char* mesh = argv[1];
     int steps = std::atoi(argv[3]);
     bool exact = std::atoi(argv[4]) != 0;
+    const float offset = argc > 6 ? (float)std::atof(argv[6]) : 0.f;
     std::srand(12345);  // the reference seeds with the clock (main.cpp:21); fixed here
 
     const int resolution = 512, INIT_INTERVAL = 200;
@@ -86,6 +91,10 @@ int main(int argc, char** argv) {
             Antialias::forward(target_aa);
             Rotation::reset(rot);
             Rotation::setRandom(rot);
+            if (offset != 0.f) {
+                rot.kernel.q = target_rot.kernel.q;
+                Rotation::addRotation(rot, offset, .3f, 1.f, .2f);
+            }
         }
         if (step == 0) {
             const quat& a = target_rot.kernel.q;
@@ -125,6 +134,9 @@ int main(int argc, char** argv) {
     std::fwrite(quats.data(), sizeof(float), quats.size(), o);
     std::fwrite(img.data(), sizeof(float), img.size(), o);
     std::fwrite(timg.data(), sizeof(float), timg.size(), o);
+    const quat& f = rot.kernel.q;
+    const float qf[4] = {f.x, f.y, f.z, f.w};
+    std::fwrite(qf, sizeof(float), 4, o);
     std::fclose(o);
     std::printf("preset_pose: %d steps, loss %.6e -> %.6e\n", steps, losses.front(), losses.back());
     return 0;
