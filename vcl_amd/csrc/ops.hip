@@ -1653,10 +1653,10 @@ __global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, cons
                                                     const float* amp, const float* normal, const float* diffuse, const float* sAxis,
                                                     const float* sSharp, const float* sAmp, float* out, float* denv, float* senv) { KSgFwdFastF{}(vb_hw(), P, C, K, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out, denv, senv); }
 
-// C == 4 forward.  A workgroup takes 1024 consecutive pixels and compacts the covered ones, so
-// its waves shade covered pixels only (a thread-per-pixel wave at a silhouette runs the whole
-// lobe loop for a few covered lanes).  Per lobe: two 16-byte LDS reads (axis * sharpness,
-// sharpness; amplitude) and one 16-byte read of the lobe constants.
+// C == 4 forward.  A workgroup takes 1024 (or 512) consecutive pixels and compacts the covered
+// ones, so its waves shade covered pixels only (a thread-per-pixel wave at a silhouette runs the whole
+// lobe loop for a few covered lanes).  Per lobe: two 16-byte LDS reads ((lam mu, cD) and the
+// amplitude) and one 4-byte read (lam log2 e).
 // AddNaNcheck per channel reduces to one finiteness test per term: the term is bounded
 // (e^{l-c} <= 1 since l <= c, and 1/max(l, 1e-6) <= 1e6), so a finite term times finite
 // amplitudes always passes the per-channel check and a non-finite one always fails it; a block
@@ -1665,28 +1665,37 @@ __global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, cons
 // (l > 8.75 gives e^{-2l} < 2^-25), from l = |w + lam*mu| >= | |lam*mu| - |w| | with |w| the
 // pixel's 2.133*normal or spec-sharpness*spec-axis length (both lengths carry a relative
 // margin for their float rounding).
-struct SgLobe4 { float4 g; float4 a; float len, cD, lamL2E, pad; };   // cD = (SG_DS + lam) log2(e), lamL2E = lam log2(e)
+// Bounded form: when every lobe of the workgroup and every pixel of the wave is finite with
+// |lam mu| - lam <= 8, |w| - base <= 8 and amplitudes within 2^32 (the normalised axes and
+// normals of a fit), each term's exponent l - c is at most 16 and 1/l at most 1e6, so a term
+// times its amplitudes is below e^16 * 1e6 * 2^32 * 2^32 < 2^108 and every partial sum of at
+// most 256 of them is finite: no term can fail AddNaNcheck, the loop tests none, and the
+// constant factors (2.34, the spec amplitude) multiply the two lobe sums once at the end
+// instead of every term.
+struct SgLobe4 { float4 g; float4 a; float lamL2E, len, lam, pad; };   // g = (lam mu, cD), cD = (SG_DS + lam) log2(e), lamL2E = lam log2(e)
 
-// One lobe term with the exponent's constant part c = (base + lam) log2(e) given: one rsq gives
-// 1/l and l, each exponent is one fma (within 1e-6 relative of the reference's (l - c) - lam
-// form), and e^{-l-c} is formed only when `both` (wave-uniform).
+// (e^{l-c} - e^{-l-c}) / l with the exponent's constant part c = (base + lam) log2(e) given: one
+// rsq gives 1/l and l, each exponent is one fma (within 1e-6 relative of the reference's
+// (l - c) - lam form), and e^{-l-c} is formed only when `both` (wave-uniform).
 __device__ __forceinline__ float sg_term_rsq(float wx, float wy, float wz, float c, bool both) {
     float d = wx * wx + wy * wy + wz * wz;
     float il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
     float l = d * il;
-    float e1 = nexp2(fmaf(l, L2E, -c));
-    float e2 = 0.f;
+    float t = nexp2(fmaf(l, L2E, -c));
     if (both) {
         // the empty asm keeps this a branch the wave skips rather than an exp computed on
         // every lobe and selected away
         asm volatile("" :::);
-        e2 = nexp2(fmaf(-l, L2E, -c));
+        t = t - nexp2(fmaf(-l, L2E, -c));
     }
-    return (e1 - e2) * il;
+    return t * il;
 }
 
 // e^{-l-c} is below half an ulp of e^{l-c} once l > SG_E2_DROP (e^{-17.5} < 2^-25).
 constexpr float SG_E2_DROP = 8.75f;
+// Bounded form's limits: |lam mu| - lam and |w| - base at most SG_FWD_EXCESS, amplitudes and
+// spec amplitudes below SG_FWD_AMAX in magnitude.
+constexpr float SG_FWD_EXCESS = 8.f, SG_FWD_AMAX = 4294967296.f;
 
 // Which lobes of chunk [c0, c0 + 64) may need e^{-l-c} for some lane of the wave: lane j tests
 // lobe c0 + j against the wave's range [lmin, lmax] of the pixel-side length |w| (the per-lane
@@ -1713,11 +1722,15 @@ __device__ __forceinline__ float wave_fmax(float v) {
     return v;
 }
 
-// The lobe loop of one pixel; CHECKED selects the per-channel AddNaNcheck form (a block whose
-// lobes hold a non-finite amplitude).  The choice is uniform per block, so the two forms are
-// separate loops rather than a branch inside one.  dmask / smask: per 64-lobe chunk, the lobes
-// whose diffuse / specular e^{-l-c} is formed (sg_e2_mask), tested as scalars.
-template <bool CHECKED>
+// Forms of the lobe loop: per-channel AddNaNcheck (a workgroup whose lobes hold a non-finite
+// amplitude), one finiteness test per term, or the bounded form (no test; 2.34 and the spec
+// amplitude applied to the sums afterwards).  The choice is uniform per wave, so the forms are
+// separate loops rather than a branch inside one.
+enum SgFwdForm { SG_FWD_CHECKED, SG_FWD_TESTED, SG_FWD_BOUNDED };
+
+// The lobe loop of one pixel.  dmask / smask: per 64-lobe chunk, the lobes whose diffuse /
+// specular e^{-l-c} is formed (sg_e2_mask), tested as scalars.
+template <SgFwdForm FORM>
 __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float dnx, float dny, float dnz, float snx, float sny, float snz,
                                               float ss, float sa, const u64* dmask, const u64* smask, float4& de, float4& se) {
     const float ssL2E = ss * L2E;
@@ -1726,21 +1739,31 @@ __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float
         if (c * 64 >= K) break;
         const u64 dm = dmask[c], sm = smask[c];
         const int iend = min(K, c * 64 + 64);
+#pragma unroll 2
         for (int i = c * 64; i < iend; i++) {
             const float4 g = lobes[i].g, a = lobes[i].a;
             const bool dboth = (dm >> (i - c * 64)) & 1ull;
             const bool sboth = (sm >> (i - c * 64)) & 1ull;
-            float dg = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, lobes[i].cD, dboth) * SG_DA;
-            float sg = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ssL2E + lobes[i].lamL2E, sboth) * sa;
-            if (!CHECKED) {
-                float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
+            const float td = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, g.w, dboth);
+            const float ts = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ssL2E + lobes[i].lamL2E, sboth);
+            if (FORM == SG_FWD_BOUNDED) {
+                de.x += td * a.x; de.y += td * a.y; de.z += td * a.z; de.w += td * a.w;
+                se.x += ts * a.x; se.y += ts * a.y; se.z += ts * a.z; se.w += ts * a.w;
+            } else if (FORM == SG_FWD_TESTED) {
+                const float dg = td * SG_DA, sg = ts * sa;
+                const float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
                 de.x += dv * a.x; de.y += dv * a.y; de.z += dv * a.z; de.w += dv * a.w;
                 se.x += sv * a.x; se.y += sv * a.y; se.z += sv * a.z; se.w += sv * a.w;
             } else {
+                const float dg = td * SG_DA, sg = ts * sa;
                 acc_checked(de.x, dg * a.x); acc_checked(de.y, dg * a.y); acc_checked(de.z, dg * a.z); acc_checked(de.w, dg * a.w);
                 acc_checked(se.x, sg * a.x); acc_checked(se.y, sg * a.y); acc_checked(se.z, sg * a.z); acc_checked(se.w, sg * a.w);
             }
         }
+    }
+    if (FORM == SG_FWD_BOUNDED) {
+        de.x *= SG_DA; de.y *= SG_DA; de.z *= SG_DA; de.w *= SG_DA;
+        se.x *= sa; se.y *= sa; se.z *= sa; se.w *= sa;
     }
 }
 
@@ -1755,24 +1778,29 @@ struct KSgFwd4F {
                                                 const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
                                                 const float* sAmp, float* out, float* denv, float* senv) const {
         __shared__ SgLobe4 lobes[SG_MAXK];
-        __shared__ int nonfinite;
+        __shared__ int nonfinite, unbounded;
         __shared__ u32 plist[SG_FWD_PIX];
         __shared__ u32 wcount[BS / 64];
-        if (threadIdx.x == 0) nonfinite = 0;
+        if (threadIdx.x == 0) { nonfinite = 0; unbounded = 0; }
         __syncthreads();
         for (int i = threadIdx.x; i < K; i += blockDim.x) {
             float lam = sharp[i];
             float4 a = make_float4(amp[4 * i], amp[4 * i + 1], amp[4 * i + 2], amp[4 * i + 3]);
             float gx = axis[3 * i] * lam, gy = axis[3 * i + 1] * lam, gz = axis[3 * i + 2] * lam;
-            lobes[i].g = make_float4(gx, gy, gz, lam);
+            const float len = sqrtf(gx * gx + gy * gy + gz * gz);
+            lobes[i].g = make_float4(gx, gy, gz, (SG_DS + lam) * L2E);
             lobes[i].a = a;
-            lobes[i].len = sqrtf(gx * gx + gy * gy + gz * gz);
-            lobes[i].cD = (SG_DS + lam) * L2E;
+            lobes[i].len = len;
+            lobes[i].lam = lam;
             lobes[i].lamL2E = lam * L2E;
             if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(a.w))) nonfinite = 1;
+            // NaN lengths or sharpness fail the comparison and so count as unbounded
+            const float amax = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
+            if (!(len - lam <= SG_FWD_EXCESS && fabsf(lam) <= 1e30f && amax < SG_FWD_AMAX)) unbounded = 1;
         }
         __syncthreads();
         const bool checked = nonfinite != 0;
+        const bool lobes_bounded = unbounded == 0;
         // uncovered pixels get their zeros here; the covered ones of the workgroup's SG_FWD_PIX are
         // compacted (in pixel order) so that every wave shades covered pixels only
         const size_t base = (size_t)vb.b.x * SG_FWD_PIX;
@@ -1792,13 +1820,16 @@ struct KSgFwd4F {
             const size_t p = act ? base + plist[i] : 0;
             float dnx = 0.f, dny = 0.f, dnz = 0.f, ss = 0.f, sa = 0.f, snx = 0.f, sny = 0.f, snz = 0.f;
             float dlen = __builtin_nanf(""), slen = __builtin_nanf("");   // lanes without a pixel: ignored by fmin/fmax
+            bool bounded = true;
             if (act) {
                 dnx = normal[3 * p] * SG_DS; dny = normal[3 * p + 1] * SG_DS; dnz = normal[3 * p + 2] * SG_DS;
                 ss = sSharp[p]; sa = sAmp[p];
                 snx = sAxis[3 * p] * ss; sny = sAxis[3 * p + 1] * ss; snz = sAxis[3 * p + 2] * ss;
                 dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz);
                 slen = sqrtf(snx * snx + sny * sny + snz * snz);
+                bounded = dlen - SG_DS <= SG_FWD_EXCESS && slen - ss <= SG_FWD_EXCESS && fabsf(ss) <= 1e30f && fabsf(sa) < SG_FWD_AMAX;
             }
+            const bool wave_bounded = lobes_bounded && __ballot(!bounded) == 0ull;
             const float dmin = wave_fmin(dlen), dmax = wave_fmax(dlen), smin = wave_fmin(slen), smax = wave_fmax(slen);
             u64 dmask[SG_MAXK / 64], smask[SG_MAXK / 64];
 #pragma unroll
@@ -1808,8 +1839,9 @@ struct KSgFwd4F {
             }
             if (act) {
                 float4 de = z4, se = z4;
-                if (checked) sg_fwd4_lobes<true>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
-                else sg_fwd4_lobes<false>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
+                if (checked) sg_fwd4_lobes<SG_FWD_CHECKED>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
+                else if (wave_bounded) sg_fwd4_lobes<SG_FWD_BOUNDED>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
+                else sg_fwd4_lobes<SG_FWD_TESTED>(lobes, K, dnx, dny, dnz, snx, sny, snz, ss, sa, dmask, smask, de, se);
                 const float4 df = ((const float4*)diffuse)[p];
                 ((float4*)out)[p] = make_float4(df.x * de.x + se.x, df.y * de.y + se.y, df.z * de.z + se.z, df.w * de.w + se.w);
                 ((float4*)denv)[p] = de;
@@ -1828,7 +1860,10 @@ __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* ra
 // LDS (deterministic order), every wave walks them in the same order (LDS broadcast), and the
 // eight lobe sums stay in registers across all chunks of the block.  grad.diffuse is
 // gout * diffenv (the forward's lobe-ordered sum) written in the staging pass.
-struct SgPix { float dnx, dny, dnz, ssL2E, snx, sny, snz, sa, X[4], G[4]; };  // ssL2E = spec sharpness * log2(e)
+// X = gout * diffuse * 2.34 and G = gout * spec amplitude: the constant factors of the diffuse
+// and specular terms are taken into the pixel's staged values, so the lobe loop never applies
+// them.
+struct __align__(16) SgPix { float dnx, dny, dnz, ssL2E, snx, sny, snz, pad, X[4], G[4]; };  // ssL2E = spec sharpness * log2(e)
 
 // One lobe term of the backward: g = (e1 - e2) il and dg/dl = ((e1 + e2) - g) il, with e2 =
 // e^{-l-c} formed only when BOTH (otherwise e2 is below half an ulp of e1 and both sums equal e1).
@@ -1850,31 +1885,31 @@ __device__ __forceinline__ float sg_term_bwd(float wx, float wy, float wz, float
 }
 
 // The pixel loop of one lobe lane (see k_sg_bwd_lobes); DB / SB: the diffuse / specular e2 is
-// needed by some pixel of the chunk for some lobe of the wave.
+// needed by some pixel of the chunk for some lobe of the wave.  The chunk's pixel list is padded
+// with zero pixels to a multiple of G, so every lane runs n / G iterations (a zero pixel adds
+// zeros: its X and G are 0).  A = 2.34 * sum(gout diffuse amp), B = sa * sum(gout amp).
 template <bool DB, bool SB>
-__device__ __forceinline__ void sg_bwd_lobe_loop(const SgLobe& L, const SgPix* pix, int j0, int ncov, int G, float accW[3], float& accS,
+__device__ __forceinline__ void sg_bwd_lobe_loop(const SgLobe& L, const SgPix* pix, int grp, int nit, int G, float accW[3],
                                                  float accX[4], float accG[4]) {
     const float cD = L.lamL2E + SG_DS * L2E;
-    for (int j = j0; j < ncov; j += G) {
-        const SgPix& x = pix[j];
-        float sdiff = 0.f, ssum = 0.f;
-#pragma unroll
-        for (int k = 0; k < 4; k++) { sdiff += x.X[k] * L.a[k]; ssum += x.G[k] * L.a[k]; }
-        const float A = sdiff * SG_DA, B = ssum * x.sa;
-        float wx = x.dnx + L.gx, wy = x.dny + L.gy, wz = x.dnz + L.gz;
+    // the record is read as four 16-byte words
+    const float4* xp = reinterpret_cast<const float4*>(pix + grp);
+    for (int t = 0; t < nit; t++, xp += 4 * G) {
+        const float4 d = xp[0], sn = xp[1], X = xp[2], Gp = xp[3];
+        const float A = ((X.x * L.a[0] + X.y * L.a[1]) + X.z * L.a[2]) + X.w * L.a[3];
+        const float B = ((Gp.x * L.a[0] + Gp.y * L.a[1]) + Gp.z * L.a[2]) + Gp.w * L.a[3];
+        const float wx = d.x + L.gx, wy = d.y + L.gy, wz = d.z + L.gz;
         float il, dgdl;
-        float dgv = sg_term_bwd<DB>(wx, wy, wz, cD, il, dgdl);
-        float qx = x.snx + L.gx, qy = x.sny + L.gy, qz = x.snz + L.gz;
+        const float dgv = sg_term_bwd<DB>(wx, wy, wz, cD, il, dgdl);
+        const float qx = sn.x + L.gx, qy = sn.y + L.gy, qz = sn.z + L.gz;
         float isl, sgdl;
-        float sgv = sg_term_bwd<SB>(qx, qy, qz, L.lamL2E + x.ssL2E, isl, sgdl);
-        const float sgvsa = sgv * x.sa;
-#pragma unroll
-        for (int k = 0; k < 4; k++) { accX[k] += x.X[k] * dgv; accG[k] += x.G[k] * sgvsa; }
-        float cd = A * dgdl * il, cs = B * sgdl * isl;
-        accW[0] += cd * wx + cs * qx;
-        accW[1] += cd * wy + cs * qy;
-        accW[2] += cd * wz + cs * qz;
-        accS -= A * dgv + B * sgv;
+        const float sgv = sg_term_bwd<SB>(qx, qy, qz, L.lamL2E + d.w, isl, sgdl);
+        accX[0] += X.x * dgv; accX[1] += X.y * dgv; accX[2] += X.z * dgv; accX[3] += X.w * dgv;
+        accG[0] += Gp.x * sgv; accG[1] += Gp.y * sgv; accG[2] += Gp.z * sgv; accG[3] += Gp.w * sgv;
+        const float cd = A * dgdl * il, cs = B * sgdl * isl;
+        accW[0] += cd * wx; accW[0] += cs * qx;
+        accW[1] += cd * wy; accW[1] += cs * qy;
+        accW[2] += cd * wz; accW[2] += cs * qz;
     }
 }
 
@@ -1965,7 +2000,7 @@ struct KSgBwdLobesF {
         float acc[SG_COLS];
 #pragma unroll
         for (int k = 0; k < SG_COLS; k++) acc[k] = 0.f;
-        float accW[3] = {0.f, 0.f, 0.f}, accS = 0.f, accX[4] = {0.f, 0.f, 0.f, 0.f}, accG[4] = {0.f, 0.f, 0.f, 0.f};
+        float accW[3] = {0.f, 0.f, 0.f}, accX[4] = {0.f, 0.f, 0.f, 0.f}, accG[4] = {0.f, 0.f, 0.f, 0.f};
         // The pixel inputs of the next chunk are loaded while this chunk's lobe loop runs (and the
         // coverage word of the chunk after that), so the chunks' loads overlap the lobe work instead
         // of every workgroup waiting on them between chunks.
@@ -2000,10 +2035,10 @@ struct KSgBwdLobesF {
                         }
                 q.dnx = cur.nx * SG_DS; q.dny = cur.ny * SG_DS; q.dnz = cur.nz * SG_DS;
                 const float ss = cur.ss;
-                q.ssL2E = ss * L2E; q.sa = cur.sa;
+                q.ssL2E = ss * L2E; q.pad = 0.f;
                 q.snx = cur.ax * ss; q.sny = cur.ay * ss; q.snz = cur.az * ss;
 #pragma unroll
-                for (int k = 0; k < 4; k++) { q.X[k] = cur.go[k] * cur.df[k]; q.G[k] = cur.go[k]; }
+                for (int k = 0; k < 4; k++) { q.X[k] = cur.go[k] * cur.df[k] * SG_DA; q.G[k] = cur.go[k] * cur.sa; }
                 const float dlen = sqrtf(q.dnx * q.dnx + q.dny * q.dny + q.dnz * q.dnz);
                 const float slen = sqrtf(q.snx * q.snx + q.sny * q.sny + q.snz * q.snz);
                 for (int w = 0; w < nlw; w++) {
@@ -2024,21 +2059,31 @@ struct KSgBwdLobesF {
             u32 cneed = 0;
             for (int w = 0; w < BS / 64; w++) { if (w < wid) off += wcount[w]; ncov += wcount[w]; cneed |= wneed[w]; }
             if (cov) pix[off + wpos] = q;
+            // zero pixels up to a multiple of G (G divides BS, so the padded count stays <= BS)
+            const int ncovp = (ncov + G - 1) / G * G;
+            if (tid >= ncov && tid < ncovp) {
+                SgPix z;
+                z.dnx = z.dny = z.dnz = z.ssL2E = z.snx = z.sny = z.snz = z.pad = 0.f;
+#pragma unroll
+                for (int k = 0; k < 4; k++) { z.X[k] = 0.f; z.G[k] = 0.f; }
+                pix[tid] = z;
+            }
             __syncthreads();
             if (li < K) {
                 // The thread's lobe (axis mu, sharpness lam) is fixed, so the lobe-constant factors
                 // come out of the pixel sum: with cd, cs the diffuse / specular chain factors,
                 //   d/d(axis)      = lam * sum(cd w + cs q)
                 //   d/d(sharpness) = mu . sum(cd w + cs q) - sum(A dg + B sg)
-                //   d/d(amplitude) = DA * sum(X dg) + sum(G sg sa)
+                //   d/d(amplitude) = sum(X dg) + sum(G sg)   (X, G carry 2.34 and sa)
                 // (mu . w enters the sharpness term only through cd mu . w), and dg/dl =
                 // il ((e1 + e2) - g) for g = (e1 - e2) il.  The e2 choice is uniform per wave.
                 const SgLobe& L = lobes[li];
                 const bool db = (cneed >> (2 * lw)) & 1u, sb = (cneed >> (2 * lw + 1)) & 1u;
-                if (db && sb) sg_bwd_lobe_loop<true, true>(L, pix, grp, ncov, G, accW, accS, accX, accG);
-                else if (db) sg_bwd_lobe_loop<true, false>(L, pix, grp, ncov, G, accW, accS, accX, accG);
-                else if (sb) sg_bwd_lobe_loop<false, true>(L, pix, grp, ncov, G, accW, accS, accX, accG);
-                else sg_bwd_lobe_loop<false, false>(L, pix, grp, ncov, G, accW, accS, accX, accG);
+                const int nit = ncovp / G;
+                if (db && sb) sg_bwd_lobe_loop<true, true>(L, pix, grp, nit, G, accW, accX, accG);
+                else if (db) sg_bwd_lobe_loop<true, false>(L, pix, grp, nit, G, accW, accX, accG);
+                else if (sb) sg_bwd_lobe_loop<false, true>(L, pix, grp, nit, G, accW, accX, accG);
+                else sg_bwd_lobe_loop<false, false>(L, pix, grp, nit, G, accW, accX, accG);
             }
             __syncthreads();
             cur = nxt;
@@ -2048,9 +2093,15 @@ struct KSgBwdLobesF {
             acc[0] = accW[0] * L.lam;
             acc[1] = accW[1] * L.lam;
             acc[2] = accW[2] * L.lam;
+            // sum(A dg + B sg) = a . sum(X dg) + a . sum(G sg): A and B are the lobe's amplitudes
+            // dotted with the pixel's X and G, so the sharpness term's second sum comes from the
+            // amplitude sums rather than from two more products per pixel
+            float accS = 0.f;
+#pragma unroll
+            for (int k = 0; k < 4; k++) accS -= L.a[k] * (accX[k] + accG[k]);
             acc[3] = (L.mx * accW[0] + L.my * accW[1] + L.mz * accW[2]) + accS;
 #pragma unroll
-            for (int k = 0; k < 4; k++) acc[4 + k] = SG_DA * accX[k] + accG[k];
+            for (int k = 0; k < 4; k++) acc[4 + k] = accX[k] + accG[k];
         }
         if (!want_lobes) return;
 #pragma unroll
