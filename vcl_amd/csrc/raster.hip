@@ -13,9 +13,9 @@
 //   3. bin_large  : one wave per listed piece counts its tiles.
 //   4. scan       : one workgroup: exclusive prefix of the tile counters (bin offsets) and the
 //                   fine-raster work list (each bin cut into chunks of up to 256 pieces).
-//   5. fill       : small pieces written into their tiles' bins (LDS-aggregated reservation),
-//                   then large pieces by one wave each.  A tile whose bin would pass the bin
-//                   capacity scans every piece instead.
+//   5. fill       : small pieces written into their tiles' bins (LDS-aggregated reservation)
+//                   and, in the same launch, large pieces by one wave each.  A tile whose bin
+//                   would pass the bin capacity scans every piece instead.
 //   6. fine       : one 256-thread workgroup per chunk; each thread walks the exact covered
 //                   row spans of one piece and takes the LDS minimum of the (ordered z_ndc,
 //                   triangle id) key per pixel; a tile split over several chunks merges the
@@ -602,7 +602,19 @@ struct KFillF {
             if (sl[i] >= 0) bin_write(a, g[i], tab.val[sl[i]] + loc[i], (u32)slot);
     }
 };
-__global__ void __launch_bounds__(256) k_fill(RastArgs a, size_t nslots) { KFillF{}(vb_hw(), a, nslots); }
+
+// The small- and large-piece bin fills in one launch: workgroups [0, nfill) run the small-piece
+// fill, the others the large-piece fill.  They write disjoint parts of each bin (the large pieces
+// its bottom through the sub-counter cursors, the small ones its top, counting down), so the two
+// overlap on the device instead of one waiting for the other's last workgroups.
+struct KFillAllF {
+    static constexpr int kBS = 256, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, RastArgs a, size_t nslots, u32 nfill) const {
+        if (vb.b.x < nfill) KFillF{}(VB{vb.b, {nfill, vb.g.y, vb.g.z}}, a, nslots);
+        else KBinLargeF<true>{}(VB{{vb.b.x - nfill, vb.b.y, vb.b.z}, {vb.g.x - nfill, vb.g.y, vb.g.z}}, a);
+    }
+};
+__global__ void __launch_bounds__(256) k_fill_all(RastArgs a, size_t nslots, u32 nfill) { KFillAllF{}(vb_hw(), a, nslots, nfill); }
 
 constexpr u32 SPAN_EMPTY = 0x00ffu;   // first column 255 > last column 0
 
@@ -1058,12 +1070,11 @@ int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_st
     VCL_LAUNCH_CHECK();
     vlaunch<KScanF>(k_scan, dim3(1), dim3(1024), 0, s, a, L.NT);
     VCL_LAUNCH_CHECK();
-    if (L.nrec_cap > 0) {
-        vlaunch<KFillF>(k_fill, dim3(cdiv(L.nrec_cap, 256)), dim3(256), 0, s, a, L.nrec_cap);
+    {
+        const u32 nfill = (u32)cdiv(L.nrec_cap, 256);
+        vlaunch<KFillAllF>(k_fill_all, dim3(nfill + LARGE_BLOCKS), dim3(256), 0, s, a, L.nrec_cap, nfill);
         VCL_LAUNCH_CHECK();
     }
-    vlaunch<KBinLargeF<true>>(k_bin_large<true>, dim3(LARGE_BLOCKS), dim3(256), 0, s, a);
-    VCL_LAUNCH_CHECK();
     const dim3 fg((unsigned)std::min(L.chunk_cap, (size_t)FINE_MAX_GRID));
     if (tshift == 3) vlaunch<KFineF<8>>(k_fine<8>, fg, dim3(256), 0, s, a);
     else if (tshift == 5) vlaunch<KFineF<32>>(k_fine<32>, fg, dim3(256), 0, s, a);
