@@ -25,6 +25,7 @@
 // is compiled without FP contraction so both produce identical triangle IDs.
 #pragma clang fp contract(off)
 #include "internal.hpp"
+#include "raster_span.hpp"
 #include <algorithm>
 
 namespace {
@@ -618,34 +619,6 @@ __global__ void __launch_bounds__(256) k_fill_all(RastArgs a, size_t nslots, u32
 
 constexpr u32 SPAN_EMPTY = 0x00ffu;   // first column 255 > last column 0
 
-// Covered pixel columns of one piece in one row of a tile.  Edge i passes at column x (pixel
-// centre X = X0 + 256 x) iff K_i + dA_i x > 0, with K_i = A_i X0 + B_i Y + C_i + tl_i (the
-// integer form of "E > 0, or E == 0 on a top-left edge") and dA_i = 256 A_i; each edge is
-// monotone along the row, so its boundary is found by bisection with exact int64 tests.
-__device__ __forceinline__ bool edge_in(long long K, long long dA, int x) { return K + dA * x > 0; }
-
-__device__ __forceinline__ void row_span(const long long* K, const long long* dA, int& lo, int& hi) {
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        if (lo > hi) return;
-        if (dA[i] == 0) {
-            if (K[i] <= 0) { lo = 1; hi = 0; }
-        } else if (dA[i] > 0) {   // false ... true: first passing column
-            if (edge_in(K[i], dA[i], lo)) continue;
-            if (!edge_in(K[i], dA[i], hi)) { lo = 1; hi = 0; continue; }
-            int l = lo, h = hi;   // pred(l) false, pred(h) true
-            while (h - l > 1) { int m = (l + h) >> 1; if (edge_in(K[i], dA[i], m)) h = m; else l = m; }
-            lo = h;
-        } else {                  // true ... false: last passing column
-            if (edge_in(K[i], dA[i], hi)) continue;
-            if (!edge_in(K[i], dA[i], lo)) { lo = 1; hi = 0; continue; }
-            int l = lo, h = hi;   // pred(l) true, pred(h) false
-            while (h - l > 1) { int m = (l + h) >> 1; if (edge_in(K[i], dA[i], m)) l = m; else h = m; }
-            hi = l;
-        }
-    }
-}
-
 // Winner of one pixel -> (u, v, z/w, id+1) and DB, written with 16-byte stores.
 __device__ void fine_resolve(const RastArgs& a, u32 view, int px, int py, float pixx, float pixy, u64 best) {
     if (px >= a.W || py >= a.H) return;
@@ -820,19 +793,28 @@ struct KFineF {
                     yl = max((r.pyr & 0xffff) - y0, 0);
                     yh = min((r.pyr >> 16) - y0, TS - 1);
                     {
-                        long long K[3], dA[3], dB[3];
-                        long long Y = (long long)(y0 + yl) * 256 + 128;
+                        // exact spans (row_span_exact): the edge functions at the first row, their
+                        // steps per column and row, and the crossings' estimates -K / dA (first row)
+                        // and their change per row
+                        SpanEdges e;
+                        double dB[3], t0[3], ts[3];
+                        const long long Y = (long long)(y0 + yl) * 256 + 128;
 #pragma unroll
                         for (int i = 0; i < 3; i++) {
-                            K[i] = (long long)r.A[i] * X0 + (long long)r.B[i] * Y + r.C[i] + ((vtl >> i) & 1u);
-                            dA[i] = (long long)r.A[i] * 256;
-                            dB[i] = (long long)r.B[i] * 256;
+                            e.K[i] = (double)((long long)r.A[i] * X0 + (long long)r.B[i] * Y + r.C[i] + ((vtl >> i) & 1u));
+                            e.dA[i] = (double)r.A[i] * 256.0;
+                            dB[i] = (double)r.B[i] * 256.0;
+                            const double rA = e.dA[i] != 0.0 ? 1.0 / e.dA[i] : 0.0;
+                            t0[i] = -e.K[i] * rA;
+                            ts[i] = -dB[i] * rA;
                         }
                         for (int y = yl; y <= yh; y++) {
                             int lo = xl, hi = xh;
-                            row_span(K, dA, lo, hi);
 #pragma unroll
-                            for (int i = 0; i < 3; i++) K[i] += dB[i];
+                            for (int i = 0; i < 3; i++) e.t[i] = fma(ts[i], (double)(y - yl), t0[i]);
+                            row_span_exact(e, lo, hi);
+#pragma unroll
+                            for (int i = 0; i < 3; i++) e.K[i] += dB[i];
                             sspan[tid][y] = lo <= hi ? (unsigned short)(lo | (hi << 8)) : (unsigned short)SPAN_EMPTY;
                             nrow += lo <= hi ? 1u : 0u;
                         }
