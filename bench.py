@@ -117,7 +117,8 @@ def algorithmic_bytes(sc, views=1):
         "shade_prep": P * (16 + 4 + 12 + 12 + 12 + 4 + 4) + V * 12 + T * 12,
         "texture": P * (4 + 8 + 16 + 4 * C),
         "spherical_gaussian": P * 100 + K * 32,
-        "antialias": P * (8 + 8 * C) + views * V * 16 + T * 12,
+        # the fit step's antialias pass also writes the L2 loss's grad (reads the target)
+        "antialias": P * (8 + 16 * C) + views * V * 16 + T * 12,
         "mse_loss": P * 12 * C,
         "antialias_bwd": P * (8 + 12 * C),
         "spherical_gaussian_bwd": P * (52 + 8 * C) + K * 64,

@@ -93,7 +93,9 @@ static Plan plan(const vcl_fit_desc* d) {
     L.add("loss", 4 * f4);
     L.add("ws_rast", vcl_rasterize_workspace_size(d->width, d->height, nv, V, Tn));
     L.add("ws_sg", vcl_spherical_gaussian_workspace_size(d->width, d->height, nv, C, K));
-    L.add("ws_loss", vcl_loss_workspace_size((int)(P * C)));
+    // the loss partials: one per 1,024 elements (the MSE pass) or one per antialias tile (the MSE
+    // fused into the antialias forward)
+    L.add("ws_loss", std::max(vcl_loss_workspace_size((int)(P * C)), vcl_antialias_loss_parts(d->width, d->height, nv) * sizeof(float) + 256));
     return L;
 }
 
@@ -365,7 +367,11 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     { ProfScope ps_(F, "spherical_gaussian", as_hip(s)); st = vcl_spherical_gaussian_forward(&gp, s); }
     if (st) return st;
     vcl_antialias_params ap = {W, H, nv, C, {W / 2.f, H / 2.f}, F->f("clip"), F->t<uint32_t>("idx"), F->f("rast"), F->f("color"), F->f("image"), d.V};
-    { ProfScope ps_(F, "antialias", as_hip(s)); st = vcl_antialias_forward(&ap, s); }
+    // the fused step's L2 loss grad and tile partials come from the antialias pass (the op-by-op
+    // sequence runs the loss as its own pass)
+    const bool loss_fused = step && !d.op_by_op;
+    { ProfScope ps_(F, "antialias", as_hip(s));
+      st = loss_fused ? vcl_antialias_forward_dev(&ap, F->f("target"), F->f("gimage"), F->f("ws_loss"), s) : vcl_antialias_forward(&ap, s); }
     if (st) return st;
     return VCL_OK;
 }
@@ -402,9 +408,13 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     double* bt = F->t<double>("adam_bt");
     if ((st = forward(F, s, true))) return st;
     vcl_loss_params lp = {F->f("image"), F->f("target"), F->f("gimage"), F->f("ws_loss"), W, H, nv, (int)(F->P * C), F->f("loss")};
-    // the loss's final sum runs in the Adam launch at the end of the step
-    { ProfScope ps_(F, "mse_loss", as_hip(s)); st = vcl_mse_loss_grad_dev(&lp, s); }
-    if (st) return st;
+    // the loss's final sum runs in the Adam launch at the end of the step, over the antialias
+    // pass's tile partials (fused step) or the MSE pass's
+    const long long loss_parts = d.op_by_op ? -1 : (long long)vcl_antialias_loss_parts(W, H, nv);
+    if (d.op_by_op) {
+        { ProfScope ps_(F, "mse_loss", as_hip(s)); st = vcl_mse_loss_grad_dev(&lp, s); }
+        if (st) return st;
+    }
     // backward
     vcl_antialias_params ap = {W, H, nv, C, {W / 2.f, H / 2.f}, F->f("clip"), F->t<uint32_t>("idx"), F->f("rast"), F->f("color"), F->f("image"), d.V};
     vcl_antialias_grad_params ag = {F->f("gclip"), F->f("gcolor"), F->f("gimage")};
@@ -490,7 +500,9 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
         {d.beta1, d.beta2, 0.0, 0.0, d.lr, d.eps, F->f("m_amp"), F->f("v_amp")}};
     // the parameter-grad clears (AttributeGrad::clear, SGBufferGrad::clear) happen inside Adam
     // pj.out: the clip-space grads the folded project backward takes
-    { ProfScope ps_(F, "adam", as_hip(s)); st = vcl_adam_step_multi_proj_dev(K > 0 ? 4 : 1, op, hp, bt, 1, fold_project ? &pj : nullptr, &lp, lobes.seg ? &lobes : nullptr, s); }
+    { ProfScope ps_(F, "adam", as_hip(s));
+      st = vcl_adam_step_multi_proj_dev(K > 0 ? 4 : 1, op, hp, bt, 1, fold_project ? &pj : nullptr, &lp, lobes.seg ? &lobes : nullptr, s,
+                                        loss_parts); }
     if (st) return st;
     return VCL_OK;
 }

@@ -62,9 +62,14 @@ int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, 
 // loss->buffer into *loss->loss (one more workgroup; n <= 8 tensors).  lobes (may be null, or
 // with seg null): tensors 1..3 are the lobe axis, sharpness and amplitude, whose grads first
 // take those sums.
+// loss_parts >= 0: the number of partials in loss->buffer (else one per 1,024 elements, k_mse's).
 int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
                                  const vcl_project_params* proj, const vcl_loss_params* loss, const vcl_sg_lobe_sums* lobes,
-                                 vcl_stream_t stream);
+                                 vcl_stream_t stream, long long loss_parts = -1);
+// vcl_antialias_forward that, given target, gout and part, also writes the L2 loss's grad
+// out - target and one sum of squared grads per 16 x 16 tile (vcl_antialias_loss_parts of them).
+int vcl_antialias_forward_dev(const vcl_antialias_params* p, const float* target, float* gout, float* part, vcl_stream_t stream);
+size_t vcl_antialias_loss_parts(int width, int height, int depth);
 // vcl_mse_loss_backward without the final sum: the grad and the per-block partials only.
 int vcl_mse_loss_grad_dev(const vcl_loss_params* p, vcl_stream_t stream);
 // bt_dev[0] *= rhom; bt_dev[1] *= rhov (one thread; the step-start bump of Adam::step).

@@ -2238,9 +2238,13 @@ __device__ __forceinline__ int aa_pair_slot(int tx, int ty, int k) {
 
 struct KAaFwdF {
     static constexpr int kBS = BS, kMINW = 1;
-    __device__ __forceinline__ void operator()(const VB& vb, AAArgs a, int tilesX, const float* in, float* out) const {
+    // target, gout, part (all or none): the L2 loss's grad out - target per pixel and the tile's
+    // sum of squared grads (part[layer * tiles + tile]), the MSE pass fused into the antialias
+    __device__ __forceinline__ void operator()(const VB& vb, AAArgs a, int tilesX, const float* in, float* out, const float* target, float* gout,
+                                               float* part) const {
 #pragma clang fp contract(off)
         __shared__ float4 to_cur[AA_PAIRS], to_nb[AA_PAIRS];
+        __shared__ float wsq[BS / 64];
         const int s = vb.b.y;
         const unsigned tile = xcd_swizzle(vb.b.x, vb.g.x, (unsigned)tilesX);
         const int x0 = (tile % tilesX) * AA_T, y0 = (tile / tilesX) * AA_T;
@@ -2248,6 +2252,13 @@ struct KAaFwdF {
         const float4* rastv = (const float4*)a.rast + (size_t)s * a.W * a.H;
         const float* inv = in + (size_t)s * a.W * a.H * a.C;
         const int C = a.C;
+        const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
+        const int px = x0 + tx, py = y0 + ty;
+        const bool inside = px < a.W && py < a.H;
+        const size_t q = inside ? (size_t)py * a.W + px : 0;
+        // the own pixel's loss target, loaded first so that it arrives during the pair work
+        float tg[AA_MAXC] = {0.f, 0.f, 0.f, 0.f};
+        if (gout && inside) aa_load_px(target + (size_t)s * a.W * a.H * C, q, C, a.v4, tg);
         for (int p = threadIdx.x; p < AA_PAIRS; p += BS) {
             int cx, cy, vertical;
             aa_tile_pair(p, x0, y0, cx, cy, vertical);
@@ -2279,26 +2290,42 @@ struct KAaFwdF {
             to_nb[p] = make_float4(tn[0], tn[1], tn[2], tn[3]);
         }
         __syncthreads();
-        const int tx = threadIdx.x % AA_T, ty = threadIdx.x / AA_T;
-        const int px = x0 + tx, py = y0 + ty;
-        if (px >= a.W || py >= a.H) return;
-        size_t q = (size_t)py * a.W + px;
-        float acc[AA_MAXC];
-        aa_load_px(inv, q, C, a.v4, acc);
+        float sq = 0.f;
+        if (inside) {
+            float acc[AA_MAXC];
+            aa_load_px(inv, q, C, a.v4, acc);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float4 v = k < 2 ? to_cur[aa_pair_slot(tx, ty, k)] : to_nb[aa_pair_slot(tx, ty, k)];
-            float vv[4] = {v.x, v.y, v.z, v.w};
+            for (int k = 0; k < 4; k++) {
+                float4 v = k < 2 ? to_cur[aa_pair_slot(tx, ty, k)] : to_nb[aa_pair_slot(tx, ty, k)];
+                float vv[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
-            for (int c = 0; c < AA_MAXC; c++) acc[c] += vv[c];
+                for (int c = 0; c < AA_MAXC; c++) acc[c] += vv[c];
+            }
+            float* o = out + ((size_t)s * a.W * a.H + q) * C;
+#pragma unroll
+            for (int c = 0; c < AA_MAXC; c++)
+                if (c < C) o[c] = acc[c];
+            if (gout) {
+                float* g = gout + ((size_t)s * a.W * a.H + q) * C;
+#pragma unroll
+                for (int c = 0; c < AA_MAXC; c++)
+                    if (c < C) {
+                        const float d = acc[c] - tg[c];
+                        g[c] = d;
+                        sq += d * d;
+                    }
+            }
         }
-        float* o = out + ((size_t)s * a.W * a.H + q) * C;
-#pragma unroll
-        for (int c = 0; c < AA_MAXC; c++)
-            if (c < C) o[c] = acc[c];
+        if (!part) return;
+        sq = wave_sum(sq);
+        if ((threadIdx.x & 63) == 0) wsq[threadIdx.x >> 6] = sq;
+        __syncthreads();
+        if (threadIdx.x == 0) part[(size_t)s * vb.g.x + tile] = ((wsq[0] + wsq[1]) + wsq[2]) + wsq[3];
     }
 };
-__global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float* in, float* out) { KAaFwdF{}(vb_hw(), a, tilesX, in, out); }
+__global__ void __launch_bounds__(BS) k_aa_fwd(AAArgs a, int tilesX, const float* in, float* out, const float* target, float* gout, float* part) {
+    KAaFwdF{}(vb_hw(), a, tilesX, in, out, target, gout, part);
+}
 
 // Backward of the tiled antialias.  Per pair, D = sum over its edges of alpha * gout[target]
 // goes to the front pixel pp with + and to the other pixel op with -; the vertex (clip-space)
@@ -2690,12 +2717,12 @@ extern "C" int vcl_texture_grad_clear(const vcl_texture* g, vcl_stream_t stream)
 // bt_dev: device {rhomt, rhovt} used instead of the per-tensor host values (null: host values).
 extern "C" int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev,
                                        int zero_grad, vcl_stream_t stream) {
-    return vcl_adam_step_multi_proj_dev(n, p, h, bt_dev, zero_grad, nullptr, nullptr, nullptr, stream);
+    return vcl_adam_step_multi_proj_dev(n, p, h, bt_dev, zero_grad, nullptr, nullptr, nullptr, stream, -1);
 }
 
 int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
                                  const vcl_project_params* proj, const vcl_loss_params* loss, const vcl_sg_lobe_sums* lobes,
-                                 vcl_stream_t stream) {
+                                 vcl_stream_t stream, long long loss_parts) {
     if (loss) VCL_REQUIRE(loss->buffer && loss->loss && loss->size >= 0 && n <= ADAM_MAX);
     if (lobes) {
         // tensors 1..3: the lobe axis (K x 3), sharpness (K) and amplitude (K x C) grads
@@ -2718,7 +2745,7 @@ int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_
         a.pV = proj ? proj->vecNum : 0;
         a.loss = base == 0 && loss ? loss->loss : nullptr;
         a.loss_part = loss ? loss->buffer : nullptr;
-        a.loss_nb = loss && loss->size > 0 ? cdiv((size_t)loss->size, (size_t)BS * 4) : 0;
+        a.loss_nb = !loss ? 0 : loss_parts >= 0 ? (size_t)loss_parts : loss->size > 0 ? cdiv((size_t)loss->size, (size_t)BS * 4) : 0;
         a.lseg = base == 0 && lobes ? lobes->seg : nullptr;
         a.lnseg = lobes ? lobes->nseg : 0;
         a.lK = lobes ? lobes->K : 0;
@@ -3029,14 +3056,25 @@ static AAArgs aa_args(const vcl_antialias_params* p, const float* gout) {
     return a;
 }
 
-extern "C" int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t stream) {
+size_t vcl_antialias_loss_parts(int width, int height, int depth) {
+    return (size_t)((width + AA_T - 1) / AA_T) * ((height + AA_T - 1) / AA_T) * (size_t)depth;
+}
+
+int vcl_antialias_forward_dev(const vcl_antialias_params* p, const float* target, float* gout, float* part, vcl_stream_t stream) {
     VCL_REQUIRE(p && p->proj && p->idx && p->rast && p->in && p->out && p->channel >= 1 && p->channel <= AA_MAXC && p->depth <= 65535);
+    VCL_REQUIRE((!target && !gout && !part) || (target && gout && part));
     size_t P = (size_t)p->width * p->height * p->depth;
     if (!P) return VCL_OK;
     int tilesX = (p->width + AA_T - 1) / AA_T, tilesY = (p->height + AA_T - 1) / AA_T;
-    vlaunch<KAaFwdF>(k_aa_fwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), aa_args(p, nullptr), tilesX, p->in, p->out);
+    AAArgs a = aa_args(p, nullptr);
+    if (gout) a.v4 = a.v4 && ((((uintptr_t)target) | (uintptr_t)gout) & 15) == 0;
+    vlaunch<KAaFwdF>(k_aa_fwd, dim3(tilesX * tilesY, p->depth), dim3(BS), 0, as_hip(stream), a, tilesX, p->in, p->out, target, gout, part);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
+}
+
+extern "C" int vcl_antialias_forward(const vcl_antialias_params* p, vcl_stream_t stream) {
+    return vcl_antialias_forward_dev(p, nullptr, nullptr, nullptr, stream);
 }
 
 extern "C" int vcl_antialias_backward(const vcl_antialias_params* p, const vcl_antialias_grad_params* g, vcl_stream_t stream) {
