@@ -486,7 +486,9 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     fill_texture(F, "gtex", gtex);
     { ProfScope ps_(F, "texture_adam", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 1 | 4, bt, s); }
     if (st) return st;
-    { ProfScope ps_(F, "texture_mip", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 2 | 4, bt, s); }
+    // the mip rebuild's last pass, when it is one workgroup, runs in the Adam launch below
+    VclMipTail mtail;
+    { ProfScope ps_(F, "texture_mip", as_hip(s)); st = vcl_texture_adam_update_parts(&tex, &gtex, &ht, 0.f, 1.f, d.fold_quarter, 2 | 4, bt, s, &mtail); }
     if (st) return st;
     vcl_optimize_params op[4] = {
         {F->f("pos"), F->f("grad_pos"), d.V * 3, d.V, 3, 1},
@@ -502,7 +504,7 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     // pj.out: the clip-space grads the folded project backward takes
     { ProfScope ps_(F, "adam", as_hip(s));
       st = vcl_adam_step_multi_proj_dev(K > 0 ? 4 : 1, op, hp, bt, 1, fold_project ? &pj : nullptr, &lp, lobes.seg ? &lobes : nullptr, s,
-                                        loss_parts); }
+                                        loss_parts, &mtail); }
     if (st) return st;
     return VCL_OK;
 }

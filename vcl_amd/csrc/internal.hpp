@@ -8,6 +8,16 @@
 // texture-backward entry points.
 extern vcl_tuning g_vcl_tuning;
 
+// The last pass of a C == 4 mip rebuild (levels l .. l + nl - 1 from the level above, of
+// sw x sh texels), when it is a single workgroup's.
+struct VclMipTail {
+    int valid;
+    const float* src;
+    int sw, sh, nl;
+    float* dst[6];
+    float* zg[6];
+};
+
 extern "C" {
 // ViewAxis forward with the per-view inverse matrices already on the device (depth x 9 floats).
 int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream);
@@ -37,8 +47,10 @@ int vcl_texture_interp_backward_dev(const vcl_texturemap_params* tp, const vcl_t
 // into the level-0 update (one 2 x 2 quad per thread), so the two calls of a split update must
 // both pass it.  bt_dev: device {rhomt, rhovt} used instead of h->rhomt/rhovt (null: the host
 // values).
+// tail (may be null): a last mip pass of a single workgroup is not launched but described here
+// (valid = 1) for the caller's Adam launch to run (vcl_adam_step_multi_proj_dev).
 int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
-                                  int quarter, int part, const double* bt_dev, vcl_stream_t stream);
+                                  int quarter, int part, const double* bt_dev, vcl_stream_t stream, VclMipTail* tail = nullptr);
 // vcl_adam_step_multi with the bias-correction powers read from the device (bt_dev, may be null).
 // zero_grad: clear each grad after it is used.
 int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
@@ -63,9 +75,10 @@ int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, 
 // with seg null): tensors 1..3 are the lobe axis, sharpness and amplitude, whose grads first
 // take those sums.
 // loss_parts >= 0: the number of partials in loss->buffer (else one per 1,024 elements, k_mse's).
+// mip_tail (may be null or not valid): a last mip pass the launch also runs (one workgroup more).
 int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
                                  const vcl_project_params* proj, const vcl_loss_params* loss, const vcl_sg_lobe_sums* lobes,
-                                 vcl_stream_t stream, long long loss_parts = -1);
+                                 vcl_stream_t stream, long long loss_parts = -1, const VclMipTail* mip_tail = nullptr);
 // vcl_antialias_forward that, given target, gout and part, also writes the L2 loss's grad
 // out - target and one sum of squared grads per 16 x 16 tile (vcl_antialias_loss_parts of them).
 int vcl_antialias_forward_dev(const vcl_antialias_params* p, const float* target, float* gout, float* part, vcl_stream_t stream);

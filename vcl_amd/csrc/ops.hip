@@ -1246,6 +1246,21 @@ struct KMipFusedF {
 };
 __global__ void __launch_bounds__(BS) k_mip_fused(MipArgs a) { KMipFusedF{}(vb_hw(), a); }
 
+// The Adam launch with one more workgroup that runs the mip rebuild's last pass when that pass
+// is a single workgroup (the coarsest levels, from a source level of at most 64 x 64 texels),
+// which then costs no launch of its own.
+struct KAdamMipF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, AdamArgs a, MipArgs m) const {
+        if (vb.b.x == vb.g.x - 1) {
+            KMipFusedF{}(VB{{0u, 0u, 0u}, {1u, 1u, 1u}}, m);
+            return;
+        }
+        KAdamF{}(VB{vb.b, {vb.g.x - 1, vb.g.y, vb.g.z}}, a);
+    }
+};
+__global__ void __launch_bounds__(BS) k_adam_mip(AdamArgs a, MipArgs m) { KAdamMipF{}(vb_hw(), a, m); }
+
 // ---------------------------------------------------------------- NormalAxis / ViewAxis
 // NormalAxis of one pixel: interpolated vertex normal, normalised, rotated (zero off-mesh).
 __device__ __forceinline__ void normal_axis_px(float4 r, const float* rot, const float* normal, const u32* nidx, float& o0, float& o1, float& o2) {
@@ -2717,12 +2732,12 @@ extern "C" int vcl_texture_grad_clear(const vcl_texture* g, vcl_stream_t stream)
 // bt_dev: device {rhomt, rhovt} used instead of the per-tensor host values (null: host values).
 extern "C" int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev,
                                        int zero_grad, vcl_stream_t stream) {
-    return vcl_adam_step_multi_proj_dev(n, p, h, bt_dev, zero_grad, nullptr, nullptr, nullptr, stream, -1);
+    return vcl_adam_step_multi_proj_dev(n, p, h, bt_dev, zero_grad, nullptr, nullptr, nullptr, stream, -1, nullptr);
 }
 
 int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_adam_hyper_params* h, const double* bt_dev, int zero_grad,
                                  const vcl_project_params* proj, const vcl_loss_params* loss, const vcl_sg_lobe_sums* lobes,
-                                 vcl_stream_t stream, long long loss_parts) {
+                                 vcl_stream_t stream, long long loss_parts, const VclMipTail* mip_tail) {
     if (loss) VCL_REQUIRE(loss->buffer && loss->loss && loss->size >= 0 && n <= ADAM_MAX);
     if (lobes) {
         // tensors 1..3: the lobe axis (K x 3), sharpness (K) and amplitude (K x C) grads
@@ -2763,7 +2778,15 @@ int vcl_adam_step_multi_proj_dev(int n, const vcl_optimize_params* p, const vcl_
             blocks += (q.size + (long long)BS * a.per_thread[i] - 1) / ((long long)BS * a.per_thread[i]);
         }
         a.block_start[a.n] = blocks;
-        if (blocks > 0 || a.loss) {
+        const bool tail = base == 0 && mip_tail && mip_tail->valid;
+        if (tail) {
+            MipArgs m;
+            m.src = (const float4*)mip_tail->src;
+            m.sw = mip_tail->sw; m.sh = mip_tail->sh; m.nl = mip_tail->nl;
+            for (int k = 0; k < 6; k++) { m.dst[k] = (float4*)mip_tail->dst[k]; m.zg[k] = (float4*)mip_tail->zg[k]; }
+            vlaunch<KAdamMipF>(k_adam_mip, dim3((unsigned)(blocks + (a.loss ? 1 : 0) + 1)), dim3(BS), 0, as_hip(stream), a, m);
+            VCL_LAUNCH_CHECK();
+        } else if (blocks > 0 || a.loss) {
             vlaunch<KAdamF>(k_adam, dim3((unsigned)(blocks + (a.loss ? 1 : 0))), dim3(BS), 0, as_hip(stream), a);
             VCL_LAUNCH_CHECK();
         }
@@ -2780,8 +2803,9 @@ extern "C" int vcl_adam_step(const vcl_optimize_params* p, const vcl_adam_hyper_
 }
 
 // part bit 1: fold + Adam + clamp on level 0; part bit 2: rebuild mips (and clear coarse grads).
-extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo,
-                                             float hi, int quarter, int part, const double* bt_dev, vcl_stream_t stream) {
+int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
+                                  int quarter, int part, const double* bt_dev, vcl_stream_t stream, VclMipTail* tail) {
+    if (tail) tail->valid = 0;
     VCL_REQUIRE(tex && grad && h && h->m && h->v && tex->miplevel == grad->miplevel && tex->width == grad->width && tex->height == grad->height &&
                 tex->channel == grad->channel && tex->miplevel >= 1 && tex->miplevel <= VCL_TEX_MAX_MIP_LEVEL);
     TexUpdArgs a;
@@ -2833,6 +2857,13 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
                 m.dst[k] = k < m.nl ? (float4*)tex->texture[l + k] : nullptr;
                 m.zg[k] = k < m.nl ? (float4*)a.grad[l + k] : nullptr;
             }
+            if (tail && l + m.nl >= a.L && m.sw <= 64 && m.sh <= 64) {
+                // the last pass is one workgroup: the caller runs it (in its Adam launch)
+                tail->valid = 1;
+                tail->src = (const float*)m.src; tail->sw = m.sw; tail->sh = m.sh; tail->nl = m.nl;
+                for (int k = 0; k < 6; k++) { tail->dst[k] = (float*)m.dst[k]; tail->zg[k] = (float*)m.zg[k]; }
+                break;
+            }
             vlaunch<KMipFusedF>(k_mip_fused, dim3(cdiv(m.sw, 64), cdiv(m.sh, 64)), dim3(BS), 0, s, m);
             VCL_LAUNCH_CHECK();
             l += m.nl;
@@ -2851,7 +2882,7 @@ extern "C" int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_t
 
 extern "C" int vcl_texture_adam_update(const vcl_texture* tex, const vcl_texture* grad, const vcl_adam_hyper_params* h, float lo, float hi,
                                        int quarter, vcl_stream_t stream) {
-    return vcl_texture_adam_update_parts(tex, grad, h, lo, hi, quarter, 3 | 4, nullptr, stream);
+    return vcl_texture_adam_update_parts(tex, grad, h, lo, hi, quarter, 3 | 4, nullptr, stream, nullptr);
 }
 
 extern "C" int vcl_normal_axis_forward(const vcl_normal_axis_params* p, vcl_stream_t stream) {
