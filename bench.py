@@ -115,6 +115,8 @@ def algorithmic_bytes(sc, views=1):
         "interpolate": P * 56 + V * 8 + T * 12,
         "interp_texture": P * (16 + 16 + 8 + 16 + 4 * C) + V * 8 + T * 12,
         "shade_prep": P * (16 + 4 + 12 + 12 + 12 + 4 + 4) + V * 12 + T * 12,
+        # the fit step's fused pass: interp_texture's and shade_prep's surfaces, rast read once
+        "pixel_prep": P * (16 + 16 + 4 + 8 + 16 + 4 * C + 12 + 12 + 12 + 4 + 4) + V * 20 + T * 12,
         "texture": P * (4 + 8 + 16 + 4 * C),
         "spherical_gaussian": P * 100 + K * 32,
         # the fit step's antialias pass also writes the L2 loss's grad (reads the target)
@@ -364,18 +366,18 @@ def kernel_roofline_entries(sc, views, kernels, ab, value, U):
         k = kernels.get(op)
         return k["avg_ms"] if k else None
 
-    for op, kern in (("texture_adam", "k_tex_update4m"), ("interp_texture", "k_interp_tex4"), ("antialias_bwd", "k_aa_bwd"),
+    for op, kern in (("texture_adam", "k_tex_update4m"), ("pixel_prep", "k_pixel_prep4"), ("antialias_bwd", "k_aa_bwd"),
                      ("antialias", "k_aa_fwd")):
         ms = t(op)
         if not ms:
             continue
-        alg = ab[op] + (16 * U if op == "interp_texture" else 0)
+        alg = ab[op] + (16 * U if op == "pixel_prep" else 0)
         tr = measured_traffic(kern)
         out.append({"kernel": kern, "op": op, "bound": "hbm", "avg_ms": ms, "alg_bytes": alg,
                     "achieved": round(alg / (ms * 1e-3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                     "frac": round(alg / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4), "traffic": tr,
                     "traffic_frac": round(tr / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if tr else None,
-                    "note": "alg_bytes include 16 B per touched texel" if op == "interp_texture" else None})
+                    "note": "alg_bytes include 16 B per touched texel" if op == "pixel_prep" else None})
     ms = t("texture_interp_bwd")
     n_at = cv.get("plain k_tex_bwd4_tiled<16> l2_atomics_per_launch")
     if ms and n_at and apeak:

@@ -4,13 +4,16 @@
 // loss, the rotation's gradient flowing through Project -> Camera -> Rotation.  The GLUT window
 // becomes the headless vcl::Window (GLbuffer::draw composites, Window::save writes a BMP).
 //
-// usage: preset_pose <mesh.obj> <out.bin> <steps> <exact 0|1> <image.bmp> [offset]
+// usage: preset_pose <mesh.obj> <out.bin> <steps> <exact 0|1> <image.bmp> [offset [seed]]
+// exact 1: the exact Camera / Rotation derivatives and the exact SSIM gradient (the derivative of
+//   the loss the program reports) instead of the reference's formulas.
 // offset (radians, optional): the fitted rotation starts at the target's rotated by this angle
-//   about a fixed axis instead of at a second random draw (a start the fit can reach whatever
-//   rand() returns; the runtime's own threads may draw from the same generator).
+//   about a fixed axis instead of at a second random draw.
+// seed (optional): the rand() seed of the random rotations (default 12345).
 // out.bin: float32 loss per step; then the first step's quaternions (x, y, z, w) of the target
 //   and of the fitted rotation; then the last rendered image [512*512*3] and the target image;
-//   then the fitted rotation after the last step.
+//   then the fitted rotation after the last step; then the rotation's gradient (x, y, z, w) of
+//   the first step.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -20,12 +23,13 @@
 using namespace vcl;
 
 int main(int argc, char** argv) {
-    if (argc < 6) { std::fprintf(stderr, "usage: %s mesh.obj out.bin steps exact image.bmp [offset]\n", argv[0]); return 2; }
+    if (argc < 6) { std::fprintf(stderr, "usage: %s mesh.obj out.bin steps exact image.bmp [offset [seed]]\n", argv[0]); return 2; }
     const char* mesh = argv[1];
     int steps = std::atoi(argv[3]);
     bool exact = std::atoi(argv[4]) != 0;
     const float offset = argc > 6 ? (float)std::atof(argv[6]) : 0.f;
-    std::srand(12345);  // the reference seeds with the clock (main.cpp:21); fixed here
+    // the reference seeds with the clock (main.cpp:21); fixed here
+    std::srand(argc > 7 ? (unsigned)std::atoi(argv[7]) : 12345u);
 
     const int resolution = 512, INIT_INTERVAL = 200;
     // ---- PresetPose::init (preset_pose.cpp:7-49)
@@ -59,6 +63,7 @@ int main(int argc, char** argv) {
     rot.exact = exact;
     CameraGradParams cam;
     cam.exact = exact;
+    if (exact) SSIMLoss::gradMode() = VCL_SSIM_GRAD_EXACT;
     Rotation::init(rot);
     Camera::init(cam, rot, vec3(0.f, 0.f, 3.5f), vec3(0.f, 0.f, -1.f), vec3(0.f, 1.f, 0.f), .5f, 1.f, 1.f, 10.f);
     ProjectGradParams proj;
@@ -79,7 +84,7 @@ int main(int argc, char** argv) {
     Rotation::setRandom(rot);
 
     // ---- PresetPose::update (preset_pose.cpp:63-97) for steps 0 .. steps-1
-    std::vector<float> losses, quats(8);
+    std::vector<float> losses, quats(8), grad0(4);
     for (int step = 0; step < steps; step++) {
         if (step % INIT_INTERVAL == 0) {
             Rotation::setRandom(target_rot);
@@ -115,6 +120,10 @@ int main(int argc, char** argv) {
         Project::backward(proj);
         Camera::backward(cam);
         Rotation::backward(rot);
+        if (step == 0) {
+            const quat& g = rot.grad.q;
+            grad0 = {g.x, g.y, g.z, g.w};
+        }
         Rotation::step(rot);
         losses.push_back(Loss::loss(loss));
     }
@@ -137,6 +146,7 @@ int main(int argc, char** argv) {
     const quat& f = rot.kernel.q;
     const float qf[4] = {f.x, f.y, f.z, f.w};
     std::fwrite(qf, sizeof(float), 4, o);
+    std::fwrite(grad0.data(), sizeof(float), 4, o);
     std::fclose(o);
     std::printf("preset_pose: %d steps, loss %.6e -> %.6e\n", steps, losses.front(), losses.back());
     return 0;

@@ -330,28 +330,30 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     np.width = W; np.height = H; np.depth = nv; np.rast = F->f("rast"); np.rot = F->f("rot");
     np.normal = F->f("normals"); np.normalidx = F->t<uint32_t>("idx"); np.out = F->f("nrm");
     vcl_sg_specular_params sp = {W, H, nv, F->f("rast"), F->f("nrm"), F->f("view"), F->f("rough"), d.ior, F->f("sAxis"), F->f("sSharp"), F->f("sAmp")};
-    if (d.op_by_op) {
-        // the reference's sequence: NormalAxis, ViewAxis, SGSpecular as separate launches
-        { ProfScope ps_(F, "normal_axis", as_hip(s)); st = vcl_normal_axis_forward(&np, s); }
-        if (!st) { ProfScope ps_(F, "view_axis", as_hip(s)); st = vcl_view_axis_forward_dev(W, H, nv, F->f("pvinv"), F->f("view"), s); }
-        if (!st) { ProfScope ps_(F, "sg_specular", as_hip(s)); st = vcl_sg_specular_forward(&sp, s); }
-    } else {
-        // NormalAxis -> ViewAxis -> SGSpecular, fused per pixel
-        ProfScope ps_(F, "shade_prep", as_hip(s));
-        st = vcl_shade_prep_dev(&np, F->f("pvinv"), &sp, s);
-    }
-    if (st) return st;
     vcl_texturemap_params tp;
     tp.width = W; tp.height = H; tp.depth = nv; tp.texwidth = d.texwidth; tp.texheight = d.texheight; tp.channel = C; tp.miplevel = d.miplevel;
     tp.rast = F->f("rast"); tp.uv = F->f("uvp"); tp.uvDA = F->f("uvDA");
     for (int l = 0; l < VCL_TEX_MAX_MIP_LEVEL; l++) tp.texture[l] = l < d.miplevel ? F->f("texture") + F->lvl_off[l] : nullptr;
     tp.out = F->f("diff");
     vcl_interpolate_params ip = {W, H, nv, 2, 1, F->f("rast"), F->f("db"), F->f("uv"), F->t<uint32_t>("idx"), F->f("uvp"), F->f("uvDA"), d.V};
-    if (C == 4 && !d.op_by_op) {
-        // uv interpolation and texture sample fused per pixel
-        ProfScope ps_(F, "interp_texture", as_hip(s));
-        st = vcl_interp_tex_dev(&ip, &tp, nullptr, s);
+    if (d.op_by_op) {
+        // the reference's sequence: NormalAxis, ViewAxis, SGSpecular as separate launches
+        { ProfScope ps_(F, "normal_axis", as_hip(s)); st = vcl_normal_axis_forward(&np, s); }
+        if (!st) { ProfScope ps_(F, "view_axis", as_hip(s)); st = vcl_view_axis_forward_dev(W, H, nv, F->f("pvinv"), F->f("view"), s); }
+        if (!st) { ProfScope ps_(F, "sg_specular", as_hip(s)); st = vcl_sg_specular_forward(&sp, s); }
+    } else if (C == 4) {
+        // NormalAxis -> ViewAxis -> SGSpecular with the uv interpolation and texture sample, fused
+        // per pixel
+        ProfScope ps_(F, "pixel_prep", as_hip(s));
+        st = vcl_pixel_prep_dev(&ip, &tp, &np, F->f("pvinv"), &sp, s);
     } else {
+        // NormalAxis -> ViewAxis -> SGSpecular, fused per pixel
+        ProfScope ps_(F, "shade_prep", as_hip(s));
+        st = vcl_shade_prep_dev(&np, F->f("pvinv"), &sp, s);
+    }
+    if (st) return st;
+    if (C != 4 || d.op_by_op) {
+        // (the fused C == 4 path did these in the pixel_prep pass)
         { ProfScope ps_(F, "interpolate", as_hip(s)); st = vcl_interpolate_forward(&ip, s); }
         if (st) return st;
         { ProfScope ps_(F, "texture", as_hip(s)); st = vcl_texturemap_forward(&tp, s); }

@@ -21,8 +21,6 @@ struct VclMipTail {
 extern "C" {
 // ViewAxis forward with the per-view inverse matrices already on the device (depth x 9 floats).
 int vcl_view_axis_forward_dev(int W, int H, int depth, const float* pvinv_dev, float* out, vcl_stream_t stream);
-// NormalAxis, ViewAxis (inverse projection-view blocks on the device) and SGSpecular forward in one
-// launch; sp->normal and sp->view are the outputs of the first two.
 // Interpolate (uv: 2 attributes with derivatives) and the C == 4 texture forward in one launch;
 // tp->uv and tp->uvDA are ip's outputs.
 // zero (may be null): a 4-float-per-pixel buffer cleared in the same pass.
@@ -36,7 +34,13 @@ int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt
 int vcl_rasterize_clear_spec(const vcl_rasterize_params* p, WsClear* c);
 // vcl_rasterize_forward; cleared: the workspace reset was already done (vcl_rasterize_clear_spec).
 int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_stream_t stream);
+// NormalAxis, ViewAxis (inverse projection-view blocks on the device) and SGSpecular forward in one
+// launch; sp->normal and sp->view are the outputs of the first two.
 int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp, vcl_stream_t stream);
+// uv interpolation + texture sample (C == 4) and the shading inputs in one pass: the outputs of
+// vcl_interp_tex_dev and vcl_shade_prep_dev (normals indexed like the uv attributes).
+int vcl_pixel_prep_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, const vcl_normal_axis_params* np, const float* pvinv_dev,
+                       const vcl_sg_specular_params* sp, vcl_stream_t stream);
 // Texture backward then interpolate backward (the latter's output grad being the former's uv
 // grad): fused into one kernel when the tiled C == 4 texture backward applies, a 2-channel
 // attribute feeds the uv and only the rast grad is wanted; otherwise the two ops in turn.

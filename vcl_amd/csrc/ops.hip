@@ -1263,23 +1263,29 @@ __global__ void __launch_bounds__(BS) k_adam_mip(AdamArgs a, MipArgs m) { KAdamM
 
 // ---------------------------------------------------------------- NormalAxis / ViewAxis
 // NormalAxis of one pixel: interpolated vertex normal, normalised, rotated (zero off-mesh).
-__device__ __forceinline__ void normal_axis_px(float4 r, const float* rot, const float* normal, const u32* nidx, float& o0, float& o1, float& o2) {
+// The covered pixel's axis from its triangle's three vertex normals n0, n1, n2.
+__device__ __forceinline__ void normal_axis_eval(float4 r, const float* rot, float3 n0, float3 n1, float3 n2, float& o0, float& o1, float& o2) {
 #pragma clang fp contract(off)
+    float N0 = (n0.x - n2.x) * r.x + (n1.x - n2.x) * r.y + n2.x;
+    float N1 = (n0.y - n2.y) * r.x + (n1.y - n2.y) * r.y + n2.y;
+    float N2 = (n0.z - n2.z) * r.x + (n1.z - n2.z) * r.y + n2.z;
+    float d = 1.f / sqrtf(N0 * N0 + N1 * N1 + N2 * N2);
+    N0 *= d; N1 *= d; N2 *= d;
+    o0 = rot[0] * N0 + rot[4] * N1 + rot[8] * N2;
+    o1 = rot[1] * N0 + rot[5] * N1 + rot[9] * N2;
+    o2 = rot[2] * N0 + rot[6] * N1 + rot[10] * N2;
+}
+
+__device__ __forceinline__ float3 load_f3(const float* a, u32 i) {
+    const float* q = a + 3 * (size_t)i;
+    return make_float3(q[0], q[1], q[2]);
+}
+
+__device__ __forceinline__ void normal_axis_px(float4 r, const float* rot, const float* normal, const u32* nidx, float& o0, float& o1, float& o2) {
     int id = (int)r.w - 1;
     o0 = 0.f; o1 = 0.f; o2 = 0.f;
-    if (id >= 0) {
-        const float* n2 = normal + 3 * (size_t)nidx[3 * id + 2];
-        const float* n0 = normal + 3 * (size_t)nidx[3 * id];
-        const float* n1 = normal + 3 * (size_t)nidx[3 * id + 1];
-        float N0 = (n0[0] - n2[0]) * r.x + (n1[0] - n2[0]) * r.y + n2[0];
-        float N1 = (n0[1] - n2[1]) * r.x + (n1[1] - n2[1]) * r.y + n2[1];
-        float N2 = (n0[2] - n2[2]) * r.x + (n1[2] - n2[2]) * r.y + n2[2];
-        float d = 1.f / sqrtf(N0 * N0 + N1 * N1 + N2 * N2);
-        N0 *= d; N1 *= d; N2 *= d;
-        o0 = rot[0] * N0 + rot[4] * N1 + rot[8] * N2;
-        o1 = rot[1] * N0 + rot[5] * N1 + rot[9] * N2;
-        o2 = rot[2] * N0 + rot[6] * N1 + rot[10] * N2;
-    }
+    if (id >= 0)
+        normal_axis_eval(r, rot, load_f3(normal, nidx[3 * id]), load_f3(normal, nidx[3 * id + 1]), load_f3(normal, nidx[3 * id + 2]), o0, o1, o2);
 }
 
 struct KNormalAxisF {
@@ -1394,6 +1400,121 @@ struct KShadePrepF {
 __global__ void __launch_bounds__(BS) k_shade_prep(int W, int H, int depth, const float* rast, const float* rot, const float* normal, const u32* nidx,
                                                    const float* pvinv, const float* rough, float ior, float* onrm, float* oview, float* oAxis,
                                                    float* oSharp, float* oAmp) { KShadePrepF{}(vb_hw(), W, H, depth, rast, rot, normal, nidx, pvinv, rough, ior, onrm, oview, oAxis, oSharp, oAmp); }
+
+// The fit step's per-pixel passes ahead of the SG forward in one (C == 4, normals indexed like
+// the uv attributes): uv interpolation with derivatives and the texture sample (the arithmetic
+// of KInterpTex4F) and the shading inputs NormalAxis, ViewAxis and SGSpecular (that of
+// KShadePrepF).  The two share the rast read and the triangle's index loads, and the specular
+// terms are formed while the texel loads are in flight.  Each thread takes NPX pixels one
+// workgroup-width apart, each stage's loads for all its pixels issued together.
+template <int NPX>
+struct KPixelPrep4F {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, int W, int H, int depth, const float* rast, const float* db, const float* uvattr,
+                                               const u32* idx, float* uvp, float* uvDA, TexArgs t, float* out, const float* rot,
+                                               const float* normal, const float* pvinv, const float* rough, float ior, float* onrm,
+                                               float* oview, float* oAxis, float* oSharp, float* oAmp) const {
+        const size_t P = (size_t)W * H * depth;
+        const size_t p0 = (size_t)vb.b.x * (BS * NPX) + threadIdx.x;
+        float4 r[NPX];
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const size_t p = p0 + (size_t)j * BS;
+            r[j] = p < P ? ((const float4*)rast)[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        u32 i0[NPX], i1[NPX], i2[NPX];
+        float4 b[NPX];
+        float rg[NPX];
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const int id = (int)r[j].w - 1;
+            i0[j] = i1[j] = i2[j] = 0;
+            b[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            rg[j] = 0.f;
+            if (id >= 0) {
+                const size_t p = p0 + (size_t)j * BS;
+                i0[j] = idx[3 * id]; i1[j] = idx[3 * id + 1]; i2[j] = idx[3 * id + 2];
+                b[j] = ((const float4*)db)[p];
+                rg[j] = rough[p];
+            }
+        }
+        float2 a0[NPX], a1[NPX], a2[NPX];
+        float3 n0[NPX], n1[NPX], n2[NPX];
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            a0[j] = a1[j] = a2[j] = make_float2(0.f, 0.f);
+            n0[j] = n1[j] = n2[j] = make_float3(0.f, 0.f, 0.f);
+            if ((int)r[j].w - 1 >= 0) {
+                a0[j] = ((const float2*)uvattr)[i0[j]];
+                a1[j] = ((const float2*)uvattr)[i1[j]];
+                a2[j] = ((const float2*)uvattr)[i2[j]];
+                n0[j] = load_f3(normal, i0[j]);
+                n1[j] = load_f3(normal, i1[j]);
+                n2[j] = load_f3(normal, i2[j]);
+            }
+        }
+        Tex4Taps q[NPX];
+        float4 tx[NPX][8];
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const size_t p = p0 + (size_t)j * BS;
+            float2 u = make_float2(0.f, 0.f);
+            float4 da = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((int)r[j].w - 1 >= 0) {
+                const double w2 = 1.0 - (double)r[j].x - (double)r[j].y;
+                float2 dau, dav;
+                interp_attr_px(r[j], b[j], w2, a0[j].x, a1[j].x, a2[j].x, u.x, dau);
+                interp_attr_px(r[j], b[j], w2, a0[j].y, a1[j].y, a2[j].y, u.y, dav);
+                da = make_float4(dau.x, dau.y, dav.x, dav.y);
+            }
+            if (p < P) {
+                ((float2*)uvp)[p] = u;
+                ((float4*)uvDA)[p] = da;
+            }
+            if (r[j].w >= 1.f) {
+                q[j] = tex4_taps(t, u, da);
+                tx[j][0] = q[j].T0[q[j].i0.x]; tx[j][1] = q[j].T0[q[j].i0.y]; tx[j][2] = q[j].T0[q[j].i0.z]; tx[j][3] = q[j].T0[q[j].i0.w];
+                if (q[j].two) {
+                    tx[j][4] = q[j].T1[q[j].i1.x]; tx[j][5] = q[j].T1[q[j].i1.y]; tx[j][6] = q[j].T1[q[j].i1.z]; tx[j][7] = q[j].T1[q[j].i1.w];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const size_t p = p0 + (size_t)j * BS;
+            if (p >= P) continue;
+            const int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
+            float nx = 0.f, ny = 0.f, nz = 0.f, vx, vy, vz;
+            if ((int)r[j].w - 1 >= 0) normal_axis_eval(r[j], rot, n0[j], n1[j], n2[j], nx, ny, nz);
+            view_axis_px(W, H, px, py, pvinv + 9 * s, vx, vy, vz);
+            onrm[3 * p] = nx; onrm[3 * p + 1] = ny; onrm[3 * p + 2] = nz;
+            if (oview) { oview[3 * p] = vx; oview[3 * p + 1] = vy; oview[3 * p + 2] = vz; }
+            float pa[3] = {0.f, 0.f, 0.f}, amp = 0.f, sh = 0.f;
+            if ((int)r[j].w - 1 >= 0) spec_terms(nx, ny, nz, vx, vy, vz, rg[j], ior, pa, amp, sh, nullptr, nullptr);
+            oAxis[3 * p] = pa[0]; oAxis[3 * p + 1] = pa[1]; oAxis[3 * p + 2] = pa[2];
+            oSharp[p] = sh; oAmp[p] = amp;
+        }
+#pragma unroll
+        for (int j = 0; j < NPX; j++) {
+            const size_t p = p0 + (size_t)j * BS;
+            if (p >= P) continue;
+            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r[j].w >= 1.f) {
+                o = bilerp4(tx[j][0], tx[j][1], tx[j][2], tx[j][3], q[j].fx, q[j].fy);
+                if (q[j].two) o = lerp4(o, bilerp4(tx[j][4], tx[j][5], tx[j][6], tx[j][7], q[j].gx, q[j].gy), q[j].f);
+            }
+            ((float4*)out)[p] = o;
+        }
+    }
+};
+constexpr int PIXEL_PREP_NPX = 2;
+__global__ void __launch_bounds__(BS) k_pixel_prep4(int W, int H, int depth, const float* rast, const float* db, const float* uvattr, const u32* idx,
+                                                    float* uvp, float* uvDA, TexArgs t, float* out, const float* rot, const float* normal,
+                                                    const float* pvinv, const float* rough, float ior, float* onrm, float* oview, float* oAxis,
+                                                    float* oSharp, float* oAmp) {
+    KPixelPrep4F<PIXEL_PREP_NPX>{}(vb_hw(), W, H, depth, rast, db, uvattr, idx, uvp, uvDA, t, out, rot, normal, pvinv, rough, ior, onrm, oview, oAxis,
+                                   oSharp, oAmp);
+}
 
 struct KSgSpecBwdF {
     static constexpr int kBS = BS, kMINW = 1;
@@ -2954,6 +3075,28 @@ int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev,
     vlaunch<KShadePrepF>(k_shade_prep, dim3(cdiv(P, BS)), dim3(BS), 0, as_hip(stream), np->width, np->height, np->depth, np->rast, np->rot, np->normal,
                        np->normalidx, pvinv_dev, sp->roughness, sp->ior, np->out, const_cast<float*>(sp->view), sp->outAxis, sp->outSharpness,
                        sp->outAmplitude);
+    VCL_LAUNCH_CHECK();
+    return VCL_OK;
+}
+
+int vcl_pixel_prep_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, const vcl_normal_axis_params* np, const float* pvinv_dev,
+                       const vcl_sg_specular_params* sp, vcl_stream_t stream) {
+    VCL_REQUIRE(ip && tp && np && sp && pvinv_dev);
+    VCL_REQUIRE(ip->rast && ip->rastDB && ip->attr && ip->idx && ip->out && ip->outDA && tp->out);
+    VCL_REQUIRE(ip->dimention == 2 && ip->enableDA && tp->channel == 4 && tp->uv == ip->out && tp->uvDA == ip->outDA && tp->rast == ip->rast);
+    VCL_REQUIRE(np->rot && np->normal && np->normalidx == ip->idx && np->out && sp->roughness && sp->outAxis && sp->outSharpness && sp->outAmplitude);
+    VCL_REQUIRE(np->rast == ip->rast && sp->rast == ip->rast && sp->normal == np->out);
+    VCL_REQUIRE(ip->width == tp->width && ip->height == tp->height && ip->depth == tp->depth && np->width == ip->width && np->height == ip->height &&
+                np->depth == ip->depth && sp->width == ip->width && sp->height == ip->height && sp->depth == ip->depth);
+    TexArgs t;
+    int st = tex_args(tp, t);
+    if (st) return st;
+    size_t P = (size_t)ip->width * ip->height * ip->depth;
+    if (!P) return VCL_OK;
+    vlaunch<KPixelPrep4F<PIXEL_PREP_NPX>>(k_pixel_prep4, dim3(cdiv(P, (size_t)BS * PIXEL_PREP_NPX)), dim3(BS), 0, as_hip(stream), ip->width,
+                                          ip->height, ip->depth, ip->rast, ip->rastDB, ip->attr, ip->idx, ip->out, ip->outDA, t, tp->out, np->rot,
+                                          np->normal, pvinv_dev, sp->roughness, sp->ior, np->out, const_cast<float*>(sp->view), sp->outAxis,
+                                          sp->outSharpness, sp->outAmplitude);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }
