@@ -1802,13 +1802,15 @@ __global__ void __launch_bounds__(BS) k_sg_fwd_fast(size_t P, int C, int K, cons
 // pixel's 2.133*normal or spec-sharpness*spec-axis length (both lengths carry a relative
 // margin for their float rounding).
 // Bounded form: when every lobe of the workgroup and every pixel of the wave is finite with
-// |lam mu| - lam <= 8, |w| - base <= 8 and amplitudes within 2^32 (the normalised axes and
-// normals of a fit), each term's exponent l - c is at most 16 and 1/l at most 1e6, so a term
-// times its amplitudes is below e^16 * 1e6 * 2^32 * 2^32 < 2^108 and every partial sum of at
-// most 256 of them is finite: no term can fail AddNaNcheck, the loop tests none, and the
+// |lam mu| - lam <= 8, |w| - base <= 8, amplitudes within 2^32 and sharpness within 2^20 (the
+// normalised axes and normals of a fit), each term's exponent l - c is at most 16 (17 with its
+// float rounding) and 1/l at most 1e6, so a term times its amplitudes is below
+// e^17 * 1e6 * 2^32 * 2^32 < 2^109 and every partial sum of at most 256 of them is finite: no
+// term can fail AddNaNcheck, the loop tests none, and the
 // constant factors (2.34, the spec amplitude) multiply the two lobe sums once at the end
 // instead of every term.
-struct SgLobe4 { float4 g; float4 a; float lamL2E, len, lam, pad; };   // g = (lam mu, cD), cD = (SG_DS + lam) log2(e), lamL2E = lam log2(e)
+// g = (lam mu, cD), cD = (SG_DS + lam) log2(e), lamL2E = lam log2(e), len2 = |lam mu|^2, len = |lam mu|
+struct SgLobe4 { float4 g; float4 a; float lamL2E, len2, len, lam; };
 
 // (e^{l-c} - e^{-l-c}) / l with the exponent's constant part c = (base + lam) log2(e) given: one
 // rsq gives 1/l and l, each exponent is one fma (within 1e-6 relative of the reference's
@@ -1827,11 +1829,27 @@ __device__ __forceinline__ float sg_term_rsq(float wx, float wy, float wz, float
     return t * il;
 }
 
+// sg_term_rsq for the bounded form, from the squared length d = |w|^2 formed as
+// (|p|^2 + |lam mu|^2) + 2 p . lam mu (one add and three fma instead of the three adds, one
+// multiply and two fma of |p + lam mu|^2); within the bounds both squares are finite.
+__device__ __forceinline__ float sg_term_d(float d, float c, bool both) {
+    float il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
+    float l = d * il;
+    float t = nexp2(fmaf(l, L2E, -c));
+    if (both) {
+        asm volatile("" :::);
+        t = t - nexp2(fmaf(-l, L2E, -c));
+    }
+    return t * il;
+}
+
 // e^{-l-c} is below half an ulp of e^{l-c} once l > SG_E2_DROP (e^{-17.5} < 2^-25).
 constexpr float SG_E2_DROP = 8.75f;
 // Bounded form's limits: |lam mu| - lam and |w| - base at most SG_FWD_EXCESS, amplitudes and
-// spec amplitudes below SG_FWD_AMAX in magnitude.
-constexpr float SG_FWD_EXCESS = 8.f, SG_FWD_AMAX = 4294967296.f;
+// spec amplitudes below SG_FWD_AMAX in magnitude, lobe and spec sharpness at most SG_FWD_LMAX
+// in magnitude (so the squared lengths are finite and the exponents' float rounding, about
+// ulp(lam log2(e)), stays below 1).
+constexpr float SG_FWD_EXCESS = 8.f, SG_FWD_AMAX = 4294967296.f, SG_FWD_LMAX = 1048576.f;
 
 // Which lobes of chunk [c0, c0 + 64) may need e^{-l-c} for some lane of the wave: lane j tests
 // lobe c0 + j against the wave's range [lmin, lmax] of the pixel-side length |w| (the per-lane
@@ -1870,6 +1888,9 @@ template <SgFwdForm FORM>
 __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float dnx, float dny, float dnz, float snx, float sny, float snz,
                                               float ss, float sa, const u64* dmask, const u64* smask, float4& de, float4& se) {
     const float ssL2E = ss * L2E;
+    // bounded form: 2 p and |p|^2 of the two pixel-side vectors
+    const float dnx2 = 2.f * dnx, dny2 = 2.f * dny, dnz2 = 2.f * dnz, dsq = dnx * dnx + dny * dny + dnz * dnz;
+    const float snx2 = 2.f * snx, sny2 = 2.f * sny, snz2 = 2.f * snz, ssq = snx * snx + sny * sny + snz * snz;
 #pragma unroll
     for (int c = 0; c < SG_MAXK / 64; c++) {
         if (c * 64 >= K) break;
@@ -1880,12 +1901,19 @@ __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float
             const float4 g = lobes[i].g, a = lobes[i].a;
             const bool dboth = (dm >> (i - c * 64)) & 1ull;
             const bool sboth = (sm >> (i - c * 64)) & 1ull;
-            const float td = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, g.w, dboth);
-            const float ts = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ssL2E + lobes[i].lamL2E, sboth);
             if (FORM == SG_FWD_BOUNDED) {
+                const float L = lobes[i].len2;
+                const float dd = fmaf(dnx2, g.x, fmaf(dny2, g.y, fmaf(dnz2, g.z, dsq + L)));
+                const float ds = fmaf(snx2, g.x, fmaf(sny2, g.y, fmaf(snz2, g.z, ssq + L)));
+                const float td = sg_term_d(dd, g.w, dboth);
+                const float ts = sg_term_d(ds, ssL2E + lobes[i].lamL2E, sboth);
                 de.x += td * a.x; de.y += td * a.y; de.z += td * a.z; de.w += td * a.w;
                 se.x += ts * a.x; se.y += ts * a.y; se.z += ts * a.z; se.w += ts * a.w;
-            } else if (FORM == SG_FWD_TESTED) {
+                continue;
+            }
+            const float td = sg_term_rsq(dnx + g.x, dny + g.y, dnz + g.z, g.w, dboth);
+            const float ts = sg_term_rsq(snx + g.x, sny + g.y, snz + g.z, ssL2E + lobes[i].lamL2E, sboth);
+            if (FORM == SG_FWD_TESTED) {
                 const float dg = td * SG_DA, sg = ts * sa;
                 const float dv = isfinite(dg) ? dg : 0.f, sv = isfinite(sg) ? sg : 0.f;
                 de.x += dv * a.x; de.y += dv * a.y; de.z += dv * a.z; de.w += dv * a.w;
@@ -1927,12 +1955,13 @@ struct KSgFwd4F {
             lobes[i].g = make_float4(gx, gy, gz, (SG_DS + lam) * L2E);
             lobes[i].a = a;
             lobes[i].len = len;
+            lobes[i].len2 = gx * gx + gy * gy + gz * gz;
             lobes[i].lam = lam;
             lobes[i].lamL2E = lam * L2E;
             if (!(isfinite(a.x) && isfinite(a.y) && isfinite(a.z) && isfinite(a.w))) nonfinite = 1;
             // NaN lengths or sharpness fail the comparison and so count as unbounded
             const float amax = fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fmaxf(fabsf(a.z), fabsf(a.w)));
-            if (!(len - lam <= SG_FWD_EXCESS && fabsf(lam) <= 1e30f && amax < SG_FWD_AMAX)) unbounded = 1;
+            if (!(len - lam <= SG_FWD_EXCESS && fabsf(lam) <= SG_FWD_LMAX && amax < SG_FWD_AMAX)) unbounded = 1;
         }
         __syncthreads();
         const bool checked = nonfinite != 0;
@@ -1963,7 +1992,7 @@ struct KSgFwd4F {
                 snx = sAxis[3 * p] * ss; sny = sAxis[3 * p + 1] * ss; snz = sAxis[3 * p + 2] * ss;
                 dlen = sqrtf(dnx * dnx + dny * dny + dnz * dnz);
                 slen = sqrtf(snx * snx + sny * sny + snz * snz);
-                bounded = dlen - SG_DS <= SG_FWD_EXCESS && slen - ss <= SG_FWD_EXCESS && fabsf(ss) <= 1e30f && fabsf(sa) < SG_FWD_AMAX;
+                bounded = dlen - SG_DS <= SG_FWD_EXCESS && slen - ss <= SG_FWD_EXCESS && fabsf(ss) <= SG_FWD_LMAX && fabsf(sa) < SG_FWD_AMAX;
             }
             const bool wave_bounded = lobes_bounded && __ballot(!bounded) == 0ull;
             const float dmin = wave_fmin(dlen), dmax = wave_fmax(dlen), smin = wave_fmin(slen), smax = wave_fmax(slen);

@@ -715,8 +715,9 @@ __device__ __forceinline__ bool piece_covers_rect(const RRec& r, int px0, int px
 // One 256-thread workgroup per work-list chunk (up to `chunk` pieces of one TS x TS tile's bin,
 // so a deep bin is spread over several workgroups).  Per batch of pieces: (1a) thread = piece
 // forms the piece's conservative depth range over the tile, and pieces covering the whole tile
-// lower the tile's depth bound; (1b) the pieces not behind that bound find their exact covered
-// column span in each row (bisection on the integer edge functions, so no per-pixel edge test);
+// lower the tile's depth bound; (1b) the pieces not behind that bound, compacted onto consecutive
+// threads, find their exact covered column span in each row (a double-precision crossing
+// estimate settled by exact edge tests, so no per-pixel edge test);
 // (2) a block prefix turns the covered rows into items; (3) thread = item evaluates the depth
 // planes at each covered pixel centre and takes the LDS 64-bit minimum of the (ordered z_ndc,
 // triangle id) key.  The result does not depend on bin order, tile size, culling or how the
@@ -736,6 +737,8 @@ struct KFineF {
         constexpr int ITEM_CAP = BATCH * TS < 2048 ? BATCH * TS : 2048;
         __shared__ u32 sitem[ITEM_CAP];                  // covered rows: piece | row | first column | length
         __shared__ u32 sbucket[4];                       // per-wave covered-row totals
+        __shared__ u32 swcnt[4];                         // per-wave surviving-piece counts
+        __shared__ u32 sslot[BATCH];                     // surviving pieces' record slots
         __shared__ u32 sbound;                           // ordered depth bound of the tile (culling)
         const int tid = threadIdx.x;
         const u32 nchunks = a.ws.misc[5];
@@ -768,10 +771,10 @@ struct KFineF {
                 bool live = false;
                 float zlo = 0.f;
                 bool has_lo = false;
-                RRec r;   // the piece's record, held in registers across the barrier below
+                u32 slot = 0;
                 if (tid < BATCH && q < n) {
-                    const u32 slot = scan_all ? q : a.ws.bins[ch.first + q];
-                    r = a.ws.rec[slot];
+                    slot = scan_all ? q : a.ws.bins[ch.first + q];
+                    const RRec r = a.ws.rec[slot];
                     int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
                     int yl0 = max((r.pyr & 0xffff) - y0, 0), yh0 = min((r.pyr >> 16) - y0, TS - 1);
                     live = r.id != INVALID && (r.view_tl >> 8) == view && xl <= xh && yl0 <= yh0;
@@ -785,9 +788,21 @@ struct KFineF {
                     }
                 }
                 __syncthreads();
-                // phase 1b: pieces not behind the bound: exact row spans and depth planes into LDS;
-                // covered rows counted
-                if (live && !(has_lo && ord_f32(zlo) > sbound)) {
+                // the pieces not behind the bound, compacted in piece order, so that the span pass
+                // below runs on consecutive threads (most pieces of a deep tile are culled)
+                const bool surv = live && !(has_lo && ord_f32(zlo) > sbound);
+                const u64 sm = __ballot(surv);
+                if ((tid & 63) == 0) swcnt[tid >> 6] = (u32)__popcll(sm);
+                __syncthreads();
+                u32 spos = (u32)__popcll(sm & ((1ull << (tid & 63)) - 1ull));
+                for (int w = 0; w < (tid >> 6); w++) spos += swcnt[w];
+                const u32 nsurv = swcnt[0] + swcnt[1] + swcnt[2] + swcnt[3];
+                if (surv) sslot[spos] = slot;
+                __syncthreads();
+                // phase 1b: thread j < nsurv takes the j-th surviving piece: exact row spans and depth
+                // planes into LDS; covered rows counted
+                if ((u32)tid < nsurv) {
+                    const RRec r = a.ws.rec[sslot[tid]];
                     u32 id = r.id, vtl = r.view_tl;
                     int xl = max((r.pxr & 0xffff) - x0, 0), xh = min((r.pxr >> 16) - x0, TS - 1);
                     yl = max((r.pyr & 0xffff) - y0, 0);
