@@ -1198,10 +1198,11 @@ struct KDownsample4F {
 };
 __global__ void __launch_bounds__(BS) k_downsample4(const float4* src, float4* dst, int w, int h, float4* zero_grad) { KDownsample4F{}(vb_hw(), src, dst, w, h, zero_grad); }
 
-// Several mip levels in one launch: each workgroup reads a 64 x 64 block of level s and writes
-// the block's texels of levels s+1 .. s+nl (32^2 down to 1^2), each from the previous level
-// kept in LDS with the same box-filter arithmetic as k_downsample4 (so the values are
-// identical), clearing the matching coarse grads.
+// Several mip levels in one launch: each workgroup reads a BSZ x BSZ block of level s (BSZ = 64
+// or 32) and writes the block's texels of levels s+1 .. s+nl (BSZ/2 squared down to 1^2, nl at
+// most log2(BSZ)), each from the previous level kept in LDS with the same box-filter
+// arithmetic as k_downsample4 (so the values are identical), clearing the matching coarse
+// grads.  32 x 32 blocks give a large first level four times the workgroups.
 struct MipArgs {
     const float4* src;
     int sw, sh, nl;
@@ -1209,14 +1210,15 @@ struct MipArgs {
     float4* zg[6];
 };
 
+template <int BSZ>
 struct KMipFusedF {
     static constexpr int kBS = BS, kMINW = 1;
     __device__ __forceinline__ void operator()(const VB& vb, MipArgs a) const {
 #pragma clang fp contract(off)
-        __shared__ float4 buf[2][32 * 32];
+        __shared__ float4 buf[2][(BSZ / 2) * (BSZ / 2)];
         const int bx = vb.b.x, by = vb.b.y;
         for (int k = 1; k <= a.nl; k++) {
-            const int bs = 64 >> k;                      // block side at level s+k
+            const int bs = BSZ >> k;                     // block side at level s+k
             const int w = a.sw >> k, h = a.sh >> k;      // level size
             const int ox = bx * bs, oy = by * bs;
             float4* out = buf[k & 1];
@@ -1244,7 +1246,8 @@ struct KMipFusedF {
         }
     }
 };
-__global__ void __launch_bounds__(BS) k_mip_fused(MipArgs a) { KMipFusedF{}(vb_hw(), a); }
+template <int BSZ>
+__global__ void __launch_bounds__(BS) k_mip_fused(MipArgs a) { KMipFusedF<BSZ>{}(vb_hw(), a); }
 
 // The Adam launch with one more workgroup that runs the mip rebuild's last pass when that pass
 // is a single workgroup (the coarsest levels, from a source level of at most 64 x 64 texels),
@@ -1253,7 +1256,7 @@ struct KAdamMipF {
     static constexpr int kBS = BS, kMINW = 1;
     __device__ __forceinline__ void operator()(const VB& vb, AdamArgs a, MipArgs m) const {
         if (vb.b.x == vb.g.x - 1) {
-            KMipFusedF{}(VB{{0u, 0u, 0u}, {1u, 1u, 1u}}, m);
+            KMipFusedF<64>{}(VB{{0u, 0u, 0u}, {1u, 1u, 1u}}, m);
             return;
         }
         KAdamF{}(VB{vb.b, {vb.g.x - 1, vb.g.y, vb.g.z}}, a);
@@ -2986,7 +2989,7 @@ int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* gra
         VCL_LAUNCH_CHECK();
     }
     // rebuild mips from the new level 0; the same pass zeroes the coarse grads.  C == 4: level 1
-    // with one thread per texel, then up to six levels per launch (k_mip_fused)
+    // with one thread per texel, then up to five or six levels per launch (k_mip_fused)
     if (!(part & 2)) return VCL_OK;
     if (vec4) {
         int l = quad ? 2 : 1;
@@ -3002,7 +3005,10 @@ int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* gra
             m.src = (const float4*)tex->texture[l - 1];
             m.sw = a.tw >> (l - 1);
             m.sh = a.th >> (l - 1);
-            m.nl = std::min(6, a.L - l);
+            // a source that one 64 x 64 block covers in one pass (six levels), any other in 32 x 32
+            // blocks (five levels per pass), so the launch sequence does not depend on the size
+            const bool b32 = !(m.sw <= 64 && m.sh <= 64);
+            m.nl = std::min(b32 ? 5 : 6, a.L - l);
             for (int k = 0; k < 6; k++) {
                 m.dst[k] = k < m.nl ? (float4*)tex->texture[l + k] : nullptr;
                 m.zg[k] = k < m.nl ? (float4*)a.grad[l + k] : nullptr;
@@ -3014,7 +3020,10 @@ int vcl_texture_adam_update_parts(const vcl_texture* tex, const vcl_texture* gra
                 for (int k = 0; k < 6; k++) { tail->dst[k] = (float*)m.dst[k]; tail->zg[k] = (float*)m.zg[k]; }
                 break;
             }
-            vlaunch<KMipFusedF>(k_mip_fused, dim3(cdiv(m.sw, 64), cdiv(m.sh, 64)), dim3(BS), 0, s, m);
+            if (b32)
+                vlaunch<KMipFusedF<32>>(k_mip_fused<32>, dim3(cdiv(m.sw, 32), cdiv(m.sh, 32)), dim3(BS), 0, s, m);
+            else
+                vlaunch<KMipFusedF<64>>(k_mip_fused<64>, dim3(cdiv(m.sw, 64), cdiv(m.sh, 64)), dim3(BS), 0, s, m);
             VCL_LAUNCH_CHECK();
             l += m.nl;
         }

@@ -725,7 +725,7 @@ __device__ __forceinline__ bool piece_covers_rect(const RRec& r, int px0, int px
 // with a device-scope 64-bit minimum.
 template <int TS>
 struct KFineF {
-    static constexpr int kBS = 256, kMINW = TS == 32 ? 4 : 5;
+    static constexpr int kBS = 256, kMINW = TS == 32 ? 4 : 6;
     __device__ __forceinline__ void operator()(const VB& vb, RastArgs a) const {
         constexpr int NP = TS * TS;
         __shared__ u64 key[NP];
@@ -734,7 +734,7 @@ struct KFineF {
         __shared__ unsigned short sspan[BATCH][TS];     // per piece: first | last << 8 column per row
         __shared__ float szp[9][BATCH];
         __shared__ u32 sid[BATCH];
-        constexpr int ITEM_CAP = BATCH * TS < 2048 ? BATCH * TS : 2048;
+        constexpr int ITEM_CAP = BATCH * TS < 1024 ? BATCH * TS : 1024;
         __shared__ u32 sitem[ITEM_CAP];                  // covered rows: piece | row | first column | length
         __shared__ u32 sbucket[4];                       // per-wave covered-row totals
         __shared__ u32 swcnt[4];                         // per-wave surviving-piece counts
@@ -902,7 +902,7 @@ struct KFineF {
     }
 };
 template <int TS>
-__global__ void __launch_bounds__(256, TS == 32 ? 4 : 5) k_fine(RastArgs a) { KFineF<TS>{}(vb_hw(), a); }
+__global__ void __launch_bounds__(256, TS == 32 ? 4 : 6) k_fine(RastArgs a) { KFineF<TS>{}(vb_hw(), a); }
 
 // One thread per pixel: winning key -> (u, v, z/w, id+1) and DB.
 struct KResolveF {
