@@ -1724,7 +1724,7 @@ __device__ __forceinline__ float nrcp(float x) { return __builtin_amdgcn_rcpf(x)
 // AddNaNcheck for an accumulator that is finite by construction.
 __device__ __forceinline__ void acc_checked(float& a, float b) { float v = a + b; a = isfinite(v) ? v : a; }
 
-struct SgLobe { float gx, gy, gz, lam, mx, my, mz, lamL2E, a[4]; };  // lamL2E = lam * log2(e)
+struct SgLobe { float gx, gy, gz, lam, mx, my, mz, lamL2E, a[4], L2; };  // lamL2E = lam * log2(e), L2 = |lam mu|^2
 
 __device__ __forceinline__ void sg_stage_lobes(SgLobe* L, int K, int C, const float* axis, const float* sharp, const float* amp) {
     for (int i = threadIdx.x; i < K; i += blockDim.x) {
@@ -1732,6 +1732,7 @@ __device__ __forceinline__ void sg_stage_lobes(SgLobe* L, int K, int C, const fl
         SgLobe l;
         l.mx = axis[3 * i]; l.my = axis[3 * i + 1]; l.mz = axis[3 * i + 2];
         l.gx = l.mx * lam; l.gy = l.my * lam; l.gz = l.mz * lam; l.lam = lam; l.lamL2E = lam * L2E;
+        l.L2 = l.gx * l.gx + l.gy * l.gy + l.gz * l.gz;
         for (int k = 0; k < 4; k++) l.a[k] = k < C ? amp[i * C + k] : 0.f;
         L[i] = l;
     }
@@ -2031,7 +2032,8 @@ __global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* ra
 // X = gout * diffuse * 2.34 and G = gout * spec amplitude: the constant factors of the diffuse
 // and specular terms are taken into the pixel's staged values, so the lobe loop never applies
 // them.
-struct __align__(16) SgPix { float dnx, dny, dnz, ssL2E, snx, sny, snz, pad, X[4], G[4]; };  // ssL2E = spec sharpness * log2(e)
+// ssL2E = spec sharpness * log2(e); dsq, ssq: the squared lengths of (dnx, dny, dnz) and (snx, sny, snz)
+struct __align__(16) SgPix { float dnx, dny, dnz, ssL2E, snx, sny, snz, dsq, X[4], G[4], ssq, pad[3]; };
 
 // One lobe term of the backward: g = (e1 - e2) il and dg/dl = ((e1 + e2) - g) il, with e2 =
 // e^{-l-c} formed only when BOTH (otherwise e2 is below half an ulp of e1 and both sums equal e1).
@@ -2052,32 +2054,55 @@ __device__ __forceinline__ float sg_term_bwd(float wx, float wy, float wz, float
     return g;
 }
 
+// sg_term_bwd from the squared length d = |w|^2.
+template <bool BOTH>
+__device__ __forceinline__ float sg_term_bwd_d(float d, float c, float& il, float& dgdl) {
+    il = __builtin_amdgcn_rsqf(fmaxf(d, 1e-12f));
+    float l = d * il;
+    float e1 = nexp2(fmaf(l, L2E, -c));
+    if (BOTH) {
+        float e2 = nexp2(fmaf(-l, L2E, -c));
+        float g = (e1 - e2) * il;
+        dgdl = ((e1 + e2) - g) * il;
+        return g;
+    }
+    float g = e1 * il;
+    dgdl = (e1 - g) * il;
+    return g;
+}
+
 // The pixel loop of one lobe lane (see k_sg_bwd_lobes); DB / SB: the diffuse / specular e2 is
 // needed by some pixel of the chunk for some lobe of the wave.  The chunk's pixel list is padded
 // with zero pixels to a multiple of G, so every lane runs n / G iterations (a zero pixel adds
 // zeros: its X and G are 0).  A = 2.34 * sum(gout diffuse amp), B = sa * sum(gout amp).
+// The squared lengths |p + lam mu|^2 are formed as (|p|^2 + |lam mu|^2) + 2 p . lam mu, and the
+// chain-factor sums over w = p + lam mu as sum(cd p) plus lam mu times sum(cd) (added by the
+// caller): the vectors w and q themselves are never formed.
 template <bool DB, bool SB>
-__device__ __forceinline__ void sg_bwd_lobe_loop(const SgLobe& L, const SgPix* pix, int grp, int nit, int G, float accW[3],
+__device__ __forceinline__ void sg_bwd_lobe_loop(const SgLobe& L, const SgPix* pix, int grp, int nit, int G, float accW[3], float& accC,
                                                  float accX[4], float accG[4]) {
     const float cD = L.lamL2E + SG_DS * L2E;
-    // the record is read as four 16-byte words
+    const float g2x = 2.f * L.gx, g2y = 2.f * L.gy, g2z = 2.f * L.gz;
+    // the record is read as five 16-byte words
     const float4* xp = reinterpret_cast<const float4*>(pix + grp);
-    for (int t = 0; t < nit; t++, xp += 4 * G) {
+    for (int t = 0; t < nit; t++, xp += 5 * G) {
         const float4 d = xp[0], sn = xp[1], X = xp[2], Gp = xp[3];
+        const float ssq = xp[4].x;
         const float A = ((X.x * L.a[0] + X.y * L.a[1]) + X.z * L.a[2]) + X.w * L.a[3];
         const float B = ((Gp.x * L.a[0] + Gp.y * L.a[1]) + Gp.z * L.a[2]) + Gp.w * L.a[3];
-        const float wx = d.x + L.gx, wy = d.y + L.gy, wz = d.z + L.gz;
         float il, dgdl;
-        const float dgv = sg_term_bwd<DB>(wx, wy, wz, cD, il, dgdl);
-        const float qx = sn.x + L.gx, qy = sn.y + L.gy, qz = sn.z + L.gz;
+        const float dd = fmaf(d.x, g2x, fmaf(d.y, g2y, fmaf(d.z, g2z, sn.w + L.L2)));
+        const float dgv = sg_term_bwd_d<DB>(dd, cD, il, dgdl);
         float isl, sgdl;
-        const float sgv = sg_term_bwd<SB>(qx, qy, qz, L.lamL2E + d.w, isl, sgdl);
+        const float ds = fmaf(sn.x, g2x, fmaf(sn.y, g2y, fmaf(sn.z, g2z, ssq + L.L2)));
+        const float sgv = sg_term_bwd_d<SB>(ds, L.lamL2E + d.w, isl, sgdl);
         accX[0] += X.x * dgv; accX[1] += X.y * dgv; accX[2] += X.z * dgv; accX[3] += X.w * dgv;
         accG[0] += Gp.x * sgv; accG[1] += Gp.y * sgv; accG[2] += Gp.z * sgv; accG[3] += Gp.w * sgv;
         const float cd = A * dgdl * il, cs = B * sgdl * isl;
-        accW[0] += cd * wx; accW[0] += cs * qx;
-        accW[1] += cd * wy; accW[1] += cs * qy;
-        accW[2] += cd * wz; accW[2] += cs * qz;
+        accW[0] += cd * d.x; accW[0] += cs * sn.x;
+        accW[1] += cd * d.y; accW[1] += cs * sn.y;
+        accW[2] += cd * d.z; accW[2] += cs * sn.z;
+        accC += cd; accC += cs;
     }
 }
 
@@ -2118,10 +2143,13 @@ struct KSgBwdLobesF {
                                                      const float* sSharp, const float* sAmp, const float* gout, const float* denv, float* gdiffuse,
                                                      float* part, int want_lobes, int gd_fresh) const {
         __shared__ SgLobe lobes[SG_MAXK];
-        __shared__ SgPix pix[BS];
+        // the chunk's pixel records, and after the last chunk the per-thread sums (one buffer)
+        static_assert(sizeof(SgPix) * BS >= sizeof(float) * BS * SG_COLS, "reduction fits the pixel buffer");
+        __shared__ float4 sbuf[sizeof(SgPix) * BS / 16];
+        SgPix* pix = reinterpret_cast<SgPix*>(sbuf);
+        float(*red)[SG_COLS] = reinterpret_cast<float(*)[SG_COLS]>(sbuf);
         __shared__ int wcount[BS / 64];
         __shared__ u32 wneed[BS / 64];
-        __shared__ float red[BS][SG_COLS];
         __shared__ unsigned long long skey[SG_MAXK];
         __shared__ int lorig[SG_MAXK];
         __shared__ float2 lrange[BS / 64];
@@ -2151,6 +2179,7 @@ struct KSgBwdLobesF {
             SgLobe l;
             l.mx = axis[3 * i]; l.my = axis[3 * i + 1]; l.mz = axis[3 * i + 2];
             l.gx = l.mx * lam; l.gy = l.my * lam; l.gz = l.mz * lam; l.lam = lam; l.lamL2E = lam * L2E;
+            l.L2 = l.gx * l.gx + l.gy * l.gy + l.gz * l.gz;
             for (int k = 0; k < 4; k++) l.a[k] = k < C ? amp[i * C + k] : 0.f;
             lobes[r] = l;
             lorig[r] = i;
@@ -2168,7 +2197,7 @@ struct KSgBwdLobesF {
         float acc[SG_COLS];
 #pragma unroll
         for (int k = 0; k < SG_COLS; k++) acc[k] = 0.f;
-        float accW[3] = {0.f, 0.f, 0.f}, accX[4] = {0.f, 0.f, 0.f, 0.f}, accG[4] = {0.f, 0.f, 0.f, 0.f};
+        float accW[3] = {0.f, 0.f, 0.f}, accX[4] = {0.f, 0.f, 0.f, 0.f}, accG[4] = {0.f, 0.f, 0.f, 0.f}, accC = 0.f;
         // The pixel inputs of the next chunk are loaded while this chunk's lobe loop runs (and the
         // coverage word of the chunk after that), so the chunks' loads overlap the lobe work instead
         // of every workgroup waiting on them between chunks.
@@ -2203,12 +2232,15 @@ struct KSgBwdLobesF {
                         }
                 q.dnx = cur.nx * SG_DS; q.dny = cur.ny * SG_DS; q.dnz = cur.nz * SG_DS;
                 const float ss = cur.ss;
-                q.ssL2E = ss * L2E; q.pad = 0.f;
+                q.ssL2E = ss * L2E;
                 q.snx = cur.ax * ss; q.sny = cur.ay * ss; q.snz = cur.az * ss;
+                q.dsq = q.dnx * q.dnx + q.dny * q.dny + q.dnz * q.dnz;
+                q.ssq = q.snx * q.snx + q.sny * q.sny + q.snz * q.snz;
+                q.pad[0] = q.pad[1] = q.pad[2] = 0.f;
 #pragma unroll
                 for (int k = 0; k < 4; k++) { q.X[k] = cur.go[k] * cur.df[k] * SG_DA; q.G[k] = cur.go[k] * cur.sa; }
-                const float dlen = sqrtf(q.dnx * q.dnx + q.dny * q.dny + q.dnz * q.dnz);
-                const float slen = sqrtf(q.snx * q.snx + q.sny * q.sny + q.snz * q.snz);
+                const float dlen = sqrtf(q.dsq);
+                const float slen = sqrtf(q.ssq);
                 for (int w = 0; w < nlw; w++) {
                     const float2 r = lrange[w];
                     need |= (sg_needs_e2(dlen, r.x, r.y) ? 1u : 0u) << (2 * w);
@@ -2231,7 +2263,8 @@ struct KSgBwdLobesF {
             const int ncovp = (ncov + G - 1) / G * G;
             if (tid >= ncov && tid < ncovp) {
                 SgPix z;
-                z.dnx = z.dny = z.dnz = z.ssL2E = z.snx = z.sny = z.snz = z.pad = 0.f;
+                z.dnx = z.dny = z.dnz = z.ssL2E = z.snx = z.sny = z.snz = z.dsq = z.ssq = 0.f;
+                z.pad[0] = z.pad[1] = z.pad[2] = 0.f;
 #pragma unroll
                 for (int k = 0; k < 4; k++) { z.X[k] = 0.f; z.G[k] = 0.f; }
                 pix[tid] = z;
@@ -2248,16 +2281,18 @@ struct KSgBwdLobesF {
                 const SgLobe& L = lobes[li];
                 const bool db = (cneed >> (2 * lw)) & 1u, sb = (cneed >> (2 * lw + 1)) & 1u;
                 const int nit = ncovp / G;
-                if (db && sb) sg_bwd_lobe_loop<true, true>(L, pix, grp, nit, G, accW, accX, accG);
-                else if (db) sg_bwd_lobe_loop<true, false>(L, pix, grp, nit, G, accW, accX, accG);
-                else if (sb) sg_bwd_lobe_loop<false, true>(L, pix, grp, nit, G, accW, accX, accG);
-                else sg_bwd_lobe_loop<false, false>(L, pix, grp, nit, G, accW, accX, accG);
+                if (db && sb) sg_bwd_lobe_loop<true, true>(L, pix, grp, nit, G, accW, accC, accX, accG);
+                else if (db) sg_bwd_lobe_loop<true, false>(L, pix, grp, nit, G, accW, accC, accX, accG);
+                else if (sb) sg_bwd_lobe_loop<false, true>(L, pix, grp, nit, G, accW, accC, accX, accG);
+                else sg_bwd_lobe_loop<false, false>(L, pix, grp, nit, G, accW, accC, accX, accG);
             }
             __syncthreads();
             cur = nxt;
         }
         if (want_lobes && li < K) {
             const SgLobe& L = lobes[li];
+            // sum(cd w + cs q) = sum(cd p_d + cs p_s) + lam mu sum(cd + cs)
+            accW[0] += L.gx * accC; accW[1] += L.gy * accC; accW[2] += L.gz * accC;
             acc[0] = accW[0] * L.lam;
             acc[1] = accW[1] * L.lam;
             acc[2] = accW[2] * L.lam;
