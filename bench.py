@@ -115,8 +115,9 @@ def algorithmic_bytes(sc, views=1):
         "interpolate": P * 56 + V * 8 + T * 12,
         "interp_texture": P * (16 + 16 + 8 + 16 + 4 * C) + V * 8 + T * 12,
         "shade_prep": P * (16 + 4 + 12 + 12 + 12 + 4 + 4) + V * 12 + T * 12,
-        # the fit step's fused pass: interp_texture's and shade_prep's surfaces, rast read once
-        "pixel_prep": P * (16 + 16 + 4 + 8 + 16 + 4 * C + 12 + 12 + 12 + 4 + 4) + V * 20 + T * 12,
+        # the fit step's fused pass: the raster's resolve (winning keys in, rast and db out) and
+        # interp_texture's and shade_prep's surfaces
+        "pixel_prep": P * (8 + 16 + 16 + 4 + 8 + 16 + 4 * C + 12 + 12 + 12 + 4 + 4) + views * V * 16 + V * 20 + T * 12,
         "texture": P * (4 + 8 + 16 + 4 * C),
         "spherical_gaussian": P * 100 + K * 32,
         # the fit step's antialias pass also writes the L2 loss's grad (reads the target)

@@ -323,7 +323,10 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
     { ProfScope ps_(F, "project", as_hip(s));
       st = step ? vcl_project_forward_dev(&pj, F->f("gclip"), F->t<double>("adam_bt"), d.beta1, d.beta2, &rc, s) : vcl_project_forward(&pj, s); }
     if (st) return st;
-    { ProfScope ps_(F, "rasterize", as_hip(s)); st = vcl_rasterize_forward_dev(&rp, step ? 1 : 0, s); }
+    // the fused C == 4 path resolves the winning keys in its per-pixel pass (pixel_prep)
+    const bool fused = C == 4 && !d.op_by_op;
+    const unsigned long long* keys = nullptr;
+    { ProfScope ps_(F, "rasterize", as_hip(s)); st = vcl_rasterize_forward_dev(&rp, step ? 1 : 0, s, fused ? &keys : nullptr); }
     if (st) return st;
     vcl_normal_axis_params np;
     std::memset(&np, 0, sizeof(np));
@@ -341,18 +344,18 @@ static int forward(vcl_fit* F, vcl_stream_t s, bool step = false) {
         { ProfScope ps_(F, "normal_axis", as_hip(s)); st = vcl_normal_axis_forward(&np, s); }
         if (!st) { ProfScope ps_(F, "view_axis", as_hip(s)); st = vcl_view_axis_forward_dev(W, H, nv, F->f("pvinv"), F->f("view"), s); }
         if (!st) { ProfScope ps_(F, "sg_specular", as_hip(s)); st = vcl_sg_specular_forward(&sp, s); }
-    } else if (C == 4) {
-        // NormalAxis -> ViewAxis -> SGSpecular with the uv interpolation and texture sample, fused
-        // per pixel
+    } else if (fused) {
+        // the raster's resolve, NormalAxis -> ViewAxis -> SGSpecular, the uv interpolation and the
+        // texture sample, fused per pixel
         ProfScope ps_(F, "pixel_prep", as_hip(s));
-        st = vcl_pixel_prep_dev(&ip, &tp, &np, F->f("pvinv"), &sp, s);
+        st = vcl_pixel_prep_dev(&rp, keys, &ip, &tp, &np, F->f("pvinv"), &sp, s);
     } else {
         // NormalAxis -> ViewAxis -> SGSpecular, fused per pixel
         ProfScope ps_(F, "shade_prep", as_hip(s));
         st = vcl_shade_prep_dev(&np, F->f("pvinv"), &sp, s);
     }
     if (st) return st;
-    if (C != 4 || d.op_by_op) {
+    if (!fused) {
         // (the fused C == 4 path did these in the pixel_prep pass)
         { ProfScope ps_(F, "interpolate", as_hip(s)); st = vcl_interpolate_forward(&ip, s); }
         if (st) return st;

@@ -33,14 +33,20 @@ int vcl_project_forward_dev(const vcl_project_params* p, float* zero, double* bt
 // The workspace reset of the next vcl_rasterize_forward(_dev) with these params (same tuning).
 int vcl_rasterize_clear_spec(const vcl_rasterize_params* p, WsClear* c);
 // vcl_rasterize_forward; cleared: the workspace reset was already done (vcl_rasterize_clear_spec).
-int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_stream_t stream);
+// keys (may be null): the per-pixel resolve is left to the caller, which gets the winning-key
+// buffer (one 64-bit key per pixel, all-ones where no triangle won; raster_resolve.hpp) in *keys
+// and must write p->out / p->outDB from it (vcl_pixel_prep_dev does).
+int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_stream_t stream, const unsigned long long** keys = nullptr);
 // NormalAxis, ViewAxis (inverse projection-view blocks on the device) and SGSpecular forward in one
 // launch; sp->normal and sp->view are the outputs of the first two.
 int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp, vcl_stream_t stream);
-// uv interpolation + texture sample (C == 4) and the shading inputs in one pass: the outputs of
-// vcl_interp_tex_dev and vcl_shade_prep_dev (normals indexed like the uv attributes).
-int vcl_pixel_prep_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, const vcl_normal_axis_params* np, const float* pvinv_dev,
-                       const vcl_sg_specular_params* sp, vcl_stream_t stream);
+// The rasterizer's resolve (keys: the winning-key buffer of vcl_rasterize_forward_dev with
+// resolve deferred; rp->out and rp->outDB written), uv interpolation + texture sample (C == 4)
+// and the shading inputs in one pass: the outputs of k_resolve, vcl_interp_tex_dev and
+// vcl_shade_prep_dev (normals indexed like the uv attributes).
+int vcl_pixel_prep_dev(const vcl_rasterize_params* rp, const unsigned long long* keys, const vcl_interpolate_params* ip,
+                       const vcl_texturemap_params* tp, const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp,
+                       vcl_stream_t stream);
 // Texture backward then interpolate backward (the latter's output grad being the former's uv
 // grad): fused into one kernel when the tiled C == 4 texture backward applies, a 2-channel
 // attribute feeds the uv and only the rast grad is wanted; otherwise the two ops in turn.

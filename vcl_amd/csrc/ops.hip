@@ -4,6 +4,7 @@
 // early); reductions that the reference does with racy or atomic float adds are done with
 // fixed-order block partials so results are deterministic.
 #include "internal.hpp"
+#include "raster_resolve.hpp"
 #include <algorithm>
 #include <cstring>
 #include <cmath>
@@ -1404,56 +1405,86 @@ __global__ void __launch_bounds__(BS) k_shade_prep(int W, int H, int depth, cons
                                                    const float* pvinv, const float* rough, float ior, float* onrm, float* oview, float* oAxis,
                                                    float* oSharp, float* oAmp) { KShadePrepF{}(vb_hw(), W, H, depth, rast, rot, normal, nidx, pvinv, rough, ior, onrm, oview, oAxis, oSharp, oAmp); }
 
+// The rasterizer's per-pixel resolve as the fit step runs it inside its per-pixel pass: the
+// winning-key buffer, the clip-space vertices (V per view) and the outputs rast / db.
+struct ResolveArgs {
+    const unsigned long long* keys;
+    const float* proj;
+    int V;
+    float xs, ys;
+    float* rast;
+    float* db;
+};
+
 // The fit step's per-pixel passes ahead of the SG forward in one (C == 4, normals indexed like
-// the uv attributes): uv interpolation with derivatives and the texture sample (the arithmetic
-// of KInterpTex4F) and the shading inputs NormalAxis, ViewAxis and SGSpecular (that of
-// KShadePrepF).  The two share the rast read and the triangle's index loads, and the specular
-// terms are formed while the texel loads are in flight.  Each thread takes NPX pixels one
-// workgroup-width apart, each stage's loads for all its pixels issued together.
+// the uv attributes): the rasterizer's resolve (a pixel's winning key -> rast and db, as
+// raster.hip's k_resolve), uv interpolation with derivatives and the texture sample (the
+// arithmetic of KInterpTex4F) and the shading inputs NormalAxis, ViewAxis and SGSpecular (that of
+// KShadePrepF).  They share the triangle's index loads, the clip-space vertices and the uv and
+// normal attributes are loaded together, and the specular terms are formed while the texel
+// loads are in flight.  Each thread takes NPX pixels one workgroup-width apart, each stage's
+// loads for all its pixels issued together.
 template <int NPX>
 struct KPixelPrep4F {
     static constexpr int kBS = BS, kMINW = 1;
-    __device__ __forceinline__ void operator()(const VB& vb, int W, int H, int depth, const float* rast, const float* db, const float* uvattr,
-                                               const u32* idx, float* uvp, float* uvDA, TexArgs t, float* out, const float* rot,
-                                               const float* normal, const float* pvinv, const float* rough, float ior, float* onrm,
-                                               float* oview, float* oAxis, float* oSharp, float* oAmp) const {
-        const size_t P = (size_t)W * H * depth;
+    __device__ __forceinline__ void operator()(const VB& vb, int W, int H, int depth, ResolveArgs ra, const float* uvattr, const u32* idx,
+                                               float* uvp, float* uvDA, TexArgs t, float* out, const float* rot, const float* normal,
+                                               const float* pvinv, const float* rough, float ior, float* onrm, float* oview, float* oAxis,
+                                               float* oSharp, float* oAmp) const {
+        const size_t WH = (size_t)W * H, P = WH * depth;
         const size_t p0 = (size_t)vb.b.x * (BS * NPX) + threadIdx.x;
-        float4 r[NPX];
+        unsigned long long key[NPX];
 #pragma unroll
         for (int j = 0; j < NPX; j++) {
             const size_t p = p0 + (size_t)j * BS;
-            r[j] = p < P ? ((const float4*)rast)[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+            key[j] = p < P ? ra.keys[p] : RAST_NO_KEY;
         }
         u32 i0[NPX], i1[NPX], i2[NPX];
-        float4 b[NPX];
         float rg[NPX];
 #pragma unroll
         for (int j = 0; j < NPX; j++) {
-            const int id = (int)r[j].w - 1;
             i0[j] = i1[j] = i2[j] = 0;
-            b[j] = make_float4(0.f, 0.f, 0.f, 0.f);
             rg[j] = 0.f;
-            if (id >= 0) {
-                const size_t p = p0 + (size_t)j * BS;
-                i0[j] = idx[3 * id]; i1[j] = idx[3 * id + 1]; i2[j] = idx[3 * id + 2];
-                b[j] = ((const float4*)db)[p];
-                rg[j] = rough[p];
+            if (key[j] != RAST_NO_KEY) {
+                const u32 tri = (u32)(key[j] & 0xffffffffu);
+                i0[j] = idx[3 * tri]; i1[j] = idx[3 * tri + 1]; i2[j] = idx[3 * tri + 2];
+                rg[j] = rough[p0 + (size_t)j * BS];
             }
         }
+        float4 r[NPX], b[NPX];
         float2 a0[NPX], a1[NPX], a2[NPX];
         float3 n0[NPX], n1[NPX], n2[NPX];
+        {
+            float4 c0[NPX], c1[NPX], c2[NPX];
 #pragma unroll
-        for (int j = 0; j < NPX; j++) {
-            a0[j] = a1[j] = a2[j] = make_float2(0.f, 0.f);
-            n0[j] = n1[j] = n2[j] = make_float3(0.f, 0.f, 0.f);
-            if ((int)r[j].w - 1 >= 0) {
-                a0[j] = ((const float2*)uvattr)[i0[j]];
-                a1[j] = ((const float2*)uvattr)[i1[j]];
-                a2[j] = ((const float2*)uvattr)[i2[j]];
-                n0[j] = load_f3(normal, i0[j]);
-                n1[j] = load_f3(normal, i1[j]);
-                n2[j] = load_f3(normal, i2[j]);
+            for (int j = 0; j < NPX; j++) {
+                a0[j] = a1[j] = a2[j] = make_float2(0.f, 0.f);
+                n0[j] = n1[j] = n2[j] = make_float3(0.f, 0.f, 0.f);
+                c0[j] = c1[j] = c2[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (key[j] != RAST_NO_KEY) {
+                    const float4* clip = (const float4*)ra.proj + (size_t)((p0 + (size_t)j * BS) / WH) * ra.V;
+                    c0[j] = clip[i0[j]]; c1[j] = clip[i1[j]]; c2[j] = clip[i2[j]];
+                    a0[j] = ((const float2*)uvattr)[i0[j]];
+                    a1[j] = ((const float2*)uvattr)[i1[j]];
+                    a2[j] = ((const float2*)uvattr)[i2[j]];
+                    n0[j] = load_f3(normal, i0[j]);
+                    n1[j] = load_f3(normal, i1[j]);
+                    n2[j] = load_f3(normal, i2[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < NPX; j++) {
+                const size_t p = p0 + (size_t)j * BS;
+                r[j] = b[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (key[j] != RAST_NO_KEY) {
+                    const int px = (int)(p % W), py = (int)((p / W) % H);
+                    resolve_pixel(c0[j], c1[j], c2[j], (u32)(key[j] & 0xffffffffu), ra.xs, ra.ys, pixel_centre(px, ra.xs), pixel_centre(py, ra.ys), true,
+                                  r[j], b[j]);
+                }
+                if (p < P) {
+                    ((float4*)ra.rast)[p] = r[j];
+                    ((float4*)ra.db)[p] = b[j];
+                }
             }
         }
         Tex4Taps q[NPX];
@@ -1486,7 +1517,7 @@ struct KPixelPrep4F {
         for (int j = 0; j < NPX; j++) {
             const size_t p = p0 + (size_t)j * BS;
             if (p >= P) continue;
-            const int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / ((size_t)W * H));
+            const int px = (int)(p % W), py = (int)((p / W) % H), s = (int)(p / WH);
             float nx = 0.f, ny = 0.f, nz = 0.f, vx, vy, vz;
             if ((int)r[j].w - 1 >= 0) normal_axis_eval(r[j], rot, n0[j], n1[j], n2[j], nx, ny, nz);
             view_axis_px(W, H, px, py, pvinv + 9 * s, vx, vy, vz);
@@ -1511,11 +1542,11 @@ struct KPixelPrep4F {
     }
 };
 constexpr int PIXEL_PREP_NPX = 2;
-__global__ void __launch_bounds__(BS) k_pixel_prep4(int W, int H, int depth, const float* rast, const float* db, const float* uvattr, const u32* idx,
-                                                    float* uvp, float* uvDA, TexArgs t, float* out, const float* rot, const float* normal,
-                                                    const float* pvinv, const float* rough, float ior, float* onrm, float* oview, float* oAxis,
-                                                    float* oSharp, float* oAmp) {
-    KPixelPrep4F<PIXEL_PREP_NPX>{}(vb_hw(), W, H, depth, rast, db, uvattr, idx, uvp, uvDA, t, out, rot, normal, pvinv, rough, ior, onrm, oview, oAxis,
+__global__ void __launch_bounds__(BS) k_pixel_prep4(int W, int H, int depth, ResolveArgs ra, const float* uvattr, const u32* idx, float* uvp,
+                                                    float* uvDA, TexArgs t, float* out, const float* rot, const float* normal, const float* pvinv,
+                                                    const float* rough, float ior, float* onrm, float* oview, float* oAxis, float* oSharp,
+                                                    float* oAmp) {
+    KPixelPrep4F<PIXEL_PREP_NPX>{}(vb_hw(), W, H, depth, ra, uvattr, idx, uvp, uvDA, t, out, rot, normal, pvinv, rough, ior, onrm, oview, oAxis,
                                    oSharp, oAmp);
 }
 
@@ -3152,10 +3183,13 @@ int vcl_shade_prep_dev(const vcl_normal_axis_params* np, const float* pvinv_dev,
     return VCL_OK;
 }
 
-int vcl_pixel_prep_dev(const vcl_interpolate_params* ip, const vcl_texturemap_params* tp, const vcl_normal_axis_params* np, const float* pvinv_dev,
-                       const vcl_sg_specular_params* sp, vcl_stream_t stream) {
-    VCL_REQUIRE(ip && tp && np && sp && pvinv_dev);
-    VCL_REQUIRE(ip->rast && ip->rastDB && ip->attr && ip->idx && ip->out && ip->outDA && tp->out);
+int vcl_pixel_prep_dev(const vcl_rasterize_params* rp, const unsigned long long* keys, const vcl_interpolate_params* ip,
+                       const vcl_texturemap_params* tp, const vcl_normal_axis_params* np, const float* pvinv_dev, const vcl_sg_specular_params* sp,
+                       vcl_stream_t stream) {
+    VCL_REQUIRE(rp && keys && ip && tp && np && sp && pvinv_dev);
+    VCL_REQUIRE(rp->out && rp->outDB && rp->enableDB && rp->proj && rp->idx && ip->rast == rp->out && ip->rastDB == rp->outDB);
+    VCL_REQUIRE(rp->width == ip->width && rp->height == ip->height && rp->depth == ip->depth && rp->idx == ip->idx);
+    VCL_REQUIRE(ip->attr && ip->idx && ip->out && ip->outDA && tp->out);
     VCL_REQUIRE(ip->dimention == 2 && ip->enableDA && tp->channel == 4 && tp->uv == ip->out && tp->uvDA == ip->outDA && tp->rast == ip->rast);
     VCL_REQUIRE(np->rot && np->normal && np->normalidx == ip->idx && np->out && sp->roughness && sp->outAxis && sp->outSharpness && sp->outAmplitude);
     VCL_REQUIRE(np->rast == ip->rast && sp->rast == ip->rast && sp->normal == np->out);
@@ -3166,10 +3200,11 @@ int vcl_pixel_prep_dev(const vcl_interpolate_params* ip, const vcl_texturemap_pa
     if (st) return st;
     size_t P = (size_t)ip->width * ip->height * ip->depth;
     if (!P) return VCL_OK;
+    const ResolveArgs ra = {keys, rp->proj, rp->projNum, rp->xs, rp->ys, rp->out, rp->outDB};
     vlaunch<KPixelPrep4F<PIXEL_PREP_NPX>>(k_pixel_prep4, dim3(cdiv(P, (size_t)BS * PIXEL_PREP_NPX)), dim3(BS), 0, as_hip(stream), ip->width,
-                                          ip->height, ip->depth, ip->rast, ip->rastDB, ip->attr, ip->idx, ip->out, ip->outDA, t, tp->out, np->rot,
-                                          np->normal, pvinv_dev, sp->roughness, sp->ior, np->out, const_cast<float*>(sp->view), sp->outAxis,
-                                          sp->outSharpness, sp->outAmplitude);
+                                          ip->height, ip->depth, ra, ip->attr, ip->idx, ip->out, ip->outDA, t, tp->out, np->rot, np->normal,
+                                          pvinv_dev, sp->roughness, sp->ior, np->out, const_cast<float*>(sp->view), sp->outAxis, sp->outSharpness,
+                                          sp->outAmplitude);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
 }

@@ -25,6 +25,7 @@
 // is compiled without FP contraction so both produce identical triangle IDs.
 #pragma clang fp contract(off)
 #include "internal.hpp"
+#include "raster_resolve.hpp"
 #include "raster_span.hpp"
 #include <algorithm>
 
@@ -188,28 +189,6 @@ __device__ bool setup_piece(float4 a, float4 b, float4 c, u32 id, u32 view, int 
     r.id = id;
     r.view_tl = (view << 8) | tl;
     return true;
-}
-
-// Depth planes of the original triangle (see the oracle's depth_planes: same float ops).
-__device__ __forceinline__ void depth_planes(float4 c0, float4 c1, float4 c2, float* zp) {
-    float4 c[3] = {c0, c1, c2};
-    float a[3], b[3], d[3];
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        float4 j = c[(i + 1) % 3], k = c[(i + 2) % 3];
-        a[i] = j.y * k.w - j.w * k.y;
-        b[i] = j.w * k.x - j.x * k.w;
-        d[i] = j.x * k.y - j.y * k.x;
-    }
-    zp[0] = (c0.z * a[0] + c1.z * a[1]) + c2.z * a[2];
-    zp[1] = (c0.z * b[0] + c1.z * b[1]) + c2.z * b[2];
-    zp[2] = (c0.z * d[0] + c1.z * d[1]) + c2.z * d[2];
-    zp[3] = (c0.w * a[0] + c1.w * a[1]) + c2.w * a[2];
-    zp[4] = (c0.w * b[0] + c1.w * b[1]) + c2.w * b[2];
-    zp[5] = (c0.w * d[0] + c1.w * d[1]) + c2.w * d[2];
-    zp[6] = (a[0] + a[1]) + a[2];
-    zp[7] = (b[0] + b[1]) + b[2];
-    zp[8] = (d[0] + d[1]) + d[2];
 }
 
 __device__ int clip_poly(const float4* in, int n, float4* out, float p0, float p1, float p2, float p3, float p4) {
@@ -625,29 +604,9 @@ __device__ void fine_resolve(const RastArgs& a, u32 view, int px, int py, float 
     const float4* clip = (const float4*)a.proj + (size_t)view * a.V;
     size_t p = ((size_t)view * a.H + py) * a.W + px;
     float4 o = make_float4(0.f, 0.f, 0.f, 0.f), d = o;
-    if (best != ~0ull) {
+    if (best != RAST_NO_KEY) {
         u32 t = (u32)(best & 0xffffffffu);
-        float4 c0 = clip[a.idx[3 * t]], c1 = clip[a.idx[3 * t + 1]], c2 = clip[a.idx[3 * t + 2]];
-        float xs = a.xs, ys = a.ys;
-        float w0 = c0.w, w1 = c1.w, w2 = c2.w;
-        float P0x = w0 * pixx - c0.x, P0y = w0 * pixy - c0.y;
-        float P1x = w1 * pixx - c1.x, P1y = w1 * pixy - c1.y;
-        float P2x = w2 * pixx - c2.x, P2y = w2 * pixy - c2.y;
-        float E0x = P0x - P2x, E0y = P0y - P2y, E1x = P1x - P2x, E1y = P1y - P2y;
-        float D = E0x * E1y - E0y * E1x;
-        float u = (P1x * P2y - P1y * P2x) / D;
-        float v = (P2x * P0y - P2y * P0x) / D;
-        float zp[9];
-        depth_planes(c0, c1, c2, zp);
-        float zz = (zp[0] * pixx + zp[1] * pixy + zp[2]) / (zp[3] * pixx + zp[4] * pixy + zp[5]);
-        o = make_float4(u, v, zz, (float)(t + 1));
-        if (a.outDB) {
-            float dNuX = xs * (w1 * P2y - w2 * P1y), dNuY = ys * (w2 * P1x - w1 * P2x);
-            float dNvX = xs * (w2 * P0y - w0 * P2y), dNvY = ys * (w0 * P2x - w2 * P0x);
-            float dDX = xs * ((w0 - w2) * E1y - (w1 - w2) * E0y);
-            float dDY = ys * ((w1 - w2) * E0x - (w0 - w2) * E1x);
-            d = make_float4((dNuX - u * dDX) / D, (dNuY - u * dDY) / D, (dNvX - v * dDX) / D, (dNvY - v * dDY) / D);
-        }
+        resolve_pixel(clip[a.idx[3 * t]], clip[a.idx[3 * t + 1]], clip[a.idx[3 * t + 2]], t, a.xs, a.ys, pixx, pixy, a.outDB != nullptr, o, d);
     }
     ((float4*)a.out)[p] = o;
     if (a.outDB) ((float4*)a.outDB)[p] = d;
@@ -913,9 +872,7 @@ struct KResolveF {
         if (p >= P) return;
         const int px = (int)(p % a.W), py = (int)((p / a.W) % a.H);
         const u32 view = (u32)(p / ((size_t)a.W * a.H));
-        const float pixx = (float)((px + 0.5) * (double)a.xs - 1.0);
-        const float pixy = (float)((py + 0.5) * (double)a.ys - 1.0);
-        fine_resolve(a, view, px, py, pixx, pixy, a.ws.keys[p]);
+        fine_resolve(a, view, px, py, pixel_centre(px, a.xs), pixel_centre(py, a.ys), a.ws.keys[p]);
     }
 };
 __global__ void __launch_bounds__(256) k_resolve(RastArgs a) { KResolveF{}(vb_hw(), a); }
@@ -1034,7 +991,7 @@ extern "C" size_t vcl_rasterize_workspace_size(int width, int height, int depth,
     return m;
 }
 
-int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_stream_t stream) {
+int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_stream_t stream, const unsigned long long** keys) {
     VCL_REQUIRE(p && p->out && p->workspace && (p->idxNum == 0 || (p->proj && p->idx)));
     VCL_REQUIRE(p->width > 0 && p->height > 0 && p->depth > 0 && p->width <= 16384 && p->height <= 16384);
     VCL_REQUIRE(p->idxNum >= 0 && p->idxNum < (1 << 24) && (p->projNum > 0 || p->idxNum == 0) && p->depth < (1 << 23));
@@ -1077,6 +1034,10 @@ int vcl_rasterize_forward_dev(const vcl_rasterize_params* p, int cleared, vcl_st
     else if (tshift == 5) vlaunch<KFineF<32>>(k_fine<32>, fg, dim3(256), 0, s, a);
     else vlaunch<KFineF<16>>(k_fine<16>, fg, dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
+    if (keys) {
+        *keys = (const unsigned long long*)a.ws.keys;
+        return VCL_OK;
+    }
     vlaunch<KResolveF>(k_resolve, dim3(cdiv(P, 256)), dim3(256), 0, s, a);
     VCL_LAUNCH_CHECK();
     return VCL_OK;
