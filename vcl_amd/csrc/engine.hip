@@ -436,7 +436,8 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
     gg.diffuse = F->f("gdiff"); gg.axis = F->f("grad_axis"); gg.sharpness = F->f("grad_sharpness"); gg.amplitude = F->f("grad_amplitude");
     gg.out = F->f("gcolor"); gg.workspace = F->buf["ws_sg"]; gg.workspace_size = F->ws_sg;
     // the lobe grads' final sums run in the Adam launch at the end of the step
-    vcl_sg_lobe_sums lobes = {nullptr, 0, 0, 0};
+    // the fused step also leaves the sums' segment stage to the texture backward's launch
+    vcl_sg_lobe_sums lobes = {nullptr, 0, 0, 0, d.op_by_op ? 0 : 1, 0, {nullptr, 0, 0, 0, nullptr}};
     // the fused forward leaves the texture-output grad uncleared: the SG backward writes it
     // (the op-by-op forward clears it, as the reference's forward(GradParams&) does)
     { ProfScope ps_(F, "spherical_gaussian_bwd", as_hip(s));
@@ -461,7 +462,7 @@ static int step_body_tuned(vcl_fit* F, vcl_stream_t s) {
         if (!st) { ProfScope ps_(F, "interpolate_bwd", as_hip(s)); st = vcl_interpolate_backward(&ip, &ig, s); }
     } else {
         ProfScope ps_(F, "texture_interp_bwd", as_hip(s));
-        st = vcl_texture_interp_backward_dev(&tp, &tg, &ip, &ig, s);
+        st = vcl_texture_interp_backward_dev(&tp, &tg, &ip, &ig, s, &lobes);
     }
     if (st) return st;
     vcl_rasterize_params rp;

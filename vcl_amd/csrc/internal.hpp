@@ -50,8 +50,11 @@ int vcl_pixel_prep_dev(const vcl_rasterize_params* rp, const unsigned long long*
 // Texture backward then interpolate backward (the latter's output grad being the former's uv
 // grad): fused into one kernel when the tiled C == 4 texture backward applies, a 2-channel
 // attribute feeds the uv and only the rast grad is wanted; otherwise the two ops in turn.
+struct vcl_sg_lobe_sums;
+// stage (may be null or not valid): an SG segment stage (vcl_sg_lobe_sums) the texture backward's
+// launch also runs, as workgroups after its own.
 int vcl_texture_interp_backward_dev(const vcl_texturemap_params* tp, const vcl_texturemap_grad_params* tg, const vcl_interpolate_params* ip,
-                                    const vcl_interpolate_grad_params* ig, vcl_stream_t stream);
+                                    const vcl_interpolate_grad_params* ig, vcl_stream_t stream, const vcl_sg_lobe_sums* stage = nullptr);
 // Fused texture update.  part bit 1: fold + Adam + clamp on level 0; bit 2: rebuild the mips and
 // clear the coarse grads; bit 4 (C == 4): level 1 and its grad clear move from the mip rebuild
 // into the level-0 update (one 2 x 2 quad per thread), so the two calls of a split update must
@@ -70,12 +73,23 @@ int vcl_adam_step_multi_dev(int n, const vcl_optimize_params* p, const vcl_adam_
 // with proj->out as the output grad (clip-space, 4 floats per vertex and view).
 // The SG backward's per-lobe sums left for the caller: seg holds nseg rows of K x 8 columns
 // (axis 3, sharpness 1, amplitude up to 4), summed over the rows in order into the lobe grads.
+// Segment stage of those sums (seg = column sums over consecutive groups of seg_rows rows of the
+// rows x cols partials in part), when the caller runs it (defer_stage).
+struct VclColsumStage {
+    const float* part;
+    int rows, cols, seg_rows;
+    float* seg;
+};
 struct vcl_sg_lobe_sums {
     const float* seg;
     int nseg, K, C;
+    int defer_stage;       // in: leave the segment stage to the caller (stage set, valid = 1)
+    int stage_valid;
+    VclColsumStage stage;
 };
 // vcl_spherical_gaussian_backward that, when lobes is not null and all three lobe grads are
-// requested, leaves the lobe sums to the caller (lobes->seg set) instead of adding them.
+// requested, leaves the lobe sums to the caller (lobes->seg set) instead of adding them, and with
+// lobes->defer_stage also the segment stage that fills seg (lobes->stage).
 // gd_fresh: g->diffuse holds nothing yet (the pass writes it as if it had been cleared, so the
 // caller needs no clear of its own).
 int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, const vcl_spherical_gaussian_grad_params* g,

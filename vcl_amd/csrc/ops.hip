@@ -118,17 +118,22 @@ constexpr int COLSUM_SEGS = 32;
 
 size_t colsum_scratch_floats(int cols) { return (size_t)COLSUM_SEGS * cols; }
 
-// First stage: segment sums of `rows` rows into seg_scratch; returns the segment count, or -1
-// when the launch failed.
-int colsum_stage(const float* part, int rows, int cols, float* seg_scratch, hipStream_t s) {
+// First stage: segment sums of `rows` rows into seg_scratch.  colsum_stage_of describes it and
+// returns the segment count; colsum_launch launches it (-1 when the launch failed).
+int colsum_stage_of(const float* part, int rows, int cols, float* seg_scratch, VclColsumStage& cs) {
     int seg_rows = (rows + COLSUM_SEGS - 1) / COLSUM_SEGS;
     if (seg_rows < 1) seg_rows = 1;
     int nseg = (rows + seg_rows - 1) / seg_rows;
     if (nseg < 1) nseg = 1;
-    vlaunch<KColsumStageF>(k_colsum_stage, dim3(cdiv(cols, 64), nseg), dim3(256), 0, s, part, rows, cols, seg_rows, seg_scratch);
-    if (hipGetLastError() != hipSuccess) return -1;
+    cs.part = part; cs.rows = rows; cs.cols = cols; cs.seg_rows = seg_rows; cs.seg = seg_scratch;
     return nseg;
 }
+int colsum_nseg(const VclColsumStage& cs) { return (cs.rows + cs.seg_rows - 1) / cs.seg_rows; }
+int colsum_launch(const VclColsumStage& cs, hipStream_t s) {
+    vlaunch<KColsumStageF>(k_colsum_stage, dim3(cdiv(cs.cols, 64), colsum_nseg(cs)), dim3(256), 0, s, cs.part, cs.rows, cs.cols, cs.seg_rows, cs.seg);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 
 // ---------------------------------------------------------------- project
 // zero (optional): a buffer shaped like out cleared in the same pass (the step's clip-space grad
@@ -834,6 +839,32 @@ template <int ROWS>
 __global__ void __launch_bounds__(BS) k_tex_bwd4_tiled(int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast, const float* uv,
                                                        const float* uvDA, const float* gout, float* guv, float* guvDA, int mode,
                                                        InterpBwdArgs ib) { KTexBwd4TiledF<ROWS>{}(vb_hw(), W, H, tilesX, t, g, rast, uv, uvDA, gout, guv, guvDA, mode, ib); }
+
+// The tiled texture backward (its nt x depth grid flattened) with the SG lobe sums' segment stage
+// as workgroups after its own: the two are independent, and the stage's workgroups run in the
+// texture backward's tail instead of as a launch of their own.
+template <int ROWS>
+struct KTexBwdStageF {
+    static constexpr int kBS = BS, kMINW = 1;
+    __device__ __forceinline__ void operator()(const VB& vb, int nt, int depth, int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast,
+                                               const float* uv, const float* uvDA, const float* gout, float* guv, float* guvDA, int mode,
+                                               InterpBwdArgs ib, VclColsumStage cs, int nseg) const {
+        const u32 ntex = (u32)nt * (u32)depth;
+        if (vb.b.x < ntex) {
+            KTexBwd4TiledF<ROWS>{}(VB{{vb.b.x % (u32)nt, vb.b.x / (u32)nt, 0u}, {(u32)nt, (u32)depth, 1u}}, W, H, tilesX, t, g, rast, uv, uvDA, gout, guv,
+                                   guvDA, mode, ib);
+            return;
+        }
+        const u32 c = vb.b.x - ntex, cx = ((u32)cs.cols + 63u) / 64u;
+        KColsumStageF{}(VB{{c % cx, c / cx, 0u}, {cx, (u32)nseg, 1u}}, cs.part, cs.rows, cs.cols, cs.seg_rows, cs.seg);
+    }
+};
+template <int ROWS>
+__global__ void __launch_bounds__(BS) k_tex_bwd_stage(int nt, int depth, int W, int H, int tilesX, TexArgs t, TexGradArgs g, const float* rast,
+                                                      const float* uv, const float* uvDA, const float* gout, float* guv, float* guvDA, int mode,
+                                                      InterpBwdArgs ib, VclColsumStage cs, int nseg) {
+    KTexBwdStageF<ROWS>{}(vb_hw(), nt, depth, W, H, tilesX, t, g, rast, uv, uvDA, gout, guv, guvDA, mode, ib, cs, nseg);
+}
 
 // ---------------------------------------------------------------- mip utilities
 struct KDownsampleF {
@@ -2836,8 +2867,9 @@ int vcl_interp_tex_dev(const vcl_interpolate_params* ip, const vcl_texturemap_pa
 // ib.grast not null: the interpolate backward to run inside the tiled kernel.  *fused_done tells
 // whether it ran; when the tiled kernel does not apply it stays false and the caller runs the
 // interpolate backward itself.
+// stage (may be null): a segment stage to run in the tiled kernel's launch, or on its own otherwise.
 static int texturemap_backward(const vcl_texturemap_params* p, const vcl_texturemap_grad_params* g, const InterpBwdArgs& ib, bool* fused_done,
-                               vcl_stream_t stream) {
+                               vcl_stream_t stream, const VclColsumStage* stage = nullptr) {
     if (fused_done) *fused_done = false;
     VCL_REQUIRE(p && g && p->rast && p->uv && g->out);
     TexArgs t;
@@ -2856,7 +2888,13 @@ static int texturemap_backward(const vcl_texturemap_params* p, const vcl_texture
         float* gda = t.L > 1 ? g->uvDA : nullptr;
         // fused: the uv grad is consumed in the kernel by the interpolate backward, not stored
         float* guv = ib.grast ? nullptr : g->uv;
-        if (rows == 64)
+        if (stage && rows == 16) {
+            const int nseg = colsum_nseg(*stage);
+            const size_t nb = (size_t)tilesX * tilesY * p->depth + cdiv(stage->cols, 64) * (size_t)nseg;
+            vlaunch<KTexBwdStageF<16>>(k_tex_bwd_stage<16>, dim3((unsigned)nb), dim3(BS), 0, as_hip(stream), tilesX * tilesY, p->depth, p->width,
+                                       p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA, g->out, guv, gda, g->daMode, ib, *stage, nseg);
+            stage = nullptr;
+        } else if (rows == 64)
             vlaunch<KTexBwd4TiledF<64>>(k_tex_bwd4_tiled<64>, grid, dim3(BS), 0, as_hip(stream), p->width, p->height, tilesX, t, ga, p->rast, p->uv, p->uvDA,
                                g->out, guv, gda, g->daMode, ib);
         else if (rows == 32)
@@ -2871,6 +2909,7 @@ static int texturemap_backward(const vcl_texturemap_params* p, const vcl_texture
                            t.L > 1 ? g->uvDA : nullptr, g->daMode);
     }
     VCL_LAUNCH_CHECK();
+    if (stage && colsum_launch(*stage, as_hip(stream))) return VCL_ERR_HIP;
     return VCL_OK;
 }
 
@@ -2879,15 +2918,16 @@ extern "C" int vcl_texturemap_backward(const vcl_texturemap_params* p, const vcl
 }
 
 int vcl_texture_interp_backward_dev(const vcl_texturemap_params* tp, const vcl_texturemap_grad_params* tg, const vcl_interpolate_params* ip,
-                                    const vcl_interpolate_grad_params* ig, vcl_stream_t stream) {
+                                    const vcl_interpolate_grad_params* ig, vcl_stream_t stream, const vcl_sg_lobe_sums* stage) {
     VCL_REQUIRE(tp && tg && ip && ig && ip->rast && ip->attr && ip->idx && ig->out);
+    const VclColsumStage* cs = stage && stage->stage_valid ? &stage->stage : nullptr;
     // fusable: the interpolate backward's output grad is this texture backward's uv grad, over
     // the same pixels, with a 2-channel attribute and only the rast grad wanted
     const bool fusable = ip->dimention == 2 && ig->out == tg->uv && !ig->attr && ig->rast && ip->rast == tp->rast && ip->width == tp->width &&
                          ip->height == tp->height && ip->depth == tp->depth;
     bool done = false;
     int st = texturemap_backward(tp, tg, fusable ? InterpBwdArgs{ip->rast, ip->idx, ip->attr, ig->rast} : InterpBwdArgs{nullptr, nullptr, nullptr, nullptr},
-                                 &done, stream);
+                                 &done, stream, cs);
     if (st || done) return st;
     return vcl_interpolate_backward(ip, ig, stream);
 }
@@ -3311,10 +3351,19 @@ int vcl_spherical_gaussian_backward_dev(const vcl_spherical_gaussian_params* p, 
     VCL_LAUNCH_CHECK();
     if (!want) return VCL_OK;
     float* seg = part + (size_t)nb * cols;
-    const int nseg = colsum_stage(part, (int)nb, (int)cols, seg, s);
-    if (nseg < 0) return VCL_ERR_HIP;
-    if (lobes && g->axis && g->sharpness && g->amplitude) {
-        // the caller's Adam launch adds the sums into the lobe grads
+    const bool to_caller = lobes && g->axis && g->sharpness && g->amplitude;
+    if (lobes) lobes->stage_valid = 0;
+    VclColsumStage cs;
+    const int nseg = colsum_stage_of(part, (int)nb, (int)cols, seg, cs);
+    if (to_caller && lobes->defer_stage) {
+        // the caller runs the segment stage (in its texture backward launch) ...
+        lobes->stage = cs;
+        lobes->stage_valid = 1;
+    } else if (colsum_launch(cs, s)) {
+        return VCL_ERR_HIP;
+    }
+    if (to_caller) {
+        // ... and its Adam launch adds the sums into the lobe grads
         lobes->seg = seg; lobes->nseg = nseg; lobes->K = K; lobes->C = C;
         return VCL_OK;
     }
