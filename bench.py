@@ -380,11 +380,12 @@ def kernel_roofline_entries(sc, views, kernels, ab, value, U):
                     "traffic_frac": round(tr / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4) if tr else None,
                     "note": "alg_bytes include 16 B per touched texel" if op == "pixel_prep" else None})
     ms = t("texture_interp_bwd")
-    n_at = cv.get("plain k_tex_bwd4_tiled<16> l2_atomics_per_launch")
+    # the tiled texture backward's launch (its workgroups plus the SG lobe sums' segment stage)
+    n_at = cv.get("plain k_tex_bwd_stage<16> l2_atomics_per_launch")
     if ms and n_at and apeak:
-        out.append({"kernel": "k_tex_bwd4_tiled<16>", "op": "texture_interp_bwd", "bound": "atomics", "avg_ms": ms,
+        out.append({"kernel": "k_tex_bwd_stage<16>", "op": "texture_interp_bwd", "bound": "atomics", "avg_ms": ms,
                     "l2_atomics_per_launch": int(n_at), "achieved": round(n_at / (ms * 1e-3) / 1e9, 2), "peak": round(apeak / 1e9, 2),
-                    "unit": "G atomics/s", "frac": round(n_at / (ms * 1e-3) / apeak, 4), "traffic": measured_traffic("k_tex_bwd4_tiled<16>")})
+                    "unit": "G atomics/s", "frac": round(n_at / (ms * 1e-3) / apeak, 4), "traffic": measured_traffic("k_tex_bwd_stage<16>")})
     for op, kern in (("spherical_gaussian", "k_sg_fwd4<512>"), ("spherical_gaussian_bwd", "k_sg_bwd_lobes")):
         ms = t(op)
         vi = cv.get(f"plain {kern} valu_issue_per_cu_cycle")
