@@ -1997,20 +1997,23 @@ __device__ __forceinline__ void sg_fwd4_lobes(const SgLobe4* lobes, int K, float
     }
 }
 
-// Pixels per k_sg_fwd4 workgroup: 1024, or 512 when that would leave fewer than 4,096
-// workgroups.  Covered pixels are unevenly spread over the image, and with few workgroups per
-// CU (all resident at once) the CUs holding interior rows finish last; halving the workgroup
-// lets the dispatcher balance them.
-template <int SG_FWD_PIX>
+// Pixels per k_sg_fwd4 workgroup (SG_FWD_PIX) and threads (NT): 1,024 pixels on 256 threads, or
+// 512 pixels on 512 threads when the 1,024-pixel grid would hold fewer than 4,096 workgroups.
+// With few workgroups all are resident at once and the launch ends with few waves per SIMD
+// left to hide the lobe loop's latencies; 512 one-round threads per 512 pixels double the waves
+// (two rounds of residency), where 512 pixels on 256 threads left the CUs under-filled
+// (1024^2: 104 -> 95 us).  Large grids keep 256 threads (the 512-thread form was 8 % slower at
+// 64 x 512^2).
+template <int SG_FWD_PIX, int NT>
 struct KSgFwd4F {
-    static constexpr int kBS = BS, kMINW = 1;
+    static constexpr int kBS = NT, kMINW = 1;
     __device__ __forceinline__ void operator()(const VB& vb, size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
                                                 const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
                                                 const float* sAmp, float* out, float* denv, float* senv) const {
         __shared__ SgLobe4 lobes[SG_MAXK];
         __shared__ int nonfinite, unbounded;
         __shared__ u32 plist[SG_FWD_PIX];
-        __shared__ u32 wcount[BS / 64];
+        __shared__ u32 wcount[NT / 64];
         if (threadIdx.x == 0) { nonfinite = 0; unbounded = 0; }
         __syncthreads();
         for (int i = threadIdx.x; i < K; i += blockDim.x) {
@@ -2037,15 +2040,15 @@ struct KSgFwd4F {
         const size_t base = (size_t)vb.b.x * SG_FWD_PIX;
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         u32 n = 0;
-        for (int k = 0; k < SG_FWD_PIX / BS; k++) {
-            const u32 local = (u32)(k * BS + threadIdx.x);
+        for (int k = 0; k < SG_FWD_PIX / NT; k++) {
+            const u32 local = (u32)(k * NT + threadIdx.x);
             const size_t p = base + local;
             const bool in = p < P;
             const bool cov = in && (int)rast[4 * p + 3] - 1 >= 0;
             if (in && !cov) { ((float4*)out)[p] = z4; ((float4*)denv)[p] = z4; ((float4*)senv)[p] = z4; }
             block_append(cov, local, plist, wcount, n);
         }
-        for (u32 r = 0; r < n; r += BS) {
+        for (u32 r = 0; r < n; r += NT) {
             const u32 i = r + threadIdx.x;
             const bool act = i < n;
             const size_t p = act ? base + plist[i] : 0;
@@ -2081,10 +2084,10 @@ struct KSgFwd4F {
         }
     }
 };
-template <int SG_FWD_PIX>
-__global__ void __launch_bounds__(BS) k_sg_fwd4(size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
+template <int SG_FWD_PIX, int NT>
+__global__ void __launch_bounds__(NT) k_sg_fwd4(size_t P, int K, const float* rast, const float* axis, const float* sharp, const float* amp,
                                                 const float* normal, const float* diffuse, const float* sAxis, const float* sSharp,
-                                                const float* sAmp, float* out, float* denv, float* senv) { KSgFwd4F<SG_FWD_PIX>{}(vb_hw(), P, K, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out, denv, senv); }
+                                                const float* sAmp, float* out, float* denv, float* senv) { KSgFwd4F<SG_FWD_PIX, NT>{}(vb_hw(), P, K, rast, axis, sharp, amp, normal, diffuse, sAxis, sSharp, sAmp, out, denv, senv); }
 
 // Lobe-parallel backward for the lobe gradients (axis, sharpness, amplitude).  Thread t owns
 // lobe t % KP for pixel sub-group t / KP; covered pixels of a 256-pixel chunk are compacted into
@@ -3299,11 +3302,11 @@ extern "C" int vcl_spherical_gaussian_forward(const vcl_spherical_gaussian_param
                       ((uintptr_t)p->outSpecenv % 16 == 0);
     if (p->sgnum <= SG_MAXK && p->channel == 4 && al16) {
         if (cdiv(P, 4 * BS) >= 4096)
-            vlaunch<KSgFwd4F<4 * BS>>(k_sg_fwd4<4 * BS>, dim3(cdiv(P, 4 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
+            vlaunch<KSgFwd4F<4 * BS, BS>>(k_sg_fwd4<4 * BS, BS>, dim3(cdiv(P, 4 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
                                p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
                                p->outSpecenv);
         else
-            vlaunch<KSgFwd4F<2 * BS>>(k_sg_fwd4<2 * BS>, dim3(cdiv(P, 2 * BS)), dim3(BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
+            vlaunch<KSgFwd4F<2 * BS, 2 * BS>>(k_sg_fwd4<2 * BS, 2 * BS>, dim3(cdiv(P, 2 * BS)), dim3(2 * BS), 0, as_hip(stream), P, p->sgnum, p->rast, p->axis, p->sharpness,
                                p->amplitude, p->normal, p->diffuse, p->specAxis, p->specSharpness, p->specAmplitude, p->out, p->outDiffenv,
                                p->outSpecenv);
     }
