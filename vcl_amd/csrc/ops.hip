@@ -1361,6 +1361,14 @@ struct KViewAxisF {
 __global__ void __launch_bounds__(BS) k_view_axis(int W, int H, int depth, const float* pvinv_dev, Mat3x8 pv_small, float* out) { KViewAxisF{}(vb_hw(), W, H, depth, pvinv_dev, pv_small, out); }
 
 // ---------------------------------------------------------------- SGSpecular
+// (float)(x / fmax((double)y, 1e-6)) for float x, y, as the reference writes it: where y is at or
+// above the smallest float not below 1e-6 the quotient of the two floats in double, rounded to
+// float, is the correctly rounded float quotient (double carries more than twice float's 24 bits),
+// so a float division gives the same bits; below it (or for a NaN y) the double form runs.
+__device__ __forceinline__ float div_fmax_1e6(float x, float y) {
+    return y >= 0x1.0c6f7cp-20f ? x / y : (float)((double)x / 1e-6);
+}
+
 __device__ __forceinline__ void spec_terms(float nx, float ny, float nz, float vx_, float vy_, float vz_, float m, float ior, float* p, float& amp,
                                            float& sharp, float* dA, float* dS) {
     float vx = -vx_, vy = -vy_, vz = -vz_;
@@ -1370,19 +1378,19 @@ __device__ __forceinline__ void spec_terms(float nx, float ny, float nz, float v
     float vn = fmaxf(vdn, 0.f);
     float vn2 = vn * vn;
     float iGv = sqrtf(vn2 + m2 * (1.f - vn2));
-    float Gv = (float)(1.0 / fmax((double)(vn + iGv), 1e-6));
+    float Gv = div_fmax_1e6(1.f, vn + iGv);
     float pdn = p[0] * nx + p[1] * ny + p[2] * nz;
     float pn = fmaxf(pdn, 0.f);
     float pn2 = pn * pn;
     float iGp = sqrtf(pn2 + m2 * (1.f - pn2));
-    float Gp = (float)(1.0 / fmax((double)(pn + iGp), 1e-6));
+    float Gp = div_fmax_1e6(1.f, pn + iGp);
     float g = sqrtf(ior * ior - 1.f + pn2);
     float gp = g + pn, gm = g - pn;
     float f0 = gm / gp;
     float f1 = (pn * gp - 1.f) / (pn * gm + 1.f);
     float F = f0 * f0 * (1.f + f1 * f1);
-    amp = (float)(F * Gp * Gv * pn / fmax((double)m2, 1e-6));
-    sharp = (float)(0.5f / fmax((double)(m2 * vn), 1e-6));
+    amp = div_fmax_1e6(F * Gp * Gv * pn, m2);
+    sharp = div_fmax_1e6(0.5f, m2 * vn);
     if (dA) {
         float dGvdm = (float)(-m * (1.f - vn2) / fmax((double)iGv, 1e-6) * Gv * Gv);
         float dGpdm = (float)(-m * (1.f - pn2) / fmax((double)iGp, 1e-6) * Gp * Gp);
