@@ -2274,20 +2274,26 @@ struct KSgBwdLobesF {
         // The pixel inputs of the next chunk are loaded while this chunk's lobe loop runs (and the
         // coverage word of the chunk after that), so the chunks' loads overlap the lobe work instead
         // of every workgroup waiting on them between chunks.
-        const size_t stride = (size_t)vb.g.x * BS;
+        // The chunks form bands of nb (the grid size); in band k the workgroup takes position
+        // (b + k s) mod nb, s = nb / 4 + 1, so its chunks step through the rows' column ranges
+        // (the first, second, third ... quarter of a row) and through the image's row bands,
+        // rather than all lying in one column range, where the covered share differs most.
+        const size_t nbk = vb.g.x, sft = nbk / 4 + 1;
+        auto cbase = [&](size_t k) -> size_t { return (k * nbk + (vb.b.x + k * sft) % nbk) * BS; };
+        const size_t nband = (P + nbk * BS - 1) / (nbk * BS);
         SgRaw cur, nxt;
         float rwn;
         {
-            const size_t p0 = (size_t)vb.b.x * BS + tid, p1 = p0 + stride;
+            const size_t p0 = cbase(0) + tid, p1 = cbase(1) + tid;
             sg_load_raw(cur, p0, p0 < P && (int)rast[4 * p0 + 3] - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp,
                         gd_fresh != 0);
             rwn = p1 < P ? rast[4 * p1 + 3] : 0.f;
         }
-        for (size_t base = (size_t)vb.b.x * BS; base < P; base += stride) {
-            const size_t p = base + tid;
+        for (size_t kb = 0; kb < nband; kb++) {
+            const size_t p = cbase(kb) + tid;
             const bool cov = cur.cov;
             {
-                const size_t pn = p + stride, p2 = pn + stride;
+                const size_t pn = cbase(kb + 1) + tid, p2 = cbase(kb + 2) + tid;
                 sg_load_raw(nxt, pn, pn < P && (int)rwn - 1 >= 0, C, gout, diffuse, denv, gdiffuse, normal, sAxis, sSharp, sAmp,
                             gd_fresh != 0);
                 rwn = p2 < P ? rast[4 * p2 + 3] : 0.f;
@@ -3282,11 +3288,12 @@ extern "C" int vcl_sg_specular_backward(const vcl_sg_specular_params* p, const v
     return VCL_OK;
 }
 
-// Workgroups of the lobe backward: one per two 256-pixel chunks, at most 4,096 (four times what
-// can be resident), so that workgroups whose chunks are mostly covered do not set the end of
-// the launch while other CUs idle; each adds a row of K x 8 partial sums to the column sum.
+// Workgroups of the lobe backward: one per four 256-pixel chunks (at 1024^2, the 1,024 that are
+// resident at once), at most 4,096; with the rotated chunk order of k_sg_bwd_lobes each gets a
+// like share of covered pixels, so they end together.  Each adds a row of K x 8 partial sums to
+// the column sum.
 static unsigned sg_bwd_blocks(size_t P) {
-    size_t sweep = (size_t)BS * SG_NPIX;
+    size_t sweep = (size_t)BS * 4;
     size_t nb = (P + sweep - 1) / sweep;
     if (nb > 4096) nb = 4096;
     if (nb < 1) nb = 1;
