@@ -2278,10 +2278,12 @@ struct KSgBwdLobesF {
         // coverage word of the chunk after that), so the chunks' loads overlap the lobe work instead
         // of every workgroup waiting on them between chunks.
         // The chunks form bands of nb (the grid size); in band k the workgroup takes position
-        // (b + k s) mod nb, s = nb / 4 + 1, so its chunks step through the rows' column ranges
-        // (the first, second, third ... quarter of a row) and through the image's row bands,
-        // rather than all lying in one column range, where the covered share differs most.
-        const size_t nbk = vb.g.x, sft = nbk / 4 + 1;
+        // (b + k s) mod nb, s = 2 nb / 3, so its chunks move through the rows' column ranges and
+        // through the image's row bands rather than all lying in one column range, where the
+        // covered share differs most (for a centred disk covering 70 % of a 1024^2 image the
+        // fullest workgroup holds 1.08x the mean covered pixels, against 1.52x in grid-stride
+        // order).
+        const size_t nbk = vb.g.x, sft = nbk / 3 * 2;
         auto cbase = [&](size_t k) -> size_t { return (k * nbk + (vb.b.x + k * sft) % nbk) * BS; };
         const size_t nband = (P + nbk * BS - 1) / (nbk * BS);
         SgRaw cur, nxt;
