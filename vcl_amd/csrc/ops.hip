@@ -2280,7 +2280,7 @@ struct KSgBwdLobesF {
         // The chunks form bands of nb (the grid size); in band k the workgroup takes position
         // (b + k s) mod nb, s = 2 nb / 3, so its chunks move through the rows' column ranges and
         // through the image's row bands rather than all lying in one column range, where the
-        // covered share differs most (for a centred disk covering 70 % of a 1024^2 image the
+        // covered share differs most (for a centred disk covering 66 % of a 1024^2 image the
         // fullest workgroup holds 1.08x the mean covered pixels, against 1.52x in grid-stride
         // order).
         const size_t nbk = vb.g.x, sft = nbk / 3 * 2;
